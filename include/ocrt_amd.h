@@ -1,0 +1,148 @@
+/*
+ * ocrt_amd.h -- C ABI of the MI355X-native successive-orders-of-scattering
+ * (SOS) solver for polarized (I, Q, U) top-of-atmosphere radiance.
+ *
+ * The boundary is a drop-in for the atmospheric solve path of OCRT:
+ *   rt_solve_case_pol / rt_solve_case_pol_aerosol / rt_solve_case_pol_lut
+ *   (code/OCRT_C/src/rt_solver.h:84, :119, :207; driver rt_solver.c:618-1437)
+ * and for its OpenMP batch driver rt_io_run_batch (code/OCRT_C/src/rt_io.c:199-262),
+ * which solves independent rows one per thread.  Here a whole batch of
+ * uneven instances (different geometry, wavelength, aerosol load, layer count
+ * and stream count) is solved by one plan on one GPU.
+ *
+ * Every signature uses plain C types and pointers.  Instance descriptors and
+ * results live in host memory; all device memory is owned by the plan.
+ *
+ * Conventions kept from the reference:
+ *   - return codes 0 ok, -1 bad arguments, -2 non-finite field, -3 allocation
+ *     (code/OCRT_C/src/rt_solver.h:11 and rt_solver.c:623-640); -4 is new and
+ *     means a HIP runtime error;
+ *   - Convention B normalisation: rho = I_TOA / mu_sun (code/OCRT_C/src/rt_types.h:6-49);
+ *   - the public relative azimuth RAA (RAA = 180 is the glint branch,
+ *     code/OCRT_C/src/rt_raa_convention.h:8-37);
+ *   - U reported in the fixed meridian-basis sine convention
+ *     (code/OCRT_C/src/rt_fourier.c:24-33).
+ */
+#ifndef OCRT_AMD_H
+#define OCRT_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Lower boundary kinds; numeric values equal rt_surface_kind_t
+ * (code/OCRT_C/src/rt_types.h:55-61).  LAMBERT behaves like BLACK in the
+ * reference atmosphere driver (rt_solver.c:979-989) and does so here. */
+enum {
+    OCRT_SURFACE_BLACK = 0,
+    OCRT_SURFACE_LAMBERT = 1,
+    OCRT_SURFACE_BLACK_FRESNEL_OCEAN = 2,
+    OCRT_SURFACE_FLAT = 3
+};
+
+enum {
+    OCRT_OK = 0,
+    OCRT_ERR_ARGS = -1,
+    OCRT_ERR_NONFINITE = -2,
+    OCRT_ERR_ALLOC = -3,
+    OCRT_ERR_DEVICE = -4
+};
+
+/* Arithmetic forms of the solve.
+ *   FP64  : every kernel in double precision (the plain parity form).
+ *   MIXED : the scattering integral of orders >= 2 runs on bf16 MFMA with fp32
+ *           accumulation and the layer sweeps of those orders in fp32; single
+ *           scattering, the angular recurrences, the surface kernels, IPSS and
+ *           the compensated fp64 order sum stay in fp64. */
+enum { OCRT_PREC_FP64 = 0, OCRT_PREC_MIXED = 1 };
+
+/* One solve.  Mirrors the fields of rt_case_t / rt_options_t /
+ * rt_aerosol_input_t that the atmospheric path reads
+ * (code/OCRT_C/src/rt_types.h:321-482, :654-754; rt_solver.h:94-114) with
+ * tau_r_source = FROM_INPUT and view_as_node = 1 semantics. */
+typedef struct {
+    double sza_deg, vza_deg, raa_deg, wavelength_nm;
+    int    surface;            /* OCRT_SURFACE_* */
+    int    decouple_sunglint;  /* 1: no direct glint added to rho */
+    double n_water;            /* real refractive index, default 1.34 */
+    double wind_speed;         /* m/s; <= 0 with BLACK_FRESNEL_OCEAN = flat Fresnel */
+    int    sigma_type;         /* 0 Nakajima-Tanaka, 1 OCRT floor law (default) */
+    int    q_convention;       /* 1 Mishchenko (Rp-Rs)/2 (default) */
+    double tau_R;              /* Rayleigh column optical depth; 0 = Rayleigh off */
+    double depol;              /* Rayleigh depolarisation (the reference fixes 0.0279) */
+    double tau_a;              /* aerosol column optical depth at this wavelength */
+    double ssa_a;              /* aerosol single-scattering albedo */
+    double aer_h_km;           /* aerosol exponential scale height, default 2 km */
+    int    L_max;              /* aerosol Legendre order; 0 = no aerosol */
+    int    n_mu;               /* Gauss-Legendre nodes per hemisphere */
+    const double *greek;       /* host [4][L_max+1]: beta(P11), gamma(P12), alpha, zeta */
+    int    n_layers;           /* plane-parallel layers (levels = n_layers + 1) */
+    int    m_max;              /* highest Fourier mode (no cap at 32 here) */
+    int    max_orders;         /* SOS order cap, default 20 */
+    int    ipss;               /* 1: spherical-shell kappa correction */
+    double tolerance;          /* order-sum stop, default 1e-7 */
+} ocrt_instance_t;
+
+/* Per-instance results; names follow rt_result_t (code/OCRT_C/src/rt_types.h:496-651). */
+typedef struct {
+    double rho_I, rho_Q, rho_U;
+    double I_TOA, Q_TOA, U_TOA;
+    double tau_R_total;
+    double T_dir_dn, T_diff_dn_hemi, T_total_dn_hemi, T_diff_dn_dir;
+    double T_dir_up_view;
+    double T_sg_up_dir, T_total_up_dir;
+    double rho_glint_direct_I, rho_glint_direct_Q, rho_glint_direct_U;
+    double kappa;              /* IPSS factor applied (1 when IPSS is off) */
+    int    n_orders_used;      /* max over m */
+    int    converged;          /* AND over m */
+    int    status;             /* OCRT_OK or an error code for this instance */
+    int    n_ring;             /* ring size per hemisphere (GL nodes + view node) */
+} ocrt_result_t;
+
+typedef struct ocrt_plan ocrt_plan;
+
+/* Build a plan: validate every instance (same rules as rt_solver.c:626-639
+ * except the m_max <= 32 cap), build the quadrature rings on the host and
+ * allocate all device memory.  One plan serves one set of instances and can be
+ * executed any number of times.  one_instance_per_launch = 1 runs the
+ * instances one after another, each with its own kernel launches (the plain
+ * form); 0 solves the whole batch in one launch sequence. */
+int ocrt_plan_create(const ocrt_instance_t *inst, int n_inst, int precision,
+                     int one_instance_per_launch, ocrt_plan **plan_out);
+
+/* Run the full solve on the given HIP stream (NULL = default stream).
+ * Asynchronous with respect to the host; call ocrt_plan_results to wait. */
+int ocrt_plan_execute(ocrt_plan *plan, void *hip_stream);
+
+/* Copy results to host memory (synchronises the stream).
+ *   out        [n_inst]
+ *   toa_modes  optional [n_inst][mmax_all+1][3] TOA (I,Q,U)^m at the view node
+ *   orders     optional [n_inst][mmax_all+1] SOS orders used per mode
+ * where mmax_all is the largest m_max of the plan (ocrt_plan_mmax). */
+int ocrt_plan_results(ocrt_plan *plan, ocrt_result_t *out, double *toa_modes,
+                      int *orders);
+
+int  ocrt_plan_mmax(const ocrt_plan *plan);
+/* Bytes of device memory the plan holds. */
+size_t ocrt_plan_device_bytes(const ocrt_plan *plan);
+/* Algorithmic work of the last execute: scattering-integral flops and
+ * layer-sweep bytes summed over the orders actually run (useful to compute
+ * roofline fractions); either pointer may be NULL. */
+int  ocrt_plan_work(const ocrt_plan *plan, double *gemm_flops, double *sweep_bytes,
+                    double *gemm_launches);
+void ocrt_plan_destroy(ocrt_plan *plan);
+
+/* One-call form: plan, execute, copy results, destroy. */
+int ocrt_solve_batch(const ocrt_instance_t *inst, int n_inst, int precision,
+                     ocrt_result_t *out, void *hip_stream);
+
+/* Library identity, for the loaders' checks. */
+const char *ocrt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCRT_AMD_H */

@@ -1,0 +1,106 @@
+"""ctypes mirror of include/ocrt_amd.h (instance descriptor and result records).
+
+The same records are understood by the HIP library (libocrt_amd.so), by the
+CPU restatement used as the test oracle and by the reference adapter, so one
+descriptor array drives all three.
+"""
+import ctypes as C
+
+import numpy as np
+
+SURFACE_BLACK = 0
+SURFACE_LAMBERT = 1
+SURFACE_BLACK_FRESNEL_OCEAN = 2
+SURFACE_FLAT = 3
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+SURFACE_NAMES = {
+    "black": SURFACE_BLACK,
+    "lambert": SURFACE_LAMBERT,
+    "black_fresnel_ocean": SURFACE_BLACK_FRESNEL_OCEAN,
+    "coxmunk": SURFACE_BLACK_FRESNEL_OCEAN,
+    "flat": SURFACE_FLAT,
+}
+
+
+class Instance(C.Structure):
+    _fields_ = [
+        ("sza_deg", C.c_double), ("vza_deg", C.c_double),
+        ("raa_deg", C.c_double), ("wavelength_nm", C.c_double),
+        ("surface", C.c_int), ("decouple_sunglint", C.c_int),
+        ("n_water", C.c_double), ("wind_speed", C.c_double),
+        ("sigma_type", C.c_int), ("q_convention", C.c_int),
+        ("tau_R", C.c_double), ("depol", C.c_double),
+        ("tau_a", C.c_double), ("ssa_a", C.c_double), ("aer_h_km", C.c_double),
+        ("L_max", C.c_int), ("n_mu", C.c_int),
+        ("greek", C.POINTER(C.c_double)),
+        ("n_layers", C.c_int), ("m_max", C.c_int),
+        ("max_orders", C.c_int), ("ipss", C.c_int),
+        ("tolerance", C.c_double),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("rho_I", C.c_double), ("rho_Q", C.c_double), ("rho_U", C.c_double),
+        ("I_TOA", C.c_double), ("Q_TOA", C.c_double), ("U_TOA", C.c_double),
+        ("tau_R_total", C.c_double),
+        ("T_dir_dn", C.c_double), ("T_diff_dn_hemi", C.c_double),
+        ("T_total_dn_hemi", C.c_double), ("T_diff_dn_dir", C.c_double),
+        ("T_dir_up_view", C.c_double),
+        ("T_sg_up_dir", C.c_double), ("T_total_up_dir", C.c_double),
+        ("rho_glint_direct_I", C.c_double), ("rho_glint_direct_Q", C.c_double),
+        ("rho_glint_direct_U", C.c_double),
+        ("kappa", C.c_double),
+        ("n_orders_used", C.c_int), ("converged", C.c_int),
+        ("status", C.c_int), ("n_ring", C.c_int),
+    ]
+
+
+RESULT_FIELDS = [f for f, _ in Result._fields_]
+
+
+def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
+                  surface="flat", n_water=1.34, wind=0.0, sigma_type=1,
+                  q_convention=1, decouple_sunglint=0, tau_R=0.0941, depol=0.0279,
+                  tau_a=0.0, ssa_a=1.0, aer_h_km=2.0, greek=None, n_mu=24,
+                  n_layers=50, m_max=2, max_orders=20, tolerance=1e-7, ipss=0):
+    """Fill one descriptor.  `greek` is a float64 array [4, L+1] (beta, gamma,
+    alpha, zeta) or None; the caller keeps it alive while the descriptor is used."""
+    d = Instance()
+    d.sza_deg, d.vza_deg, d.raa_deg, d.wavelength_nm = sza, vza, raa, wavelength
+    d.surface = SURFACE_NAMES[surface] if isinstance(surface, str) else int(surface)
+    d.decouple_sunglint = int(decouple_sunglint)
+    d.n_water, d.wind_speed = n_water, wind
+    d.sigma_type, d.q_convention = sigma_type, q_convention
+    d.tau_R, d.depol = tau_R, depol
+    d.tau_a, d.ssa_a, d.aer_h_km = tau_a, ssa_a, aer_h_km
+    if greek is not None:
+        g = np.ascontiguousarray(greek, dtype=np.float64)
+        if g.ndim != 2 or g.shape[0] != 4:
+            raise ValueError("greek must have shape [4, L+1]")
+        d.L_max = g.shape[1] - 1
+        d.greek = g.ctypes.data_as(C.POINTER(C.c_double))
+    else:
+        d.L_max = 0
+        d.greek = None
+    d.n_mu, d.n_layers, d.m_max = n_mu, n_layers, m_max
+    d.max_orders, d.tolerance, d.ipss = max_orders, tolerance, ipss
+    return d
+
+
+def instance_array(descs):
+    arr = (Instance * len(descs))()
+    for i, d in enumerate(descs):
+        arr[i] = d
+    return arr
+
+
+def results_to_dict(res):
+    """Array of Result -> dict of numpy arrays."""
+    out = {}
+    for f in RESULT_FIELDS:
+        out[f] = np.array([getattr(r, f) for r in res])
+    return out

@@ -1,0 +1,81 @@
+/*
+ * ocrt_oracle.h -- TEST INFRASTRUCTURE ONLY (the oracle).
+ *
+ * Plain-C, double-precision restatement of the reference OCRT atmospheric
+ * SOS path.  It is the checker of the HIP library: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  It is
+ * never linked into, or called by, the product (libocrt_amd.so).
+ *
+ * Parity status of each part is documented in ocrt_oracle.c next to the
+ * function, with the reference file:line it follows.
+ */
+#ifndef OCRT_ORACLE_H
+#define OCRT_ORACLE_H
+
+#include "ocrt_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Emulation switch for precision-budget studies: when bf16 != 0 the
+ * scattering integral of orders >= 2 rounds the operator coefficients and the
+ * incoming field to bfloat16 before the contraction (the MFMA input rounding
+ * of the mixed form).  Default 0 (pure double precision). */
+void oracle_set_bf16_emulation(int bf16);
+
+/* Full atmospheric solve of one instance (rt_solve_case_pol_impl,
+ * code/OCRT_C/src/rt_solver.c:618-1437).  toa_modes [m_max+1][3], orders and
+ * residuals [m_max+1] are optional.  Unlike the reference, m_max is not
+ * capped at 32. */
+int oracle_solve(const ocrt_instance_t *in, ocrt_result_t *out, double *toa_modes,
+                 int *orders, double *residuals);
+
+/* LUT / full-grid solve (rt_solver.c:1031-1043, 1280-1424). */
+int oracle_solve_lut(const ocrt_instance_t *in, int n_vza, const double *vza,
+                     int n_raa, const double *raa,
+                     double *rho_I, double *rho_Q, double *rho_U,
+                     double *T_diff_dn_dir, double *T_sg_up_dir, double *T_total_up_dir,
+                     double *T_dir_dn, double *T_diff_dn_hemi);
+
+/* OpenMP batch, one instance per thread (rt_io.c:199). */
+int oracle_solve_batch(const ocrt_instance_t *in, int n, ocrt_result_t *out, int n_threads);
+
+/* ---- component entry points used by the unit tests ---- */
+int oracle_gauss_legendre_pos(int n, double *mu, double *w);
+/* ring of positive cosines: GL(n_gauss) plus mu_view inserted (sorted,
+ * dedup 1e-12) when mu_view >= 0; returns ring size, *view_index (0-based). */
+int oracle_ring(int n_gauss, double mu_view, double *mu, double *w, int *view_index);
+/* medium arrays [n_layers+1]; returns 0 on success */
+int oracle_medium(const ocrt_instance_t *in, double *h, double *ch, double *xdel,
+                  double *ydel, double *z_km);
+double oracle_us62_grid_fraction_above(double z_km);
+/* basis tables [(l_max+1)][n_dir] for directions mu[0..n_dir-1] */
+int oracle_basis(int l_max, int m, int n_dir, const double *mu,
+                 double *plm, double *rrl, double *rtl);
+/* 3x3 Cox-Munk reflection Mueller matrix (surface.c:514-646) */
+void oracle_coxmunk_R(double mu_out, double mu_in_dn, double cos_phi, double sin_phi,
+                      double ws, int sigma_type, double n_water, int q_convention,
+                      double *R);
+void oracle_flat_fresnel(double mu, double n_water, int q_convention, double *MF);
+/* Fourier kernels of the Cox-Munk BRDF for all m <= m_max at once:
+ * out [(m_max+1)][n_o][n_i][9] (sign fold for m > 0 applied). */
+int oracle_coxmunk_fourier(const double *mu_o, int n_o, const double *mu_i, int n_i,
+                           int m_max, int n_phi, double ws, int sigma_type,
+                           double n_water, int q_convention, double *out);
+/* IPSS kappa for an instance at (vza, raa) (rt_ipss.c:607-644) */
+double oracle_ipss_kappa(const ocrt_instance_t *in, double vza_deg, double raa_deg);
+double oracle_ipss_tau_dir_dn(const ocrt_instance_t *in);
+/* Fourier reconstruction (rt_fourier.c:9-33) */
+double oracle_reconstruct_cos(const double *modes, int m_max, double raa_rad);
+double oracle_reconstruct_sin(const double *modes, int m_max, double raa_rad);
+/* Complete the spin-2 families from ordinary Legendre moments
+ * (rt_phase_moments.c:22-64). */
+int oracle_complete_spherical(int l_max, const double *beta11, const double *beta22,
+                              const double *delta33, double *alpha, double *zeta);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
