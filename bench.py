@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X SOS solver on the batched uneven workload.
+
+Workload (one step = one solve of the whole batch, inputs resident on the GPU):
+  8 bands (412..865 nm) x 1024 geometries = 8192 instances drawn from one
+  seed: SZA U(0,80), VZA U(0,70), RAA U(0,180), AOD(865) logU(0.01,0.5),
+  Angstrom U(0,2), HG g U(0.6,0.8) with 80 Legendre terms, SSA U(0.85,1),
+  wind U(1,15) m/s rough ocean, 50..400 layers, 16..64 streams, Fourier
+  modes 0..16.  Headline value: solves/s of the mixed form (bf16 MFMA
+  scattering integral, orders >= 2).
+
+Also reported: the FP64 form (batched and one instance per launch), the
+error of the mixed form against the FP64 form, the single-instance
+Rayleigh + aerosol case (400 layers, 48 streams, modes 0..95, IPSS), the
+roofline of the dominant kernel from HIP events over the timed steps, and
+the reference CPU solver on a bounded sample of the same workload.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TOA Stokes solves/s at <=1e-4 rel err (mixed vs FP64 form); adjoint/fwd cost"
+BF16_PEAK_TFLOPS = 2500.0      # MI355X dense bf16 MFMA
+FP64_PEAK_TFLOPS = 78.6        # MI355X fp64 matrix
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+
+
+def log(msg):
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def timed_steps(plan, steps, warmup, sync):
+    for _ in range(warmup):
+        plan.execute()
+        sync()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute()
+    sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_baseline(descs, n_sample, threads):
+    """Reference solver (oracle/_ref, built from the reference sources) on the
+    first n_sample instances, OpenMP over instances as rt_io_run_batch does;
+    falls back to the C restatement when the reference build is absent."""
+    from ocrt_amd.abi import Result, instance_array
+    sample = descs[:n_sample]
+    arr = instance_array(sample)
+    out = (Result * len(sample))()
+    ref = os.path.join(ROOT, "oracle", "_ref", "libocrt_ref.so")
+    if os.path.exists(ref):
+        lib, fn, kind = C.CDLL(ref), "ref_solve_batch", "reference"
+    else:
+        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            import subprocess
+            subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
+                           stdout=subprocess.DEVNULL)
+        lib, fn, kind = C.CDLL(path), "oracle_solve_batch", "port"
+    t0 = time.perf_counter()
+    failed = getattr(lib, fn)(arr, len(sample), out, threads)
+    dt = time.perf_counter() - t0
+    return {"value": len(sample) / dt, "unit": "solves/s", "cores": threads, "kind": kind,
+            "sample": f"first {len(sample)} instances of the workload ({failed} failed), "
+                      f"OpenMP dynamic over instances, {dt:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--geoms", type=int, default=1024, help="geometries per band (workload size)")
+    ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (instances are independent; a "
+              "multi-GPU run would shard them with no exchange). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ocrt_amd
+    from ocrt_amd import synthetic as S
+
+    sync = torch.cuda.synchronize
+    log("building workload")
+    descs, keep = S.config2(n_geom=args.geoms)
+    n = len(descs)
+
+    log(f"mixed plan over {n} instances")
+    pm = ocrt_amd.Plan(descs, precision="mixed")
+    pm.profile(True)
+    ms_mixed = timed_steps(pm, args.steps, args.warmup, sync) * 1e3
+    kt = pm.kernel_times()
+    res_m = pm.results()
+    work = pm.work()
+    log(f"mixed: {ms_mixed:.1f} ms/step, kernels {kt}")
+
+    log("fp64 batched plan")
+    pf = ocrt_amd.Plan(descs, precision="fp64")
+    ms_fp64 = timed_steps(pf, 1, 1, sync) * 1e3
+    res_f = pf.results()
+    pf.close()
+    log(f"fp64 batched: {ms_fp64:.1f} ms/step")
+
+    ok = (res_m["status"] == 0) & (res_f["status"] == 0)
+    err_I = float(np.max(np.abs(res_m["rho_I"] - res_f["rho_I"])[ok] / np.abs(res_f["rho_I"])[ok]))
+    err_QU = float(max(np.max(np.abs(res_m["rho_Q"] - res_f["rho_Q"])[ok]),
+                       np.max(np.abs(res_m["rho_U"] - res_f["rho_U"])[ok])))
+
+    log(f"fp64 one instance per launch on {args.plain_sample} instances")
+    pp = ocrt_amd.Plan(descs[: args.plain_sample], precision="fp64", one_per_launch=True)
+    ms_plain = timed_steps(pp, 1, 1, sync) * 1e3
+    pp.close()
+    plain_rate = args.plain_sample / (ms_plain / 1e3)
+
+    single = None
+    if not args.no_single:
+        log("single instance: Rayleigh + aerosol, 400 layers, 48 streams, modes 0..95, IPSS")
+        d1, k1 = S.config1()
+        single = {}
+        for prec in ("mixed", "fp64"):
+            p1 = ocrt_amd.Plan(d1, precision=prec)
+            t = timed_steps(p1, 3, 1, sync)
+            single[prec] = {"ms": t * 1e3, "solves_per_s": 1.0 / t, "rho_I": float(p1.results()["rho_I"][0])}
+            p1.close()
+        single["rel_err_I_mixed_vs_fp64"] = abs(single["mixed"]["rho_I"] - single["fp64"]["rho_I"]) / abs(
+            single["fp64"]["rho_I"])
+
+    value = n / (ms_mixed / 1e3)
+    gemm_s = kt["gemm_ms"] / 1e3
+    sweep_s = kt["sweep_ms"] / 1e3
+    gemm_tf = work["gemm_flops"] / gemm_s / 1e12 if gemm_s > 0 else 0.0
+    sweep_gbs = work["sweep_bytes"] / sweep_s / 1e9 if sweep_s > 0 else 0.0
+    rl_gemm = {"kernel": "gemm_bf16", "bound": "mfma", "achieved": gemm_tf, "peak": BF16_PEAK_TFLOPS,
+               "unit": "TFLOP/s", "frac": gemm_tf / BF16_PEAK_TFLOPS, "traffic": None,
+               "ms_total": kt["gemm_ms"], "launches": kt["gemm_launches"]}
+    rl_sweep = {"kernel": "sweep<float,true>", "bound": "hbm", "achieved": sweep_gbs, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS, "traffic": None,
+                "ms_total": kt["sweep_ms"], "launches": kt["sweep_launches"]}
+    roofline = rl_gemm if kt["gemm_ms"] >= kt["sweep_ms"] else rl_sweep
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"cpu baseline on {args.cpu_sample} instances, {threads} threads")
+        cpu = cpu_baseline(descs, args.cpu_sample, threads)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_mixed,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched uneven SOS solves: 8 bands x {args.geoms} geometries = {n} instances, "
+                        "rough ocean, 50-400 layers, 16-64 streams, modes 0..16, HG aerosol L=80",
+            "instances": n,
+            "form": "mixed (split-bf16 MFMA scattering integral, fp32 sweeps for orders >= 2, fp64 elsewhere)",
+        },
+        "fp64_batched_solves_per_s": n / (ms_fp64 / 1e3),
+        "fp64_plain_solves_per_s": plain_rate,
+        "mixed_over_plain_fp64": value / plain_rate,
+        "mixed_over_fp64_batched": ms_fp64 / ms_mixed,
+        "max_rel_err_I_vs_fp64": err_I,
+        "max_abs_err_QU_vs_fp64": err_QU,
+        "adjoint_fwd_ratio": None,
+        "single_instance_config1": single,
+        "roofline": roofline,
+        "roofline_other": rl_sweep if roofline is rl_gemm else rl_gemm,
+        "cpu_baseline": cpu,
+        "device_bytes": pm.device_bytes(),
+    }
+    pm.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

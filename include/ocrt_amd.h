@@ -132,6 +132,13 @@ size_t ocrt_plan_device_bytes(const ocrt_plan *plan);
  * roofline fractions); either pointer may be NULL. */
 int  ocrt_plan_work(const ocrt_plan *plan, double *gemm_flops, double *sweep_bytes,
                     double *gemm_launches);
+/* Per-kernel timing for roofline reports: when enabled, every scattering-
+ * integral (GEMM) and layer-sweep launch of the following executes is
+ * bracketed by HIP events on the execution stream; ocrt_plan_kernel_times
+ * returns the summed durations (ms) and launch counts of the last execute. */
+int  ocrt_plan_profile(ocrt_plan *plan, int enable);
+int  ocrt_plan_kernel_times(ocrt_plan *plan, double *gemm_ms, int *gemm_launches,
+                            double *sweep_ms, int *sweep_launches);
 void ocrt_plan_destroy(ocrt_plan *plan);
 
 /* One-call form: plan, execute, copy results, destroy. */

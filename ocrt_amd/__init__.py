@@ -1,0 +1,153 @@
+"""MI355X-native successive-orders-of-scattering solver (OCRT atmospheric path).
+
+The compute path is the HIP library libocrt_amd.so built by ocrt_amd.build;
+this module only loads it and marshals descriptors.  There is no CPU fallback:
+every entry point raises when the library or a GPU is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (Instance, Result, RESULT_FIELDS, PREC_FP64, PREC_MIXED, make_instance,
+                  instance_array, results_to_dict)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libocrt_amd.so")
+_lib = None
+
+PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
+
+
+class OcrtError(RuntimeError):
+    pass
+
+
+def library():
+    """Load libocrt_amd.so (built in-tree by ocrt_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OcrtError(f"{LIB_PATH} is missing: run `python -m ocrt_amd.build` first")
+        # One HIP runtime per process: torch ships its own libamdhip64 with the
+        # same soname (libamdhip64.so.7).  Loading torch first makes the solver
+        # library bind to that runtime; loading it first would let torch open a
+        # second runtime that no longer sees the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        lib = C.CDLL(LIB_PATH)
+        lib.ocrt_plan_create.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.c_int,
+                                         C.POINTER(C.c_void_p)]
+        lib.ocrt_plan_execute.argtypes = [C.c_void_p, C.c_void_p]
+        lib.ocrt_plan_results.argtypes = [C.c_void_p, C.POINTER(Result), C.c_void_p, C.c_void_p]
+        lib.ocrt_plan_mmax.argtypes = [C.c_void_p]
+        lib.ocrt_plan_device_bytes.argtypes = [C.c_void_p]
+        lib.ocrt_plan_device_bytes.restype = C.c_size_t
+        lib.ocrt_plan_work.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+        lib.ocrt_plan_profile.argtypes = [C.c_void_p, C.c_int]
+        lib.ocrt_plan_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                               C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        lib.ocrt_plan_destroy.argtypes = [C.c_void_p]
+        lib.ocrt_solve_batch.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.POINTER(Result),
+                                         C.c_void_p]
+        lib.ocrt_version.restype = C.c_char_p
+        _lib = lib
+    return _lib
+
+
+def _require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise OcrtError("the OCRT MI355X solver needs a ROCm GPU; none is visible")
+
+
+class Plan:
+    """A solve plan over a list of descriptors (ocrt_amd.abi.Instance).
+
+    precision: "fp64" (plain parity form) or "mixed" (bf16 MFMA scattering
+    integral for orders >= 2).  one_per_launch=True runs instances one after
+    another (the plain one-instance-per-launch form)."""
+
+    def __init__(self, descs, precision="mixed", one_per_launch=False):
+        _require_gpu()
+        self.lib = library()
+        self.n = len(descs)
+        self._arr = instance_array(descs)
+        self._handle = C.c_void_p()
+        rc = self.lib.ocrt_plan_create(self._arr, self.n, PRECISIONS[precision],
+                                       1 if one_per_launch else 0, C.byref(self._handle))
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_create failed with code {rc}")
+        self.mmax = self.lib.ocrt_plan_mmax(self._handle)
+
+    def execute(self, stream=None):
+        ptr = None
+        if stream is not None:
+            ptr = C.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+        rc = self.lib.ocrt_plan_execute(self._handle, ptr)
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_execute failed with code {rc}")
+
+    def results(self, modes=False):
+        res = (Result * self.n)()
+        M = self.mmax + 1
+        tm = np.zeros((self.n, M, 3)) if modes else None
+        orders = np.zeros((self.n, M), dtype=np.int32) if modes else None
+        rc = self.lib.ocrt_plan_results(
+            self._handle, res,
+            tm.ctypes.data_as(C.c_void_p) if modes else None,
+            orders.ctypes.data_as(C.c_void_p) if modes else None)
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_results failed with code {rc}")
+        out = results_to_dict(res)
+        if modes:
+            out["toa_modes"] = tm
+            out["orders_per_m"] = orders
+        return out
+
+    def work(self):
+        f, b, l = C.c_double(), C.c_double(), C.c_double()
+        self.lib.ocrt_plan_work(self._handle, C.byref(f), C.byref(b), C.byref(l))
+        return {"gemm_flops": f.value, "sweep_bytes": b.value, "order_steps": l.value}
+
+    def profile(self, enable=True):
+        self.lib.ocrt_plan_profile(self._handle, 1 if enable else 0)
+
+    def kernel_times(self):
+        gm, sm = C.c_double(), C.c_double()
+        gl, sl = C.c_int(), C.c_int()
+        rc = self.lib.ocrt_plan_kernel_times(self._handle, C.byref(gm), C.byref(gl), C.byref(sm), C.byref(sl))
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_kernel_times failed with code {rc}")
+        return {"gemm_ms": gm.value, "gemm_launches": gl.value, "sweep_ms": sm.value,
+                "sweep_launches": sl.value}
+
+    def device_bytes(self):
+        return self.lib.ocrt_plan_device_bytes(self._handle)
+
+    def close(self):
+        if self._handle:
+            self.lib.ocrt_plan_destroy(self._handle)
+            self._handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve(descs, precision="mixed", modes=False, one_per_launch=False):
+    """Solve a batch of instances on the GPU; returns a dict of numpy arrays
+    keyed by the result fields of include/ocrt_amd.h."""
+    plan = Plan(descs, precision=precision, one_per_launch=one_per_launch)
+    plan.execute()
+    out = plan.results(modes=modes)
+    plan.close()
+    return out
+
+
+__all__ = ["Plan", "solve", "library", "make_instance", "OcrtError", "PREC_FP64", "PREC_MIXED"]

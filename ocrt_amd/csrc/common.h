@@ -1,0 +1,90 @@
+// Shared device/host definitions of the MI355X SOS solver.
+//
+// Memory layout (one "chunk" of instances is resident at a time):
+//   per instance : level arrays h, ch, xdel, ydel, z [L = nt + 1]; ring cosines
+//                  and weights [n]; Beer-Lambert factors T[layer][node]
+//                  (layer-major so a wavefront reads consecutive nodes);
+//                  Greek arrays [4][La+1]; rough-surface Fourier kernels
+//                  R[m][o][i][9] and solar kernels Rsun[m][o][9].
+//   per item     : one (instance, Fourier mode m) pair.  Angular basis
+//                  plm/rrl/rtl [(lmax+1)][2n+1], six phase tables
+//                  [(n+1)][2n+1], the dense scattering operator A[Mp][K]
+//                  and the fields F (previous order), J (source), total
+//                  (fp64 order sum) and comp (compensation, mixed form).
+//   Field row k (level) holds 6n state columns ordered
+//     [I_up(n) I_dn(n) Q_up(n) Q_dn(n) U_up(n) U_dn(n)], node o = 0..n-1
+//   padded to the row pitch P; pad columns stay zero.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define OCRT_PI 3.14159265358979323846
+
+namespace ocrt {
+
+constexpr int kNPhi = 1024;      // Cox-Munk azimuth rule (rt_surface_boundary.c:111)
+constexpr int kMaxRing = 640;    // ring cap (rt_angle_grid.h:11)
+
+struct DevInst {
+    double mus, muv, sza, vza, raa;
+    double tauR, taua, ssa, aer_h, n_water, wind, tol;
+    double beta0, beta2, gamma2, alpha2;      // Rayleigh moments
+    int n, vidx, nt, mmax, La, lmax, aer, ray;
+    int surf;            // 0 none, 1 flat, 2 rough
+    int sigma_type, q_conv, decouple, ipss, max_orders, item0, surface_kind, surf_owner;
+    int pad_;
+    long long off_lev;   // pool_lev: 5 arrays of L doubles (h, ch, xd, yd, z)
+    long long off_ring;  // pool_ring: mu[n], w[n]
+    long long off_T;     // pool_T (double) / pool_Tf (float): [nt][n]
+    long long off_greek; // pool_greek: [4][La+1]
+    long long off_R;     // pool_R: [(mmax+1)][n][n][9]
+    long long off_Rsun;  // pool_R: [(mmax+1)][n][9]
+};
+
+struct DevItem {
+    int inst, m, n, nt, L, P, K, parts;   // parts: bit0 particle, bit1 molecular
+    int Mp, tiles_m, tiles_n, lmax;
+    long long off_basis;  // pool_basis: 3 x [(lmax+1)][2n+1]
+    long long off_tab;    // pool_tab: 6 x [(n+1)][2n+1]
+    long long off_A;      // pool_A: [Mp][K]
+    long long off_F;      // pool_F / pool_J: [L][P]
+    long long off_tot;    // pool_tot / pool_comp: [L][P]
+    long long off_bc;     // pool_bc: [3][n]
+};
+
+struct ItemState {
+    int active, orders, converged, status;
+    double residual;
+    unsigned long long max_cur, max_tot;   // bit patterns of non-negative doubles
+};
+
+// Per-item outputs gathered after the order loop.
+struct ItemOut {
+    double mode[3];       // TOA (I, Q, U)^m at the view node
+    double boa_view;      // I^m downward at BOA, view node
+    double boa_hemi;      // sum_o mu_o w_o I^m downward at BOA
+    int orders, converged;
+    double residual;
+};
+
+struct InstOut {
+    double kappa, tau_dir_ipss;
+};
+
+struct Pools {
+    double *lev, *ring, *T, *greek, *R, *basis, *tab, *tot, *comp, *bc;
+    float *Tf;            // fp32 1 - T for the mixed-form sweeps
+    void *A, *F, *J;      // double or bf16/float depending on the form
+    double *cm_w;         // cos/sin(m phi) table [(Mall+1)][2][kNPhi]
+    double *toa_up;       // optional [items][3][n] TOA upward at every node (LUT)
+    int toa_up_stride;
+};
+
+struct GemmTile { int item, tm, tn, pad; };
+
+// One IPSS kappa evaluation: instance and viewing direction.
+struct IpssJob { int inst; int pad; double vza, raa; };
+
+__host__ __device__ inline int roundup(int x, int a) { return (x + a - 1) / a * a; }
+
+}  // namespace ocrt

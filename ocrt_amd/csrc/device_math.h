@@ -1,0 +1,208 @@
+// Device-side fp64 helpers: US Standard Atmosphere 1962 column profile,
+// Fresnel and Cox-Munk surface Mueller matrices, storage conversions.
+#pragma once
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+
+namespace ocrt {
+
+// US62 anchors (rt_molecular_profile.c:10-16) and WGS84 gravity (rt_rayleigh.c:46-52).
+__constant__ static const double c_us62_z[8] = {0.0, 11.0, 20.0, 32.0, 47.0, 51.0, 71.0, 84.852};
+__constant__ static const double c_us62_p[8] = {101325.0, 22632.1, 5474.89, 868.019,
+                                                110.906, 66.9389, 3.95642, 0.3734};
+
+__device__ inline double gravity_wgs84(double lat_deg, double alt_m)
+{
+    const double lat = lat_deg * OCRT_PI / 180.0;
+    const double s = sin(lat) * sin(lat);
+    const double s2 = sin(2.0 * lat) * sin(2.0 * lat);
+    const double g0 = 9.780327 * (1.0 + 0.0053024 * s - 0.0000058 * s2);
+    return g0 - 3.086e-6 * alt_m;
+}
+
+__device__ inline double clamp01(double x) { return x <= 0.0 ? 0.0 : (x >= 1.0 ? 1.0 : x); }
+
+__device__ inline double us62_pressure(double z)
+{
+    if (z <= c_us62_z[0]) return c_us62_p[0];
+    if (z >= c_us62_z[7]) return c_us62_p[7];
+    int lo = 0, hi = 7;
+    while (hi - lo > 1) {
+        const int mid = lo + (hi - lo) / 2;
+        if (z < c_us62_z[mid]) hi = mid; else lo = mid;
+    }
+    const double t = (z - c_us62_z[lo]) / (c_us62_z[hi] - c_us62_z[lo]);
+    const double a = log(c_us62_p[lo]), b = log(c_us62_p[hi]);
+    return exp(a + t * (b - a));
+}
+
+__device__ inline double us62_fraction_above(double z)
+{
+    if (z <= 0.0) return 1.0;
+    const double g0 = gravity_wgs84(45.0, 0.0);
+    if (z >= c_us62_z[7]) {
+        const double gt = gravity_wgs84(45.0, c_us62_z[7] * 1000.0);
+        return clamp01((c_us62_p[7] / gt) / (c_us62_p[0] / g0));
+    }
+    const double gz = gravity_wgs84(45.0, z * 1000.0);
+    return clamp01((us62_pressure(z) / gz) / (c_us62_p[0] / g0));
+}
+
+__device__ inline double us62_grid_fraction_above(double z)
+{
+    if (z <= 0.0) return 1.0;
+    if (z >= c_us62_z[7]) return 0.0;
+    const double ft = us62_fraction_above(c_us62_z[7]);
+    return clamp01((us62_fraction_above(z) - ft) / (1.0 - ft));
+}
+
+__device__ inline double us62_altitude_from_fraction(double f)
+{
+    if (f >= 1.0) return 0.0;
+    if (f <= 0.0) return c_us62_z[7];
+    double lo = 0.0, hi = c_us62_z[7];
+    for (int it = 0; it < 64; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (us62_grid_fraction_above(mid) > f) lo = mid; else hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+__device__ inline double aerosol_fraction_above(double z, double ztop, double H)
+{
+    if (z <= 0.0) return 1.0;
+    if (z >= ztop) return 0.0;
+    const double et = exp(-ztop / H);
+    return (exp(-z / H) - et) / (1.0 - et);
+}
+
+__host__ __device__ inline double diag_norm(int m)
+{
+    double v = 1.0;
+    for (int i = 1; i <= m; ++i) v *= sqrt((double)(i + m) / (double)i) * 0.5;
+    return v;
+}
+
+// Slope variance of the isotropic Gaussian facet distribution (surface.c:57-64).
+__device__ inline double slope_variance(double ws, int sigma_type)
+{
+    if (sigma_type == 1) return 0.003 + 0.00512 * fmax(0.01, ws);
+    const double s = 0.0731 * sqrt(fmax(0.0, ws));
+    return fmax(s * s, 1e-10);
+}
+
+// Air-side Fresnel reflection Mueller matrix, meridian basis (surface.c:385-468).
+__device__ inline void fresnel_R(double mu_i, double n_water, int qc, double *M)
+{
+    for (int i = 0; i < 9; ++i) M[i] = 0.0;
+    const double sin2 = fmax(0.0, 1.0 - mu_i * mu_i);
+    const double st2 = (1.0 / n_water) * (1.0 / n_water) * sin2;
+    if (st2 >= 1.0) { M[0] = M[4] = M[8] = 1.0; return; }
+    const double mt = sqrt(1.0 - st2);
+    const double rs = (mu_i - n_water * mt) / (mu_i + n_water * mt);
+    const double rp = (n_water * mu_i - mt) / (n_water * mu_i + mt);
+    const double Rs = rs * rs, Rp = rp * rp;
+    const double q = (qc == 1) ? 0.5 * (Rp - Rs) : 0.5 * (Rs - Rp);
+    M[0] = 0.5 * (Rs + Rp); M[1] = q; M[3] = q; M[4] = 0.5 * (Rs + Rp); M[8] = rs * rp;
+}
+
+// Cox-Munk Mueller BRDF with Sancer shadowing and the meridian/facet-plane
+// rotations, including the exact pole limits (surface.c:514-646).
+// Sancer bistatic shadowing S = 1 / (1 + Lambda_i + Lambda_v) (surface.c:591-612);
+// it depends on the two zenith cosines only.
+__device__ inline double sancer_shadow(double mu_out, double mu_in_dn, double s2v)
+{
+    const double sl = sqrt(s2v);
+    double li = 0.0, lv = 0.0;
+    if (mu_in_dn < 1.0 - 1e-10) {
+        const double si = sqrt(fmax(1e-20, 1.0 - mu_in_dn * mu_in_dn));
+        const double nu = (mu_in_dn / si) / sl;
+        if (nu <= 20.0) li = 0.5 * (exp(-nu * nu) / (nu * sqrt(OCRT_PI)) - erfc(nu));
+    }
+    if (mu_out < 1.0 - 1e-10) {
+        const double sv = sqrt(fmax(1e-20, 1.0 - mu_out * mu_out));
+        const double nu = (mu_out / sv) / sl;
+        if (nu <= 20.0) lv = 0.5 * (exp(-nu * nu) / (nu * sqrt(OCRT_PI)) - erfc(nu));
+    }
+    return 1.0 / (1.0 + li + lv);
+}
+
+// Body of the Cox-Munk Mueller BRDF for a given slope variance and shadowing
+// factor (shadow < 0: computed here).
+__device__ inline void coxmunk_R_core(double mu_out, double mu_in_dn, double cphi, double sphi, double s2v,
+                                      double shadow, double nw, int qc, double *R)
+{
+    for (int i = 0; i < 9; ++i) R[i] = 0.0;
+    const double mia = -mu_in_dn;
+    const double s1 = sqrt(fmax(0.0, 1.0 - mu_out * mu_out));
+    const double s2 = sqrt(fmax(0.0, 1.0 - mia * mia));
+    double cT = mu_out * mia + s1 * s2 * cphi;
+    cT = fmin(1.0, fmax(-1.0, cT));
+    const double co = sqrt(fmax(0.0, 0.5 * (1.0 - cT)));
+    if (co <= 1e-6) return;
+    const double cb = (mu_out + mu_in_dn) / (2.0 * co);
+    if (cb <= 1e-6) return;
+    const double sT = sqrt(fmax(0.0, 1.0 - cT * cT));
+    const double den = fmax(1e-12, sT);
+    double ci1, si1, ci2, si2;
+    if (s2 <= 1.0e-8) {
+        ci1 = -copysign(1.0, mia) * cphi; si1 = sphi;
+    } else {
+        ci1 = (mu_out - mia * cT) / (den * fmax(1e-12, s2));
+        si1 = s1 * sphi / den;
+        const double q = sqrt(ci1 * ci1 + si1 * si1);
+        if (q < 1e-12) { ci1 = 1.0; si1 = 0.0; } else { ci1 /= q; si1 /= q; }
+    }
+    if (s1 <= 1.0e-8) {
+        ci2 = -copysign(1.0, mu_out) * cphi; si2 = sphi;
+    } else {
+        ci2 = (mia - mu_out * cT) / (den * fmax(1e-12, s1));
+        si2 = s2 * sphi / den;
+        const double q = sqrt(ci2 * ci2 + si2 * si2);
+        if (q < 1e-12) { ci2 = 1.0; si2 = 0.0; } else { ci2 /= q; si2 /= q; }
+    }
+    double MF[9];
+    fresnel_R(co, nw, qc, MF);
+    const double cb2 = cb * cb;
+    const double tb2 = (1.0 - cb2) / fmax(cb2, 1e-8);
+    const double Ps = (1.0 / (OCRT_PI * s2v)) * exp(-tb2 / s2v);
+    const double S = shadow >= 0.0 ? shadow : sancer_shadow(mu_out, mu_in_dn, s2v);
+    const double brdf = (Ps * S) / (4.0 * mu_in_dn * mu_out * cb2 * cb2);
+    // R = L(i2) . MF . L(i1) . brdf with L(i) = diag(1, rot2(2i))
+    const double c1 = ci1 * ci1 - si1 * si1, sn1 = 2.0 * ci1 * si1;
+    const double c2 = ci2 * ci2 - si2 * si2, sn2 = 2.0 * ci2 * si2;
+    const double L1[9] = {1, 0, 0, 0, c1, -sn1, 0, sn1, c1};
+    const double L2[9] = {1, 0, 0, 0, c2, -sn2, 0, sn2, c2};
+    double T1[9];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += MF[a * 3 + k] * L1[k * 3 + b];
+            T1[a * 3 + b] = s;
+        }
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += L2[a * 3 + k] * T1[k * 3 + b];
+            R[a * 3 + b] = s * brdf;
+        }
+}
+
+__device__ inline void coxmunk_R(double mu_out, double mu_in_dn, double cphi, double sphi, double ws,
+                                 int sigma_type, double nw, int qc, double *R)
+{
+    coxmunk_R_core(mu_out, mu_in_dn, cphi, sphi, slope_variance(ws, sigma_type), -1.0, nw, qc, R);
+}
+
+template <typename T> __device__ inline T to_storage(double v);
+template <> __device__ inline double to_storage<double>(double v) { return v; }
+template <> __device__ inline float to_storage<float>(double v) { return (float)v; }
+template <> __device__ inline __hip_bfloat16 to_storage<__hip_bfloat16>(double v)
+{
+    return __float2bfloat16((float)v);
+}
+
+__device__ inline double as_double(double v) { return v; }
+__device__ inline double as_double(float v) { return (double)v; }
+
+}  // namespace ocrt

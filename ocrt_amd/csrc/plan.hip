@@ -1,0 +1,642 @@
+// Host side of the MI355X SOS solver: instance validation, quadrature rings,
+// device memory layout, chunking of large batches and the launch sequence.
+// Implements the C ABI declared in include/ocrt_amd.h.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include <hip/hip_bf16.h>
+
+#include "common.h"
+#include "ocrt_amd.h"
+
+namespace ocrt {
+// kernels (setup.hip, sos.hip, ipss.hip)
+__global__ void medium_levels(const DevInst *, Pools);
+__global__ void medium_fractions(const DevInst *, Pools);
+__global__ void fourier_weights(double *, int);
+__global__ void coxmunk_fourier(const DevInst *, Pools);
+__global__ void angular_basis(const DevInst *, const DevItem *, Pools);
+__global__ void phase_tables(const DevInst *, const DevItem *, Pools);
+template <typename AT> __global__ void pack_operator(const DevInst *, const DevItem *, Pools);
+template <typename FT, bool MIXED> __global__ void order_one(const DevInst *, const DevItem *, Pools, ItemState *);
+template <typename FT> __global__ void order_status(const DevInst *, const DevItem *, Pools, ItemState *, int);
+__global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *);
+__global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *);
+template <typename FT, bool MIXED> __global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *);
+__global__ void extract(const DevInst *, const DevItem *, Pools, const ItemState *, ItemOut *, int);
+__global__ void finalize(const DevInst *, int, const ItemOut *, const InstOut *, const Pools, ocrt_result_t *);
+__global__ void ipss_kappa(const DevInst *, const IpssJob *, Pools, double *, double *);
+}  // namespace ocrt
+
+using namespace ocrt;
+
+namespace {
+
+// Positive half of the 2n-point Gauss-Legendre rule, ascending, weights summing
+// to one: Newton iteration from Tricomi's estimate on the three-term
+// recurrence (rt_quadrature.c:20-61).
+int gauss_legendre_pos(int n, double *mu, double *w)
+{
+    const int N = 2 * n;
+    for (int k = 1; k <= n; ++k) {
+        double t = cos(OCRT_PI * ((double)k - 0.25) / ((double)N + 0.5));
+        double pn = 0, pm = 0, dp = 0;
+        bool ok = false;
+        auto eval = [&](double x) {
+            double p0 = 1.0, p1 = x;
+            for (int q = 1; q < N; ++q) {
+                const double p2 = ((2.0 * q + 1.0) * x * p1 - (double)q * p0) / (double)(q + 1);
+                p0 = p1;
+                p1 = p2;
+            }
+            pn = p1;
+            pm = p0;
+        };
+        for (int it = 0; it < 100; ++it) {
+            eval(t);
+            dp = (double)N * (t * pn - pm) / (t * t - 1.0);
+            const double dt = -pn / dp;
+            t += dt;
+            if (fabs(dt) < 1.0e-15 * (fabs(t) + 1.0)) { ok = true; break; }
+        }
+        if (!ok) return -1;
+        eval(t);
+        dp = (double)N * (t * pn - pm) / (t * t - 1.0);
+        mu[n - k] = t;
+        w[n - k] = 2.0 / ((1.0 - t * t) * dp * dp);
+    }
+    return 0;
+}
+
+// Insert the view cosine at its sorted position with zero weight unless a
+// node lies within 1e-12 (rt_angle_grid.c:21-73).  Returns the ring size.
+int build_ring(int n_gl, double mu_view, std::vector<double> &mu, std::vector<double> &w, int &vidx)
+{
+    mu.assign(n_gl + 1, 0.0);
+    w.assign(n_gl + 1, 0.0);
+    if (gauss_legendre_pos(n_gl, mu.data(), w.data()) != 0) return -1;
+    int n = n_gl;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (mu[mid] < mu_view) lo = mid + 1; else hi = mid;
+    }
+    const int pos = lo;
+    if (pos > 0 && fabs(mu[pos - 1] - mu_view) < 1.0e-12) { vidx = pos - 1; mu.resize(n); w.resize(n); return n; }
+    if (pos < n && fabs(mu[pos] - mu_view) < 1.0e-12) { vidx = pos; mu.resize(n); w.resize(n); return n; }
+    for (int i = n; i > pos; --i) { mu[i] = mu[i - 1]; w[i] = w[i - 1]; }
+    mu[pos] = mu_view;
+    w[pos] = 0.0;
+    vidx = pos;
+    return n + 1;
+}
+
+struct HostInst {
+    DevInst d;           // offsets of chunk-local pools filled per chunk
+    std::vector<double> mu, w;
+    int status;
+    long long lev_n, T_n, R_n, greek_n;
+};
+
+struct Chunk {
+    int i0, i1, it0, it1;       // instance and item ranges
+    std::vector<DevInst> hinst;
+    std::vector<DevItem> hitem;
+    std::vector<GemmTile> htiles;
+    std::vector<IpssJob> hjobs;
+    DevInst *dinst = nullptr;
+    DevItem *ditem = nullptr;
+    GemmTile *dtiles = nullptr;
+    IpssJob *djobs = nullptr;
+    int *djob_of = nullptr;
+    int max_L = 0, max_P = 0, max_pairs = 0, max_tab = 0, maxit = 1, any_rough = 0, n_jobs = 0;
+    long long max_pack = 0;
+    long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0;
+};
+
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) {                                                           \
+            fprintf(stderr, "ocrt_amd: %s failed: %s\n", #x, hipGetErrorString(e_));      \
+            return OCRT_ERR_DEVICE;                                                       \
+        }                                                                                 \
+    } while (0)
+
+}  // namespace
+
+struct ocrt_plan {
+    int n_inst = 0, prec = 0, per_inst = 0, mmax_all = 0, n_items = 0;
+    std::vector<HostInst> inst;
+    std::vector<Chunk> chunks;
+    std::vector<int> item_of_inst;   // first global item of each instance
+    // global device arrays
+    double *ring = nullptr, *greek = nullptr, *cm_w = nullptr;
+    ItemState *state = nullptr;
+    ItemOut *iout = nullptr;
+    InstOut *ipss = nullptr;
+    ocrt_result_t *res = nullptr;
+    double *kappa = nullptr, *taudir = nullptr;
+    // chunk pools (sized for the largest chunk)
+    Pools pools{};
+    size_t dev_bytes = 0;
+    std::vector<void *> allocs;
+    double gemm_flops = 0.0, gemm_flops_alg = 0.0, sweep_bytes = 0.0, gemm_launches = 0.0;
+    // optional per-kernel timing: events recorded around every GEMM and sweep launch
+    int profile = 0;
+    std::vector<hipEvent_t> ev;       // pairs (start, stop): [gemm..., sweep...]
+    std::vector<int> ev_kind;         // 0 gemm, 1 sweep, per pair
+    size_t ev_used = 0;
+};
+
+static hipEvent_t *ev_pair(ocrt_plan *p, int kind)
+{
+    if (p->ev_used + 2 > p->ev.size()) {
+        for (int q = 0; q < 64; ++q) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            p->ev.push_back(e);
+        }
+        p->ev_kind.resize(p->ev.size() / 2, 0);
+    }
+    hipEvent_t *e = &p->ev[p->ev_used];
+    p->ev_kind[p->ev_used / 2] = kind;
+    p->ev_used += 2;
+    return e;
+}
+
+static int dmalloc(ocrt_plan *p, void **ptr, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    bytes = (bytes + 255) & ~(size_t)255;
+    const hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        fprintf(stderr, "ocrt_amd: hipMalloc(%zu) failed: %s\n", bytes, hipGetErrorString(e));
+        *ptr = nullptr;
+        return OCRT_ERR_ALLOC;
+    }
+    p->allocs.push_back(*ptr);
+    p->dev_bytes += bytes;
+    return 0;
+}
+
+static int validate(const ocrt_instance_t &d)
+{
+    if (!(d.wavelength_nm >= 330.0 - 1e-9 && d.wavelength_nm <= 1100.0 + 1e-9)) return OCRT_ERR_ARGS;
+    if (!(d.sza_deg >= 0.0 && d.sza_deg < 90.0)) return OCRT_ERR_ARGS;
+    if (!(d.vza_deg >= 0.0 && d.vza_deg < 90.0)) return OCRT_ERR_ARGS;
+    if (d.m_max < 0 || d.m_max > 1024) return OCRT_ERR_ARGS;
+    if (d.n_mu < 2 || d.n_mu > kMaxRing - 1 || d.n_layers < 2) return OCRT_ERR_ARGS;
+    if (d.max_orders < 1 || !(d.tolerance > 0.0)) return OCRT_ERR_ARGS;
+    const bool aer = d.L_max > 0 && d.tau_a > 0.0 && d.greek;
+    if (aer && (d.ssa_a < 0.0 || d.ssa_a > 1.0 || d.L_max < 2)) return OCRT_ERR_ARGS;
+    if (d.tau_a < 0.0 || d.tau_R < 0.0) return OCRT_ERR_ARGS;
+    if (!(d.tau_R + (aer ? d.tau_a : 0.0) > 0.0)) return OCRT_ERR_ARGS;
+    if (d.surface < 0 || d.surface > 3) return OCRT_ERR_ARGS;
+    if ((d.surface == OCRT_SURFACE_FLAT || d.surface == OCRT_SURFACE_BLACK_FRESNEL_OCEAN) && !(d.n_water > 0.0))
+        return OCRT_ERR_ARGS;
+    if (d.ipss && d.n_layers > 1536) return OCRT_ERR_ARGS;
+    return 0;
+}
+
+extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int precision, int one_per_launch,
+                                ocrt_plan **out)
+{
+    if (!in || n_inst < 1 || !out || (precision != OCRT_PREC_FP64 && precision != OCRT_PREC_MIXED))
+        return OCRT_ERR_ARGS;
+    *out = nullptr;
+    ocrt_plan *p = new ocrt_plan();
+    p->n_inst = n_inst;
+    p->prec = precision;
+    p->per_inst = one_per_launch ? 1 : 0;
+    const bool mixed = precision == OCRT_PREC_MIXED;
+    const int Palign = mixed ? 32 : 16;
+    p->inst.resize(n_inst);
+    long long ring_n = 0, greek_n = 0;
+    for (int i = 0; i < n_inst; ++i) {
+        const ocrt_instance_t &d = in[i];
+        HostInst &h = p->inst[i];
+        h.status = validate(d);
+        if (h.status != 0) { delete p; return h.status; }
+        DevInst &I = h.d;
+        memset(&I, 0, sizeof(I));
+        I.sza = d.sza_deg; I.vza = d.vza_deg; I.raa = d.raa_deg;
+        I.mus = cos(d.sza_deg * OCRT_PI / 180.0);
+        I.muv = cos(d.vza_deg * OCRT_PI / 180.0);
+        I.aer = (d.L_max > 0 && d.tau_a > 0.0 && d.greek) ? 1 : 0;
+        I.tauR = d.tau_R > 0.0 ? d.tau_R : 0.0;
+        I.ray = I.tauR > 0.0;
+        I.taua = I.aer ? d.tau_a : 0.0;
+        I.ssa = d.ssa_a; I.aer_h = d.aer_h_km; I.n_water = d.n_water; I.wind = d.wind_speed;
+        I.tol = d.tolerance;
+        const double dp = (d.depol >= 0.0 && d.depol < 1.0) ? d.depol : 0.0279;
+        const double an = 2.0 * (1.0 - dp) / (2.0 + dp);
+        I.beta0 = 1.0; I.beta2 = 0.5 * an; I.gamma2 = -an * sqrt(1.5); I.alpha2 = 3.0 * an;
+        int vidx = 0;
+        const int n = build_ring(d.n_mu, I.muv, h.mu, h.w, vidx);
+        if (n < 2) { delete p; return OCRT_ERR_ARGS; }
+        I.n = n; I.vidx = vidx; I.nt = d.n_layers; I.mmax = d.m_max;
+        I.La = I.aer ? d.L_max : 0;
+        I.lmax = std::max(std::max(d.m_max, 2), I.aer ? d.L_max : 0);
+        const bool surf_aware = d.surface == OCRT_SURFACE_FLAT || d.surface == OCRT_SURFACE_BLACK_FRESNEL_OCEAN;
+        const bool flat = d.surface == OCRT_SURFACE_FLAT ||
+                          (d.surface == OCRT_SURFACE_BLACK_FRESNEL_OCEAN && d.wind_speed <= 0.0);
+        I.surf = surf_aware ? (flat ? 1 : 2) : 0;
+        I.surface_kind = d.surface;
+        I.sigma_type = d.sigma_type; I.q_conv = d.q_convention; I.decouple = d.decouple_sunglint;
+        I.ipss = d.ipss ? 1 : 0; I.max_orders = d.max_orders;
+        I.off_ring = ring_n;
+        ring_n += 2 * n;
+        I.off_greek = greek_n;
+        greek_n += I.aer ? 4LL * (I.La + 1) : 0;
+        h.lev_n = 5LL * (I.nt + 1);
+        h.T_n = (long long)I.nt * n;
+        h.R_n = (I.surf == 2) ? (long long)(I.mmax + 1) * n * n * 9 + (long long)(I.mmax + 1) * n * 9 : 0;
+        p->mmax_all = std::max(p->mmax_all, I.mmax);
+    }
+    // ---- chunking: instances grouped so one chunk's working set stays bounded
+    const double budget = mixed ? 48e9 : 48e9;
+    const size_t fe = mixed ? 4 : 8, ae = mixed ? 2 : 8;
+    int item_global = 0;
+    p->item_of_inst.resize(n_inst);
+    int i = 0;
+    while (i < n_inst) {
+        Chunk c;
+        c.i0 = i;
+        c.it0 = item_global;
+        double bytes = 0.0;
+        while (i < n_inst) {
+            HostInst &h = p->inst[i];
+            const DevInst &I = h.d;
+            double b = (double)(h.lev_n + 2 * h.T_n + h.R_n) * 8.0;
+            const int n = I.n, L = I.nt + 1, P = roundup(6 * n, Palign), Mp = roundup(6 * n, 64);
+            for (int m = 0; m <= I.mmax; ++m) {
+                const int parts = (I.aer ? 1 : 0) + ((m <= 2 && I.ray) ? 1 : 0);
+                const int K = std::max(parts, 1) * P;
+                b += 3.0 * (I.lmax + 1) * (2 * n + 1) * 8 + 6.0 * (n + 1) * (2 * n + 1) * 8 +
+                     (double)Mp * K * ae * (mixed ? 2 : 1) + (double)L * P * (2 * fe + (mixed ? 16 : 8)) + 3.0 * n * 8;
+            }
+            const int items_after = (item_global - c.it0) + I.mmax + 1;
+            if (i > c.i0 && (bytes + b > budget || items_after > 60000 || i - c.i0 >= 60000 || p->per_inst)) break;
+            bytes += b;
+            p->item_of_inst[i] = item_global;
+            item_global += I.mmax + 1;
+            ++i;
+        }
+        c.i1 = i;
+        c.it1 = item_global;
+        p->chunks.push_back(c);
+    }
+    p->n_items = item_global;
+    // ---- per-chunk layout
+    for (Chunk &c : p->chunks) {
+        long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0;
+        for (int ii = c.i0; ii < c.i1; ++ii) {
+            HostInst &h = p->inst[ii];
+            DevInst I = h.d;
+            I.off_lev = lev; lev += h.lev_n;
+            I.off_T = T; T += h.T_n;
+            if (I.surf == 2) {
+                // share the surface kernels with an earlier instance of this chunk
+                // that has the same ring, sun and surface
+                int owner = -1;
+                for (int jj = std::max(c.i0, ii - 64); jj < ii && owner < 0; ++jj) {
+                    const HostInst &g = p->inst[jj];
+                    const DevInst &J = c.hinst[jj - c.i0];
+                    if (J.surf == 2 && J.surf_owner && J.n == I.n && J.mmax == I.mmax && J.mus == I.mus &&
+                        J.wind == I.wind && J.sigma_type == I.sigma_type && J.n_water == I.n_water &&
+                        J.q_conv == I.q_conv && g.mu == h.mu && g.w == h.w)
+                        owner = jj - c.i0;
+                }
+                if (owner >= 0) {
+                    I.off_R = c.hinst[owner].off_R;
+                    I.off_Rsun = c.hinst[owner].off_Rsun;
+                    I.surf_owner = 0;
+                } else {
+                    I.off_R = R;
+                    I.off_Rsun = R + (long long)(I.mmax + 1) * I.n * I.n * 9;
+                    R += h.R_n;
+                    I.surf_owner = 1;
+                }
+            }
+            I.item0 = p->item_of_inst[ii] - c.it0;
+            const int n = I.n, L = I.nt + 1, Pp = roundup(6 * n, Palign), Mp = roundup(6 * n, 64);
+            c.max_L = std::max(c.max_L, L);
+            c.max_P = std::max(c.max_P, Pp);
+            c.max_pairs = std::max(c.max_pairs, n * n + n);
+            c.max_tab = std::max(c.max_tab, (n + 1) * (2 * n + 1));
+            c.maxit = std::max(c.maxit, I.max_orders);
+            c.any_rough |= (I.surf == 2);
+            for (int m = 0; m <= I.mmax; ++m) {
+                DevItem it;
+                memset(&it, 0, sizeof(it));
+                it.inst = ii - c.i0; it.m = m; it.n = n; it.nt = I.nt; it.L = L; it.P = Pp;
+                int kinds[2], np = 0;
+                if (I.aer) kinds[np++] = 1;
+                if (m <= 2 && I.ray) kinds[np++] = 2;
+                if (np == 0) kinds[np++] = 1;
+                it.parts = kinds[0] | (np > 1 ? (kinds[1] << 8) : 0);
+                it.K = np * Pp;
+                it.Mp = Mp; it.tiles_m = Mp / 64; it.tiles_n = (L + 63) / 64;
+                it.lmax = I.lmax;
+                it.off_basis = basis; basis += 3LL * (I.lmax + 1) * (2 * n + 1);
+                it.off_tab = tab; tab += 6LL * (n + 1) * (2 * n + 1);
+                it.off_A = A; A += (long long)Mp * it.K * (mixed ? 2 : 1);
+                it.off_F = F; it.off_tot = F; F += (long long)L * Pp;
+                it.off_bc = bc; bc += 3LL * n;
+                c.max_pack = std::max(c.max_pack, (long long)Mp * it.K);
+                const int item_local = (int)c.hitem.size();
+                for (int tm = 0; tm < it.tiles_m; ++tm)
+                    for (int tn = 0; tn < it.tiles_n; ++tn) c.htiles.push_back(GemmTile{item_local, tm, tn, 0});
+                c.hitem.push_back(it);
+            }
+            if (I.ipss) c.hjobs.push_back(IpssJob{ii - c.i0, 0, I.vza, I.raa});
+            c.hinst.push_back(I);
+        }
+        c.n_jobs = (int)c.hjobs.size();
+        c.lev = lev; c.T = T; c.R = R; c.basis = basis; c.tab = tab; c.A = A; c.F = F; c.bc = bc;
+    }
+    // ---- device allocations
+    long long mx_lev = 0, mx_T = 0, mx_R = 0, mx_basis = 0, mx_tab = 0, mx_A = 0, mx_F = 0, mx_bc = 0;
+    for (const Chunk &c : p->chunks) {
+        mx_lev = std::max(mx_lev, c.lev); mx_T = std::max(mx_T, c.T); mx_R = std::max(mx_R, c.R);
+        mx_basis = std::max(mx_basis, c.basis); mx_tab = std::max(mx_tab, c.tab);
+        mx_A = std::max(mx_A, c.A); mx_F = std::max(mx_F, c.F); mx_bc = std::max(mx_bc, c.bc);
+    }
+    if (getenv("OCRT_DEBUG_PLAN"))
+        fprintf(stderr, "ocrt plan: %zu chunks items %d ring %lld greek %lld lev %lld T %lld R %lld basis %lld tab %lld "
+                        "A %lld F %lld bc %lld\n", p->chunks.size(), p->n_items, ring_n, greek_n, mx_lev, mx_T,
+                mx_R, mx_basis, mx_tab, mx_A, mx_F, mx_bc);
+    int rc = 0;
+    Pools &P = p->pools;
+    rc |= dmalloc(p, (void **)&p->ring, ring_n * 8);
+    rc |= dmalloc(p, (void **)&p->greek, std::max(greek_n, 1LL) * 8);
+    rc |= dmalloc(p, (void **)&p->cm_w, (size_t)(p->mmax_all + 1) * 2 * kNPhi * 8);
+    rc |= dmalloc(p, (void **)&p->state, (size_t)p->n_items * sizeof(ItemState));
+    rc |= dmalloc(p, (void **)&p->iout, (size_t)p->n_items * sizeof(ItemOut));
+    rc |= dmalloc(p, (void **)&p->ipss, (size_t)n_inst * sizeof(InstOut));
+    rc |= dmalloc(p, (void **)&p->res, (size_t)n_inst * sizeof(ocrt_result_t));
+    rc |= dmalloc(p, (void **)&p->kappa, (size_t)n_inst * 8);
+    rc |= dmalloc(p, (void **)&p->taudir, (size_t)n_inst * 8);
+    rc |= dmalloc(p, (void **)&P.lev, mx_lev * 8);
+    rc |= dmalloc(p, (void **)&P.T, mx_T * 8);
+    rc |= dmalloc(p, (void **)&P.Tf, mx_T * 4);
+    rc |= dmalloc(p, (void **)&P.R, mx_R * 8);
+    rc |= dmalloc(p, (void **)&P.basis, mx_basis * 8);
+    rc |= dmalloc(p, (void **)&P.tab, mx_tab * 8);
+    rc |= dmalloc(p, (void **)&P.A, mx_A * ae);
+    rc |= dmalloc(p, (void **)&P.F, mx_F * fe);
+    rc |= dmalloc(p, (void **)&P.J, mx_F * fe);
+    rc |= dmalloc(p, (void **)&P.tot, mx_F * 8);
+    if (mixed) rc |= dmalloc(p, (void **)&P.comp, mx_F * 8);
+    rc |= dmalloc(p, (void **)&P.bc, mx_bc * 8);
+    if (rc) { ocrt_plan_destroy(p); return OCRT_ERR_ALLOC; }
+    P.ring = p->ring;
+    P.greek = p->greek;
+    P.cm_w = p->cm_w;
+    P.toa_up = nullptr;
+    // ---- uploads of inputs and descriptors
+    std::vector<double> hring(ring_n), hgreek(std::max(greek_n, 1LL));
+    for (int ii = 0; ii < n_inst; ++ii) {
+        const HostInst &h = p->inst[ii];
+        const int n = h.d.n;
+        for (int o = 0; o < n; ++o) { hring[h.d.off_ring + o] = h.mu[o]; hring[h.d.off_ring + n + o] = h.w[o]; }
+        if (h.d.aer) {
+            const int q = h.d.La + 1;
+            for (int e = 0; e < 4 * q; ++e) hgreek[h.d.off_greek + e] = in[ii].greek[e];
+        }
+    }
+    if (hipMemcpy(p->ring, hring.data(), ring_n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        (greek_n > 0 && hipMemcpy(p->greek, hgreek.data(), greek_n * 8, hipMemcpyHostToDevice) != hipSuccess)) {
+        ocrt_plan_destroy(p);
+        return OCRT_ERR_DEVICE;
+    }
+    // chunk descriptors: one device buffer, sub-allocated 256-byte aligned
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t desc_bytes = 0;
+    for (const Chunk &c : p->chunks)
+        desc_bytes += al(c.hinst.size() * sizeof(DevInst)) + al(c.hitem.size() * sizeof(DevItem)) +
+                      al(std::max<size_t>(c.htiles.size(), 1) * sizeof(GemmTile)) +
+                      al(std::max<size_t>(c.hjobs.size(), 1) * sizeof(IpssJob)) +
+                      al((size_t)(c.i1 - c.i0) * sizeof(int));
+    std::vector<unsigned char> hdesc(desc_bytes, 0);
+    unsigned char *ddesc = nullptr;
+    if (dmalloc(p, (void **)&ddesc, desc_bytes)) { ocrt_plan_destroy(p); return OCRT_ERR_ALLOC; }
+    size_t off = 0;
+    auto put = [&](const void *src, size_t bytes, size_t room) -> void * {
+        if (bytes) memcpy(hdesc.data() + off, src, bytes);
+        void *d = ddesc + off;
+        off += al(room);
+        return d;
+    };
+    for (Chunk &c : p->chunks) {
+        std::vector<int> job_of(c.i1 - c.i0, -1);
+        for (size_t j = 0; j < c.hjobs.size(); ++j) job_of[c.hjobs[j].inst] = (int)j;
+        c.dinst = (DevInst *)put(c.hinst.data(), c.hinst.size() * sizeof(DevInst), c.hinst.size() * sizeof(DevInst));
+        c.ditem = (DevItem *)put(c.hitem.data(), c.hitem.size() * sizeof(DevItem), c.hitem.size() * sizeof(DevItem));
+        c.dtiles = (GemmTile *)put(c.htiles.data(), c.htiles.size() * sizeof(GemmTile),
+                                   std::max<size_t>(c.htiles.size(), 1) * sizeof(GemmTile));
+        c.djobs = (IpssJob *)put(c.hjobs.data(), c.hjobs.size() * sizeof(IpssJob),
+                                 std::max<size_t>(c.hjobs.size(), 1) * sizeof(IpssJob));
+        c.djob_of = (int *)put(job_of.data(), job_of.size() * sizeof(int), job_of.size() * sizeof(int));
+    }
+    if (hipMemcpy(ddesc, hdesc.data(), desc_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        ocrt_plan_destroy(p);
+        return OCRT_ERR_DEVICE;
+    }
+    hipLaunchKernelGGL(fourier_weights, dim3((kNPhi + 255) / 256, p->mmax_all + 1), dim3(256), 0, 0, p->cm_w,
+                       p->mmax_all);
+    if (hipDeviceSynchronize() != hipSuccess) { ocrt_plan_destroy(p); return OCRT_ERR_DEVICE; }
+    *out = p;
+    return OCRT_OK;
+}
+
+static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
+{
+    const bool mixed = p->prec == OCRT_PREC_MIXED;
+    Pools P = p->pools;
+    const int ni = c.i1 - c.i0, nit = c.it1 - c.it0;
+    ItemState *st = p->state + c.it0;
+    ItemOut *io = p->iout + c.it0;
+    hipLaunchKernelGGL(medium_levels, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
+    hipLaunchKernelGGL(medium_fractions, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
+    if (c.any_rough)
+        hipLaunchKernelGGL(coxmunk_fourier, dim3(c.max_pairs, ni), dim3(256), 0, s, c.dinst, P);
+    hipLaunchKernelGGL(angular_basis, dim3(nit), dim3(128), 0, s, c.dinst, c.ditem, P);
+    hipLaunchKernelGGL(phase_tables, dim3((c.max_tab + 255) / 256, nit), dim3(256), 0, s, c.dinst, c.ditem, P);
+    const unsigned pack_blocks = (unsigned)((c.max_pack + 255) / 256);
+    if (mixed)
+        hipLaunchKernelGGL(pack_operator<__hip_bfloat16>, dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst,
+                           c.ditem, P);
+    else
+        hipLaunchKernelGGL(pack_operator<double>, dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst, c.ditem, P);
+    const dim3 cols((c.max_P + 255) / 256, nit);
+    if (mixed)
+        hipLaunchKernelGGL((order_one<float, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
+    else
+        hipLaunchKernelGGL((order_one<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
+    const unsigned ntiles = (unsigned)c.htiles.size();
+    for (int it = 2; it <= c.maxit; ++it) {
+        hipEvent_t *eg = p->profile ? ev_pair(p, 0) : nullptr;
+        hipEvent_t *es = p->profile ? ev_pair(p, 1) : nullptr;
+        if (mixed) {
+            hipLaunchKernelGGL(order_status<float>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
+            if (eg) (void)hipEventRecord(eg[0], s);
+            hipLaunchKernelGGL(gemm_bf16, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st);
+            if (eg) (void)hipEventRecord(eg[1], s);
+            if (es) (void)hipEventRecord(es[0], s);
+            hipLaunchKernelGGL((sweep<float, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
+            if (es) (void)hipEventRecord(es[1], s);
+        } else {
+            hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
+            if (eg) (void)hipEventRecord(eg[0], s);
+            hipLaunchKernelGGL(gemm_f64, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st);
+            if (eg) (void)hipEventRecord(eg[1], s);
+            if (es) (void)hipEventRecord(es[0], s);
+            hipLaunchKernelGGL((sweep<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
+            if (es) (void)hipEventRecord(es[1], s);
+        }
+        p->gemm_launches += 1.0;
+    }
+    if (mixed)
+        hipLaunchKernelGGL(order_status<float>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, c.maxit + 1);
+    else
+        hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, c.maxit + 1);
+    hipLaunchKernelGGL(extract, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, io, mixed ? 1 : 0);
+    if (c.n_jobs > 0)
+        hipLaunchKernelGGL(ipss_kappa, dim3(c.n_jobs), dim3(256), 0, s, c.dinst, c.djobs, P,
+                           p->kappa + c.i0, p->taudir + c.i0);
+    return hipGetLastError() == hipSuccess ? 0 : OCRT_ERR_DEVICE;
+}
+
+// kappa/taudir per chunk job -> InstOut per instance, then the instance results.
+__global__ static void gather_ipss(const DevInst *inst, int ni, const double *kappa, const double *taudir,
+                                   const int *job_of, InstOut *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    const int j = job_of[i];
+    out[i].kappa = j >= 0 ? kappa[j] : 1.0;
+    out[i].tau_dir_ipss = j >= 0 ? taudir[j] : -1.0;
+}
+
+extern "C" int ocrt_plan_execute(ocrt_plan *p, void *stream)
+{
+    if (!p) return OCRT_ERR_ARGS;
+    hipStream_t s = (hipStream_t)stream;
+    p->gemm_launches = 0.0;
+    p->ev_used = 0;
+    for (Chunk &c : p->chunks) {
+        if (run_chunk(p, c, s) != 0) return OCRT_ERR_DEVICE;
+        const int ni = c.i1 - c.i0;
+        hipLaunchKernelGGL(gather_ipss, dim3((ni + 127) / 128), dim3(128), 0, s, c.dinst, ni, p->kappa + c.i0,
+                           p->taudir + c.i0, c.djob_of, p->ipss + c.i0);
+        hipLaunchKernelGGL(finalize, dim3((ni + 127) / 128), dim3(128), 0, s, c.dinst, ni, p->iout + c.it0,
+                           p->ipss + c.i0, p->pools, p->res + c.i0);
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
+}
+
+extern "C" int ocrt_plan_results(ocrt_plan *p, ocrt_result_t *out, double *toa_modes, int *orders)
+{
+    if (!p || !out) return OCRT_ERR_ARGS;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(out, p->res, (size_t)p->n_inst * sizeof(ocrt_result_t), hipMemcpyDeviceToHost));
+    std::vector<ItemOut> io(p->n_items);
+    HIPCHK(hipMemcpy(io.data(), p->iout, (size_t)p->n_items * sizeof(ItemOut), hipMemcpyDeviceToHost));
+    const int M = p->mmax_all + 1;
+    double flops = 0.0, alg = 0.0, bytes = 0.0;
+    const bool mixed = p->prec == OCRT_PREC_MIXED;
+    for (int i = 0; i < p->n_inst; ++i) {
+        const DevInst &I = p->inst[i].d;
+        int n_in = 0;
+        for (double w : p->inst[i].w) n_in += (w != 0.0);
+        for (int m = 0; m <= I.mmax; ++m) {
+            const ItemOut &r = io[p->item_of_inst[i] + m];
+            if (toa_modes)
+                for (int c = 0; c < 3; ++c) toa_modes[((long long)i * M + m) * 3 + c] = r.mode[c];
+            if (orders) orders[(long long)i * M + m] = r.orders;
+            const int ord = std::max(r.orders - 1, 0);
+            const double L = I.nt + 1.0;
+            const int parts = (I.aer ? 1 : 0) + ((m <= 2 && I.ray) ? 1 : 0);
+            flops += ord * 2.0 * (6.0 * I.n) * (6.0 * I.n * std::max(parts, 1)) * L;
+            alg += ord * 72.0 * I.n * n_in * L * std::max(parts, 1);
+            bytes += ord * (2.0 * 3.0 * 2.0 * I.n * L + 2.0 * I.n * I.nt) * (mixed ? 4.0 : 8.0);
+        }
+        if (toa_modes)
+            for (int m = I.mmax + 1; m < M; ++m)
+                for (int c = 0; c < 3; ++c) toa_modes[((long long)i * M + m) * 3 + c] = 0.0;
+        if (orders)
+            for (int m = I.mmax + 1; m < M; ++m) orders[(long long)i * M + m] = 0;
+    }
+    p->gemm_flops = flops;
+    p->gemm_flops_alg = alg;
+    p->sweep_bytes = bytes;
+    return OCRT_OK;
+}
+
+extern "C" int ocrt_plan_mmax(const ocrt_plan *p) { return p ? p->mmax_all : -1; }
+extern "C" size_t ocrt_plan_device_bytes(const ocrt_plan *p) { return p ? p->dev_bytes : 0; }
+
+extern "C" int ocrt_plan_work(const ocrt_plan *p, double *gemm_flops, double *sweep_bytes, double *launches)
+{
+    if (!p) return OCRT_ERR_ARGS;
+    if (gemm_flops) *gemm_flops = p->gemm_flops_alg;
+    if (sweep_bytes) *sweep_bytes = p->sweep_bytes;
+    if (launches) *launches = p->gemm_launches;
+    return OCRT_OK;
+}
+
+extern "C" int ocrt_plan_profile(ocrt_plan *p, int enable)
+{
+    if (!p) return OCRT_ERR_ARGS;
+    p->profile = enable ? 1 : 0;
+    return OCRT_OK;
+}
+
+extern "C" int ocrt_plan_kernel_times(ocrt_plan *p, double *gemm_ms, int *gemm_launches, double *sweep_ms,
+                                      int *sweep_launches)
+{
+    if (!p) return OCRT_ERR_ARGS;
+    HIPCHK(hipDeviceSynchronize());
+    double t[2] = {0.0, 0.0};
+    int cnt[2] = {0, 0};
+    for (size_t q = 0; q + 1 < p->ev_used; q += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev[q], p->ev[q + 1]));
+        t[p->ev_kind[q / 2]] += ms;
+        cnt[p->ev_kind[q / 2]] += 1;
+    }
+    if (gemm_ms) *gemm_ms = t[0];
+    if (gemm_launches) *gemm_launches = cnt[0];
+    if (sweep_ms) *sweep_ms = t[1];
+    if (sweep_launches) *sweep_launches = cnt[1];
+    return OCRT_OK;
+}
+
+extern "C" void ocrt_plan_destroy(ocrt_plan *p)
+{
+    if (!p) return;
+    for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+    for (void *a : p->allocs) (void)hipFree(a);
+    delete p;
+}
+
+extern "C" int ocrt_solve_batch(const ocrt_instance_t *in, int n, int precision, ocrt_result_t *out, void *stream)
+{
+    ocrt_plan *p = nullptr;
+    int rc = ocrt_plan_create(in, n, precision, 0, &p);
+    if (rc) return rc;
+    rc = ocrt_plan_execute(p, stream);
+    if (!rc) rc = ocrt_plan_results(p, out, nullptr, nullptr);
+    ocrt_plan_destroy(p);
+    return rc;
+}
+
+extern "C" const char *ocrt_version(void) { return "ocrt_amd 0.1 gfx950"; }

@@ -1,0 +1,408 @@
+// Once-per-instance and once-per-mode kernels of the SOS solve, all fp64:
+//   medium_levels / medium_fractions : uniform-tau level grid, US62 + aerosol
+//       altitude mapping and the Beer-Lambert factors (rt_atm.c:177-400,
+//       rt_molecular_profile.c, rt_layer_transport.c:36-87)
+//   coxmunk_fourier : azimuthal Fourier kernels of the rough-ocean Mueller
+//       BRDF for every mode at once (shared/surface.c:514-646, 839-938, 1263-1380)
+//   angular_basis   : generalized-spherical-function recurrences
+//       (rt_angular_basis.c:126-276)
+//   phase_tables    : phase-matrix Fourier tables (rt_angular_basis.c:278-358)
+//   pack_operator   : dense scattering operator with quadrature weights and
+//       the Stokes sign pattern folded in (rt_sos_operator.c:50-126, 353-407)
+#include "common.h"
+#include "device_math.h"
+
+namespace ocrt {
+
+// ---------------------------------------------------------------------------
+// Medium.  One thread per (instance, level).  The aerosol altitude of each
+// level is found by bisection on [0, 100 km] independently per level (the
+// reference walks levels sequentially with the previous altitude as the
+// upper bracket; both converge to the same root to the last bits).
+__global__ void medium_levels(const DevInst *__restrict__ inst, Pools P)
+{
+    const DevInst &I = inst[blockIdx.y];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = I.nt;
+    if (k > nt) return;
+    const int L = nt + 1;
+    double *h = P.lev + I.off_lev, *ch = h + L, *xd = ch + L, *yd = xd + L, *z = yd + L;
+    const double tau = I.tauR + (I.aer ? I.taua : 0.0);
+    const double hk = (double)k * tau / (double)nt;
+    h[k] = hk;
+    ch[k] = 0.5 * exp(-hk / I.mus);
+    double zk;
+    if (!I.aer) {
+        zk = (k == 0) ? 100.0 : us62_altitude_from_fraction(hk / tau);
+        xd[k] = 0.0;
+        yd[k] = 1.0;
+    } else if (k == 0) {
+        zk = 100.0;
+    } else if (k == nt) {
+        zk = 0.0;
+    } else {
+        const double H = I.aer_h > 0.0 ? I.aer_h : 2.0;
+        double lo = 0.0, hi = 100.0;
+        for (int it = 0; it < 80; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            const double above = I.tauR * us62_grid_fraction_above(mid) +
+                                 I.taua * aerosol_fraction_above(mid, 100.0, H);
+            if (above > hk) lo = mid; else hi = mid;
+        }
+        zk = 0.5 * (lo + hi);
+    }
+    z[k] = zk;
+}
+
+// Scattering fractions of each level (aerosol case) and Beer-Lambert factors
+// T[layer][node] = exp(-dtau / mu_node) in fp64, 1 - T in fp32.
+__global__ void medium_fractions(const DevInst *__restrict__ inst, Pools P)
+{
+    const DevInst &I = inst[blockIdx.y];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = I.nt;
+    if (k > nt) return;
+    const int L = nt + 1;
+    const double *h = P.lev + I.off_lev, *z = h + 4 * L;
+    double *xd = P.lev + I.off_lev + 2 * L, *yd = xd + L;
+    if (I.aer) {
+        if (k == 0) {
+            if (I.tauR == 0.0 && I.taua > 0.0) { xd[0] = I.ssa; yd[0] = 0.0; }
+            else { xd[0] = 0.0; yd[0] = 1.0; }
+        } else {
+            const double H = I.aer_h > 0.0 ? I.aer_h : 2.0;
+            const double ray = I.tauR * us62_grid_fraction_above(z[k]);
+            const double ae = I.taua * aerosol_fraction_above(z[k], 100.0, H);
+            const double rp = (k == 1) ? 0.0 : I.tauR * us62_grid_fraction_above(z[k - 1]);
+            const double ap = (k == 1) ? 0.0 : I.taua * aerosol_fraction_above(z[k - 1], 100.0, H);
+            const double dr = ray - rp, da = ae - ap, dt = dr + da;
+            if (dt > 0.0) { xd[k] = da * I.ssa / dt; yd[k] = dr / dt; }
+            else { xd[k] = 0.0; yd[k] = (I.tauR > 0.0) ? 1.0 : 0.0; }
+        }
+    }
+    if (k < nt) {
+        const double *mu = P.ring + I.off_ring;
+        double *T = P.T + I.off_T + (long long)k * I.n;
+        float *Tf = P.Tf + I.off_T + (long long)k * I.n;
+        const double dtau = h[k + 1] - h[k];
+        for (int o = 0; o < I.n; ++o) {
+            T[o] = exp(-dtau / fabs(mu[o]));
+            // fp32 sweeps carry 1 - T: with dtau / mu ~ 1e-4 per layer, T itself
+            // in fp32 would leave 1 - T with only ~1e-3 relative accuracy
+            Tf[o] = (float)(-expm1(-dtau / fabs(mu[o])));
+        }
+    }
+}
+
+// cos(m phi_p), sin(m phi_p) on the midpoint azimuth rule, [m][2][kNPhi].
+__global__ void fourier_weights(double *w, int mmax)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (p >= kNPhi || m > mmax) return;
+    const double phi = (p + 0.5) * (2.0 * OCRT_PI / (double)kNPhi);
+    w[((long long)m * 2 + 0) * kNPhi + p] = cos(m * phi);
+    w[((long long)m * 2 + 1) * kNPhi + p] = sin(m * phi);
+}
+
+// ---------------------------------------------------------------------------
+// Rough-ocean Fourier kernels.  One workgroup per (instance, outgoing node,
+// incoming node or the sun).  The isotropic-slope BRDF is even in the
+// relative azimuth for the II IQ QI QQ UU entries and odd for the others, so
+// the midpoint rule over [0, 2 pi) folds onto its first half (phi_p and
+// 2 pi - phi_p pair up) and only kNPhi / 2 azimuths are evaluated.  The
+// shadowing factor is hoisted out of the azimuth loop.  Instances of a batch
+// that share the ring, the sun and the surface (the bands of one geometry)
+// share one kernel: only owners (surf_owner) compute it.
+__global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict__ inst, Pools P)
+{
+    constexpr int kHalf = kNPhi / 2;
+    __shared__ double R[9][kHalf];
+    const DevInst &I = inst[blockIdx.y];
+    if (I.surf != 2 || !I.surf_owner) return;
+    const int n = I.n;
+    const int pair = blockIdx.x;
+    if (pair >= n * n + n) return;
+    const double *mu = P.ring + I.off_ring;
+    const double *w = mu + n;
+    int o, i;
+    double mo, mi;
+    const bool solar = pair >= n * n;
+    if (!solar) {
+        o = pair / n; i = pair % n;
+        if (w[i] == 0.0) {   // zero-weight incoming columns never enter the boundary sum
+            const int nout = (I.mmax + 1) * 9;
+            for (int t = threadIdx.x; t < nout; t += blockDim.x) {
+                const int m = t / 9, kl = t % 9;
+                P.R[I.off_R + (((long long)m * n + o) * n + i) * 9 + kl] = 0.0;
+            }
+            return;
+        }
+        mo = mu[o]; mi = mu[i];
+    } else {
+        o = pair - n * n; i = 0;
+        mo = mu[o]; mi = I.mus;
+    }
+    const double dphi = 2.0 * OCRT_PI / (double)kNPhi;
+    const double s2v = slope_variance(I.wind, I.sigma_type);
+    const double shadow = sancer_shadow(mo, mi, s2v);
+    for (int p = threadIdx.x; p < kHalf; p += blockDim.x) {
+        const double phi = (p + 0.5) * dphi;
+        double r[9];
+        coxmunk_R_core(mo, mi, cos(phi), sin(phi), s2v, shadow, I.n_water, I.q_conv, r);
+        for (int e = 0; e < 9; ++e) R[e][p] = r[e];
+    }
+    __syncthreads();
+    const int nout = (I.mmax + 1) * 9;
+    for (int t = threadIdx.x; t < nout; t += blockDim.x) {
+        const int m = t / 9, kl = t % 9;
+        const bool use_cos = (kl == 0 || kl == 1 || kl == 3 || kl == 4 || kl == 8);
+        const double *W = P.cm_w + ((long long)m * 2 + (use_cos ? 0 : 1)) * kNPhi;
+        double acc = 0.0;
+        for (int p = 0; p < kHalf; ++p) acc += R[kl][p] * W[p];
+        const double nrm = (m == 0) ? 1.0 / (2.0 * OCRT_PI) : 1.0 / OCRT_PI;
+        double v = 2.0 * acc * (dphi * nrm);
+        if (m > 0 && (kl == 2 || kl == 5)) v = -v;
+        if (!solar) P.R[I.off_R + (((long long)m * n + o) * n + i) * 9 + kl] = v;
+        else P.R[I.off_Rsun + ((long long)m * n + o) * 9 + kl] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Signed direction cosine of direction index d = j + n, j in [-n, n]; j = 0 is
+// the solar slot (-mu_sun).
+__device__ inline double dir_mu(const double *ring_mu, int n, double mus, int d)
+{
+    const int j = d - n;
+    if (j == 0) return -mus;
+    return j > 0 ? ring_mu[j - 1] : -ring_mu[-j - 1];
+}
+
+// Basis of one item: one thread per direction runs the three recurrences in l.
+__global__ void angular_basis(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P)
+{
+    const DevItem &it = items[blockIdx.x];
+    const DevInst &I = inst[it.inst];
+    const int n = it.n, dirs = 2 * n + 1, m = it.m, lmax = it.lmax;
+    double *plm = P.basis + it.off_basis;
+    double *rrl = plm + (long long)(lmax + 1) * dirs;
+    double *rtl = rrl + (long long)(lmax + 1) * dirs;
+    const double *mu = P.ring + I.off_ring;
+    for (int d = threadIdx.x; d < dirs; d += blockDim.x) {
+        const double x = dir_mu(mu, n, I.mus, d);
+        for (int l = 0; l < m && l <= lmax; ++l) {
+            plm[(long long)l * dirs + d] = 0.0;
+            rrl[(long long)l * dirs + d] = 0.0;
+            rtl[(long long)l * dirs + d] = 0.0;
+        }
+        double p0, p1, r0 = 0.0, r1, t0 = 0.0, t1;   // values at l-1 and l
+        int l0;
+        if (m == 0) {
+            plm[0 * dirs + d] = 1.0; rrl[0 * dirs + d] = 0.0; rtl[0 * dirs + d] = 0.0;
+            plm[1 * dirs + d] = x; rrl[1 * dirs + d] = 0.0; rtl[1 * dirs + d] = 0.0;
+            p0 = x; p1 = 0.5 * (3.0 * x * x - 1.0);
+            r1 = 3.0 * (1.0 - x * x) / (2.0 * sqrt(6.0)); t1 = 0.0;
+            l0 = 2;
+        } else if (m == 1) {
+            const double s = sqrt(0.5 * (1.0 - x * x));
+            p0 = s; p1 = x * s * sqrt(3.0);
+            const double rt = sqrt(1.0 - x * x);
+            r1 = -x * rt * 0.5; t1 = -rt * 0.5;
+            plm[1 * dirs + d] = s; rrl[1 * dirs + d] = 0.0; rtl[1 * dirs + d] = 0.0;
+            l0 = 2;
+        } else {
+            const double nm = diag_norm(m);
+            const double sn = nm * sqrt((double)m / (double)(m + 1)) * sqrt((double)(m - 1) / (double)(m + 2));
+            const double omx = 1.0 - x * x;
+            p0 = 0.0; p1 = nm * pow(omx, 0.5 * (double)m);
+            const double env = pow(omx, 0.5 * (double)m - 1.0);
+            r1 = sn * (1.0 + x * x) * env; t1 = sn * 2.0 * x * env;
+            l0 = m;
+        }
+        plm[(long long)l0 * dirs + d] = p1;
+        rrl[(long long)l0 * dirs + d] = r1;
+        rtl[(long long)l0 * dirs + d] = t1;
+        for (int l = l0; l < lmax; ++l) {
+            const double fw = (2.0 * l + 1.0) / sqrt((double)(l + m + 1) * (double)(l - m + 1));
+            const double bw = sqrt((double)(l + m) * (double)(l - m)) / (2.0 * l + 1.0);
+            const double sc = (double)(l + 1) * (double)(2 * l + 1) /
+                              sqrt((double)(l + 3) * (double)(l - 1) * (double)(l + m + 1) * (double)(l - m + 1));
+            const double dc = sqrt((double)(l + 2) * (double)(l - 2) * (double)(l + m) * (double)(l - m)) /
+                              ((double)l * (double)(2 * l + 1));
+            const double spc = 2.0 * (double)m / ((double)l * (double)(l + 1));
+            const double p2 = fw * (x * p1 - bw * p0);
+            const double r2 = sc * (x * r1 - spc * t1 - dc * r0);
+            const double t2 = sc * (x * t1 - spc * r1 - dc * t0);
+            plm[(long long)(l + 1) * dirs + d] = p2;
+            rrl[(long long)(l + 1) * dirs + d] = r2;
+            rtl[(long long)(l + 1) * dirs + d] = t2;
+            p0 = p1; p1 = p2; r0 = r1; r1 = r2; t0 = t1; t1 = t2;
+        }
+    }
+}
+
+// Six phase tables [(n+1)][2n+1] of one item; one thread per (j, k).
+__global__ void phase_tables(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P)
+{
+    const DevItem &it = items[blockIdx.y];
+    const DevInst &I = inst[it.inst];
+    const int n = it.n, dirs = 2 * n + 1, m = it.m, lmax = it.lmax;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (n + 1) * dirs) return;
+    const int j = idx / dirs, k = idx % dirs;
+    const long long cnt = (long long)(n + 1) * dirs;
+    double *tab = P.tab + it.off_tab;
+    if (!I.aer) {
+        for (int t = 0; t < 6; ++t) tab[t * cnt + idx] = 0.0;
+        return;
+    }
+    const double *plm = P.basis + it.off_basis;
+    const double *rrl = plm + (long long)(lmax + 1) * dirs;
+    const double *rtl = rrl + (long long)(lmax + 1) * dirs;
+    const double *g = P.greek + I.off_greek;
+    const int q = I.La + 1;
+    const int jd = j + n;
+    double p = 0, gr = 0, gt = 0, rr = 0, rt = 0, tt = 0;
+    const int lend = lmax < I.La ? lmax : I.La;
+    for (int l = m; l <= lend; ++l) {
+        const long long b = (long long)l * dirs;
+        const double pj = plm[b + jd], pk = plm[b + k];
+        const double rj = rrl[b + jd], rk = rrl[b + k];
+        const double tj = rtl[b + jd], tk = rtl[b + k];
+        const double be = g[l], ga = g[q + l], al = g[2 * q + l], ze = g[3 * q + l];
+        p += pj * pk * be;
+        gr += pj * rk * ga;
+        gt += pj * tk * ga;
+        tt += tj * tk * al + rj * rk * ze;
+        rr += tj * tk * ze + rj * rk * al;
+        rt += tj * rk * al + rj * tk * ze;
+    }
+    tab[0 * cnt + idx] = p;
+    tab[1 * cnt + idx] = gr;
+    tab[2 * cnt + idx] = gt;
+    tab[3 * cnt + idx] = rr;
+    tab[4 * cnt + idx] = rt;
+    tab[5 * cnt + idx] = tt;
+}
+
+// Coefficient selector of the operator: for output type (row block) and input
+// type (column block) in the order [I_up I_dn Q_up Q_dn U_up U_dn], which of
+// the 18 pair coefficients enters and with which sign
+// (rt_sos_operator.c:383-406).
+__constant__ signed char c_sel_q[6][6] = {
+    {0, 1, 8, 9, 4, 5}, {1, 0, 9, 8, 5, 4}, {6, 7, 10, 11, 12, 13},
+    {7, 6, 11, 10, 13, 12}, {2, 3, 14, 15, 16, 17}, {3, 2, 15, 14, 17, 16}};
+__constant__ signed char c_sel_s[6][6] = {
+    {1, 1, 1, 1, -1, -1}, {1, 1, 1, 1, 1, 1}, {1, 1, 1, 1, -1, 1},
+    {1, 1, 1, 1, -1, 1}, {-1, 1, -1, -1, 1, 1}, {-1, 1, 1, 1, 1, 1}};
+
+// Particle pair coefficient q for outgoing node o and incoming node i (1-based).
+__device__ inline double pair_particle(const double *tab, long long cnt, int dirs, int n, int q, int o, int i)
+{
+    // table t, row, signed column
+    int t, row, col;
+    switch (q) {
+    case 0: t = 0; row = i; col = o; break;
+    case 1: t = 0; row = i; col = -o; break;
+    case 2: t = 2; row = i; col = o; break;
+    case 3: t = 2; row = i; col = -o; break;
+    case 4: t = 2; row = o; col = i; break;
+    case 5: t = 2; row = o; col = -i; break;
+    case 6: t = 1; row = i; col = o; break;
+    case 7: t = 1; row = i; col = -o; break;
+    case 8: t = 1; row = o; col = i; break;
+    case 9: t = 1; row = o; col = -i; break;
+    case 10: t = 3; row = i; col = o; break;
+    case 11: t = 3; row = i; col = -o; break;
+    case 12: t = 4; row = i; col = o; break;
+    case 13: t = 4; row = i; col = -o; break;
+    case 14: t = 4; row = o; col = i; break;
+    case 15: t = 4; row = o; col = -i; break;
+    case 16: t = 5; row = i; col = o; break;
+    default: t = 5; row = i; col = -o; break;
+    }
+    return tab[t * cnt + (long long)row * dirs + col + n];
+}
+
+// Molecular pair coefficient q (rt_sos_operator.c:102-119).
+__device__ inline double pair_molecular(const DevInst &I, int m, const double *plm2, const double *rrl2,
+                                        const double *rtl2, int n, int q, int o, int i)
+{
+    const double b0 = (m == 0) ? I.beta0 : 0.0;
+    const double po = plm2[n + o], pom = plm2[n - o], pi = plm2[n + i];
+    const double ro = rrl2[n + o], rom = rrl2[n - o], ri = rrl2[n + i], rim = rrl2[n - i];
+    const double to = rtl2[n + o], tom = rtl2[n - o], ti = rtl2[n + i], tim = rtl2[n - i];
+    switch (q) {
+    case 0: return b0 + I.beta2 * pi * po;
+    case 1: return b0 + I.beta2 * pi * pom;
+    case 2: return I.gamma2 * pi * to;
+    case 3: return I.gamma2 * pi * tom;
+    case 4: return I.gamma2 * po * ti;
+    case 5: return I.gamma2 * po * tim;
+    case 6: return I.gamma2 * pi * ro;
+    case 7: return I.gamma2 * pi * rom;
+    case 8: return I.gamma2 * po * ri;
+    case 9: return I.gamma2 * po * rim;
+    case 10: return I.alpha2 * ri * ro;
+    case 11: return I.alpha2 * ri * rom;
+    case 12: return I.alpha2 * ti * ro;
+    case 13: return I.alpha2 * ti * rom;
+    case 14: return I.alpha2 * to * ri;
+    case 15: return I.alpha2 * to * rim;
+    case 16: return I.alpha2 * ti * to;
+    default: return I.alpha2 * ti * tom;
+    }
+}
+
+// Dense operator A[Mp][K] of one item; element (r, c) with row r = outgoing
+// state column and c = (part, incoming state column).  Part 0/1 of the
+// column range is the particle (scaled by xdel at apply time) or molecular
+// (scaled by ydel) block.
+template <typename AT>
+__global__ void pack_operator(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P)
+{
+    const DevItem &it = items[blockIdx.y];
+    const DevInst &I = inst[it.inst];
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)it.Mp * it.K;
+    if (idx >= total) return;
+    const int r = (int)(idx / it.K), c = (int)(idx % it.K);
+    const int n = it.n, dirs = 2 * n + 1, six_n = 6 * n;
+    AT *A = reinterpret_cast<AT *>(P.A) + it.off_A;
+    const int part = c / it.P, cc = c % it.P;
+    double v = 0.0;
+    if (r < six_n && cc < six_n) {
+        const int to = r / n, o = r % n + 1;
+        const int ti = cc / n, i = cc % n + 1;
+        const double w = P.ring[I.off_ring + n + (i - 1)];
+        if (w != 0.0) {
+            const int q = c_sel_q[to][ti];
+            const int s = c_sel_s[to][ti];
+            const int kind = (it.parts >> (8 * part)) & 0xff;   // 1 particle, 2 molecular
+            double e;
+            if (kind == 1) {
+                const long long cnt = (long long)(n + 1) * dirs;
+                e = pair_particle(P.tab + it.off_tab, cnt, dirs, n, q, o, i);
+            } else {
+                const double *plm = P.basis + it.off_basis;
+                const double *plm2 = plm + 2LL * dirs;
+                const double *rrl2 = plm + (long long)(it.lmax + 1) * dirs + 2LL * dirs;
+                const double *rtl2 = plm + 2LL * (it.lmax + 1) * dirs + 2LL * dirs;
+                e = pair_molecular(I, it.m, plm2, rrl2, rtl2, n, q, o, i);
+            }
+            v = w * (double)s * e;
+        }
+    }
+    A[idx] = to_storage<AT>(v);
+    if (sizeof(AT) == 2) {
+        // bf16 operator carried as hi + lo: the rounding of a fixed operator is
+        // systematic across orders, the residual term removes it to ~2^-16
+        const double hi = (double)__bfloat162float(*reinterpret_cast<const __hip_bfloat16 *>(&A[idx]));
+        A[total + idx] = to_storage<AT>(v - hi);
+    }
+}
+
+template __global__ void pack_operator<double>(const DevInst *, const DevItem *, Pools);
+template __global__ void pack_operator<__hip_bfloat16>(const DevInst *, const DevItem *, Pools);
+
+}  // namespace ocrt

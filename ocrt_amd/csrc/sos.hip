@@ -1,0 +1,691 @@
+// Order loop of the SOS solve.
+//   order_one      : single scattering and first-order surface fields, fp64
+//                    (rt_primary_source.c:17-198, rt_layer_transport.c:107-208,
+//                    rt_surface_boundary.c:210-527)
+//   order_status   : per-item convergence test of the previous order and the
+//                    lower-boundary reflection of the current field
+//                    (rt_solver.c:3053-3129, rt_surface_boundary.c:137-208)
+//   gemm_f64 / gemm_bf16 : scattering integral J = A . diag-mixed(F) on MFMA
+//                    (rt_sos_operator.c:206-424 as a dense product)
+//   sweep          : formal solution of every (Stokes, direction) column, one
+//                    thread per column scanning the levels, fused with the
+//                    order sum and its max-norms
+//   extract / finalize : view sampling, Fourier reconstruction, reflectance,
+//                    IPSS scaling, direct glint and transmittances
+//                    (rt_solver.c:1010-1276, rt_fourier.c:9-33, surface.c:1813-1907)
+#include "common.h"
+#include "device_math.h"
+#include "ocrt_amd.h"
+
+namespace ocrt {
+
+typedef double double4_ __attribute__((ext_vector_type(4)));
+typedef float float4_ __attribute__((ext_vector_type(4)));
+typedef short short4_ __attribute__((ext_vector_type(4)));
+
+struct LevelView {
+    const double *h, *ch, *xd, *yd, *z;
+    __device__ LevelView(const Pools &P, const DevInst &I)
+    {
+        const int L = I.nt + 1;
+        h = P.lev + I.off_lev; ch = h + L; xd = ch + L; yd = xd + L; z = yd + L;
+    }
+};
+
+// Formal solution of one column with the reference's linear-source closed form
+// (rt_layer_transport.c:89-99, 144-203), in fp64.  S(k) gives the source at
+// level k; emit(k, value) receives the field.
+template <typename SrcFn, typename EmitFn>
+__device__ inline void sweep_column_f64(int hemi, int nt, double mu, double bc, const double *h,
+                                        const double *T, int n, int o, SrcFn S, EmitFn emit)
+{
+    if (hemi == 0) {
+        double run = bc;
+        emit(nt, run);
+        double s_in = S(nt);
+        for (int k = nt - 1; k >= 0; --k) {
+            const double s_out = S(k);
+            const double dt = h[k + 1] - h[k];
+            const double tr = T[(long long)k * n + o];
+            const double a = (s_in - s_out) / dt;
+            const double b = s_out - a * h[k];
+            run = tr * run + ((1.0 - tr) * (b + a * mu) + a * (h[k] - h[k + 1] * tr)) * 0.5;
+            emit(k, run);
+            s_in = s_out;
+        }
+    } else {
+        double run = 0.0;
+        emit(0, run);
+        double s_in = S(0);
+        for (int k = 1; k <= nt; ++k) {
+            const double s_out = S(k);
+            const double dt = h[k] - h[k - 1];
+            const double tr = T[(long long)(k - 1) * n + o];
+            const double a = (s_out - s_in) / dt;
+            const double b = s_out - a * h[k];
+            run = tr * run + ((1.0 - tr) * (b - a * mu) + a * (h[k] - h[k - 1] * tr)) * 0.5;
+            emit(k, run);
+            s_in = s_out;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Order 1: one thread per state column of one item.
+template <typename FT, bool MIXED>
+__global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ inst,
+                                                 const DevItem *__restrict__ items, Pools P,
+                                                 ItemState *__restrict__ st)
+{
+    const DevItem &it = items[blockIdx.y];
+    const DevInst &I = inst[it.inst];
+    const int n = it.n, dirs = 2 * n + 1, nt = it.nt, Pp = it.P, m = it.m, lmax = it.lmax;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ItemState s;
+        s.active = I.max_orders >= 2 ? 1 : 0;
+        s.orders = 1;
+        s.converged = I.max_orders >= 2 ? 0 : 1;
+        s.status = 0;
+        s.residual = 0.0;
+        s.max_cur = 0ull;
+        s.max_tot = 0ull;
+        st[blockIdx.y] = s;
+    }
+    if (col >= Pp) return;
+    FT *F = reinterpret_cast<FT *>(P.F) + it.off_F;
+    double *tot = P.tot + it.off_tot;
+    double *comp = MIXED ? P.comp + it.off_tot : nullptr;
+    if (col >= 6 * n) {
+        for (int k = 0; k <= nt; ++k) {
+            F[(long long)k * Pp + col] = (FT)0;
+            tot[(long long)k * Pp + col] = 0.0;
+            if (MIXED) comp[(long long)k * Pp + col] = 0.0;
+        }
+        return;
+    }
+    const LevelView lv(P, I);
+    const int type = col / n, c = type >> 1, hemi = type & 1, o = col % n;
+    const int d = hemi ? n - (o + 1) : n + (o + 1);      // direction index of this column
+    const double *mu_ring = P.ring + I.off_ring;
+    const double mu = mu_ring[o];
+    const double *T = P.T + I.off_T;
+    const double *plm = P.basis + it.off_basis;
+    const double *rrl = plm + (long long)(lmax + 1) * dirs;
+    const double *rtl = rrl + (long long)(lmax + 1) * dirs;
+    const double *plm2 = plm + 2LL * dirs, *rrl2 = rrl + 2LL * dirs, *rtl2 = rtl + 2LL * dirs;
+    const long long cnt = (long long)(n + 1) * dirs;
+    const double *tab = P.tab + it.off_tab;
+    const int mol = (m <= 2);
+    const double b0 = (m == 0) ? I.beta0 : 0.0;
+    const double xs = plm2[n];                             // solar-slot basis at l = 2
+    // single-scattering phase entries of this column (rt_primary_source.c:61-70, 164-171)
+    double pc, mc, sgn;
+    if (c == 0) { pc = tab[0 * cnt + d]; mc = b0 + I.beta2 * plm2[d] * xs; sgn = 1.0; }
+    else if (c == 1) { pc = tab[1 * cnt + d]; mc = mol ? I.gamma2 * rrl2[d] * xs : 0.0; sgn = 1.0; }
+    else { pc = tab[2 * cnt + d]; mc = mol ? I.gamma2 * rtl2[d] * xs : 0.0; sgn = -1.0; }
+    auto S1 = [&](int k) { return sgn * lv.ch[k] * (lv.xd[k] * pc + lv.yd[k] * mc); };
+    sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o, S1,
+                     [&](int k, double v) { tot[(long long)k * Pp + col] = v; });
+
+    if (I.surf == 1) {
+        // flat Fresnel: the specularly reflected direct beam scattered once
+        // (rt_surface_boundary.c:210-428)
+        const double mus = I.mus, nw = I.n_water, n2 = nw * nw;
+        const double tn = sqrt(n2 - 1.0 + mus * mus);
+        const double rp = (n2 * mus - tn) / (n2 * mus + tn), rs = (mus - tn) / (mus + tn);
+        const double RI = (rp * rp + rs * rs) / 2.0;
+        const double RQ = (I.q_conv == 1) ? (rp * rp - rs * rs) / 2.0 : (rs * rs - rp * rp) / 2.0;
+        const double sb = 0.5 * exp(-2.0 * lv.h[nt] / mus);
+        const int dk = hemi ? n + (o + 1) : n - (o + 1);   // mirrored direction
+        const double bP = xs, bR = rrl2[n];
+        double mgr = 0.0, mart = 0.0;
+        if (I.aer) {
+            const double *g = P.greek + I.off_greek;
+            const int q = I.La + 1;
+            const int lend = lmax < I.La ? lmax : I.La;
+            for (int l = m; l <= lend; ++l) {
+                const long long b = (long long)l * dirs;
+                mgr += plm[b + dk] * rrl[b + n] * g[q + l];
+                mart += rtl[b + dk] * rrl[b + n] * g[2 * q + l] + rrl[b + dk] * rtl[b + n] * g[3 * q + l];
+            }
+        }
+        double pA, mA, pB, mB;
+        if (c == 0) {
+            pA = tab[0 * cnt + dk]; mA = mol ? b0 + I.beta2 * plm2[dk] * bP : 0.0;
+            pB = mgr; mB = mol ? I.gamma2 * bR * plm2[dk] : 0.0;
+        } else if (c == 1) {
+            pA = tab[1 * cnt + dk]; mA = mol ? I.gamma2 * rrl2[dk] * bP : 0.0;
+            pB = tab[3 * cnt + dk]; mB = mol ? I.alpha2 * bR * rrl2[dk] : 0.0;
+        } else {
+            pA = tab[2 * cnt + dk]; mA = mol ? I.gamma2 * rtl2[dk] * bP : 0.0;
+            pB = mart; mB = mol ? I.alpha2 * rtl2[dk] * bR : 0.0;
+        }
+        auto SF = [&](int k) -> double {
+            if ((hemi == 0 && k == nt) || (hemi == 1 && k == 0)) return 0.0;
+            const double x = lv.xd[k], y = lv.yd[k];
+            const double kA = pA * x + mA * y, kB = pB * x + mB * y;
+            const double beam = sb * exp(lv.h[k] / mus);
+            return beam * (RI * kA + RQ * kB);
+        };
+        sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o, SF,
+                         [&](int k, double v) { tot[(long long)k * Pp + col] += v; });
+    } else if (I.surf == 2 && hemi == 0) {
+        // rough ocean: direct beam reflected into every upward node
+        // (rt_surface_boundary.c:430-493)
+        const double tau = lv.h[nt], down = exp(-tau / I.mus);
+        const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
+        const double r = P.R[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c];
+        for (int k = 0; k <= nt; ++k)
+            tot[(long long)k * Pp + col] += r * (FS * (down * exp(-(tau - lv.h[k]) / mu)));
+    }
+    for (int k = 0; k <= nt; ++k) {
+        const long long ix = (long long)k * Pp + col;
+        F[ix] = (FT)tot[ix];
+        if (MIXED) comp[ix] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Convergence of the order just finished and the lower boundary condition of
+// the next one.  One workgroup per item.
+template <typename FT>
+__global__ void order_status(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P,
+                             ItemState *__restrict__ st, int it_next)
+{
+    const int item = blockIdx.x;
+    const DevItem &it = items[item];
+    const DevInst &I = inst[it.inst];
+    __shared__ int act;
+    if (threadIdx.x == 0) {
+        ItemState s = st[item];
+        if (it_next > 2 && s.active) {
+            const double mc = __longlong_as_double((long long)s.max_cur);
+            const double mt = __longlong_as_double((long long)s.max_tot);
+            const double res = (mt > 0.0) ? mc / mt : 0.0;
+            s.orders = it_next - 1;
+            s.residual = res;
+            if (!(mc == mc) || !(mt == mt)) { s.status = -2; s.active = 0; }
+            else if (res < I.tol) { s.converged = 1; s.active = 0; }
+            else if (it_next - 1 >= I.max_orders) s.active = 0;
+        }
+        s.max_cur = 0ull;
+        s.max_tot = 0ull;
+        st[item] = s;
+        act = s.active;
+    }
+    __syncthreads();
+    if (!act || I.surf == 0) return;
+    const int n = it.n, nt = it.nt, Pp = it.P, m = it.m;
+    const FT *F = reinterpret_cast<const FT *>(P.F) + it.off_F + (long long)nt * Pp;
+    const double *mu = P.ring + I.off_ring, *w = mu + n;
+    double *bc = P.bc + it.off_bc;
+    const double W = (I.surf == 1) ? 2.0 * OCRT_PI : ((m == 0) ? 2.0 * OCRT_PI : OCRT_PI);
+    for (int o = threadIdx.x; o < n; o += blockDim.x) {
+        double si = 0.0, sq = 0.0, su = 0.0;
+        if (I.surf == 1) {
+            double Fr[9];
+            fresnel_R(mu[o], I.n_water, I.q_conv, Fr);
+            const double a = (double)F[n + o], b = (double)F[3 * n + o], cc = (double)F[5 * n + o];
+            if (w[o] == 0.0) {
+                si = Fr[0] * a + Fr[1] * b + Fr[2] * cc;
+                sq = Fr[3] * a + Fr[4] * b + Fr[5] * cc;
+                su = Fr[6] * a + Fr[7] * b + Fr[8] * cc;
+            } else {
+                const double dd = 1.0 / (2.0 * OCRT_PI * mu[o] * w[o]);
+                const double pw = W * mu[o] * w[o];
+                double R[9];
+                for (int e = 0; e < 9; ++e) R[e] = dd * Fr[e];
+                si = pw * (R[0] * a + R[1] * b + R[2] * cc);
+                sq = pw * (R[3] * a + R[4] * b + R[5] * cc);
+                su = pw * (R[6] * a + R[7] * b + R[8] * cc);
+            }
+        } else {
+            const double *R0 = P.R + I.off_R + ((long long)m * n + o) * n * 9;
+            for (int i = 0; i < n; ++i) {
+                if (w[i] == 0.0) continue;
+                const double a = (double)F[n + i], b = (double)F[3 * n + i], cc = (double)F[5 * n + i];
+                const double pw = W * mu[i] * w[i];
+                const double *R = R0 + (long long)i * 9;
+                si += pw * (R[0] * a + R[1] * b + R[2] * cc);
+                sq += pw * (R[3] * a + R[4] * b + R[5] * cc);
+                su += pw * (R[6] * a + R[7] * b + R[8] * cc);
+            }
+        }
+        bc[o] = si;
+        bc[n + o] = sq;
+        bc[2 * n + o] = su;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scattering integral in fp64 on v_mfma_f64_16x16x4f64.  Tile 64 (outgoing
+// columns) x 64 (levels), K step 16, four waves of 32 x 32.
+// J[lev][r] = sum_k A[r][k] * s_part(lev) * F[lev][k mod P].
+__global__ __launch_bounds__(256) void gemm_f64(const DevInst *__restrict__ inst,
+                                                const DevItem *__restrict__ items, Pools P,
+                                                const GemmTile *__restrict__ tiles,
+                                                const ItemState *__restrict__ st)
+{
+    const GemmTile tl = tiles[blockIdx.x];
+    if (!st[tl.item].active) return;
+    const DevItem &it = items[tl.item];
+    const DevInst &I = inst[it.inst];
+    const int K = it.K, Pp = it.P, L = it.L;
+    const double *A = reinterpret_cast<const double *>(P.A) + it.off_A;
+    const double *F = reinterpret_cast<const double *>(P.F) + it.off_F;
+    double *J = reinterpret_cast<double *>(P.J) + it.off_F;
+    const LevelView lv(P, I);
+    __shared__ double As[64][17];
+    __shared__ double Bs[16][65];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int row0 = tl.tm * 64, lev0 = tl.tn * 64;
+    double4_ acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = (double4_){0.0, 0.0, 0.0, 0.0};
+    const int lr = t >> 2, lk = (t & 3) * 4;
+    const int lev_load = lev0 + lr;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const double *src = A + (long long)(row0 + lr) * K + k0 + lk;
+        for (int q = 0; q < 4; ++q) As[lr][lk + q] = src[q];
+        if (lev_load < L) {
+            const int part = (k0 + lk) / Pp;
+            const int kind = (it.parts >> (8 * part)) & 0xff;
+            const double s = (kind == 1) ? lv.xd[lev_load] : lv.yd[lev_load];
+            const double *fs = F + (long long)lev_load * Pp + (k0 + lk) % Pp;
+            for (int q = 0; q < 4; ++q) Bs[lk + q][lr] = s * fs[q];
+        } else {
+            for (int q = 0; q < 4; ++q) Bs[lk + q][lr] = 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 16; ks += 4) {
+            double av[2], bv[2];
+            for (int a = 0; a < 2; ++a) av[a] = As[wm + a * 16 + (lane & 15)][ks + (lane >> 4)];
+            for (int b = 0; b < 2; ++b) bv[b] = Bs[ks + (lane >> 4)][wn + b * 16 + (lane & 15)];
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // f64 16x16 result layout: lane holds rows (lane >> 4) + 4 i of column lane & 15
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const int lev = lev0 + wn + b * 16 + (lane & 15);
+            const int r = row0 + wm + a * 16 + (lane >> 4);
+            if (lev < L) {
+                double *dst = J + (long long)lev * Pp;
+                for (int i = 0; i < 4; ++i)
+                    if (r + 4 * i < Pp) dst[r + 4 * i] = acc[a][b][i];
+            }
+        }
+}
+
+// Scattering integral on bf16 MFMA (v_mfma_f32_16x16x16bf16_1k) with fp32
+// accumulation.  Tile 64 x 64, K step 32.  The operator is stored as a bf16
+// pair A = A_hi + A_lo (rows Mp..2Mp-1 hold A_lo) and both halves multiply the
+// same bf16 field fragment; the field is mixed by the level scattering
+// fractions in fp32 and rounded to bf16 once, on its way into LDS.
+__global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ inst,
+                                                 const DevItem *__restrict__ items, Pools P,
+                                                 const GemmTile *__restrict__ tiles,
+                                                 const ItemState *__restrict__ st)
+{
+    const GemmTile tl = tiles[blockIdx.x];
+    if (!st[tl.item].active) return;
+    const DevItem &it = items[tl.item];
+    const DevInst &I = inst[it.inst];
+    const int K = it.K, Pp = it.P, L = it.L;
+    const unsigned short *A = reinterpret_cast<const unsigned short *>(P.A) + it.off_A;
+    const float *F = reinterpret_cast<const float *>(P.F) + it.off_F;
+    float *J = reinterpret_cast<float *>(P.J) + it.off_F;
+    const LevelView lv(P, I);
+    __shared__ unsigned short As[64][40];
+    __shared__ unsigned short Al[64][40];
+    __shared__ unsigned short Bs[64][40];
+    const unsigned short *Alo = A + (long long)it.Mp * K;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int row0 = tl.tm * 64, lev0 = tl.tn * 64;
+    float4_ acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = (float4_){0.f, 0.f, 0.f, 0.f};
+    const int lr = t >> 2, lk = (t & 3) * 8;
+    const int lev_load = lev0 + lr;
+    const float sx = lev_load < L ? (float)lv.xd[lev_load] : 0.f;
+    const float sy = lev_load < L ? (float)lv.yd[lev_load] : 0.f;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const uint4 av = *reinterpret_cast<const uint4 *>(A + (long long)(row0 + lr) * K + k0 + lk);
+        *reinterpret_cast<uint4 *>(&As[lr][lk]) = av;
+        const uint4 avl = *reinterpret_cast<const uint4 *>(Alo + (long long)(row0 + lr) * K + k0 + lk);
+        *reinterpret_cast<uint4 *>(&Al[lr][lk]) = avl;
+        unsigned short bb[8];
+        if (lev_load < L) {
+            const int part = (k0 + lk) / Pp;
+            const int kind = (it.parts >> (8 * part)) & 0xff;
+            const float s = (kind == 1) ? sx : sy;
+            const float *fs = F + (long long)lev_load * Pp + (k0 + lk) % Pp;
+            const float4 f0 = *reinterpret_cast<const float4 *>(fs);
+            const float4 f1 = *reinterpret_cast<const float4 *>(fs + 4);
+            const float fv[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
+            for (int q = 0; q < 8; ++q) {
+                const __hip_bfloat16 hb = __float2bfloat16(s * fv[q]);
+                bb[q] = *reinterpret_cast<const unsigned short *>(&hb);
+            }
+        } else {
+            for (int q = 0; q < 8; ++q) bb[q] = 0;
+        }
+        uint4 bv;
+        bv.x = bb[0] | ((unsigned)bb[1] << 16); bv.y = bb[2] | ((unsigned)bb[3] << 16);
+        bv.z = bb[4] | ((unsigned)bb[5] << 16); bv.w = bb[6] | ((unsigned)bb[7] << 16);
+        *reinterpret_cast<uint4 *>(&Bs[lr][lk]) = bv;
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 32; ks += 16) {
+            short4_ a4[2], l4[2], b4[2];
+            for (int a = 0; a < 2; ++a) {
+                a4[a] = *reinterpret_cast<const short4_ *>(&As[wm + a * 16 + (lane & 15)][ks + 4 * (lane >> 4)]);
+                l4[a] = *reinterpret_cast<const short4_ *>(&Al[wm + a * 16 + (lane & 15)][ks + 4 * (lane >> 4)]);
+            }
+            for (int b = 0; b < 2; ++b)
+                b4[b] = *reinterpret_cast<const short4_ *>(&Bs[wn + b * 16 + (lane & 15)][ks + 4 * (lane >> 4)]);
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(l4[a], b4[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a4[a], b4[b], acc[a][b], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const int lev = lev0 + wn + b * 16 + (lane & 15);
+            const int r = row0 + wm + a * 16 + 4 * (lane >> 4);
+            if (lev < L && r < Pp) {
+                float4 v;
+                v.x = acc[a][b][0]; v.y = acc[a][b][1]; v.z = acc[a][b][2]; v.w = acc[a][b][3];
+                *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v;
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+__device__ inline double block_max(double v, double *red)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) r = fmax(r, red[i]);
+    }
+    __syncthreads();
+    return r;
+}
+
+// Formal solution of the current order for every column, fused with the order
+// sum total += cur (plain in fp64, compensated in the mixed form) and the two
+// max-norms of the convergence test (rt_solver.c:1855-1868, 3093-3124).
+template <typename FT, bool MIXED>
+__global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
+                                             const DevItem *__restrict__ items, Pools P,
+                                             ItemState *__restrict__ st)
+{
+    const int item = blockIdx.y;
+    if (!st[item].active) return;
+    const DevItem &it = items[item];
+    const DevInst &I = inst[it.inst];
+    const int n = it.n, nt = it.nt, Pp = it.P;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[8];
+    double mcur = 0.0, mtot = 0.0;
+    if (col < 6 * n) {
+        const LevelView lv(P, I);
+        const int type = col / n, c = type >> 1, hemi = type & 1, o = col % n;
+        const FT *J = reinterpret_cast<const FT *>(P.J) + it.off_F;
+        FT *F = reinterpret_cast<FT *>(P.F) + it.off_F;
+        double *tot = P.tot + it.off_tot;
+        double *cmp = MIXED ? P.comp + it.off_tot : nullptr;
+        const double mu = P.ring[I.off_ring + o];
+        const double bc = (I.surf != 0 && hemi == 0) ? P.bc[it.off_bc + c * n + o] : 0.0;
+        auto emit = [&](int k, double v) {
+            const long long ix = (long long)k * Pp + col;
+            F[ix] = (FT)v;
+            mcur = fmax(mcur, fabs(v));
+            if (MIXED) {
+                const double s = tot[ix], cc = cmp[ix];
+                const double tsum = s + v;
+                const double nc = cc + ((fabs(s) >= fabs(v)) ? ((s - tsum) + v) : ((v - tsum) + s));
+                tot[ix] = tsum;
+                cmp[ix] = nc;
+                mtot = fmax(mtot, fabs(tsum + nc));
+            } else {
+                const double tsum = tot[ix] + v;
+                tot[ix] = tsum;
+                mtot = fmax(mtot, fabs(tsum));
+            }
+        };
+        if (!MIXED) {
+            const double *T = P.T + I.off_T;
+            sweep_column_f64(hemi, nt, mu, bc, lv.h, T, n, o,
+                             [&](int k) { return (double)J[(long long)k * Pp + col]; }, emit);
+        } else {
+            // fp32 scan written with om = 1 - T (accurate from expm1):
+            //   run <- run - om run + (S_out - S_in + om (S_in + slope mu_signed)) / 2
+            // which is the linear-source recurrence with no 1 - T cancellation
+            const float *OM = P.Tf + I.off_T;
+            const float smu = hemi ? -(float)mu : (float)mu;
+            if (hemi == 0) {
+                float run = (float)bc;
+                emit(nt, (double)run);
+                float s_in = (float)J[(long long)nt * Pp + col];
+                for (int k = nt - 1; k >= 0; --k) {
+                    const float s_out = (float)J[(long long)k * Pp + col];
+                    const float idt = (float)(1.0 / (lv.h[k + 1] - lv.h[k]));
+                    const float om = OM[(long long)k * n + o];
+                    const float a = (s_in - s_out) * idt;
+                    run = run - om * run + 0.5f * ((s_out - s_in) + om * (s_in + a * smu));
+                    emit(k, (double)run);
+                    s_in = s_out;
+                }
+            } else {
+                float run = 0.f;
+                emit(0, 0.0);
+                float s_in = (float)J[col];
+                for (int k = 1; k <= nt; ++k) {
+                    const float s_out = (float)J[(long long)k * Pp + col];
+                    const float idt = (float)(1.0 / (lv.h[k] - lv.h[k - 1]));
+                    const float om = OM[(long long)(k - 1) * n + o];
+                    const float a = (s_out - s_in) * idt;
+                    run = run - om * run + 0.5f * ((s_out - s_in) + om * (s_in + a * smu));
+                    emit(k, (double)run);
+                    s_in = s_out;
+                }
+            }
+        }
+    }
+    const double bm = block_max(mcur, red);
+    const double bt = block_max(mtot, red);
+    if (threadIdx.x == 0) {
+        atomicMax(&st[item].max_cur, (unsigned long long)__double_as_longlong(bm));
+        atomicMax(&st[item].max_tot, (unsigned long long)__double_as_longlong(bt));
+    }
+}
+
+// ---------------------------------------------------------------------------
+__device__ inline double total_at(const Pools &P, const DevItem &it, bool mixed, long long ix)
+{
+    const double *tot = P.tot + it.off_tot;
+    return mixed ? tot[ix] + P.comp[it.off_tot + ix] : tot[ix];
+}
+
+// Per-item outputs after the order loop (rt_solver.c:1010-1061).
+__global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P,
+                        const ItemState *__restrict__ st, ItemOut *__restrict__ out, int mixed)
+{
+    const int item = blockIdx.x;
+    const DevItem &it = items[item];
+    const DevInst &I = inst[it.inst];
+    const int n = it.n, nt = it.nt, Pp = it.P, m = it.m, v = I.vidx;
+    const double *mu = P.ring + I.off_ring, *w = mu + n;
+    const LevelView lv(P, I);
+    __shared__ double red[8];
+    double hs = 0.0;
+    for (int o = threadIdx.x; o < n; o += blockDim.x)
+        hs += mu[o] * w[o] * total_at(P, it, mixed, (long long)nt * Pp + n + o);
+    // ordered sum over nodes for the hemispheric flux
+    red[0] = 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double F = 0.0;
+        for (int o = 0; o < n; ++o) F += mu[o] * w[o] * total_at(P, it, mixed, (long long)nt * Pp + n + o);
+        const ItemState s = st[item];
+        ItemOut r;
+        const double tau = lv.h[nt];
+        for (int c = 0; c < 3; ++c) {
+            double val = total_at(P, it, mixed, (long long)(2 * c) * n + v);
+            if (I.surf == 2) {
+                const double down = exp(-tau / I.mus);
+                const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
+                const double rk = P.R[I.off_Rsun + ((long long)m * n + v) * 9 + 3 * c];
+                val -= rk * (FS * (down * exp(-(tau - lv.h[0]) / mu[v])));
+            }
+            r.mode[c] = val;
+        }
+        r.boa_view = total_at(P, it, mixed, (long long)nt * Pp + n + v);
+        r.boa_hemi = F;
+        r.orders = s.orders;
+        r.converged = s.converged;
+        r.residual = s.residual;
+        out[item] = r;
+        (void)hs;
+    }
+    if (P.toa_up) {
+        double *dst = P.toa_up + (long long)item * P.toa_up_stride;
+        for (int q = threadIdx.x; q < 3 * n; q += blockDim.x) {
+            const int c = q / n, o = q % n;
+            double val = total_at(P, it, mixed, (long long)(2 * c) * n + o);
+            if (I.surf == 2) {
+                const double tau = lv.h[nt], down = exp(-tau / I.mus);
+                const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
+                val -= P.R[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c] *
+                       (FS * (down * exp(-(tau - lv.h[0]) / mu[o])));
+            }
+            dst[q] = val;
+        }
+    }
+}
+
+// Direct sunglint reflectance (surface.c:1813-1907).
+__device__ inline void sunglint_rho(double mu_v, double phi_v, double mu_0, double ws, int st, double nw,
+                                    double tau, int qc, double *rho)
+{
+    const double trans = exp(-tau / mu_0) * exp(-tau / mu_v);
+    if (ws < 1e-6) {
+        const double sf = 0.5 * OCRT_PI / 180.0;
+        const double st0 = sqrt(fmax(0.0, 1.0 - mu_0 * mu_0));
+        const double smu = sf * fmax(st0, 1e-3);
+        double de = phi_v;
+        while (de > OCRT_PI) de -= 2.0 * OCRT_PI;
+        while (de < -OCRT_PI) de += 2.0 * OCRT_PI;
+        const double dmu = mu_v - mu_0;
+        const double ex = -0.5 * ((dmu * dmu) / (smu * smu) + (de * de) / (sf * sf));
+        if (ex < -50.0) { rho[0] = rho[1] = rho[2] = 0.0; return; }
+        double MF[9];
+        fresnel_R(mu_0, nw, qc, MF);
+        const double den = fmax(2.0 * mu_0 * st0 * mu_0 * mu_v, 1e-12);
+        const double r = 4.0 * OCRT_PI * trans / den * exp(ex);
+        rho[0] = MF[0] * r; rho[1] = MF[3] * r; rho[2] = MF[6] * r;
+        return;
+    }
+    double R[9];
+    coxmunk_R(mu_v, mu_0, cos(phi_v), sin(phi_v), ws, st, nw, qc, R);
+    rho[0] = OCRT_PI * R[0] * trans; rho[1] = OCRT_PI * R[3] * trans; rho[2] = OCRT_PI * R[6] * trans;
+}
+
+// Instance results (rt_solver.c:1062-1276): Fourier reconstruction at the view
+// azimuth, rho = X / mu_s, IPSS kappa, direct glint and transmittances.
+
+
+__global__ void finalize(const DevInst *__restrict__ inst, int n_inst, const ItemOut *__restrict__ io,
+                         const InstOut *__restrict__ ipss, const Pools P, ocrt_result_t *__restrict__ res)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    const DevInst &I = inst[i];
+    const LevelView lv(P, I);
+    const int mm = I.mmax;
+    const double phi = I.raa * OCRT_PI / 180.0 + OCRT_PI;
+    double Iv = io[I.item0].mode[0], Qv = io[I.item0].mode[1], Uv = 0.0, Iboa = io[I.item0].boa_view;
+    int maxo = 0, conv = 1, status = 0;
+    for (int m = 0; m <= mm; ++m) {
+        const ItemOut &r = io[I.item0 + m];
+        if (m > 0) {
+            const double cm = cos((double)m * phi), sm = sin((double)m * phi);
+            Iv += 2.0 * r.mode[0] * cm;
+            Qv += 2.0 * r.mode[1] * cm;
+            Uv += 2.0 * r.mode[2] * sm;
+            Iboa += 2.0 * r.boa_view * cm;
+        }
+        if (r.orders > maxo) maxo = r.orders;
+        if (!r.converged) conv = 0;
+    }
+    ocrt_result_t o;
+    memset(&o, 0, sizeof(o));
+    const double mus = I.mus, muv = I.muv;
+    o.I_TOA = Iv; o.Q_TOA = Qv; o.U_TOA = Uv;
+    o.rho_I = Iv / mus; o.rho_Q = Qv / mus; o.rho_U = Uv / mus;
+    o.tau_R_total = I.tauR;
+    double kappa = 1.0, tdir = -1.0;
+    if (I.ipss) { kappa = ipss[i].kappa; tdir = ipss[i].tau_dir_ipss; }
+    o.kappa = kappa;
+    if (I.ipss) {
+        o.rho_I *= kappa; o.rho_Q *= kappa; o.rho_U *= kappa;
+        o.I_TOA *= kappa; o.Q_TOA *= kappa; o.U_TOA *= kappa;
+    }
+    const double tau = lv.h[I.nt];
+    const double phv = (I.raa - 180.0) * OCRT_PI / 180.0;
+    const int bfo = (I.surface_kind == 2);
+    if (bfo && I.decouple == 0) {
+        double rg[3];
+        sunglint_rho(I.vza > 0.0 ? muv : 1.0, phv, mus, I.wind, I.sigma_type, I.n_water, tau, I.q_conv, rg);
+        if (I.ipss && tdir >= 0.0) { const double c = exp(-(tdir - tau / mus)); rg[0] *= c; rg[1] *= c; rg[2] *= c; }
+        o.rho_I += rg[0]; o.rho_Q += rg[1]; o.rho_U += -rg[2];
+    }
+    o.n_orders_used = maxo;
+    o.converged = conv;
+    o.T_dir_dn = (I.ipss && tdir >= 0.0) ? exp(-tdir) : exp(-tau / mus);
+    o.T_diff_dn_hemi = (io[I.item0].boa_hemi * (2.0 * OCRT_PI)) / (OCRT_PI * mus);
+    o.T_total_dn_hemi = o.T_dir_dn + o.T_diff_dn_hemi;
+    o.T_diff_dn_dir = (2.0 * OCRT_PI * muv * Iboa) / (OCRT_PI * mus);
+    o.T_total_up_dir = o.rho_I * mus;
+    o.T_dir_up_view = exp(-tau / (muv > 0.0 ? muv : 1.0));
+    o.T_sg_up_dir = 0.0;
+    o.rho_glint_direct_I = o.rho_glint_direct_Q = o.rho_glint_direct_U = 0.0;
+    if (bfo && I.wind > 0.0) {
+        double rg[3];
+        sunglint_rho(I.vza > 0.0 ? muv : 1.0, phv, mus, I.wind, I.sigma_type, I.n_water, tau, I.q_conv, rg);
+        if (I.ipss && tdir >= 0.0) { const double c = exp(-(tdir - tau / mus)); rg[0] *= c; rg[1] *= c; rg[2] *= c; }
+        o.T_sg_up_dir = rg[0] * mus;
+        o.rho_glint_direct_I = rg[0]; o.rho_glint_direct_Q = rg[1]; o.rho_glint_direct_U = rg[2];
+    }
+    if (!(o.rho_I == o.rho_I) || !(o.rho_Q == o.rho_Q) || !(o.rho_U == o.rho_U) || !isfinite(o.rho_I))
+        status = -2;
+    o.status = status;
+    o.n_ring = I.n;
+    res[i] = o;
+}
+
+// explicit instantiations
+template __global__ void order_one<double, false>(const DevInst *, const DevItem *, Pools, ItemState *);
+template __global__ void order_one<float, true>(const DevInst *, const DevItem *, Pools, ItemState *);
+template __global__ void order_status<double>(const DevInst *, const DevItem *, Pools, ItemState *, int);
+template __global__ void order_status<float>(const DevInst *, const DevItem *, Pools, ItemState *, int);
+template __global__ void sweep<double, false>(const DevInst *, const DevItem *, Pools, ItemState *);
+template __global__ void sweep<float, true>(const DevInst *, const DevItem *, Pools, ItemState *);
+
+}  // namespace ocrt

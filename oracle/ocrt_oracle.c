@@ -33,18 +33,18 @@
 
 #define PI 3.14159265358979323846
 
-static int g_bf16 = 0;
-void oracle_set_bf16_emulation(int bf16) { g_bf16 = bf16; }
+static int g_bf16 = 0;      /* packed: significant bits of operator (>> 8) and field (& 0xff) */
+void oracle_set_bf16_emulation(int bf16) { g_bf16 = (bf16 == 1) ? ((8 << 8) | 8) : bf16; }
 
-static double bf16_round(double x)
+/* round to p significant bits, nearest-even (p = 8: bfloat16, 11: binary16) */
+static double round_bits(double x, int p)
 {
-    float f = (float)x;
-    uint32_t u;
-    memcpy(&u, &f, 4);
-    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
-    memcpy(&f, &u, 4);
-    return (double)f;
+    if (p <= 0 || x == 0.0 || !isfinite(x)) return x;
+    int e;
+    const double f = frexp(x, &e);
+    return ldexp(nearbyint(ldexp(f, p)), e - p);
 }
+
 
 /* ------------------------------------------------------------------------ */
 /* Quadrature: positive half of the 2n-point Gauss-Legendre rule, ascending,
@@ -643,14 +643,13 @@ static void ipss_weights(ipss_t *G, double vza, double raa)
     for (int j = 0; j < G->nlay; ++j) G->w_src[j] = G->w_ray[j] * PR + G->w_aer[j] * PA;
 }
 
-/* Solar slant optical depth from r0 to TOA by ray-shell crossings
- * (rt_ipss.c:209-333); INFINITY when the point is in the Earth's shadow. */
-static double ipss_tau_sun(const ipss_t *G, const double r0[3])
+/* Optical depth along the ray r0 + t u from ray-shell crossings sorted by
+ * construction, each interval assigned to the layer of its midpoint, the ray
+ * truncated at the first ground hit (rt_ipss.c:209-316). */
+static double ipss_ray_tau(const ipss_t *G, const double r0[3], const double uin[3], int *hit_out)
 {
-    const double mu = G->mu_sun, sn = sqrt(fmax(0.0, 1.0 - mu * mu));
-    double u[3] = {-sn, 0.0, mu};
-    const double un = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    u[0] /= un; u[1] /= un; u[2] /= un;
+    const double un = sqrt(uin[0] * uin[0] + uin[1] * uin[1] + uin[2] * uin[2]);
+    const double u[3] = {uin[0] / un, uin[1] / un, uin[2] / un};
     const double b = r0[0] * u[0] + r0[1] * u[1] + r0[2] * u[2];
     const double c0 = r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2];
     const int nl = G->nlay;
@@ -661,7 +660,6 @@ static double ipss_tau_sun(const ipss_t *G, const double r0[3])
         const double q = sqrt(dg), c[2] = {-b - q, -b + q};
         for (int i = 0; i < 2; ++i) if (c[i] > 1e-9 && c[i] < tstop) { tstop = c[i]; hit = 1; }
     }
-    if (hit) return INFINITY;
     double *ts = G->ts;
     const int cap = 2 * nl + 8;
     int n = 0;
@@ -678,6 +676,7 @@ static double ipss_tau_sun(const ipss_t *G, const double r0[3])
         const double t = -b + sqrt(d);
         if (t > 1e-9 && t <= tstop && n < cap) ts[n++] = t;
     }
+    if (hit && n < cap) ts[n++] = tstop;
     int m = 0;
     for (int i = 0; i < n; ++i) if (i == 0 || ts[i] > ts[m - 1] + 1e-12) ts[m++] = ts[i];
     double tau = 0.0;
@@ -690,7 +689,19 @@ static double ipss_tau_sun(const ipss_t *G, const double r0[3])
         if (lo > nl - 1) lo = nl - 1;
         tau += (ts[i + 1] - ts[i]) * G->beta[lo];
     }
+    if (hit_out) *hit_out = hit;
     return tau;
+}
+
+/* Solar slant optical depth from r0 to TOA; INFINITY in the Earth's shadow
+ * (rt_ipss.c:320-333). */
+static double ipss_tau_sun(const ipss_t *G, const double r0[3])
+{
+    const double mu = G->mu_sun, sn = sqrt(fmax(0.0, 1.0 - mu * mu));
+    const double u[3] = {-sn, 0.0, mu};
+    int hit = 0;
+    const double t = ipss_ray_tau(G, r0, u, &hit);
+    return hit ? INFINITY : t;
 }
 
 static int ipss_layer_of(const ipss_t *G, double h)
@@ -963,12 +974,13 @@ static int apply_operator(const med_t *M, int m, const ang_t *A, const double *I
                 double e[18];
                 for (int q = 0; q < 18; ++q) {
                     double pc = c[q] * x + y * r[q];
-                    e[q] = emu ? bf16_round(pc) : pc;
+                    e[q] = emu ? round_bits(pc, emu >> 8) : pc;
                 }
                 double a0 = Ip[FI(k, i)], a1 = Ip[FI(k, -i)], a2_ = Qp[FI(k, i)], a3 = Qp[FI(k, -i)];
                 double a4 = Up[FI(k, i)], a5 = Up[FI(k, -i)];
-                if (emu) { a0 = bf16_round(a0); a1 = bf16_round(a1); a2_ = bf16_round(a2_);
-                           a3 = bf16_round(a3); a4 = bf16_round(a4); a5 = bf16_round(a5); }
+                if (emu) { const int pf = emu & 0xff;
+                           a0 = round_bits(a0, pf); a1 = round_bits(a1, pf); a2_ = round_bits(a2_, pf);
+                           a3 = round_bits(a3, pf); a4 = round_bits(a4, pf); a5 = round_bits(a5, pf); }
                 iu += w * (a0 * e[0] + a2_ * e[8] - a4 * e[4]) + w * (a1 * e[1] + a3 * e[9] - a5 * e[5]);
                 id += w * (a0 * e[1] + a2_ * e[9] + a4 * e[5]) + w * (a1 * e[0] + a3 * e[8] + a5 * e[4]);
                 qu += w * (a0 * e[6] + a2_ * e[10] - a4 * e[12]) + w * (a1 * e[7] + a3 * e[11] + a5 * e[13]);
@@ -1558,4 +1570,51 @@ double oracle_ipss_tau_dir_dn(const ocrt_instance_t *in)
     ipss_free(&G);
     solve_free(&s);
     return t;
+}
+
+/* Homogeneous 0-100 km shell atmosphere of total vertical depth tau_t in nlay
+ * equal layers, used by the reference's IPSS geometry gate
+ * (tests/test_ipss_gates.c:43-58, 75-130). */
+static int ipss_homog(ipss_t *G, double tau_t, int nlay, double sza)
+{
+    memset(G, 0, sizeof(*G));
+    G->nlay = nlay; G->Re = 6371.0; G->H = 100.0; G->mu_sun = cos(sza * PI / 180.0);
+    G->L_aer = -1;
+    G->h_edge = malloc(sizeof(double) * (nlay + 1));
+    G->beta = malloc(sizeof(double) * nlay);
+    G->omega = malloc(sizeof(double) * nlay);
+    G->tau_above = malloc(sizeof(double) * (nlay + 1));
+    G->ts = malloc(sizeof(double) * (2 * nlay + 8));
+    G->los_t = malloc(sizeof(double) * (2 * nlay + 8));
+    G->los_tau = malloc(sizeof(double) * (2 * nlay + 8));
+    G->los_lay = malloc(sizeof(int) * (2 * nlay + 8));
+    for (int k = 0; k <= nlay; ++k) G->h_edge[k] = 100.0 * (double)k / nlay;
+    for (int j = 0; j < nlay; ++j) { G->beta[j] = tau_t / 100.0; G->omega[j] = 1.0; }
+    G->tau_above[nlay] = 0.0;
+    for (int j = nlay - 1; j >= 0; --j)
+        G->tau_above[j] = G->tau_above[j + 1] + G->beta[j] * (G->h_edge[j + 1] - G->h_edge[j]);
+    ipss_gl2(G->glx, G->glw);
+    return 0;
+}
+
+double oracle_ipss_homog_tau_view(double tau_t, int nlay, double vza_deg)
+{
+    ipss_t G;
+    ipss_homog(&G, tau_t, nlay, 0.0);
+    const double tv = vza_deg * PI / 180.0;
+    const double r0[3] = {0.0, 0.0, G.Re + G.H};
+    const double u[3] = {-sin(tv), 0.0, -cos(tv)};
+    int hit = 0;
+    const double t = ipss_ray_tau(&G, r0, u, &hit);
+    ipss_free(&G);
+    return t;
+}
+
+double oracle_ipss_homog_tau_sun(double tau_t, int nlay, double sza_deg, const double *rB)
+{
+    ipss_t G;
+    ipss_homog(&G, tau_t, nlay, sza_deg);
+    const double t = ipss_tau_sun(&G, rB);
+    ipss_free(&G);
+    return isfinite(t) ? t : -1.0;
 }

@@ -18,10 +18,11 @@
 extern "C" {
 #endif
 
-/* Emulation switch for precision-budget studies: when bf16 != 0 the
- * scattering integral of orders >= 2 rounds the operator coefficients and the
- * incoming field to bfloat16 before the contraction (the MFMA input rounding
- * of the mixed form).  Default 0 (pure double precision). */
+/* Emulation switch for precision-budget studies: the scattering integral of
+ * orders >= 2 rounds the operator coefficients to (mode >> 8) and the incoming
+ * field to (mode & 0xff) significant bits before the contraction (8 =
+ * bfloat16, 11 = binary16; mode 1 = both bfloat16).  Default 0 (pure double
+ * precision). */
 void oracle_set_bf16_emulation(int bf16);
 
 /* Full atmospheric solve of one instance (rt_solve_case_pol_impl,
@@ -66,6 +67,11 @@ int oracle_coxmunk_fourier(const double *mu_o, int n_o, const double *mu_i, int 
 /* IPSS kappa for an instance at (vza, raa) (rt_ipss.c:607-644) */
 double oracle_ipss_kappa(const ocrt_instance_t *in, double vza_deg, double raa_deg);
 double oracle_ipss_tau_dir_dn(const ocrt_instance_t *in);
+/* IPSS geometry on the homogeneous 0-100 km test atmosphere of the
+ * reference gates: TOA-anchored line-of-sight depth and solar depth at rB
+ * (-1 when shadowed). */
+double oracle_ipss_homog_tau_view(double tau_t, int nlay, double vza_deg);
+double oracle_ipss_homog_tau_sun(double tau_t, int nlay, double sza_deg, const double *rB);
 /* Fourier reconstruction (rt_fourier.c:9-33) */
 double oracle_reconstruct_cos(const double *modes, int m_max, double raa_rad);
 double oracle_reconstruct_sin(const double *modes, int m_max, double raa_rad);

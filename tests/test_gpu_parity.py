@@ -1,0 +1,140 @@
+"""Parity of the HIP solver (through the C ABI) with the reference and the oracle.
+
+FP64 form: every output within 1e-11 relative of the reference golden vectors
+(the same inputs solved by the reference C solver), identical SOS order counts.
+Mixed form (bf16 MFMA scattering integral, orders >= 2): rho_I within 1e-4
+relative, rho_Q and rho_U within 1e-4 absolute of the reference.
+Beyond the reference (Fourier modes above 32): FP64 form against the oracle.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import desc_from_dict, oracle_solve
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-11
+MIXED_TOL_I = 1e-4     # relative, rho_I
+MIXED_TOL_QU = 1e-4    # absolute, rho_Q and rho_U
+
+
+@pytest.fixture(scope="module")
+def ocrt():
+    import ocrt_amd
+    ocrt_amd.library()
+    return ocrt_amd
+
+
+@pytest.fixture(scope="module")
+def golden_descs(golden_cases):
+    keep, descs, names = [], [], []
+    for rec in golden_cases:
+        d, g = desc_from_dict(rec["desc"])
+        keep.append(g)
+        descs.append(d)
+        names.append(rec["name"])
+    return descs, keep, names
+
+
+@pytest.fixture(scope="module")
+def fp64_out(ocrt, golden_descs):
+    descs, keep, names = golden_descs
+    print(f"[gpu] fp64 batch of {len(descs)} golden cases", flush=True)
+    return ocrt.solve(descs, precision="fp64", modes=True)
+
+
+@pytest.fixture(scope="module")
+def mixed_out(ocrt, golden_descs):
+    descs, keep, names = golden_descs
+    print(f"[gpu] mixed batch of {len(descs)} golden cases", flush=True)
+    return ocrt.solve(descs, precision="mixed", modes=True)
+
+
+def test_fp64_matches_reference(golden_cases, fp64_out):
+    worst = 0.0
+    for i, rec in enumerate(golden_cases):
+        ref = rec["result"]
+        assert fp64_out["status"][i] == 0
+        scale = abs(ref["rho_I"])
+        for f in ("rho_I", "rho_Q", "rho_U"):
+            err = abs(fp64_out[f][i] - ref[f]) / scale
+            worst = max(worst, err)
+            assert err <= FP64_TOL, (rec["name"], f, fp64_out[f][i], ref[f])
+        for f in ("T_dir_dn", "T_diff_dn_hemi", "T_total_dn_hemi", "T_diff_dn_dir", "T_dir_up_view",
+                  "T_total_up_dir"):
+            assert abs(fp64_out[f][i] - ref[f]) <= FP64_TOL * abs(ref[f]), (rec["name"], f)
+        for f in ("T_sg_up_dir", "rho_glint_direct_I", "rho_glint_direct_Q", "rho_glint_direct_U"):
+            assert abs(fp64_out[f][i] - ref[f]) <= FP64_TOL * max(abs(ref[f]), 1e-12), (rec["name"], f)
+        assert fp64_out["n_orders_used"][i] == ref["n_orders_used"], rec["name"]
+        assert fp64_out["converged"][i] == ref["converged"], rec["name"]
+        mm = len(rec["orders_per_m"])
+        rm = np.array(rec["toa_modes"]).reshape(-1, 3)
+        gm = fp64_out["toa_modes"][i, :mm]
+        assert np.max(np.abs(gm - rm)) <= FP64_TOL * np.max(np.abs(rm[:, 0])), rec["name"]
+        assert list(fp64_out["orders_per_m"][i, :mm]) == rec["orders_per_m"], rec["name"]
+    print(f"[gpu] fp64 worst rho error / rho_I = {worst:.2e}")
+
+
+def test_mixed_within_budget(golden_cases, mixed_out):
+    wi = wq = 0.0
+    for i, rec in enumerate(golden_cases):
+        ref = rec["result"]
+        assert mixed_out["status"][i] == 0
+        ei = abs(mixed_out["rho_I"][i] - ref["rho_I"]) / abs(ref["rho_I"])
+        eq = max(abs(mixed_out["rho_Q"][i] - ref["rho_Q"]), abs(mixed_out["rho_U"][i] - ref["rho_U"]))
+        wi, wq = max(wi, ei), max(wq, eq)
+        assert ei <= MIXED_TOL_I, (rec["name"], ei)
+        assert eq <= MIXED_TOL_QU, (rec["name"], eq)
+    print(f"[gpu] mixed worst rel I {wi:.2e}, abs Q/U {wq:.2e}")
+
+
+def test_fp64_beyond_reference_modes(ocrt, oracle):
+    """Fourier modes above the reference's cap of 32: FP64 GPU vs the oracle."""
+    from ocrt_amd import synthetic as S
+    gk = S.hg_greek(0.72, 70)
+    from ocrt_amd.abi import make_instance
+    descs = [make_instance(surface="black_fresnel_ocean", wind=6.0, tau_a=0.25, ssa_a=0.93, greek=gk,
+                           n_mu=16, n_layers=40, m_max=44, sza=50.0, vza=35.0, raa=60.0),
+             make_instance(surface="flat", tau_a=0.12, ssa_a=0.98, greek=gk, n_mu=12, n_layers=30,
+                           m_max=40, sza=30.0, vza=62.0, raa=140.0, ipss=1)]
+    out = ocrt.solve(descs, precision="fp64", modes=True)
+    for i, d in enumerate(descs):
+        rc, r, modes, orders = oracle_solve(oracle, d)
+        assert rc == 0
+        for f in ("rho_I", "rho_Q", "rho_U"):
+            assert abs(out[f][i] - getattr(r, f)) <= FP64_TOL * abs(r.rho_I), (i, f)
+        assert np.max(np.abs(out["toa_modes"][i, : d.m_max + 1] - modes)) <= FP64_TOL * np.max(np.abs(modes))
+        assert list(out["orders_per_m"][i, : d.m_max + 1]) == list(orders)
+
+
+def test_fp64_deterministic(ocrt, golden_descs):
+    descs, keep, names = golden_descs
+    sub = descs[:12]
+    a = ocrt.solve(sub, precision="fp64", modes=True)
+    b = ocrt.solve(sub, precision="fp64", modes=True)
+    for f in ("rho_I", "rho_Q", "rho_U", "toa_modes"):
+        assert np.array_equal(a[f], b[f]), f
+
+
+def test_one_instance_per_launch_equals_batched(ocrt, golden_descs):
+    descs, keep, names = golden_descs
+    sub = descs[:10]
+    a = ocrt.solve(sub, precision="fp64", modes=True)
+    b = ocrt.solve(sub, precision="fp64", modes=True, one_per_launch=True)
+    for f in ("rho_I", "rho_Q", "rho_U", "toa_modes"):
+        assert np.array_equal(a[f], b[f]), f
+
+
+def test_mixed_vs_fp64_full_size_batch(ocrt):
+    """Config-2 instances at their full sizes (16..64 streams, 50..400 layers):
+    the mixed form stays inside the error budget of the FP64 form."""
+    from ocrt_amd import synthetic as S
+    descs, keep = S.config2(n_geom=6)
+    a = ocrt.solve(descs, precision="fp64")
+    b = ocrt.solve(descs, precision="mixed")
+    ei = np.abs(b["rho_I"] - a["rho_I"]) / np.abs(a["rho_I"])
+    eq = np.maximum(np.abs(b["rho_Q"] - a["rho_Q"]), np.abs(b["rho_U"] - a["rho_U"]))
+    print(f"[gpu] config-2 sample: mixed vs fp64 worst rel I {ei.max():.2e}, abs Q/U {eq.max():.2e}")
+    assert ei.max() <= MIXED_TOL_I and eq.max() <= MIXED_TOL_QU
