@@ -83,6 +83,7 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
             raise ValueError("greek must have shape [4, L+1]")
         d.L_max = g.shape[1] - 1
         d.greek = g.ctypes.data_as(C.POINTER(C.c_double))
+        d._greek_keepalive = g   # a converted copy lives as long as the descriptor
     else:
         d.L_max = 0
         d.greek = None

@@ -49,7 +49,7 @@ struct DevItem {
     long long off_A;      // pool_A: [Mp][K]
     long long off_F;      // pool_F / pool_J: [L][P]
     long long off_tot;    // pool_tot / pool_comp: [L][P]
-    long long off_bc;     // pool_bc: [3][n]
+    long long off_bc;     // pool_bc: [3][n] boundary condition, then [3][n] bottom downward field
 };
 
 struct ItemState {
@@ -72,7 +72,8 @@ struct InstOut {
 };
 
 struct Pools {
-    double *lev, *ring, *T, *greek, *R, *basis, *tab, *tot, *comp, *bc;
+    double *lev, *ring, *T, *greek, *R, *basis, *tab, *tot, *bc;
+    float *comp;          // fp32 compensation of the mixed-form order sum
     float *Tf;            // fp32 1 - T for the mixed-form sweeps
     void *A, *F, *J;      // double or bf16/float depending on the form
     double *cm_w;         // cos/sin(m phi) table [(Mall+1)][2][kNPhi]

@@ -27,7 +27,7 @@ template <typename FT, bool MIXED> __global__ void order_one(const DevInst *, co
 template <typename FT> __global__ void order_status(const DevInst *, const DevItem *, Pools, ItemState *, int);
 __global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *);
 __global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *);
-template <typename FT, bool MIXED> __global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *);
+template <typename FT, typename JT, bool MIXED> __global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *);
 __global__ void extract(const DevInst *, const DevItem *, Pools, const ItemState *, ItemOut *, int);
 __global__ void finalize(const DevInst *, int, const ItemOut *, const InstOut *, const Pools, ocrt_result_t *);
 __global__ void ipss_kappa(const DevInst *, const IpssJob *, Pools, double *, double *);
@@ -261,7 +261,7 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     }
     // ---- chunking: instances grouped so one chunk's working set stays bounded
     const double budget = mixed ? 48e9 : 48e9;
-    const size_t fe = mixed ? 4 : 8, ae = mixed ? 2 : 8;
+    const size_t fe = mixed ? 2 : 8, je = mixed ? 4 : 8, ae = mixed ? 2 : 8;
     int item_global = 0;
     p->item_of_inst.resize(n_inst);
     int i = 0;
@@ -279,7 +279,7 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
                 const int parts = (I.aer ? 1 : 0) + ((m <= 2 && I.ray) ? 1 : 0);
                 const int K = std::max(parts, 1) * P;
                 b += 3.0 * (I.lmax + 1) * (2 * n + 1) * 8 + 6.0 * (n + 1) * (2 * n + 1) * 8 +
-                     (double)Mp * K * ae * (mixed ? 2 : 1) + (double)L * P * (2 * fe + (mixed ? 16 : 8)) + 3.0 * n * 8;
+                     (double)Mp * K * ae * (mixed ? 2 : 1) + (double)L * P * (fe + je + (mixed ? 12 : 8)) + 6.0 * n * 8;
             }
             const int items_after = (item_global - c.it0) + I.mmax + 1;
             if (i > c.i0 && (bytes + b > budget || items_after > 60000 || i - c.i0 >= 60000 || p->per_inst)) break;
@@ -348,7 +348,7 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
                 it.off_tab = tab; tab += 6LL * (n + 1) * (2 * n + 1);
                 it.off_A = A; A += (long long)Mp * it.K * (mixed ? 2 : 1);
                 it.off_F = F; it.off_tot = F; F += (long long)L * Pp;
-                it.off_bc = bc; bc += 3LL * n;
+                it.off_bc = bc; bc += 6LL * n;
                 c.max_pack = std::max(c.max_pack, (long long)Mp * it.K);
                 const int item_local = (int)c.hitem.size();
                 for (int tm = 0; tm < it.tiles_m; ++tm)
@@ -391,9 +391,9 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     rc |= dmalloc(p, (void **)&P.tab, mx_tab * 8);
     rc |= dmalloc(p, (void **)&P.A, mx_A * ae);
     rc |= dmalloc(p, (void **)&P.F, mx_F * fe);
-    rc |= dmalloc(p, (void **)&P.J, mx_F * fe);
+    rc |= dmalloc(p, (void **)&P.J, mx_F * je);
     rc |= dmalloc(p, (void **)&P.tot, mx_F * 8);
-    if (mixed) rc |= dmalloc(p, (void **)&P.comp, mx_F * 8);
+    if (mixed) rc |= dmalloc(p, (void **)&P.comp, mx_F * 4);
     rc |= dmalloc(p, (void **)&P.bc, mx_bc * 8);
     if (rc) { ocrt_plan_destroy(p); return OCRT_ERR_ALLOC; }
     P.ring = p->ring;
@@ -477,7 +477,7 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
         hipLaunchKernelGGL(pack_operator<double>, dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst, c.ditem, P);
     const dim3 cols((c.max_P + 255) / 256, nit);
     if (mixed)
-        hipLaunchKernelGGL((order_one<float, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
+        hipLaunchKernelGGL((order_one<__hip_bfloat16, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
     else
         hipLaunchKernelGGL((order_one<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
     const unsigned ntiles = (unsigned)c.htiles.size();
@@ -485,12 +485,12 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
         hipEvent_t *eg = p->profile ? ev_pair(p, 0) : nullptr;
         hipEvent_t *es = p->profile ? ev_pair(p, 1) : nullptr;
         if (mixed) {
-            hipLaunchKernelGGL(order_status<float>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
+            hipLaunchKernelGGL(order_status<__hip_bfloat16>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
             if (eg) (void)hipEventRecord(eg[0], s);
             hipLaunchKernelGGL(gemm_bf16, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st);
             if (eg) (void)hipEventRecord(eg[1], s);
             if (es) (void)hipEventRecord(es[0], s);
-            hipLaunchKernelGGL((sweep<float, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
+            hipLaunchKernelGGL((sweep<__hip_bfloat16, float, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
             if (es) (void)hipEventRecord(es[1], s);
         } else {
             hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
@@ -498,13 +498,13 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             hipLaunchKernelGGL(gemm_f64, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st);
             if (eg) (void)hipEventRecord(eg[1], s);
             if (es) (void)hipEventRecord(es[0], s);
-            hipLaunchKernelGGL((sweep<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
+            hipLaunchKernelGGL((sweep<double, double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
             if (es) (void)hipEventRecord(es[1], s);
         }
         p->gemm_launches += 1.0;
     }
     if (mixed)
-        hipLaunchKernelGGL(order_status<float>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, c.maxit + 1);
+        hipLaunchKernelGGL(order_status<__hip_bfloat16>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, c.maxit + 1);
     else
         hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, c.maxit + 1);
     hipLaunchKernelGGL(extract, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, io, mixed ? 1 : 0);

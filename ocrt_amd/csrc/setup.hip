@@ -370,6 +370,9 @@ __global__ void pack_operator(const DevInst *__restrict__ inst, const DevItem *_
     const int n = it.n, dirs = 2 * n + 1, six_n = 6 * n;
     AT *A = reinterpret_cast<AT *>(P.A) + it.off_A;
     const int part = c / it.P, cc = c % it.P;
+    // fp64: A[Mp][K], parts side by side along K.  bf16: [part][hi, lo][Mp][P].
+    const long long dst = (sizeof(AT) == 2) ? ((long long)(part * 2) * it.Mp + r) * it.P + cc : idx;
+    const long long lo_off = (long long)it.Mp * it.P;
     double v = 0.0;
     if (r < six_n && cc < six_n) {
         const int to = r / n, o = r % n + 1;
@@ -393,12 +396,12 @@ __global__ void pack_operator(const DevInst *__restrict__ inst, const DevItem *_
             v = w * (double)s * e;
         }
     }
-    A[idx] = to_storage<AT>(v);
+    A[dst] = to_storage<AT>(v);
     if (sizeof(AT) == 2) {
         // bf16 operator carried as hi + lo: the rounding of a fixed operator is
         // systematic across orders, the residual term removes it to ~2^-16
-        const double hi = (double)__bfloat162float(*reinterpret_cast<const __hip_bfloat16 *>(&A[idx]));
-        A[total + idx] = to_storage<AT>(v - hi);
+        const double hi = (double)__bfloat162float(*reinterpret_cast<const __hip_bfloat16 *>(&A[dst]));
+        A[dst + lo_off] = to_storage<AT>(v - hi);
     }
 }
 

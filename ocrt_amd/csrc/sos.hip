@@ -95,12 +95,12 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     if (col >= Pp) return;
     FT *F = reinterpret_cast<FT *>(P.F) + it.off_F;
     double *tot = P.tot + it.off_tot;
-    double *comp = MIXED ? P.comp + it.off_tot : nullptr;
+    float *comp = MIXED ? P.comp + it.off_tot : nullptr;
     if (col >= 6 * n) {
         for (int k = 0; k <= nt; ++k) {
-            F[(long long)k * Pp + col] = (FT)0;
+            F[(long long)k * Pp + col] = to_storage<FT>(0.0);
             tot[(long long)k * Pp + col] = 0.0;
-            if (MIXED) comp[(long long)k * Pp + col] = 0.0;
+            if (MIXED) comp[(long long)k * Pp + col] = 0.f;
         }
         return;
     }
@@ -181,9 +181,10 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     }
     for (int k = 0; k <= nt; ++k) {
         const long long ix = (long long)k * Pp + col;
-        F[ix] = (FT)tot[ix];
-        if (MIXED) comp[ix] = 0.0;
+        F[ix] = to_storage<FT>(tot[ix]);
+        if (MIXED) comp[ix] = 0.f;
     }
+    if (hemi == 1) P.bc[it.off_bc + 3 * n + c * n + o] = tot[(long long)nt * Pp + col];
 }
 
 // ---------------------------------------------------------------------------
@@ -217,7 +218,7 @@ __global__ void order_status(const DevInst *__restrict__ inst, const DevItem *__
     __syncthreads();
     if (!act || I.surf == 0) return;
     const int n = it.n, nt = it.nt, Pp = it.P, m = it.m;
-    const FT *F = reinterpret_cast<const FT *>(P.F) + it.off_F + (long long)nt * Pp;
+    const double *bot = P.bc + it.off_bc + 3 * n;   // [3][n] downward field at BOA
     const double *mu = P.ring + I.off_ring, *w = mu + n;
     double *bc = P.bc + it.off_bc;
     const double W = (I.surf == 1) ? 2.0 * OCRT_PI : ((m == 0) ? 2.0 * OCRT_PI : OCRT_PI);
@@ -226,7 +227,7 @@ __global__ void order_status(const DevInst *__restrict__ inst, const DevItem *__
         if (I.surf == 1) {
             double Fr[9];
             fresnel_R(mu[o], I.n_water, I.q_conv, Fr);
-            const double a = (double)F[n + o], b = (double)F[3 * n + o], cc = (double)F[5 * n + o];
+            const double a = bot[o], b = bot[n + o], cc = bot[2 * n + o];
             if (w[o] == 0.0) {
                 si = Fr[0] * a + Fr[1] * b + Fr[2] * cc;
                 sq = Fr[3] * a + Fr[4] * b + Fr[5] * cc;
@@ -244,7 +245,7 @@ __global__ void order_status(const DevInst *__restrict__ inst, const DevItem *__
             const double *R0 = P.R + I.off_R + ((long long)m * n + o) * n * 9;
             for (int i = 0; i < n; ++i) {
                 if (w[i] == 0.0) continue;
-                const double a = (double)F[n + i], b = (double)F[3 * n + i], cc = (double)F[5 * n + i];
+                const double a = bot[i], b = bot[n + i], cc = bot[2 * n + i];
                 const double pw = W * mu[i] * w[i];
                 const double *R = R0 + (long long)i * 9;
                 si += pw * (R[0] * a + R[1] * b + R[2] * cc);
@@ -324,10 +325,12 @@ __global__ __launch_bounds__(256) void gemm_f64(const DevInst *__restrict__ inst
 }
 
 // Scattering integral on bf16 MFMA (v_mfma_f32_16x16x16bf16_1k) with fp32
-// accumulation.  Tile 64 x 64, K step 32.  The operator is stored as a bf16
-// pair A = A_hi + A_lo (rows Mp..2Mp-1 hold A_lo) and both halves multiply the
-// same bf16 field fragment; the field is mixed by the level scattering
-// fractions in fp32 and rounded to bf16 once, on its way into LDS.
+// accumulation.  Tile 64 (outgoing columns) x 64 (levels), K step 32, four
+// waves of 32 x 32.  The operator of each part (particle, molecular) is
+// stored as a bf16 pair A = A_hi + A_lo, [part][hi, lo][Mp][P]; both halves
+// multiply the same bf16 field fragment.  The level scattering fractions are
+// applied to the fp32 accumulators in the epilogue:
+//   J[lev][r] = xdel(lev) (A_p F)[r][lev] + ydel(lev) (A_r F)[r][lev].
 __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ inst,
                                                  const DevItem *__restrict__ items, Pools P,
                                                  const GemmTile *__restrict__ tiles,
@@ -337,78 +340,71 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
     if (!st[tl.item].active) return;
     const DevItem &it = items[tl.item];
     const DevInst &I = inst[it.inst];
-    const int K = it.K, Pp = it.P, L = it.L;
+    const int Pp = it.P, L = it.L, Mp = it.Mp;
+    const int nparts = it.K / Pp;
     const unsigned short *A = reinterpret_cast<const unsigned short *>(P.A) + it.off_A;
-    const float *F = reinterpret_cast<const float *>(P.F) + it.off_F;
+    const unsigned short *F = reinterpret_cast<const unsigned short *>(P.F) + it.off_F;
     float *J = reinterpret_cast<float *>(P.J) + it.off_F;
     const LevelView lv(P, I);
-    __shared__ unsigned short As[64][40];
-    __shared__ unsigned short Al[64][40];
+    __shared__ unsigned short As[2][2][64][40];
     __shared__ unsigned short Bs[64][40];
-    const unsigned short *Alo = A + (long long)it.Mp * K;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int row0 = tl.tm * 64, lev0 = tl.tn * 64;
-    float4_ acc[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) acc[a][b] = (float4_){0.f, 0.f, 0.f, 0.f};
+    float4_ acc[2][2][2];
+    for (int q = 0; q < 2; ++q)
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) acc[q][a][b] = (float4_){0.f, 0.f, 0.f, 0.f};
     const int lr = t >> 2, lk = (t & 3) * 8;
     const int lev_load = lev0 + lr;
-    const float sx = lev_load < L ? (float)lv.xd[lev_load] : 0.f;
-    const float sy = lev_load < L ? (float)lv.yd[lev_load] : 0.f;
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        const uint4 av = *reinterpret_cast<const uint4 *>(A + (long long)(row0 + lr) * K + k0 + lk);
-        *reinterpret_cast<uint4 *>(&As[lr][lk]) = av;
-        const uint4 avl = *reinterpret_cast<const uint4 *>(Alo + (long long)(row0 + lr) * K + k0 + lk);
-        *reinterpret_cast<uint4 *>(&Al[lr][lk]) = avl;
-        unsigned short bb[8];
-        if (lev_load < L) {
-            const int part = (k0 + lk) / Pp;
-            const int kind = (it.parts >> (8 * part)) & 0xff;
-            const float s = (kind == 1) ? sx : sy;
-            const float *fs = F + (long long)lev_load * Pp + (k0 + lk) % Pp;
-            const float4 f0 = *reinterpret_cast<const float4 *>(fs);
-            const float4 f1 = *reinterpret_cast<const float4 *>(fs + 4);
-            const float fv[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
-            for (int q = 0; q < 8; ++q) {
-                const __hip_bfloat16 hb = __float2bfloat16(s * fv[q]);
-                bb[q] = *reinterpret_cast<const unsigned short *>(&hb);
+    const long long slab = (long long)Mp * Pp;
+    for (int k0 = 0; k0 < Pp; k0 += 32) {
+        for (int q = 0; q < nparts; ++q)
+            for (int hl = 0; hl < 2; ++hl) {
+                const unsigned short *src = A + (long long)(q * 2 + hl) * slab + (long long)(row0 + lr) * Pp + k0 + lk;
+                *reinterpret_cast<uint4 *>(&As[q][hl][lr][lk]) = *reinterpret_cast<const uint4 *>(src);
             }
-        } else {
-            for (int q = 0; q < 8; ++q) bb[q] = 0;
-        }
-        uint4 bv;
-        bv.x = bb[0] | ((unsigned)bb[1] << 16); bv.y = bb[2] | ((unsigned)bb[3] << 16);
-        bv.z = bb[4] | ((unsigned)bb[5] << 16); bv.w = bb[6] | ((unsigned)bb[7] << 16);
+        uint4 bv = {0u, 0u, 0u, 0u};
+        if (lev_load < L) bv = *reinterpret_cast<const uint4 *>(F + (long long)lev_load * Pp + k0 + lk);
         *reinterpret_cast<uint4 *>(&Bs[lr][lk]) = bv;
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < 32; ks += 16) {
-            short4_ a4[2], l4[2], b4[2];
-            for (int a = 0; a < 2; ++a) {
-                a4[a] = *reinterpret_cast<const short4_ *>(&As[wm + a * 16 + (lane & 15)][ks + 4 * (lane >> 4)]);
-                l4[a] = *reinterpret_cast<const short4_ *>(&Al[wm + a * 16 + (lane & 15)][ks + 4 * (lane >> 4)]);
-            }
+            short4_ b4[2];
             for (int b = 0; b < 2; ++b)
                 b4[b] = *reinterpret_cast<const short4_ *>(&Bs[wn + b * 16 + (lane & 15)][ks + 4 * (lane >> 4)]);
-            for (int a = 0; a < 2; ++a)
-                for (int b = 0; b < 2; ++b) {
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(l4[a], b4[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a4[a], b4[b], acc[a][b], 0, 0, 0);
+            for (int q = 0; q < nparts; ++q)
+                for (int a = 0; a < 2; ++a) {
+                    const int row = wm + a * 16 + (lane & 15);
+                    const short4_ hi = *reinterpret_cast<const short4_ *>(&As[q][0][row][ks + 4 * (lane >> 4)]);
+                    const short4_ lo = *reinterpret_cast<const short4_ *>(&As[q][1][row][ks + 4 * (lane >> 4)]);
+                    for (int b = 0; b < 2; ++b) {
+                        acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lo, b4[b], acc[q][a][b], 0, 0, 0);
+                        acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(hi, b4[b], acc[q][a][b], 0, 0, 0);
+                    }
                 }
         }
         __syncthreads();
     }
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) {
-            const int lev = lev0 + wn + b * 16 + (lane & 15);
+    const int kind0 = it.parts & 0xff;
+    for (int b = 0; b < 2; ++b) {
+        const int lev = lev0 + wn + b * 16 + (lane & 15);
+        if (lev >= L) continue;
+        const float x = (float)lv.xd[lev], y = (float)lv.yd[lev];
+        const float s0 = (kind0 == 1) ? x : y;
+        for (int a = 0; a < 2; ++a) {
             const int r = row0 + wm + a * 16 + 4 * (lane >> 4);
-            if (lev < L && r < Pp) {
-                float4 v;
-                v.x = acc[a][b][0]; v.y = acc[a][b][1]; v.z = acc[a][b][2]; v.w = acc[a][b][3];
-                *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v;
+            if (r >= Pp) continue;
+            float4 v;
+            v.x = s0 * acc[0][a][b][0]; v.y = s0 * acc[0][a][b][1];
+            v.z = s0 * acc[0][a][b][2]; v.w = s0 * acc[0][a][b][3];
+            if (nparts > 1) {
+                v.x += y * acc[1][a][b][0]; v.y += y * acc[1][a][b][1];
+                v.z += y * acc[1][a][b][2]; v.w += y * acc[1][a][b][3];
             }
+            *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v;
         }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -428,9 +424,15 @@ __device__ inline double block_max(double v, double *red)
 }
 
 // Formal solution of the current order for every column, fused with the order
-// sum total += cur (plain in fp64, compensated in the mixed form) and the two
-// max-norms of the convergence test (rt_solver.c:1855-1868, 3093-3124).
-template <typename FT, bool MIXED>
+// sum total += cur (plain in fp64, Neumaier-compensated in the mixed form with
+// an fp32 compensation term) and the two max-norms of the convergence test
+// (rt_solver.c:1855-1868, 3093-3124).  One thread per column; the levels are
+// walked in blocks of kU: every load of a block is issued before its
+// dependent recurrence steps run, so the sequential scan overlaps its memory
+// latency instead of exposing it once per level.
+constexpr int kU = 8;
+
+template <typename FT, typename JT, bool MIXED>
 __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                                              const DevItem *__restrict__ items, Pools P,
                                              ItemState *__restrict__ st)
@@ -446,64 +448,101 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
     if (col < 6 * n) {
         const LevelView lv(P, I);
         const int type = col / n, c = type >> 1, hemi = type & 1, o = col % n;
-        const FT *J = reinterpret_cast<const FT *>(P.J) + it.off_F;
-        FT *F = reinterpret_cast<FT *>(P.F) + it.off_F;
-        double *tot = P.tot + it.off_tot;
-        double *cmp = MIXED ? P.comp + it.off_tot : nullptr;
+        const JT *J = reinterpret_cast<const JT *>(P.J) + it.off_F + col;
+        FT *F = reinterpret_cast<FT *>(P.F) + it.off_F + col;
+        double *tot = P.tot + it.off_tot + col;
+        float *cmp = MIXED ? P.comp + it.off_tot + col : nullptr;
         const double mu = P.ring[I.off_ring + o];
         const double bc = (I.surf != 0 && hemi == 0) ? P.bc[it.off_bc + c * n + o] : 0.0;
-        auto emit = [&](int k, double v) {
-            const long long ix = (long long)k * Pp + col;
-            F[ix] = (FT)v;
+        // level order of the scan and the layer crossed when stepping to level k
+        const int k_first = hemi ? 0 : nt;
+        const int step = hemi ? 1 : -1;
+        auto layer_of = [&](int k) { return hemi ? k - 1 : k; };
+        auto emit = [&](int k, double v, double t_old, float c_old) {
+            const long long ix = (long long)k * Pp;
+            F[ix] = to_storage<FT>(v);
             mcur = fmax(mcur, fabs(v));
+            if (hemi == 1 && k == nt) P.bc[it.off_bc + 3 * n + c * n + o] = v;
             if (MIXED) {
-                const double s = tot[ix], cc = cmp[ix];
-                const double tsum = s + v;
-                const double nc = cc + ((fabs(s) >= fabs(v)) ? ((s - tsum) + v) : ((v - tsum) + s));
+                const double tsum = t_old + v;
+                const double nc = (double)c_old + ((fabs(t_old) >= fabs(v)) ? ((t_old - tsum) + v) : ((v - tsum) + t_old));
                 tot[ix] = tsum;
-                cmp[ix] = nc;
+                cmp[ix] = (float)nc;
                 mtot = fmax(mtot, fabs(tsum + nc));
             } else {
-                const double tsum = tot[ix] + v;
+                const double tsum = t_old + v;
                 tot[ix] = tsum;
                 mtot = fmax(mtot, fabs(tsum));
             }
         };
+        {
+            const long long ix = (long long)k_first * Pp;
+            emit(k_first, hemi ? 0.0 : bc, tot[ix], MIXED ? cmp[ix] : 0.f);
+        }
         if (!MIXED) {
             const double *T = P.T + I.off_T;
-            sweep_column_f64(hemi, nt, mu, bc, lv.h, T, n, o,
-                             [&](int k) { return (double)J[(long long)k * Pp + col]; }, emit);
+            double run = hemi ? 0.0 : bc;
+            double s_in = (double)J[(long long)k_first * Pp];
+            const double smu = hemi ? -mu : mu;
+            for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * kU) {
+                double sj[kU], tr[kU], to[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int k = k0 + step * u;
+                    if (k >= 0 && k <= nt) {
+                        sj[u] = (double)J[(long long)k * Pp];
+                        tr[u] = T[(long long)layer_of(k) * n + o];
+                        to[u] = tot[(long long)k * Pp];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int k = k0 + step * u;
+                    if (k < 0 || k > nt) break;
+                    const int ku = k - step;                // upstream level
+                    const double s_out = sj[u];
+                    const double dt = hemi ? lv.h[k] - lv.h[ku] : lv.h[ku] - lv.h[k];
+                    const double a = hemi ? (s_out - s_in) / dt : (s_in - s_out) / dt;
+                    const double b = s_out - a * lv.h[k];
+                    run = tr[u] * run + ((1.0 - tr[u]) * (b + a * smu) + a * (lv.h[k] - lv.h[ku] * tr[u])) * 0.5;
+                    emit(k, run, to[u], 0.f);
+                    s_in = s_out;
+                }
+            }
         } else {
             // fp32 scan written with om = 1 - T (accurate from expm1):
             //   run <- run - om run + (S_out - S_in + om (S_in + slope mu_signed)) / 2
             // which is the linear-source recurrence with no 1 - T cancellation
             const float *OM = P.Tf + I.off_T;
             const float smu = hemi ? -(float)mu : (float)mu;
-            if (hemi == 0) {
-                float run = (float)bc;
-                emit(nt, (double)run);
-                float s_in = (float)J[(long long)nt * Pp + col];
-                for (int k = nt - 1; k >= 0; --k) {
-                    const float s_out = (float)J[(long long)k * Pp + col];
-                    const float idt = (float)(1.0 / (lv.h[k + 1] - lv.h[k]));
-                    const float om = OM[(long long)k * n + o];
-                    const float a = (s_in - s_out) * idt;
-                    run = run - om * run + 0.5f * ((s_out - s_in) + om * (s_in + a * smu));
-                    emit(k, (double)run);
-                    s_in = s_out;
+            float run = hemi ? 0.f : (float)bc;
+            float s_in = (float)J[(long long)k_first * Pp];
+            for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * kU) {
+                float sj[kU], om[kU], cm[kU];
+                double to[kU], hk[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int k = k0 + step * u;
+                    if (k >= 0 && k <= nt) {
+                        sj[u] = (float)J[(long long)k * Pp];
+                        om[u] = OM[(long long)layer_of(k) * n + o];
+                        to[u] = tot[(long long)k * Pp];
+                        cm[u] = cmp[(long long)k * Pp];
+                        hk[u] = lv.h[k];
+                    }
                 }
-            } else {
-                float run = 0.f;
-                emit(0, 0.0);
-                float s_in = (float)J[col];
-                for (int k = 1; k <= nt; ++k) {
-                    const float s_out = (float)J[(long long)k * Pp + col];
-                    const float idt = (float)(1.0 / (lv.h[k] - lv.h[k - 1]));
-                    const float om = OM[(long long)(k - 1) * n + o];
-                    const float a = (s_out - s_in) * idt;
-                    run = run - om * run + 0.5f * ((s_out - s_in) + om * (s_in + a * smu));
-                    emit(k, (double)run);
+                double h_up = lv.h[k0 - step];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int k = k0 + step * u;
+                    if (k < 0 || k > nt) break;
+                    const float idt = (float)(1.0 / fabs(hk[u] - h_up));
+                    const float s_out = sj[u];
+                    const float a = hemi ? (s_out - s_in) * idt : (s_in - s_out) * idt;
+                    run = run - om[u] * run + 0.5f * ((s_out - s_in) + om[u] * (s_in + a * smu));
+                    emit(k, (double)run, to[u], cm[u]);
                     s_in = s_out;
+                    h_up = hk[u];
                 }
             }
         }
@@ -520,7 +559,7 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
 __device__ inline double total_at(const Pools &P, const DevItem &it, bool mixed, long long ix)
 {
     const double *tot = P.tot + it.off_tot;
-    return mixed ? tot[ix] + P.comp[it.off_tot + ix] : tot[ix];
+    return mixed ? tot[ix] + (double)P.comp[it.off_tot + ix] : tot[ix];
 }
 
 // Per-item outputs after the order loop (rt_solver.c:1010-1061).
@@ -682,10 +721,10 @@ __global__ void finalize(const DevInst *__restrict__ inst, int n_inst, const Ite
 
 // explicit instantiations
 template __global__ void order_one<double, false>(const DevInst *, const DevItem *, Pools, ItemState *);
-template __global__ void order_one<float, true>(const DevInst *, const DevItem *, Pools, ItemState *);
+template __global__ void order_one<__hip_bfloat16, true>(const DevInst *, const DevItem *, Pools, ItemState *);
 template __global__ void order_status<double>(const DevInst *, const DevItem *, Pools, ItemState *, int);
-template __global__ void order_status<float>(const DevInst *, const DevItem *, Pools, ItemState *, int);
-template __global__ void sweep<double, false>(const DevInst *, const DevItem *, Pools, ItemState *);
-template __global__ void sweep<float, true>(const DevInst *, const DevItem *, Pools, ItemState *);
+template __global__ void order_status<__hip_bfloat16>(const DevInst *, const DevItem *, Pools, ItemState *, int);
+template __global__ void sweep<double, double, false>(const DevInst *, const DevItem *, Pools, ItemState *);
+template __global__ void sweep<__hip_bfloat16, float, true>(const DevInst *, const DevItem *, Pools, ItemState *);
 
 }  // namespace ocrt
