@@ -88,6 +88,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="time only the mixed form (for counter-collection runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (3 times): the reference "
+                         "for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (instances are independent; a "
@@ -99,6 +104,15 @@ def main():
     from ocrt_amd import synthetic as S
 
     sync = torch.cuda.synchronize
+    if args.copy_calibration > 0:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        sync()
+        log(f"copy calibration: 3 copies of {nbytes} bytes (read {nbytes}, write {nbytes} each)")
+        del src, dst
     log("building workload")
     descs, keep = S.config2(n_geom=args.geoms)
     n = len(descs)
@@ -112,6 +126,11 @@ def main():
     work = pm.work()
     log(f"mixed: {ms_mixed:.1f} ms/step, kernels {kt}")
 
+    if args.mixed_only:
+        print(json.dumps({"mixed_ms_per_step": ms_mixed, "instances": n, "kernels": kt}))
+        pm.close()
+        return
+
     log("fp64 batched plan")
     pf = ocrt_amd.Plan(descs, precision="fp64")
     ms_fp64 = timed_steps(pf, 1, 1, sync) * 1e3
@@ -120,9 +139,12 @@ def main():
     log(f"fp64 batched: {ms_fp64:.1f} ms/step")
 
     ok = (res_m["status"] == 0) & (res_f["status"] == 0)
-    err_I = float(np.max(np.abs(res_m["rho_I"] - res_f["rho_I"])[ok] / np.abs(res_f["rho_I"])[ok]))
-    err_QU = float(max(np.max(np.abs(res_m["rho_Q"] - res_f["rho_Q"])[ok]),
-                       np.max(np.abs(res_m["rho_U"] - res_f["rho_U"])[ok])))
+    rel_I = np.abs(res_m["rho_I"] - res_f["rho_I"]) / np.abs(res_f["rho_I"])
+    abs_QU = np.maximum(np.abs(res_m["rho_Q"] - res_f["rho_Q"]), np.abs(res_m["rho_U"] - res_f["rho_U"]))
+    err_I = float(np.max(rel_I[ok]))
+    err_QU = float(np.max(abs_QU[ok]))
+    worst = int(np.argmax(np.where(ok, rel_I, 0.0)))
+    log(f"mixed vs fp64: max rel I {err_I:.3e} (instance {worst}), max abs Q/U {err_QU:.3e}")
 
     log(f"fp64 one instance per launch on {args.plain_sample} instances")
     pp = ocrt_amd.Plan(descs[: args.plain_sample], precision="fp64", one_per_launch=True)
@@ -187,6 +209,7 @@ def main():
         "mixed_over_fp64_batched": ms_fp64 / ms_mixed,
         "max_rel_err_I_vs_fp64": err_I,
         "max_abs_err_QU_vs_fp64": err_QU,
+        "worst_instance_I": worst,
         "adjoint_fwd_ratio": None,
         "single_instance_config1": single,
         "roofline": roofline,

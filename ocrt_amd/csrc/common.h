@@ -44,12 +44,16 @@ struct DevInst {
 struct DevItem {
     int inst, m, n, nt, L, P, K, parts;   // parts: bit0 particle, bit1 molecular
     int Mp, tiles_m, tiles_n, lmax;
+    int fsplit;           // mixed form: field stored as a bf16 pair hi + lo (low Fourier modes)
+    int pad_;
     long long off_basis;  // pool_basis: 3 x [(lmax+1)][2n+1]
     long long off_tab;    // pool_tab: 6 x [(n+1)][2n+1]
     long long off_A;      // pool_A: [Mp][K]
     long long off_F;      // pool_F / pool_J: [L][P]
-    long long off_tot;    // pool_tot / pool_comp: [L][P]
+    long long off_tot;    // pool_tot: [L][P]
     long long off_bc;     // pool_bc: [3][n] boundary condition, then [3][n] bottom downward field
+    long long off_cmp;    // pool_comp: [2][P] compensation of the order sum at k = 0 and k = nt
+    long long off_Flo;    // pool_Flo: [L][P] low half of the bf16 field pair (fsplit items)
 };
 
 struct ItemState {
@@ -73,15 +77,22 @@ struct InstOut {
 
 struct Pools {
     double *lev, *ring, *T, *greek, *R, *basis, *tab, *tot, *bc;
-    float *comp;          // fp32 compensation of the mixed-form order sum
+    float *comp;          // fp32 compensation of the mixed-form order sum on the two output rows
     float *Tf;            // fp32 1 - T for the mixed-form sweeps
+    float *idt;           // fp32 1 / dtau per layer, indexed like lev (mixed-form sweeps)
     void *A, *F, *J;      // double or bf16/float depending on the form
+    void *Flo;            // bf16 low half of the field pair (mixed form, fsplit items)
     double *cm_w;         // cos/sin(m phi) table [(Mall+1)][2][kNPhi]
     double *toa_up;       // optional [items][3][n] TOA upward at every node (LUT)
     int toa_up_stride;
 };
 
-struct GemmTile { int item, tm, tn, pad; };
+struct GemmTile { int item, tm, tn, pad; };   // item < 0: idle slot
+
+// 64 consecutive state columns of one item: the unit of the per-column kernels
+// (one wavefront each).  item < 0: idle slot.
+struct ColTile { int item, col0; };
+constexpr int kColTile = 64;
 
 // One IPSS kappa evaluation: instance and viewing direction.
 struct IpssJob { int inst; int pad; double vza, raa; };

@@ -23,11 +23,13 @@ __global__ void coxmunk_fourier(const DevInst *, Pools);
 __global__ void angular_basis(const DevInst *, const DevItem *, Pools);
 __global__ void phase_tables(const DevInst *, const DevItem *, Pools);
 template <typename AT> __global__ void pack_operator(const DevInst *, const DevItem *, Pools);
-template <typename FT, bool MIXED> __global__ void order_one(const DevInst *, const DevItem *, Pools, ItemState *);
+template <typename FT, bool MIXED>
+__global__ void order_one(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int);
 template <typename FT> __global__ void order_status(const DevInst *, const DevItem *, Pools, ItemState *, int);
 __global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *);
 __global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *);
-template <typename FT, typename JT, bool MIXED> __global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *);
+template <typename FT, typename JT, bool MIXED>
+__global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int);
 __global__ void extract(const DevInst *, const DevItem *, Pools, const ItemState *, ItemOut *, int);
 __global__ void finalize(const DevInst *, int, const ItemOut *, const InstOut *, const Pools, ocrt_result_t *);
 __global__ void ipss_kappa(const DevInst *, const IpssJob *, Pools, double *, double *);
@@ -108,16 +110,55 @@ struct Chunk {
     std::vector<DevInst> hinst;
     std::vector<DevItem> hitem;
     std::vector<GemmTile> htiles;
+    std::vector<ColTile> hcols;
     std::vector<IpssJob> hjobs;
     DevInst *dinst = nullptr;
     DevItem *ditem = nullptr;
     GemmTile *dtiles = nullptr;
+    ColTile *dcols = nullptr;
     IpssJob *djobs = nullptr;
     int *djob_of = nullptr;
     int max_L = 0, max_P = 0, max_pairs = 0, max_tab = 0, maxit = 1, any_rough = 0, n_jobs = 0;
     long long max_pack = 0;
-    long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0;
+    long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0, cmp = 0, flo = 0;
 };
+
+// Mixed form: Fourier modes m < kSplitModes keep their field as a bf16 pair
+// (about 16 significant bits) and pay a third MFMA (A_hi F_lo).  These modes
+// carry most of the radiance; a single bf16 field there leaves errors near
+// 1e-4 in I after 20 orders of a thick aerosol layer, while at m >= 3 the
+// same rounding contributes ~1e-6.
+constexpr int kSplitModes = 3;
+
+// Workgroups of one launch are dealt round-robin over the 8 XCDs (block b runs
+// on XCD b mod 8), each with its own L2.  The GEMM tiles of one item share
+// operator rows and field columns, so every item's tiles are kept on one XCD:
+// items go to the least-loaded of 8 groups, and slot 8j + x of the launch
+// takes the j-th tile of group x (idle slots pad the shorter groups).
+constexpr int kXcds = 8;
+
+std::vector<GemmTile> xcd_tile_order(const std::vector<std::vector<GemmTile>> &per_item)
+{
+    std::vector<std::vector<GemmTile>> grp(kXcds);
+    std::vector<size_t> load(kXcds, 0);
+    std::vector<int> order(per_item.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    // largest items first keeps the groups balanced
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return per_item[a].size() > per_item[b].size(); });
+    for (int i : order) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        grp[g].insert(grp[g].end(), per_item[i].begin(), per_item[i].end());
+        load[g] += per_item[i].size();
+    }
+    const size_t rows = *std::max_element(load.begin(), load.end());
+    std::vector<GemmTile> out;
+    out.reserve(rows * kXcds);
+    for (size_t j = 0; j < rows; ++j)
+        for (int x = 0; x < kXcds; ++x)
+            out.push_back(j < grp[x].size() ? grp[x][j] : GemmTile{-1, 0, 0, 0});
+    return out;
+}
 
 #define HIPCHK(x)                                                                         \
     do {                                                                                  \
@@ -279,7 +320,8 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
                 const int parts = (I.aer ? 1 : 0) + ((m <= 2 && I.ray) ? 1 : 0);
                 const int K = std::max(parts, 1) * P;
                 b += 3.0 * (I.lmax + 1) * (2 * n + 1) * 8 + 6.0 * (n + 1) * (2 * n + 1) * 8 +
-                     (double)Mp * K * ae * (mixed ? 2 : 1) + (double)L * P * (fe + je + (mixed ? 12 : 8)) + 6.0 * n * 8;
+                     (double)Mp * K * ae * (mixed ? 2 : 1) + (double)L * P * (fe + je + 8) +
+                     (mixed ? 8.0 * P : 0.0) + ((mixed && m < kSplitModes) ? 2.0 * L * P : 0.0) + 6.0 * n * 8;
             }
             const int items_after = (item_global - c.it0) + I.mmax + 1;
             if (i > c.i0 && (bytes + b > budget || items_after > 60000 || i - c.i0 >= 60000 || p->per_inst)) break;
@@ -295,7 +337,8 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     p->n_items = item_global;
     // ---- per-chunk layout
     for (Chunk &c : p->chunks) {
-        long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0;
+        long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0, cmp = 0, flo = 0;
+        std::vector<std::vector<GemmTile>> gemm_of_item;
         for (int ii = c.i0; ii < c.i1; ++ii) {
             HostInst &h = p->inst[ii];
             DevInst I = h.d;
@@ -349,24 +392,35 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
                 it.off_A = A; A += (long long)Mp * it.K * (mixed ? 2 : 1);
                 it.off_F = F; it.off_tot = F; F += (long long)L * Pp;
                 it.off_bc = bc; bc += 6LL * n;
+                it.off_cmp = cmp; cmp += mixed ? 2LL * Pp : 0;
+                it.fsplit = (mixed && m < kSplitModes) ? 1 : 0;
+                it.off_Flo = flo; flo += it.fsplit ? (long long)L * Pp : 0;
                 c.max_pack = std::max(c.max_pack, (long long)Mp * it.K);
                 const int item_local = (int)c.hitem.size();
+                std::vector<GemmTile> gt;
                 for (int tm = 0; tm < it.tiles_m; ++tm)
-                    for (int tn = 0; tn < it.tiles_n; ++tn) c.htiles.push_back(GemmTile{item_local, tm, tn, 0});
+                    for (int tn = 0; tn < it.tiles_n; ++tn) gt.push_back(GemmTile{item_local, tm, tn, 0});
+                gemm_of_item.push_back(std::move(gt));
+                for (int c0 = 0; c0 < Pp; c0 += kColTile) c.hcols.push_back(ColTile{item_local, c0});
                 c.hitem.push_back(it);
             }
             if (I.ipss) c.hjobs.push_back(IpssJob{ii - c.i0, 0, I.vza, I.raa});
             c.hinst.push_back(I);
         }
+        c.htiles = xcd_tile_order(gemm_of_item);
         c.n_jobs = (int)c.hjobs.size();
         c.lev = lev; c.T = T; c.R = R; c.basis = basis; c.tab = tab; c.A = A; c.F = F; c.bc = bc;
+        c.cmp = cmp; c.flo = flo;
     }
     // ---- device allocations
-    long long mx_lev = 0, mx_T = 0, mx_R = 0, mx_basis = 0, mx_tab = 0, mx_A = 0, mx_F = 0, mx_bc = 0;
+    long long mx_lev = 0, mx_T = 0, mx_R = 0, mx_basis = 0, mx_tab = 0, mx_A = 0, mx_F = 0, mx_bc = 0,
+              mx_cmp = 0, mx_flo = 0;
     for (const Chunk &c : p->chunks) {
         mx_lev = std::max(mx_lev, c.lev); mx_T = std::max(mx_T, c.T); mx_R = std::max(mx_R, c.R);
         mx_basis = std::max(mx_basis, c.basis); mx_tab = std::max(mx_tab, c.tab);
         mx_A = std::max(mx_A, c.A); mx_F = std::max(mx_F, c.F); mx_bc = std::max(mx_bc, c.bc);
+        mx_cmp = std::max(mx_cmp, c.cmp);
+        mx_flo = std::max(mx_flo, c.flo);
     }
     if (getenv("OCRT_DEBUG_PLAN"))
         fprintf(stderr, "ocrt plan: %zu chunks items %d ring %lld greek %lld lev %lld T %lld R %lld basis %lld tab %lld "
@@ -386,6 +440,7 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     rc |= dmalloc(p, (void **)&P.lev, mx_lev * 8);
     rc |= dmalloc(p, (void **)&P.T, mx_T * 8);
     rc |= dmalloc(p, (void **)&P.Tf, mx_T * 4);
+    rc |= dmalloc(p, (void **)&P.idt, mx_lev * 4);
     rc |= dmalloc(p, (void **)&P.R, mx_R * 8);
     rc |= dmalloc(p, (void **)&P.basis, mx_basis * 8);
     rc |= dmalloc(p, (void **)&P.tab, mx_tab * 8);
@@ -393,7 +448,8 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     rc |= dmalloc(p, (void **)&P.F, mx_F * fe);
     rc |= dmalloc(p, (void **)&P.J, mx_F * je);
     rc |= dmalloc(p, (void **)&P.tot, mx_F * 8);
-    if (mixed) rc |= dmalloc(p, (void **)&P.comp, mx_F * 4);
+    if (mixed) rc |= dmalloc(p, (void **)&P.comp, std::max(mx_cmp, 1LL) * 4);
+    if (mixed) rc |= dmalloc(p, (void **)&P.Flo, std::max(mx_flo, 1LL) * 2);
     rc |= dmalloc(p, (void **)&P.bc, mx_bc * 8);
     if (rc) { ocrt_plan_destroy(p); return OCRT_ERR_ALLOC; }
     P.ring = p->ring;
@@ -422,6 +478,7 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     for (const Chunk &c : p->chunks)
         desc_bytes += al(c.hinst.size() * sizeof(DevInst)) + al(c.hitem.size() * sizeof(DevItem)) +
                       al(std::max<size_t>(c.htiles.size(), 1) * sizeof(GemmTile)) +
+                      al(std::max<size_t>(c.hcols.size(), 1) * sizeof(ColTile)) +
                       al(std::max<size_t>(c.hjobs.size(), 1) * sizeof(IpssJob)) +
                       al((size_t)(c.i1 - c.i0) * sizeof(int));
     std::vector<unsigned char> hdesc(desc_bytes, 0);
@@ -441,6 +498,8 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
         c.ditem = (DevItem *)put(c.hitem.data(), c.hitem.size() * sizeof(DevItem), c.hitem.size() * sizeof(DevItem));
         c.dtiles = (GemmTile *)put(c.htiles.data(), c.htiles.size() * sizeof(GemmTile),
                                    std::max<size_t>(c.htiles.size(), 1) * sizeof(GemmTile));
+        c.dcols = (ColTile *)put(c.hcols.data(), c.hcols.size() * sizeof(ColTile),
+                                 std::max<size_t>(c.hcols.size(), 1) * sizeof(ColTile));
         c.djobs = (IpssJob *)put(c.hjobs.data(), c.hjobs.size() * sizeof(IpssJob),
                                  std::max<size_t>(c.hjobs.size(), 1) * sizeof(IpssJob));
         c.djob_of = (int *)put(job_of.data(), job_of.size() * sizeof(int), job_of.size() * sizeof(int));
@@ -475,11 +534,15 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
                            c.ditem, P);
     else
         hipLaunchKernelGGL(pack_operator<double>, dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst, c.ditem, P);
-    const dim3 cols((c.max_P + 255) / 256, nit);
+    // per-column kernels: one 64-column tile per wavefront, four per workgroup
+    const int ncols = (int)c.hcols.size();
+    const dim3 cols((unsigned)((ncols + 3) / 4));
     if (mixed)
-        hipLaunchKernelGGL((order_one<__hip_bfloat16, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
+        hipLaunchKernelGGL((order_one<__hip_bfloat16, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st,
+                           c.dcols, ncols);
     else
-        hipLaunchKernelGGL((order_one<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
+        hipLaunchKernelGGL((order_one<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st, c.dcols,
+                           ncols);
     const unsigned ntiles = (unsigned)c.htiles.size();
     for (int it = 2; it <= c.maxit; ++it) {
         hipEvent_t *eg = p->profile ? ev_pair(p, 0) : nullptr;
@@ -490,7 +553,8 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             hipLaunchKernelGGL(gemm_bf16, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st);
             if (eg) (void)hipEventRecord(eg[1], s);
             if (es) (void)hipEventRecord(es[0], s);
-            hipLaunchKernelGGL((sweep<__hip_bfloat16, float, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
+            hipLaunchKernelGGL((sweep<__hip_bfloat16, float, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st,
+                               c.dcols, ncols);
             if (es) (void)hipEventRecord(es[1], s);
         } else {
             hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
@@ -498,7 +562,8 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             hipLaunchKernelGGL(gemm_f64, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st);
             if (eg) (void)hipEventRecord(eg[1], s);
             if (es) (void)hipEventRecord(es[0], s);
-            hipLaunchKernelGGL((sweep<double, double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st);
+            hipLaunchKernelGGL((sweep<double, double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st,
+                               c.dcols, ncols);
             if (es) (void)hipEventRecord(es[1], s);
         }
         p->gemm_launches += 1.0;
@@ -566,7 +631,9 @@ extern "C" int ocrt_plan_results(ocrt_plan *p, ocrt_result_t *out, double *toa_m
             const double L = I.nt + 1.0;
             const int parts = (I.aer ? 1 : 0) + ((m <= 2 && I.ray) ? 1 : 0);
             flops += ord * 2.0 * (6.0 * I.n) * (6.0 * I.n * std::max(parts, 1)) * L;
-            alg += ord * 72.0 * I.n * n_in * L * std::max(parts, 1);
+            // dense particle operator 2 (6 n)(6 n_in) per level; the molecular
+            // operator has rank <= 6 and counts as 2 * 2 * (6 n) * 6 per level
+            alg += ord * L * ((I.aer ? 72.0 * I.n * n_in : 0.0) + ((m <= 2 && I.ray) ? 144.0 * I.n : 0.0));
             bytes += ord * (2.0 * 3.0 * 2.0 * I.n * L + 2.0 * I.n * I.nt) * (mixed ? 4.0 : 8.0);
         }
         if (toa_modes)

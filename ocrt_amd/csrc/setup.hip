@@ -85,6 +85,7 @@ __global__ void medium_fractions(const DevInst *__restrict__ inst, Pools P)
         double *T = P.T + I.off_T + (long long)k * I.n;
         float *Tf = P.Tf + I.off_T + (long long)k * I.n;
         const double dtau = h[k + 1] - h[k];
+        P.idt[I.off_lev + k] = (float)(1.0 / dtau);
         for (int o = 0; o < I.n; ++o) {
             T[o] = exp(-dtau / fabs(mu[o]));
             // fp32 sweeps carry 1 - T: with dtau / mu ~ 1e-4 per layer, T itself

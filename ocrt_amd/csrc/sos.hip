@@ -70,18 +70,33 @@ __device__ inline void sweep_column_f64(int hemi, int nt, double mu, double bc, 
     }
 }
 
+// Field store of the mixed form: bf16, plus the bf16 residual v - hi for the
+// items whose field is kept as a pair.
+__device__ inline void store_field_pair(__hip_bfloat16 *hi, __hip_bfloat16 *lo, long long ix, double v)
+{
+    const __hip_bfloat16 h = __float2bfloat16((float)v);
+    hi[ix] = h;
+    if (lo) lo[ix] = __float2bfloat16((float)(v - (double)__bfloat162float(h)));
+}
+
 // ---------------------------------------------------------------------------
-// Order 1: one thread per state column of one item.
+// Order 1: one thread per state column; each wavefront takes one 64-column
+// tile of one item from the plan's tile list.
 template <typename FT, bool MIXED>
 __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ inst,
                                                  const DevItem *__restrict__ items, Pools P,
-                                                 ItemState *__restrict__ st)
+                                                 ItemState *__restrict__ st, const ColTile *__restrict__ tiles,
+                                                 int n_tiles)
 {
-    const DevItem &it = items[blockIdx.y];
+    const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (tix >= n_tiles) return;
+    const ColTile ct = tiles[tix];
+    if (ct.item < 0) return;
+    const DevItem &it = items[ct.item];
     const DevInst &I = inst[it.inst];
     const int n = it.n, dirs = 2 * n + 1, nt = it.nt, Pp = it.P, m = it.m, lmax = it.lmax;
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int col = ct.col0 + lane;
+    if (ct.col0 == 0 && lane == 0) {
         ItemState s;
         s.active = I.max_orders >= 2 ? 1 : 0;
         s.orders = 1;
@@ -90,17 +105,21 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
         s.residual = 0.0;
         s.max_cur = 0ull;
         s.max_tot = 0ull;
-        st[blockIdx.y] = s;
+        st[ct.item] = s;
     }
     if (col >= Pp) return;
     FT *F = reinterpret_cast<FT *>(P.F) + it.off_F;
     double *tot = P.tot + it.off_tot;
-    float *comp = MIXED ? P.comp + it.off_tot : nullptr;
+    if (MIXED) {
+        P.comp[it.off_cmp + col] = 0.f;
+        P.comp[it.off_cmp + Pp + col] = 0.f;
+    }
+    __hip_bfloat16 *Flo = (MIXED && it.fsplit) ? reinterpret_cast<__hip_bfloat16 *>(P.Flo) + it.off_Flo : nullptr;
     if (col >= 6 * n) {
         for (int k = 0; k <= nt; ++k) {
             F[(long long)k * Pp + col] = to_storage<FT>(0.0);
+            if (Flo) Flo[(long long)k * Pp + col] = __float2bfloat16(0.f);
             tot[(long long)k * Pp + col] = 0.0;
-            if (MIXED) comp[(long long)k * Pp + col] = 0.f;
         }
         return;
     }
@@ -181,8 +200,10 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     }
     for (int k = 0; k <= nt; ++k) {
         const long long ix = (long long)k * Pp + col;
-        F[ix] = to_storage<FT>(tot[ix]);
-        if (MIXED) comp[ix] = 0.f;
+        if constexpr (MIXED)
+            store_field_pair(reinterpret_cast<__hip_bfloat16 *>(F), Flo, ix, tot[ix]);
+        else
+            F[ix] = to_storage<FT>(tot[ix]);
     }
     if (hemi == 1) P.bc[it.off_bc + 3 * n + c * n + o] = tot[(long long)nt * Pp + col];
 }
@@ -269,7 +290,7 @@ __global__ __launch_bounds__(256) void gemm_f64(const DevInst *__restrict__ inst
                                                 const ItemState *__restrict__ st)
 {
     const GemmTile tl = tiles[blockIdx.x];
-    if (!st[tl.item].active) return;
+    if (tl.item < 0 || !st[tl.item].active) return;
     const DevItem &it = items[tl.item];
     const DevInst &I = inst[it.inst];
     const int K = it.K, Pp = it.P, L = it.L;
@@ -337,17 +358,19 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
                                                  const ItemState *__restrict__ st)
 {
     const GemmTile tl = tiles[blockIdx.x];
-    if (!st[tl.item].active) return;
+    if (tl.item < 0 || !st[tl.item].active) return;
     const DevItem &it = items[tl.item];
     const DevInst &I = inst[it.inst];
     const int Pp = it.P, L = it.L, Mp = it.Mp;
     const int nparts = it.K / Pp;
     const unsigned short *A = reinterpret_cast<const unsigned short *>(P.A) + it.off_A;
     const unsigned short *F = reinterpret_cast<const unsigned short *>(P.F) + it.off_F;
+    const bool split = it.fsplit != 0;
+    const unsigned short *Flo = reinterpret_cast<const unsigned short *>(P.Flo) + it.off_Flo;
     float *J = reinterpret_cast<float *>(P.J) + it.off_F;
     const LevelView lv(P, I);
     __shared__ unsigned short As[2][2][64][40];
-    __shared__ unsigned short Bs[64][40];
+    __shared__ unsigned short Bs[2][64][40];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int row0 = tl.tm * 64, lev0 = tl.tn * 64;
@@ -364,21 +387,30 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
                 const unsigned short *src = A + (long long)(q * 2 + hl) * slab + (long long)(row0 + lr) * Pp + k0 + lk;
                 *reinterpret_cast<uint4 *>(&As[q][hl][lr][lk]) = *reinterpret_cast<const uint4 *>(src);
             }
-        uint4 bv = {0u, 0u, 0u, 0u};
-        if (lev_load < L) bv = *reinterpret_cast<const uint4 *>(F + (long long)lev_load * Pp + k0 + lk);
-        *reinterpret_cast<uint4 *>(&Bs[lr][lk]) = bv;
+        uint4 bv = {0u, 0u, 0u, 0u}, bl = {0u, 0u, 0u, 0u};
+        if (lev_load < L) {
+            bv = *reinterpret_cast<const uint4 *>(F + (long long)lev_load * Pp + k0 + lk);
+            if (split) bl = *reinterpret_cast<const uint4 *>(Flo + (long long)lev_load * Pp + k0 + lk);
+        }
+        *reinterpret_cast<uint4 *>(&Bs[0][lr][lk]) = bv;
+        if (split) *reinterpret_cast<uint4 *>(&Bs[1][lr][lk]) = bl;
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < 32; ks += 16) {
-            short4_ b4[2];
-            for (int b = 0; b < 2; ++b)
-                b4[b] = *reinterpret_cast<const short4_ *>(&Bs[wn + b * 16 + (lane & 15)][ks + 4 * (lane >> 4)]);
+            short4_ b4[2], b4l[2];
+            for (int b = 0; b < 2; ++b) {
+                b4[b] = *reinterpret_cast<const short4_ *>(&Bs[0][wn + b * 16 + (lane & 15)][ks + 4 * (lane >> 4)]);
+                if (split)
+                    b4l[b] = *reinterpret_cast<const short4_ *>(&Bs[1][wn + b * 16 + (lane & 15)][ks + 4 * (lane >> 4)]);
+            }
             for (int q = 0; q < nparts; ++q)
                 for (int a = 0; a < 2; ++a) {
                     const int row = wm + a * 16 + (lane & 15);
                     const short4_ hi = *reinterpret_cast<const short4_ *>(&As[q][0][row][ks + 4 * (lane >> 4)]);
                     const short4_ lo = *reinterpret_cast<const short4_ *>(&As[q][1][row][ks + 4 * (lane >> 4)]);
                     for (int b = 0; b < 2; ++b) {
+                        if (split)
+                            acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(hi, b4l[b], acc[q][a][b], 0, 0, 0);
                         acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lo, b4[b], acc[q][a][b], 0, 0, 0);
                         acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(hi, b4[b], acc[q][a][b], 0, 0, 0);
                     }
@@ -408,77 +440,73 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
 }
 
 // ---------------------------------------------------------------------------
-__device__ inline double block_max(double v, double *red)
+// Largest value over one wavefront (every lane receives it).
+__device__ inline double wave_max(double v)
 {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0) {
-        const int nw = (blockDim.x + 63) >> 6;
-        for (int i = 0; i < nw; ++i) r = fmax(r, red[i]);
-    }
-    __syncthreads();
-    return r;
+    return v;
 }
 
 // Formal solution of the current order for every column, fused with the order
-// sum total += cur (plain in fp64, Neumaier-compensated in the mixed form with
-// an fp32 compensation term) and the two max-norms of the convergence test
-// (rt_solver.c:1855-1868, 3093-3124).  One thread per column; the levels are
-// walked in blocks of kU: every load of a block is issued before its
-// dependent recurrence steps run, so the sequential scan overlaps its memory
-// latency instead of exposing it once per level.
+// sum total += cur and the two max-norms of the convergence test
+// (rt_solver.c:1855-1868, 3093-3124).  One thread per column, one 64-column
+// tile per wavefront; the levels are walked in blocks of kU, every load of a
+// block issued before its dependent recurrence steps, so the sequential scan
+// overlaps its memory latency instead of exposing it once per level.
+//
+// The order sum is fp64 over the whole field (it feeds the convergence test).
+// In the mixed form the two rows that become outputs, k = 0 (TOA) and k = nt
+// (BOA), also carry an fp32 Neumaier compensation term.
 constexpr int kU = 8;
 
 template <typename FT, typename JT, bool MIXED>
 __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                                              const DevItem *__restrict__ items, Pools P,
-                                             ItemState *__restrict__ st)
+                                             ItemState *__restrict__ st, const ColTile *__restrict__ tiles,
+                                             int n_tiles)
 {
-    const int item = blockIdx.y;
-    if (!st[item].active) return;
+    const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (tix >= n_tiles) return;
+    const ColTile ct = tiles[tix];
+    if (ct.item < 0 || !st[ct.item].active) return;
+    const int item = ct.item;
     const DevItem &it = items[item];
     const DevInst &I = inst[it.inst];
     const int n = it.n, nt = it.nt, Pp = it.P;
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    __shared__ double red[8];
+    const int col = ct.col0 + lane;
     double mcur = 0.0, mtot = 0.0;
     if (col < 6 * n) {
         const LevelView lv(P, I);
         const int type = col / n, c = type >> 1, hemi = type & 1, o = col % n;
         const JT *J = reinterpret_cast<const JT *>(P.J) + it.off_F + col;
         FT *F = reinterpret_cast<FT *>(P.F) + it.off_F + col;
+        __hip_bfloat16 *Flo = (MIXED && it.fsplit)
+                                  ? reinterpret_cast<__hip_bfloat16 *>(P.Flo) + it.off_Flo + col : nullptr;
         double *tot = P.tot + it.off_tot + col;
-        float *cmp = MIXED ? P.comp + it.off_tot + col : nullptr;
         const double mu = P.ring[I.off_ring + o];
         const double bc = (I.surf != 0 && hemi == 0) ? P.bc[it.off_bc + c * n + o] : 0.0;
         // level order of the scan and the layer crossed when stepping to level k
         const int k_first = hemi ? 0 : nt;
         const int step = hemi ? 1 : -1;
         auto layer_of = [&](int k) { return hemi ? k - 1 : k; };
-        auto emit = [&](int k, double v, double t_old, float c_old) {
+        auto emit = [&](int k, double v, double t_old) {
             const long long ix = (long long)k * Pp;
-            F[ix] = to_storage<FT>(v);
+            if constexpr (MIXED)
+                store_field_pair(reinterpret_cast<__hip_bfloat16 *>(F), Flo, ix, v);
+            else
+                F[ix] = to_storage<FT>(v);
             mcur = fmax(mcur, fabs(v));
             if (hemi == 1 && k == nt) P.bc[it.off_bc + 3 * n + c * n + o] = v;
-            if (MIXED) {
-                const double tsum = t_old + v;
-                const double nc = (double)c_old + ((fabs(t_old) >= fabs(v)) ? ((t_old - tsum) + v) : ((v - tsum) + t_old));
-                tot[ix] = tsum;
-                cmp[ix] = (float)nc;
-                mtot = fmax(mtot, fabs(tsum + nc));
-            } else {
-                const double tsum = t_old + v;
-                tot[ix] = tsum;
-                mtot = fmax(mtot, fabs(tsum));
+            const double tsum = t_old + v;
+            tot[ix] = tsum;
+            mtot = fmax(mtot, fabs(tsum));
+            if (MIXED && (k == 0 || k == nt)) {
+                float *cp = P.comp + it.off_cmp + (k == 0 ? 0 : Pp) + col;
+                const double e = (fabs(t_old) >= fabs(v)) ? ((t_old - tsum) + v) : ((v - tsum) + t_old);
+                *cp = (float)((double)*cp + e);
             }
         };
-        {
-            const long long ix = (long long)k_first * Pp;
-            emit(k_first, hemi ? 0.0 : bc, tot[ix], MIXED ? cmp[ix] : 0.f);
-        }
+        emit(k_first, hemi ? 0.0 : bc, tot[(long long)k_first * Pp]);
         if (!MIXED) {
             const double *T = P.T + I.off_T;
             double run = hemi ? 0.0 : bc;
@@ -505,7 +533,7 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                     const double a = hemi ? (s_out - s_in) / dt : (s_in - s_out) / dt;
                     const double b = s_out - a * lv.h[k];
                     run = tr[u] * run + ((1.0 - tr[u]) * (b + a * smu) + a * (lv.h[k] - lv.h[ku] * tr[u])) * 0.5;
-                    emit(k, run, to[u], 0.f);
+                    emit(k, run, to[u]);
                     s_in = s_out;
                 }
             }
@@ -514,52 +542,52 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
             //   run <- run - om run + (S_out - S_in + om (S_in + slope mu_signed)) / 2
             // which is the linear-source recurrence with no 1 - T cancellation
             const float *OM = P.Tf + I.off_T;
+            const float *IDT = P.idt + I.off_lev;
             const float smu = hemi ? -(float)mu : (float)mu;
             float run = hemi ? 0.f : (float)bc;
             float s_in = (float)J[(long long)k_first * Pp];
             for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * kU) {
-                float sj[kU], om[kU], cm[kU];
-                double to[kU], hk[kU];
+                float sj[kU], om[kU], idt[kU];
+                double to[kU];
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
                     const int k = k0 + step * u;
                     if (k >= 0 && k <= nt) {
                         sj[u] = (float)J[(long long)k * Pp];
                         om[u] = OM[(long long)layer_of(k) * n + o];
+                        idt[u] = IDT[layer_of(k)];
                         to[u] = tot[(long long)k * Pp];
-                        cm[u] = cmp[(long long)k * Pp];
-                        hk[u] = lv.h[k];
                     }
                 }
-                double h_up = lv.h[k0 - step];
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
                     const int k = k0 + step * u;
                     if (k < 0 || k > nt) break;
-                    const float idt = (float)(1.0 / fabs(hk[u] - h_up));
                     const float s_out = sj[u];
-                    const float a = hemi ? (s_out - s_in) * idt : (s_in - s_out) * idt;
+                    const float a = hemi ? (s_out - s_in) * idt[u] : (s_in - s_out) * idt[u];
                     run = run - om[u] * run + 0.5f * ((s_out - s_in) + om[u] * (s_in + a * smu));
-                    emit(k, (double)run, to[u], cm[u]);
+                    emit(k, (double)run, to[u]);
                     s_in = s_out;
-                    h_up = hk[u];
                 }
             }
         }
     }
-    const double bm = block_max(mcur, red);
-    const double bt = block_max(mtot, red);
-    if (threadIdx.x == 0) {
+    const double bm = wave_max(mcur);
+    const double bt = wave_max(mtot);
+    if (lane == 0) {
         atomicMax(&st[item].max_cur, (unsigned long long)__double_as_longlong(bm));
         atomicMax(&st[item].max_tot, (unsigned long long)__double_as_longlong(bt));
     }
 }
 
 // ---------------------------------------------------------------------------
-__device__ inline double total_at(const Pools &P, const DevItem &it, bool mixed, long long ix)
+// Order sum at level k of column col; the output rows k = 0 and k = nt of the
+// mixed form add their compensation term.
+__device__ inline double total_at(const Pools &P, const DevItem &it, bool mixed, int k, int col)
 {
-    const double *tot = P.tot + it.off_tot;
-    return mixed ? tot[ix] + (double)P.comp[it.off_tot + ix] : tot[ix];
+    const double t = P.tot[it.off_tot + (long long)k * it.P + col];
+    if (mixed && (k == 0 || k == it.nt)) return t + (double)P.comp[it.off_cmp + (k == 0 ? 0 : it.P) + col];
+    return t;
 }
 
 // Per-item outputs after the order loop (rt_solver.c:1010-1061).
@@ -575,18 +603,18 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
     __shared__ double red[8];
     double hs = 0.0;
     for (int o = threadIdx.x; o < n; o += blockDim.x)
-        hs += mu[o] * w[o] * total_at(P, it, mixed, (long long)nt * Pp + n + o);
+        hs += mu[o] * w[o] * total_at(P, it, mixed, nt, n + o);
     // ordered sum over nodes for the hemispheric flux
     red[0] = 0.0;
     __syncthreads();
     if (threadIdx.x == 0) {
         double F = 0.0;
-        for (int o = 0; o < n; ++o) F += mu[o] * w[o] * total_at(P, it, mixed, (long long)nt * Pp + n + o);
+        for (int o = 0; o < n; ++o) F += mu[o] * w[o] * total_at(P, it, mixed, nt, n + o);
         const ItemState s = st[item];
         ItemOut r;
         const double tau = lv.h[nt];
         for (int c = 0; c < 3; ++c) {
-            double val = total_at(P, it, mixed, (long long)(2 * c) * n + v);
+            double val = total_at(P, it, mixed, 0, 2 * c * n + v);
             if (I.surf == 2) {
                 const double down = exp(-tau / I.mus);
                 const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
@@ -595,7 +623,7 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
             }
             r.mode[c] = val;
         }
-        r.boa_view = total_at(P, it, mixed, (long long)nt * Pp + n + v);
+        r.boa_view = total_at(P, it, mixed, nt, n + v);
         r.boa_hemi = F;
         r.orders = s.orders;
         r.converged = s.converged;
@@ -607,7 +635,7 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
         double *dst = P.toa_up + (long long)item * P.toa_up_stride;
         for (int q = threadIdx.x; q < 3 * n; q += blockDim.x) {
             const int c = q / n, o = q % n;
-            double val = total_at(P, it, mixed, (long long)(2 * c) * n + o);
+            double val = total_at(P, it, mixed, 0, 2 * c * n + o);
             if (I.surf == 2) {
                 const double tau = lv.h[nt], down = exp(-tau / I.mus);
                 const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
@@ -720,11 +748,15 @@ __global__ void finalize(const DevInst *__restrict__ inst, int n_inst, const Ite
 }
 
 // explicit instantiations
-template __global__ void order_one<double, false>(const DevInst *, const DevItem *, Pools, ItemState *);
-template __global__ void order_one<__hip_bfloat16, true>(const DevInst *, const DevItem *, Pools, ItemState *);
+template __global__ void order_one<double, false>(const DevInst *, const DevItem *, Pools, ItemState *,
+                                                  const ColTile *, int);
+template __global__ void order_one<__hip_bfloat16, true>(const DevInst *, const DevItem *, Pools, ItemState *,
+                                                         const ColTile *, int);
 template __global__ void order_status<double>(const DevInst *, const DevItem *, Pools, ItemState *, int);
 template __global__ void order_status<__hip_bfloat16>(const DevInst *, const DevItem *, Pools, ItemState *, int);
-template __global__ void sweep<double, double, false>(const DevInst *, const DevItem *, Pools, ItemState *);
-template __global__ void sweep<__hip_bfloat16, float, true>(const DevInst *, const DevItem *, Pools, ItemState *);
+template __global__ void sweep<double, double, false>(const DevInst *, const DevItem *, Pools, ItemState *,
+                                                      const ColTile *, int);
+template __global__ void sweep<__hip_bfloat16, float, true>(const DevInst *, const DevItem *, Pools, ItemState *,
+                                                            const ColTile *, int);
 
 }  // namespace ocrt

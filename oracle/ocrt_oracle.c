@@ -33,7 +33,10 @@
 
 #define PI 3.14159265358979323846
 
-static int g_bf16 = 0;      /* packed: significant bits of operator (>> 8) and field (& 0xff) */
+/* Emulated rounding of the scattering integral's operands, packed as
+ * (q << 16) | (operator significant bits << 8) | field significant bits;
+ * Fourier modes m < q round the field to 16 bits instead (a two-term split). */
+static int g_bf16 = 0;
 void oracle_set_bf16_emulation(int bf16) { g_bf16 = (bf16 == 1) ? ((8 << 8) | 8) : bf16; }
 
 /* round to p significant bits, nearest-even (p = 8: bfloat16, 11: binary16) */
@@ -974,11 +977,11 @@ static int apply_operator(const med_t *M, int m, const ang_t *A, const double *I
                 double e[18];
                 for (int q = 0; q < 18; ++q) {
                     double pc = c[q] * x + y * r[q];
-                    e[q] = emu ? round_bits(pc, emu >> 8) : pc;
+                    e[q] = emu ? round_bits(pc, (emu >> 8) & 0xff) : pc;
                 }
                 double a0 = Ip[FI(k, i)], a1 = Ip[FI(k, -i)], a2_ = Qp[FI(k, i)], a3 = Qp[FI(k, -i)];
                 double a4 = Up[FI(k, i)], a5 = Up[FI(k, -i)];
-                if (emu) { const int pf = emu & 0xff;
+                if (emu) { const int pf = (m < ((emu >> 16) & 0xff)) ? 16 : (emu & 0xff);
                            a0 = round_bits(a0, pf); a1 = round_bits(a1, pf); a2_ = round_bits(a2_, pf);
                            a3 = round_bits(a3, pf); a4 = round_bits(a4, pf); a5 = round_bits(a5, pf); }
                 iu += w * (a0 * e[0] + a2_ * e[8] - a4 * e[4]) + w * (a1 * e[1] + a3 * e[9] - a5 * e[5]);

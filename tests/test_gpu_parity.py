@@ -128,13 +128,19 @@ def test_one_instance_per_launch_equals_batched(ocrt, golden_descs):
 
 
 def test_mixed_vs_fp64_full_size_batch(ocrt):
-    """Config-2 instances at their full sizes (16..64 streams, 50..400 layers):
-    the mixed form stays inside the error budget of the FP64 form."""
+    """The whole batched workload (8 bands x 1024 geometries, 16..64 streams,
+    50..400 layers, rough ocean, modes 0..16): every instance of the mixed form
+    stays inside the error budget of the FP64 form, and both converge alike."""
     from ocrt_amd import synthetic as S
-    descs, keep = S.config2(n_geom=6)
+    descs, keep = S.config2()
+    print(f"[gpu] full workload: {len(descs)} instances, fp64 then mixed", flush=True)
     a = ocrt.solve(descs, precision="fp64")
     b = ocrt.solve(descs, precision="mixed")
+    assert (a["status"] == 0).all() and (b["status"] == 0).all()
     ei = np.abs(b["rho_I"] - a["rho_I"]) / np.abs(a["rho_I"])
     eq = np.maximum(np.abs(b["rho_Q"] - a["rho_Q"]), np.abs(b["rho_U"] - a["rho_U"]))
-    print(f"[gpu] config-2 sample: mixed vs fp64 worst rel I {ei.max():.2e}, abs Q/U {eq.max():.2e}")
+    print(f"[gpu] full workload: mixed vs fp64 worst rel I {ei.max():.2e} (instance {int(ei.argmax())}), "
+          f"abs Q/U {eq.max():.2e}", flush=True)
     assert ei.max() <= MIXED_TOL_I and eq.max() <= MIXED_TOL_QU
+    # the order count may differ by one where the residual sits at the tolerance
+    assert np.abs(a["n_orders_used"] - b["n_orders_used"]).max() <= 1
