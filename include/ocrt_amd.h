@@ -83,6 +83,17 @@ typedef struct {
     int    max_orders;         /* SOS order cap, default 20 */
     int    ipss;               /* 1: spherical-shell kappa correction */
     double tolerance;          /* order-sum stop, default 1e-7 */
+    /* Optional explicit layering, host [4][n_layers], layers listed top down:
+     *   [0] Rayleigh scattering optical depth, [1] aerosol extinction optical
+     *   depth, [2] aerosol single-scattering albedo, [3] absorption optical
+     *   depth (gas).
+     * NULL selects the column generator above (tau_R, tau_a, ssa_a,
+     * aer_h_km on US62 altitudes, rt_atm.c:272-400).  With layers, level k
+     * sits below layer k-1 and carries that layer's scattering fractions
+     * (the reference's level convention, rt_atm.c:343-360 and :521-538 for
+     * absorption); tau_R and tau_a are ignored and IPSS is not available
+     * (the layering carries no altitudes). */
+    const double *layers;
 } ocrt_instance_t;
 
 /* Per-instance results; names follow rt_result_t (code/OCRT_C/src/rt_types.h:496-651). */

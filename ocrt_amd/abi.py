@@ -39,6 +39,7 @@ class Instance(C.Structure):
         ("n_layers", C.c_int), ("m_max", C.c_int),
         ("max_orders", C.c_int), ("ipss", C.c_int),
         ("tolerance", C.c_double),
+        ("layers", C.POINTER(C.c_double)),
     ]
 
 
@@ -66,9 +67,12 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
                   surface="flat", n_water=1.34, wind=0.0, sigma_type=1,
                   q_convention=1, decouple_sunglint=0, tau_R=0.0941, depol=0.0279,
                   tau_a=0.0, ssa_a=1.0, aer_h_km=2.0, greek=None, n_mu=24,
-                  n_layers=50, m_max=2, max_orders=20, tolerance=1e-7, ipss=0):
+                  n_layers=50, m_max=2, max_orders=20, tolerance=1e-7, ipss=0, layers=None):
     """Fill one descriptor.  `greek` is a float64 array [4, L+1] (beta, gamma,
-    alpha, zeta) or None; the caller keeps it alive while the descriptor is used."""
+    alpha, zeta) or None.  `layers` is None (column generator) or a float64
+    array [4, n_layers] of per-layer Rayleigh tau, aerosol tau, aerosol SSA and
+    absorption tau, top down; n_layers then follows its shape.  Converted
+    copies are kept alive by the descriptor."""
     d = Instance()
     d.sza_deg, d.vza_deg, d.raa_deg, d.wavelength_nm = sza, vza, raa, wavelength
     d.surface = SURFACE_NAMES[surface] if isinstance(surface, str) else int(surface)
@@ -89,6 +93,15 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
         d.greek = None
     d.n_mu, d.n_layers, d.m_max = n_mu, n_layers, m_max
     d.max_orders, d.tolerance, d.ipss = max_orders, tolerance, ipss
+    if layers is not None:
+        lay = np.ascontiguousarray(layers, dtype=np.float64)
+        if lay.ndim != 2 or lay.shape[0] != 4:
+            raise ValueError("layers must have shape [4, n_layers]")
+        d.n_layers = lay.shape[1]
+        d.layers = lay.ctypes.data_as(C.POINTER(C.c_double))
+        d._layers_keepalive = lay
+    else:
+        d.layers = None
     return d
 
 

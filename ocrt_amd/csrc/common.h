@@ -32,13 +32,14 @@ struct DevInst {
     int n, vidx, nt, mmax, La, lmax, aer, ray;
     int surf;            // 0 none, 1 flat, 2 rough
     int sigma_type, q_conv, decouple, ipss, max_orders, item0, surface_kind, surf_owner;
-    int pad_;
+    int layered;         // 1: explicit per-layer optical depths (pool_layers)
     long long off_lev;   // pool_lev: 5 arrays of L doubles (h, ch, xd, yd, z)
     long long off_ring;  // pool_ring: mu[n], w[n]
     long long off_T;     // pool_T (double) / pool_Tf (float): [nt][n]
     long long off_greek; // pool_greek: [4][La+1]
     long long off_R;     // pool_R: [(mmax+1)][n][n][9]
     long long off_Rsun;  // pool_R: [(mmax+1)][n][9]
+    long long off_layers;// pool_layers: [4][nt] Rayleigh tau, aerosol tau, aerosol ssa, absorption tau
 };
 
 struct DevItem {
@@ -77,6 +78,7 @@ struct InstOut {
 
 struct Pools {
     double *lev, *ring, *T, *greek, *R, *basis, *tab, *tot, *bc;
+    double *layers;       // explicit layering of the instances that carry one
     float *comp;          // fp32 compensation of the mixed-form order sum on the two output rows
     float *Tf;            // fp32 1 - T for the mixed-form sweeps
     float *idt;           // fp32 1 / dtau per layer, indexed like lev (mixed-form sweeps)

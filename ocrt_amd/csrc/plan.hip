@@ -177,7 +177,7 @@ struct ocrt_plan {
     std::vector<Chunk> chunks;
     std::vector<int> item_of_inst;   // first global item of each instance
     // global device arrays
-    double *ring = nullptr, *greek = nullptr, *cm_w = nullptr;
+    double *ring = nullptr, *greek = nullptr, *cm_w = nullptr, *layers = nullptr;
     ItemState *state = nullptr;
     ItemOut *iout = nullptr;
     InstOut *ipss = nullptr;
@@ -226,6 +226,20 @@ static int dmalloc(ocrt_plan *p, void **ptr, size_t bytes)
     return 0;
 }
 
+static double layer_sum(const ocrt_instance_t &d, int row)
+{
+    double s = 0.0;
+    for (int l = 0; l < d.n_layers; ++l) s += d.layers[(size_t)row * d.n_layers + l];
+    return s;
+}
+
+// Aerosol scattering enters only with a phase matrix and a positive column.
+static bool has_aerosol(const ocrt_instance_t &d)
+{
+    if (d.layers) return d.L_max > 0 && d.greek && layer_sum(d, 1) > 0.0;
+    return d.L_max > 0 && d.tau_a > 0.0 && d.greek;
+}
+
 static int validate(const ocrt_instance_t &d)
 {
     if (!(d.wavelength_nm >= 330.0 - 1e-9 && d.wavelength_nm <= 1100.0 + 1e-9)) return OCRT_ERR_ARGS;
@@ -234,10 +248,22 @@ static int validate(const ocrt_instance_t &d)
     if (d.m_max < 0 || d.m_max > 1024) return OCRT_ERR_ARGS;
     if (d.n_mu < 2 || d.n_mu > kMaxRing - 1 || d.n_layers < 2) return OCRT_ERR_ARGS;
     if (d.max_orders < 1 || !(d.tolerance > 0.0)) return OCRT_ERR_ARGS;
-    const bool aer = d.L_max > 0 && d.tau_a > 0.0 && d.greek;
-    if (aer && (d.ssa_a < 0.0 || d.ssa_a > 1.0 || d.L_max < 2)) return OCRT_ERR_ARGS;
-    if (d.tau_a < 0.0 || d.tau_R < 0.0) return OCRT_ERR_ARGS;
-    if (!(d.tau_R + (aer ? d.tau_a : 0.0) > 0.0)) return OCRT_ERR_ARGS;
+    const bool aer = has_aerosol(d);
+    if (d.layers) {
+        // every layer needs a positive extinction and an albedo in [0, 1]
+        const int nt = d.n_layers;
+        const double *R = d.layers, *A = R + nt, *W = A + nt, *G = W + nt;
+        for (int l = 0; l < nt; ++l) {
+            if (!(R[l] >= 0.0 && A[l] >= 0.0 && G[l] >= 0.0 && W[l] >= 0.0 && W[l] <= 1.0)) return OCRT_ERR_ARGS;
+            if (!(R[l] + A[l] + G[l] > 0.0)) return OCRT_ERR_ARGS;
+        }
+        if (d.ipss) return OCRT_ERR_ARGS;
+        if (aer && d.L_max < 2) return OCRT_ERR_ARGS;
+    } else {
+        if (aer && (d.ssa_a < 0.0 || d.ssa_a > 1.0 || d.L_max < 2)) return OCRT_ERR_ARGS;
+        if (d.tau_a < 0.0 || d.tau_R < 0.0) return OCRT_ERR_ARGS;
+        if (!(d.tau_R + (aer ? d.tau_a : 0.0) > 0.0)) return OCRT_ERR_ARGS;
+    }
     if (d.surface < 0 || d.surface > 3) return OCRT_ERR_ARGS;
     if ((d.surface == OCRT_SURFACE_FLAT || d.surface == OCRT_SURFACE_BLACK_FRESNEL_OCEAN) && !(d.n_water > 0.0))
         return OCRT_ERR_ARGS;
@@ -258,7 +284,7 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     const bool mixed = precision == OCRT_PREC_MIXED;
     const int Palign = mixed ? 32 : 16;
     p->inst.resize(n_inst);
-    long long ring_n = 0, greek_n = 0;
+    long long ring_n = 0, greek_n = 0, layers_n = 0;
     for (int i = 0; i < n_inst; ++i) {
         const ocrt_instance_t &d = in[i];
         HostInst &h = p->inst[i];
@@ -269,10 +295,16 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
         I.sza = d.sza_deg; I.vza = d.vza_deg; I.raa = d.raa_deg;
         I.mus = cos(d.sza_deg * OCRT_PI / 180.0);
         I.muv = cos(d.vza_deg * OCRT_PI / 180.0);
-        I.aer = (d.L_max > 0 && d.tau_a > 0.0 && d.greek) ? 1 : 0;
-        I.tauR = d.tau_R > 0.0 ? d.tau_R : 0.0;
+        I.aer = has_aerosol(d) ? 1 : 0;
+        I.layered = d.layers ? 1 : 0;
+        if (d.layers) {
+            I.tauR = layer_sum(d, 0);
+            I.taua = I.aer ? layer_sum(d, 1) : 0.0;
+        } else {
+            I.tauR = d.tau_R > 0.0 ? d.tau_R : 0.0;
+            I.taua = I.aer ? d.tau_a : 0.0;
+        }
         I.ray = I.tauR > 0.0;
-        I.taua = I.aer ? d.tau_a : 0.0;
         I.ssa = d.ssa_a; I.aer_h = d.aer_h_km; I.n_water = d.n_water; I.wind = d.wind_speed;
         I.tol = d.tolerance;
         const double dp = (d.depol >= 0.0 && d.depol < 1.0) ? d.depol : 0.0279;
@@ -295,6 +327,8 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
         ring_n += 2 * n;
         I.off_greek = greek_n;
         greek_n += I.aer ? 4LL * (I.La + 1) : 0;
+        I.off_layers = layers_n;
+        layers_n += I.layered ? 4LL * I.nt : 0;
         h.lev_n = 5LL * (I.nt + 1);
         h.T_n = (long long)I.nt * n;
         h.R_n = (I.surf == 2) ? (long long)(I.mmax + 1) * n * n * 9 + (long long)(I.mmax + 1) * n * 9 : 0;
@@ -430,6 +464,7 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     Pools &P = p->pools;
     rc |= dmalloc(p, (void **)&p->ring, ring_n * 8);
     rc |= dmalloc(p, (void **)&p->greek, std::max(greek_n, 1LL) * 8);
+    rc |= dmalloc(p, (void **)&p->layers, std::max(layers_n, 1LL) * 8);
     rc |= dmalloc(p, (void **)&p->cm_w, (size_t)(p->mmax_all + 1) * 2 * kNPhi * 8);
     rc |= dmalloc(p, (void **)&p->state, (size_t)p->n_items * sizeof(ItemState));
     rc |= dmalloc(p, (void **)&p->iout, (size_t)p->n_items * sizeof(ItemOut));
@@ -454,10 +489,11 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     if (rc) { ocrt_plan_destroy(p); return OCRT_ERR_ALLOC; }
     P.ring = p->ring;
     P.greek = p->greek;
+    P.layers = p->layers;
     P.cm_w = p->cm_w;
     P.toa_up = nullptr;
     // ---- uploads of inputs and descriptors
-    std::vector<double> hring(ring_n), hgreek(std::max(greek_n, 1LL));
+    std::vector<double> hring(ring_n), hgreek(std::max(greek_n, 1LL)), hlayers(std::max(layers_n, 1LL));
     for (int ii = 0; ii < n_inst; ++ii) {
         const HostInst &h = p->inst[ii];
         const int n = h.d.n;
@@ -466,9 +502,12 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
             const int q = h.d.La + 1;
             for (int e = 0; e < 4 * q; ++e) hgreek[h.d.off_greek + e] = in[ii].greek[e];
         }
+        if (h.d.layered)
+            for (long long e = 0; e < 4LL * h.d.nt; ++e) hlayers[h.d.off_layers + e] = in[ii].layers[e];
     }
     if (hipMemcpy(p->ring, hring.data(), ring_n * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        (greek_n > 0 && hipMemcpy(p->greek, hgreek.data(), greek_n * 8, hipMemcpyHostToDevice) != hipSuccess)) {
+        (greek_n > 0 && hipMemcpy(p->greek, hgreek.data(), greek_n * 8, hipMemcpyHostToDevice) != hipSuccess) ||
+        (layers_n > 0 && hipMemcpy(p->layers, hlayers.data(), layers_n * 8, hipMemcpyHostToDevice) != hipSuccess)) {
         ocrt_plan_destroy(p);
         return OCRT_ERR_DEVICE;
     }

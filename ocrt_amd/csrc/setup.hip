@@ -27,6 +27,25 @@ __global__ void medium_levels(const DevInst *__restrict__ inst, Pools P)
     if (k > nt) return;
     const int L = nt + 1;
     double *h = P.lev + I.off_lev, *ch = h + L, *xd = ch + L, *yd = xd + L, *z = yd + L;
+    if (I.layered) {
+        // running extinction sum from the top, in layer order, so every level
+        // carries exactly the sequential prefix sum
+        const double *R = P.layers + I.off_layers, *A = R + nt, *W = A + nt, *G = W + nt;
+        double acc = 0.0;
+        for (int l = 0; l < k; ++l) acc += R[l] + A[l] + G[l];
+        h[k] = acc;
+        ch[k] = 0.5 * exp(-acc / I.mus);
+        z[k] = 0.0;
+        if (k == 0) {
+            if (I.tauR > 0.0) { xd[0] = 0.0; yd[0] = 1.0; }
+            else { xd[0] = I.aer ? W[0] : 0.0; yd[0] = 0.0; }
+        } else {
+            const double dt = R[k - 1] + A[k - 1] + G[k - 1];
+            xd[k] = I.aer ? W[k - 1] * A[k - 1] / dt : 0.0;
+            yd[k] = R[k - 1] / dt;
+        }
+        return;
+    }
     const double tau = I.tauR + (I.aer ? I.taua : 0.0);
     const double hk = (double)k * tau / (double)nt;
     h[k] = hk;
@@ -65,7 +84,7 @@ __global__ void medium_fractions(const DevInst *__restrict__ inst, Pools P)
     const int L = nt + 1;
     const double *h = P.lev + I.off_lev, *z = h + 4 * L;
     double *xd = P.lev + I.off_lev + 2 * L, *yd = xd + L;
-    if (I.aer) {
+    if (I.aer && !I.layered) {
         if (k == 0) {
             if (I.tauR == 0.0 && I.taua > 0.0) { xd[0] = I.ssa; yd[0] = 0.0; }
             else { xd[0] = 0.0; yd[0] = 1.0; }

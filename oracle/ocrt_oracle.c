@@ -184,9 +184,48 @@ static double aer_fraction_above(double z, double ztop, double H)
     return (exp(-z / H) - et) / (1.0 - et);
 }
 
+static double layers_sum(const ocrt_instance_t *in, int row)
+{
+    double s = 0.0;
+    for (int l = 0; l < in->n_layers; ++l) s += in->layers[(size_t)row * in->n_layers + l];
+    return s;
+}
+
 static int has_aerosol(const ocrt_instance_t *in)
 {
+    if (in->layers) return in->L_max > 0 && in->greek != NULL && layers_sum(in, 1) > 0.0;
     return in->L_max > 0 && in->tau_a > 0.0 && in->greek != NULL;
+}
+
+/* Explicit layering: h[k] is the running sum of the layer extinctions
+ * (Rayleigh + aerosol + absorption) from the top; level k >= 1 carries the
+ * scattering fractions of the layer above it, xdel = ssa_a tau_a / dtau and
+ * ydel = tau_R / dtau (rt_atm.c:343-360, renormalised for absorption as in
+ * rt_atm.c:521-538); level 0 keeps the generator's top value (pure Rayleigh
+ * when any Rayleigh is present, rt_atm.c:310-316). */
+static int medium_layers(const ocrt_instance_t *in, int aer, double mus, double *h, double *ch, double *xdel,
+                         double *ydel, double *z_km)
+{
+    const int nt = in->n_layers;
+    const double *R = in->layers, *A = R + nt, *W = A + nt, *G = W + nt;
+    double acc = 0.0;
+    h[0] = 0.0;
+    for (int l = 0; l < nt; ++l) {
+        if (!(R[l] >= 0.0 && A[l] >= 0.0 && G[l] >= 0.0 && W[l] >= 0.0 && W[l] <= 1.0)) return -1;
+        const double dt = R[l] + A[l] + G[l];
+        if (!(dt > 0.0)) return -1;
+        acc += dt;
+        h[l + 1] = acc;
+        xdel[l + 1] = aer ? W[l] * A[l] / dt : 0.0;
+        ydel[l + 1] = R[l] / dt;
+    }
+    if (layers_sum(in, 0) > 0.0) { xdel[0] = 0.0; ydel[0] = 1.0; }
+    else { xdel[0] = aer ? W[0] : 0.0; ydel[0] = 0.0; }
+    for (int k = 0; k <= nt; ++k) {
+        ch[k] = 0.5 * exp(-h[k] / mus);
+        z_km[k] = 0.0;
+    }
+    return 0;
 }
 
 /* Medium (rt_atm.c:177-400): uniform-tau levels h[k] = k tau / nt,
@@ -198,6 +237,7 @@ int oracle_medium(const ocrt_instance_t *in, double *h, double *ch, double *xdel
     const int nt = in->n_layers;
     const double mus = cos(in->sza_deg * PI / 180.0);
     const int aer = has_aerosol(in);
+    if (in->layers) return medium_layers(in, aer, mus, h, ch, xdel, ydel, z_km);
     const double tauR = in->tau_R > 0.0 ? in->tau_R : 0.0;
     const double taua = aer ? in->tau_a : 0.0;
     const double tau = tauR + taua;
@@ -1281,6 +1321,7 @@ static int solve_core(const ocrt_instance_t *in, solve_t *s)
     if (in->max_orders < 1 || !(in->tolerance > 0.0)) return -1;
     const int aer = has_aerosol(in);
     if (aer && (in->tau_a < 0.0 || in->ssa_a < 0.0 || in->ssa_a > 1.0 || in->L_max < 2)) return -1;
+    if (in->layers && in->ipss) return -1;
     med_t *M = &s->M;
     M->mus = cos(in->sza_deg * PI / 180.0);
     M->muv = cos(in->vza_deg * PI / 180.0);
@@ -1413,7 +1454,7 @@ int oracle_solve(const ocrt_instance_t *in, ocrt_result_t *out, double *toa_mode
     double I = oracle_reconstruct_cos(pm, mm, raa);
     double Q = oracle_reconstruct_cos(pm + (mm + 1), mm, raa);
     double U = oracle_reconstruct_sin(pm + 2 * (mm + 1), mm, raa);
-    out->tau_R_total = in->tau_R > 0.0 ? in->tau_R : 0.0;
+    out->tau_R_total = in->layers ? layers_sum(in, 0) : (in->tau_R > 0.0 ? in->tau_R : 0.0);
     out->I_TOA = I; out->Q_TOA = Q; out->U_TOA = U;
     out->rho_I = I / mus; out->rho_Q = Q / mus; out->rho_U = U / mus;
     double kappa = 1.0, tdir_ipss = -1.0;
@@ -1620,4 +1661,460 @@ double oracle_ipss_homog_tau_sun(double tau_t, int nlay, double sza_deg, const d
     const double t = ipss_tau_sun(&G, rB);
     ipss_free(&G);
     return isfinite(t) ? t : -1.0;
+}
+
+/* ======================================================================== */
+/* Reverse-mode derivative of the solve (test infrastructure; the reference
+ * has no adjoint: it is pinned by central finite differences of the forward
+ * solve above).  Every order of every mode is stored, then the order
+ * recursion is run backward:
+ *   lambda_N = e,  lambda_{o-1} = e + M^T Sbar_o + B^T R^T bbar_o,
+ * where e seeds the TOA view-node entries with the reconstruction weights,
+ * (Sbar_o, bbar_o) is the adjoint sweep of lambda_o, M the scattering
+ * integral and R the surface boundary operator.  Parameters: the per-level
+ * medium h[k], xdel[k], ydel[k] and the wind speed. */
+
+/* Per-step coefficients of the linear-source recurrence written as
+ * r_k = T r_prev + al S_k + be S_prev, al = (1 - q) / 2, be = (q - T) / 2,
+ * q = (1 - T) |mu| / dtau, and their derivatives in dtau. */
+static void step_coefs(double dt, double amu, double *T, double *al, double *be, double *dT, double *dal,
+                       double *dbe)
+{
+    const double x = dt / amu, t = exp(-x), om = -expm1(-x), q = om / x;
+    const double dq = (t - q) / dt;
+    *T = t; *al = 0.5 * (1.0 - q); *be = 0.5 * (q - t);
+    *dT = -t / amu; *dal = -0.5 * dq; *dbe = 0.5 * (dq + t / amu);
+}
+
+/* Adjoint of F = sweep(S, bc): lam is the adjoint of F, Sbar (zeroed by the
+ * caller) receives the adjoint of S, bcbar[n] that of the bottom boundary of
+ * the upward columns (may be NULL), gh the h-gradient of the layer steps. */
+static void sweep_adj(const med_t *M, const double *lam, const double *F, const double *S, double *Sbar,
+                      double *bcbar, double *gh)
+{
+    const int n = M->n, dirs = M->dirs, nt = M->nt;
+    const double *h = M->h;
+    for (int j = -n; j <= n; ++j) {
+        if (j == 0) continue;
+        const double amu = fabs(M->rm[j]);
+        if (j > 0) {
+            double rho = lam[FI(0, j)];
+            for (int k = 0; k < nt; ++k) {
+                double T, al, be, dT, dal, dbe;
+                step_coefs(h[k + 1] - h[k], amu, &T, &al, &be, &dT, &dal, &dbe);
+                Sbar[FI(k, j)] += al * rho;
+                Sbar[FI(k + 1, j)] += be * rho;
+                const double dd = rho * (dT * F[FI(k + 1, j)] + dal * S[FI(k, j)] + dbe * S[FI(k + 1, j)]);
+                gh[k + 1] += dd; gh[k] -= dd;
+                rho = lam[FI(k + 1, j)] + T * rho;
+            }
+            if (bcbar) bcbar[j - 1] = rho;
+        } else {
+            double rho = lam[FI(nt, j)];
+            for (int k = nt; k >= 1; --k) {
+                double T, al, be, dT, dal, dbe;
+                step_coefs(h[k] - h[k - 1], amu, &T, &al, &be, &dT, &dal, &dbe);
+                Sbar[FI(k, j)] += al * rho;
+                Sbar[FI(k - 1, j)] += be * rho;
+                const double dd = rho * (dT * F[FI(k - 1, j)] + dal * S[FI(k, j)] + dbe * S[FI(k - 1, j)]);
+                gh[k] += dd; gh[k - 1] -= dd;
+                rho = lam[FI(k - 1, j)] + T * rho;
+            }
+        }
+    }
+}
+
+/* Adjoint of J = M(x, y) F (apply_operator): adds M^T Jbar to Fbar and the
+ * x / y gradients sum_q c_q E_q, sum_q r_q E_q, E_q = d(Jbar . J) / d e_q. */
+static void apply_operator_adj(const med_t *M, int m, const ang_t *A, const double *JbI, const double *JbQ,
+                               const double *JbU, const double *Ip, const double *Qp, const double *Up,
+                               double *FbI, double *FbQ, double *FbU, double *gx, double *gy)
+{
+    const int n = M->n, dirs = M->dirs, nt = M->nt;
+    const int mol = (m <= 2);
+    const double b0 = (m == 0) ? M->beta0 : 0.0;
+    const double be2 = M->beta2, g2 = M->gamma2, a2 = M->alpha2;
+    for (int o = 1; o <= n; ++o) {
+        const double po = XPL(o), ro = XRL(o), to = XTL(o);
+        const double pom = XPL(-o), rom = XRL(-o), tom = XTL(-o);
+        for (int k = 0; k <= nt; ++k) {
+            const double x = M->xd[k], y = M->yd[k];
+            const double biu = JbI[FI(k, o)], bid = JbI[FI(k, -o)], bqu = JbQ[FI(k, o)], bqd = JbQ[FI(k, -o)];
+            const double buu = JbU[FI(k, o)], bud = JbU[FI(k, -o)];
+            for (int i = 1; i <= n; ++i) {
+                const double w = M->gb[i];
+                if (w == 0.0) continue;
+                const double pi_ = XPL(i), ri = XRL(i), ti = XTL(i), rim = XRL(-i), tim = XTL(-i);
+                double c[18], r[18], e[18], E[18];
+                c[0] = TB(A->P, i, o); c[1] = TB(A->P, i, -o);
+                c[2] = TB(A->GT, i, o); c[3] = TB(A->GT, i, -o);
+                c[4] = TB(A->GT, o, i); c[5] = TB(A->GT, o, -i);
+                c[6] = TB(A->GR, i, o); c[7] = TB(A->GR, i, -o);
+                c[8] = TB(A->GR, o, i); c[9] = TB(A->GR, o, -i);
+                c[10] = TB(A->ARR, i, o); c[11] = TB(A->ARR, i, -o);
+                c[12] = TB(A->ART, i, o); c[13] = TB(A->ART, i, -o);
+                c[14] = TB(A->ART, o, i); c[15] = TB(A->ART, o, -i);
+                c[16] = TB(A->ATT, i, o); c[17] = TB(A->ATT, i, -o);
+                if (mol) {
+                    r[0] = b0 + be2 * pi_ * po; r[1] = b0 + be2 * pi_ * pom;
+                    r[2] = g2 * pi_ * to; r[3] = g2 * pi_ * tom;
+                    r[4] = g2 * po * ti; r[5] = g2 * po * tim;
+                    r[6] = g2 * pi_ * ro; r[7] = g2 * pi_ * rom;
+                    r[8] = g2 * po * ri; r[9] = g2 * po * rim;
+                    r[10] = a2 * ri * ro; r[11] = a2 * ri * rom;
+                    r[12] = a2 * ti * ro; r[13] = a2 * ti * rom;
+                    r[14] = a2 * to * ri; r[15] = a2 * to * rim;
+                    r[16] = a2 * ti * to; r[17] = a2 * ti * tom;
+                } else {
+                    for (int q = 0; q < 18; ++q) r[q] = 0.0;
+                }
+                for (int q = 0; q < 18; ++q) e[q] = c[q] * x + y * r[q];
+                const double a0 = Ip[FI(k, i)], a1 = Ip[FI(k, -i)], a2_ = Qp[FI(k, i)], a3 = Qp[FI(k, -i)];
+                const double a4 = Up[FI(k, i)], a5 = Up[FI(k, -i)];
+                FbI[FI(k, i)] += w * (biu * e[0] + bid * e[1] + bqu * e[6] + bqd * e[7] - buu * e[2] - bud * e[3]);
+                FbI[FI(k, -i)] += w * (biu * e[1] + bid * e[0] + bqu * e[7] + bqd * e[6] + buu * e[3] + bud * e[2]);
+                FbQ[FI(k, i)] += w * (biu * e[8] + bid * e[9] + bqu * e[10] + bqd * e[11] - buu * e[14] + bud * e[15]);
+                FbQ[FI(k, -i)] += w * (biu * e[9] + bid * e[8] + bqu * e[11] + bqd * e[10] - buu * e[15] + bud * e[14]);
+                FbU[FI(k, i)] += w * (-biu * e[4] + bid * e[5] - bqu * e[12] - bqd * e[13] + buu * e[16] + bud * e[17]);
+                FbU[FI(k, -i)] += w * (-biu * e[5] + bid * e[4] + bqu * e[13] + bqd * e[12] + buu * e[17] + bud * e[16]);
+                E[0] = biu * a0 + bid * a1;    E[1] = biu * a1 + bid * a0;
+                E[2] = -buu * a0 + bud * a1;   E[3] = buu * a1 - bud * a0;
+                E[4] = -biu * a4 + bid * a5;   E[5] = -biu * a5 + bid * a4;
+                E[6] = bqu * a0 + bqd * a1;    E[7] = bqu * a1 + bqd * a0;
+                E[8] = biu * a2_ + bid * a3;   E[9] = biu * a3 + bid * a2_;
+                E[10] = bqu * a2_ + bqd * a3;  E[11] = bqu * a3 + bqd * a2_;
+                E[12] = -bqu * a4 + bqd * a5;  E[13] = bqu * a5 - bqd * a4;
+                E[14] = -buu * a2_ + bud * a3; E[15] = -buu * a3 + bud * a2_;
+                E[16] = buu * a4 + bud * a5;   E[17] = buu * a5 + bud * a4;
+                double sx = 0.0, sy = 0.0;
+                for (int q = 0; q < 18; ++q) { sx += c[q] * E[q]; sy += r[q] * E[q]; }
+                gx[k] += w * sx;
+                gy[k] += w * sy;
+            }
+        }
+    }
+}
+
+/* Direct sunglint of the result (rho_I, rho_Q, rho_U) contributions. */
+static void glint3(const ocrt_instance_t *in, const med_t *M, double ws, double tau, double *g)
+{
+    const double muv = M->muv, phv = (in->raa_deg - 180.0) * PI / 180.0;
+    double rg[3];
+    sunglint_rho(in->vza_deg > 0 ? muv : 1.0, phv, M->mus, ws, in->sigma_type, in->n_water, tau,
+                 in->q_convention, rg);
+    g[0] = rg[0]; g[1] = rg[1]; g[2] = -rg[2];
+}
+
+int oracle_adjoint(const ocrt_instance_t *in, const double *gout, double *g_h, double *g_x, double *g_y,
+                   double *g_wind)
+{
+    if (in->ipss) return -1;
+    solve_t s;
+    int rc = solve_core(in, &s);
+    if (rc) { solve_free(&s); return rc; }
+    const med_t *M = &s.M;
+    const int n = M->n, nt = M->nt, dirs = M->dirs, mm = in->m_max;
+    const size_t cnt = (size_t)(nt + 1) * dirs;
+    const double mus = M->mus, phi = in->raa_deg * PI / 180.0 + PI;
+    for (int k = 0; k <= nt; ++k) g_h[k] = g_x[k] = g_y[k] = 0.0;
+    *g_wind = 0.0;
+    const int surf_aware = (in->surface == OCRT_SURFACE_BLACK_FRESNEL_OCEAN || in->surface == OCRT_SURFACE_FLAT);
+    const int flat = in->surface == OCRT_SURFACE_FLAT ||
+                     (in->surface == OCRT_SURFACE_BLACK_FRESNEL_OCEAN && in->wind_speed <= 0.0);
+    const int kind = surf_aware ? (flat ? 1 : 2) : 0;
+    /* surface kernels and their wind derivative (central difference in wind) */
+    const size_t rsz = (size_t)(mm + 1) * n * n * 9, ssz = (size_t)(mm + 1) * n * 9;
+    double *Rall = NULL, *Rsun = NULL, *dRall = NULL, *dRsun = NULL, *tmp = NULL;
+    if (kind == 2) {
+        Rall = malloc(sizeof(double) * rsz); dRall = malloc(sizeof(double) * rsz);
+        Rsun = malloc(sizeof(double) * ssz); dRsun = malloc(sizeof(double) * ssz);
+        tmp = malloc(sizeof(double) * (rsz + ssz));
+        const double smu = mus, ws = in->wind_speed, dw = 1e-5 * ws;
+        oracle_coxmunk_fourier(&M->rm[1], n, &M->rm[1], n, mm, 1024, ws, in->sigma_type, in->n_water,
+                               in->q_convention, Rall);
+        oracle_coxmunk_fourier(&M->rm[1], n, &smu, 1, mm, 1024, ws, in->sigma_type, in->n_water,
+                               in->q_convention, Rsun);
+        oracle_coxmunk_fourier(&M->rm[1], n, &M->rm[1], n, mm, 1024, ws + dw, in->sigma_type, in->n_water,
+                               in->q_convention, dRall);
+        oracle_coxmunk_fourier(&M->rm[1], n, &M->rm[1], n, mm, 1024, ws - dw, in->sigma_type, in->n_water,
+                               in->q_convention, tmp);
+        for (size_t e = 0; e < rsz; ++e) dRall[e] = (dRall[e] - tmp[e]) / (2.0 * dw);
+        oracle_coxmunk_fourier(&M->rm[1], n, &smu, 1, mm, 1024, ws + dw, in->sigma_type, in->n_water,
+                               in->q_convention, dRsun);
+        oracle_coxmunk_fourier(&M->rm[1], n, &smu, 1, mm, 1024, ws - dw, in->sigma_type, in->n_water,
+                               in->q_convention, tmp);
+        for (size_t e = 0; e < ssz; ++e) dRsun[e] = (dRsun[e] - tmp[e]) / (2.0 * dw);
+    }
+    const size_t bcount = (size_t)(M->lmax + 1) * dirs, kcount = (size_t)(n + 1) * dirs;
+    ang_t Ast;
+    ang_t *const A = &Ast;
+    double *abuf = calloc(3 * bcount + 6 * kcount, sizeof(double));
+    A->plm = abuf; A->rrl = A->plm + bcount; A->rtl = A->rrl + bcount;
+    A->P = A->rtl + bcount; A->GR = A->P + kcount; A->GT = A->GR + kcount;
+    A->ARR = A->GT + kcount; A->ART = A->ARR + kcount; A->ATT = A->ART + kcount;
+    int maxN = 1;
+    for (int m = 0; m <= mm; ++m) if (s.orders[m] > maxN) maxN = s.orders[m];
+    /* per order: field u (3 cnt), source J (3 cnt); work: lam, Sbar, Fbar, seeds */
+    double *U = calloc((size_t)(maxN + 1) * 6 * cnt + 15 * cnt + 6 * (size_t)n, sizeof(double));
+    if (!U || !abuf) { free(U); free(abuf); free(Rall); free(Rsun); free(dRall); free(dRsun); free(tmp);
+                       solve_free(&s); return -3; }
+    double *lam = U + (size_t)(maxN + 1) * 6 * cnt;
+    double *sb3 = lam + 3 * cnt, *fb3 = sb3 + 3 * cnt, *f3 = fb3 + 3 * cnt, *d3 = f3 + 3 * cnt;
+    double *bc3 = d3 + 3 * cnt;                      /* [3][n] */
+    double *bcb3 = bc3 + 3 * n;                      /* [3][n] */
+    double wt[3];
+    for (int m = 0; m <= mm && rc == 0; ++m) {
+        oracle_basis(M->lmax, m, dirs, M->rm_s, A->plm, A->rrl, A->rtl);
+        build_tables(M, m, A);
+        const double W = (kind == 1) ? 2.0 * PI : ((m == 0) ? 2.0 * PI : PI);
+        const double *Rm = kind == 2 ? Rall + (size_t)m * n * n * 9 : NULL;
+        const double *dRm = kind == 2 ? dRall + (size_t)m * n * n * 9 : NULL;
+        const double *Rs = kind == 2 ? Rsun + (size_t)m * n * 9 : NULL;
+        const double *dRs = kind == 2 ? dRsun + (size_t)m * n * 9 : NULL;
+        wt[0] = wt[1] = (m == 0) ? 1.0 : 2.0 * cos(m * phi);
+        wt[2] = (m == 0) ? 0.0 : 2.0 * sin(m * phi);
+        /* ---- forward with storage: u_o at U + o*6cnt, J_o right after */
+#define UF(o, c) (U + (size_t)(o) * 6 * cnt + (size_t)(c) * cnt)
+#define JF(o, c) (U + (size_t)(o) * 6 * cnt + (size_t)(3 + (c)) * cnt)
+        memset(U, 0, sizeof(double) * (size_t)(maxN + 1) * 6 * cnt);
+        memset(f3, 0, sizeof(double) * 3 * cnt);
+        memset(d3, 0, sizeof(double) * 3 * cnt);
+        primary_source(M, m, A, JF(1, 0), JF(1, 1), JF(1, 2));
+        double fl_RI = 0, fl_RQ = 0, fl_sb = 0;
+        if (kind == 1) {
+            /* flat seed source, as in sos_loop */
+            const double nw = in->n_water, n2 = nw * nw;
+            const double tn = sqrt(n2 - 1.0 + mus * mus);
+            const double rp = (n2 * mus - tn) / (n2 * mus + tn), rs = (mus - tn) / (mus + tn);
+            fl_RI = (rp * rp + rs * rs) / 2.0;
+            fl_RQ = (in->q_convention == 1) ? (rp * rp - rs * rs) / 2.0 : (rs * rs - rp * rp) / 2.0;
+            fl_sb = 0.5 * exp(-2.0 * M->h[nt] / mus);
+            const double bP = XPL(0), bR = XRL(0), b0 = (m == 0) ? M->beta0 : 0.0;
+            const int mol = (m <= 2);
+            for (int L = 0; L < nt; ++L) {
+                const double yt = M->yd[L], xt = M->xd[L], yb = M->yd[L + 1], xb = M->xd[L + 1];
+                const double bt = fl_sb * exp(M->h[L] / mus), bb = fl_sb * exp(M->h[L + 1] / mus);
+                for (int d = 1; d <= n; ++d) {
+                    const double pu = XPL(d), pd = XPL(-d), ru = XRL(d), rd = XRL(-d), tu = XTL(d), td = XTL(-d);
+                    const double kPd = TB(A->P, 0, -d) * xt + (mol ? b0 + M->beta2 * pd * bP : 0.0) * yt;
+                    const double kRd = TB(A->GR, 0, -d) * xt + (mol ? M->gamma2 * rd * bP : 0.0) * yt;
+                    const double kTd = TB(A->GT, 0, -d) * xt + (mol ? M->gamma2 * td * bP : 0.0) * yt;
+                    const double kRTd = moment_gr(M, A, m, -d, 0) * xt + (mol ? M->gamma2 * bR * pd : 0.0) * yt;
+                    const double kRRd = TB(A->ARR, 0, -d) * xt + (mol ? M->alpha2 * bR * rd : 0.0) * yt;
+                    const double kTRd = moment_art(M, A, m, -d, 0) * xt + (mol ? M->alpha2 * td * bR : 0.0) * yt;
+                    const double kPu = TB(A->P, 0, d) * xb + (mol ? b0 + M->beta2 * pu * bP : 0.0) * yb;
+                    const double kRu = TB(A->GR, 0, d) * xb + (mol ? M->gamma2 * ru * bP : 0.0) * yb;
+                    const double kTu = TB(A->GT, 0, d) * xb + (mol ? M->gamma2 * tu * bP : 0.0) * yb;
+                    const double kRTu = moment_gr(M, A, m, d, 0) * xb + (mol ? M->gamma2 * bR * pu : 0.0) * yb;
+                    const double kRRu = TB(A->ARR, 0, d) * xb + (mol ? M->alpha2 * bR * ru : 0.0) * yb;
+                    const double kTRu = moment_art(M, A, m, d, 0) * xb + (mol ? M->alpha2 * tu * bR : 0.0) * yb;
+                    f3[FI(L, d)] = bt * (fl_RI * kPd + fl_RQ * kRTd);
+                    f3[cnt + FI(L, d)] = bt * (fl_RI * kRd + fl_RQ * kRRd);
+                    f3[2 * cnt + FI(L, d)] = bt * (fl_RI * kTd + fl_RQ * kTRd);
+                    f3[FI(L + 1, -d)] = bb * (fl_RI * kPu + fl_RQ * kRTu);
+                    f3[cnt + FI(L + 1, -d)] = bb * (fl_RI * kRu + fl_RQ * kRRu);
+                    f3[2 * cnt + FI(L + 1, -d)] = bb * (fl_RI * kTu + fl_RQ * kTRu);
+                }
+            }
+        }
+        /* order-1 source = primary + flat seed source (the sweep is linear) */
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < cnt; ++i) JF(1, c)[i] += f3[c * cnt + i];
+        for (int k = 0; k <= nt; ++k) for (int c = 0; c < 3; ++c) JF(1, c)[FI(k, 0)] = 0.0;
+        for (int c = 0; c < 3; ++c) sweep(M, JF(1, c), NULL, UF(1, c));
+        if (kind == 2) {
+            const double tau = M->h[nt], down = exp(-tau / mus);
+            const double FS = ((m == 0) ? PI : 0.5 * PI) * mus;
+            for (int k = 0; k <= nt; ++k)
+                for (int o = 0; o < n; ++o) {
+                    const double f = FS * (down * exp(-(tau - M->h[k]) / M->rm[o + 1]));
+                    for (int c = 0; c < 3; ++c) {
+                        d3[c * cnt + FI(k, o + 1)] = Rs[o * 9 + 3 * c] * f;
+                        UF(1, c)[FI(k, o + 1)] += d3[c * cnt + FI(k, o + 1)];
+                    }
+                }
+        }
+        const int N = s.orders[m];
+        for (int it = 2; it <= N; ++it) {
+            apply_operator(M, m, A, UF(it - 1, 0), UF(it - 1, 1), UF(it - 1, 2), JF(it, 0), JF(it, 1), JF(it, 2));
+            for (int k = 0; k <= nt; ++k) for (int c = 0; c < 3; ++c) JF(it, c)[FI(k, 0)] = 0.0;
+            const double *bptr[3] = {NULL, NULL, NULL};
+            if (kind != 0) {
+                for (int o = 1; o <= n; ++o) {
+                    double sv[3] = {0, 0, 0};
+                    if (kind == 1) {
+                        double F[9];
+                        fresnel_R(M->rm[o], in->n_water, in->q_convention, F);
+                        const double a = UF(it - 1, 0)[FI(nt, -o)], b = UF(it - 1, 1)[FI(nt, -o)],
+                                     c = UF(it - 1, 2)[FI(nt, -o)];
+                        /* flat: diagonal discrete delta, W mu w (1 / (2 pi mu w)) F = F */
+                        if (M->gb[o] == 0.0) {
+                            for (int r = 0; r < 3; ++r) sv[r] = F[3 * r] * a + F[3 * r + 1] * b + F[3 * r + 2] * c;
+                        } else {
+                            const double dd = 1.0 / (2.0 * PI * M->rm[o] * M->gb[o]), pw = W * M->rm[o] * M->gb[o];
+                            for (int r = 0; r < 3; ++r)
+                                sv[r] = pw * (dd * F[3 * r] * a + dd * F[3 * r + 1] * b + dd * F[3 * r + 2] * c);
+                        }
+                    } else {
+                        for (int i = 1; i <= n; ++i) {
+                            const double w = M->gb[i];
+                            if (w == 0.0) continue;
+                            const double pw = W * M->rm[i] * w;
+                            const double *R = Rm + ((size_t)(o - 1) * n + (i - 1)) * 9;
+                            for (int r = 0; r < 3; ++r)
+                                sv[r] += pw * (R[3 * r] * UF(it - 1, 0)[FI(nt, -i)] + R[3 * r + 1] * UF(it - 1, 1)[FI(nt, -i)] +
+                                               R[3 * r + 2] * UF(it - 1, 2)[FI(nt, -i)]);
+                        }
+                    }
+                    for (int r = 0; r < 3; ++r) bc3[r * n + o - 1] = sv[r];
+                }
+                bptr[0] = bc3; bptr[1] = bc3 + n; bptr[2] = bc3 + 2 * n;
+            }
+            for (int c = 0; c < 3; ++c) sweep(M, JF(it, c), bptr[c], UF(it, c));
+        }
+        /* ---- backward */
+        double ev[3];
+        for (int c = 0; c < 3; ++c) ev[c] = gout[c] * wt[c] / mus;
+        memset(lam, 0, sizeof(double) * 3 * cnt);
+        for (int c = 0; c < 3; ++c) lam[c * cnt + FI(0, M->vidx)] = ev[c];
+        for (int it = N; it >= 2; --it) {
+            memset(sb3, 0, sizeof(double) * 3 * cnt);
+            memset(bcb3, 0, sizeof(double) * 3 * n);
+            for (int c = 0; c < 3; ++c)
+                sweep_adj(M, lam + c * cnt, UF(it, c), JF(it, c), sb3 + c * cnt, kind ? bcb3 + c * n : NULL, g_h);
+            for (int k = 0; k <= nt; ++k) for (int c = 0; c < 3; ++c) sb3[c * cnt + FI(k, 0)] = 0.0;
+            memset(fb3, 0, sizeof(double) * 3 * cnt);
+            apply_operator_adj(M, m, A, sb3, sb3 + cnt, sb3 + 2 * cnt, UF(it - 1, 0), UF(it - 1, 1), UF(it - 1, 2),
+                               fb3, fb3 + cnt, fb3 + 2 * cnt, g_x, g_y);
+            if (kind != 0) {
+                for (int o = 1; o <= n; ++o) {
+                    const double bb[3] = {bcb3[o - 1], bcb3[n + o - 1], bcb3[2 * n + o - 1]};
+                    if (kind == 1) {
+                        double F[9];
+                        fresnel_R(M->rm[o], in->n_water, in->q_convention, F);
+                        const double sc = (M->gb[o] == 0.0) ? 1.0
+                                          : (W * M->rm[o] * M->gb[o]) / (2.0 * PI * M->rm[o] * M->gb[o]);
+                        for (int c2 = 0; c2 < 3; ++c2)
+                            fb3[c2 * cnt + FI(nt, -o)] += sc * (bb[0] * F[c2] + bb[1] * F[3 + c2] + bb[2] * F[6 + c2]);
+                    } else {
+                        for (int i = 1; i <= n; ++i) {
+                            const double w = M->gb[i];
+                            if (w == 0.0) continue;
+                            const double pw = W * M->rm[i] * w;
+                            const double *R = Rm + ((size_t)(o - 1) * n + (i - 1)) * 9;
+                            const double *dR = dRm + ((size_t)(o - 1) * n + (i - 1)) * 9;
+                            for (int c2 = 0; c2 < 3; ++c2) {
+                                fb3[c2 * cnt + FI(nt, -i)] += pw * (bb[0] * R[c2] + bb[1] * R[3 + c2] + bb[2] * R[6 + c2]);
+                                *g_wind += pw * (bb[0] * dR[c2] + bb[1] * dR[3 + c2] + bb[2] * dR[6 + c2]) *
+                                           UF(it - 1, c2)[FI(nt, -i)];
+                            }
+                        }
+                    }
+                }
+            }
+            for (int c = 0; c < 3; ++c)
+                for (size_t i = 0; i < cnt; ++i) lam[c * cnt + i] = fb3[c * cnt + i];
+            for (int c = 0; c < 3; ++c) lam[c * cnt + FI(0, M->vidx)] += ev[c];
+        }
+        /* order 1: adjoint sweep through the combined primary + flat source */
+        memset(sb3, 0, sizeof(double) * 3 * cnt);
+        for (int c = 0; c < 3; ++c) {
+            /* the sweep part of u_1 excludes the rough seed */
+            double *Fp = fb3 + c * cnt;
+            for (size_t i = 0; i < cnt; ++i) Fp[i] = UF(1, c)[i] - d3[c * cnt + i];
+            sweep_adj(M, lam + c * cnt, Fp, JF(1, c), sb3 + c * cnt, NULL, g_h);
+        }
+        {
+            const double b0 = (m == 0) ? M->beta0 : 0.0;
+            const int mol = (m <= 2);
+            for (int k = 0; k <= nt; ++k) {
+                const double beam = M->ch[k], x = M->xd[k], y = M->yd[k];
+                for (int j = -n; j <= n; ++j) {
+                    if (j == 0) continue;
+                    const double mi = b0 + M->beta2 * XPL(j) * XPL(0);
+                    const double mq = mol ? M->gamma2 * XRL(j) * XPL(0) : 0.0;
+                    const double mu_ = mol ? M->gamma2 * XTL(j) * XPL(0) : 0.0;
+                    const double sI = sb3[FI(k, j)], sQ = sb3[cnt + FI(k, j)], sU = sb3[2 * cnt + FI(k, j)];
+                    g_x[k] += beam * (sI * TB(A->P, 0, j) + sQ * TB(A->GR, 0, j) - sU * TB(A->GT, 0, j));
+                    g_y[k] += beam * (sI * mi + sQ * mq - sU * mu_);
+                    const double vI = beam * (x * TB(A->P, 0, j) + y * mi);
+                    const double vQ = beam * (x * TB(A->GR, 0, j) + y * mq);
+                    const double vU = -beam * (x * TB(A->GT, 0, j) + y * mu_);
+                    g_h[k] += -(sI * vI + sQ * vQ + sU * vU) / mus;
+                }
+            }
+            if (kind == 1) {
+                const double bP = XPL(0), bR = XRL(0);
+                for (int L = 0; L < nt; ++L) {
+                    const double bt = fl_sb * exp(M->h[L] / mus), bb = fl_sb * exp(M->h[L + 1] / mus);
+                    for (int d = 1; d <= n; ++d) {
+                        const double pu = XPL(d), pd = XPL(-d), ru = XRL(d), rd = XRL(-d), tu = XTL(d), td = XTL(-d);
+                        /* upward entries at level L use (xt, yt) = level L */
+                        const double axI = TB(A->P, 0, -d) * fl_RI + moment_gr(M, A, m, -d, 0) * fl_RQ;
+                        const double axQ = TB(A->GR, 0, -d) * fl_RI + TB(A->ARR, 0, -d) * fl_RQ;
+                        const double axU = TB(A->GT, 0, -d) * fl_RI + moment_art(M, A, m, -d, 0) * fl_RQ;
+                        const double ayI = (mol ? b0 + M->beta2 * pd * bP : 0.0) * fl_RI + (mol ? M->gamma2 * bR * pd : 0.0) * fl_RQ;
+                        const double ayQ = (mol ? M->gamma2 * rd * bP : 0.0) * fl_RI + (mol ? M->alpha2 * bR * rd : 0.0) * fl_RQ;
+                        const double ayU = (mol ? M->gamma2 * td * bP : 0.0) * fl_RI + (mol ? M->alpha2 * td * bR : 0.0) * fl_RQ;
+                        const size_t iu = FI(L, d);
+                        const double su[3] = {sb3[iu], sb3[cnt + iu], sb3[2 * cnt + iu]};
+                        g_x[L] += bt * (su[0] * axI + su[1] * axQ + su[2] * axU);
+                        g_y[L] += bt * (su[0] * ayI + su[1] * ayQ + su[2] * ayU);
+                        const double fu = su[0] * f3[iu] + su[1] * f3[cnt + iu] + su[2] * f3[2 * cnt + iu];
+                        g_h[L] += fu / mus; g_h[nt] += -2.0 * fu / mus;
+                        /* downward entries at level L + 1 use level L + 1 */
+                        const double bxI = TB(A->P, 0, d) * fl_RI + moment_gr(M, A, m, d, 0) * fl_RQ;
+                        const double bxQ = TB(A->GR, 0, d) * fl_RI + TB(A->ARR, 0, d) * fl_RQ;
+                        const double bxU = TB(A->GT, 0, d) * fl_RI + moment_art(M, A, m, d, 0) * fl_RQ;
+                        const double byI = (mol ? b0 + M->beta2 * pu * bP : 0.0) * fl_RI + (mol ? M->gamma2 * bR * pu : 0.0) * fl_RQ;
+                        const double byQ = (mol ? M->gamma2 * ru * bP : 0.0) * fl_RI + (mol ? M->alpha2 * bR * ru : 0.0) * fl_RQ;
+                        const double byU = (mol ? M->gamma2 * tu * bP : 0.0) * fl_RI + (mol ? M->alpha2 * tu * bR : 0.0) * fl_RQ;
+                        const size_t id = FI(L + 1, -d);
+                        const double sd[3] = {sb3[id], sb3[cnt + id], sb3[2 * cnt + id]};
+                        g_x[L + 1] += bb * (sd[0] * bxI + sd[1] * bxQ + sd[2] * bxU);
+                        g_y[L + 1] += bb * (sd[0] * byI + sd[1] * byQ + sd[2] * byU);
+                        const double fd = sd[0] * f3[id] + sd[1] * f3[cnt + id] + sd[2] * f3[2 * cnt + id];
+                        g_h[L + 1] += fd / mus; g_h[nt] += -2.0 * fd / mus;
+                    }
+                }
+            }
+            if (kind == 2) {
+                /* the seed enters u_1 (adjoint lam) and is removed from the output (-e) */
+                const double tau = M->h[nt], down = exp(-tau / mus);
+                const double FS = ((m == 0) ? PI : 0.5 * PI) * mus;
+                for (int k = 0; k <= nt; ++k)
+                    for (int o = 0; o < n; ++o) {
+                        const double mo = M->rm[o + 1];
+                        const double f = FS * (down * exp(-(tau - M->h[k]) / mo));
+                        for (int c = 0; c < 3; ++c) {
+                            double gd = lam[c * cnt + FI(k, o + 1)];
+                            if (k == 0 && o + 1 == M->vidx) gd -= ev[c];
+                            const double dv = d3[c * cnt + FI(k, o + 1)];
+                            g_h[k] += gd * dv / mo;
+                            g_h[nt] += -gd * dv * (1.0 / mus + 1.0 / mo);
+                            *g_wind += gd * dRs[o * 9 + 3 * c] * f;
+                        }
+                    }
+            }
+        }
+#undef UF
+#undef JF
+    }
+    /* direct glint in rho (black_fresnel_ocean, glint not decoupled) */
+    if (in->surface == OCRT_SURFACE_BLACK_FRESNEL_OCEAN && in->decouple_sunglint == 0) {
+        double g[3];
+        const double tau = M->h[nt], muv1 = in->vza_deg > 0 ? M->muv : 1.0;
+        glint3(in, M, in->wind_speed, tau, g);
+        g_h[nt] += -(gout[0] * g[0] + gout[1] * g[1] + gout[2] * g[2]) * (1.0 / mus + 1.0 / muv1);
+        if (in->wind_speed > 0.0) {
+            const double dw = 1e-5 * in->wind_speed;
+            double gp[3], gm[3];
+            glint3(in, M, in->wind_speed + dw, tau, gp);
+            glint3(in, M, in->wind_speed - dw, tau, gm);
+            for (int c = 0; c < 3; ++c) *g_wind += gout[c] * (gp[c] - gm[c]) / (2.0 * dw);
+        }
+    }
+    free(U); free(abuf); free(Rall); free(Rsun); free(dRall); free(dRsun); free(tmp);
+    solve_free(&s);
+    return rc;
 }

@@ -39,6 +39,13 @@ int oracle_solve_lut(const ocrt_instance_t *in, int n_vza, const double *vza,
                      double *T_diff_dn_dir, double *T_sg_up_dir, double *T_total_up_dir,
                      double *T_dir_dn, double *T_diff_dn_hemi);
 
+/* Reverse-mode gradient of gout . (rho_I, rho_Q, rho_U) with respect to the
+ * per-level medium h[k], xdel[k], ydel[k] (k = 0..n_layers) and the wind
+ * speed, IPSS off.  No counterpart in the reference: pinned by central finite
+ * differences of oracle_solve. */
+int oracle_adjoint(const ocrt_instance_t *in, const double *gout, double *g_h, double *g_x, double *g_y,
+                   double *g_wind);
+
 /* OpenMP batch, one instance per thread (rt_io.c:199). */
 int oracle_solve_batch(const ocrt_instance_t *in, int n, ocrt_result_t *out, int n_threads);
 
