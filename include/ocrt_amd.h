@@ -152,6 +152,37 @@ int  ocrt_plan_kernel_times(ocrt_plan *plan, double *gemm_ms, int *gemm_launches
                             double *sweep_ms, int *sweep_launches);
 void ocrt_plan_destroy(ocrt_plan *plan);
 
+/* Plan options for ocrt_plan_create_ex. */
+enum {
+    OCRT_PLAN_ONE_PER_LAUNCH = 1,  /* instances one after another (the plain form) */
+    OCRT_PLAN_ADJOINT = 2          /* keep order checkpoints for ocrt_plan_adjoint (FP64, IPSS off) */
+};
+
+/* ocrt_plan_create with option flags.  No counterpart in the reference; the
+ * adjoint is the Jacobian path of the batched solver (ocrt_adjoint_batch). */
+int ocrt_plan_create_ex(const ocrt_instance_t *inst, int n_inst, int precision, int flags,
+                        ocrt_plan **plan_out);
+
+/* Reverse-mode gradient of sum_i grad_out[i] . (rho_I, rho_Q, rho_U)_i for an
+ * OCRT_PLAN_ADJOINT plan, by the adjoint SOS recursion (orders checkpointed
+ * every 4 and recomputed).  Runs the forward pass itself unless the last
+ * ocrt_plan_execute of a single-chunk plan left its checkpoints.
+ *   grad_out     host [n_inst][3]
+ *   grad_levels  optional host, per instance in order, 3 x (n_layers + 1):
+ *                d/dh[k] (level optical depth from the top), d/dxdel[k],
+ *                d/dydel[k] (aerosol and Rayleigh scattering fractions of
+ *                level k, rt_atm.c:343-360)
+ *   grad_wind    optional host [n_inst], d/d wind speed
+ * The per-layer Jacobian of an explicitly layered instance follows by the
+ * chain rule through the layer-to-level map (ocrt_amd/adjoint.py). */
+int ocrt_plan_adjoint(ocrt_plan *plan, const double *grad_out, double *grad_levels, double *grad_wind,
+                      void *hip_stream);
+
+/* One-call Jacobian form (the boundary's ocrt_adjoint_batch): plan, forward,
+ * adjoint, copy results and gradients, destroy.  precision must be FP64. */
+int ocrt_adjoint_batch(const ocrt_instance_t *inst, int n_inst, int precision, const double *grad_out,
+                       ocrt_result_t *out, double *grad_levels, double *grad_wind, void *hip_stream);
+
 /* One-call form: plan, execute, copy results, destroy. */
 int ocrt_solve_batch(const ocrt_instance_t *inst, int n_inst, int precision,
                      ocrt_result_t *out, void *hip_stream);

@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(_HERE, "libocrt_amd.so")
 _lib = None
 
 PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
+PLAN_ONE_PER_LAUNCH = 1
+PLAN_ADJOINT = 2
 
 
 class OcrtError(RuntimeError):
@@ -54,6 +56,10 @@ def library():
         lib.ocrt_solve_batch.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.POINTER(Result),
                                          C.c_void_p]
         lib.ocrt_version.restype = C.c_char_p
+        lib.ocrt_plan_create_ex.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        lib.ocrt_plan_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ocrt_adjoint_batch.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.c_void_p, C.POINTER(Result),
+                                           C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = lib
     return _lib
 
@@ -71,17 +77,43 @@ class Plan:
     integral for orders >= 2).  one_per_launch=True runs instances one after
     another (the plain one-instance-per-launch form)."""
 
-    def __init__(self, descs, precision="mixed", one_per_launch=False):
+    def __init__(self, descs, precision="mixed", one_per_launch=False, adjoint=False):
         _require_gpu()
         self.lib = library()
         self.n = len(descs)
         self._arr = instance_array(descs)
         self._handle = C.c_void_p()
-        rc = self.lib.ocrt_plan_create(self._arr, self.n, PRECISIONS[precision],
-                                       1 if one_per_launch else 0, C.byref(self._handle))
+        self.n_layers = [d.n_layers for d in descs]
+        flags = (PLAN_ONE_PER_LAUNCH if one_per_launch else 0) | (PLAN_ADJOINT if adjoint else 0)
+        rc = self.lib.ocrt_plan_create_ex(self._arr, self.n, PRECISIONS[precision], flags, C.byref(self._handle))
         if rc != 0:
             raise OcrtError(f"ocrt_plan_create failed with code {rc}")
         self.mmax = self.lib.ocrt_plan_mmax(self._handle)
+
+    def adjoint(self, grad_out, stream=None):
+        """Gradient of sum_i grad_out[i] . (rho_I, rho_Q, rho_U)_i (adjoint plans).
+        Returns per-instance arrays d/dh, d/dxdel, d/dydel over the levels and
+        d/dwind."""
+        g = np.ascontiguousarray(grad_out, dtype=np.float64).reshape(self.n, 3)
+        total = sum(3 * (nt + 1) for nt in self.n_layers)
+        lev = np.zeros(total)
+        wind = np.zeros(self.n)
+        ptr = None
+        if stream is not None:
+            ptr = C.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+        rc = self.lib.ocrt_plan_adjoint(self._handle, g.ctypes.data_as(C.c_void_p), lev.ctypes.data_as(C.c_void_p),
+                                        wind.ctypes.data_as(C.c_void_p), ptr)
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_adjoint failed with code {rc}")
+        out = {"grad_h": [], "grad_x": [], "grad_y": [], "grad_wind": wind}
+        off = 0
+        for nt in self.n_layers:
+            L = nt + 1
+            out["grad_h"].append(lev[off:off + L].copy())
+            out["grad_x"].append(lev[off + L:off + 2 * L].copy())
+            out["grad_y"].append(lev[off + 2 * L:off + 3 * L].copy())
+            off += 3 * L
+        return out
 
     def execute(self, stream=None):
         ptr = None

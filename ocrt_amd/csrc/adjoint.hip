@@ -1,0 +1,367 @@
+// Adjoint of the SOS solve (fp64).  For one Fourier mode the forward solve is
+//   u_1 = sweep(S_1) (+ rough-surface seed),
+//   u_o = sweep(M u_{o-1}, R B u_{o-1}),  o = 2..N,
+//   X   = sum_o u_o at the TOA view node (minus the rough seed there),
+// linear in the fields, so the adjoint runs the order recursion backward:
+//   lambda_N = e,  lambda_{o-1} = e + M^T Sbar_o + B^T R^T bbar_o,
+// where e carries the reconstruction weights of the TOA view entries and
+// (Sbar_o, bbar_o) is the adjoint sweep of lambda_o.  Orders are
+// checkpointed every 4 and recomputed segment by segment (plan.hip).
+// Gradients accumulate per instance with respect to the level optical depths
+// (explicit dependence gH, layer steps gD), the scattering fractions gX, gY
+// and the wind speed gW.  The reference has no adjoint; the finite-difference
+// checks of tests/ and the oracle's reverse mode pin this one.
+#include "ocrt_amd.h"
+#include "sos_common.h"
+
+namespace ocrt {
+
+__global__ void fill_f64(double *__restrict__ p, long long n, double v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Reconstruction weight of mode m for Stokes component c over 1 / mu_s
+// (rt_fourier.c:9-33, rho = X / mu_s).
+__device__ inline double seed_weight(const DevInst &I, const Pools &P, int m, int c)
+{
+    const double phi = I.raa * OCRT_PI / 180.0 + OCRT_PI;
+    double wt;
+    if (c < 2) wt = (m == 0) ? 1.0 : 2.0 * cos(m * phi);
+    else wt = (m == 0) ? 0.0 : 2.0 * sin(m * phi);
+    return P.gout[3LL * I.gidx + c] * wt / I.mus;
+}
+
+// lambda += e at the TOA view entries of one item.
+__device__ inline void add_seed(const DevInst &I, const DevItem &it, const Pools &P)
+{
+    double *lam = P.lam + it.off_F;
+    for (int c = 0; c < 3; ++c) lam[2 * c * it.n + I.vidx] += seed_weight(I, P, it.m, c);
+}
+
+// Zero the instance accumulators (one thread per entry of one instance).
+__global__ void adj_zero_grad(const DevInst *__restrict__ inst, Pools P)
+{
+    const DevInst &I = inst[blockIdx.y];
+    const int L = I.nt + 1, cnt = 4 * L;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += gridDim.x * blockDim.x)
+        P.grad[I.off_grad + e] = 0.0;
+}
+
+__global__ void adj_seed(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P)
+{
+    const DevItem &it = items[blockIdx.x];
+    if (threadIdx.x == 0) add_seed(inst[it.inst], it, P);
+}
+
+// Bottom downward field of the field in P.F into bc + 3n (order `order`
+// reads the field of order - 1).
+__global__ void adj_bottom(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P,
+                           const ItemState *__restrict__ st, int order)
+{
+    const DevItem &it = items[blockIdx.x];
+    if (!item_runs(st[blockIdx.x], order)) return;
+    const int n = it.n;
+    const double *F = reinterpret_cast<const double *>(P.F) + it.off_F + (long long)it.nt * it.P;
+    for (int q = threadIdx.x; q < 3 * n; q += blockDim.x) {
+        const int c = q / n, o = q % n;
+        P.bc[it.off_bc + 3 * n + q] = F[(2 * c + 1) * n + o];
+    }
+}
+
+__global__ void adj_surface_bc(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P,
+                               const ItemState *__restrict__ st, int order)
+{
+    const DevItem &it = items[blockIdx.x];
+    const DevInst &I = inst[it.inst];
+    if (!item_runs(st[blockIdx.x], order) || I.surf == 0) return;
+    surface_bc_item(I, it, P);
+}
+
+// Sums of v over the upward-direction lanes and over the downward ones.
+__device__ inline void hemi_sums(double v, int hemi, double &su, double &sd)
+{
+    su = wave_sum(hemi == 0 ? v : 0.0);
+    sd = wave_sum(hemi == 1 ? v : 0.0);
+}
+
+// Adjoint sweep of order `order` >= 2.  P.F holds u_o, P.J / P.J2 the
+// unscaled operator parts A_0 u_{o-1} / A_1 u_{o-1}, P.lam lambda_o.  Writes
+// Sbar_o over P.J, bbar_o over the boundary slots of P.bc, and accumulates
+// gX, gY (Sbar . part) per level and gD per layer.  One thread per column,
+// one 64-column tile per wavefront, walking each column against its forward
+// direction; at every step the upward lanes share one level and the downward
+// lanes another, so the per-level sums are two wavefront reductions.
+__global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ inst, const DevItem *__restrict__ items,
+                                                 Pools P, const ItemState *__restrict__ st,
+                                                 const ColTile *__restrict__ tiles, int n_tiles, int order)
+{
+    const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (tix >= n_tiles) return;
+    const ColTile ct = tiles[tix];
+    if (ct.item < 0 || !item_runs(st[ct.item], order)) return;
+    const DevItem &it = items[ct.item];
+    const DevInst &I = inst[it.inst];
+    const int n = it.n, nt = it.nt, Pp = it.P, L = nt + 1;
+    const int nparts = it.K / Pp, kind0 = it.parts & 0xff;
+    const int col = ct.col0 + lane;
+    const bool live = col < 6 * n;
+    const int type = live ? col / n : 0, c = type >> 1, hemi = type & 1, o = live ? col % n : 0;
+    const LevelView lv(P, I);
+    const double *U = reinterpret_cast<const double *>(P.F) + it.off_F + col;
+    double *J0 = reinterpret_cast<double *>(P.J) + it.off_F + col;
+    const double *J1 = reinterpret_cast<const double *>(P.J2) + it.off_F + col;
+    const double *lam = P.lam + it.off_F + col;
+    double *gH = P.grad + I.off_grad, *gX = gH + L, *gY = gX + L, *gD = gY + L;
+    (void)gH;
+    const double amu = P.ring[I.off_ring + o];
+    auto parts_at = [&](int k, double &p0, double &p1) {
+        p0 = live ? J0[(long long)k * Pp] : 0.0;
+        p1 = (live && nparts > 1) ? J1[(long long)k * Pp] : 0.0;
+    };
+    auto source = [&](int k, double p0, double p1) {
+        const double s0 = (kind0 == 1) ? lv.xd[k] : lv.yd[k];
+        return s0 * p0 + lv.yd[k] * p1;
+    };
+    // d(Sbar . J) / d xdel and / d ydel at one level
+    auto level_terms = [&](double sbar, double p0, double p1, double &vx, double &vy) {
+        if (kind0 == 1) { vx = sbar * p0; vy = sbar * p1; }
+        else { vx = 0.0; vy = sbar * p0; }
+    };
+    const int k0 = hemi ? nt : 0;
+    double rho = live ? lam[(long long)k0 * Pp] : 0.0;
+    double pk0, pk1;
+    parts_at(k0, pk0, pk1);
+    double sk = source(k0, pk0, pk1);
+    double pending = 0.0;
+    for (int t = 0; t < nt; ++t) {
+        const int k = hemi ? nt - t : t;
+        const int kp = hemi ? k - 1 : k + 1;
+        double pp0, pp1;
+        parts_at(kp, pp0, pp1);
+        const double sp = source(kp, pp0, pp1);
+        double T, al, be, dT, dal, dbe;
+        step_coefs(fabs(lv.h[k] - lv.h[kp]), amu, T, al, be, dT, dal, dbe);
+        const double sbar = pending + al * rho;
+        pending = be * rho;
+        const double up = live ? U[(long long)kp * Pp] : 0.0;
+        const double dd = rho * (dT * up + dal * sk + dbe * sp);
+        if (live) J0[(long long)k * Pp] = sbar;
+        double vx, vy;
+        level_terms(sbar, pk0, pk1, vx, vy);
+        double xu, xd, yu, yd, du, dn;
+        hemi_sums(vx, hemi, xu, xd);
+        hemi_sums(vy, hemi, yu, yd);
+        hemi_sums(dd, hemi, du, dn);
+        if (lane == 0) {
+            // upward lanes sit at level t (layer t), downward ones at nt - t (layer nt - t - 1)
+            if (xu != 0.0) atomicAdd(&gX[t], xu);
+            if (yu != 0.0) atomicAdd(&gY[t], yu);
+            if (du != 0.0) atomicAdd(&gD[t], du);
+            if (xd != 0.0) atomicAdd(&gX[nt - t], xd);
+            if (yd != 0.0) atomicAdd(&gY[nt - t], yd);
+            if (dn != 0.0) atomicAdd(&gD[nt - t - 1], dn);
+        }
+        rho = (live ? lam[(long long)kp * Pp] : 0.0) + T * rho;
+        pk0 = pp0; pk1 = pp1; sk = sp;
+    }
+    // last level of the walk: nt for upward columns, 0 for downward ones
+    const int kl = hemi ? 0 : nt;
+    if (live) J0[(long long)kl * Pp] = pending;
+    double vx, vy;
+    level_terms(pending, pk0, pk1, vx, vy);
+    double xu, xd, yu, yd;
+    hemi_sums(vx, hemi, xu, xd);
+    hemi_sums(vy, hemi, yu, yd);
+    if (lane == 0) {
+        if (xu != 0.0) atomicAdd(&gX[nt], xu);
+        if (yu != 0.0) atomicAdd(&gY[nt], yu);
+        if (xd != 0.0) atomicAdd(&gX[0], xd);
+        if (yd != 0.0) atomicAdd(&gY[0], yd);
+    }
+    // the upward columns start from the boundary condition: its adjoint is rho
+    if (live && hemi == 0 && I.surf != 0) P.bc[it.off_bc + c * n + o] = rho;
+}
+
+// After M^T Sbar_o has been written to P.lam: lambda_{o-1} += e (TOA view
+// entries) + B^T R^T bbar_o (bottom downward entries), and the wind gradient
+// bbar_o . dR/dwind . u_{o-1}(bottom), with P.F = u_{o-1}.  One workgroup
+// (64 threads) per item.
+__global__ void surface_adj(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P,
+                            const ItemState *__restrict__ st, int order)
+{
+    const DevItem &it = items[blockIdx.x];
+    if (!item_runs(st[blockIdx.x], order)) return;
+    const DevInst &I = inst[it.inst];
+    if (threadIdx.x == 0) add_seed(I, it, P);
+    if (I.surf == 0) return;
+    const int n = it.n, nt = it.nt, m = it.m, L = nt + 1;
+    const double *mu = P.ring + I.off_ring, *w = mu + n;
+    const double *bb = P.bc + it.off_bc;   // [3][n] adjoint of the boundary condition
+    double *lamb = P.lam + it.off_F + (long long)nt * it.P;
+    const double *Ub = reinterpret_cast<const double *>(P.F) + it.off_F + (long long)nt * it.P;
+    const double W = (I.surf == 1) ? 2.0 * OCRT_PI : ((m == 0) ? 2.0 * OCRT_PI : OCRT_PI);
+    double gw = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        if (I.surf == 1) {
+            double Fr[9];
+            fresnel_R(mu[i], I.n_water, I.q_conv, Fr);
+            const double f = (w[i] == 0.0) ? 1.0
+                             : (W * mu[i] * w[i]) * (1.0 / (2.0 * OCRT_PI * mu[i] * w[i]));
+            for (int c2 = 0; c2 < 3; ++c2)
+                acc[c2] = f * (bb[i] * Fr[c2] + bb[n + i] * Fr[3 + c2] + bb[2 * n + i] * Fr[6 + c2]);
+        } else if (w[i] != 0.0) {
+            const double pw = W * mu[i] * w[i];
+            for (int op = 0; op < n; ++op) {
+                const double *R = P.R + I.off_R + (((long long)m * n + op) * n + i) * 9;
+                const double *dR = P.dR + I.off_R + (((long long)m * n + op) * n + i) * 9;
+                const double b0 = bb[op], b1 = bb[n + op], b2 = bb[2 * n + op];
+                for (int c2 = 0; c2 < 3; ++c2) {
+                    acc[c2] += pw * (b0 * R[c2] + b1 * R[3 + c2] + b2 * R[6 + c2]);
+                    gw += pw * (b0 * dR[c2] + b1 * dR[3 + c2] + b2 * dR[6 + c2]) * Ub[(2 * c2 + 1) * n + i];
+                }
+            }
+        }
+        for (int c2 = 0; c2 < 3; ++c2) lamb[(2 * c2 + 1) * n + i] += acc[c2];
+    }
+    if (I.surf == 2) {
+        gw = wave_sum(gw);
+        if (threadIdx.x == 0 && gw != 0.0)
+            atomicAdd(&P.grad[I.off_grad + 4 * L - 1], gw * slope_variance_dwind(I.wind, I.sigma_type));
+    }
+}
+
+// Adjoint of order 1 with P.F = u_1 and P.lam = lambda_1: the adjoint sweep of
+// the combined single-scattering and flat-seed source, the explicit
+// dependence of that source on xdel, ydel and h, and the rough direct-beam
+// seed (which enters u_1 and leaves the TOA output).
+__global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__ inst,
+                                                     const DevItem *__restrict__ items, Pools P,
+                                                     const ColTile *__restrict__ tiles, int n_tiles)
+{
+    const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (tix >= n_tiles) return;
+    const ColTile ct = tiles[tix];
+    if (ct.item < 0) return;
+    const DevItem &it = items[ct.item];
+    const DevInst &I = inst[it.inst];
+    const int n = it.n, nt = it.nt, Pp = it.P, L = nt + 1, m = it.m;
+    const int col = ct.col0 + lane;
+    const bool live = col < 6 * n;
+    const int type = live ? col / n : 0, c = type >> 1, hemi = type & 1, o = live ? col % n : 0;
+    const LevelView lv(P, I);
+    const ColumnSource src(I, it, P, lv, live ? col : 0);
+    const double mus = I.mus, amu = P.ring[I.off_ring + o];
+    const double *U = reinterpret_cast<const double *>(P.F) + it.off_F + col;
+    const double *lam = P.lam + it.off_F + col;
+    double *gH = P.grad + I.off_grad, *gX = gH + L, *gY = gX + L, *gD = gY + L, *gW = gD + nt;
+    // rough direct-beam seed of the upward columns
+    const bool rough = live && I.surf == 2 && hemi == 0;
+    const double tau = lv.h[nt], down = exp(-tau / mus);
+    const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * mus;
+    const double rsun = rough ? P.R[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c] : 0.0;
+    const double drsun = rough ? P.dR[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c] : 0.0;
+    const double e_here = (live && hemi == 0 && o == I.vidx) ? seed_weight(I, P, m, c) : 0.0;
+    auto seed_f = [&](int k) { return FS * (down * exp(-(tau - lv.h[k]) / amu)); };
+    auto S = [&](int k) { return live ? src.primary(lv, k) + src.flat_value(lv, mus, k) : 0.0; };
+    auto R = [&](int k) { return live ? U[(long long)k * Pp] - (rough ? rsun * seed_f(k) : 0.0) : 0.0; };
+    double g_nt = 0.0, g_w = 0.0;   // per-lane d/dh[nt] and d/dsigma^2 terms
+    // explicit level terms of the source and the seed at level k for adjoint sbar
+    auto level_terms = [&](int k, double sbar, double &vx, double &vy, double &vh) {
+        vx = vy = vh = 0.0;
+        if (!live) return;
+        const double ch = lv.ch[k];
+        vx = sbar * src.sgn * ch * src.pc;
+        vy = sbar * src.sgn * ch * src.mc;
+        vh = -sbar * src.primary(lv, k) / mus;
+        if (src.flat && !src.flat_zero(k)) {
+            const double b = src.flat_beam(lv, mus, k);
+            vx += sbar * b * (src.RI * src.pA + src.RQ * src.pB);
+            vy += sbar * b * (src.RI * src.mA + src.RQ * src.mB);
+            const double fv = sbar * src.flat_value(lv, mus, k);
+            vh += fv / mus;
+            g_nt += -2.0 * fv / mus;
+        }
+        if (rough) {
+            const double gd = lam[(long long)k * Pp] - (k == 0 ? e_here : 0.0);
+            const double f = seed_f(k), dv = rsun * f;
+            vh += gd * dv / amu;
+            g_nt += -gd * dv * (1.0 / mus + 1.0 / amu);
+            g_w += gd * drsun * f;
+        }
+    };
+    const int k0 = hemi ? nt : 0;
+    double rho = live ? lam[(long long)k0 * Pp] : 0.0;
+    double sk = S(k0);
+    double pending = 0.0;
+    for (int t = 0; t < nt; ++t) {
+        const int k = hemi ? nt - t : t;
+        const int kp = hemi ? k - 1 : k + 1;
+        const double sp = S(kp);
+        double T, al, be, dT, dal, dbe;
+        step_coefs(fabs(lv.h[k] - lv.h[kp]), amu, T, al, be, dT, dal, dbe);
+        const double sbar = pending + al * rho;
+        pending = be * rho;
+        const double dd = rho * (dT * R(kp) + dal * sk + dbe * sp);
+        double vx, vy, vh;
+        level_terms(k, sbar, vx, vy, vh);
+        double xu, xd, yu, yd, hu, hd, du, dn;
+        hemi_sums(vx, hemi, xu, xd);
+        hemi_sums(vy, hemi, yu, yd);
+        hemi_sums(vh, hemi, hu, hd);
+        hemi_sums(dd, hemi, du, dn);
+        if (lane == 0) {
+            if (xu != 0.0) atomicAdd(&gX[t], xu);
+            if (yu != 0.0) atomicAdd(&gY[t], yu);
+            if (hu != 0.0) atomicAdd(&gH[t], hu);
+            if (du != 0.0) atomicAdd(&gD[t], du);
+            if (xd != 0.0) atomicAdd(&gX[nt - t], xd);
+            if (yd != 0.0) atomicAdd(&gY[nt - t], yd);
+            if (hd != 0.0) atomicAdd(&gH[nt - t], hd);
+            if (dn != 0.0) atomicAdd(&gD[nt - t - 1], dn);
+        }
+        rho = (live ? lam[(long long)kp * Pp] : 0.0) + T * rho;
+        sk = sp;
+    }
+    double vx, vy, vh;
+    level_terms(hemi ? 0 : nt, pending, vx, vy, vh);
+    double xu, xd, yu, yd, hu, hd;
+    hemi_sums(vx, hemi, xu, xd);
+    hemi_sums(vy, hemi, yu, yd);
+    hemi_sums(vh, hemi, hu, hd);
+    const double snt = wave_sum(g_nt), sw = wave_sum(g_w);
+    if (lane == 0) {
+        if (xu != 0.0) atomicAdd(&gX[nt], xu);
+        if (yu != 0.0) atomicAdd(&gY[nt], yu);
+        if (hu != 0.0) atomicAdd(&gH[nt], hu);
+        if (xd != 0.0) atomicAdd(&gX[0], xd);
+        if (yd != 0.0) atomicAdd(&gY[0], yd);
+        if (hd != 0.0) atomicAdd(&gH[0], hd);
+        if (snt != 0.0) atomicAdd(&gH[nt], snt);
+        if (sw != 0.0) atomicAdd(gW, sw * slope_variance_dwind(I.wind, I.sigma_type));
+    }
+}
+
+// Direct glint in rho (black_fresnel_ocean with the glint coupled): its
+// dependence on the column optical depth and the wind.  One thread per instance.
+__global__ void glint_adj(const DevInst *__restrict__ inst, int n_inst, Pools P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    const DevInst &I = inst[i];
+    if (I.surface_kind != OCRT_SURFACE_BLACK_FRESNEL_OCEAN || I.decouple) return;
+    const LevelView lv(P, I);
+    const int nt = I.nt, L = nt + 1;
+    const double muv1 = I.vza > 0.0 ? I.muv : 1.0, phv = (I.raa - 180.0) * OCRT_PI / 180.0;
+    double rg[3], dlog;
+    sunglint_rho(muv1, phv, I.mus, I.wind, I.sigma_type, I.n_water, lv.h[nt], I.q_conv, rg, &dlog);
+    const double *g = P.gout + 3LL * I.gidx;
+    const double v = g[0] * rg[0] + g[1] * rg[1] - g[2] * rg[2];
+    double *gH = P.grad + I.off_grad;
+    gH[nt] += -v * (1.0 / I.mus + 1.0 / muv1);
+    if (I.wind >= 1e-6) gH[4 * L - 1] += v * dlog * slope_variance_dwind(I.wind, I.sigma_type);
+}
+
+}  // namespace ocrt

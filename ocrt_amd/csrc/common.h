@@ -33,6 +33,8 @@ struct DevInst {
     int surf;            // 0 none, 1 flat, 2 rough
     int sigma_type, q_conv, decouple, ipss, max_orders, item0, surface_kind, surf_owner;
     int layered;         // 1: explicit per-layer optical depths (pool_layers)
+    int gidx;            // index of the instance in the plan
+    int pad2_;
     long long off_lev;   // pool_lev: 5 arrays of L doubles (h, ch, xd, yd, z)
     long long off_ring;  // pool_ring: mu[n], w[n]
     long long off_T;     // pool_T (double) / pool_Tf (float): [nt][n]
@@ -40,6 +42,7 @@ struct DevInst {
     long long off_R;     // pool_R: [(mmax+1)][n][n][9]
     long long off_Rsun;  // pool_R: [(mmax+1)][n][9]
     long long off_layers;// pool_layers: [4][nt] Rayleigh tau, aerosol tau, aerosol ssa, absorption tau
+    long long off_grad;  // pool_grad: gH[L], gX[L], gY[L], gD[nt], gW[1] (adjoint)
 };
 
 struct DevItem {
@@ -79,10 +82,15 @@ struct InstOut {
 struct Pools {
     double *lev, *ring, *T, *greek, *R, *basis, *tab, *tot, *bc;
     double *layers;       // explicit layering of the instances that carry one
+    double *dR;           // adjoint: d R^m / d sigma^2, layout of R
+    double *lam;          // adjoint: [L][P] per item, adjoint of the current order's field
+    double *grad;         // adjoint: per-instance accumulators (see DevInst::off_grad)
+    const double *gout;  // adjoint: upstream gradient [instance][3] of (rho_I, rho_Q, rho_U)
     float *comp;          // fp32 compensation of the mixed-form order sum on the two output rows
     float *Tf;            // fp32 1 - T for the mixed-form sweeps
     float *idt;           // fp32 1 / dtau per layer, indexed like lev (mixed-form sweeps)
     void *A, *F, *J;      // double or bf16/float depending on the form
+    void *J2;             // second output of a parts-split scattering integral (adjoint)
     void *Flo;            // bf16 low half of the field pair (mixed form, fsplit items)
     double *cm_w;         // cos/sin(m phi) table [(Mall+1)][2][kNPhi]
     double *toa_up;       // optional [items][3][n] TOA upward at every node (LUT)
@@ -90,6 +98,10 @@ struct Pools {
 };
 
 struct GemmTile { int item, tm, tn, pad; };   // item < 0: idle slot
+
+// Epilogues of the scattering-integral kernels.
+constexpr int kGemmMix = 0;     // J = xdel A_p F + ydel A_r F
+constexpr int kGemmParts = 1;   // J = A_0 F, J2 = A_1 F (unscaled)
 
 // 64 consecutive state columns of one item: the unit of the per-column kernels
 // (one wavefront each).  item < 0: idle slot.

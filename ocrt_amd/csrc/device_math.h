@@ -127,12 +127,34 @@ __device__ inline double sancer_shadow(double mu_out, double mu_in_dn, double s2
     return 1.0 / (1.0 + li + lv);
 }
 
+// d ln S / d sigma^2 of the Sancer factor S = 1 / (1 + Lambda_i + Lambda_v):
+// Lambda(nu) = (exp(-nu^2) / (nu sqrt(pi)) - erfc(nu)) / 2 with
+// nu = cot(theta) / sigma, so dLambda / dsigma^2 = exp(-nu^2) / (4 sqrt(pi) nu sigma^2).
+__device__ inline double sancer_dlog(double mu_out, double mu_in_dn, double s2v, double shadow)
+{
+    const double sl = sqrt(s2v);
+    double d = 0.0;
+    const double mus[2] = {mu_in_dn, mu_out};
+    for (int q = 0; q < 2; ++q) {
+        const double m = mus[q];
+        if (m < 1.0 - 1e-10) {
+            const double sn = sqrt(fmax(1e-20, 1.0 - m * m));
+            const double nu = (m / sn) / sl;
+            if (nu <= 20.0) d += exp(-nu * nu) / (4.0 * sqrt(OCRT_PI) * nu * s2v);
+        }
+    }
+    return -shadow * d;
+}
+
 // Body of the Cox-Munk Mueller BRDF for a given slope variance and shadowing
-// factor (shadow < 0: computed here).
+// factor (shadow < 0: computed here).  tb2_out (optional) receives tan^2 of
+// the facet tilt, from which d ln(brdf) / d sigma^2 = tb2 / sigma^4 - 1 / sigma^2
+// + d ln S / d sigma^2 follows (-1 when the entry is identically zero).
 __device__ inline void coxmunk_R_core(double mu_out, double mu_in_dn, double cphi, double sphi, double s2v,
-                                      double shadow, double nw, int qc, double *R)
+                                      double shadow, double nw, int qc, double *R, double *tb2_out = nullptr)
 {
     for (int i = 0; i < 9; ++i) R[i] = 0.0;
+    if (tb2_out) *tb2_out = -1.0;
     const double mia = -mu_in_dn;
     const double s1 = sqrt(fmax(0.0, 1.0 - mu_out * mu_out));
     const double s2 = sqrt(fmax(0.0, 1.0 - mia * mia));
@@ -165,6 +187,7 @@ __device__ inline void coxmunk_R_core(double mu_out, double mu_in_dn, double cph
     fresnel_R(co, nw, qc, MF);
     const double cb2 = cb * cb;
     const double tb2 = (1.0 - cb2) / fmax(cb2, 1e-8);
+    if (tb2_out) *tb2_out = tb2;
     const double Ps = (1.0 / (OCRT_PI * s2v)) * exp(-tb2 / s2v);
     const double S = shadow >= 0.0 ? shadow : sancer_shadow(mu_out, mu_in_dn, s2v);
     const double brdf = (Ps * S) / (4.0 * mu_in_dn * mu_out * cb2 * cb2);
@@ -192,6 +215,14 @@ __device__ inline void coxmunk_R(double mu_out, double mu_in_dn, double cphi, do
                                  int sigma_type, double nw, int qc, double *R)
 {
     coxmunk_R_core(mu_out, mu_in_dn, cphi, sphi, slope_variance(ws, sigma_type), -1.0, nw, qc, R);
+}
+
+// d sigma^2 / d wind of slope_variance.
+__device__ inline double slope_variance_dwind(double ws, int sigma_type)
+{
+    if (sigma_type == 1) return ws > 0.01 ? 0.00512 : 0.0;
+    const double s = 0.0731 * sqrt(fmax(0.0, ws));
+    return (s * s > 1e-10) ? 0.0731 * 0.0731 : 0.0;
 }
 
 template <typename T> __device__ inline T to_storage(double v);

@@ -19,20 +19,31 @@ namespace ocrt {
 __global__ void medium_levels(const DevInst *, Pools);
 __global__ void medium_fractions(const DevInst *, Pools);
 __global__ void fourier_weights(double *, int);
-__global__ void coxmunk_fourier(const DevInst *, Pools);
+template <bool DERIV> __global__ void coxmunk_fourier(const DevInst *, Pools);
 __global__ void angular_basis(const DevInst *, const DevItem *, Pools);
 __global__ void phase_tables(const DevInst *, const DevItem *, Pools);
-template <typename AT> __global__ void pack_operator(const DevInst *, const DevItem *, Pools);
+template <typename AT, bool TRANS> __global__ void pack_operator(const DevInst *, const DevItem *, Pools);
 template <typename FT, bool MIXED>
 __global__ void order_one(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int);
 template <typename FT> __global__ void order_status(const DevInst *, const DevItem *, Pools, ItemState *, int);
-__global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *);
-__global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *);
+template <int MODE>
+__global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
+__global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <typename FT, typename JT, bool MIXED>
-__global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int);
+__global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int, int);
 __global__ void extract(const DevInst *, const DevItem *, Pools, const ItemState *, ItemOut *, int);
 __global__ void finalize(const DevInst *, int, const ItemOut *, const InstOut *, const Pools, ocrt_result_t *);
 __global__ void ipss_kappa(const DevInst *, const IpssJob *, Pools, double *, double *);
+// adjoint.hip
+__global__ void fill_f64(double *, long long, double);
+__global__ void adj_zero_grad(const DevInst *, Pools);
+__global__ void adj_seed(const DevInst *, const DevItem *, Pools);
+__global__ void adj_bottom(const DevInst *, const DevItem *, Pools, const ItemState *, int);
+__global__ void adj_surface_bc(const DevInst *, const DevItem *, Pools, const ItemState *, int);
+__global__ void sweep_adj(const DevInst *, const DevItem *, Pools, const ItemState *, const ColTile *, int, int);
+__global__ void surface_adj(const DevInst *, const DevItem *, Pools, const ItemState *, int);
+__global__ void order_one_adj(const DevInst *, const DevItem *, Pools, const ColTile *, int);
+__global__ void glint_adj(const DevInst *, int, Pools);
 }  // namespace ocrt
 
 using namespace ocrt;
@@ -193,7 +204,19 @@ struct ocrt_plan {
     std::vector<hipEvent_t> ev;       // pairs (start, stop): [gemm..., sweep...]
     std::vector<int> ev_kind;         // 0 gemm, 1 sweep, per pair
     size_t ev_used = 0;
+    // adjoint plans: transposed operator, order checkpoints (u_1, u_5, u_9, ...),
+    // the three recomputed fields of a segment, per-instance accumulators
+    int adjoint = 0;
+    int fwd_ready = 0;                // the last execute left this plan's checkpoints intact
+    void *At = nullptr;
+    std::vector<void *> ckpt;
+    void *seg[3] = {nullptr, nullptr, nullptr};
+    double *grad = nullptr, *gout = nullptr;
+    long long grad_n = 0;
 };
+
+// Orders between two checkpoints of the adjoint's forward pass.
+constexpr int kCkptEvery = 4;
 
 static hipEvent_t *ev_pair(ocrt_plan *p, int kind)
 {
@@ -271,16 +294,24 @@ static int validate(const ocrt_instance_t &d)
     return 0;
 }
 
-extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int precision, int one_per_launch,
-                                ocrt_plan **out)
+extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int precision, int flags,
+                                   ocrt_plan **out)
 {
     if (!in || n_inst < 1 || !out || (precision != OCRT_PREC_FP64 && precision != OCRT_PREC_MIXED))
         return OCRT_ERR_ARGS;
+    if (flags & ~(OCRT_PLAN_ONE_PER_LAUNCH | OCRT_PLAN_ADJOINT)) return OCRT_ERR_ARGS;
+    const bool adjoint = (flags & OCRT_PLAN_ADJOINT) != 0;
+    // the adjoint runs the fp64 form and needs the plane-parallel solve (no IPSS)
+    if (adjoint && precision != OCRT_PREC_FP64) return OCRT_ERR_ARGS;
+    if (adjoint)
+        for (int i = 0; i < n_inst; ++i)
+            if (in[i].ipss) return OCRT_ERR_ARGS;
     *out = nullptr;
     ocrt_plan *p = new ocrt_plan();
     p->n_inst = n_inst;
     p->prec = precision;
-    p->per_inst = one_per_launch ? 1 : 0;
+    p->per_inst = (flags & OCRT_PLAN_ONE_PER_LAUNCH) ? 1 : 0;
+    p->adjoint = adjoint ? 1 : 0;
     const bool mixed = precision == OCRT_PREC_MIXED;
     const int Palign = mixed ? 32 : 16;
     p->inst.resize(n_inst);
@@ -329,6 +360,9 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
         greek_n += I.aer ? 4LL * (I.La + 1) : 0;
         I.off_layers = layers_n;
         layers_n += I.layered ? 4LL * I.nt : 0;
+        I.gidx = i;
+        I.off_grad = p->grad_n;
+        p->grad_n += 4LL * (I.nt + 1);
         h.lev_n = 5LL * (I.nt + 1);
         h.T_n = (long long)I.nt * n;
         h.R_n = (I.surf == 2) ? (long long)(I.mmax + 1) * n * n * 9 + (long long)(I.mmax + 1) * n * 9 : 0;
@@ -356,7 +390,11 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
                 b += 3.0 * (I.lmax + 1) * (2 * n + 1) * 8 + 6.0 * (n + 1) * (2 * n + 1) * 8 +
                      (double)Mp * K * ae * (mixed ? 2 : 1) + (double)L * P * (fe + je + 8) +
                      (mixed ? 8.0 * P : 0.0) + ((mixed && m < kSplitModes) ? 2.0 * L * P : 0.0) + 6.0 * n * 8;
+                if (adjoint)   // transposed operator, checkpoints, segment fields, lambda, second parts
+                    b += (double)Mp * K * ae +
+                         (double)L * P * 8 * ((I.max_orders - 1) / kCkptEvery + 1 + 3 + 2);
             }
+            if (adjoint) b += (double)h.R_n * 8;
             const int items_after = (item_global - c.it0) + I.mmax + 1;
             if (i > c.i0 && (bytes + b > budget || items_after > 60000 || i - c.i0 >= 60000 || p->per_inst)) break;
             bytes += b;
@@ -486,6 +524,21 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     if (mixed) rc |= dmalloc(p, (void **)&P.comp, std::max(mx_cmp, 1LL) * 4);
     if (mixed) rc |= dmalloc(p, (void **)&P.Flo, std::max(mx_flo, 1LL) * 2);
     rc |= dmalloc(p, (void **)&P.bc, mx_bc * 8);
+    if (adjoint) {
+        int maxit = 1;
+        for (const HostInst &h : p->inst) maxit = std::max(maxit, h.d.max_orders);
+        p->ckpt.assign((maxit - 1) / kCkptEvery + 1, nullptr);
+        for (void *&q : p->ckpt) rc |= dmalloc(p, &q, mx_F * 8);
+        for (void *&q : p->seg) rc |= dmalloc(p, &q, mx_F * 8);
+        rc |= dmalloc(p, &p->At, mx_A * ae);
+        rc |= dmalloc(p, (void **)&P.lam, mx_F * 8);
+        rc |= dmalloc(p, (void **)&P.J2, mx_F * 8);
+        rc |= dmalloc(p, (void **)&P.dR, std::max(mx_R, 1LL) * 8);
+        rc |= dmalloc(p, (void **)&p->grad, p->grad_n * 8);
+        rc |= dmalloc(p, (void **)&p->gout, (size_t)n_inst * 3 * 8);
+        P.grad = p->grad;
+        P.gout = p->gout;
+    }
     if (rc) { ocrt_plan_destroy(p); return OCRT_ERR_ALLOC; }
     P.ring = p->ring;
     P.greek = p->greek;
@@ -554,6 +607,12 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     return OCRT_OK;
 }
 
+extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int precision, int one_per_launch,
+                                ocrt_plan **out)
+{
+    return ocrt_plan_create_ex(in, n_inst, precision, one_per_launch ? OCRT_PLAN_ONE_PER_LAUNCH : 0, out);
+}
+
 static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
 {
     const bool mixed = p->prec == OCRT_PREC_MIXED;
@@ -563,16 +622,32 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     ItemOut *io = p->iout + c.it0;
     hipLaunchKernelGGL(medium_levels, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
     hipLaunchKernelGGL(medium_fractions, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
-    if (c.any_rough)
-        hipLaunchKernelGGL(coxmunk_fourier, dim3(c.max_pairs, ni), dim3(256), 0, s, c.dinst, P);
+    if (c.any_rough) {
+        if (p->adjoint)
+            hipLaunchKernelGGL(coxmunk_fourier<true>, dim3(c.max_pairs, ni), dim3(256), 0, s, c.dinst, P);
+        else
+            hipLaunchKernelGGL(coxmunk_fourier<false>, dim3(c.max_pairs, ni), dim3(256), 0, s, c.dinst, P);
+    }
     hipLaunchKernelGGL(angular_basis, dim3(nit), dim3(128), 0, s, c.dinst, c.ditem, P);
     hipLaunchKernelGGL(phase_tables, dim3((c.max_tab + 255) / 256, nit), dim3(256), 0, s, c.dinst, c.ditem, P);
     const unsigned pack_blocks = (unsigned)((c.max_pack + 255) / 256);
     if (mixed)
-        hipLaunchKernelGGL(pack_operator<__hip_bfloat16>, dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst,
+        hipLaunchKernelGGL((pack_operator<__hip_bfloat16, false>), dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst,
                            c.ditem, P);
     else
-        hipLaunchKernelGGL(pack_operator<double>, dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst, c.ditem, P);
+        hipLaunchKernelGGL((pack_operator<double, false>), dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst, c.ditem, P);
+    if (p->adjoint) {
+        Pools Pt = P;
+        Pt.A = p->At;
+        hipLaunchKernelGGL((pack_operator<double, true>), dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst, c.ditem,
+                           Pt);
+    }
+    // adjoint plans keep u_1, u_5, u_9, ... of every item
+    auto checkpoint = [&](int order) -> int {
+        if (!p->adjoint || (order - 1) % kCkptEvery != 0) return 0;
+        return hipMemcpyAsync(p->ckpt[(order - 1) / kCkptEvery], P.F, (size_t)c.F * 8, hipMemcpyDeviceToDevice, s) ==
+                       hipSuccess ? 0 : OCRT_ERR_DEVICE;
+    };
     // per-column kernels: one 64-column tile per wavefront, four per workgroup
     const int ncols = (int)c.hcols.size();
     const dim3 cols((unsigned)((ncols + 3) / 4));
@@ -582,6 +657,7 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     else
         hipLaunchKernelGGL((order_one<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st, c.dcols,
                            ncols);
+    if (checkpoint(1)) return OCRT_ERR_DEVICE;
     const unsigned ntiles = (unsigned)c.htiles.size();
     for (int it = 2; it <= c.maxit; ++it) {
         hipEvent_t *eg = p->profile ? ev_pair(p, 0) : nullptr;
@@ -589,23 +665,24 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
         if (mixed) {
             hipLaunchKernelGGL(order_status<__hip_bfloat16>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
             if (eg) (void)hipEventRecord(eg[0], s);
-            hipLaunchKernelGGL(gemm_bf16, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st);
+            hipLaunchKernelGGL(gemm_bf16, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st, 0);
             if (eg) (void)hipEventRecord(eg[1], s);
             if (es) (void)hipEventRecord(es[0], s);
             hipLaunchKernelGGL((sweep<__hip_bfloat16, float, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st,
-                               c.dcols, ncols);
+                               c.dcols, ncols, 0);
             if (es) (void)hipEventRecord(es[1], s);
         } else {
             hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
             if (eg) (void)hipEventRecord(eg[0], s);
-            hipLaunchKernelGGL(gemm_f64, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st);
+            hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st, 0);
             if (eg) (void)hipEventRecord(eg[1], s);
             if (es) (void)hipEventRecord(es[0], s);
             hipLaunchKernelGGL((sweep<double, double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st,
-                               c.dcols, ncols);
+                               c.dcols, ncols, 0);
             if (es) (void)hipEventRecord(es[1], s);
         }
         p->gemm_launches += 1.0;
+        if (checkpoint(it)) return OCRT_ERR_DEVICE;
     }
     if (mixed)
         hipLaunchKernelGGL(order_status<__hip_bfloat16>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, c.maxit + 1);
@@ -615,6 +692,66 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     if (c.n_jobs > 0)
         hipLaunchKernelGGL(ipss_kappa, dim3(c.n_jobs), dim3(256), 0, s, c.dinst, c.djobs, P,
                            p->kappa + c.i0, p->taudir + c.i0);
+    return hipGetLastError() == hipSuccess ? 0 : OCRT_ERR_DEVICE;
+}
+
+// Backward pass of one chunk after its forward pass (fp64).  Segment s holds
+// u_{4s+1} (the checkpoint) and the recomputed u_{4s+2..4s+4}; the backward
+// step of order o needs u_o and u_{o-1}, so segments are rebuilt from the top
+// down and order 4s+5 (whose u_o is the next checkpoint) is taken with segment s.
+static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
+{
+    const Pools P0 = p->pools;
+    const int ni = c.i1 - c.i0, nit = c.it1 - c.it0;
+    ItemState *st = p->state + c.it0;
+    const int ncols = (int)c.hcols.size();
+    const dim3 cols((unsigned)((ncols + 3) / 4));
+    const unsigned ntiles = (unsigned)c.htiles.size();
+    const int maxit = c.maxit;
+    hipLaunchKernelGGL(adj_zero_grad, dim3(4, ni), dim3(256), 0, s, c.dinst, P0);
+    hipLaunchKernelGGL(fill_f64, dim3((unsigned)((c.F + 255) / 256)), dim3(256), 0, s, P0.lam, c.F, 0.0);
+    hipLaunchKernelGGL(adj_seed, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P0);
+    auto slot = [&](int seg, int j) -> void * { return j == 0 ? p->ckpt[seg] : p->seg[j - 1]; };
+    for (int sg = (maxit - 1) / kCkptEvery; sg >= 0; --sg) {
+        const int base = kCkptEvery * sg + 1;   // order of the checkpoint
+        // rebuild u_{base+1..base+3}
+        for (int j = 1; j < kCkptEvery && base + j <= maxit; ++j) {
+            const int o = base + j;
+            Pools Pin = P0;
+            Pin.F = slot(sg, j - 1);
+            hipLaunchKernelGGL(adj_bottom, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pin, st, o);
+            hipLaunchKernelGGL(adj_surface_bc, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pin, st, o);
+            hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pin, c.dtiles, st,
+                               o);
+            Pools Pout = P0;
+            Pout.F = slot(sg, j);
+            hipLaunchKernelGGL((sweep<double, double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, Pout, st,
+                               c.dcols, ncols, o);
+        }
+        // backward steps
+        for (int o = std::min(base + kCkptEvery, maxit); o >= std::max(base + 1, 2); --o) {
+            void *u_o = (o == base + kCkptEvery) ? p->ckpt[sg + 1] : slot(sg, o - base);
+            void *u_prev = slot(sg, o - 1 - base);
+            Pools Pg = P0;
+            Pg.F = u_prev;
+            hipLaunchKernelGGL(gemm_f64<kGemmParts>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pg, c.dtiles, st,
+                               o);
+            Pools Ps = P0;
+            Ps.F = u_o;
+            hipLaunchKernelGGL(sweep_adj, cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st, c.dcols, ncols, o);
+            Pools Pt = P0;
+            Pt.A = p->At;
+            Pt.F = P0.J;
+            Pt.J = P0.lam;
+            hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pt, c.dtiles, st,
+                               o);
+            hipLaunchKernelGGL(surface_adj, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pg, st, o);
+        }
+    }
+    Pools P1 = P0;
+    P1.F = p->ckpt[0];
+    hipLaunchKernelGGL(order_one_adj, cols, dim3(256), 0, s, c.dinst, c.ditem, P1, c.dcols, ncols);
+    hipLaunchKernelGGL(glint_adj, dim3((ni + 63) / 64), dim3(64), 0, s, c.dinst, ni, P0);
     return hipGetLastError() == hipSuccess ? 0 : OCRT_ERR_DEVICE;
 }
 
@@ -629,6 +766,17 @@ __global__ static void gather_ipss(const DevInst *inst, int ni, const double *ka
     out[i].tau_dir_ipss = j >= 0 ? taudir[j] : -1.0;
 }
 
+static int forward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
+{
+    if (run_chunk(p, c, s) != 0) return OCRT_ERR_DEVICE;
+    const int ni = c.i1 - c.i0;
+    hipLaunchKernelGGL(gather_ipss, dim3((ni + 127) / 128), dim3(128), 0, s, c.dinst, ni, p->kappa + c.i0,
+                       p->taudir + c.i0, c.djob_of, p->ipss + c.i0);
+    hipLaunchKernelGGL(finalize, dim3((ni + 127) / 128), dim3(128), 0, s, c.dinst, ni, p->iout + c.it0,
+                       p->ipss + c.i0, p->pools, p->res + c.i0);
+    return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
+}
+
 extern "C" int ocrt_plan_execute(ocrt_plan *p, void *stream)
 {
     if (!p) return OCRT_ERR_ARGS;
@@ -636,15 +784,47 @@ extern "C" int ocrt_plan_execute(ocrt_plan *p, void *stream)
     p->gemm_launches = 0.0;
     p->ev_used = 0;
     for (Chunk &c : p->chunks) {
-        if (run_chunk(p, c, s) != 0) return OCRT_ERR_DEVICE;
-        const int ni = c.i1 - c.i0;
-        hipLaunchKernelGGL(gather_ipss, dim3((ni + 127) / 128), dim3(128), 0, s, c.dinst, ni, p->kappa + c.i0,
-                           p->taudir + c.i0, c.djob_of, p->ipss + c.i0);
-        hipLaunchKernelGGL(finalize, dim3((ni + 127) / 128), dim3(128), 0, s, c.dinst, ni, p->iout + c.it0,
-                           p->ipss + c.i0, p->pools, p->res + c.i0);
+        if (forward_chunk(p, c, s) != 0) return OCRT_ERR_DEVICE;
         HIPCHK(hipStreamSynchronize(s));
     }
+    // a single-chunk adjoint plan keeps the checkpoints of this forward pass
+    p->fwd_ready = (p->adjoint && p->chunks.size() == 1) ? 1 : 0;
     return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
+}
+
+extern "C" int ocrt_plan_adjoint(ocrt_plan *p, const double *grad_out, double *grad_levels, double *grad_wind,
+                                 void *stream)
+{
+    if (!p || !grad_out || !p->adjoint) return OCRT_ERR_ARGS;
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(hipMemcpyAsync(p->gout, grad_out, (size_t)p->n_inst * 3 * 8, hipMemcpyHostToDevice, s));
+    for (Chunk &c : p->chunks) {
+        if (!p->fwd_ready && forward_chunk(p, c, s) != 0) return OCRT_ERR_DEVICE;
+        if (backward_chunk(p, c, s) != 0) return OCRT_ERR_DEVICE;
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    p->fwd_ready = (p->chunks.size() == 1) ? 1 : 0;
+    if (!grad_levels && !grad_wind) return OCRT_OK;
+    std::vector<double> g(p->grad_n);
+    HIPCHK(hipMemcpy(g.data(), p->grad, (size_t)p->grad_n * 8, hipMemcpyDeviceToHost));
+    long long off = 0;
+    for (int i = 0; i < p->n_inst; ++i) {
+        const DevInst &I = p->inst[i].d;
+        const int nt = I.nt, L = nt + 1;
+        const double *gH = g.data() + I.off_grad, *gX = gH + L, *gY = gX + L, *gD = gY + L;
+        if (grad_levels) {
+            double *dh = grad_levels + off, *dx = dh + L, *dy = dx + L;
+            for (int k = 0; k <= nt; ++k) {
+                // h[k] enters explicitly (gH) and through the layers above (k-1) and below (k)
+                dh[k] = gH[k] + (k > 0 ? gD[k - 1] : 0.0) - (k < nt ? gD[k] : 0.0);
+                dx[k] = gX[k];
+                dy[k] = gY[k];
+            }
+        }
+        if (grad_wind) grad_wind[i] = gH[4 * L - 1];
+        off += 3LL * L;
+    }
+    return OCRT_OK;
 }
 
 extern "C" int ocrt_plan_results(ocrt_plan *p, ocrt_result_t *out, double *toa_modes, int *orders)
@@ -741,6 +921,19 @@ extern "C" int ocrt_solve_batch(const ocrt_instance_t *in, int n, int precision,
     if (rc) return rc;
     rc = ocrt_plan_execute(p, stream);
     if (!rc) rc = ocrt_plan_results(p, out, nullptr, nullptr);
+    ocrt_plan_destroy(p);
+    return rc;
+}
+
+extern "C" int ocrt_adjoint_batch(const ocrt_instance_t *in, int n, int precision, const double *grad_out,
+                                  ocrt_result_t *out, double *grad_levels, double *grad_wind, void *stream)
+{
+    ocrt_plan *p = nullptr;
+    int rc = ocrt_plan_create_ex(in, n, precision, OCRT_PLAN_ADJOINT, &p);
+    if (rc) return rc;
+    rc = ocrt_plan_execute(p, stream);
+    if (!rc) rc = ocrt_plan_adjoint(p, grad_out, grad_levels, grad_wind, stream);
+    if (!rc && out) rc = ocrt_plan_results(p, out, nullptr, nullptr);
     ocrt_plan_destroy(p);
     return rc;
 }

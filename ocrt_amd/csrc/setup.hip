@@ -134,10 +134,16 @@ __global__ void fourier_weights(double *w, int mmax)
 // shadowing factor is hoisted out of the azimuth loop.  Instances of a batch
 // that share the ring, the sun and the surface (the bands of one geometry)
 // share one kernel: only owners (surf_owner) compute it.
+//
+// DERIV also writes d R^m / d sigma^2 (pool dR, the layout of pool R) for the
+// adjoint: every entry is proportional to the brdf, so its derivative is the
+// entry times d ln(brdf) / d sigma^2.
+template <bool DERIV>
 __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict__ inst, Pools P)
 {
     constexpr int kHalf = kNPhi / 2;
     __shared__ double R[9][kHalf];
+    __shared__ double Rd[DERIV ? 9 : 1][DERIV ? kHalf : 1];
     const DevInst &I = inst[blockIdx.y];
     if (I.surf != 2 || !I.surf_owner) return;
     const int n = I.n;
@@ -155,6 +161,7 @@ __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict
             for (int t = threadIdx.x; t < nout; t += blockDim.x) {
                 const int m = t / 9, kl = t % 9;
                 P.R[I.off_R + (((long long)m * n + o) * n + i) * 9 + kl] = 0.0;
+                if (DERIV) P.dR[I.off_R + (((long long)m * n + o) * n + i) * 9 + kl] = 0.0;
             }
             return;
         }
@@ -166,11 +173,16 @@ __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict
     const double dphi = 2.0 * OCRT_PI / (double)kNPhi;
     const double s2v = slope_variance(I.wind, I.sigma_type);
     const double shadow = sancer_shadow(mo, mi, s2v);
+    const double dls = DERIV ? sancer_dlog(mo, mi, s2v, shadow) : 0.0;
     for (int p = threadIdx.x; p < kHalf; p += blockDim.x) {
         const double phi = (p + 0.5) * dphi;
-        double r[9];
-        coxmunk_R_core(mo, mi, cos(phi), sin(phi), s2v, shadow, I.n_water, I.q_conv, r);
+        double r[9], tb2;
+        coxmunk_R_core(mo, mi, cos(phi), sin(phi), s2v, shadow, I.n_water, I.q_conv, r, &tb2);
         for (int e = 0; e < 9; ++e) R[e][p] = r[e];
+        if (DERIV) {
+            const double dl = (tb2 >= 0.0) ? tb2 / (s2v * s2v) - 1.0 / s2v + dls : 0.0;
+            for (int e = 0; e < 9; ++e) Rd[e][p] = r[e] * dl;
+        }
     }
     __syncthreads();
     const int nout = (I.mmax + 1) * 9;
@@ -178,15 +190,22 @@ __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict
         const int m = t / 9, kl = t % 9;
         const bool use_cos = (kl == 0 || kl == 1 || kl == 3 || kl == 4 || kl == 8);
         const double *W = P.cm_w + ((long long)m * 2 + (use_cos ? 0 : 1)) * kNPhi;
-        double acc = 0.0;
+        double acc = 0.0, accd = 0.0;
         for (int p = 0; p < kHalf; ++p) acc += R[kl][p] * W[p];
+        if (DERIV)
+            for (int p = 0; p < kHalf; ++p) accd += Rd[kl][p] * W[p];
         const double nrm = (m == 0) ? 1.0 / (2.0 * OCRT_PI) : 1.0 / OCRT_PI;
-        double v = 2.0 * acc * (dphi * nrm);
-        if (m > 0 && (kl == 2 || kl == 5)) v = -v;
-        if (!solar) P.R[I.off_R + (((long long)m * n + o) * n + i) * 9 + kl] = v;
-        else P.R[I.off_Rsun + ((long long)m * n + o) * 9 + kl] = v;
+        double v = 2.0 * acc * (dphi * nrm), vd = 2.0 * accd * (dphi * nrm);
+        if (m > 0 && (kl == 2 || kl == 5)) { v = -v; vd = -vd; }
+        const long long at = !solar ? I.off_R + (((long long)m * n + o) * n + i) * 9 + kl
+                                    : I.off_Rsun + ((long long)m * n + o) * 9 + kl;
+        P.R[at] = v;
+        if (DERIV) P.dR[at] = vd;
     }
 }
+
+template __global__ void coxmunk_fourier<false>(const DevInst *, Pools);
+template __global__ void coxmunk_fourier<true>(const DevInst *, Pools);
 
 // ---------------------------------------------------------------------------
 // Signed direction cosine of direction index d = j + n, j in [-n, n]; j = 0 is
@@ -378,7 +397,9 @@ __device__ inline double pair_molecular(const DevInst &I, int m, const double *p
 // state column and c = (part, incoming state column).  Part 0/1 of the
 // column range is the particle (scaled by xdel at apply time) or molecular
 // (scaled by ydel) block.
-template <typename AT>
+// TRANS packs the transposed operator (A_part^T of every part) in the same
+// layout, for the adjoint's M^T products.
+template <typename AT, bool TRANS>
 __global__ void pack_operator(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P)
 {
     const DevItem &it = items[blockIdx.y];
@@ -395,8 +416,9 @@ __global__ void pack_operator(const DevInst *__restrict__ inst, const DevItem *_
     const long long lo_off = (long long)it.Mp * it.P;
     double v = 0.0;
     if (r < six_n && cc < six_n) {
-        const int to = r / n, o = r % n + 1;
-        const int ti = cc / n, i = cc % n + 1;
+        const int ro = TRANS ? cc : r, ci = TRANS ? r : cc;   // (outgoing, incoming) state columns
+        const int to = ro / n, o = ro % n + 1;
+        const int ti = ci / n, i = ci % n + 1;
         const double w = P.ring[I.off_ring + n + (i - 1)];
         if (w != 0.0) {
             const int q = c_sel_q[to][ti];
@@ -425,7 +447,9 @@ __global__ void pack_operator(const DevInst *__restrict__ inst, const DevItem *_
     }
 }
 
-template __global__ void pack_operator<double>(const DevInst *, const DevItem *, Pools);
-template __global__ void pack_operator<__hip_bfloat16>(const DevInst *, const DevItem *, Pools);
+template __global__ void pack_operator<double, false>(const DevInst *, const DevItem *, Pools);
+template __global__ void pack_operator<__hip_bfloat16, false>(const DevInst *, const DevItem *, Pools);
+template __global__ void pack_operator<double, true>(const DevInst *, const DevItem *, Pools);
+template __global__ void pack_operator<__hip_bfloat16, true>(const DevInst *, const DevItem *, Pools);
 
 }  // namespace ocrt

@@ -13,24 +13,10 @@
 //   extract / finalize : view sampling, Fourier reconstruction, reflectance,
 //                    IPSS scaling, direct glint and transmittances
 //                    (rt_solver.c:1010-1276, rt_fourier.c:9-33, surface.c:1813-1907)
-#include "common.h"
-#include "device_math.h"
 #include "ocrt_amd.h"
+#include "sos_common.h"
 
 namespace ocrt {
-
-typedef double double4_ __attribute__((ext_vector_type(4)));
-typedef float float4_ __attribute__((ext_vector_type(4)));
-typedef short short4_ __attribute__((ext_vector_type(4)));
-
-struct LevelView {
-    const double *h, *ch, *xd, *yd, *z;
-    __device__ LevelView(const Pools &P, const DevInst &I)
-    {
-        const int L = I.nt + 1;
-        h = P.lev + I.off_lev; ch = h + L; xd = ch + L; yd = xd + L; z = yd + L;
-    }
-};
 
 // Formal solution of one column with the reference's linear-source closed form
 // (rt_layer_transport.c:89-99, 144-203), in fp64.  S(k) gives the source at
@@ -125,69 +111,16 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     }
     const LevelView lv(P, I);
     const int type = col / n, c = type >> 1, hemi = type & 1, o = col % n;
-    const int d = hemi ? n - (o + 1) : n + (o + 1);      // direction index of this column
     const double *mu_ring = P.ring + I.off_ring;
     const double mu = mu_ring[o];
     const double *T = P.T + I.off_T;
-    const double *plm = P.basis + it.off_basis;
-    const double *rrl = plm + (long long)(lmax + 1) * dirs;
-    const double *rtl = rrl + (long long)(lmax + 1) * dirs;
-    const double *plm2 = plm + 2LL * dirs, *rrl2 = rrl + 2LL * dirs, *rtl2 = rtl + 2LL * dirs;
-    const long long cnt = (long long)(n + 1) * dirs;
-    const double *tab = P.tab + it.off_tab;
-    const int mol = (m <= 2);
-    const double b0 = (m == 0) ? I.beta0 : 0.0;
-    const double xs = plm2[n];                             // solar-slot basis at l = 2
-    // single-scattering phase entries of this column (rt_primary_source.c:61-70, 164-171)
-    double pc, mc, sgn;
-    if (c == 0) { pc = tab[0 * cnt + d]; mc = b0 + I.beta2 * plm2[d] * xs; sgn = 1.0; }
-    else if (c == 1) { pc = tab[1 * cnt + d]; mc = mol ? I.gamma2 * rrl2[d] * xs : 0.0; sgn = 1.0; }
-    else { pc = tab[2 * cnt + d]; mc = mol ? I.gamma2 * rtl2[d] * xs : 0.0; sgn = -1.0; }
-    auto S1 = [&](int k) { return sgn * lv.ch[k] * (lv.xd[k] * pc + lv.yd[k] * mc); };
-    sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o, S1,
+    const ColumnSource src(I, it, P, lv, col);
+    sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o, [&](int k) { return src.primary(lv, k); },
                      [&](int k, double v) { tot[(long long)k * Pp + col] = v; });
-
     if (I.surf == 1) {
         // flat Fresnel: the specularly reflected direct beam scattered once
         // (rt_surface_boundary.c:210-428)
-        const double mus = I.mus, nw = I.n_water, n2 = nw * nw;
-        const double tn = sqrt(n2 - 1.0 + mus * mus);
-        const double rp = (n2 * mus - tn) / (n2 * mus + tn), rs = (mus - tn) / (mus + tn);
-        const double RI = (rp * rp + rs * rs) / 2.0;
-        const double RQ = (I.q_conv == 1) ? (rp * rp - rs * rs) / 2.0 : (rs * rs - rp * rp) / 2.0;
-        const double sb = 0.5 * exp(-2.0 * lv.h[nt] / mus);
-        const int dk = hemi ? n + (o + 1) : n - (o + 1);   // mirrored direction
-        const double bP = xs, bR = rrl2[n];
-        double mgr = 0.0, mart = 0.0;
-        if (I.aer) {
-            const double *g = P.greek + I.off_greek;
-            const int q = I.La + 1;
-            const int lend = lmax < I.La ? lmax : I.La;
-            for (int l = m; l <= lend; ++l) {
-                const long long b = (long long)l * dirs;
-                mgr += plm[b + dk] * rrl[b + n] * g[q + l];
-                mart += rtl[b + dk] * rrl[b + n] * g[2 * q + l] + rrl[b + dk] * rtl[b + n] * g[3 * q + l];
-            }
-        }
-        double pA, mA, pB, mB;
-        if (c == 0) {
-            pA = tab[0 * cnt + dk]; mA = mol ? b0 + I.beta2 * plm2[dk] * bP : 0.0;
-            pB = mgr; mB = mol ? I.gamma2 * bR * plm2[dk] : 0.0;
-        } else if (c == 1) {
-            pA = tab[1 * cnt + dk]; mA = mol ? I.gamma2 * rrl2[dk] * bP : 0.0;
-            pB = tab[3 * cnt + dk]; mB = mol ? I.alpha2 * bR * rrl2[dk] : 0.0;
-        } else {
-            pA = tab[2 * cnt + dk]; mA = mol ? I.gamma2 * rtl2[dk] * bP : 0.0;
-            pB = mart; mB = mol ? I.alpha2 * rtl2[dk] * bR : 0.0;
-        }
-        auto SF = [&](int k) -> double {
-            if ((hemi == 0 && k == nt) || (hemi == 1 && k == 0)) return 0.0;
-            const double x = lv.xd[k], y = lv.yd[k];
-            const double kA = pA * x + mA * y, kB = pB * x + mB * y;
-            const double beam = sb * exp(lv.h[k] / mus);
-            return beam * (RI * kA + RQ * kB);
-        };
-        sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o, SF,
+        sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o, [&](int k) { return src.flat_value(lv, I.mus, k); },
                          [&](int k, double v) { tot[(long long)k * Pp + col] += v; });
     } else if (I.surf == 2 && hemi == 0) {
         // rough ocean: direct beam reflected into every upward node
@@ -238,62 +171,28 @@ __global__ void order_status(const DevInst *__restrict__ inst, const DevItem *__
     }
     __syncthreads();
     if (!act || I.surf == 0) return;
-    const int n = it.n, nt = it.nt, Pp = it.P, m = it.m;
-    const double *bot = P.bc + it.off_bc + 3 * n;   // [3][n] downward field at BOA
-    const double *mu = P.ring + I.off_ring, *w = mu + n;
-    double *bc = P.bc + it.off_bc;
-    const double W = (I.surf == 1) ? 2.0 * OCRT_PI : ((m == 0) ? 2.0 * OCRT_PI : OCRT_PI);
-    for (int o = threadIdx.x; o < n; o += blockDim.x) {
-        double si = 0.0, sq = 0.0, su = 0.0;
-        if (I.surf == 1) {
-            double Fr[9];
-            fresnel_R(mu[o], I.n_water, I.q_conv, Fr);
-            const double a = bot[o], b = bot[n + o], cc = bot[2 * n + o];
-            if (w[o] == 0.0) {
-                si = Fr[0] * a + Fr[1] * b + Fr[2] * cc;
-                sq = Fr[3] * a + Fr[4] * b + Fr[5] * cc;
-                su = Fr[6] * a + Fr[7] * b + Fr[8] * cc;
-            } else {
-                const double dd = 1.0 / (2.0 * OCRT_PI * mu[o] * w[o]);
-                const double pw = W * mu[o] * w[o];
-                double R[9];
-                for (int e = 0; e < 9; ++e) R[e] = dd * Fr[e];
-                si = pw * (R[0] * a + R[1] * b + R[2] * cc);
-                sq = pw * (R[3] * a + R[4] * b + R[5] * cc);
-                su = pw * (R[6] * a + R[7] * b + R[8] * cc);
-            }
-        } else {
-            const double *R0 = P.R + I.off_R + ((long long)m * n + o) * n * 9;
-            for (int i = 0; i < n; ++i) {
-                if (w[i] == 0.0) continue;
-                const double a = bot[i], b = bot[n + i], cc = bot[2 * n + i];
-                const double pw = W * mu[i] * w[i];
-                const double *R = R0 + (long long)i * 9;
-                si += pw * (R[0] * a + R[1] * b + R[2] * cc);
-                sq += pw * (R[3] * a + R[4] * b + R[5] * cc);
-                su += pw * (R[6] * a + R[7] * b + R[8] * cc);
-            }
-        }
-        bc[o] = si;
-        bc[n + o] = sq;
-        bc[2 * n + o] = su;
-    }
+    surface_bc_item(I, it, P);
 }
 
 // ---------------------------------------------------------------------------
 // Scattering integral in fp64 on v_mfma_f64_16x16x4f64.  Tile 64 (outgoing
-// columns) x 64 (levels), K step 16, four waves of 32 x 32.
-// J[lev][r] = sum_k A[r][k] * s_part(lev) * F[lev][k mod P].
+// columns) x 64 (levels), K step 16, four waves of 32 x 32, one accumulator
+// set per operator part.  Epilogue:
+//   kGemmMix   J[lev][r]  = s0(lev) (A_0 F)[r][lev] + ydel(lev) (A_1 F)[r][lev]
+//   kGemmParts J[lev][r]  = (A_0 F)[r][lev], J2[lev][r] = (A_1 F)[r][lev]
+// with s0 = xdel for a particle first part, ydel for a molecular one.
+template <int MODE>
 __global__ __launch_bounds__(256) void gemm_f64(const DevInst *__restrict__ inst,
                                                 const DevItem *__restrict__ items, Pools P,
                                                 const GemmTile *__restrict__ tiles,
-                                                const ItemState *__restrict__ st)
+                                                const ItemState *__restrict__ st, int order)
 {
     const GemmTile tl = tiles[blockIdx.x];
-    if (tl.item < 0 || !st[tl.item].active) return;
+    if (tl.item < 0 || !item_runs(st[tl.item], order)) return;
     const DevItem &it = items[tl.item];
     const DevInst &I = inst[it.inst];
     const int K = it.K, Pp = it.P, L = it.L;
+    const int nparts = K / Pp;
     const double *A = reinterpret_cast<const double *>(P.A) + it.off_A;
     const double *F = reinterpret_cast<const double *>(P.F) + it.off_F;
     double *J = reinterpret_cast<double *>(P.J) + it.off_F;
@@ -303,44 +202,57 @@ __global__ __launch_bounds__(256) void gemm_f64(const DevInst *__restrict__ inst
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int row0 = tl.tm * 64, lev0 = tl.tn * 64;
-    double4_ acc[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) acc[a][b] = (double4_){0.0, 0.0, 0.0, 0.0};
+    double4_ acc[2][2][2];
+    for (int q = 0; q < 2; ++q)
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) acc[q][a][b] = (double4_){0.0, 0.0, 0.0, 0.0};
     const int lr = t >> 2, lk = (t & 3) * 4;
     const int lev_load = lev0 + lr;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        const double *src = A + (long long)(row0 + lr) * K + k0 + lk;
-        for (int q = 0; q < 4; ++q) As[lr][lk + q] = src[q];
-        if (lev_load < L) {
-            const int part = (k0 + lk) / Pp;
-            const int kind = (it.parts >> (8 * part)) & 0xff;
-            const double s = (kind == 1) ? lv.xd[lev_load] : lv.yd[lev_load];
-            const double *fs = F + (long long)lev_load * Pp + (k0 + lk) % Pp;
-            for (int q = 0; q < 4; ++q) Bs[lk + q][lr] = s * fs[q];
-        } else {
-            for (int q = 0; q < 4; ++q) Bs[lk + q][lr] = 0.0;
-        }
-        __syncthreads();
+    for (int q = 0; q < nparts; ++q) {
+        for (int k0 = 0; k0 < Pp; k0 += 16) {
+            const double *src = A + (long long)(row0 + lr) * K + q * Pp + k0 + lk;
+            for (int e = 0; e < 4; ++e) As[lr][lk + e] = src[e];
+            if (lev_load < L) {
+                const double *fs = F + (long long)lev_load * Pp + k0 + lk;
+                for (int e = 0; e < 4; ++e) Bs[lk + e][lr] = fs[e];
+            } else {
+                for (int e = 0; e < 4; ++e) Bs[lk + e][lr] = 0.0;
+            }
+            __syncthreads();
 #pragma unroll
-        for (int ks = 0; ks < 16; ks += 4) {
-            double av[2], bv[2];
-            for (int a = 0; a < 2; ++a) av[a] = As[wm + a * 16 + (lane & 15)][ks + (lane >> 4)];
-            for (int b = 0; b < 2; ++b) bv[b] = Bs[ks + (lane >> 4)][wn + b * 16 + (lane & 15)];
-            for (int a = 0; a < 2; ++a)
-                for (int b = 0; b < 2; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+            for (int ks = 0; ks < 16; ks += 4) {
+                double av[2], bv[2];
+                for (int a = 0; a < 2; ++a) av[a] = As[wm + a * 16 + (lane & 15)][ks + (lane >> 4)];
+                for (int b = 0; b < 2; ++b) bv[b] = Bs[ks + (lane >> 4)][wn + b * 16 + (lane & 15)];
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b)
+                        acc[q][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[q][a][b], 0, 0, 0);
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     // f64 16x16 result layout: lane holds rows (lane >> 4) + 4 i of column lane & 15
+    const int kind0 = it.parts & 0xff;
+    double *J2 = reinterpret_cast<double *>(P.J2) + it.off_F;
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) {
             const int lev = lev0 + wn + b * 16 + (lane & 15);
             const int r = row0 + wm + a * 16 + (lane >> 4);
-            if (lev < L) {
-                double *dst = J + (long long)lev * Pp;
-                for (int i = 0; i < 4; ++i)
-                    if (r + 4 * i < Pp) dst[r + 4 * i] = acc[a][b][i];
+            if (lev >= L) continue;
+            const double x = lv.xd[lev], y = lv.yd[lev];
+            const double s0 = (kind0 == 1) ? x : y;
+            double *dst = J + (long long)lev * Pp;
+            double *dst2 = J2 + (long long)lev * Pp;
+            for (int i = 0; i < 4; ++i) {
+                if (r + 4 * i >= Pp) continue;
+                if (MODE == kGemmMix) {
+                    double v = s0 * acc[0][a][b][i];
+                    if (nparts > 1) v += y * acc[1][a][b][i];
+                    dst[r + 4 * i] = v;
+                } else {
+                    dst[r + 4 * i] = acc[0][a][b][i];
+                    if (nparts > 1) dst2[r + 4 * i] = acc[1][a][b][i];
+                }
             }
         }
 }
@@ -355,10 +267,10 @@ __global__ __launch_bounds__(256) void gemm_f64(const DevInst *__restrict__ inst
 __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ inst,
                                                  const DevItem *__restrict__ items, Pools P,
                                                  const GemmTile *__restrict__ tiles,
-                                                 const ItemState *__restrict__ st)
+                                                 const ItemState *__restrict__ st, int order)
 {
     const GemmTile tl = tiles[blockIdx.x];
-    if (tl.item < 0 || !st[tl.item].active) return;
+    if (tl.item < 0 || !item_runs(st[tl.item], order)) return;
     const DevItem &it = items[tl.item];
     const DevInst &I = inst[it.inst];
     const int Pp = it.P, L = it.L, Mp = it.Mp;
@@ -439,14 +351,6 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
     }
 }
 
-// ---------------------------------------------------------------------------
-// Largest value over one wavefront (every lane receives it).
-__device__ inline double wave_max(double v)
-{
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-    return v;
-}
-
 // Formal solution of the current order for every column, fused with the order
 // sum total += cur and the two max-norms of the convergence test
 // (rt_solver.c:1855-1868, 3093-3124).  One thread per column, one 64-column
@@ -463,12 +367,13 @@ template <typename FT, typename JT, bool MIXED>
 __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                                              const DevItem *__restrict__ items, Pools P,
                                              ItemState *__restrict__ st, const ColTile *__restrict__ tiles,
-                                             int n_tiles)
+                                             int n_tiles, int order)
 {
     const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (tix >= n_tiles) return;
     const ColTile ct = tiles[tix];
-    if (ct.item < 0 || !st[ct.item].active) return;
+    if (ct.item < 0 || !item_runs(st[ct.item], order)) return;
+    const bool accumulate = order == 0;   // recomputation only rebuilds the field
     const int item = ct.item;
     const DevItem &it = items[item];
     const DevInst &I = inst[it.inst];
@@ -497,6 +402,7 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                 F[ix] = to_storage<FT>(v);
             mcur = fmax(mcur, fabs(v));
             if (hemi == 1 && k == nt) P.bc[it.off_bc + 3 * n + c * n + o] = v;
+            if (!accumulate) return;
             const double tsum = t_old + v;
             tot[ix] = tsum;
             mtot = fmax(mtot, fabs(tsum));
@@ -506,7 +412,7 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                 *cp = (float)((double)*cp + e);
             }
         };
-        emit(k_first, hemi ? 0.0 : bc, tot[(long long)k_first * Pp]);
+        emit(k_first, hemi ? 0.0 : bc, accumulate ? tot[(long long)k_first * Pp] : 0.0);
         if (!MIXED) {
             const double *T = P.T + I.off_T;
             double run = hemi ? 0.0 : bc;
@@ -520,7 +426,7 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                     if (k >= 0 && k <= nt) {
                         sj[u] = (double)J[(long long)k * Pp];
                         tr[u] = T[(long long)layer_of(k) * n + o];
-                        to[u] = tot[(long long)k * Pp];
+                        to[u] = accumulate ? tot[(long long)k * Pp] : 0.0;
                     }
                 }
 #pragma unroll
@@ -556,7 +462,7 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                         sj[u] = (float)J[(long long)k * Pp];
                         om[u] = OM[(long long)layer_of(k) * n + o];
                         idt[u] = IDT[layer_of(k)];
-                        to[u] = tot[(long long)k * Pp];
+                        to[u] = accumulate ? tot[(long long)k * Pp] : 0.0;
                     }
                 }
 #pragma unroll
@@ -572,6 +478,7 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
             }
         }
     }
+    if (!accumulate) return;
     const double bm = wave_max(mcur);
     const double bt = wave_max(mtot);
     if (lane == 0) {
@@ -645,33 +552,6 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
             dst[q] = val;
         }
     }
-}
-
-// Direct sunglint reflectance (surface.c:1813-1907).
-__device__ inline void sunglint_rho(double mu_v, double phi_v, double mu_0, double ws, int st, double nw,
-                                    double tau, int qc, double *rho)
-{
-    const double trans = exp(-tau / mu_0) * exp(-tau / mu_v);
-    if (ws < 1e-6) {
-        const double sf = 0.5 * OCRT_PI / 180.0;
-        const double st0 = sqrt(fmax(0.0, 1.0 - mu_0 * mu_0));
-        const double smu = sf * fmax(st0, 1e-3);
-        double de = phi_v;
-        while (de > OCRT_PI) de -= 2.0 * OCRT_PI;
-        while (de < -OCRT_PI) de += 2.0 * OCRT_PI;
-        const double dmu = mu_v - mu_0;
-        const double ex = -0.5 * ((dmu * dmu) / (smu * smu) + (de * de) / (sf * sf));
-        if (ex < -50.0) { rho[0] = rho[1] = rho[2] = 0.0; return; }
-        double MF[9];
-        fresnel_R(mu_0, nw, qc, MF);
-        const double den = fmax(2.0 * mu_0 * st0 * mu_0 * mu_v, 1e-12);
-        const double r = 4.0 * OCRT_PI * trans / den * exp(ex);
-        rho[0] = MF[0] * r; rho[1] = MF[3] * r; rho[2] = MF[6] * r;
-        return;
-    }
-    double R[9];
-    coxmunk_R(mu_v, mu_0, cos(phi_v), sin(phi_v), ws, st, nw, qc, R);
-    rho[0] = OCRT_PI * R[0] * trans; rho[1] = OCRT_PI * R[3] * trans; rho[2] = OCRT_PI * R[6] * trans;
 }
 
 // Instance results (rt_solver.c:1062-1276): Fourier reconstruction at the view
@@ -755,8 +635,12 @@ template __global__ void order_one<__hip_bfloat16, true>(const DevInst *, const 
 template __global__ void order_status<double>(const DevInst *, const DevItem *, Pools, ItemState *, int);
 template __global__ void order_status<__hip_bfloat16>(const DevInst *, const DevItem *, Pools, ItemState *, int);
 template __global__ void sweep<double, double, false>(const DevInst *, const DevItem *, Pools, ItemState *,
-                                                      const ColTile *, int);
+                                                      const ColTile *, int, int);
 template __global__ void sweep<__hip_bfloat16, float, true>(const DevInst *, const DevItem *, Pools, ItemState *,
-                                                            const ColTile *, int);
+                                                            const ColTile *, int, int);
+template __global__ void gemm_f64<kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                            const ItemState *, int);
+template __global__ void gemm_f64<kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                              const ItemState *, int);
 
 }  // namespace ocrt

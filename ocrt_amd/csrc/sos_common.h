@@ -1,0 +1,215 @@
+// Device helpers shared by the forward order loop (sos.hip) and its adjoint
+// (adjoint.hip).
+#pragma once
+#include "common.h"
+#include "device_math.h"
+
+namespace ocrt {
+
+typedef double double4_ __attribute__((ext_vector_type(4)));
+typedef float float4_ __attribute__((ext_vector_type(4)));
+typedef short short4_ __attribute__((ext_vector_type(4)));
+
+struct LevelView {
+    const double *h, *ch, *xd, *yd, *z;
+    __device__ LevelView(const Pools &P, const DevInst &I)
+    {
+        const int L = I.nt + 1;
+        h = P.lev + I.off_lev; ch = h + L; xd = ch + L; yd = xd + L; z = yd + L;
+    }
+};
+
+// Whether an item takes part in a launch: the forward order loop follows the
+// convergence flag (order == 0); recomputation and the adjoint pass an order
+// and take every item whose forward solve reached it.
+__device__ inline bool item_runs(const ItemState &s, int order)
+{
+    return order > 0 ? (order <= s.orders) : (s.active != 0);
+}
+
+// Largest value / sum over one wavefront (every lane receives it).
+__device__ inline double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ inline double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Order-1 source of one state column (rt_primary_source.c:61-70, 164-171) and,
+// over the flat Fresnel surface, the source of the specularly reflected direct
+// beam scattered once (rt_surface_boundary.c:210-428):
+//   primary(k) = sgn ch[k] (xdel[k] pc + ydel[k] mc)
+//   flat(k)    = sb e^{h[k]/mu_s} (RI (pA xdel[k] + mA ydel[k]) + RQ (pB xdel[k] + mB ydel[k]))
+// with sb = e^{-2 h[nt]/mu_s} / 2 and flat(k) = 0 at the column's boundary level.
+struct ColumnSource {
+    double pc, mc, sgn;
+    bool flat;
+    double pA, mA, pB, mB, RI, RQ, sb;
+    int hemi, nt;
+
+    __device__ ColumnSource(const DevInst &I, const DevItem &it, const Pools &P, const LevelView &lv, int col)
+    {
+        const int n = it.n, dirs = 2 * n + 1, m = it.m, lmax = it.lmax;
+        nt = it.nt;
+        const int type = col / n, c = type >> 1, o = col % n;
+        hemi = type & 1;
+        const int d = hemi ? n - (o + 1) : n + (o + 1);
+        const double *plm = P.basis + it.off_basis;
+        const double *rrl = plm + (long long)(lmax + 1) * dirs;
+        const double *rtl = rrl + (long long)(lmax + 1) * dirs;
+        const double *plm2 = plm + 2LL * dirs, *rrl2 = rrl + 2LL * dirs, *rtl2 = rtl + 2LL * dirs;
+        const long long cnt = (long long)(n + 1) * dirs;
+        const double *tab = P.tab + it.off_tab;
+        const int mol = (m <= 2);
+        const double b0 = (m == 0) ? I.beta0 : 0.0;
+        const double xs = plm2[n];
+        if (c == 0) { pc = tab[0 * cnt + d]; mc = b0 + I.beta2 * plm2[d] * xs; sgn = 1.0; }
+        else if (c == 1) { pc = tab[1 * cnt + d]; mc = mol ? I.gamma2 * rrl2[d] * xs : 0.0; sgn = 1.0; }
+        else { pc = tab[2 * cnt + d]; mc = mol ? I.gamma2 * rtl2[d] * xs : 0.0; sgn = -1.0; }
+        flat = I.surf == 1;
+        pA = mA = pB = mB = RI = RQ = sb = 0.0;
+        if (!flat) return;
+        const double mus = I.mus, nw = I.n_water, n2 = nw * nw;
+        const double tn = sqrt(n2 - 1.0 + mus * mus);
+        const double rp = (n2 * mus - tn) / (n2 * mus + tn), rs = (mus - tn) / (mus + tn);
+        RI = (rp * rp + rs * rs) / 2.0;
+        RQ = (I.q_conv == 1) ? (rp * rp - rs * rs) / 2.0 : (rs * rs - rp * rp) / 2.0;
+        sb = 0.5 * exp(-2.0 * lv.h[nt] / mus);
+        const int dk = hemi ? n + (o + 1) : n - (o + 1);   // mirrored direction
+        const double bP = xs, bR = rrl2[n];
+        double mgr = 0.0, mart = 0.0;
+        if (I.aer) {
+            const double *g = P.greek + I.off_greek;
+            const int q = I.La + 1;
+            const int lend = lmax < I.La ? lmax : I.La;
+            for (int l = m; l <= lend; ++l) {
+                const long long b = (long long)l * dirs;
+                mgr += plm[b + dk] * rrl[b + n] * g[q + l];
+                mart += rtl[b + dk] * rrl[b + n] * g[2 * q + l] + rrl[b + dk] * rtl[b + n] * g[3 * q + l];
+            }
+        }
+        if (c == 0) {
+            pA = tab[0 * cnt + dk]; mA = mol ? b0 + I.beta2 * plm2[dk] * bP : 0.0;
+            pB = mgr; mB = mol ? I.gamma2 * bR * plm2[dk] : 0.0;
+        } else if (c == 1) {
+            pA = tab[1 * cnt + dk]; mA = mol ? I.gamma2 * rrl2[dk] * bP : 0.0;
+            pB = tab[3 * cnt + dk]; mB = mol ? I.alpha2 * bR * rrl2[dk] : 0.0;
+        } else {
+            pA = tab[2 * cnt + dk]; mA = mol ? I.gamma2 * rtl2[dk] * bP : 0.0;
+            pB = mart; mB = mol ? I.alpha2 * rtl2[dk] * bR : 0.0;
+        }
+    }
+    __device__ double primary(const LevelView &lv, int k) const
+    {
+        return sgn * lv.ch[k] * (lv.xd[k] * pc + lv.yd[k] * mc);
+    }
+    __device__ bool flat_zero(int k) const { return (hemi == 0 && k == nt) || (hemi == 1 && k == 0); }
+    __device__ double flat_beam(const LevelView &lv, double mus, int k) const { return sb * exp(lv.h[k] / mus); }
+    __device__ double flat_value(const LevelView &lv, double mus, int k) const
+    {
+        if (!flat || flat_zero(k)) return 0.0;
+        const double x = lv.xd[k], y = lv.yd[k];
+        return flat_beam(lv, mus, k) * (RI * (pA * x + mA * y) + RQ * (pB * x + mB * y));
+    }
+};
+
+// Lower boundary condition of order `it` for the upward columns from the
+// bottom downward field of the previous order (bc + 3n), threads over the
+// outgoing nodes (rt_surface_boundary.c:137-208):
+//   flat  : the Fresnel matrix of the node itself,
+//   rough : sum_i W mu_i w_i R^m[o][i] down(i), W = 2 pi (m = 0) or pi.
+__device__ inline void surface_bc_item(const DevInst &I, const DevItem &it, const Pools &P)
+{
+    const int n = it.n, m = it.m;
+    const double *bot = P.bc + it.off_bc + 3 * n;   // [3][n] downward field at BOA
+    const double *mu = P.ring + I.off_ring, *w = mu + n;
+    double *bc = P.bc + it.off_bc;
+    const double W = (I.surf == 1) ? 2.0 * OCRT_PI : ((m == 0) ? 2.0 * OCRT_PI : OCRT_PI);
+    for (int o = threadIdx.x; o < n; o += blockDim.x) {
+        double si = 0.0, sq = 0.0, su = 0.0;
+        if (I.surf == 1) {
+            double Fr[9];
+            fresnel_R(mu[o], I.n_water, I.q_conv, Fr);
+            const double a = bot[o], b = bot[n + o], cc = bot[2 * n + o];
+            if (w[o] == 0.0) {
+                si = Fr[0] * a + Fr[1] * b + Fr[2] * cc;
+                sq = Fr[3] * a + Fr[4] * b + Fr[5] * cc;
+                su = Fr[6] * a + Fr[7] * b + Fr[8] * cc;
+            } else {
+                const double dd = 1.0 / (2.0 * OCRT_PI * mu[o] * w[o]);
+                const double pw = W * mu[o] * w[o];
+                double R[9];
+                for (int e = 0; e < 9; ++e) R[e] = dd * Fr[e];
+                si = pw * (R[0] * a + R[1] * b + R[2] * cc);
+                sq = pw * (R[3] * a + R[4] * b + R[5] * cc);
+                su = pw * (R[6] * a + R[7] * b + R[8] * cc);
+            }
+        } else {
+            const double *R0 = P.R + I.off_R + ((long long)m * n + o) * n * 9;
+            for (int i = 0; i < n; ++i) {
+                if (w[i] == 0.0) continue;
+                const double a = bot[i], b = bot[n + i], cc = bot[2 * n + i];
+                const double pw = W * mu[i] * w[i];
+                const double *R = R0 + (long long)i * 9;
+                si += pw * (R[0] * a + R[1] * b + R[2] * cc);
+                sq += pw * (R[3] * a + R[4] * b + R[5] * cc);
+                su += pw * (R[6] * a + R[7] * b + R[8] * cc);
+            }
+        }
+        bc[o] = si;
+        bc[n + o] = sq;
+        bc[2 * n + o] = su;
+    }
+}
+
+// Direct sunglint reflectance (surface.c:1813-1907); dlog_out (optional)
+// receives d ln(rho) / d sigma^2 on the rough branch (0 elsewhere).
+__device__ inline void sunglint_rho(double mu_v, double phi_v, double mu_0, double ws, int st, double nw,
+                                    double tau, int qc, double *rho, double *dlog_out = nullptr)
+{
+    const double trans = exp(-tau / mu_0) * exp(-tau / mu_v);
+    if (dlog_out) *dlog_out = 0.0;
+    if (ws < 1e-6) {
+        const double sf = 0.5 * OCRT_PI / 180.0;
+        const double st0 = sqrt(fmax(0.0, 1.0 - mu_0 * mu_0));
+        const double smu = sf * fmax(st0, 1e-3);
+        double de = phi_v;
+        while (de > OCRT_PI) de -= 2.0 * OCRT_PI;
+        while (de < -OCRT_PI) de += 2.0 * OCRT_PI;
+        const double dmu = mu_v - mu_0;
+        const double ex = -0.5 * ((dmu * dmu) / (smu * smu) + (de * de) / (sf * sf));
+        if (ex < -50.0) { rho[0] = rho[1] = rho[2] = 0.0; return; }
+        double MF[9];
+        fresnel_R(mu_0, nw, qc, MF);
+        const double den = fmax(2.0 * mu_0 * st0 * mu_0 * mu_v, 1e-12);
+        const double r = 4.0 * OCRT_PI * trans / den * exp(ex);
+        rho[0] = MF[0] * r; rho[1] = MF[3] * r; rho[2] = MF[6] * r;
+        return;
+    }
+    double R[9], tb2;
+    const double s2v = slope_variance(ws, st);
+    const double shadow = sancer_shadow(mu_v, mu_0, s2v);
+    coxmunk_R_core(mu_v, mu_0, cos(phi_v), sin(phi_v), s2v, shadow, nw, qc, R, &tb2);
+    rho[0] = OCRT_PI * R[0] * trans; rho[1] = OCRT_PI * R[3] * trans; rho[2] = OCRT_PI * R[6] * trans;
+    if (dlog_out && tb2 >= 0.0)
+        *dlog_out = tb2 / (s2v * s2v) - 1.0 / s2v + sancer_dlog(mu_v, mu_0, s2v, shadow);
+}
+
+// One linear-source step of the formal solution written as
+//   r_k = T r_prev + al S_k + be S_prev,  al = (1 - q) / 2,  be = (q - T) / 2,
+//   q = (1 - T) |mu| / dtau  (rt_layer_transport.c:89-99 rearranged),
+// and the derivatives of T, al, be in dtau (adjoint).
+__device__ inline void step_coefs(double dt, double amu, double &T, double &al, double &be, double &dT,
+                                  double &dal, double &dbe)
+{
+    const double x = dt / amu, t = exp(-x), om = -expm1(-x), q = om / x;
+    const double dq = (t - q) / dt;
+    T = t; al = 0.5 * (1.0 - q); be = 0.5 * (q - t);
+    dT = -t / amu; dal = -0.5 * dq; dbe = 0.5 * (dq + t / amu);
+}
+
+}  // namespace ocrt
