@@ -88,6 +88,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time only the mixed form (for counter-collection runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -152,6 +153,32 @@ def main():
     pp.close()
     plain_rate = args.plain_sample / (ms_plain / 1e3)
 
+    adjoint = None
+    if not args.no_adjoint:
+        log("adjoint workload: 256 instances, 200 layers, 32 streams, modes 0..16")
+        d3, k3 = S.config3()
+        pa = ocrt_amd.Plan(d3, precision="fp64", adjoint=True)
+        g3 = np.tile(np.array([1.0, 0.5, -0.5]), (len(d3), 1))
+        t_fwd = timed_steps(pa, 3, 1, sync)
+        # backward only: the execute just before leaves the checkpoints
+        t_adj = 0.0
+        for _ in range(3):
+            pa.execute()
+            sync()
+            t0 = time.perf_counter()
+            pa.adjoint(g3)
+            sync()
+            t_adj += (time.perf_counter() - t0) / 3
+        pa.close()
+        pm3 = ocrt_amd.Plan(d3, precision="mixed")
+        t_mix = timed_steps(pm3, 3, 1, sync)
+        pm3.close()
+        adjoint = {"instances": len(d3), "fp64_forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3,
+                   "mixed_forward_ms": t_mix * 1e3, "adjoint_over_fp64_forward": t_adj / t_fwd,
+                   "adjoint_over_mixed_forward": t_adj / t_mix,
+                   "jacobian_rows_per_s": len(d3) / t_adj}
+        log(f"adjoint: {adjoint}")
+
     single = None
     if not args.no_single:
         log("single instance: Rayleigh + aerosol, 400 layers, 48 streams, modes 0..95, IPSS")
@@ -210,7 +237,8 @@ def main():
         "max_rel_err_I_vs_fp64": err_I,
         "max_abs_err_QU_vs_fp64": err_QU,
         "worst_instance_I": worst,
-        "adjoint_fwd_ratio": None,
+        "adjoint_fwd_ratio": adjoint["adjoint_over_fp64_forward"] if adjoint else None,
+        "adjoint": adjoint,
         "single_instance_config1": single,
         "roofline": roofline,
         "roofline_other": rl_sweep if roofline is rl_gemm else rl_gemm,

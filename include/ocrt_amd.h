@@ -183,6 +183,20 @@ int ocrt_plan_adjoint(ocrt_plan *plan, const double *grad_out, double *grad_leve
 int ocrt_adjoint_batch(const ocrt_instance_t *inst, int n_inst, int precision, const double *grad_out,
                        ocrt_result_t *out, double *grad_levels, double *grad_wind, void *hip_stream);
 
+/* LUT / full-grid output (rt_solve_case_pol_lut, code/OCRT_C/src/rt_solver.h:207,
+ * rt_solver.c:1031-1043 and 1280-1424): after this call every execute also
+ * samples each instance on the grid vza_deg[n_vza] x raa_deg[n_raa] (PCHIP in
+ * the view cosine over the ring nodes, reconstruction at each azimuth, IPSS
+ * factor per direction, direct glint).  Call once per plan, before execute. */
+int ocrt_plan_set_lut(ocrt_plan *plan, int n_vza, const double *vza_deg, int n_raa, const double *raa_deg);
+
+/* Copy the LUT of the last execute (synchronises):
+ *   grid     [n_inst][n_vza][n_raa][6]: rho_I, rho_Q, rho_U, T_diff_dn_dir,
+ *            T_sg_up_dir, T_total_up_dir (rt_lut_grid_out_t, rt_solver.h:180-205)
+ *   scalars  [n_inst][2]: T_dir_dn, T_diff_dn_hemi
+ * either pointer may be NULL. */
+int ocrt_plan_lut_results(ocrt_plan *plan, double *grid, double *scalars);
+
 /* One-call form: plan, execute, copy results, destroy. */
 int ocrt_solve_batch(const ocrt_instance_t *inst, int n_inst, int precision,
                      ocrt_result_t *out, void *hip_stream);

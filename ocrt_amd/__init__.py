@@ -56,6 +56,8 @@ def library():
         lib.ocrt_solve_batch.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.POINTER(Result),
                                          C.c_void_p]
         lib.ocrt_version.restype = C.c_char_p
+        lib.ocrt_plan_set_lut.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+        lib.ocrt_plan_lut_results.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_plan_create_ex.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         lib.ocrt_plan_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_adjoint_batch.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.c_void_p, C.POINTER(Result),
@@ -89,6 +91,31 @@ class Plan:
         if rc != 0:
             raise OcrtError(f"ocrt_plan_create failed with code {rc}")
         self.mmax = self.lib.ocrt_plan_mmax(self._handle)
+
+    def set_lut(self, vza_deg, raa_deg):
+        """Sample every following execute on the grid vza x raa (full-grid / LUT mode)."""
+        self._lut_vza = np.ascontiguousarray(vza_deg, dtype=np.float64)
+        self._lut_raa = np.ascontiguousarray(raa_deg, dtype=np.float64)
+        rc = self.lib.ocrt_plan_set_lut(self._handle, len(self._lut_vza), self._lut_vza.ctypes.data_as(C.c_void_p),
+                                        len(self._lut_raa), self._lut_raa.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_set_lut failed with code {rc}")
+
+    def lut(self):
+        """LUT of the last execute: arrays [n_inst, n_vza, n_raa] per column and the
+        per-instance T_dir_dn, T_diff_dn_hemi."""
+        nv, nr = len(self._lut_vza), len(self._lut_raa)
+        grid = np.zeros((self.n, nv, nr, 6))
+        scal = np.zeros((self.n, 2))
+        rc = self.lib.ocrt_plan_lut_results(self._handle, grid.ctypes.data_as(C.c_void_p),
+                                            scal.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_lut_results failed with code {rc}")
+        names = ("rho_I", "rho_Q", "rho_U", "T_diff_dn_dir", "T_sg_up_dir", "T_total_up_dir")
+        out = {k: grid[..., q] for q, k in enumerate(names)}
+        out["T_dir_dn"] = scal[:, 0]
+        out["T_diff_dn_hemi"] = scal[:, 1]
+        return out
 
     def adjoint(self, grad_out, stream=None):
         """Gradient of sum_i grad_out[i] . (rho_I, rho_Q, rho_U)_i (adjoint plans).

@@ -44,6 +44,9 @@ __global__ void sweep_adj(const DevInst *, const DevItem *, Pools, const ItemSta
 __global__ void surface_adj(const DevInst *, const DevItem *, Pools, const ItemState *, int);
 __global__ void order_one_adj(const DevInst *, const DevItem *, Pools, const ColTile *, int);
 __global__ void glint_adj(const DevInst *, int, Pools);
+// lut.hip
+__global__ void lut_eval(const DevInst *, const DevItem *, Pools, const InstOut *, const int *, const double *, int,
+                         const double *, int, const double *, double *, double *);
 }  // namespace ocrt
 
 using namespace ocrt;
@@ -123,6 +126,9 @@ struct Chunk {
     std::vector<GemmTile> htiles;
     std::vector<ColTile> hcols;
     std::vector<IpssJob> hjobs;
+    std::vector<IpssJob> hjobs_lut;   // IPSS factors of every LUT direction
+    IpssJob *djobs_lut = nullptr;
+    int *djob_lut_of = nullptr;       // first LUT job of each instance, -1 without IPSS
     DevInst *dinst = nullptr;
     DevItem *ditem = nullptr;
     GemmTile *dtiles = nullptr;
@@ -213,6 +219,11 @@ struct ocrt_plan {
     void *seg[3] = {nullptr, nullptr, nullptr};
     double *grad = nullptr, *gout = nullptr;
     long long grad_n = 0;
+    // LUT output (ocrt_plan_set_lut): per-node fields of every item and the grid
+    int lut = 0, lut_nv = 0, lut_nr = 0;
+    long long nodes_n = 0;
+    double *nodes = nullptr, *lut_vza = nullptr, *lut_raa = nullptr, *lut_grid = nullptr, *lut_scal = nullptr;
+    double *kappa_lut = nullptr, *taudir_lut = nullptr;
 };
 
 // Orders between two checkpoints of the adjoint's forward pass.
@@ -465,6 +476,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 it.off_F = F; it.off_tot = F; F += (long long)L * Pp;
                 it.off_bc = bc; bc += 6LL * n;
                 it.off_cmp = cmp; cmp += mixed ? 2LL * Pp : 0;
+                it.off_nodes = p->nodes_n;
+                p->nodes_n += 4LL * n;
                 it.fsplit = (mixed && m < kSplitModes) ? 1 : 0;
                 it.off_Flo = flo; flo += it.fsplit ? (long long)L * Pp : 0;
                 c.max_pack = std::max(c.max_pack, (long long)Mp * it.K);
@@ -544,7 +557,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     P.greek = p->greek;
     P.layers = p->layers;
     P.cm_w = p->cm_w;
-    P.toa_up = nullptr;
+    P.nodes = nullptr;
     // ---- uploads of inputs and descriptors
     std::vector<double> hring(ring_n), hgreek(std::max(greek_n, 1LL)), hlayers(std::max(layers_n, 1LL));
     for (int ii = 0; ii < n_inst; ++ii) {
@@ -688,10 +701,15 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
         hipLaunchKernelGGL(order_status<__hip_bfloat16>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, c.maxit + 1);
     else
         hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, c.maxit + 1);
-    hipLaunchKernelGGL(extract, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, io, mixed ? 1 : 0);
+    Pools Pe = P;
+    Pe.nodes = p->lut ? p->nodes : nullptr;
+    hipLaunchKernelGGL(extract, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pe, st, io, mixed ? 1 : 0);
     if (c.n_jobs > 0)
         hipLaunchKernelGGL(ipss_kappa, dim3(c.n_jobs), dim3(256), 0, s, c.dinst, c.djobs, P,
                            p->kappa + c.i0, p->taudir + c.i0);
+    if (p->lut && !c.hjobs_lut.empty())
+        hipLaunchKernelGGL(ipss_kappa, dim3((unsigned)c.hjobs_lut.size()), dim3(256), 0, s, c.dinst, c.djobs_lut, P,
+                           p->kappa_lut, p->taudir_lut);
     return hipGetLastError() == hipSuccess ? 0 : OCRT_ERR_DEVICE;
 }
 
@@ -774,7 +792,68 @@ static int forward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
                        p->taudir + c.i0, c.djob_of, p->ipss + c.i0);
     hipLaunchKernelGGL(finalize, dim3((ni + 127) / 128), dim3(128), 0, s, c.dinst, ni, p->iout + c.it0,
                        p->ipss + c.i0, p->pools, p->res + c.i0);
+    if (p->lut) {
+        Pools Pl = p->pools;
+        Pl.nodes = p->nodes;
+        hipLaunchKernelGGL(lut_eval, dim3(p->lut_nv, ni), dim3(128), 4 * (p->mmax_all + 1) * sizeof(double), s,
+                           c.dinst, c.ditem, Pl, p->ipss + c.i0, c.djob_lut_of, p->kappa_lut, p->lut_nv, p->lut_vza,
+                           p->lut_nr, p->lut_raa, p->lut_grid, p->lut_scal);
+    }
     return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
+}
+
+extern "C" int ocrt_plan_set_lut(ocrt_plan *p, int n_vza, const double *vza_deg, int n_raa, const double *raa_deg)
+{
+    if (!p || n_vza < 1 || n_raa < 1 || !vza_deg || !raa_deg || p->lut) return OCRT_ERR_ARGS;
+    for (int i = 0; i < n_vza; ++i)
+        if (!(vza_deg[i] >= 0.0 && vza_deg[i] < 90.0)) return OCRT_ERR_ARGS;
+    for (int i = 0; i < n_raa; ++i)
+        if (!std::isfinite(raa_deg[i])) return OCRT_ERR_ARGS;
+    const long long grid = (long long)n_vza * n_raa;
+    int rc = 0;
+    rc |= dmalloc(p, (void **)&p->nodes, p->nodes_n * 8);
+    rc |= dmalloc(p, (void **)&p->lut_vza, (size_t)n_vza * 8);
+    rc |= dmalloc(p, (void **)&p->lut_raa, (size_t)n_raa * 8);
+    rc |= dmalloc(p, (void **)&p->lut_grid, (size_t)p->n_inst * grid * 6 * 8);
+    rc |= dmalloc(p, (void **)&p->lut_scal, (size_t)p->n_inst * 2 * 8);
+    size_t max_jobs = 1;
+    for (Chunk &c : p->chunks) {
+        c.hjobs_lut.clear();
+        std::vector<int> first(c.i1 - c.i0, -1);
+        for (int i = c.i0; i < c.i1; ++i) {
+            if (!p->inst[i].d.ipss) continue;
+            first[i - c.i0] = (int)c.hjobs_lut.size();
+            for (int iv = 0; iv < n_vza; ++iv)
+                for (int ir = 0; ir < n_raa; ++ir) c.hjobs_lut.push_back(IpssJob{i - c.i0, 0, vza_deg[iv], raa_deg[ir]});
+        }
+        max_jobs = std::max(max_jobs, c.hjobs_lut.size());
+        rc |= dmalloc(p, (void **)&c.djob_lut_of, first.size() * sizeof(int));
+        rc |= dmalloc(p, (void **)&c.djobs_lut, std::max<size_t>(c.hjobs_lut.size(), 1) * sizeof(IpssJob));
+        if (rc) return OCRT_ERR_ALLOC;
+        HIPCHK(hipMemcpy(c.djob_lut_of, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!c.hjobs_lut.empty())
+            HIPCHK(hipMemcpy(c.djobs_lut, c.hjobs_lut.data(), c.hjobs_lut.size() * sizeof(IpssJob),
+                             hipMemcpyHostToDevice));
+    }
+    rc |= dmalloc(p, (void **)&p->kappa_lut, max_jobs * 8);
+    rc |= dmalloc(p, (void **)&p->taudir_lut, max_jobs * 8);
+    if (rc) return OCRT_ERR_ALLOC;
+    HIPCHK(hipMemcpy(p->lut_vza, vza_deg, (size_t)n_vza * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p->lut_raa, raa_deg, (size_t)n_raa * 8, hipMemcpyHostToDevice));
+    p->lut_nv = n_vza;
+    p->lut_nr = n_raa;
+    p->lut = 1;
+    return OCRT_OK;
+}
+
+extern "C" int ocrt_plan_lut_results(ocrt_plan *p, double *grid, double *scalars)
+{
+    if (!p || !p->lut) return OCRT_ERR_ARGS;
+    HIPCHK(hipDeviceSynchronize());
+    const size_t g = (size_t)p->n_inst * p->lut_nv * p->lut_nr * 6;
+    if (grid) HIPCHK(hipMemcpy(grid, p->lut_grid, g * 8, hipMemcpyDeviceToHost));
+    if (scalars) HIPCHK(hipMemcpy(scalars, p->lut_scal, (size_t)p->n_inst * 2 * 8, hipMemcpyDeviceToHost));
+    return OCRT_OK;
 }
 
 extern "C" int ocrt_plan_execute(ocrt_plan *p, void *stream)

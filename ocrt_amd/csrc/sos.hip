@@ -13,6 +13,8 @@
 //   extract / finalize : view sampling, Fourier reconstruction, reflectance,
 //                    IPSS scaling, direct glint and transmittances
 //                    (rt_solver.c:1010-1276, rt_fourier.c:9-33, surface.c:1813-1907)
+#include <type_traits>
+
 #include "ocrt_amd.h"
 #include "sos_common.h"
 
@@ -208,7 +210,10 @@ __global__ __launch_bounds__(256) void gemm_f64(const DevInst *__restrict__ inst
             for (int b = 0; b < 2; ++b) acc[q][a][b] = (double4_){0.0, 0.0, 0.0, 0.0};
     const int lr = t >> 2, lk = (t & 3) * 4;
     const int lev_load = lev0 + lr;
-    for (int q = 0; q < nparts; ++q) {
+    // one K sweep per operator part; the part index is a compile-time
+    // constant so the accumulators stay in registers
+    auto run_part = [&](auto part) {
+        constexpr int q = decltype(part)::value;
         for (int k0 = 0; k0 < Pp; k0 += 16) {
             const double *src = A + (long long)(row0 + lr) * K + q * Pp + k0 + lk;
             for (int e = 0; e < 4; ++e) As[lr][lk + e] = src[e];
@@ -230,7 +235,9 @@ __global__ __launch_bounds__(256) void gemm_f64(const DevInst *__restrict__ inst
             }
             __syncthreads();
         }
-    }
+    };
+    run_part(std::integral_constant<int, 0>{});
+    if (nparts > 1) run_part(std::integral_constant<int, 1>{});
     // f64 16x16 result layout: lane holds rows (lane >> 4) + 4 i of column lane & 15
     const int kind0 = it.parts & 0xff;
     double *J2 = reinterpret_cast<double *>(P.J2) + it.off_F;
@@ -361,7 +368,10 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
 // The order sum is fp64 over the whole field (it feeds the convergence test).
 // In the mixed form the two rows that become outputs, k = 0 (TOA) and k = nt
 // (BOA), also carry an fp32 Neumaier compensation term.
-constexpr int kU = 8;
+#ifndef OCRT_SWEEP_UNROLL
+#define OCRT_SWEEP_UNROLL 8
+#endif
+constexpr int kU = OCRT_SWEEP_UNROLL;
 
 template <typename FT, typename JT, bool MIXED>
 __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
@@ -538,16 +548,23 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
         out[item] = r;
         (void)hs;
     }
-    if (P.toa_up) {
-        double *dst = P.toa_up + (long long)item * P.toa_up_stride;
-        for (int q = threadIdx.x; q < 3 * n; q += blockDim.x) {
+    if (P.nodes) {
+        // every node of the TOA upward field (rough seed removed) and of the
+        // BOA downward intensity, for the LUT's view sampling
+        double *dst = P.nodes + it.off_nodes;
+        for (int q = threadIdx.x; q < 4 * n; q += blockDim.x) {
             const int c = q / n, o = q % n;
-            double val = total_at(P, it, mixed, 0, 2 * c * n + o);
-            if (I.surf == 2) {
-                const double tau = lv.h[nt], down = exp(-tau / I.mus);
-                const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
-                val -= P.R[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c] *
-                       (FS * (down * exp(-(tau - lv.h[0]) / mu[o])));
+            double val;
+            if (c < 3) {
+                val = total_at(P, it, mixed, 0, 2 * c * n + o);
+                if (I.surf == 2) {
+                    const double tau = lv.h[nt], down = exp(-tau / I.mus);
+                    const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
+                    val -= P.R[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c] *
+                           (FS * (down * exp(-(tau - lv.h[0]) / mu[o])));
+                }
+            } else {
+                val = total_at(P, it, mixed, nt, n + o);
             }
             dst[q] = val;
         }
