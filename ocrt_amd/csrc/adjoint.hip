@@ -18,8 +18,8 @@ namespace ocrt {
 
 __global__ void fill_f64(double *__restrict__ p, long long n, double v)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 // Reconstruction weight of mode m for Stokes component c over 1 / mu_s

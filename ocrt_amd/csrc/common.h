@@ -97,6 +97,26 @@ struct Pools {
     double *nodes;        // LUT plans: per item [4][n] TOA upward I, Q, U and BOA downward I at every node
 };
 
+// Workgroups of a launch are dealt round-robin over the 8 XCDs (block b runs
+// on XCD b mod 8, each XCD with its own L2).  Launches that give every item X
+// blocks decode their block index so that all blocks of an item share one
+// XCD and its L2: item = b mod 8 + 8 (b / 8 / X), part = (b / 8) mod X, over
+// a grid of ceil(items / 8) * 8 * X blocks (blocks past the last item idle).
+constexpr int kXcdCount = 8;
+__device__ inline void xcd_item_block(int X, int &item, int &part)
+{
+    const int b = blockIdx.x, w = b / kXcdCount;
+    item = b % kXcdCount + kXcdCount * (w / X);
+    part = w % X;
+}
+__host__ inline unsigned xcd_item_grid(int items, int X)
+{
+    return (unsigned)(((items + kXcdCount - 1) / kXcdCount) * kXcdCount * X);
+}
+
+// Rows of the phase tables computed by one workgroup.
+constexpr int kTabRows = 8;
+
 struct GemmTile { int item, tm, tn, pad; };   // item < 0: idle slot
 
 // Epilogues of the scattering-integral kernels.

@@ -21,8 +21,9 @@ __global__ void medium_fractions(const DevInst *, Pools);
 __global__ void fourier_weights(double *, int);
 template <bool DERIV> __global__ void coxmunk_fourier(const DevInst *, Pools);
 __global__ void angular_basis(const DevInst *, const DevItem *, Pools);
-__global__ void phase_tables(const DevInst *, const DevItem *, Pools);
-template <typename AT, bool TRANS> __global__ void pack_operator(const DevInst *, const DevItem *, Pools);
+__global__ void phase_tables(const DevInst *, const DevItem *, Pools, int, int);
+template <typename AT, bool TRANS>
+__global__ void pack_operator(const DevInst *, const DevItem *, Pools, int, int);
 template <typename FT, bool MIXED>
 __global__ void order_one(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int);
 template <typename FT> __global__ void order_status(const DevInst *, const DevItem *, Pools, ItemState *, int);
@@ -135,7 +136,7 @@ struct Chunk {
     ColTile *dcols = nullptr;
     IpssJob *djobs = nullptr;
     int *djob_of = nullptr;
-    int max_L = 0, max_P = 0, max_pairs = 0, max_tab = 0, maxit = 1, any_rough = 0, n_jobs = 0;
+    int max_L = 0, max_P = 0, max_pairs = 0, max_tab = 0, tab_groups = 1, maxit = 1, any_rough = 0, n_jobs = 0;
     long long max_pack = 0;
     long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0, cmp = 0, flo = 0;
 };
@@ -456,6 +457,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             c.max_P = std::max(c.max_P, Pp);
             c.max_pairs = std::max(c.max_pairs, n * n + n);
             c.max_tab = std::max(c.max_tab, (n + 1) * (2 * n + 1));
+            c.tab_groups = std::max(c.tab_groups, (n + 1 + kTabRows - 1) / kTabRows);
             c.maxit = std::max(c.maxit, I.max_orders);
             c.any_rough |= (I.surf == 2);
             for (int m = 0; m <= I.mmax; ++m) {
@@ -511,6 +513,10 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         fprintf(stderr, "ocrt plan: %zu chunks items %d ring %lld greek %lld lev %lld T %lld R %lld basis %lld tab %lld "
                         "A %lld F %lld bc %lld\n", p->chunks.size(), p->n_items, ring_n, greek_n, mx_lev, mx_T,
                 mx_R, mx_basis, mx_tab, mx_A, mx_F, mx_bc);
+    if (getenv("OCRT_DEBUG_PLAN"))
+        for (const Chunk &c : p->chunks)
+            fprintf(stderr, "  chunk instances [%d, %d) items %d tiles %zu cols %zu max_pack %lld tab_groups %d\n", c.i0,
+                    c.i1, c.it1 - c.it0, c.htiles.size(), c.hcols.size(), c.max_pack, c.tab_groups);
     int rc = 0;
     Pools &P = p->pools;
     rc |= dmalloc(p, (void **)&p->ring, ring_n * 8);
@@ -642,18 +648,23 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             hipLaunchKernelGGL(coxmunk_fourier<false>, dim3(c.max_pairs, ni), dim3(256), 0, s, c.dinst, P);
     }
     hipLaunchKernelGGL(angular_basis, dim3(nit), dim3(128), 0, s, c.dinst, c.ditem, P);
-    hipLaunchKernelGGL(phase_tables, dim3((c.max_tab + 255) / 256, nit), dim3(256), 0, s, c.dinst, c.ditem, P);
-    const unsigned pack_blocks = (unsigned)((c.max_pack + 255) / 256);
+    hipLaunchKernelGGL(phase_tables, dim3(xcd_item_grid(nit, c.tab_groups)), dim3(256), 0, s, c.dinst, c.ditem, P,
+                       nit, c.tab_groups);
+    // a few blocks per item striding over its operator: an item's table reads
+    // stay in one L2 and the launch stays far below 2^31 work-items
+    const int pack_blocks = (int)std::min<long long>(16, (c.max_pack + 255) / 256);
+    const dim3 pack_grid(xcd_item_grid(nit, pack_blocks));
     if (mixed)
-        hipLaunchKernelGGL((pack_operator<__hip_bfloat16, false>), dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst,
-                           c.ditem, P);
+        hipLaunchKernelGGL((pack_operator<__hip_bfloat16, false>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem, P,
+                           nit, pack_blocks);
     else
-        hipLaunchKernelGGL((pack_operator<double, false>), dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst, c.ditem, P);
+        hipLaunchKernelGGL((pack_operator<double, false>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem, P, nit,
+                           pack_blocks);
     if (p->adjoint) {
         Pools Pt = P;
         Pt.A = p->At;
-        hipLaunchKernelGGL((pack_operator<double, true>), dim3(pack_blocks, nit), dim3(256), 0, s, c.dinst, c.ditem,
-                           Pt);
+        hipLaunchKernelGGL((pack_operator<double, true>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem, Pt, nit,
+                           pack_blocks);
     }
     // adjoint plans keep u_1, u_5, u_9, ... of every item
     auto checkpoint = [&](int order) -> int {
@@ -727,7 +738,8 @@ static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     const unsigned ntiles = (unsigned)c.htiles.size();
     const int maxit = c.maxit;
     hipLaunchKernelGGL(adj_zero_grad, dim3(4, ni), dim3(256), 0, s, c.dinst, P0);
-    hipLaunchKernelGGL(fill_f64, dim3((unsigned)((c.F + 255) / 256)), dim3(256), 0, s, P0.lam, c.F, 0.0);
+    hipLaunchKernelGGL(fill_f64, dim3((unsigned)std::min<long long>((c.F + 255) / 256, 65536)), dim3(256), 0, s,
+                       P0.lam, c.F, 0.0);
     hipLaunchKernelGGL(adj_seed, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P0);
     auto slot = [&](int seg, int j) -> void * { return j == 0 ? p->ckpt[seg] : p->seg[j - 1]; };
     for (int sg = (maxit - 1) / kCkptEvery; sg >= 0; --sg) {

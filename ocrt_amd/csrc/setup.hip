@@ -280,48 +280,99 @@ __global__ void angular_basis(const DevInst *__restrict__ inst, const DevItem *_
     }
 }
 
-// Six phase tables [(n+1)][2n+1] of one item; one thread per (j, k).
-__global__ void phase_tables(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P)
+// Six phase tables [(n+1)][2n+1] of one item (rt_angular_basis.c:278-358):
+//   P[j][k]   = sum_l plm_j plm_k beta_l       GR[j][k] = sum_l plm_j rrl_k gamma_l
+//   GT[j][k]  = sum_l plm_j rtl_k gamma_l      ATT      = sum_l rtl_j rtl_k alpha_l + rrl_j rrl_k zeta_l
+//   ARR       = sum_l rtl_j rtl_k zeta_l + rrl_j rrl_k alpha_l
+//   ART       = sum_l rtl_j rrl_k alpha_l + rrl_j rtl_k zeta_l
+// for rows j = 0..n (directions j + n) and every column direction k.  One
+// workgroup per (item, group of kTabRows rows), the groups of an item on one
+// XCD; the rows' basis values and the Greek coefficients are staged in LDS
+// in chunks of kTabL orders and every thread accumulates its column for all
+// rows of the group.
+constexpr int kTabL = 64;
+
+__global__ __launch_bounds__(256) void phase_tables(const DevInst *__restrict__ inst,
+                                                    const DevItem *__restrict__ items, Pools P, int n_items,
+                                                    int groups)
 {
-    const DevItem &it = items[blockIdx.y];
+    int item, g;
+    xcd_item_block(groups, item, g);
+    if (item >= n_items) return;
+    const DevItem &it = items[item];
     const DevInst &I = inst[it.inst];
     const int n = it.n, dirs = 2 * n + 1, m = it.m, lmax = it.lmax;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (n + 1) * dirs) return;
-    const int j = idx / dirs, k = idx % dirs;
+    const int j0 = g * kTabRows;
+    if (j0 > n) return;
+    const int nr = min(kTabRows, n + 1 - j0);
     const long long cnt = (long long)(n + 1) * dirs;
     double *tab = P.tab + it.off_tab;
     if (!I.aer) {
-        for (int t = 0; t < 6; ++t) tab[t * cnt + idx] = 0.0;
+        for (int e = threadIdx.x; e < nr * dirs; e += blockDim.x)
+            for (int t = 0; t < 6; ++t) tab[t * cnt + (long long)j0 * dirs + e] = 0.0;
         return;
     }
     const double *plm = P.basis + it.off_basis;
     const double *rrl = plm + (long long)(lmax + 1) * dirs;
     const double *rtl = rrl + (long long)(lmax + 1) * dirs;
-    const double *g = P.greek + I.off_greek;
+    const double *gk = P.greek + I.off_greek;
     const int q = I.La + 1;
-    const int jd = j + n;
-    double p = 0, gr = 0, gt = 0, rr = 0, rt = 0, tt = 0;
     const int lend = lmax < I.La ? lmax : I.La;
-    for (int l = m; l <= lend; ++l) {
-        const long long b = (long long)l * dirs;
-        const double pj = plm[b + jd], pk = plm[b + k];
-        const double rj = rrl[b + jd], rk = rrl[b + k];
-        const double tj = rtl[b + jd], tk = rtl[b + k];
-        const double be = g[l], ga = g[q + l], al = g[2 * q + l], ze = g[3 * q + l];
-        p += pj * pk * be;
-        gr += pj * rk * ga;
-        gt += pj * tk * ga;
-        tt += tj * tk * al + rj * rk * ze;
-        rr += tj * tk * ze + rj * rk * al;
-        rt += tj * rk * al + rj * tk * ze;
+    __shared__ double sp[kTabL][kTabRows], sr[kTabL][kTabRows], st[kTabL][kTabRows];
+    __shared__ double cb[kTabL], cg[kTabL], ca[kTabL], cz[kTabL];
+    for (int k0 = 0; k0 < dirs; k0 += blockDim.x) {
+        const int k = k0 + threadIdx.x;
+        const bool live = k < dirs;
+        double acc[kTabRows][6];
+#pragma unroll
+        for (int r = 0; r < kTabRows; ++r)
+#pragma unroll
+            for (int t = 0; t < 6; ++t) acc[r][t] = 0.0;
+        for (int l0 = m; l0 <= lend; l0 += kTabL) {
+            const int nl = min(kTabL, lend - l0 + 1);
+            __syncthreads();
+            for (int e = threadIdx.x; e < nl * kTabRows; e += blockDim.x) {
+                const int li = e / kTabRows, r = e % kTabRows;
+                const long long b = (long long)(l0 + li) * dirs + j0 + r + n;
+                const bool in = r < nr;
+                sp[li][r] = in ? plm[b] : 0.0;
+                sr[li][r] = in ? rrl[b] : 0.0;
+                st[li][r] = in ? rtl[b] : 0.0;
+            }
+            for (int li = threadIdx.x; li < nl; li += blockDim.x) {
+                const int l = l0 + li;
+                cb[li] = gk[l]; cg[li] = gk[q + l]; ca[li] = gk[2 * q + l]; cz[li] = gk[3 * q + l];
+            }
+            __syncthreads();
+            if (live) {
+                for (int li = 0; li < nl; ++li) {
+                    const long long b = (long long)(l0 + li) * dirs + k;
+                    const double pk = plm[b], rk = rrl[b], tk = rtl[b];
+                    const double pkb = pk * cb[li], rkg = rk * cg[li], tkg = tk * cg[li];
+                    const double tka = tk * ca[li], tkz = tk * cz[li], rka = rk * ca[li], rkz = rk * cz[li];
+#pragma unroll
+                    for (int r = 0; r < kTabRows; ++r) {
+                        const double pj = sp[li][r], rj = sr[li][r], tj = st[li][r];
+                        acc[r][0] += pj * pkb;
+                        acc[r][1] += pj * rkg;
+                        acc[r][2] += pj * tkg;
+                        acc[r][3] += tj * tkz + rj * rka;
+                        acc[r][4] += tj * rka + rj * tkz;
+                        acc[r][5] += tj * tka + rj * rkz;
+                    }
+                }
+            }
+        }
+        if (live) {
+#pragma unroll
+            for (int r = 0; r < kTabRows; ++r) {
+                if (r >= nr) break;
+                const long long idx = (long long)(j0 + r) * dirs + k;
+#pragma unroll
+                for (int t = 0; t < 6; ++t) tab[t * cnt + idx] = acc[r][t];
+            }
+        }
     }
-    tab[0 * cnt + idx] = p;
-    tab[1 * cnt + idx] = gr;
-    tab[2 * cnt + idx] = gt;
-    tab[3 * cnt + idx] = rr;
-    tab[4 * cnt + idx] = rt;
-    tab[5 * cnt + idx] = tt;
 }
 
 // Coefficient selector of the operator: for output type (row block) and input
@@ -400,56 +451,62 @@ __device__ inline double pair_molecular(const DevInst &I, int m, const double *p
 // TRANS packs the transposed operator (A_part^T of every part) in the same
 // layout, for the adjoint's M^T products.
 template <typename AT, bool TRANS>
-__global__ void pack_operator(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P)
+__global__ void pack_operator(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P,
+                              int n_items, int blocks_per_item)
 {
-    const DevItem &it = items[blockIdx.y];
+    int item, bx;
+    xcd_item_block(blocks_per_item, item, bx);   // an item's tables stay in one XCD's L2
+    if (item >= n_items) return;
+    const DevItem &it = items[item];
     const DevInst &I = inst[it.inst];
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)it.Mp * it.K;
-    if (idx >= total) return;
-    const int r = (int)(idx / it.K), c = (int)(idx % it.K);
     const int n = it.n, dirs = 2 * n + 1, six_n = 6 * n;
     AT *A = reinterpret_cast<AT *>(P.A) + it.off_A;
-    const int part = c / it.P, cc = c % it.P;
-    // fp64: A[Mp][K], parts side by side along K.  bf16: [part][hi, lo][Mp][P].
-    const long long dst = (sizeof(AT) == 2) ? ((long long)(part * 2) * it.Mp + r) * it.P + cc : idx;
-    const long long lo_off = (long long)it.Mp * it.P;
-    double v = 0.0;
-    if (r < six_n && cc < six_n) {
-        const int ro = TRANS ? cc : r, ci = TRANS ? r : cc;   // (outgoing, incoming) state columns
-        const int to = ro / n, o = ro % n + 1;
-        const int ti = ci / n, i = ci % n + 1;
-        const double w = P.ring[I.off_ring + n + (i - 1)];
-        if (w != 0.0) {
-            const int q = c_sel_q[to][ti];
-            const int s = c_sel_s[to][ti];
-            const int kind = (it.parts >> (8 * part)) & 0xff;   // 1 particle, 2 molecular
-            double e;
-            if (kind == 1) {
-                const long long cnt = (long long)(n + 1) * dirs;
-                e = pair_particle(P.tab + it.off_tab, cnt, dirs, n, q, o, i);
-            } else {
-                const double *plm = P.basis + it.off_basis;
-                const double *plm2 = plm + 2LL * dirs;
-                const double *rrl2 = plm + (long long)(it.lmax + 1) * dirs + 2LL * dirs;
-                const double *rtl2 = plm + 2LL * (it.lmax + 1) * dirs + 2LL * dirs;
-                e = pair_molecular(I, it.m, plm2, rrl2, rtl2, n, q, o, i);
+    // the item's blocks stride over its entries
+    for (long long idx = (long long)bx * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)blocks_per_item * blockDim.x) {
+        const int r = (int)(idx / it.K), c = (int)(idx % it.K);
+        const int part = c / it.P, cc = c % it.P;
+        // fp64: A[Mp][K], parts side by side along K.  bf16: [part][hi, lo][Mp][P].
+        const long long dst = (sizeof(AT) == 2) ? ((long long)(part * 2) * it.Mp + r) * it.P + cc : idx;
+        const long long lo_off = (long long)it.Mp * it.P;
+        double v = 0.0;
+        if (r < six_n && cc < six_n) {
+            const int ro = TRANS ? cc : r, ci = TRANS ? r : cc;   // (outgoing, incoming) state columns
+            const int to = ro / n, o = ro % n + 1;
+            const int ti = ci / n, i = ci % n + 1;
+            const double w = P.ring[I.off_ring + n + (i - 1)];
+            if (w != 0.0) {
+                const int q = c_sel_q[to][ti];
+                const int s = c_sel_s[to][ti];
+                const int kind = (it.parts >> (8 * part)) & 0xff;   // 1 particle, 2 molecular
+                double e;
+                if (kind == 1) {
+                    const long long cnt = (long long)(n + 1) * dirs;
+                    e = pair_particle(P.tab + it.off_tab, cnt, dirs, n, q, o, i);
+                } else {
+                    const double *plm = P.basis + it.off_basis;
+                    const double *plm2 = plm + 2LL * dirs;
+                    const double *rrl2 = plm + (long long)(it.lmax + 1) * dirs + 2LL * dirs;
+                    const double *rtl2 = plm + 2LL * (it.lmax + 1) * dirs + 2LL * dirs;
+                    e = pair_molecular(I, it.m, plm2, rrl2, rtl2, n, q, o, i);
+                }
+                v = w * (double)s * e;
             }
-            v = w * (double)s * e;
         }
-    }
-    A[dst] = to_storage<AT>(v);
-    if (sizeof(AT) == 2) {
-        // bf16 operator carried as hi + lo: the rounding of a fixed operator is
-        // systematic across orders, the residual term removes it to ~2^-16
-        const double hi = (double)__bfloat162float(*reinterpret_cast<const __hip_bfloat16 *>(&A[dst]));
-        A[dst + lo_off] = to_storage<AT>(v - hi);
+        A[dst] = to_storage<AT>(v);
+        if (sizeof(AT) == 2) {
+            // bf16 operator carried as hi + lo: the rounding of a fixed operator is
+            // systematic across orders, the residual term removes it to ~2^-16
+            const double hi = (double)__bfloat162float(*reinterpret_cast<const __hip_bfloat16 *>(&A[dst]));
+            A[dst + lo_off] = to_storage<AT>(v - hi);
+        }
     }
 }
 
-template __global__ void pack_operator<double, false>(const DevInst *, const DevItem *, Pools);
-template __global__ void pack_operator<__hip_bfloat16, false>(const DevInst *, const DevItem *, Pools);
-template __global__ void pack_operator<double, true>(const DevInst *, const DevItem *, Pools);
-template __global__ void pack_operator<__hip_bfloat16, true>(const DevInst *, const DevItem *, Pools);
+template __global__ void pack_operator<double, false>(const DevInst *, const DevItem *, Pools, int, int);
+template __global__ void pack_operator<__hip_bfloat16, false>(const DevInst *, const DevItem *, Pools, int, int);
+template __global__ void pack_operator<double, true>(const DevInst *, const DevItem *, Pools, int, int);
+template __global__ void pack_operator<__hip_bfloat16, true>(const DevInst *, const DevItem *, Pools, int, int);
 
 }  // namespace ocrt

@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     if (ct.item < 0) return;
     const DevItem &it = items[ct.item];
     const DevInst &I = inst[it.inst];
-    const int n = it.n, dirs = 2 * n + 1, nt = it.nt, Pp = it.P, m = it.m, lmax = it.lmax;
+    const int n = it.n, nt = it.nt, Pp = it.P, m = it.m;
     const int col = ct.col0 + lane;
     if (ct.col0 == 0 && lane == 0) {
         ItemState s;
@@ -117,30 +117,27 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     const double mu = mu_ring[o];
     const double *T = P.T + I.off_T;
     const ColumnSource src(I, it, P, lv, col);
-    sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o, [&](int k) { return src.primary(lv, k); },
-                     [&](int k, double v) { tot[(long long)k * Pp + col] = v; });
-    if (I.surf == 1) {
-        // flat Fresnel: the specularly reflected direct beam scattered once
-        // (rt_surface_boundary.c:210-428)
-        sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o, [&](int k) { return src.flat_value(lv, I.mus, k); },
-                         [&](int k, double v) { tot[(long long)k * Pp + col] += v; });
-    } else if (I.surf == 2 && hemi == 0) {
-        // rough ocean: direct beam reflected into every upward node
-        // (rt_surface_boundary.c:430-493)
-        const double tau = lv.h[nt], down = exp(-tau / I.mus);
-        const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
-        const double r = P.R[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c];
-        for (int k = 0; k <= nt; ++k)
-            tot[(long long)k * Pp + col] += r * (FS * (down * exp(-(tau - lv.h[k]) / mu)));
-    }
-    for (int k = 0; k <= nt; ++k) {
-        const long long ix = (long long)k * Pp + col;
-        if constexpr (MIXED)
-            store_field_pair(reinterpret_cast<__hip_bfloat16 *>(F), Flo, ix, tot[ix]);
-        else
-            F[ix] = to_storage<FT>(tot[ix]);
-    }
-    if (hemi == 1) P.bc[it.off_bc + 3 * n + c * n + o] = tot[(long long)nt * Pp + col];
+    // one pass: the sweep is linear in its source, so single scattering and
+    // the flat-surface seed (the specularly reflected direct beam scattered
+    // once, rt_surface_boundary.c:210-428) share it; over the rough ocean the
+    // direct beam reflected into the upward nodes is added level by level
+    // (rt_surface_boundary.c:430-493)
+    const bool rough_seed = I.surf == 2 && hemi == 0;
+    const double tau = lv.h[nt], down = exp(-tau / I.mus);
+    const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
+    const double rsun = rough_seed ? P.R[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c] : 0.0;
+    sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o,
+                     [&](int k) { return src.primary(lv, k) + src.flat_value(lv, I.mus, k); },
+                     [&](int k, double v) {
+                         if (rough_seed) v += rsun * (FS * (down * exp(-(tau - lv.h[k]) / mu)));
+                         const long long ix = (long long)k * Pp + col;
+                         tot[ix] = v;
+                         if constexpr (MIXED)
+                             store_field_pair(reinterpret_cast<__hip_bfloat16 *>(F), Flo, ix, v);
+                         else
+                             F[ix] = to_storage<FT>(v);
+                         if (hemi == 1 && k == nt) P.bc[it.off_bc + 3 * n + c * n + o] = v;
+                     });
 }
 
 // ---------------------------------------------------------------------------
@@ -300,20 +297,41 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
     const int lr = t >> 2, lk = (t & 3) * 8;
     const int lev_load = lev0 + lr;
     const long long slab = (long long)Mp * Pp;
-    for (int k0 = 0; k0 < Pp; k0 += 32) {
-        for (int q = 0; q < nparts; ++q)
-            for (int hl = 0; hl < 2; ++hl) {
-                const unsigned short *src = A + (long long)(q * 2 + hl) * slab + (long long)(row0 + lr) * Pp + k0 + lk;
-                *reinterpret_cast<uint4 *>(&As[q][hl][lr][lk]) = *reinterpret_cast<const uint4 *>(src);
-            }
-        uint4 bv = {0u, 0u, 0u, 0u}, bl = {0u, 0u, 0u, 0u};
-        if (lev_load < L) {
-            bv = *reinterpret_cast<const uint4 *>(F + (long long)lev_load * Pp + k0 + lk);
-            if (split) bl = *reinterpret_cast<const uint4 *>(Flo + (long long)lev_load * Pp + k0 + lk);
+    // K slabs of 32 go global -> registers -> LDS; the next slab's loads are
+    // issued before the current slab's MFMAs so their latency overlaps them
+    uint4 ra0h, ra0l, ra1h, ra1l, rb, rbl;
+    const bool two = nparts > 1;
+    auto load_slab = [&](int k0) {
+        const long long ro = (long long)(row0 + lr) * Pp + k0 + lk;
+        ra0h = *reinterpret_cast<const uint4 *>(A + ro);
+        ra0l = *reinterpret_cast<const uint4 *>(A + slab + ro);
+        if (two) {
+            ra1h = *reinterpret_cast<const uint4 *>(A + 2 * slab + ro);
+            ra1l = *reinterpret_cast<const uint4 *>(A + 3 * slab + ro);
         }
-        *reinterpret_cast<uint4 *>(&Bs[0][lr][lk]) = bv;
-        if (split) *reinterpret_cast<uint4 *>(&Bs[1][lr][lk]) = bl;
-        __syncthreads();
+        rb = make_uint4(0u, 0u, 0u, 0u);
+        rbl = make_uint4(0u, 0u, 0u, 0u);
+        if (lev_load < L) {
+            rb = *reinterpret_cast<const uint4 *>(F + (long long)lev_load * Pp + k0 + lk);
+            if (split) rbl = *reinterpret_cast<const uint4 *>(Flo + (long long)lev_load * Pp + k0 + lk);
+        }
+    };
+    auto store_slab = [&]() {
+        *reinterpret_cast<uint4 *>(&As[0][0][lr][lk]) = ra0h;
+        *reinterpret_cast<uint4 *>(&As[0][1][lr][lk]) = ra0l;
+        if (two) {
+            *reinterpret_cast<uint4 *>(&As[1][0][lr][lk]) = ra1h;
+            *reinterpret_cast<uint4 *>(&As[1][1][lr][lk]) = ra1l;
+        }
+        *reinterpret_cast<uint4 *>(&Bs[0][lr][lk]) = rb;
+        if (split) *reinterpret_cast<uint4 *>(&Bs[1][lr][lk]) = rbl;
+    };
+    load_slab(0);
+    store_slab();
+    __syncthreads();
+    for (int k0 = 0; k0 < Pp; k0 += 32) {
+        const bool more = k0 + 32 < Pp;
+        if (more) load_slab(k0 + 32);
 #pragma unroll
         for (int ks = 0; ks < 32; ks += 16) {
             short4_ b4[2], b4l[2];
@@ -336,6 +354,10 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
                 }
         }
         __syncthreads();
+        if (more) {
+            store_slab();
+            __syncthreads();
+        }
     }
     const int kind0 = it.parts & 0xff;
     for (int b = 0; b < 2; ++b) {
