@@ -37,7 +37,7 @@ __device__ inline double seed_weight(const DevInst &I, const Pools &P, int m, in
 __device__ inline void add_seed(const DevInst &I, const DevItem &it, const Pools &P)
 {
     double *lam = P.lam + it.off_F;
-    for (int c = 0; c < 3; ++c) lam[2 * c * it.n + I.vidx] += seed_weight(I, P, it.m, c);
+    for (int c = 0; c < 3; ++c) lam[col_up(it.n, c, I.vidx)] += seed_weight(I, P, it.m, c);
 }
 
 // Zero the instance accumulators (one thread per entry of one instance).
@@ -66,7 +66,7 @@ __global__ void adj_bottom(const DevInst *__restrict__ inst, const DevItem *__re
     const double *F = reinterpret_cast<const double *>(P.F) + it.off_F + (long long)it.nt * it.P;
     for (int q = threadIdx.x; q < 3 * n; q += blockDim.x) {
         const int c = q / n, o = q % n;
-        P.bc[it.off_bc + 3 * n + q] = F[(2 * c + 1) * n + o];
+        P.bc[it.off_bc + 3 * n + q] = F[col_dn(n, c, o)];
     }
 }
 
@@ -100,88 +100,88 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
     const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (tix >= n_tiles) return;
     const ColTile ct = tiles[tix];
-    if (ct.item < 0 || !item_runs(st[ct.item], order)) return;
-    const DevItem &it = items[ct.item];
+    const int item = __builtin_amdgcn_readfirstlane(ct.item);
+    if (item < 0 || !item_runs(st[item], order)) return;
+    const DevItem &it = items[item];
     const DevInst &I = inst[it.inst];
     const int n = it.n, nt = it.nt, Pp = it.P, L = nt + 1;
     const int nparts = it.K / Pp, kind0 = it.parts & 0xff;
+    const long long off_F = it.off_F, off_bc = it.off_bc;
     const int col = ct.col0 + lane;
-    const bool live = col < 6 * n;
-    const int type = live ? col / n : 0, c = type >> 1, hemi = type & 1, o = live ? col % n : 0;
+    const bool live = col < ct.col_end && col < 6 * n;
+    // tiles never mix directions: the whole wavefront walks one way
+    const int hemi = __builtin_amdgcn_readfirstlane(ct.col0 >= 3 * n ? 1 : 0);
+    int c = 0, hh = 0, o = 0;
+    if (live) col_decode(col, n, c, hh, o);
     const LevelView lv(P, I);
-    const double *U = reinterpret_cast<const double *>(P.F) + it.off_F + col;
-    double *J0 = reinterpret_cast<double *>(P.J) + it.off_F + col;
-    const double *J1 = reinterpret_cast<const double *>(P.J2) + it.off_F + col;
-    const double *lam = P.lam + it.off_F + col;
-    double *gH = P.grad + I.off_grad, *gX = gH + L, *gY = gX + L, *gD = gY + L;
-    (void)gH;
+    const double *__restrict__ U = reinterpret_cast<const double *>(P.F) + off_F + col;
+    double *__restrict__ J0 = reinterpret_cast<double *>(P.J) + off_F + col;
+    const double *__restrict__ J1 = reinterpret_cast<const double *>(P.J2) + off_F + col;
+    const double *__restrict__ lam = P.lam + off_F + col;
+    double *gX = P.grad + I.off_grad + L, *gY = gX + L, *gD = gY + L;
     const double amu = P.ring[I.off_ring + o];
-    auto parts_at = [&](int k, double &p0, double &p1) {
-        p0 = live ? J0[(long long)k * Pp] : 0.0;
-        p1 = (live && nparts > 1) ? J1[(long long)k * Pp] : 0.0;
-    };
+    const bool two = nparts > 1;
     auto source = [&](int k, double p0, double p1) {
         const double s0 = (kind0 == 1) ? lv.xd[k] : lv.yd[k];
         return s0 * p0 + lv.yd[k] * p1;
     };
-    // d(Sbar . J) / d xdel and / d ydel at one level
-    auto level_terms = [&](double sbar, double p0, double p1, double &vx, double &vy) {
+    // d(Sbar . J) / d xdel and / d ydel at one level, summed over the wavefront
+    auto level_sums = [&](int k, double sbar, double p0, double p1) {
+        double vx, vy;
         if (kind0 == 1) { vx = sbar * p0; vy = sbar * p1; }
         else { vx = 0.0; vy = sbar * p0; }
+        vx = wave_sum(live ? vx : 0.0);
+        vy = wave_sum(live ? vy : 0.0);
+        if (lane == 0) {
+            if (vx != 0.0) atomicAdd(&gX[k], vx);
+            if (vy != 0.0) atomicAdd(&gY[k], vy);
+        }
     };
     const int k0 = hemi ? nt : 0;
     double rho = live ? lam[(long long)k0 * Pp] : 0.0;
-    double pk0, pk1;
-    parts_at(k0, pk0, pk1);
+    double pk0 = live ? J0[(long long)k0 * Pp] : 0.0, pk1 = (live && two) ? J1[(long long)k0 * Pp] : 0.0;
     double sk = source(k0, pk0, pk1);
     double pending = 0.0;
-    for (int t = 0; t < nt; ++t) {
-        const int k = hemi ? nt - t : t;
-        const int kp = hemi ? k - 1 : k + 1;
-        double pp0, pp1;
-        parts_at(kp, pp0, pp1);
-        const double sp = source(kp, pp0, pp1);
-        double T, al, be, dT, dal, dbe;
-        step_coefs(fabs(lv.h[k] - lv.h[kp]), amu, T, al, be, dT, dal, dbe);
-        const double sbar = pending + al * rho;
-        pending = be * rho;
-        const double up = live ? U[(long long)kp * Pp] : 0.0;
-        const double dd = rho * (dT * up + dal * sk + dbe * sp);
-        if (live) J0[(long long)k * Pp] = sbar;
-        double vx, vy;
-        level_terms(sbar, pk0, pk1, vx, vy);
-        double xu, xd, yu, yd, du, dn;
-        hemi_sums(vx, hemi, xu, xd);
-        hemi_sums(vy, hemi, yu, yd);
-        hemi_sums(dd, hemi, du, dn);
-        if (lane == 0) {
-            // upward lanes sit at level t (layer t), downward ones at nt - t (layer nt - t - 1)
-            if (xu != 0.0) atomicAdd(&gX[t], xu);
-            if (yu != 0.0) atomicAdd(&gY[t], yu);
-            if (du != 0.0) atomicAdd(&gD[t], du);
-            if (xd != 0.0) atomicAdd(&gX[nt - t], xd);
-            if (yd != 0.0) atomicAdd(&gY[nt - t], yd);
-            if (dn != 0.0) atomicAdd(&gD[nt - t - 1], dn);
+    constexpr int kB = 8;
+    for (int t0 = 0; t0 < nt; t0 += kB) {
+        // gather the block's upstream levels first (clamped: no branches)
+        double q0[kB], q1[kB], uv[kB], lv_[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int tc = t0 + u < nt ? t0 + u : nt - 1;
+            const int kp = hemi ? nt - tc - 1 : tc + 1;
+            const long long ix = (long long)kp * Pp;
+            q0[u] = live ? J0[ix] : 0.0;
+            q1[u] = (live && two) ? J1[ix] : 0.0;
+            uv[u] = live ? U[ix] : 0.0;
+            lv_[u] = live ? lam[ix] : 0.0;
         }
-        rho = (live ? lam[(long long)kp * Pp] : 0.0) + T * rho;
-        pk0 = pp0; pk1 = pp1; sk = sp;
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int t = t0 + u;
+            if (t >= nt) break;
+            const int k = hemi ? nt - t : t;
+            const int kp = hemi ? k - 1 : k + 1;
+            const double sp = source(kp, q0[u], q1[u]);
+            double T, al, be, dT, dal, dbe;
+            step_coefs(fabs(lv.h[k] - lv.h[kp]), amu, T, al, be, dT, dal, dbe);
+            const double sbar = pending + al * rho;
+            pending = be * rho;
+            const double dd = rho * (dT * uv[u] + dal * sk + dbe * sp);
+            if (live) J0[(long long)k * Pp] = sbar;
+            level_sums(k, sbar, pk0, pk1);
+            const double ds = wave_sum(live ? dd : 0.0);
+            if (lane == 0 && ds != 0.0) atomicAdd(&gD[hemi ? k - 1 : k], ds);
+            rho = lv_[u] + T * rho;
+            pk0 = q0[u]; pk1 = q1[u]; sk = sp;
+        }
     }
     // last level of the walk: nt for upward columns, 0 for downward ones
     const int kl = hemi ? 0 : nt;
     if (live) J0[(long long)kl * Pp] = pending;
-    double vx, vy;
-    level_terms(pending, pk0, pk1, vx, vy);
-    double xu, xd, yu, yd;
-    hemi_sums(vx, hemi, xu, xd);
-    hemi_sums(vy, hemi, yu, yd);
-    if (lane == 0) {
-        if (xu != 0.0) atomicAdd(&gX[nt], xu);
-        if (yu != 0.0) atomicAdd(&gY[nt], yu);
-        if (xd != 0.0) atomicAdd(&gX[0], xd);
-        if (yd != 0.0) atomicAdd(&gY[0], yd);
-    }
+    level_sums(kl, pending, pk0, pk1);
     // the upward columns start from the boundary condition: its adjoint is rho
-    if (live && hemi == 0 && I.surf != 0) P.bc[it.off_bc + c * n + o] = rho;
+    if (live && hemi == 0 && I.surf != 0) P.bc[off_bc + c * n + o] = rho;
 }
 
 // After M^T Sbar_o has been written to P.lam: lambda_{o-1} += e (TOA view
@@ -220,11 +220,11 @@ __global__ void surface_adj(const DevInst *__restrict__ inst, const DevItem *__r
                 const double b0 = bb[op], b1 = bb[n + op], b2 = bb[2 * n + op];
                 for (int c2 = 0; c2 < 3; ++c2) {
                     acc[c2] += pw * (b0 * R[c2] + b1 * R[3 + c2] + b2 * R[6 + c2]);
-                    gw += pw * (b0 * dR[c2] + b1 * dR[3 + c2] + b2 * dR[6 + c2]) * Ub[(2 * c2 + 1) * n + i];
+                    gw += pw * (b0 * dR[c2] + b1 * dR[3 + c2] + b2 * dR[6 + c2]) * Ub[col_dn(n, c2, i)];
                 }
             }
         }
-        for (int c2 = 0; c2 < 3; ++c2) lamb[(2 * c2 + 1) * n + i] += acc[c2];
+        for (int c2 = 0; c2 < 3; ++c2) lamb[col_dn(n, c2, i)] += acc[c2];
     }
     if (I.surf == 2) {
         gw = wave_sum(gw);
@@ -249,8 +249,9 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
     const DevInst &I = inst[it.inst];
     const int n = it.n, nt = it.nt, Pp = it.P, L = nt + 1, m = it.m;
     const int col = ct.col0 + lane;
-    const bool live = col < 6 * n;
-    const int type = live ? col / n : 0, c = type >> 1, hemi = type & 1, o = live ? col % n : 0;
+    const bool live = col < ct.col_end && col < 6 * n;
+    int c = 0, hemi = 0, o = 0;
+    if (live) col_decode(col, n, c, hemi, o);
     const LevelView lv(P, I);
     const ColumnSource src(I, it, P, lv, live ? col : 0);
     const double mus = I.mus, amu = P.ring[I.off_ring + o];

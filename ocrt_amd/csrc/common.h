@@ -123,10 +123,30 @@ struct GemmTile { int item, tm, tn, pad; };   // item < 0: idle slot
 constexpr int kGemmMix = 0;     // J = xdel A_p F + ydel A_r F
 constexpr int kGemmParts = 1;   // J = A_0 F, J2 = A_1 F (unscaled)
 
-// 64 consecutive state columns of one item: the unit of the per-column kernels
-// (one wavefront each).  item < 0: idle slot.
-struct ColTile { int item, col0; };
+// Up to 64 consecutive state columns [col0, col_end) of one item: the unit of
+// the per-column kernels (one wavefront each).  A tile never mixes upward and
+// downward columns, so all its lanes walk the levels the same way.  item < 0:
+// idle slot.
+struct ColTile { int item, col0, col_end, pad; };
 constexpr int kColTile = 64;
+
+// State columns of an item: [I_up Q_up U_up | I_dn Q_dn U_dn], n nodes each.
+__host__ __device__ inline int col_up(int n, int c, int o) { return c * n + o; }
+__host__ __device__ inline int col_dn(int n, int c, int o) { return (3 + c) * n + o; }
+__host__ __device__ inline void col_decode(int col, int n, int &c, int &hemi, int &o)
+{
+    const int t = col / n;
+    hemi = t >= 3 ? 1 : 0;
+    c = t - 3 * hemi;
+    o = col - t * n;
+}
+// Type of a column in the order [I_up I_dn Q_up Q_dn U_up U_dn] of the
+// operator's coefficient selectors.
+__host__ __device__ inline int col_sel_type(int col, int n)
+{
+    const int t = col / n;
+    return t < 3 ? 2 * t : 2 * (t - 3) + 1;
+}
 
 // One IPSS kappa evaluation: instance and viewing direction.
 struct IpssJob { int inst; int pad; double vza, raa; };

@@ -488,7 +488,11 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 for (int tm = 0; tm < it.tiles_m; ++tm)
                     for (int tn = 0; tn < it.tiles_n; ++tn) gt.push_back(GemmTile{item_local, tm, tn, 0});
                 gemm_of_item.push_back(std::move(gt));
-                for (int c0 = 0; c0 < Pp; c0 += kColTile) c.hcols.push_back(ColTile{item_local, c0});
+                // direction-pure column tiles: upward half, downward half, padding
+                const int bounds[4] = {0, 3 * n, 6 * n, Pp};
+                for (int h = 0; h < 3; ++h)
+                    for (int c0 = bounds[h]; c0 < bounds[h + 1]; c0 += kColTile)
+                        c.hcols.push_back(ColTile{item_local, c0, std::min(c0 + kColTile, bounds[h + 1]), 0});
                 c.hitem.push_back(it);
             }
             if (I.ipss) c.hjobs.push_back(IpssJob{ii - c.i0, 0, I.vza, I.raa});

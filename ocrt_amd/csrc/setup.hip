@@ -473,8 +473,8 @@ __global__ void pack_operator(const DevInst *__restrict__ inst, const DevItem *_
         double v = 0.0;
         if (r < six_n && cc < six_n) {
             const int ro = TRANS ? cc : r, ci = TRANS ? r : cc;   // (outgoing, incoming) state columns
-            const int to = ro / n, o = ro % n + 1;
-            const int ti = ci / n, i = ci % n + 1;
+            const int to = col_sel_type(ro, n), o = ro % n + 1;
+            const int ti = col_sel_type(ci, n), i = ci % n + 1;
             const double w = P.ring[I.off_ring + n + (i - 1)];
             if (w != 0.0) {
                 const int q = c_sel_q[to][ti];

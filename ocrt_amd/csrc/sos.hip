@@ -27,31 +27,44 @@ template <typename SrcFn, typename EmitFn>
 __device__ inline void sweep_column_f64(int hemi, int nt, double mu, double bc, const double *h,
                                         const double *T, int n, int o, SrcFn S, EmitFn emit)
 {
-    if (hemi == 0) {
-        double run = bc;
-        emit(nt, run);
-        double s_in = S(nt);
-        for (int k = nt - 1; k >= 0; --k) {
-            const double s_out = S(k);
-            const double dt = h[k + 1] - h[k];
-            const double tr = T[(long long)k * n + o];
-            const double a = (s_in - s_out) / dt;
-            const double b = s_out - a * h[k];
-            run = tr * run + ((1.0 - tr) * (b + a * mu) + a * (h[k] - h[k + 1] * tr)) * 0.5;
-            emit(k, run);
-            s_in = s_out;
+    // levels are taken in blocks of 8: the block's sources, depths and
+    // transmittances are gathered first (clamped indices, no branches), so
+    // their loads overlap, then the recurrence runs over them
+    constexpr int kB = 8;
+    const int step = hemi ? 1 : -1;
+    const int k_first = hemi ? 0 : nt;
+    double run = hemi ? 0.0 : bc;
+    emit(k_first, run);
+    double s_in = S(k_first);
+    for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * kB) {
+        double sv[kB], hk[kB], hu[kB], tr[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            int k = k0 + step * u;
+            k = k < 0 ? 0 : (k > nt ? nt : k);
+            const int ku = hemi ? (k > 0 ? k - 1 : 0) : (k < nt ? k + 1 : nt);   // upstream level
+            const int layer = hemi ? ku : k;
+            sv[u] = S(k);
+            hk[u] = h[k];
+            hu[u] = h[ku];
+            tr[u] = T[(long long)(layer < nt ? layer : nt - 1) * n + o];
         }
-    } else {
-        double run = 0.0;
-        emit(0, run);
-        double s_in = S(0);
-        for (int k = 1; k <= nt; ++k) {
-            const double s_out = S(k);
-            const double dt = h[k] - h[k - 1];
-            const double tr = T[(long long)(k - 1) * n + o];
-            const double a = (s_out - s_in) / dt;
-            const double b = s_out - a * h[k];
-            run = tr * run + ((1.0 - tr) * (b - a * mu) + a * (h[k] - h[k - 1] * tr)) * 0.5;
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int k = k0 + step * u;
+            if (k < 0 || k > nt) break;
+            const double s_out = sv[u];
+            if (hemi == 0) {
+                const double dt = hu[u] - hk[u];
+                const double a = (s_in - s_out) / dt;
+                const double b = s_out - a * hk[u];
+                run = tr[u] * run + ((1.0 - tr[u]) * (b + a * mu) + a * (hk[u] - hu[u] * tr[u])) * 0.5;
+            } else {
+                const double dt = hk[u] - hu[u];
+                const double a = (s_out - s_in) / dt;
+                const double b = s_out - a * hk[u];
+                run = tr[u] * run + ((1.0 - tr[u]) * (b - a * mu) + a * (hk[u] - hu[u] * tr[u])) * 0.5;
+            }
             emit(k, run);
             s_in = s_out;
         }
@@ -95,7 +108,7 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
         s.max_tot = 0ull;
         st[ct.item] = s;
     }
-    if (col >= Pp) return;
+    if (col >= ct.col_end) return;
     FT *F = reinterpret_cast<FT *>(P.F) + it.off_F;
     double *tot = P.tot + it.off_tot;
     if (MIXED) {
@@ -112,7 +125,8 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
         return;
     }
     const LevelView lv(P, I);
-    const int type = col / n, c = type >> 1, hemi = type & 1, o = col % n;
+    int c, hemi, o;
+    col_decode(col, n, c, hemi, o);
     const double *mu_ring = P.ring + I.off_ring;
     const double mu = mu_ring[o];
     const double *T = P.T + I.off_T;
@@ -390,6 +404,9 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
 // The order sum is fp64 over the whole field (it feeds the convergence test).
 // In the mixed form the two rows that become outputs, k = 0 (TOA) and k = nt
 // (BOA), also carry an fp32 Neumaier compensation term.
+__device__ inline int clamp_level(int k, int nt) { return k < 0 ? 0 : (k > nt ? nt : k); }
+__device__ inline int clamp_layer(int l, int nt) { return l < 0 ? 0 : (l > nt - 1 ? nt - 1 : l); }
+
 #ifndef OCRT_SWEEP_UNROLL
 #define OCRT_SWEEP_UNROLL 8
 #endif
@@ -404,24 +421,32 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
     const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (tix >= n_tiles) return;
     const ColTile ct = tiles[tix];
-    if (ct.item < 0 || !item_runs(st[ct.item], order)) return;
+    // the tile is wavefront-uniform: keep its item in scalar registers
+    const int item = __builtin_amdgcn_readfirstlane(ct.item);
+    if (item < 0 || !item_runs(st[item], order)) return;
     const bool accumulate = order == 0;   // recomputation only rebuilds the field
-    const int item = ct.item;
     const DevItem &it = items[item];
     const DevInst &I = inst[it.inst];
+    // every descriptor field the level loop needs is read once here: the loop's
+    // stores could otherwise alias the descriptors and force reloads
     const int n = it.n, nt = it.nt, Pp = it.P;
+    const long long off_F = it.off_F, off_bc = it.off_bc, off_cmp = it.off_cmp;
     const int col = ct.col0 + lane;
     double mcur = 0.0, mtot = 0.0;
-    if (col < 6 * n) {
+    if (col < ct.col_end && col < 6 * n) {
         const LevelView lv(P, I);
-        const int type = col / n, c = type >> 1, hemi = type & 1, o = col % n;
-        const JT *J = reinterpret_cast<const JT *>(P.J) + it.off_F + col;
-        FT *F = reinterpret_cast<FT *>(P.F) + it.off_F + col;
-        __hip_bfloat16 *Flo = (MIXED && it.fsplit)
-                                  ? reinterpret_cast<__hip_bfloat16 *>(P.Flo) + it.off_Flo + col : nullptr;
-        double *tot = P.tot + it.off_tot + col;
+        int c, hemi, o;
+        col_decode(col, n, c, hemi, o);
+        const JT *__restrict__ J = reinterpret_cast<const JT *>(P.J) + off_F + col;
+        FT *__restrict__ F = reinterpret_cast<FT *>(P.F) + off_F + col;
+        __hip_bfloat16 *__restrict__ Flo =
+            (MIXED && it.fsplit) ? reinterpret_cast<__hip_bfloat16 *>(P.Flo) + it.off_Flo + col : nullptr;
+        double *__restrict__ tot = P.tot + it.off_tot + col;
+        double *__restrict__ bottom = P.bc + off_bc + 3 * n + c * n + o;
+        float *__restrict__ cmp_top = MIXED ? P.comp + off_cmp + col : nullptr;
+        float *__restrict__ cmp_bot = MIXED ? P.comp + off_cmp + Pp + col : nullptr;
         const double mu = P.ring[I.off_ring + o];
-        const double bc = (I.surf != 0 && hemi == 0) ? P.bc[it.off_bc + c * n + o] : 0.0;
+        const double bc = (I.surf != 0 && hemi == 0) ? P.bc[off_bc + c * n + o] : 0.0;
         // level order of the scan and the layer crossed when stepping to level k
         const int k_first = hemi ? 0 : nt;
         const int step = hemi ? 1 : -1;
@@ -433,13 +458,13 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
             else
                 F[ix] = to_storage<FT>(v);
             mcur = fmax(mcur, fabs(v));
-            if (hemi == 1 && k == nt) P.bc[it.off_bc + 3 * n + c * n + o] = v;
+            if (hemi == 1 && k == nt) *bottom = v;
             if (!accumulate) return;
             const double tsum = t_old + v;
             tot[ix] = tsum;
             mtot = fmax(mtot, fabs(tsum));
             if (MIXED && (k == 0 || k == nt)) {
-                float *cp = P.comp + it.off_cmp + (k == 0 ? 0 : Pp) + col;
+                float *cp = k == 0 ? cmp_top : cmp_bot;
                 const double e = (fabs(t_old) >= fabs(v)) ? ((t_old - tsum) + v) : ((v - tsum) + t_old);
                 *cp = (float)((double)*cp + e);
             }
@@ -454,12 +479,12 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                 double sj[kU], tr[kU], to[kU];
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
-                    const int k = k0 + step * u;
-                    if (k >= 0 && k <= nt) {
-                        sj[u] = (double)J[(long long)k * Pp];
-                        tr[u] = T[(long long)layer_of(k) * n + o];
-                        to[u] = accumulate ? tot[(long long)k * Pp] : 0.0;
-                    }
+                    // clamped indices keep the block's loads unconditional, so
+                    // they are all in flight before the first one is used
+                    const int k = clamp_level(k0 + step * u, nt);
+                    sj[u] = (double)J[(long long)k * Pp];
+                    tr[u] = T[(long long)clamp_layer(layer_of(k0 + step * u), nt) * n + o];
+                    to[u] = tot[(long long)k * Pp];
                 }
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
@@ -489,13 +514,14 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                 double to[kU];
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
-                    const int k = k0 + step * u;
-                    if (k >= 0 && k <= nt) {
-                        sj[u] = (float)J[(long long)k * Pp];
-                        om[u] = OM[(long long)layer_of(k) * n + o];
-                        idt[u] = IDT[layer_of(k)];
-                        to[u] = accumulate ? tot[(long long)k * Pp] : 0.0;
-                    }
+                    // clamped indices keep the block's loads unconditional, so
+                    // they are all in flight before the first one is used
+                    const int k = clamp_level(k0 + step * u, nt);
+                    const int l = clamp_layer(layer_of(k0 + step * u), nt);
+                    sj[u] = (float)J[(long long)k * Pp];
+                    om[u] = OM[(long long)l * n + o];
+                    idt[u] = IDT[l];
+                    to[u] = tot[(long long)k * Pp];
                 }
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
@@ -542,18 +568,18 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
     __shared__ double red[8];
     double hs = 0.0;
     for (int o = threadIdx.x; o < n; o += blockDim.x)
-        hs += mu[o] * w[o] * total_at(P, it, mixed, nt, n + o);
+        hs += mu[o] * w[o] * total_at(P, it, mixed, nt, col_dn(n, 0, o));
     // ordered sum over nodes for the hemispheric flux
     red[0] = 0.0;
     __syncthreads();
     if (threadIdx.x == 0) {
         double F = 0.0;
-        for (int o = 0; o < n; ++o) F += mu[o] * w[o] * total_at(P, it, mixed, nt, n + o);
+        for (int o = 0; o < n; ++o) F += mu[o] * w[o] * total_at(P, it, mixed, nt, col_dn(n, 0, o));
         const ItemState s = st[item];
         ItemOut r;
         const double tau = lv.h[nt];
         for (int c = 0; c < 3; ++c) {
-            double val = total_at(P, it, mixed, 0, 2 * c * n + v);
+            double val = total_at(P, it, mixed, 0, col_up(n, c, v));
             if (I.surf == 2) {
                 const double down = exp(-tau / I.mus);
                 const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
@@ -562,7 +588,7 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
             }
             r.mode[c] = val;
         }
-        r.boa_view = total_at(P, it, mixed, nt, n + v);
+        r.boa_view = total_at(P, it, mixed, nt, col_dn(n, 0, v));
         r.boa_hemi = F;
         r.orders = s.orders;
         r.converged = s.converged;
@@ -578,7 +604,7 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
             const int c = q / n, o = q % n;
             double val;
             if (c < 3) {
-                val = total_at(P, it, mixed, 0, 2 * c * n + o);
+                val = total_at(P, it, mixed, 0, col_up(n, c, o));
                 if (I.surf == 2) {
                     const double tau = lv.h[nt], down = exp(-tau / I.mus);
                     const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
@@ -586,7 +612,7 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
                            (FS * (down * exp(-(tau - lv.h[0]) / mu[o])));
                 }
             } else {
-                val = total_at(P, it, mixed, nt, n + o);
+                val = total_at(P, it, mixed, nt, col_dn(n, 0, o));
             }
             dst[q] = val;
         }

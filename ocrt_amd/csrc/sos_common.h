@@ -55,8 +55,8 @@ struct ColumnSource {
     {
         const int n = it.n, dirs = 2 * n + 1, m = it.m, lmax = it.lmax;
         nt = it.nt;
-        const int type = col / n, c = type >> 1, o = col % n;
-        hemi = type & 1;
+        int c, o;
+        col_decode(col, n, c, hemi, o);
         const int d = hemi ? n - (o + 1) : n + (o + 1);
         const double *plm = P.basis + it.off_basis;
         const double *rrl = plm + (long long)(lmax + 1) * dirs;
