@@ -155,7 +155,10 @@ void ocrt_plan_destroy(ocrt_plan *plan);
 /* Plan options for ocrt_plan_create_ex. */
 enum {
     OCRT_PLAN_ONE_PER_LAUNCH = 1,  /* instances one after another (the plain form) */
-    OCRT_PLAN_ADJOINT = 2          /* keep order checkpoints for ocrt_plan_adjoint (FP64, IPSS off) */
+    OCRT_PLAN_ADJOINT = 2,         /* keep order checkpoints for ocrt_plan_adjoint (FP64, IPSS off) */
+    OCRT_PLAN_CKPT_SPARSE = 4      /* adjoint: checkpoint every 4th order and recompute the rest, whatever
+                                      the memory allows (default: every order kept when the batch fits one
+                                      chunk, else every 2nd, else every 4th) */
 };
 
 /* ocrt_plan_create with option flags.  No counterpart in the reference; the
@@ -164,9 +167,10 @@ int ocrt_plan_create_ex(const ocrt_instance_t *inst, int n_inst, int precision, 
                         ocrt_plan **plan_out);
 
 /* Reverse-mode gradient of sum_i grad_out[i] . (rho_I, rho_Q, rho_U)_i for an
- * OCRT_PLAN_ADJOINT plan, by the adjoint SOS recursion (orders checkpointed
- * every 4 and recomputed).  Runs the forward pass itself unless the last
- * ocrt_plan_execute of a single-chunk plan left its checkpoints.
+ * OCRT_PLAN_ADJOINT plan, by the adjoint SOS recursion (order fields kept
+ * every 1, 2 or 4 orders; the orders between checkpoints are recomputed).
+ * Runs the forward pass itself unless the last ocrt_plan_execute of a
+ * single-chunk plan left its checkpoints.
  *   grad_out     host [n_inst][3]
  *   grad_levels  optional host, per instance in order, 3 x (n_layers + 1):
  *                d/dh[k] (level optical depth from the top), d/dxdel[k],

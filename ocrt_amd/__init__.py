@@ -19,6 +19,7 @@ _lib = None
 PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
 PLAN_ONE_PER_LAUNCH = 1
 PLAN_ADJOINT = 2
+PLAN_CKPT_SPARSE = 4
 
 
 class OcrtError(RuntimeError):
@@ -79,14 +80,15 @@ class Plan:
     integral for orders >= 2).  one_per_launch=True runs instances one after
     another (the plain one-instance-per-launch form)."""
 
-    def __init__(self, descs, precision="mixed", one_per_launch=False, adjoint=False):
+    def __init__(self, descs, precision="mixed", one_per_launch=False, adjoint=False, ckpt_sparse=False):
         _require_gpu()
         self.lib = library()
         self.n = len(descs)
         self._arr = instance_array(descs)
         self._handle = C.c_void_p()
         self.n_layers = [d.n_layers for d in descs]
-        flags = (PLAN_ONE_PER_LAUNCH if one_per_launch else 0) | (PLAN_ADJOINT if adjoint else 0)
+        flags = ((PLAN_ONE_PER_LAUNCH if one_per_launch else 0) | (PLAN_ADJOINT if adjoint else 0) |
+                 (PLAN_CKPT_SPARSE if ckpt_sparse else 0))
         rc = self.lib.ocrt_plan_create_ex(self._arr, self.n, PRECISIONS[precision], flags, C.byref(self._handle))
         if rc != 0:
             raise OcrtError(f"ocrt_plan_create failed with code {rc}")

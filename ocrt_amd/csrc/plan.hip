@@ -211,13 +211,15 @@ struct ocrt_plan {
     std::vector<hipEvent_t> ev;       // pairs (start, stop): [gemm..., sweep...]
     std::vector<int> ev_kind;         // 0 gemm, 1 sweep, per pair
     size_t ev_used = 0;
-    // adjoint plans: transposed operator, order checkpoints (u_1, u_5, u_9, ...),
-    // the three recomputed fields of a segment, per-instance accumulators
+    // adjoint plans: transposed operator, order checkpoints (u_1, u_{1+e},
+    // u_{1+2e}, ... for interval e), the e - 1 recomputed fields of a segment,
+    // per-instance accumulators
     int adjoint = 0;
     int fwd_ready = 0;                // the last execute left this plan's checkpoints intact
+    int ckpt_every = 1;
     void *At = nullptr;
     std::vector<void *> ckpt;
-    void *seg[3] = {nullptr, nullptr, nullptr};
+    std::vector<void *> seg;
     double *grad = nullptr, *gout = nullptr;
     long long grad_n = 0;
     // LUT output (ocrt_plan_set_lut): per-node fields of every item and the grid
@@ -227,8 +229,6 @@ struct ocrt_plan {
     double *kappa_lut = nullptr, *taudir_lut = nullptr;
 };
 
-// Orders between two checkpoints of the adjoint's forward pass.
-constexpr int kCkptEvery = 4;
 
 static hipEvent_t *ev_pair(ocrt_plan *p, int kind)
 {
@@ -311,7 +311,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
 {
     if (!in || n_inst < 1 || !out || (precision != OCRT_PREC_FP64 && precision != OCRT_PREC_MIXED))
         return OCRT_ERR_ARGS;
-    if (flags & ~(OCRT_PLAN_ONE_PER_LAUNCH | OCRT_PLAN_ADJOINT)) return OCRT_ERR_ARGS;
+    if (flags & ~(OCRT_PLAN_ONE_PER_LAUNCH | OCRT_PLAN_ADJOINT | OCRT_PLAN_CKPT_SPARSE)) return OCRT_ERR_ARGS;
     const bool adjoint = (flags & OCRT_PLAN_ADJOINT) != 0;
     // the adjoint runs the fp64 form and needs the plane-parallel solve (no IPSS)
     if (adjoint && precision != OCRT_PREC_FP64) return OCRT_ERR_ARGS;
@@ -385,6 +385,13 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     const size_t fe = mixed ? 2 : 8, je = mixed ? 4 : 8, ae = mixed ? 2 : 8;
     int item_global = 0;
     p->item_of_inst.resize(n_inst);
+    // Adjoint plans keep the field of every ckpt-th order and recompute the
+    // orders between.  The smallest interval of 1, 2, 4 that keeps the whole
+    // batch in one chunk is taken (no recomputation when the fields of all
+    // orders fit), unless the caller asks for the sparse form.
+    auto partition = [&](int every) {
+    p->chunks.clear();
+    item_global = 0;
     int i = 0;
     while (i < n_inst) {
         Chunk c;
@@ -404,7 +411,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                      (mixed ? 8.0 * P : 0.0) + ((mixed && m < kSplitModes) ? 2.0 * L * P : 0.0) + 6.0 * n * 8;
                 if (adjoint)   // transposed operator, checkpoints, segment fields, lambda, second parts
                     b += (double)Mp * K * ae +
-                         (double)L * P * 8 * ((I.max_orders - 1) / kCkptEvery + 1 + 3 + 2);
+                         (double)L * P * 8 * ((I.max_orders - 1) / every + 1 + (every - 1) + 2);
             }
             if (adjoint) b += (double)h.R_n * 8;
             const int items_after = (item_global - c.it0) + I.mmax + 1;
@@ -418,6 +425,11 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         c.it1 = item_global;
         p->chunks.push_back(c);
     }
+    };
+    p->ckpt_every = 1;
+    if (adjoint && (flags & OCRT_PLAN_CKPT_SPARSE)) p->ckpt_every = 4;
+    partition(p->ckpt_every);
+    while (adjoint && p->chunks.size() > 1 && p->ckpt_every < 4) partition(p->ckpt_every *= 2);
     p->n_items = item_global;
     // ---- per-chunk layout
     for (Chunk &c : p->chunks) {
@@ -550,7 +562,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     if (adjoint) {
         int maxit = 1;
         for (const HostInst &h : p->inst) maxit = std::max(maxit, h.d.max_orders);
-        p->ckpt.assign((maxit - 1) / kCkptEvery + 1, nullptr);
+        p->ckpt.assign((maxit - 1) / p->ckpt_every + 1, nullptr);
+        p->seg.assign(p->ckpt_every - 1, nullptr);
         for (void *&q : p->ckpt) rc |= dmalloc(p, &q, mx_F * 8);
         for (void *&q : p->seg) rc |= dmalloc(p, &q, mx_F * 8);
         rc |= dmalloc(p, &p->At, mx_A * ae);
@@ -670,10 +683,11 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
         hipLaunchKernelGGL((pack_operator<double, true>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem, Pt, nit,
                            pack_blocks);
     }
-    // adjoint plans keep u_1, u_5, u_9, ... of every item
+    // adjoint plans keep u_1, u_{1+e}, u_{1+2e}, ... of every item
+    const int every = p->ckpt_every;
     auto checkpoint = [&](int order) -> int {
-        if (!p->adjoint || (order - 1) % kCkptEvery != 0) return 0;
-        return hipMemcpyAsync(p->ckpt[(order - 1) / kCkptEvery], P.F, (size_t)c.F * 8, hipMemcpyDeviceToDevice, s) ==
+        if (!p->adjoint || (order - 1) % every != 0) return 0;
+        return hipMemcpyAsync(p->ckpt[(order - 1) / every], P.F, (size_t)c.F * 8, hipMemcpyDeviceToDevice, s) ==
                        hipSuccess ? 0 : OCRT_ERR_DEVICE;
     };
     // per-column kernels: one 64-column tile per wavefront, four per workgroup
@@ -728,10 +742,11 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : OCRT_ERR_DEVICE;
 }
 
-// Backward pass of one chunk after its forward pass (fp64).  Segment s holds
-// u_{4s+1} (the checkpoint) and the recomputed u_{4s+2..4s+4}; the backward
-// step of order o needs u_o and u_{o-1}, so segments are rebuilt from the top
-// down and order 4s+5 (whose u_o is the next checkpoint) is taken with segment s.
+// Backward pass of one chunk after its forward pass (fp64).  With checkpoint
+// interval e, segment s holds u_{es+1} (the checkpoint) and the recomputed
+// u_{es+2..es+e}; the backward step of order o needs u_o and u_{o-1}, so
+// segments are rebuilt from the top down and order e(s+1)+1 (whose u_o is the
+// next checkpoint) is taken with segment s.  With e = 1 nothing is recomputed.
 static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
 {
     const Pools P0 = p->pools;
@@ -746,10 +761,11 @@ static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
                        P0.lam, c.F, 0.0);
     hipLaunchKernelGGL(adj_seed, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P0);
     auto slot = [&](int seg, int j) -> void * { return j == 0 ? p->ckpt[seg] : p->seg[j - 1]; };
-    for (int sg = (maxit - 1) / kCkptEvery; sg >= 0; --sg) {
-        const int base = kCkptEvery * sg + 1;   // order of the checkpoint
-        // rebuild u_{base+1..base+3}
-        for (int j = 1; j < kCkptEvery && base + j <= maxit; ++j) {
+    const int every = p->ckpt_every;
+    for (int sg = (maxit - 1) / every; sg >= 0; --sg) {
+        const int base = every * sg + 1;   // order of the checkpoint
+        // rebuild u_{base+1..base+every-1}
+        for (int j = 1; j < every && base + j <= maxit; ++j) {
             const int o = base + j;
             Pools Pin = P0;
             Pin.F = slot(sg, j - 1);
@@ -763,8 +779,8 @@ static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
                                c.dcols, ncols, o);
         }
         // backward steps
-        for (int o = std::min(base + kCkptEvery, maxit); o >= std::max(base + 1, 2); --o) {
-            void *u_o = (o == base + kCkptEvery) ? p->ckpt[sg + 1] : slot(sg, o - base);
+        for (int o = std::min(base + every, maxit); o >= std::max(base + 1, 2); --o) {
+            void *u_o = (o == base + every) ? p->ckpt[sg + 1] : slot(sg, o - base);
             void *u_prev = slot(sg, o - 1 - base);
             Pools Pg = P0;
             Pg.F = u_prev;

@@ -79,6 +79,24 @@ def test_adjoint_matches_oracle_reverse_mode(ocrt, oracle):
             assert abs(adj["grad_wind"][i] - gw) <= 1e-6 * abs(gw) + 1e-14, (i, adj["grad_wind"][i], gw)
 
 
+def test_sparse_checkpoints_equal_every_order_kept(ocrt):
+    """Checkpointing every 4th order and recomputing the rest gives the
+    gradients of the form that keeps every order (same kernels, same inputs)."""
+    cases, gk = layered_cases(seed=7)
+    descs = [make_instance(layers=lay, **kw) for lay, kw in cases]
+    g = np.array([[0.7, -0.4, 0.9], [1.0, 0.2, 0.0], [0.0, 1.0, 0.3], [0.3, 0.2, -1.0], [-0.5, 0.8, 0.25]])
+    print(f"[gpu] adjoint, sparse vs dense checkpoints, {len(descs)} instances", flush=True)
+    out = []
+    for sparse in (False, True):
+        plan = ocrt.Plan(descs, precision="fp64", adjoint=True, ckpt_sparse=sparse)
+        out.append(plan.adjoint(g))
+        plan.close()
+    for key in ("grad_h", "grad_x", "grad_y"):
+        for i in range(len(descs)):
+            assert close(out[1][key][i], out[0][key][i], 1e-12), (key, i)
+    assert np.allclose(out[1]["grad_wind"], out[0]["grad_wind"], rtol=1e-12, atol=0.0)
+
+
 def test_adjoint_plan_forward_equals_plain_fp64(ocrt):
     cases, gk = layered_cases()
     descs = [make_instance(layers=lay, **kw) for lay, kw in cases]
