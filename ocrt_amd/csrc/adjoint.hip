@@ -156,6 +156,10 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
             uv[u] = live ? U[ix] : 0.0;
             lv_[u] = live ? lam[ix] : 0.0;
         }
+        // per step: d/d xdel, d/d ydel at level k and d/d(layer step)
+        double red[3 * kB];
+#pragma unroll
+        for (int e = 0; e < 3 * kB; ++e) red[e] = 0.0;
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
             const int t = t0 + u;
@@ -167,13 +171,22 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
             step_coefs(fabs(lv.h[k] - lv.h[kp]), amu, T, al, be, dT, dal, dbe);
             const double sbar = pending + al * rho;
             pending = be * rho;
-            const double dd = rho * (dT * uv[u] + dal * sk + dbe * sp);
-            if (live) J0[(long long)k * Pp] = sbar;
-            level_sums(k, sbar, pk0, pk1);
-            const double ds = wave_sum(live ? dd : 0.0);
-            if (lane == 0 && ds != 0.0) atomicAdd(&gD[hemi ? k - 1 : k], ds);
+            if (live) {
+                J0[(long long)k * Pp] = sbar;
+                red[3 * u + 0] = (kind0 == 1) ? sbar * pk0 : 0.0;
+                red[3 * u + 1] = (kind0 == 1) ? sbar * pk1 : sbar * pk0;
+                red[3 * u + 2] = rho * (dT * uv[u] + dal * sk + dbe * sp);
+            }
             rho = lv_[u] + T * rho;
             pk0 = q0[u]; pk1 = q1[u]; sk = sp;
+        }
+        wave_sum_8x3(red, lane);
+        const int u = lane >> 3, t = t0 + u;
+        if ((lane & 7) == 0 && t < nt) {
+            const int k = hemi ? nt - t : t;
+            if (red[0] != 0.0) atomicAdd(&gX[k], red[0]);
+            if (red[1] != 0.0) atomicAdd(&gY[k], red[1]);
+            if (red[2] != 0.0) atomicAdd(&gD[hemi ? k - 1 : k], red[2]);
         }
     }
     // last level of the walk: nt for upward columns, 0 for downward ones

@@ -683,22 +683,29 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
         hipLaunchKernelGGL((pack_operator<double, true>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem, Pt, nit,
                            pack_blocks);
     }
-    // adjoint plans keep u_1, u_{1+e}, u_{1+2e}, ... of every item
+    // adjoint plans keep u_1, u_{1+e}, u_{1+2e}, ... of every item.  With
+    // e = 1 the sweeps write straight into the checkpoint of their order and
+    // the next order reads it there (no copies).
     const int every = p->ckpt_every;
+    const bool direct = p->adjoint && every == 1;
+    auto field_of = [&](int order) -> void * { return direct ? p->ckpt[order - 1] : P.F; };
     auto checkpoint = [&](int order) -> int {
-        if (!p->adjoint || (order - 1) % every != 0) return 0;
+        if (!p->adjoint || direct || (order - 1) % every != 0) return 0;
         return hipMemcpyAsync(p->ckpt[(order - 1) / every], P.F, (size_t)c.F * 8, hipMemcpyDeviceToDevice, s) ==
                        hipSuccess ? 0 : OCRT_ERR_DEVICE;
     };
     // per-column kernels: one 64-column tile per wavefront, four per workgroup
     const int ncols = (int)c.hcols.size();
     const dim3 cols((unsigned)((ncols + 3) / 4));
-    if (mixed)
+    if (mixed) {
         hipLaunchKernelGGL((order_one<__hip_bfloat16, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st,
                            c.dcols, ncols);
-    else
-        hipLaunchKernelGGL((order_one<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st, c.dcols,
+    } else {
+        Pools P1 = P;
+        P1.F = field_of(1);
+        hipLaunchKernelGGL((order_one<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P1, st, c.dcols,
                            ncols);
+    }
     if (checkpoint(1)) return OCRT_ERR_DEVICE;
     const unsigned ntiles = (unsigned)c.htiles.size();
     for (int it = 2; it <= c.maxit; ++it) {
@@ -715,11 +722,15 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             if (es) (void)hipEventRecord(es[1], s);
         } else {
             hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
+            Pools Pg = P, Ps = P;
+            Pg.F = field_of(it - 1);
+            Ps.F = field_of(it);
             if (eg) (void)hipEventRecord(eg[0], s);
-            hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st, 0);
+            hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pg, c.dtiles, st,
+                               0);
             if (eg) (void)hipEventRecord(eg[1], s);
             if (es) (void)hipEventRecord(es[0], s);
-            hipLaunchKernelGGL((sweep<double, double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st,
+            hipLaunchKernelGGL((sweep<double, double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st,
                                c.dcols, ncols, 0);
             if (es) (void)hipEventRecord(es[1], s);
         }

@@ -39,6 +39,35 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
+// One halving exchange of a transposed wavefront reduction: lanes with
+// (lane & off) == 0 keep the first H values, their partners the last H, and
+// each adds what the partner sends for the half it keeps.
+template <int H>
+__device__ inline void halve_exchange(double *v, int off, bool upper)
+{
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const double give = upper ? v[i] : v[H + i];
+        const double keep = upper ? v[H + i] : v[i];
+        v[i] = keep + __shfl_xor(give, off);
+    }
+}
+
+// Sums 24 per-lane values over the wavefront in 30 exchanges (instead of
+// 24 x 6 for one reduction per value).  With the values ordered [8][3], every
+// lane of the group of eight lanes l = 8 r .. 8 r + 7 ends with the totals of
+// row r in v[0..2].
+__device__ inline void wave_sum_8x3(double (&v)[24], int lane)
+{
+    halve_exchange<12>(v, 32, (lane & 32) != 0);
+    halve_exchange<6>(v, 16, (lane & 16) != 0);
+    halve_exchange<3>(v, 8, (lane & 8) != 0);
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) v[i] += __shfl_xor(v[i], off);
+}
+
 // Order-1 source of one state column (rt_primary_source.c:61-70, 164-171) and,
 // over the flat Fresnel surface, the source of the specularly reflected direct
 // beam scattered once (rt_surface_boundary.c:210-428):
