@@ -5,8 +5,9 @@
 // linear in the fields, so the adjoint runs the order recursion backward:
 //   lambda_N = e,  lambda_{o-1} = e + M^T Sbar_o + B^T R^T bbar_o,
 // where e carries the reconstruction weights of the TOA view entries and
-// (Sbar_o, bbar_o) is the adjoint sweep of lambda_o.  Orders are
-// checkpointed every 4 and recomputed segment by segment (plan.hip).
+// (Sbar_o, bbar_o) is the adjoint sweep of lambda_o.  The forward fields are
+// kept for every order, or checkpointed every 2 or 4 orders and recomputed
+// segment by segment (plan.hip).
 // Gradients accumulate per instance with respect to the level optical depths
 // (explicit dependence gH, layer steps gD), the scattering fractions gX, gY
 // and the wind speed gW.  The reference has no adjoint; the finite-difference
@@ -79,20 +80,13 @@ __global__ void adj_surface_bc(const DevInst *__restrict__ inst, const DevItem *
     surface_bc_item(I, it, P);
 }
 
-// Sums of v over the upward-direction lanes and over the downward ones.
-__device__ inline void hemi_sums(double v, int hemi, double &su, double &sd)
-{
-    su = wave_sum(hemi == 0 ? v : 0.0);
-    sd = wave_sum(hemi == 1 ? v : 0.0);
-}
-
 // Adjoint sweep of order `order` >= 2.  P.F holds u_o, P.J / P.J2 the
 // unscaled operator parts A_0 u_{o-1} / A_1 u_{o-1}, P.lam lambda_o.  Writes
 // Sbar_o over P.J, bbar_o over the boundary slots of P.bc, and accumulates
 // gX, gY (Sbar . part) per level and gD per layer.  One thread per column,
-// one 64-column tile per wavefront, walking each column against its forward
-// direction; at every step the upward lanes share one level and the downward
-// lanes another, so the per-level sums are two wavefront reductions.
+// one 64-column tile (of one direction) per wavefront, walking each column
+// against its forward direction; the per-level sums of eight steps are
+// reduced over the wavefront together.
 __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ inst, const DevItem *__restrict__ items,
                                                  Pools P, const ItemState *__restrict__ st,
                                                  const ColTile *__restrict__ tiles, int n_tiles, int order)
@@ -180,7 +174,7 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
             rho = lv_[u] + T * rho;
             pk0 = q0[u]; pk1 = q1[u]; sk = sp;
         }
-        wave_sum_8x3(red, lane);
+        wave_sum_8xC<3>(red, lane);
         const int u = lane >> 3, t = t0 + u;
         if ((lane & 7) == 0 && t < nt) {
             const int k = hemi ? nt - t : t;
@@ -257,19 +251,22 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
     const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (tix >= n_tiles) return;
     const ColTile ct = tiles[tix];
-    if (ct.item < 0) return;
-    const DevItem &it = items[ct.item];
+    const int item = __builtin_amdgcn_readfirstlane(ct.item);
+    if (item < 0) return;
+    const DevItem &it = items[item];
     const DevInst &I = inst[it.inst];
     const int n = it.n, nt = it.nt, Pp = it.P, L = nt + 1, m = it.m;
     const int col = ct.col0 + lane;
     const bool live = col < ct.col_end && col < 6 * n;
-    int c = 0, hemi = 0, o = 0;
-    if (live) col_decode(col, n, c, hemi, o);
+    // tiles never mix directions: the whole wavefront walks one way
+    const int hemi = __builtin_amdgcn_readfirstlane(ct.col0 >= 3 * n ? 1 : 0);
+    int c = 0, hh = 0, o = 0;
+    if (live) col_decode(col, n, c, hh, o);
     const LevelView lv(P, I);
     const ColumnSource src(I, it, P, lv, live ? col : 0);
     const double mus = I.mus, amu = P.ring[I.off_ring + o];
-    const double *U = reinterpret_cast<const double *>(P.F) + it.off_F + col;
-    const double *lam = P.lam + it.off_F + col;
+    const double *__restrict__ U = reinterpret_cast<const double *>(P.F) + it.off_F + col;
+    const double *__restrict__ lam = P.lam + it.off_F + col;
     double *gH = P.grad + I.off_grad, *gX = gH + L, *gY = gX + L, *gD = gY + L, *gW = gD + nt;
     // rough direct-beam seed of the upward columns
     const bool rough = live && I.surf == 2 && hemi == 0;
@@ -280,10 +277,10 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
     const double e_here = (live && hemi == 0 && o == I.vidx) ? seed_weight(I, P, m, c) : 0.0;
     auto seed_f = [&](int k) { return FS * (down * exp(-(tau - lv.h[k]) / amu)); };
     auto S = [&](int k) { return live ? src.primary(lv, k) + src.flat_value(lv, mus, k) : 0.0; };
-    auto R = [&](int k) { return live ? U[(long long)k * Pp] - (rough ? rsun * seed_f(k) : 0.0) : 0.0; };
     double g_nt = 0.0, g_w = 0.0;   // per-lane d/dh[nt] and d/dsigma^2 terms
-    // explicit level terms of the source and the seed at level k for adjoint sbar
-    auto level_terms = [&](int k, double sbar, double &vx, double &vy, double &vh) {
+    // explicit level terms of the source and the seed at level k for adjoint
+    // sbar; lam_k is lambda_1 at level k
+    auto level_terms = [&](int k, double sbar, double lam_k, double &vx, double &vy, double &vh) {
         vx = vy = vh = 0.0;
         if (!live) return;
         const double ch = lv.ch[k];
@@ -299,7 +296,7 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
             g_nt += -2.0 * fv / mus;
         }
         if (rough) {
-            const double gd = lam[(long long)k * Pp] - (k == 0 ? e_here : 0.0);
+            const double gd = lam_k - (k == 0 ? e_here : 0.0);
             const double f = seed_f(k), dv = rsun * f;
             vh += gd * dv / amu;
             g_nt += -gd * dv * (1.0 / mus + 1.0 / amu);
@@ -308,51 +305,64 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
     };
     const int k0 = hemi ? nt : 0;
     double rho = live ? lam[(long long)k0 * Pp] : 0.0;
+    double lam_k = rho;
     double sk = S(k0);
     double pending = 0.0;
-    for (int t = 0; t < nt; ++t) {
-        const int k = hemi ? nt - t : t;
-        const int kp = hemi ? k - 1 : k + 1;
-        const double sp = S(kp);
-        double T, al, be, dT, dal, dbe;
-        step_coefs(fabs(lv.h[k] - lv.h[kp]), amu, T, al, be, dT, dal, dbe);
-        const double sbar = pending + al * rho;
-        pending = be * rho;
-        const double dd = rho * (dT * R(kp) + dal * sk + dbe * sp);
-        double vx, vy, vh;
-        level_terms(k, sbar, vx, vy, vh);
-        double xu, xd, yu, yd, hu, hd, du, dn;
-        hemi_sums(vx, hemi, xu, xd);
-        hemi_sums(vy, hemi, yu, yd);
-        hemi_sums(vh, hemi, hu, hd);
-        hemi_sums(dd, hemi, du, dn);
-        if (lane == 0) {
-            if (xu != 0.0) atomicAdd(&gX[t], xu);
-            if (yu != 0.0) atomicAdd(&gY[t], yu);
-            if (hu != 0.0) atomicAdd(&gH[t], hu);
-            if (du != 0.0) atomicAdd(&gD[t], du);
-            if (xd != 0.0) atomicAdd(&gX[nt - t], xd);
-            if (yd != 0.0) atomicAdd(&gY[nt - t], yd);
-            if (hd != 0.0) atomicAdd(&gH[nt - t], hd);
-            if (dn != 0.0) atomicAdd(&gD[nt - t - 1], dn);
+    constexpr int kB = 8;
+    for (int t0 = 0; t0 < nt; t0 += kB) {
+        // gather the block's upstream levels first (clamped: no branches)
+        double uv[kB], lv_[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int tc = t0 + u < nt ? t0 + u : nt - 1;
+            const long long ix = (long long)(hemi ? nt - tc - 1 : tc + 1) * Pp;
+            uv[u] = live ? U[ix] : 0.0;
+            lv_[u] = live ? lam[ix] : 0.0;
         }
-        rho = (live ? lam[(long long)kp * Pp] : 0.0) + T * rho;
-        sk = sp;
+        // per step: d/d xdel, d/d ydel, d/dh at level k and d/d(layer step)
+        double red[4 * kB];
+#pragma unroll
+        for (int e = 0; e < 4 * kB; ++e) red[e] = 0.0;
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int t = t0 + u;
+            if (t >= nt) break;
+            const int k = hemi ? nt - t : t;
+            const int kp = hemi ? k - 1 : k + 1;
+            const double sp = S(kp);
+            double T, al, be, dT, dal, dbe;
+            step_coefs(fabs(lv.h[k] - lv.h[kp]), amu, T, al, be, dT, dal, dbe);
+            const double sbar = pending + al * rho;
+            pending = be * rho;
+            const double rk = live ? uv[u] - (rough ? rsun * seed_f(kp) : 0.0) : 0.0;
+            level_terms(k, sbar, lam_k, red[4 * u + 0], red[4 * u + 1], red[4 * u + 2]);
+            red[4 * u + 3] = rho * (dT * rk + dal * sk + dbe * sp);
+            rho = lv_[u] + T * rho;
+            lam_k = lv_[u];
+            sk = sp;
+        }
+        wave_sum_8xC<4>(red, lane);
+        const int u = lane >> 3, t = t0 + u;
+        if ((lane & 7) == 0 && t < nt) {
+            const int k = hemi ? nt - t : t;
+            if (red[0] != 0.0) atomicAdd(&gX[k], red[0]);
+            if (red[1] != 0.0) atomicAdd(&gY[k], red[1]);
+            if (red[2] != 0.0) atomicAdd(&gH[k], red[2]);
+            if (red[3] != 0.0) atomicAdd(&gD[hemi ? k - 1 : k], red[3]);
+        }
     }
+    // last level of the walk: nt for upward columns, 0 for downward ones
+    const int kl = hemi ? 0 : nt;
     double vx, vy, vh;
-    level_terms(hemi ? 0 : nt, pending, vx, vy, vh);
-    double xu, xd, yu, yd, hu, hd;
-    hemi_sums(vx, hemi, xu, xd);
-    hemi_sums(vy, hemi, yu, yd);
-    hemi_sums(vh, hemi, hu, hd);
+    level_terms(kl, pending, lam_k, vx, vy, vh);
+    vx = wave_sum(vx);
+    vy = wave_sum(vy);
+    vh = wave_sum(vh);
     const double snt = wave_sum(g_nt), sw = wave_sum(g_w);
     if (lane == 0) {
-        if (xu != 0.0) atomicAdd(&gX[nt], xu);
-        if (yu != 0.0) atomicAdd(&gY[nt], yu);
-        if (hu != 0.0) atomicAdd(&gH[nt], hu);
-        if (xd != 0.0) atomicAdd(&gX[0], xd);
-        if (yd != 0.0) atomicAdd(&gY[0], yd);
-        if (hd != 0.0) atomicAdd(&gH[0], hd);
+        if (vx != 0.0) atomicAdd(&gX[kl], vx);
+        if (vy != 0.0) atomicAdd(&gY[kl], vy);
+        if (vh != 0.0) atomicAdd(&gH[kl], vh);
         if (snt != 0.0) atomicAdd(&gH[nt], snt);
         if (sw != 0.0) atomicAdd(gW, sw * slope_variance_dwind(I.wind, I.sigma_type));
     }

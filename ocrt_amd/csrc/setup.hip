@@ -142,8 +142,10 @@ template <bool DERIV>
 __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict__ inst, Pools P)
 {
     constexpr int kHalf = kNPhi / 2;
-    __shared__ double R[9][kHalf];
-    __shared__ double Rd[DERIV ? 9 : 1][DERIV ? kHalf : 1];
+    // rows padded by one entry: the nine rows are read together at one phi
+    // and must not fall on one LDS bank
+    __shared__ double R[9][kHalf + 1];
+    __shared__ double DL[DERIV ? kHalf : 1];
     const DevInst &I = inst[blockIdx.y];
     if (I.surf != 2 || !I.surf_owner) return;
     const int n = I.n;
@@ -179,10 +181,7 @@ __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict
         double r[9], tb2;
         coxmunk_R_core(mo, mi, cos(phi), sin(phi), s2v, shadow, I.n_water, I.q_conv, r, &tb2);
         for (int e = 0; e < 9; ++e) R[e][p] = r[e];
-        if (DERIV) {
-            const double dl = (tb2 >= 0.0) ? tb2 / (s2v * s2v) - 1.0 / s2v + dls : 0.0;
-            for (int e = 0; e < 9; ++e) Rd[e][p] = r[e] * dl;
-        }
+        if (DERIV) DL[p] = (tb2 >= 0.0) ? tb2 / (s2v * s2v) - 1.0 / s2v + dls : 0.0;
     }
     __syncthreads();
     const int nout = (I.mmax + 1) * 9;
@@ -191,9 +190,11 @@ __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict
         const bool use_cos = (kl == 0 || kl == 1 || kl == 3 || kl == 4 || kl == 8);
         const double *W = P.cm_w + ((long long)m * 2 + (use_cos ? 0 : 1)) * kNPhi;
         double acc = 0.0, accd = 0.0;
-        for (int p = 0; p < kHalf; ++p) acc += R[kl][p] * W[p];
-        if (DERIV)
-            for (int p = 0; p < kHalf; ++p) accd += Rd[kl][p] * W[p];
+        for (int p = 0; p < kHalf; ++p) {
+            const double rw = R[kl][p] * W[p];
+            acc += rw;
+            if (DERIV) accd += rw * DL[p];
+        }
         const double nrm = (m == 0) ? 1.0 / (2.0 * OCRT_PI) : 1.0 / OCRT_PI;
         double v = 2.0 * acc * (dphi * nrm), vd = 2.0 * accd * (dphi * nrm);
         if (m > 0 && (kl == 2 || kl == 5)) { v = -v; vd = -vd; }

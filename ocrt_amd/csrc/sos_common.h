@@ -53,19 +53,20 @@ __device__ inline void halve_exchange(double *v, int off, bool upper)
     }
 }
 
-// Sums 24 per-lane values over the wavefront in 30 exchanges (instead of
-// 24 x 6 for one reduction per value).  With the values ordered [8][3], every
-// lane of the group of eight lanes l = 8 r .. 8 r + 7 ends with the totals of
-// row r in v[0..2].
-__device__ inline void wave_sum_8x3(double (&v)[24], int lane)
+// Sums 8 C per-lane values over the wavefront in 7 C + 3 C exchanges
+// (instead of 8 C x 6 for one reduction per value).  With the values ordered
+// [8][C], every lane of the group of eight lanes l = 8 r .. 8 r + 7 ends with
+// the totals of row r in v[0..C-1].
+template <int C>
+__device__ inline void wave_sum_8xC(double (&v)[8 * C], int lane)
 {
-    halve_exchange<12>(v, 32, (lane & 32) != 0);
-    halve_exchange<6>(v, 16, (lane & 16) != 0);
-    halve_exchange<3>(v, 8, (lane & 8) != 0);
+    halve_exchange<4 * C>(v, 32, (lane & 32) != 0);
+    halve_exchange<2 * C>(v, 16, (lane & 16) != 0);
+    halve_exchange<C>(v, 8, (lane & 8) != 0);
 #pragma unroll
     for (int off = 4; off > 0; off >>= 1)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) v[i] += __shfl_xor(v[i], off);
+        for (int i = 0; i < C; ++i) v[i] += __shfl_xor(v[i], off);
 }
 
 // Order-1 source of one state column (rt_primary_source.c:61-70, 164-171) and,
