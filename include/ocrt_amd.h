@@ -201,6 +201,19 @@ int ocrt_plan_set_lut(ocrt_plan *plan, int n_vza, const double *vza_deg, int n_r
  * either pointer may be NULL. */
 int ocrt_plan_lut_results(ocrt_plan *plan, double *grid, double *scalars);
 
+/* The remaining LUT outputs of rt_lut_grid_out_t (rt_types.h:820-833),
+ * either pointer may be NULL:
+ *   raw         [n_inst][n_vza][n_raa][3]: TOA (I, Q, U) before the division
+ *               by mu_sun, the IPSS factor and the glint (raw_I/Q/U)
+ *   view_modes  [n_inst][n_vza][m_max_all + 1][3]: the Fourier modes sampled
+ *               at each view zenith (view_per_m_I/Q/U), zero above an
+ *               instance's m_max; m_max_all = ocrt_plan_mmax(plan). */
+int ocrt_plan_lut_extras(ocrt_plan *plan, double *raw, double *view_modes);
+
+/* Final relative residual of every (instance, mode) of the last execute,
+ * [n_inst][m_max_all + 1] (sos_residual_per_m of rt_result_t, rt_types.h:511). */
+int ocrt_plan_residuals(ocrt_plan *plan, double *residuals);
+
 /* One-call form: plan, execute, copy results, destroy. */
 int ocrt_solve_batch(const ocrt_instance_t *inst, int n_inst, int precision,
                      ocrt_result_t *out, void *hip_stream);

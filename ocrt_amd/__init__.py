@@ -63,6 +63,8 @@ def library():
         lib.ocrt_plan_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_adjoint_batch.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.c_void_p, C.POINTER(Result),
                                            C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ocrt_plan_lut_extras.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ocrt_plan_residuals.argtypes = [C.c_void_p, C.c_void_p]
         _lib = lib
     return _lib
 

@@ -73,6 +73,8 @@ __device__ inline double sample_pchip(const double *y, const double *x, int n, d
 //   grid   [instance][n_vza][n_raa][6]: rho_I, rho_Q, rho_U, T_diff_dn_dir,
 //          T_sg_up_dir, T_total_up_dir
 //   scal   [instance][2]: T_dir_dn, T_diff_dn_hemi
+//   raw    [instance][n_vza][n_raa][3]: TOA (I, Q, U) before rho, IPSS and glint
+//   vpm    [instance][n_vza][m_stride][3]: sampled Fourier modes at each zenith
 // kappa_lut[job_of[i] + iv * n_raa + ir] is the IPSS factor of a direction
 // (job_of[i] < 0: IPSS off).
 __global__ __launch_bounds__(128) void lut_eval(const DevInst *__restrict__ inst, const DevItem *__restrict__ items,
@@ -80,7 +82,8 @@ __global__ __launch_bounds__(128) void lut_eval(const DevInst *__restrict__ inst
                                                 const int *__restrict__ job_of, const double *__restrict__ kappa_lut,
                                                 int n_vza, const double *__restrict__ vza, int n_raa,
                                                 const double *__restrict__ raa, double *__restrict__ grid,
-                                                double *__restrict__ scal)
+                                                double *__restrict__ scal, double *__restrict__ raw,
+                                                double *__restrict__ vpm, int m_stride)
 {
     extern __shared__ double vals[];
     const int iv = blockIdx.x, i = blockIdx.y;
@@ -94,6 +97,12 @@ __global__ __launch_bounds__(128) void lut_eval(const DevInst *__restrict__ inst
         vals[comp * M + m] = comp < 3 ? sample_pchip(y, mu, n, mv) : sample_lin(y, mu, n, mv);
     }
     __syncthreads();
+    // per-view Fourier samples [instance][vza][m_stride][3] (zero above m_max)
+    double *vq = vpm + ((long long)I.gidx * n_vza + iv) * m_stride * 3;
+    for (int q = threadIdx.x; q < 3 * m_stride; q += blockDim.x) {
+        const int m = q / 3, comp = q % 3;
+        vq[q] = m < M ? vals[comp * M + m] : 0.0;
+    }
     const LevelView lv(P, I);
     const double mus = I.mus, tau = lv.h[I.nt];
     const bool ip = I.ipss && job_of[i] >= 0;
@@ -110,6 +119,8 @@ __global__ __launch_bounds__(128) void lut_eval(const DevInst *__restrict__ inst
             rU += 2.0 * vals[2 * M + m] * sm;
             bd += 2.0 * vals[3 * M + m] * cm;
         }
+        double *w3 = raw + (((long long)I.gidx * n_vza + iv) * n_raa + ir) * 3;
+        w3[0] = rI; w3[1] = rQ; w3[2] = rU;
         rI /= mus; rQ /= mus; rU /= mus;
         if (ip) {
             const double kg = kappa_lut[job_of[i] + (long long)iv * n_raa + ir];

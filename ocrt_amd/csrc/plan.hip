@@ -47,7 +47,7 @@ __global__ void order_one_adj(const DevInst *, const DevItem *, Pools, const Col
 __global__ void glint_adj(const DevInst *, int, Pools);
 // lut.hip
 __global__ void lut_eval(const DevInst *, const DevItem *, Pools, const InstOut *, const int *, const double *, int,
-                         const double *, int, const double *, double *, double *);
+                         const double *, int, const double *, double *, double *, double *, double *, int);
 }  // namespace ocrt
 
 using namespace ocrt;
@@ -226,6 +226,7 @@ struct ocrt_plan {
     int lut = 0, lut_nv = 0, lut_nr = 0;
     long long nodes_n = 0;
     double *nodes = nullptr, *lut_vza = nullptr, *lut_raa = nullptr, *lut_grid = nullptr, *lut_scal = nullptr;
+    double *lut_raw = nullptr, *lut_vpm = nullptr;
     double *kappa_lut = nullptr, *taudir_lut = nullptr;
 };
 
@@ -840,7 +841,8 @@ static int forward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
         Pl.nodes = p->nodes;
         hipLaunchKernelGGL(lut_eval, dim3(p->lut_nv, ni), dim3(128), 4 * (p->mmax_all + 1) * sizeof(double), s,
                            c.dinst, c.ditem, Pl, p->ipss + c.i0, c.djob_lut_of, p->kappa_lut, p->lut_nv, p->lut_vza,
-                           p->lut_nr, p->lut_raa, p->lut_grid, p->lut_scal);
+                           p->lut_nr, p->lut_raa, p->lut_grid, p->lut_scal, p->lut_raw, p->lut_vpm,
+                           p->mmax_all + 1);
     }
     return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
 }
@@ -859,6 +861,8 @@ extern "C" int ocrt_plan_set_lut(ocrt_plan *p, int n_vza, const double *vza_deg,
     rc |= dmalloc(p, (void **)&p->lut_raa, (size_t)n_raa * 8);
     rc |= dmalloc(p, (void **)&p->lut_grid, (size_t)p->n_inst * grid * 6 * 8);
     rc |= dmalloc(p, (void **)&p->lut_scal, (size_t)p->n_inst * 2 * 8);
+    rc |= dmalloc(p, (void **)&p->lut_raw, (size_t)p->n_inst * grid * 3 * 8);
+    rc |= dmalloc(p, (void **)&p->lut_vpm, (size_t)p->n_inst * n_vza * (p->mmax_all + 1) * 3 * 8);
     size_t max_jobs = 1;
     for (Chunk &c : p->chunks) {
         c.hjobs_lut.clear();
@@ -896,6 +900,30 @@ extern "C" int ocrt_plan_lut_results(ocrt_plan *p, double *grid, double *scalars
     const size_t g = (size_t)p->n_inst * p->lut_nv * p->lut_nr * 6;
     if (grid) HIPCHK(hipMemcpy(grid, p->lut_grid, g * 8, hipMemcpyDeviceToHost));
     if (scalars) HIPCHK(hipMemcpy(scalars, p->lut_scal, (size_t)p->n_inst * 2 * 8, hipMemcpyDeviceToHost));
+    return OCRT_OK;
+}
+
+extern "C" int ocrt_plan_lut_extras(ocrt_plan *p, double *raw, double *view_modes)
+{
+    if (!p || !p->lut) return OCRT_ERR_ARGS;
+    HIPCHK(hipDeviceSynchronize());
+    const size_t g = (size_t)p->n_inst * p->lut_nv * p->lut_nr * 3;
+    const size_t v = (size_t)p->n_inst * p->lut_nv * (p->mmax_all + 1) * 3;
+    if (raw) HIPCHK(hipMemcpy(raw, p->lut_raw, g * 8, hipMemcpyDeviceToHost));
+    if (view_modes) HIPCHK(hipMemcpy(view_modes, p->lut_vpm, v * 8, hipMemcpyDeviceToHost));
+    return OCRT_OK;
+}
+
+extern "C" int ocrt_plan_residuals(ocrt_plan *p, double *residuals)
+{
+    if (!p || !residuals) return OCRT_ERR_ARGS;
+    HIPCHK(hipDeviceSynchronize());
+    std::vector<ItemOut> io(p->n_items);
+    HIPCHK(hipMemcpy(io.data(), p->iout, (size_t)p->n_items * sizeof(ItemOut), hipMemcpyDeviceToHost));
+    const int M = p->mmax_all + 1;
+    for (int i = 0; i < p->n_inst; ++i)
+        for (int m = 0; m < M; ++m)
+            residuals[(long long)i * M + m] = m <= p->inst[i].d.mmax ? io[p->item_of_inst[i] + m].residual : 0.0;
     return OCRT_OK;
 }
 

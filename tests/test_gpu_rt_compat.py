@@ -1,0 +1,188 @@
+"""Reference-compatible entry points on the GPU (include/ocrt_rt.h).
+
+The same rt_case_t / rt_options_t / rt_aerosol_input_t / rt_lut_grid_out_t
+buffers are handed to this library's rt_solve_case_pol{,_aerosol,_lut} and to
+the reference's own functions (compiled from its sources into
+oracle/_ref/libocrt_ref.so, run on the host); every field the atmosphere
+path writes must agree: rho and TOA Stokes to 1e-11 relative (of rho_I),
+order counts exactly, residuals to 1e-6 relative, transmittances and glint
+diagnostics to 1e-11.  The batch entry ocrt_rt_solve_batch equals the
+per-case calls.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from ocrt_amd import rt_abi
+from ocrt_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+P = C.POINTER
+
+
+@pytest.fixture(scope="module")
+def libs(ref_lib):
+    mine = rt_abi.library()
+    rt_abi.bind(ref_lib)
+    return mine, ref_lib
+
+
+def case(surface="black", sza=40.0, vza=30.0, raa=90.0, wl=555.0, wind=0.0, decouple=0, tau_r=None,
+         pressure=1013.25, sigma_type=1, q_conv=1):
+    cs = rt_abi.Case()
+    cs.sza_deg, cs.vza_deg, cs.raa_deg, cs.wavelength_nm = sza, vza, raa, wl
+    cs.surface = rt_abi.SURFACES[surface]
+    cs.rayleigh_on, cs.pressure_hpa = 1, pressure
+    cs.n_water, cs.wind_speed, cs.sigma_type, cs.q_convention = 1.34, wind, sigma_type, q_conv
+    cs.decouple_sunglint = decouple
+    cs.F_sun = math.pi
+    if tau_r is not None:
+        cs.tau_R_input = tau_r
+    return cs
+
+
+def options(n_mu=16, n_layers=30, m_max=4, pssa=0, from_input=False, tol=1e-9, max_it=40):
+    o = rt_abi.options_default()
+    o.n_mu, o.n_layers, o.fourier_m_max, o.pssa = n_mu, n_layers, m_max, pssa
+    o.sos.max_iterations, o.sos.tolerance = max_it, tol
+    o.tau_r_source = 1 if from_input else 0
+    return o
+
+
+class Aer:
+    def __init__(self, tau_a=0.15, ssa=0.93, g=0.7, L=40):
+        self.gk = S.hg_greek(g, L)
+        self.arrs = [np.ascontiguousarray(self.gk[i]) for i in range(4)]
+        a = rt_abi.Aerosol()
+        a.tau_a, a.ssa_a, a.L_max = tau_a, ssa, L
+        ptr = [x.ctypes.data_as(P(C.c_double)) for x in self.arrs]
+        a.betal_aer, a.gammal_aer, a.alphal_aer, a.zetal_aer = ptr
+        self.s = a
+
+
+CASES = [
+    (dict(surface="black"), dict(), None),
+    (dict(surface="flat", sza=52.0, vza=23.0, raa=140.0), dict(m_max=3), None),
+    (dict(surface="black_fresnel_ocean", wind=7.0, sza=38.0, vza=47.0, raa=65.0), dict(), None),
+    (dict(surface="black_fresnel_ocean", wind=4.0, decouple=1, raa=170.0, wl=443.0), dict(), None),
+    (dict(surface="black", tau_r=0.25), dict(from_input=True, pssa=1, n_layers=60), None),
+    (dict(surface="black_fresnel_ocean", wind=5.0, sza=60.0, vza=12.0, raa=10.0, wl=865.0),
+     dict(m_max=8, pssa=1), dict(tau_a=0.2)),
+    (dict(surface="flat", wl=412.0), dict(m_max=6), dict(tau_a=0.1, ssa=0.97, g=0.6)),
+    (dict(surface="black_fresnel_ocean", wind=11.0, sigma_type=0, q_conv=0), dict(m_max=5), dict(tau_a=0.3)),
+]
+
+SCALARS = ["rho_I", "rho_Q", "rho_U", "I_TOA", "Q_TOA", "U_TOA", "T_dir_dn", "T_diff_dn_hemi", "T_diff_dn_dir",
+           "T_sg_up_dir", "T_total_up_dir", "rho_atm_path_I", "rho_atm_path_Q", "rho_atm_path_U",
+           "rho_glint_direct_I", "rho_glint_direct_Q", "rho_glint_direct_U", "T_total_dn_hemi", "T_dir_up_view"]
+
+
+def run(lib, cs, o, aer):
+    out = rt_abi.Result()
+    if aer is None:
+        rc = lib.rt_solve_case_pol(C.byref(cs), C.byref(o), C.byref(out))
+    else:
+        rc = lib.rt_solve_case_pol_aerosol(C.byref(cs), C.byref(o), C.byref(aer.s), C.byref(out))
+    return rc, out
+
+
+def compare(a, b, m_max, tol=1e-11):
+    scale = abs(b.rho_I)
+    assert a.tau_R_total == pytest.approx(b.tau_R_total, rel=1e-12)
+    for f in SCALARS:
+        va, vb = getattr(a, f), getattr(b, f)
+        assert abs(va - vb) <= tol * max(scale, abs(vb)), (f, va, vb)
+    assert a.n_orders_used == b.n_orders_used and a.converged == b.converged
+    for m in range(m_max + 1):
+        assert a.sos_orders_per_m[m] == b.sos_orders_per_m[m], m
+        assert a.sos_residual_per_m[m] == pytest.approx(b.sos_residual_per_m[m], rel=1e-6, abs=1e-300)
+    for f in ("T_diff_up_view", "T_total_up_view", "I_TOA_water_signal"):
+        assert math.isnan(getattr(a, f)) and math.isnan(getattr(b, f)), f
+    assert a.T_up_rt_valid == b.T_up_rt_valid == 0
+
+
+@pytest.mark.parametrize("k", range(len(CASES)))
+def test_single_case_equals_reference(libs, k):
+    mine, ref = libs
+    ckw, okw, akw = CASES[k]
+    cs, o = case(**ckw), options(**okw)
+    aer = Aer(**akw) if akw is not None else None
+    print(f"[gpu] rt_solve_case_pol case {k}: {ckw} {okw} {akw}", flush=True)
+    ra, a = run(mine, cs, o, aer)
+    rb, b = run(ref, cs, o, aer)
+    assert ra == rb == 0
+    compare(a, b, o.fourier_m_max)
+
+
+def test_toa_mode_export(libs):
+    mine, ref = libs
+    cs, o = case(surface="black_fresnel_ocean", wind=6.0), options(m_max=5)
+    got = []
+    for lib in (mine, ref):
+        arrs = [np.zeros(6) for _ in range(3)]
+        o.toa_view_per_m_I, o.toa_view_per_m_Q, o.toa_view_per_m_U = [x.ctypes.data_as(P(C.c_double)) for x in arrs]
+        rc, _ = run(lib, cs, o, None)
+        assert rc == 0
+        got.append(np.stack(arrs))
+    assert np.max(np.abs(got[0] - got[1])) <= 1e-11 * np.max(np.abs(got[1]))
+
+
+def test_lut_equals_reference(libs):
+    mine, ref = libs
+    vza = np.arange(0.0, 86.0, 5.0)
+    raa = np.arange(0.0, 360.0, 15.0)
+    nv, nr = len(vza), len(raa)
+    for ckw, okw, akw in [(dict(surface="black_fresnel_ocean", wind=3.0, decouple=1), dict(m_max=2), None),
+                          (dict(surface="black_fresnel_ocean", wind=5.0), dict(m_max=6, pssa=1), dict(tau_a=0.1)),
+                          (dict(surface="flat", sza=60.0), dict(m_max=4), None)]:
+        print(f"[gpu] rt_solve_case_pol_lut {ckw} {okw} {akw}", flush=True)
+        cs, o = case(**ckw), options(**okw)
+        aer = Aer(**akw) if akw is not None else None
+        mm = o.fourier_m_max
+        outs = []
+        for lib in (mine, ref):
+            bufs = {k: np.full(nv * nr, np.nan) for k in ("rho_I", "rho_Q", "rho_U", "raw_I", "raw_Q", "raw_U",
+                                                           "T_diff_dn_dir", "T_sg_up_dir", "T_total_up_dir")}
+            vpm = {k: np.full((mm + 1) * nv, np.nan) for k in ("view_per_m_I", "view_per_m_Q", "view_per_m_U")}
+            g = rt_abi.LutGrid()
+            g.n_vza, g.n_raa = nv, nr
+            g.vza_deg = vza.ctypes.data_as(P(C.c_double))
+            g.raa_deg = raa.ctypes.data_as(P(C.c_double))
+            for k, v in list(bufs.items()) + list(vpm.items()):
+                setattr(g, k, v.ctypes.data_as(P(C.c_double)))
+            g.view_m_max = mm
+            out = rt_abi.Result()
+            rc = lib.rt_solve_case_pol_lut(C.byref(cs), C.byref(o), C.byref(aer.s) if aer else None,
+                                           C.byref(out), C.byref(g))
+            assert rc == 0
+            outs.append((bufs, vpm, g.T_dir_dn, g.T_diff_dn_hemi, g.view_m_max_filled, out))
+        (ba, va, ta, ha, fa, oa), (bb, vb, tb, hb, fb, ob) = outs
+        scale = np.max(np.abs(bb["rho_I"]))
+        for k in ba:
+            assert np.max(np.abs(ba[k] - bb[k])) <= 1e-11 * max(scale, np.max(np.abs(bb[k]))), k
+        for k in va:
+            assert np.max(np.abs(va[k] - vb[k])) <= 1e-11 * np.max(np.abs(vb[k]) + 1e-300), k
+        assert ta == pytest.approx(tb, rel=1e-12) and ha == pytest.approx(hb, rel=1e-11)
+        assert fa == fb == mm
+        compare(oa, ob, mm)
+
+
+def test_batch_equals_single_calls(libs):
+    mine, _ = libs
+    o = options(m_max=4)
+    cases = [case(surface="black_fresnel_ocean", wind=w, sza=s, vza=v, raa=r)
+             for w, s, v, r in [(3.0, 30.0, 10.0, 40.0), (7.0, 55.0, 35.0, 120.0), (0.0, 20.0, 60.0, 300.0)]]
+    cases.append(case(surface="black", sza=95.0))   # invalid: refused per case
+    arr = (rt_abi.Case * len(cases))(*cases)
+    outs = (rt_abi.Result * len(cases))()
+    rcs = (C.c_int * len(cases))()
+    rc = mine.ocrt_rt_solve_batch(arr, C.byref(o), None, len(cases), outs, rcs)
+    assert rc == -1 and list(rcs) == [0, 0, 0, -1]
+    for i in range(3):
+        rc1, one = run(mine, cases[i], o, None)
+        assert rc1 == 0
+        for f in SCALARS:
+            assert getattr(outs[i], f) == getattr(one, f), (i, f)
