@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libocrt_amd.so")
+CLI = os.path.join(HERE, "ocrt")
 SOURCES = ["plan.hip", "setup.hip", "sos.hip", "ipss.hip", "adjoint.hip", "lut.hip", "rt_compat.hip"]
 ARCH = os.environ.get("OCRT_OFFLOAD_ARCH", "gfx950")
 
@@ -43,7 +44,22 @@ def build(verbose=False, jobs=4):
     cmd = [hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+def build_cli(verbose=False):
+    """The `ocrt` command-line program (csrc/cli.cpp), linked against the
+    library next to it."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = CLI + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "cli.cpp"),
+           "-L", HERE, "-locrt_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, CLI)
+    return CLI
 
 
 def _wait(item):

@@ -69,6 +69,7 @@ struct Prep {
     ocrt_instance_t in{};
     std::vector<double> greek;
     double tau_R = 0.0;
+    bool zero_out = false;   // refused after the reference has cleared its output
 };
 
 int prepare(const rt_case_t *cs, const rt_options_t *opts, const rt_aerosol_input_t *aer, Prep &p)
@@ -107,6 +108,12 @@ int prepare(const rt_case_t *cs, const rt_options_t *opts, const rt_aerosol_inpu
         p.tau_R = cs->tau_R_input;
     } else {
         p.tau_R = rayleigh_tau(wl, cs->pressure_hpa, 45.0, 0.0, 360.0);
+    }
+    // an atmosphere with neither Rayleigh nor aerosol is refused by the
+    // reference's medium build, after the output was cleared (rt_atm.c:182)
+    if (!aerosol && !(p.tau_R > 0.0)) {
+        p.zero_out = true;
+        return OCRT_ERR_ARGS;
     }
     ocrt_instance_t &d = p.in;
     d.sza_deg = cs->sza_deg;
@@ -195,6 +202,7 @@ int solve(const rt_case_t *cs, const rt_options_t *opts_in, const rt_aerosol_inp
     const rt_options_t *opts = opts_in ? opts_in : &defaults;
     Prep p;
     int rc = prepare(cs, opts, aer, p);
+    if (rc && p.zero_out) std::memset(out, 0, sizeof(*out));
     if (rc) return rc;
     if (lut) {
         if (lut->n_vza <= 0 || !lut->vza_deg || lut->n_raa <= 0 || !lut->raa_deg || !lut->rho_I || !lut->rho_Q ||
@@ -295,6 +303,7 @@ extern "C" int ocrt_rt_solve_batch(const rt_case_t *cs, const rt_options_t *opts
     for (int i = 0; i < n; ++i) {
         rcs[i] = prepare(&cs[i], opts, aer ? &aer[i] : nullptr, prep[i]);
         if (!rcs[i]) live.push_back(i);
+        else if (prep[i].zero_out) std::memset(&out[i], 0, sizeof(rt_result_t));
     }
     for (int i : live) {
         prep[i].in.greek = prep[i].greek.empty() ? nullptr : prep[i].greek.data();
