@@ -94,7 +94,25 @@ typedef struct {
      * absorption); tau_R and tau_a are ignored and IPSS is not available
      * (the layering carries no altitudes). */
     const double *layers;
+    /* Optional gas absorption table (ocrt_gas_create); NULL: none.  Applies
+     * to the column generator above (with layers, absorption is layers[3]). */
+    const struct ocrt_gas *gas;
 } ocrt_instance_t;
+
+/* Gas absorption of an AFGL standard atmosphere with layered HITRAN cross
+ * sections (rt_absorption.c, rt_atm.c:437-548): made once on the host from
+ * the reference's data files, shared by any number of instances and plans.
+ *   profile           0 usstd76, 1 tropical, 2 mlsumm, 3 mlwint, 4 sasumm,
+ *                     5 sawint, 6 userdef (afgl_<name>.dat under afgl_dir)
+ *   column_override   [6] molecules cm^-2 for h2o, o3, o2, co2, no2, ch4;
+ *                     < 0 keeps the AFGL column; NULL keeps all
+ * Returns OCRT_OK, OCRT_ERR_ARGS (unreadable profile or malformed table). */
+typedef struct ocrt_gas ocrt_gas;
+int    ocrt_gas_create(const char *afgl_dir, const char *xsec_dir, int profile, const double *column_override,
+                       ocrt_gas **gas_out);
+/* Column absorption optical depth of gas g (0..5), or of all gases (g = -1). */
+double ocrt_gas_tau(const ocrt_gas *gas, int g, double wavelength_nm);
+void   ocrt_gas_destroy(ocrt_gas *gas);
 
 /* Per-instance results; names follow rt_result_t (code/OCRT_C/src/rt_types.h:496-651). */
 typedef struct {

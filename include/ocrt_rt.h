@@ -30,9 +30,10 @@
  *   - aerosol: tau_a > 0 with its Greek coefficients (L_max >= 2).
  * Requests this build does not serve return -1 and print one line on stderr:
  * integration_method CONSTANT (a diagnostic of the reference), view_as_node
- * 0 (its diagnostic interpolation route), gas absorption (abs_state), the
- * ocean-coupling bottom source (ext_bottom_*, bottom_source_only), the
- * aerosol value kernel (use_value_kernel) and surface OCEAN.
+ * 0 (its diagnostic interpolation route), the ocean-coupling bottom source
+ * (ext_bottom_*, bottom_source_only), the aerosol value kernel
+ * (use_value_kernel) and surface OCEAN.  Gas absorption is served: abs_state
+ * is an rt_absorption_t (below).
  *
  * Every call solves one case in FP64 on the current HIP device, synchronously.
  * Calls from several host threads are safe; each builds its own plan.
@@ -178,7 +179,7 @@ typedef struct {
     int    lut_enable;
     const char *lut_output_dir;
     double lut_vza_step, lut_raa_step, lut_vza_max;
-    const void *abs_state;     /* gas absorption state; not served here */
+    const void *abs_state;     /* gas absorption: an rt_absorption_t, or NULL */
     double aer_h_km;           /* aerosol scale height */
     double conv_tol;
     int    verbose;
@@ -241,6 +242,58 @@ static inline rt_options_t rt_options_default(void)
     o.conv_tol = 1.0e-6;
     return o;
 }
+
+/* ---- gas absorption state (code/OCRT_C/src/rt_absorption.h:24-86) ------
+ * rt_options_t.abs_state points at an rt_absorption_t: an AFGL standard
+ * atmosphere (levels ascending from the ground; mixing ratios in ppmv) and
+ * one cross-section table per gas (sigma in cm^2 per molecule, layered
+ * [n_layers][n_wl] on the first n_layers AFGL levels, or a single layer),
+ * with the effective column of each gas (the AFGL default or an override in
+ * molecules cm^-2).  The solver adds the layer absorption optical depth
+ * sum_gas sigma(z_mid, lambda) N_gas(layer) to the medium and rescales the
+ * scattering fractions (rt_atm.c:437-548). */
+typedef enum {
+    RT_AFGL_USSTD76 = 0, RT_AFGL_TROPICAL = 1, RT_AFGL_MLSUMM = 2, RT_AFGL_MLWINT = 3,
+    RT_AFGL_SASUMM = 4, RT_AFGL_SAWINT = 5, RT_AFGL_USERDEF = 6
+} rt_afgl_profile_t;
+
+#define RT_N_GAS 6
+enum { RT_GAS_H2O = 0, RT_GAS_O3, RT_GAS_O2, RT_GAS_CO2, RT_GAS_NO2, RT_GAS_CH4 };
+
+typedef struct {
+    int n_levels;
+    double *z_km, *P_mbar, *T_K, *n_total_cm3;
+    double *mr[RT_N_GAS];                /* ppmv per level */
+    double column_default[RT_N_GAS];     /* molecules cm^-2, trapezoid over the levels */
+} rt_afgl_atm_t;
+
+typedef struct {
+    int n_wl;
+    int n_layers;
+    double *wl_nm;                       /* [n_wl], ascending */
+    double *sigma_cm2;                   /* [n_layers][n_wl] */
+    double T_ref_K, P_ref_atm;
+    int available;
+} rt_xsec_t;
+
+typedef struct {
+    rt_afgl_atm_t atm;
+    rt_xsec_t xsec[RT_N_GAS];
+    double column_eff[RT_N_GAS];
+    int column_overridden[RT_N_GAS];
+    int initialized;
+} rt_absorption_t;
+
+/* Read inputs/afgl_atm/afgl_<profile>.dat and inputs/xsec/xsec_<gas>.dat
+ * style files (the reference's formats, rt_absorption.c:40-331);
+ * gas_column_override[g] < 0 keeps the AFGL column.  A missing cross-section
+ * file leaves that gas out.  0 on success. */
+int rt_absorption_init(const char *afgl_dir, const char *xsec_dir, rt_afgl_profile_t profile,
+                       const double gas_column_override[RT_N_GAS], rt_absorption_t *abs_state);
+void rt_absorption_free(rt_absorption_t *abs_state);
+/* Column absorption optical depth of one gas / of all gases. */
+double rt_absorption_tau_per_gas(const rt_absorption_t *abs_state, int gas_idx, double wavelength_nm);
+double rt_absorption_tau_total(const rt_absorption_t *abs_state, double wavelength_nm);
 
 int rt_solve_case_pol(const rt_case_t *cs, const rt_options_t *opts, rt_result_t *out);
 int rt_solve_case_pol_aerosol(const rt_case_t *cs, const rt_options_t *opts,

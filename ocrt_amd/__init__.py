@@ -63,6 +63,11 @@ def library():
         lib.ocrt_plan_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_adjoint_batch.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.c_void_p, C.POINTER(Result),
                                            C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ocrt_gas_create.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]
+        lib.ocrt_gas_tau.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        lib.ocrt_gas_tau.restype = C.c_double
+        lib.ocrt_gas_destroy.argtypes = [C.c_void_p]
+        lib.ocrt_gas_destroy.restype = None
         lib.ocrt_plan_lut_extras.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_plan_residuals.argtypes = [C.c_void_p, C.c_void_p]
         _lib = lib
@@ -73,6 +78,38 @@ def _require_gpu():
     import torch
     if not torch.cuda.is_available():
         raise OcrtError("the OCRT MI355X solver needs a ROCm GPU; none is visible")
+
+
+PROFILES = {"usstd76": 0, "tropical": 1, "mlsumm": 2, "mlwint": 3, "sasumm": 4, "sawint": 5, "userdef": 6}
+
+
+class GasTable:
+    """Gas absorption table (ocrt_gas_create): an AFGL profile with layered
+    cross sections read from afgl_dir / xsec_dir in the reference's formats.
+    `columns` optionally overrides the total column of gases by name
+    (h2o, o3, o2, co2, no2, ch4; molecules cm^-2).  Pass it as `gas=` to
+    ocrt_amd.abi.make_instance."""
+
+    def __init__(self, afgl_dir, xsec_dir, profile="usstd76", columns=None):
+        self.lib = library()
+        ov = (C.c_double * 6)(*[-1.0] * 6)
+        for k, v in (columns or {}).items():
+            ov[("h2o", "o3", "o2", "co2", "no2", "ch4").index(k)] = float(v)
+        h = C.c_void_p()
+        rc = self.lib.ocrt_gas_create(afgl_dir.encode(), xsec_dir.encode(), PROFILES.get(profile, profile), ov,
+                                      C.byref(h))
+        if rc != 0:
+            raise OcrtError(f"ocrt_gas_create failed with code {rc}")
+        self.handle = h.value
+
+    def tau(self, wavelength_nm, gas=-1):
+        """Column absorption optical depth of one gas (index) or of all (-1)."""
+        return self.lib.ocrt_gas_tau(self.handle, gas, wavelength_nm)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self.lib.ocrt_gas_destroy(self.handle)
+            self.handle = None
 
 
 class Plan:

@@ -40,6 +40,7 @@ class Instance(C.Structure):
         ("max_orders", C.c_int), ("ipss", C.c_int),
         ("tolerance", C.c_double),
         ("layers", C.POINTER(C.c_double)),
+        ("gas", C.c_void_p),
     ]
 
 
@@ -67,12 +68,13 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
                   surface="flat", n_water=1.34, wind=0.0, sigma_type=1,
                   q_convention=1, decouple_sunglint=0, tau_R=0.0941, depol=0.0279,
                   tau_a=0.0, ssa_a=1.0, aer_h_km=2.0, greek=None, n_mu=24,
-                  n_layers=50, m_max=2, max_orders=20, tolerance=1e-7, ipss=0, layers=None):
+                  n_layers=50, m_max=2, max_orders=20, tolerance=1e-7, ipss=0, layers=None, gas=None):
     """Fill one descriptor.  `greek` is a float64 array [4, L+1] (beta, gamma,
     alpha, zeta) or None.  `layers` is None (column generator) or a float64
     array [4, n_layers] of per-layer Rayleigh tau, aerosol tau, aerosol SSA and
-    absorption tau, top down; n_layers then follows its shape.  Converted
-    copies are kept alive by the descriptor."""
+    absorption tau, top down; n_layers then follows its shape.  `gas` is an
+    ocrt_gas handle (GasTable) or None.  Converted copies are kept alive by the
+    descriptor."""
     d = Instance()
     d.sza_deg, d.vza_deg, d.raa_deg, d.wavelength_nm = sza, vza, raa, wavelength
     d.surface = SURFACE_NAMES[surface] if isinstance(surface, str) else int(surface)
@@ -102,6 +104,8 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
         d._layers_keepalive = lay
     else:
         d.layers = None
+    d.gas = gas.handle if gas is not None else None
+    d._gas_keepalive = gas
     return d
 
 

@@ -12,9 +12,10 @@
 //   batch            --batch IN.csv --output OUT.csv (rt_io.c:64-448); all rows
 //                    are solved together by one GPU plan
 // The solves go through the reference-compatible entry points of
-// libocrt_amd.so (include/ocrt_rt.h).  What this build does not serve (the
-// coupled ocean, the Mie aerosol pipeline, gas absorption other than all six
-// columns set to zero) ends with an error message and exit status 2.
+// libocrt_amd.so (include/ocrt_rt.h); gas absorption reads the reference's
+// data files under ./inputs like the reference does.  What this build does not
+// serve (the coupled ocean, the Mie aerosol pipeline) ends with an error
+// message and exit status 2.
 #include <cerrno>
 #include <chrono>
 #include <cmath>
@@ -79,9 +80,10 @@ void usage(FILE *fp, const char *prog)
         "  --wind-speed MS   required for black_fresnel_ocean (0 = flat Fresnel)\n"
         "  --n-water X (1.34)   --decouple-sunglint   --pssa (IPSS spherical-shell correction)\n"
         "  --pressure HPA (1013.25; 0 = no Rayleigh)   --no-rayleigh   --rayleigh on|off\n"
-        "Gas absorption is on by default; this build serves the absorption-free atmosphere:\n"
-        "  --gas-column-h2o 0 --gas-column-o3 0 --gas-column-no2 0 --gas-column-o2 0\n"
-        "  --gas-column-co2 0 --gas-column-ch4 0   (--atm-profile NAME accepted)\n"
+        "Gas absorption (always on; AFGL profile + cross sections read from ./inputs/afgl_atm, ./inputs/xsec):\n"
+        "  --atm-profile usstd76|tropical|mlsumm|mlwint|sasumm|sawint|userdef\n"
+        "  --gas-column-h2o G (g/cm2, alias --pwv-g-cm)  --gas-column-o3 DU  --gas-column-no2 DU\n"
+        "  --gas-column-o2 N  --gas-column-co2 N  --gas-column-ch4 N (molecules/cm2); all six 0 = no absorption\n"
         "Aerosol (--aod-555/--aod-865/--aod > 0 with --mie FILE): the Mie pipeline is not served;\n"
         "  aerosol solves go through rt_solve_case_pol_aerosol with Greek coefficients.\n"
         "Output: --output-full-grid F  --output-advanced (batch CSV detail columns)  --verbose\n"
@@ -98,11 +100,10 @@ void usage(FILE *fp, const char *prog)
 void examples(FILE *fp, const char *prog)
 {
     std::fprintf(fp,
-        "# single geometry, Rayleigh over a rough ocean, no gas absorption\n"
-        "%s --surface black_fresnel_ocean --wind-speed 5 --sza 40 --vza 30 --raa 90 --wavelength 555 \\\n"
-        "   --gas-column-h2o 0 --gas-column-o3 0 --gas-column-no2 0 --gas-column-o2 0 --gas-column-co2 0 --gas-column-ch4 0\n"
-        "# LUT grid with IPSS, black surface\n"
-        "%s --surface black --sza 70 --wavelength 443 --pssa --output-full-grid lut.csv [gas columns 0]\n"
+        "# single geometry, Rayleigh and US76 gases over a rough ocean (run where ./inputs holds the data)\n"
+        "%s --surface black_fresnel_ocean --wind-speed 5 --sza 40 --vza 30 --raa 90 --wavelength 555\n"
+        "# LUT grid with IPSS over a black surface, 300 DU of ozone\n"
+        "%s --surface black --sza 70 --wavelength 443 --pssa --gas-column-o3 300 --output-full-grid lut.csv\n"
         "# batch CSV: case_id,sza_deg,vza_deg,raa_deg,wavelength_nm[,tau_R_ref[,rho_I_ref]]\n"
         "%s --surface flat --batch in.csv --output out.csv\n", prog, prog, prog);
 }
@@ -417,6 +418,7 @@ int main(int argc, char **argv)
     double lut_vza_step = 2.5, lut_raa_step = 2.5, lut_vza_max = 85.0;
     int aer_L_max = 32;
     double gas_col[6] = {-1, -1, -1, -1, -1, -1};   // h2o o3 o2 co2 no2 ch4 (mol/cm^2)
+    int atm_profile = RT_AFGL_USSTD76;
 
     for (int i = 1; i < argc; ++i) {
         const char *a = argv[i];
@@ -517,9 +519,11 @@ int main(int argc, char **argv)
         else if (!std::strcmp(a, "--atm-profile") || !std::strcmp(a, "--gas-profile")) {
             const char *s = next_str(argc, argv, i, a);
             static const char *ok[] = {"usstd76", "tropical", "mlsumm", "mlwint", "sasumm", "sawint", "userdef"};
-            bool found = false;
-            for (const char *p : ok) found |= !std::strcmp(s, p);
-            if (!found) {
+            int found = -1;
+            for (int q = 0; q < 7; ++q)
+                if (!std::strcmp(s, ok[q])) found = q;
+            atm_profile = found;
+            if (found < 0) {
                 std::fprintf(stderr, "error: --atm-profile must be one of "
                                      "usstd76|tropical|mlsumm|mlwint|sasumm|sawint|userdef\n");
                 return 2;
@@ -742,14 +746,24 @@ int main(int argc, char **argv)
                              "= flat Fresnel)\n");
         return 2;
     }
-    // gas absorption: on unless all six columns are set to zero
-    bool gas_free = true;
-    for (double g : gas_col) gas_free &= (g == 0.0);
-    if (!gas_free) {
-        std::fprintf(stderr, "error: gas absorption (AFGL profiles + cross sections) is not served by this build; "
-                             "set all six --gas-column-* to 0 for the absorption-free atmosphere\n");
-        return 2;
+    // gas absorption, always on: AFGL profile and cross sections from the
+    // fixed data directories under the working directory (main.c:3592-3613);
+    // six zero columns give the absorption-free atmosphere
+    rt_absorption_t abs_state;
+    const int arc = rt_absorption_init("inputs/afgl_atm", "inputs/xsec", (rt_afgl_profile_t)atm_profile, gas_col,
+                                       &abs_state);
+    if (arc < 0) {
+        std::fprintf(stderr, "error: rt_absorption_init failed rc=%d\n", arc);
+        return 1;
     }
+    opts.abs_state = &abs_state;
+    std::fprintf(stderr, "rt_absorption (SOS-integrated): wl=%.1fnm tau_abs_col=%.4e\n", cs.wavelength_nm,
+                 rt_absorption_tau_total(&abs_state, cs.wavelength_nm));
+    static const char *gas_names[6] = {"h2o", "o3", "o2", "co2", "no2", "ch4"};
+    for (int g = 0; g < RT_N_GAS; ++g)
+        std::fprintf(stderr, "  %s: column=%.3e mol/cm2%s  tau_abs_col=%.4e\n", gas_names[g],
+                     abs_state.column_eff[g], abs_state.column_overridden[g] ? " (overridden)" : "",
+                     rt_absorption_tau_per_gas(&abs_state, g, cs.wavelength_nm));
     if (mie_path) {
         std::fprintf(stderr, "error: --mie: the Mie-table aerosol pipeline is not served by this build; use "
                              "rt_solve_case_pol_aerosol with Greek coefficients\n");
@@ -764,6 +778,7 @@ int main(int argc, char **argv)
         }
         Grid g(lut_vza_max, lut_vza_step, lut_raa_step);
         const int rc = rt_solve_case_pol_lut(&cs, &opts, nullptr, &res, &g.lut);
+        rt_absorption_free(&abs_state);
         if (rc != 0) { std::fprintf(stderr, "error: rt_solve_case_pol_lut rc=%d\n", rc); return 1; }
         FILE *fp = std::fopen(lut_name.c_str(), "w");
         if (!fp) { std::fprintf(stderr, "error: cannot open LUT output '%s'\n", lut_name.c_str()); return 1; }
@@ -777,6 +792,7 @@ int main(int argc, char **argv)
         return 0;
     }
     const int rc = rt_solve_case_pol(&cs, &opts, &res);
+    rt_absorption_free(&abs_state);
     if (rc != 0) { std::fprintf(stderr, "error: rt_solve_case_pol rc=%d\n", rc); return 1; }
     res.aerosol_aod_ref = user_aod;
     res.aerosol_aod_ref_nm = user_aod_ref_nm;

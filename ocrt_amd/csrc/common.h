@@ -1,7 +1,8 @@
 // Shared device/host definitions of the MI355X SOS solver.
 //
 // Memory layout (one "chunk" of instances is resident at a time):
-//   per instance : level arrays h, ch, xdel, ydel, z [L = nt + 1]; ring cosines
+//   per instance : level arrays h, ch, xdel, ydel, z [L = nt + 1] and the gas
+//                  absorption of each layer [nt] (in a sixth row); ring cosines
 //                  and weights [n]; Beer-Lambert factors T[layer][node]
 //                  (layer-major so a wavefront reads consecutive nodes);
 //                  Greek arrays [4][La+1]; rough-surface Fourier kernels
@@ -12,7 +13,7 @@
 //                  and the fields F (previous order), J (source), total
 //                  (fp64 order sum) and comp (compensation, mixed form).
 //   Field row k (level) holds 6n state columns ordered
-//     [I_up(n) I_dn(n) Q_up(n) Q_dn(n) U_up(n) U_dn(n)], node o = 0..n-1
+//     [I_up(n) Q_up(n) U_up(n) I_dn(n) Q_dn(n) U_dn(n)], node o = 0..n-1
 //   padded to the row pitch P; pad columns stay zero.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -34,8 +35,8 @@ struct DevInst {
     int sigma_type, q_conv, decouple, ipss, max_orders, item0, surface_kind, surf_owner;
     int layered;         // 1: explicit per-layer optical depths (pool_layers)
     int gidx;            // index of the instance in the plan
-    int pad2_;
-    long long off_lev;   // pool_lev: 5 arrays of L doubles (h, ch, xd, yd, z)
+    int gas;             // gas absorption table (Pools::gas_tab), -1: none
+    long long off_lev;   // pool_lev: 6 arrays of L doubles (h, ch, xd, yd, z, layer absorption)
     long long off_ring;  // pool_ring: mu[n], w[n]
     long long off_T;     // pool_T (double) / pool_Tf (float): [nt][n]
     long long off_greek; // pool_greek: [4][La+1]
@@ -43,6 +44,19 @@ struct DevInst {
     long long off_Rsun;  // pool_R: [(mmax+1)][n][9]
     long long off_layers;// pool_layers: [4][nt] Rayleigh tau, aerosol tau, aerosol ssa, absorption tau
     long long off_grad;  // pool_grad: gH[L], gX[L], gY[L], gD[nt], gW[1] (adjoint)
+    double wl;           // wavelength (nm)
+};
+
+// One gas absorption table in the gas pool: AFGL levels z[n_lev] (km,
+// ascending) and number densities n[6][n_lev] (cm^-3); per gas a
+// cross-section table sigma[n_layers][n_wl] over wl[n_wl] (cm^2), layer i at
+// altitude z[i].
+struct DevGas {
+    int n_lev, pad_;
+    int n_wl[6], n_layers[6], avail[6], overridden[6];
+    double col_def[6], col_eff[6];
+    long long off_z, off_n;
+    long long off_wl[6], off_sig[6];
 };
 
 struct DevItem {
@@ -95,6 +109,8 @@ struct Pools {
     void *Flo;            // bf16 low half of the field pair (mixed form, fsplit items)
     double *cm_w;         // cos/sin(m phi) table [(Mall+1)][2][kNPhi]
     double *nodes;        // LUT plans: per item [4][n] TOA upward I, Q, U and BOA downward I at every node
+    const DevGas *gas_tab;    // gas absorption tables of the plan
+    const double *gas_pool;   // their arrays
 };
 
 // Workgroups of a launch are dealt round-robin over the 8 XCDs (block b runs

@@ -114,7 +114,11 @@ __global__ __launch_bounds__(256) void ipss_kappa(const DevInst *__restrict__ in
             const double wa = xd[k + 1] > 0.0 ? xd[k + 1] : 0.0;
             src[j] = wr * PR + wa * PA;
         } else {
-            src[j] = dtau > 0.0 ? 1.0 : 0.0;
+            // single-scattering albedo of the layer: gas absorption removed
+            // (rt_ipss.c:64-70)
+            double ta = I.gas >= 0 ? h[5 * L + k] : 0.0;
+            ta = ta < 0.0 ? 0.0 : (ta > dtau ? dtau : ta);
+            src[j] = dtau > 0.0 ? (dtau - ta) / dtau : 0.0;
         }
     }
     __syncthreads();

@@ -12,12 +12,15 @@
 #include <hip/hip_bf16.h>
 
 #include "common.h"
+#include "gas.h"
 #include "ocrt_amd.h"
 
 namespace ocrt {
 // kernels (setup.hip, sos.hip, ipss.hip)
 __global__ void medium_levels(const DevInst *, Pools);
 __global__ void medium_fractions(const DevInst *, Pools);
+__global__ void medium_gas(const DevInst *, Pools);
+__global__ void medium_transmit(const DevInst *, Pools);
 __global__ void fourier_weights(double *, int);
 template <bool DERIV> __global__ void coxmunk_fourier(const DevInst *, Pools);
 __global__ void angular_basis(const DevInst *, const DevItem *, Pools);
@@ -137,6 +140,7 @@ struct Chunk {
     IpssJob *djobs = nullptr;
     int *djob_of = nullptr;
     int max_L = 0, max_P = 0, max_pairs = 0, max_tab = 0, tab_groups = 1, maxit = 1, any_rough = 0, n_jobs = 0;
+    int any_gas = 0;
     long long max_pack = 0;
     long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0, cmp = 0, flo = 0;
 };
@@ -201,6 +205,8 @@ struct ocrt_plan {
     InstOut *ipss = nullptr;
     ocrt_result_t *res = nullptr;
     double *kappa = nullptr, *taudir = nullptr;
+    DevGas *gas_tab = nullptr;
+    double *gas_pool = nullptr;
     // chunk pools (sized for the largest chunk)
     Pools pools{};
     size_t dev_bytes = 0;
@@ -293,7 +299,7 @@ static int validate(const ocrt_instance_t &d)
             if (!(R[l] >= 0.0 && A[l] >= 0.0 && G[l] >= 0.0 && W[l] >= 0.0 && W[l] <= 1.0)) return OCRT_ERR_ARGS;
             if (!(R[l] + A[l] + G[l] > 0.0)) return OCRT_ERR_ARGS;
         }
-        if (d.ipss) return OCRT_ERR_ARGS;
+        if (d.ipss || d.gas) return OCRT_ERR_ARGS;   // absorption of a layering is its layers[3]
         if (aer && d.L_max < 2) return OCRT_ERR_ARGS;
     } else {
         if (aer && (d.ssa_a < 0.0 || d.ssa_a > 1.0 || d.L_max < 2)) return OCRT_ERR_ARGS;
@@ -329,6 +335,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     const int Palign = mixed ? 32 : 16;
     p->inst.resize(n_inst);
     long long ring_n = 0, greek_n = 0, layers_n = 0;
+    std::vector<const ocrt_gas *> gas_used;
     for (int i = 0; i < n_inst; ++i) {
         const ocrt_instance_t &d = in[i];
         HostInst &h = p->inst[i];
@@ -374,9 +381,17 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         I.off_layers = layers_n;
         layers_n += I.layered ? 4LL * I.nt : 0;
         I.gidx = i;
+        I.wl = d.wavelength_nm;
+        I.gas = -1;
+        if (d.gas) {
+            size_t gi = 0;
+            while (gi < gas_used.size() && gas_used[gi] != d.gas) ++gi;
+            if (gi == gas_used.size()) gas_used.push_back(d.gas);
+            I.gas = (int)gi;
+        }
         I.off_grad = p->grad_n;
         p->grad_n += 4LL * (I.nt + 1);
-        h.lev_n = 5LL * (I.nt + 1);
+        h.lev_n = 6LL * (I.nt + 1);
         h.T_n = (long long)I.nt * n;
         h.R_n = (I.surf == 2) ? (long long)(I.mmax + 1) * n * n * 9 + (long long)(I.mmax + 1) * n * 9 : 0;
         p->mmax_all = std::max(p->mmax_all, I.mmax);
@@ -473,6 +488,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             c.tab_groups = std::max(c.tab_groups, (n + 1 + kTabRows - 1) / kTabRows);
             c.maxit = std::max(c.maxit, I.max_orders);
             c.any_rough |= (I.surf == 2);
+            c.any_gas |= (I.gas >= 0);
             for (int m = 0; m <= I.mmax; ++m) {
                 DevItem it;
                 memset(&it, 0, sizeof(it));
@@ -539,6 +555,12 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     rc |= dmalloc(p, (void **)&p->ring, ring_n * 8);
     rc |= dmalloc(p, (void **)&p->greek, std::max(greek_n, 1LL) * 8);
     rc |= dmalloc(p, (void **)&p->layers, std::max(layers_n, 1LL) * 8);
+    // gas absorption tables used by the instances, packed once
+    std::vector<DevGas> hgas;
+    std::vector<double> hgas_pool;
+    for (const ocrt_gas *g : gas_used) hgas.push_back(pack_gas(*g, hgas_pool));
+    rc |= dmalloc(p, (void **)&p->gas_tab, std::max<size_t>(hgas.size(), 1) * sizeof(DevGas));
+    rc |= dmalloc(p, (void **)&p->gas_pool, std::max<size_t>(hgas_pool.size(), 1) * 8);
     rc |= dmalloc(p, (void **)&p->cm_w, (size_t)(p->mmax_all + 1) * 2 * kNPhi * 8);
     rc |= dmalloc(p, (void **)&p->state, (size_t)p->n_items * sizeof(ItemState));
     rc |= dmalloc(p, (void **)&p->iout, (size_t)p->n_items * sizeof(ItemOut));
@@ -582,6 +604,16 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     P.layers = p->layers;
     P.cm_w = p->cm_w;
     P.nodes = nullptr;
+    P.gas_tab = p->gas_tab;
+    P.gas_pool = p->gas_pool;
+    if ((!hgas.empty() && hipMemcpy(p->gas_tab, hgas.data(), hgas.size() * sizeof(DevGas), hipMemcpyHostToDevice) !=
+                              hipSuccess) ||
+        (!hgas_pool.size() ? false
+                           : hipMemcpy(p->gas_pool, hgas_pool.data(), hgas_pool.size() * 8, hipMemcpyHostToDevice) !=
+                                 hipSuccess)) {
+        ocrt_plan_destroy(p);
+        return OCRT_ERR_DEVICE;
+    }
     // ---- uploads of inputs and descriptors
     std::vector<double> hring(ring_n), hgreek(std::max(greek_n, 1LL)), hlayers(std::max(layers_n, 1LL));
     for (int ii = 0; ii < n_inst; ++ii) {
@@ -659,6 +691,8 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     ItemOut *io = p->iout + c.it0;
     hipLaunchKernelGGL(medium_levels, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
     hipLaunchKernelGGL(medium_fractions, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
+    if (c.any_gas) hipLaunchKernelGGL(medium_gas, dim3(ni), dim3(256), 0, s, c.dinst, P);
+    hipLaunchKernelGGL(medium_transmit, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
     if (c.any_rough) {
         if (p->adjoint)
             hipLaunchKernelGGL(coxmunk_fourier<true>, dim3(c.max_pairs, ni), dim3(256), 0, s, c.dinst, P);

@@ -8,8 +8,14 @@
 #include <cstring>
 #include <vector>
 
+#include <memory>
+
 #include "ocrt_amd.h"
 #include "ocrt_rt.h"
+
+// gas.hip: the solver's copy of a reference absorption state (nullptr when
+// the state is not initialised or malformed)
+ocrt_gas *ocrt_gas_from_state(const rt_absorption_t *st);
 
 namespace {
 
@@ -65,6 +71,11 @@ void say(const char *what)
 
 // One case of the reference API as an instance descriptor; the checks and
 // their order follow rt_solver.c:623-661.
+struct GasDeleter {
+    void operator()(ocrt_gas *g) const { ocrt_gas_destroy(g); }
+};
+using GasPtr = std::unique_ptr<ocrt_gas, GasDeleter>;
+
 struct Prep {
     ocrt_instance_t in{};
     std::vector<double> greek;
@@ -72,7 +83,9 @@ struct Prep {
     bool zero_out = false;   // refused after the reference has cleared its output
 };
 
-int prepare(const rt_case_t *cs, const rt_options_t *opts, const rt_aerosol_input_t *aer, Prep &p)
+// gas: the converted opts->abs_state (nullptr when there is none)
+int prepare(const rt_case_t *cs, const rt_options_t *opts, const rt_aerosol_input_t *aer, const ocrt_gas *gas,
+            Prep &p)
 {
     const double wl = cs->wavelength_nm;
     if (!(std::isfinite(wl) && wl >= 330.0 - 1e-9 && wl <= 1100.0 + 1e-9)) return OCRT_ERR_ARGS;
@@ -91,7 +104,7 @@ int prepare(const rt_case_t *cs, const rt_options_t *opts, const rt_aerosol_inpu
     if (cs->surface < RT_SURFACE_BLACK || cs->surface > RT_SURFACE_OCEAN) return OCRT_ERR_ARGS;
     if (opts->integration_method != RT_INTEGRATION_METHOD_LINEAR) { say("integration_method CONSTANT"); return OCRT_ERR_ARGS; }
     if (!opts->view_as_node) { say("view_as_node 0"); return OCRT_ERR_ARGS; }
-    if (opts->abs_state) { say("gas absorption (abs_state)"); return OCRT_ERR_ARGS; }
+    if (opts->abs_state && !gas) return OCRT_ERR_ARGS;
     if (opts->ext_bottom_per_m_I || opts->ext_bottom_per_m_Q || opts->ext_bottom_per_m_U ||
         opts->bottom_source_only || opts->sos.ext_bottom_I || opts->sos.bottom_source_only) {
         say("the ocean-coupling bottom source");
@@ -136,6 +149,7 @@ int prepare(const rt_case_t *cs, const rt_options_t *opts, const rt_aerosol_inpu
     d.tolerance = opts->sos.tolerance;
     d.ipss = opts->pssa ? 1 : 0;
     d.layers = nullptr;
+    d.gas = gas;
     if (aerosol) {
         const int Lm = aer->L_max;
         p.greek.resize(4 * (size_t)(Lm + 1));
@@ -200,8 +214,9 @@ int solve(const rt_case_t *cs, const rt_options_t *opts_in, const rt_aerosol_inp
     if (!cs || !out) return OCRT_ERR_ARGS;
     const rt_options_t defaults = rt_options_default();
     const rt_options_t *opts = opts_in ? opts_in : &defaults;
+    GasPtr gas(opts->abs_state ? ocrt_gas_from_state((const rt_absorption_t *)opts->abs_state) : nullptr);
     Prep p;
-    int rc = prepare(cs, opts, aer, p);
+    int rc = prepare(cs, opts, aer, gas.get(), p);
     if (rc && p.zero_out) std::memset(out, 0, sizeof(*out));
     if (rc) return rc;
     if (lut) {
@@ -300,8 +315,9 @@ extern "C" int ocrt_rt_solve_batch(const rt_case_t *cs, const rt_options_t *opts
     std::vector<Prep> prep(n);
     std::vector<int> rcs(n, 0), live;
     std::vector<ocrt_instance_t> inst;
+    GasPtr gas(opts->abs_state ? ocrt_gas_from_state((const rt_absorption_t *)opts->abs_state) : nullptr);
     for (int i = 0; i < n; ++i) {
-        rcs[i] = prepare(&cs[i], opts, aer ? &aer[i] : nullptr, prep[i]);
+        rcs[i] = prepare(&cs[i], opts, aer ? &aer[i] : nullptr, gas.get(), prep[i]);
         if (!rcs[i]) live.push_back(i);
         else if (prep[i].zero_out) std::memset(&out[i], 0, sizeof(rt_result_t));
     }

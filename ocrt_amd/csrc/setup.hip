@@ -73,8 +73,7 @@ __global__ void medium_levels(const DevInst *__restrict__ inst, Pools P)
     z[k] = zk;
 }
 
-// Scattering fractions of each level (aerosol case) and Beer-Lambert factors
-// T[layer][node] = exp(-dtau / mu_node) in fp64, 1 - T in fp32.
+// Scattering fractions of each level (aerosol case).
 __global__ void medium_fractions(const DevInst *__restrict__ inst, Pools P)
 {
     const DevInst &I = inst[blockIdx.y];
@@ -99,6 +98,129 @@ __global__ void medium_fractions(const DevInst *__restrict__ inst, Pools P)
             else { xd[k] = 0.0; yd[k] = (I.tauR > 0.0) ? 1.0 : 0.0; }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Gas absorption (rt_atm.c:437-548, rt_absorption.c:344-455).  One workgroup
+// per instance: every thread takes layers k (between levels k and k + 1,
+// altitudes z[k] > z[k + 1]) and sums over the six gases sigma(z_mid, wl)
+// times the gas column of the layer; then one thread adds the cumulative
+// absorption to h and ch and rescales the scattering fractions by
+// dtau_old / dtau_new, level by level in the reference's order.
+__device__ inline double gas_sigma_row(const DevGas &G, const double *pool, int g, int layer, double wl)
+{
+    const int n = G.n_wl[g];
+    if (!G.avail[g] || n < 2) return 0.0;
+    layer = layer < 0 ? 0 : (layer >= G.n_layers[g] ? G.n_layers[g] - 1 : layer);
+    const double *w = pool + G.off_wl[g], *s = pool + G.off_sig[g] + (long long)layer * n;
+    if (wl < w[0] || wl > w[n - 1]) return __builtin_nan("");
+    if (wl == w[0]) return s[0];
+    if (wl == w[n - 1]) return s[n - 1];
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (w[mid] <= wl) lo = mid; else hi = mid;
+    }
+    const double u = (wl - w[lo]) / (w[hi] - w[lo]);
+    return (1.0 - u) * s[lo] + u * s[hi];
+}
+
+__device__ inline double gas_sigma(const DevGas &G, const double *pool, int g, double z, double wl)
+{
+    if (!G.avail[g]) return 0.0;
+    const int K = G.n_layers[g];
+    if (K == 1) return gas_sigma_row(G, pool, g, 0, wl);
+    const double *zg = pool + G.off_z;
+    if (z <= zg[0]) return gas_sigma_row(G, pool, g, 0, wl);
+    if (z >= zg[K - 1]) return gas_sigma_row(G, pool, g, K - 1, wl);
+    int lo = 0, hi = K - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (zg[mid] <= z) lo = mid; else hi = mid;
+    }
+    const double dz = zg[hi] - zg[lo];
+    if (dz <= 0.0) return gas_sigma_row(G, pool, g, lo, wl);
+    const double u = (z - zg[lo]) / dz;
+    return (1.0 - u) * gas_sigma_row(G, pool, g, lo, wl) + u * gas_sigma_row(G, pool, g, hi, wl);
+}
+
+// Column of gas g above altitude z (molecules cm^-2): trapezoid of the
+// linearly interpolated number density over the AFGL levels.
+__device__ inline double gas_column_above(const DevGas &G, const double *pool, int g, double z)
+{
+    const double *zz = pool + G.off_z, *nn = pool + G.off_n + (long long)g * G.n_lev;
+    double cum = 0.0;
+    for (int i = 0; i + 1 < G.n_lev; ++i) {
+        const double z1 = zz[i], z2 = zz[i + 1];
+        if (z2 <= z) continue;
+        const double lo = z1 < z ? z : z1;
+        if (z2 <= lo) continue;
+        const double f1 = (lo - z1) / (z2 - z1), f2 = (z2 - z1) / (z2 - z1);
+        const double a = (1.0 - f1) * nn[i] + f1 * nn[i + 1];
+        const double b = (1.0 - f2) * nn[i] + f2 * nn[i + 1];
+        cum += 0.5 * (a + b) * ((z2 - lo) * 1.0e5);
+    }
+    return cum;
+}
+
+__global__ __launch_bounds__(256) void medium_gas(const DevInst *__restrict__ inst, Pools P)
+{
+    const DevInst &I = inst[blockIdx.x];
+    if (I.gas < 0 || I.layered) return;
+    const DevGas &G = P.gas_tab[I.gas];
+    const double *pool = P.gas_pool;
+    const int nt = I.nt, L = nt + 1;
+    double *h = P.lev + I.off_lev, *ch = h + L, *xd = ch + L, *yd = xd + L;
+    const double *z = yd + L;
+    double *dab = h + 5 * L;
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) {
+        double lo = z[k + 1], hi = z[k];
+        if (lo > hi) { const double t = lo; lo = hi; hi = t; }
+        const double zm = 0.5 * (lo + hi);
+        double acc = 0.0;
+        for (int g = 0; g < 6; ++g) {
+            double N;
+            if (G.overridden[g]) {
+                N = 0.0;
+                if (G.col_def[g] > 0.0) {
+                    const double Nd = hi > lo ? gas_column_above(G, pool, g, lo) - gas_column_above(G, pool, g, hi)
+                                              : 0.0;
+                    N = G.col_eff[g] * (Nd / G.col_def[g]);
+                }
+            } else {
+                N = hi > lo ? gas_column_above(G, pool, g, lo) - gas_column_above(G, pool, g, hi) : 0.0;
+            }
+            acc += gas_sigma(G, pool, g, zm, I.wl) * N;
+        }
+        dab[k] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double cum = 0.0, h_prev = h[0];
+    for (int k = 0; k <= nt; ++k) {
+        const double h_old = h[k];
+        h[k] = h_old + cum;
+        ch[k] = 0.5 * exp(-h[k] / I.mus);
+        if (k >= 1) {
+            const double dt_old = h_old - h_prev, dt_new = dt_old + dab[k - 1];
+            if (dt_new > 0.0 && dt_old > 0.0) {
+                xd[k] = xd[k] * dt_old / dt_new;
+                yd[k] = yd[k] * dt_old / dt_new;
+            }
+        }
+        h_prev = h_old;
+        if (k < nt) cum += dab[k];
+    }
+}
+
+// Beer-Lambert factors T[layer][node] = exp(-dtau / mu_node) in fp64, 1 - T
+// in fp32, and 1 / dtau in fp32.
+__global__ void medium_transmit(const DevInst *__restrict__ inst, Pools P)
+{
+    const DevInst &I = inst[blockIdx.y];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = I.nt;
+    const double *h = P.lev + I.off_lev;
     if (k < nt) {
         const double *mu = P.ring + I.off_ring;
         double *T = P.T + I.off_T + (long long)k * I.n;

@@ -8,8 +8,9 @@ and the layout check in tests/ all see one declaration:
     rc = rt_abi.library().rt_solve_case_pol(ctypes.byref(cs), ctypes.byref(opts), ctypes.byref(out))
 
 `rt_case_t`, `rt_options_t`, `rt_result_t`, `rt_aerosol_input_t`,
-`rt_lut_grid_out_t` and `rt_solver_sos_options_t` are built by `structs()`;
-`Case`, `Options`, `Result`, `Aerosol`, `LutGrid` are the usual aliases.
+`rt_lut_grid_out_t`, `rt_solver_sos_options_t` and the gas absorption state
+`rt_absorption_t` are built by `structs()`; `Case`, `Options`, `Result`,
+`Aerosol`, `LutGrid`, `Absorption` are the usual aliases.
 """
 import ctypes as C
 import os
@@ -35,7 +36,11 @@ def parse_structs(path=HEADER):
 
     A spec is ("scalar", base), ("pointer", base), ("array", base, n) or
     ("struct", typename)."""
-    text = _strip_comments(open(path).read())
+    raw = open(path).read()
+    macros = dict(re.findall(r"#define\s+(\w+)\s+(\d+)\s*$", raw, flags=re.M))
+    text = _strip_comments(raw)
+    for k, v in macros.items():
+        text = re.sub(r"\b%s\b" % k, v, text)
     out = {}
     for body, name in re.findall(r"typedef\s+struct\s*\{(.*?)\}\s*(\w+)\s*;", text, flags=re.S):
         fields = []
@@ -51,7 +56,9 @@ def parse_structs(path=HEADER):
                 ptr = item.startswith("*")
                 item = item.lstrip("*").strip()
                 arr = re.match(r"(\w+)\[(\d+)\]$", item)
-                if ptr:
+                if ptr and arr:
+                    fields.append((arr.group(1), ("pointer_array", base, int(arr.group(2)))))
+                elif ptr:
                     fields.append((item, ("pointer", base)))
                 elif arr:
                     fields.append((arr.group(1), ("array", base, int(arr.group(2)))))
@@ -79,7 +86,9 @@ def structs(path=HEADER):
             elif kind == "pointer":
                 t = _POINTERS[spec[1]]
             elif kind == "array":
-                t = _SCALARS[spec[1]] * spec[2]
+                t = (_SCALARS[spec[1]] if spec[1] in _SCALARS else build(spec[1])) * spec[2]
+            elif kind == "pointer_array":
+                t = _POINTERS[spec[1]] * spec[2]
             else:
                 t = build(spec[1])
             fields.append((fname, t))
@@ -99,6 +108,8 @@ Result = _S["rt_result_t"]
 Aerosol = _S["rt_aerosol_input_t"]
 LutGrid = _S["rt_lut_grid_out_t"]
 SosOptions = _S["rt_solver_sos_options_t"]
+Absorption = _S["rt_absorption_t"]
+GASES = ("h2o", "o3", "o2", "co2", "no2", "ch4")
 
 SURFACES = {"black": 0, "lambert": 1, "black_fresnel_ocean": 2, "flat": 3, "ocean": 4}
 
@@ -127,7 +138,26 @@ def bind(lib):
     lib.rt_solve_case_pol_lut.argtypes = [P(Case), P(Options), P(Aerosol), P(Result), P(LutGrid)]
     for f in (lib.rt_solve_case_pol, lib.rt_solve_case_pol_aerosol, lib.rt_solve_case_pol_lut):
         f.restype = C.c_int
+    lib.rt_absorption_init.argtypes = [C.c_char_p, C.c_char_p, C.c_int, P(C.c_double), P(Absorption)]
+    lib.rt_absorption_init.restype = C.c_int
+    lib.rt_absorption_free.argtypes = [P(Absorption)]
+    lib.rt_absorption_free.restype = None
+    lib.rt_absorption_tau_per_gas.argtypes = [P(Absorption), C.c_int, C.c_double]
+    lib.rt_absorption_tau_per_gas.restype = C.c_double
+    lib.rt_absorption_tau_total.argtypes = [P(Absorption), C.c_double]
+    lib.rt_absorption_tau_total.restype = C.c_double
     return lib
+
+
+def absorption(lib, afgl_dir, xsec_dir, profile=0, override=None):
+    """rt_absorption_init through `lib`; returns the state (free with
+    lib.rt_absorption_free)."""
+    st = Absorption()
+    ov = (C.c_double * 6)(*(override if override is not None else [-1.0] * 6))
+    rc = lib.rt_absorption_init(afgl_dir.encode(), xsec_dir.encode(), profile, ov, C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"rt_absorption_init failed with {rc}")
+    return st
 
 
 def library():

@@ -4,8 +4,9 @@ Builds the reference command-line program from its sources
 (`make -C oracle ref-cli` -> oracle/_ref/ocrt_ref), runs it from
 code/OCRT_C of the reference tree (its inputs/ are read relative to that
 directory) on each case below, and records exit status, stdout and the files
-it wrote in tests/golden/cli_cases.json.  Needs the reference tree, so it runs
-only in the container that has it:
+it wrote in tests/golden/cli_cases.json.  Gas absorption (always on in the
+CLI) reads tests/golden/gas_inputs, copied to ./inputs of each run.  Needs the
+reference tree, so it runs only in the container that has it:
 
     python tests/golden/make_cli_golden.py [/root/reference]
 """
@@ -19,6 +20,7 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 OUT = os.path.join(HERE, "cli_cases.json")
+GAS_INPUTS = os.path.join(HERE, "gas_inputs")
 
 NOGAS = ["--gas-column-h2o", "0", "--gas-column-o3", "0", "--gas-column-no2", "0", "--gas-column-o2", "0",
          "--gas-column-co2", "0", "--gas-column-ch4", "0"]
@@ -54,6 +56,14 @@ CASES = [
      ["out.csv"]),
     ("batch6_flat_advanced", ["--surface", "flat", "--batch", "in.csv", "--output", "out.csv", "--output-advanced"],
      {}, {"in.csv": BATCH6}, ["out.csv"]),
+    ("gas_single_black", ["--surface", "black", "--sza", "40", "--vza", "30", "--raa", "90", "--wavelength", "555"],
+     {}, {}, []),
+    ("gas_single_bfo_ipss", ["--surface", "black_fresnel_ocean", "--wind-speed", "6", "--pssa", "--sza", "55",
+                             "--vza", "35", "--raa", "120", "--wavelength", "443"], {}, {}, []),
+    ("gas_single_o3_pwv", ["--surface", "flat", "--gas-column-o3", "300", "--pwv-g-cm", "2.5", "--sza", "35",
+                           "--vza", "20", "--raa", "60", "--wavelength", "660"], {}, {}, []),
+    ("gas_lut_black", ["--surface", "black", "--sza", "50", "--wavelength", "865", "--lut-vza-step", "10",
+                       "--lut-raa-step", "30", "--output-full-grid", "lut_gas.csv"], {}, {}, ["lut_gas.csv"]),
     ("err_no_surface", ["--sza", "40", "--vza", "30", "--raa", "90", "--wavelength", "555"] + NOGAS, {}, {}, []),
     ("err_vza_only", ["--surface", "black", "--sza", "40", "--vza", "30", "--wavelength", "555"] + NOGAS, {}, {}, []),
     ("err_n_mu_gate", ["--surface", "black", "--n-mu", "16", "--sza", "40", "--wavelength", "555"] + NOGAS, {}, {},
@@ -77,9 +87,9 @@ def run_case(exe, ref_dir, args, env, inputs, outputs):
     try:
         for name, text in inputs.items():
             open(os.path.join(work, name), "w").write(text)
-        # relative input/output names resolve in the work dir; inputs/ of the
-        # reference through a link
-        os.symlink(os.path.join(ref_dir, "inputs"), os.path.join(work, "inputs"))
+        # relative input/output names resolve in the work dir; the gas data
+        # (inputs/afgl_atm, inputs/xsec) is the reduced test set
+        shutil.copytree(GAS_INPUTS, os.path.join(work, "inputs"))
         e = dict(os.environ, OMP_NUM_THREADS="1")
         e.pop("OCRT_ADVANCED", None)
         e.pop("OCRT_DEBUG", None)

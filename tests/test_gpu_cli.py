@@ -3,7 +3,8 @@
 tests/golden/cli_cases.json holds, per case, the arguments, environment,
 input files, exit status, stdout and written files of the reference CLI
 (built from its sources and run by tests/golden/make_cli_golden.py).  The same
-command is run here with ocrt_amd/ocrt on the GPU: exit statuses must match;
+command is run here with ocrt_amd/ocrt on the GPU, with the same gas data
+(tests/golden/gas_inputs as ./inputs): exit statuses must match;
 for successful runs every printed number must agree to 2e-10 relative (the
 outputs are printed with 11 significant digits) and every text token
 (keys, column names, nan) must be identical, in stdout and in every file.
@@ -11,6 +12,7 @@ outputs are printed with 11 significant digits) and every text token
 import json
 import math
 import os
+import shutil
 import subprocess
 
 import pytest
@@ -20,6 +22,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "ocrt_amd", "ocrt")
 CASES = json.load(open(os.path.join(ROOT, "tests", "golden", "cli_cases.json")))
+GAS_INPUTS = os.path.join(ROOT, "tests", "golden", "gas_inputs")
 
 
 def tokens(text):
@@ -64,6 +67,7 @@ def same_text(got, want, where):
 @pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
 def test_cli_matches_reference(case, tmp_path):
     assert os.path.exists(CLI), "build the CLI first (python -m ocrt_amd.build)"
+    shutil.copytree(GAS_INPUTS, tmp_path / "inputs")
     for name, text in case["inputs"].items():
         (tmp_path / name).write_text(text)
     env = dict(os.environ)

@@ -11,6 +11,7 @@ per-case calls.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 import pytest
@@ -21,6 +22,8 @@ from ocrt_amd import synthetic as S
 pytestmark = pytest.mark.gpu
 
 P = C.POINTER
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GAS_DIR = os.path.join(ROOT, "tests", "golden", "gas_inputs")
 
 
 @pytest.fixture(scope="module")
@@ -186,3 +189,67 @@ def test_batch_equals_single_calls(libs):
         assert rc1 == 0
         for f in SCALARS:
             assert getattr(outs[i], f) == getattr(one, f), (i, f)
+
+
+GAS_CASES = [
+    (dict(surface="black"), dict(), None, None),
+    (dict(surface="black_fresnel_ocean", wind=6.0, wl=443.0, sza=55.0, vza=35.0, raa=120.0), dict(pssa=1), None,
+     None),
+    (dict(surface="flat", wl=865.0), dict(m_max=6), dict(tau_a=0.12), None),
+    (dict(surface="black", wl=555.0, sza=30.0), dict(m_max=4), dict(tau_a=0.2), [-1.0, 450 * 2.6868e16, -1, -1, -1, -1]),
+    (dict(surface="black_fresnel_ocean", wind=2.0, wl=660.0), dict(), None, [0.0] * 6),
+]
+
+
+@pytest.mark.parametrize("k", range(len(GAS_CASES)))
+def test_gas_absorption_equals_reference(libs, k):
+    """Each library reads the same AFGL profile and cross sections with its own
+    rt_absorption_init; the solves agree as without absorption."""
+    mine, ref = libs
+    ckw, okw, akw, override = GAS_CASES[k]
+    print(f"[gpu] gas absorption case {k}: {ckw} {okw} {akw} override={override}", flush=True)
+    outs = []
+    for lib in (mine, ref):
+        st = rt_abi.absorption(lib, os.path.join(GAS_DIR, "afgl_atm"), os.path.join(GAS_DIR, "xsec"), 0, override)
+        cs, o = case(**ckw), options(**okw)
+        o.abs_state = C.addressof(st)
+        aer = Aer(**akw) if akw is not None else None
+        rc, out = run(lib, cs, o, aer)
+        lib.rt_absorption_free(C.byref(st))
+        assert rc == 0
+        outs.append(out)
+    compare(outs[0], outs[1], options(**okw).fourier_m_max)
+    # absorption is really there (or really off for zero columns)
+    _, plain = run(mine, case(**ckw), options(**okw), Aer(**akw) if akw is not None else None)
+    if override == [0.0] * 6:
+        assert outs[0].rho_I == plain.rho_I
+    else:
+        assert outs[0].T_dir_dn < plain.T_dir_dn
+
+
+def test_gas_lut_equals_reference(libs):
+    mine, ref = libs
+    vza = np.arange(0.0, 81.0, 10.0)
+    raa = np.arange(0.0, 360.0, 30.0)
+    nv, nr = len(vza), len(raa)
+    outs = []
+    for lib in (mine, ref):
+        st = rt_abi.absorption(lib, os.path.join(GAS_DIR, "afgl_atm"), os.path.join(GAS_DIR, "xsec"))
+        cs, o = case(surface="black_fresnel_ocean", wind=5.0, wl=555.0, sza=45.0), options(m_max=4, pssa=1)
+        o.abs_state = C.addressof(st)
+        bufs = {k: np.full(nv * nr, np.nan) for k in ("rho_I", "rho_Q", "rho_U", "T_diff_dn_dir")}
+        g = rt_abi.LutGrid()
+        g.n_vza, g.n_raa = nv, nr
+        g.vza_deg, g.raa_deg = vza.ctypes.data_as(P(C.c_double)), raa.ctypes.data_as(P(C.c_double))
+        for key, v in bufs.items():
+            setattr(g, key, v.ctypes.data_as(P(C.c_double)))
+        g.view_m_max = -1
+        out = rt_abi.Result()
+        assert lib.rt_solve_case_pol_lut(C.byref(cs), C.byref(o), None, C.byref(out), C.byref(g)) == 0
+        lib.rt_absorption_free(C.byref(st))
+        outs.append((bufs, g.T_dir_dn, g.T_diff_dn_hemi))
+    (a, ta, ha), (b, tb, hb) = outs
+    scale = np.max(np.abs(b["rho_I"]))
+    for key in a:
+        assert np.max(np.abs(a[key] - b[key])) <= 1e-11 * max(scale, np.max(np.abs(b[key]))), key
+    assert ta == pytest.approx(tb, rel=1e-12) and ha == pytest.approx(hb, rel=1e-11)
