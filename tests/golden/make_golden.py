@@ -107,6 +107,11 @@ def copy_reference_fixtures():
     src = os.path.join(REF_TREE, "tests", "fixtures", "ipss_oracle_geometry.csv")
     with open(src) as f, open(os.path.join(dst, "ipss_oracle_geometry.csv"), "w") as g:
         g.write(f.read())
+    # the non-IPSS gate of the reference (gas absorption on, whole file, gzip)
+    import gzip
+    src = os.path.join(REF_TREE, "tests", "fixtures", "ipss_nonpssa_reference.csv")
+    with open(src, "rb") as f, gzip.open(os.path.join(dst, "ipss_nonpssa_reference.csv.gz"), "wb", 9) as g:
+        g.write(f.read())
     # IPSS paper cases: sub-sample VZA {0,20,40,60,70} x RAA {0,45,90,135,180}
     keep_v = {0.0, 20.0, 40.0, 60.0, 70.0}
     keep_r = {0.0, 45.0, 90.0, 135.0, 180.0}

@@ -85,3 +85,38 @@ def test_cli_matches_reference(case, tmp_path):
     for name, want in case["files"].items():
         got = (tmp_path / name).read_text()
         same_text(got, want, name)
+
+
+def test_reference_gate_fixture_with_gas(tmp_path):
+    """The reference's own non-IPSS gate (tests/fixtures/ipss_nonpssa_reference.csv
+    of the reference, stored gzip-compressed; command of
+    scripts/run_ipss_gates.sh:30-33): Rayleigh with the default US76 gas
+    absorption over the rough ocean, SZA 75, 555 nm, full 5-degree LUT grid.
+    Every value within 1e-12 absolute (the reference gate's floor for numerical
+    zeros, run_ipss_gates.sh:49-88) or within the printing precision of the
+    CSV (%.10e: 11 significant digits, so 2e-10 relative covers a flipped last
+    digit)."""
+    import csv
+    import gzip
+    shutil.copytree(GAS_INPUTS, tmp_path / "inputs")
+    args = ["--sza", "75", "--wavelength", "555", "--pressure", "1013.25", "--surface", "black_fresnel_ocean",
+            "--wind-speed", "5", "--decouple-sunglint", "--lut-vza-step", "5", "--lut-vza-max", "85",
+            "--lut-raa-step", "5", "--output-full-grid", "gate.csv"]
+    print(f"[gpu] ocrt {' '.join(args)}", flush=True)
+    env = dict(os.environ)
+    env.pop("OCRT_ADVANCED", None)
+    p = subprocess.run([CLI] + args, cwd=tmp_path, env=env, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr[-2000:]
+    want = list(csv.reader(gzip.open(os.path.join(ROOT, "tests", "golden", "ref_fixtures",
+                                                  "ipss_nonpssa_reference.csv.gz"), "rt")))
+    got = list(csv.reader(open(tmp_path / "gate.csv")))
+    assert got[0] == want[0] and len(got) == len(want) == 1297
+    worst = 0.0
+    for ra, rb in zip(got[1:], want[1:]):
+        for a, b in zip(ra, rb):
+            fa, fb = float(a), float(b)
+            d = abs(fa - fb)
+            if d > 1e-12:
+                worst = max(worst, d / abs(fb))
+    print(f"[gpu] reference gate fixture: worst relative difference beyond 1e-12 absolute: {worst:.2e}", flush=True)
+    assert worst <= 2e-10
