@@ -118,14 +118,23 @@ def main():
     descs, keep = S.config2(n_geom=args.geoms)
     n = len(descs)
 
-    log(f"mixed plan over {n} instances")
-    pm = ocrt_amd.Plan(descs, precision="mixed")
+    log(f"mixed plan over {n} instances (sweep launch shape autotuned at the first execute)")
+    pm = ocrt_amd.Plan(descs, precision="mixed", autotune=True)
+    pm.execute()            # autotuning: one warm-up and one execute per sweep shape
+    sync()
+    tuning = pm.tuning()
+    log(f"autotune: {tuning}")
     pm.profile(True)
     ms_mixed = timed_steps(pm, args.steps, args.warmup, sync) * 1e3
     kt = pm.kernel_times()
     res_m = pm.results()
     work = pm.work()
     log(f"mixed: {ms_mixed:.1f} ms/step, kernels {kt}")
+    pg = ocrt_amd.Plan(descs, precision="mixed", graph=True)
+    ms_graph = timed_steps(pg, args.steps, 1, sync) * 1e3
+    same_bits = all(np.array_equal(pg.results()[f], res_m[f]) for f in ("rho_I", "rho_Q", "rho_U"))
+    pg.close()
+    log(f"mixed, hipGraph replay: {ms_graph:.1f} ms/step (results equal to eager: {same_bits})")
 
     if args.mixed_only:
         print(json.dumps({"mixed_ms_per_step": ms_mixed, "instances": n, "kernels": kt}))
@@ -244,6 +253,8 @@ def main():
         "roofline_other": rl_sweep if roofline is rl_gemm else rl_gemm,
         "cpu_baseline": cpu,
         "device_bytes": pm.device_bytes(),
+        "sweep_autotune": tuning,
+        "graph_replay_ms_per_step": ms_graph,
     }
     pm.close()
     print(json.dumps(line))

@@ -174,10 +174,20 @@ void ocrt_plan_destroy(ocrt_plan *plan);
 enum {
     OCRT_PLAN_ONE_PER_LAUNCH = 1,  /* instances one after another (the plain form) */
     OCRT_PLAN_ADJOINT = 2,         /* keep order checkpoints for ocrt_plan_adjoint (FP64, IPSS off) */
-    OCRT_PLAN_CKPT_SPARSE = 4      /* adjoint: checkpoint every 4th order and recompute the rest, whatever
+    OCRT_PLAN_CKPT_SPARSE = 4,     /* adjoint: checkpoint every 4th order and recompute the rest, whatever
                                       the memory allows (default: every order kept when the batch fits one
                                       chunk, else every 2nd, else every 4th) */
+    OCRT_PLAN_AUTOTUNE = 8,        /* mixed form: the first execute times every launch shape of the layer
+                                      sweep on this plan's work and keeps the fastest (ocrt_plan_tuning) */
+    OCRT_PLAN_GRAPH = 16           /* the first execute captures the whole forward launch sequence into a
+                                      hipGraph; later executes replay it (not with profiling or adjoint) */
 };
+
+/* Launch shape chosen for the mixed-form sweep (levels per block, wavefronts
+ * per workgroup) and, after autotuning, the summed sweep time of every shape
+ * tried (ms_per_shape[n_shapes], may be NULL). */
+int ocrt_plan_tuning(const ocrt_plan *plan, int *sweep_unroll, int *sweep_waves, double *ms_per_shape,
+                     int *n_shapes);
 
 /* ocrt_plan_create with option flags.  No counterpart in the reference; the
  * adjoint is the Jacobian path of the batched solver (ocrt_adjoint_batch). */

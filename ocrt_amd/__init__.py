@@ -20,6 +20,8 @@ PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
 PLAN_ONE_PER_LAUNCH = 1
 PLAN_ADJOINT = 2
 PLAN_CKPT_SPARSE = 4
+PLAN_AUTOTUNE = 8
+PLAN_GRAPH = 16
 
 
 class OcrtError(RuntimeError):
@@ -69,6 +71,7 @@ def library():
         lib.ocrt_gas_destroy.argtypes = [C.c_void_p]
         lib.ocrt_gas_destroy.restype = None
         lib.ocrt_plan_lut_extras.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ocrt_plan_tuning.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_plan_residuals.argtypes = [C.c_void_p, C.c_void_p]
         _lib = lib
     return _lib
@@ -119,7 +122,8 @@ class Plan:
     integral for orders >= 2).  one_per_launch=True runs instances one after
     another (the plain one-instance-per-launch form)."""
 
-    def __init__(self, descs, precision="mixed", one_per_launch=False, adjoint=False, ckpt_sparse=False):
+    def __init__(self, descs, precision="mixed", one_per_launch=False, adjoint=False, ckpt_sparse=False,
+                 autotune=False, graph=False):
         _require_gpu()
         self.lib = library()
         self.n = len(descs)
@@ -127,11 +131,20 @@ class Plan:
         self._handle = C.c_void_p()
         self.n_layers = [d.n_layers for d in descs]
         flags = ((PLAN_ONE_PER_LAUNCH if one_per_launch else 0) | (PLAN_ADJOINT if adjoint else 0) |
-                 (PLAN_CKPT_SPARSE if ckpt_sparse else 0))
+                 (PLAN_CKPT_SPARSE if ckpt_sparse else 0) | (PLAN_AUTOTUNE if autotune else 0) |
+                 (PLAN_GRAPH if graph else 0))
         rc = self.lib.ocrt_plan_create_ex(self._arr, self.n, PRECISIONS[precision], flags, C.byref(self._handle))
         if rc != 0:
             raise OcrtError(f"ocrt_plan_create failed with code {rc}")
         self.mmax = self.lib.ocrt_plan_mmax(self._handle)
+
+    def tuning(self):
+        """(levels per block, wavefronts per workgroup) of the mixed-form sweep
+        and, after autotuning, the summed sweep ms of every shape tried."""
+        u, w, n = C.c_int(), C.c_int(), C.c_int()
+        ms = (C.c_double * 16)()
+        self.lib.ocrt_plan_tuning(self._handle, C.byref(u), C.byref(w), ms, C.byref(n))
+        return {"unroll": u.value, "waves": w.value, "ms_per_shape": list(ms[: n.value])}
 
     def set_lut(self, vza_deg, raa_deg):
         """Sample every following execute on the grid vza x raa (full-grid / LUT mode)."""

@@ -33,7 +33,7 @@ template <typename FT> __global__ void order_status(const DevInst *, const DevIt
 template <int MODE>
 __global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 __global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
-template <typename FT, typename JT, bool MIXED>
+template <typename FT, typename JT, bool MIXED, int U, int W>
 __global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int, int);
 __global__ void extract(const DevInst *, const DevItem *, Pools, const ItemState *, ItemOut *, int);
 __global__ void finalize(const DevInst *, int, const ItemOut *, const InstOut *, const Pools, ocrt_result_t *);
@@ -214,6 +214,17 @@ struct ocrt_plan {
     double gemm_flops = 0.0, gemm_flops_alg = 0.0, sweep_bytes = 0.0, gemm_launches = 0.0;
     // optional per-kernel timing: events recorded around every GEMM and sweep launch
     int profile = 0;
+    // launch shape of the mixed-form sweep (kSweepShapes), optionally chosen
+    // by timing every shape on the plan's own work at the first execute
+    int sweep_variant = 0;
+    int autotune = 0, tuned = 0;
+    std::vector<double> tune_ms;
+    // replay of the captured forward launch sequence (hipGraph)
+    int use_graph = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t own_stream = nullptr;
+    double graph_gemm_launches = 0.0;
     std::vector<hipEvent_t> ev;       // pairs (start, stop): [gemm..., sweep...]
     std::vector<int> ev_kind;         // 0 gemm, 1 sweep, per pair
     size_t ev_used = 0;
@@ -318,7 +329,9 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
 {
     if (!in || n_inst < 1 || !out || (precision != OCRT_PREC_FP64 && precision != OCRT_PREC_MIXED))
         return OCRT_ERR_ARGS;
-    if (flags & ~(OCRT_PLAN_ONE_PER_LAUNCH | OCRT_PLAN_ADJOINT | OCRT_PLAN_CKPT_SPARSE)) return OCRT_ERR_ARGS;
+    if (flags & ~(OCRT_PLAN_ONE_PER_LAUNCH | OCRT_PLAN_ADJOINT | OCRT_PLAN_CKPT_SPARSE | OCRT_PLAN_AUTOTUNE |
+                  OCRT_PLAN_GRAPH))
+        return OCRT_ERR_ARGS;
     const bool adjoint = (flags & OCRT_PLAN_ADJOINT) != 0;
     // the adjoint runs the fp64 form and needs the plane-parallel solve (no IPSS)
     if (adjoint && precision != OCRT_PREC_FP64) return OCRT_ERR_ARGS;
@@ -330,6 +343,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     p->n_inst = n_inst;
     p->prec = precision;
     p->per_inst = (flags & OCRT_PLAN_ONE_PER_LAUNCH) ? 1 : 0;
+    p->autotune = (flags & OCRT_PLAN_AUTOTUNE) ? 1 : 0;
+    p->use_graph = (flags & OCRT_PLAN_GRAPH) && !adjoint ? 1 : 0;
     p->adjoint = adjoint ? 1 : 0;
     const bool mixed = precision == OCRT_PREC_MIXED;
     const int Palign = mixed ? 32 : 16;
@@ -682,6 +697,36 @@ extern "C" int ocrt_plan_create(const ocrt_instance_t *in, int n_inst, int preci
     return ocrt_plan_create_ex(in, n_inst, precision, one_per_launch ? OCRT_PLAN_ONE_PER_LAUNCH : 0, out);
 }
 
+static void drop_graph(ocrt_plan *p);
+
+// Launch shapes of the mixed-form sweep the autotuner chooses from: levels
+// per block U and wavefronts per workgroup W (all give the same bits).
+struct SweepShape { int U, W; };
+constexpr SweepShape kSweepShapes[] = {{8, 4}, {4, 4}, {16, 4}, {8, 2}, {4, 2}, {16, 2}};
+constexpr int kSweepShapeCount = sizeof(kSweepShapes) / sizeof(kSweepShapes[0]);
+
+static void launch_sweep_mixed(int v, int ncols, hipStream_t s, const DevInst *di, const DevItem *dt, Pools P,
+                               ItemState *st, const ColTile *tiles)
+{
+    const int W = kSweepShapes[v].W;
+    const dim3 grid((unsigned)((ncols + W - 1) / W)), block((unsigned)(64 * W));
+#define OCRT_SWEEP_CASE(I, U_, W_)                                                                              \
+    case I:                                                                                                      \
+        hipLaunchKernelGGL((sweep<__hip_bfloat16, float, true, U_, W_>), grid, block, 0, s, di, dt, P, st, tiles, \
+                           ncols, 0);                                                                            \
+        break;
+    switch (v) {
+        OCRT_SWEEP_CASE(0, 8, 4)
+        OCRT_SWEEP_CASE(1, 4, 4)
+        OCRT_SWEEP_CASE(2, 16, 4)
+        OCRT_SWEEP_CASE(3, 8, 2)
+        OCRT_SWEEP_CASE(4, 4, 2)
+        OCRT_SWEEP_CASE(5, 16, 2)
+        default: break;
+    }
+#undef OCRT_SWEEP_CASE
+}
+
 static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
 {
     const bool mixed = p->prec == OCRT_PREC_MIXED;
@@ -752,8 +797,7 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             hipLaunchKernelGGL(gemm_bf16, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st, 0);
             if (eg) (void)hipEventRecord(eg[1], s);
             if (es) (void)hipEventRecord(es[0], s);
-            hipLaunchKernelGGL((sweep<__hip_bfloat16, float, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st,
-                               c.dcols, ncols, 0);
+            launch_sweep_mixed(p->sweep_variant, ncols, s, c.dinst, c.ditem, P, st, c.dcols);
             if (es) (void)hipEventRecord(es[1], s);
         } else {
             hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
@@ -765,7 +809,7 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
                                0);
             if (eg) (void)hipEventRecord(eg[1], s);
             if (es) (void)hipEventRecord(es[0], s);
-            hipLaunchKernelGGL((sweep<double, double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st,
+            hipLaunchKernelGGL((sweep<double, double, false, 8, 4>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st,
                                c.dcols, ncols, 0);
             if (es) (void)hipEventRecord(es[1], s);
         }
@@ -821,7 +865,7 @@ static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
                                o);
             Pools Pout = P0;
             Pout.F = slot(sg, j);
-            hipLaunchKernelGGL((sweep<double, double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, Pout, st,
+            hipLaunchKernelGGL((sweep<double, double, false, 8, 4>), cols, dim3(256), 0, s, c.dinst, c.ditem, Pout, st,
                                c.dcols, ncols, o);
         }
         // backward steps
@@ -884,6 +928,7 @@ static int forward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
 extern "C" int ocrt_plan_set_lut(ocrt_plan *p, int n_vza, const double *vza_deg, int n_raa, const double *raa_deg)
 {
     if (!p || n_vza < 1 || n_raa < 1 || !vza_deg || !raa_deg || p->lut) return OCRT_ERR_ARGS;
+    drop_graph(p);   // the LUT epilogue changes the launch sequence
     for (int i = 0; i < n_vza; ++i)
         if (!(vza_deg[i] >= 0.0 && vza_deg[i] < 90.0)) return OCRT_ERR_ARGS;
     for (int i = 0; i < n_raa; ++i)
@@ -961,10 +1006,8 @@ extern "C" int ocrt_plan_residuals(ocrt_plan *p, double *residuals)
     return OCRT_OK;
 }
 
-extern "C" int ocrt_plan_execute(ocrt_plan *p, void *stream)
+static int execute_eager(ocrt_plan *p, hipStream_t s)
 {
-    if (!p) return OCRT_ERR_ARGS;
-    hipStream_t s = (hipStream_t)stream;
     p->gemm_launches = 0.0;
     p->ev_used = 0;
     for (Chunk &c : p->chunks) {
@@ -974,6 +1017,104 @@ extern "C" int ocrt_plan_execute(ocrt_plan *p, void *stream)
     // a single-chunk adjoint plan keeps the checkpoints of this forward pass
     p->fwd_ready = (p->adjoint && p->chunks.size() == 1) ? 1 : 0;
     return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
+}
+
+// Autotuner: one warm-up execute, then one execute per sweep shape with the
+// sweeps bracketed by events; the shape with the least summed sweep time is
+// kept.  Every shape computes the same bits, so the results of the plan do
+// not depend on the choice.
+static int autotune(ocrt_plan *p, hipStream_t s)
+{
+    const int saved = p->profile;
+    p->profile = 0;
+    p->sweep_variant = 0;
+    int rc = execute_eager(p, s);
+    p->profile = 1;
+    p->tune_ms.assign(kSweepShapeCount, 0.0);
+    int best = 0;
+    for (int v = 0; v < kSweepShapeCount && !rc; ++v) {
+        p->sweep_variant = v;
+        rc = execute_eager(p, s);
+        double sweep_ms = 0.0;
+        if (!rc) rc = ocrt_plan_kernel_times(p, nullptr, nullptr, &sweep_ms, nullptr);
+        p->tune_ms[v] = sweep_ms;
+        if (sweep_ms < p->tune_ms[best]) best = v;
+    }
+    p->profile = saved;
+    p->sweep_variant = best;
+    p->tuned = 1;
+    if (getenv("OCRT_DEBUG_PLAN"))
+        for (int v = 0; v < kSweepShapeCount; ++v)
+            fprintf(stderr, "ocrt autotune: sweep U %d W %d: %.3f ms%s\n", kSweepShapes[v].U, kSweepShapes[v].W,
+                    p->tune_ms[v], v == best ? " (chosen)" : "");
+    return rc;
+}
+
+// Capture the forward launch sequence of every chunk into one graph.
+static int capture(ocrt_plan *p)
+{
+    if (!p->own_stream) HIPCHK(hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking));
+    hipStream_t cs = p->own_stream;
+    const int saved = p->profile;
+    p->profile = 0;
+    p->gemm_launches = 0.0;
+    HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (Chunk &c : p->chunks)
+        if (!rc && forward_chunk(p, c, cs) != 0) rc = OCRT_ERR_DEVICE;
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &g);
+    p->profile = saved;
+    if (rc || ec != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return OCRT_ERR_DEVICE;
+    }
+    p->graph = g;
+    p->graph_gemm_launches = p->gemm_launches;
+    HIPCHK(hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0));
+    return OCRT_OK;
+}
+
+static void drop_graph(ocrt_plan *p)
+{
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
+}
+
+extern "C" int ocrt_plan_execute(ocrt_plan *p, void *stream)
+{
+    if (!p) return OCRT_ERR_ARGS;
+    hipStream_t s = (hipStream_t)stream;
+    if (p->autotune && !p->tuned && p->prec == OCRT_PREC_MIXED) {
+        const int rc = autotune(p, s);
+        if (rc) return rc;
+    }
+    if (p->use_graph && !p->profile) {
+        if (!p->graph_exec) {
+            const int rc = capture(p);
+            if (rc) return rc;
+        }
+        HIPCHK(hipGraphLaunch(p->graph_exec, s));
+        p->gemm_launches = p->graph_gemm_launches;
+        p->ev_used = 0;
+        p->fwd_ready = 0;
+        return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
+    }
+    return execute_eager(p, s);
+}
+
+extern "C" int ocrt_plan_tuning(const ocrt_plan *p, int *sweep_unroll, int *sweep_waves, double *ms_per_shape,
+                                int *n_shapes)
+{
+    if (!p) return OCRT_ERR_ARGS;
+    if (sweep_unroll) *sweep_unroll = kSweepShapes[p->sweep_variant].U;
+    if (sweep_waves) *sweep_waves = kSweepShapes[p->sweep_variant].W;
+    if (n_shapes) *n_shapes = p->tuned ? kSweepShapeCount : 0;
+    if (ms_per_shape && p->tuned)
+        for (int v = 0; v < kSweepShapeCount; ++v) ms_per_shape[v] = p->tune_ms[v];
+    return OCRT_OK;
 }
 
 extern "C" int ocrt_plan_adjoint(ocrt_plan *p, const double *grad_out, double *grad_levels, double *grad_wind,
@@ -1093,6 +1234,8 @@ extern "C" int ocrt_plan_kernel_times(ocrt_plan *p, double *gemm_ms, int *gemm_l
 extern "C" void ocrt_plan_destroy(ocrt_plan *p)
 {
     if (!p) return;
+    drop_graph(p);
+    if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
     for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
     for (void *a : p->allocs) (void)hipFree(a);
     delete p;

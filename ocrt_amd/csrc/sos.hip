@@ -397,9 +397,11 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
 // Formal solution of the current order for every column, fused with the order
 // sum total += cur and the two max-norms of the convergence test
 // (rt_solver.c:1855-1868, 3093-3124).  One thread per column, one 64-column
-// tile per wavefront; the levels are walked in blocks of kU, every load of a
-// block issued before its dependent recurrence steps, so the sequential scan
-// overlaps its memory latency instead of exposing it once per level.
+// tile per wavefront, W wavefronts per workgroup; the levels are walked in
+// blocks of U, every load of a block issued before its dependent recurrence
+// steps, so the sequential scan overlaps its memory latency instead of
+// exposing it once per level.  U and W do not change the arithmetic (every
+// variant gives the same bits); the plan's autotuner picks them.
 //
 // The order sum is fp64 over the whole field (it feeds the convergence test).
 // In the mixed form the two rows that become outputs, k = 0 (TOA) and k = nt
@@ -407,18 +409,13 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
 __device__ inline int clamp_level(int k, int nt) { return k < 0 ? 0 : (k > nt ? nt : k); }
 __device__ inline int clamp_layer(int l, int nt) { return l < 0 ? 0 : (l > nt - 1 ? nt - 1 : l); }
 
-#ifndef OCRT_SWEEP_UNROLL
-#define OCRT_SWEEP_UNROLL 8
-#endif
-constexpr int kU = OCRT_SWEEP_UNROLL;
-
-template <typename FT, typename JT, bool MIXED>
-__global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
-                                             const DevItem *__restrict__ items, Pools P,
-                                             ItemState *__restrict__ st, const ColTile *__restrict__ tiles,
-                                             int n_tiles, int order)
+template <typename FT, typename JT, bool MIXED, int U, int W>
+__global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst,
+                                                const DevItem *__restrict__ items, Pools P,
+                                                ItemState *__restrict__ st, const ColTile *__restrict__ tiles,
+                                                int n_tiles, int order)
 {
-    const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int tix = blockIdx.x * W + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (tix >= n_tiles) return;
     const ColTile ct = tiles[tix];
     // the tile is wavefront-uniform: keep its item in scalar registers
@@ -475,10 +472,10 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
             double run = hemi ? 0.0 : bc;
             double s_in = (double)J[(long long)k_first * Pp];
             const double smu = hemi ? -mu : mu;
-            for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * kU) {
-                double sj[kU], tr[kU], to[kU];
+            for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * U) {
+                double sj[U], tr[U], to[U];
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
+                for (int u = 0; u < U; ++u) {
                     // clamped indices keep the block's loads unconditional, so
                     // they are all in flight before the first one is used
                     const int k = clamp_level(k0 + step * u, nt);
@@ -487,7 +484,7 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                     to[u] = tot[(long long)k * Pp];
                 }
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
+                for (int u = 0; u < U; ++u) {
                     const int k = k0 + step * u;
                     if (k < 0 || k > nt) break;
                     const int ku = k - step;                // upstream level
@@ -509,11 +506,11 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
             const float smu = hemi ? -(float)mu : (float)mu;
             float run = hemi ? 0.f : (float)bc;
             float s_in = (float)J[(long long)k_first * Pp];
-            for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * kU) {
-                float sj[kU], om[kU], idt[kU];
-                double to[kU];
+            for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * U) {
+                float sj[U], om[U], idt[U];
+                double to[U];
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
+                for (int u = 0; u < U; ++u) {
                     // clamped indices keep the block's loads unconditional, so
                     // they are all in flight before the first one is used
                     const int k = clamp_level(k0 + step * u, nt);
@@ -524,7 +521,7 @@ __global__ __launch_bounds__(256) void sweep(const DevInst *__restrict__ inst,
                     to[u] = tot[(long long)k * Pp];
                 }
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
+                for (int u = 0; u < U; ++u) {
                     const int k = k0 + step * u;
                     if (k < 0 || k > nt) break;
                     const float s_out = sj[u];
@@ -699,10 +696,17 @@ template __global__ void order_one<__hip_bfloat16, true>(const DevInst *, const 
                                                          const ColTile *, int);
 template __global__ void order_status<double>(const DevInst *, const DevItem *, Pools, ItemState *, int);
 template __global__ void order_status<__hip_bfloat16>(const DevInst *, const DevItem *, Pools, ItemState *, int);
-template __global__ void sweep<double, double, false>(const DevInst *, const DevItem *, Pools, ItemState *,
-                                                      const ColTile *, int, int);
-template __global__ void sweep<__hip_bfloat16, float, true>(const DevInst *, const DevItem *, Pools, ItemState *,
-                                                            const ColTile *, int, int);
+#define OCRT_SWEEP_INST(FT, JT, MX, U, W)                                                                   \
+    template __global__ void sweep<FT, JT, MX, U, W>(const DevInst *, const DevItem *, Pools, ItemState *,    \
+                                                     const ColTile *, int, int);
+OCRT_SWEEP_INST(double, double, false, 8, 4)
+OCRT_SWEEP_INST(__hip_bfloat16, float, true, 4, 2)
+OCRT_SWEEP_INST(__hip_bfloat16, float, true, 4, 4)
+OCRT_SWEEP_INST(__hip_bfloat16, float, true, 8, 2)
+OCRT_SWEEP_INST(__hip_bfloat16, float, true, 8, 4)
+OCRT_SWEEP_INST(__hip_bfloat16, float, true, 16, 2)
+OCRT_SWEEP_INST(__hip_bfloat16, float, true, 16, 4)
+#undef OCRT_SWEEP_INST
 template __global__ void gemm_f64<kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                             const ItemState *, int);
 template __global__ void gemm_f64<kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
