@@ -134,6 +134,10 @@ __host__ inline unsigned xcd_item_grid(int items, int X)
 constexpr int kTabRows = 8;
 
 struct GemmTile { int item, tm, tn, pad; };   // item < 0: idle slot
+// Level width of a scattering-integral tile: bf16 form (gemm_bf16) and fp64
+// form (gemm_f64); the row width is 64 in both.
+constexpr int kGemmLevels = 128;
+constexpr int kGemmLevelsF64 = 64;
 
 // Epilogues of the scattering-integral kernels.
 constexpr int kGemmMix = 0;     // J = xdel A_p F + ydel A_r F

@@ -514,7 +514,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 if (np == 0) kinds[np++] = 1;
                 it.parts = kinds[0] | (np > 1 ? (kinds[1] << 8) : 0);
                 it.K = np * Pp;
-                it.Mp = Mp; it.tiles_m = Mp / 64; it.tiles_n = (L + 63) / 64;
+                const int tile_lev = mixed ? kGemmLevels : kGemmLevelsF64;
+                it.Mp = Mp; it.tiles_m = Mp / 64; it.tiles_n = (L + tile_lev - 1) / tile_lev;
                 it.lmax = I.lmax;
                 it.off_basis = basis; basis += 3LL * (I.lmax + 1) * (2 * n + 1);
                 it.off_tab = tab; tab += 6LL * (n + 1) * (2 * n + 1);

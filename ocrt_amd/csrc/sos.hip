@@ -275,13 +275,24 @@ __global__ __launch_bounds__(256) void gemm_f64(const DevInst *__restrict__ inst
         }
 }
 
-// Scattering integral on bf16 MFMA (v_mfma_f32_16x16x16bf16_1k) with fp32
-// accumulation.  Tile 64 (outgoing columns) x 64 (levels), K step 32, four
-// waves of 32 x 32.  The operator of each part (particle, molecular) is
-// stored as a bf16 pair A = A_hi + A_lo, [part][hi, lo][Mp][P]; both halves
-// multiply the same bf16 field fragment.  The level scattering fractions are
-// applied to the fp32 accumulators in the epilogue:
+// Scattering integral on bf16 MFMA (v_mfma_f32_16x16x32_bf16, gfx950) with
+// fp32 accumulation.  Tile 64 (outgoing columns) x 128 (levels), K step 32:
+// the operator tile is read once per 128 levels, which halves its share of
+// the L2 traffic against a 64 x 64 tile.  Four waves of 32 x 64, each holding
+// 2 x 4 accumulator blocks of 16 x 16 per operator part.  The operator of
+// each part (particle, molecular) is stored as a bf16 pair A = A_hi + A_lo,
+// [part][hi, lo][Mp][P]; both halves multiply the same bf16 field fragment
+// (and, for the low Fourier modes, the hi half also its bf16 residual).  The
+// level scattering fractions are applied to the fp32 accumulators in the
+// epilogue:
 //   J[lev][r] = xdel(lev) (A_p F)[r][lev] + ydel(lev) (A_r F)[r][lev].
+typedef __bf16 bf16x8_ __attribute__((ext_vector_type(8)));
+
+__device__ inline bf16x8_ lds_frag(const unsigned short *p)
+{
+    return __builtin_bit_cast(bf16x8_, *reinterpret_cast<const uint4 *>(p));
+}
+
 __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ inst,
                                                  const DevItem *__restrict__ items, Pools P,
                                                  const GemmTile *__restrict__ tiles,
@@ -299,22 +310,24 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
     const unsigned short *Flo = reinterpret_cast<const unsigned short *>(P.Flo) + it.off_Flo;
     float *J = reinterpret_cast<float *>(P.J) + it.off_F;
     const LevelView lv(P, I);
-    __shared__ unsigned short As[2][2][64][40];
-    __shared__ unsigned short Bs[2][64][40];
+    __shared__ unsigned short As[2][2][64][40];   // [part][hi, lo][row][k], rows padded to 40
+    __shared__ unsigned short Bs[2][kGemmLevels][40];   // [hi, lo][level][k]
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int row0 = tl.tm * 64, lev0 = tl.tn * 64;
-    float4_ acc[2][2][2];
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+    const int row0 = tl.tm * 64, lev0 = tl.tn * kGemmLevels;
+    float4_ acc[2][2][4];
+#pragma unroll
     for (int q = 0; q < 2; ++q)
+#pragma unroll
         for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b) acc[q][a][b] = (float4_){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[q][a][b] = (float4_){0.f, 0.f, 0.f, 0.f};
+    // global -> LDS staging: one 8-wide k chunk of one operator row per thread
+    // and part, two of the field (levels lr and lr + 64)
     const int lr = t >> 2, lk = (t & 3) * 8;
-    const int lev_load = lev0 + lr;
     const long long slab = (long long)Mp * Pp;
-    // K slabs of 32 go global -> registers -> LDS; the next slab's loads are
-    // issued before the current slab's MFMAs so their latency overlaps them
-    uint4 ra0h, ra0l, ra1h, ra1l, rb, rbl;
     const bool two = nparts > 1;
+    uint4 ra0h, ra0l, ra1h, ra1l, rb0, rb1, rl0, rl1;
     auto load_slab = [&](int k0) {
         const long long ro = (long long)(row0 + lr) * Pp + k0 + lk;
         ra0h = *reinterpret_cast<const uint4 *>(A + ro);
@@ -323,11 +336,15 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
             ra1h = *reinterpret_cast<const uint4 *>(A + 2 * slab + ro);
             ra1l = *reinterpret_cast<const uint4 *>(A + 3 * slab + ro);
         }
-        rb = make_uint4(0u, 0u, 0u, 0u);
-        rbl = make_uint4(0u, 0u, 0u, 0u);
-        if (lev_load < L) {
-            rb = *reinterpret_cast<const uint4 *>(F + (long long)lev_load * Pp + k0 + lk);
-            if (split) rbl = *reinterpret_cast<const uint4 *>(Flo + (long long)lev_load * Pp + k0 + lk);
+        const int l0 = lev0 + lr, l1 = l0 + 64;
+        rb0 = rb1 = rl0 = rl1 = make_uint4(0u, 0u, 0u, 0u);
+        if (l0 < L) {
+            rb0 = *reinterpret_cast<const uint4 *>(F + (long long)l0 * Pp + k0 + lk);
+            if (split) rl0 = *reinterpret_cast<const uint4 *>(Flo + (long long)l0 * Pp + k0 + lk);
+        }
+        if (l1 < L) {
+            rb1 = *reinterpret_cast<const uint4 *>(F + (long long)l1 * Pp + k0 + lk);
+            if (split) rl1 = *reinterpret_cast<const uint4 *>(Flo + (long long)l1 * Pp + k0 + lk);
         }
     };
     auto store_slab = [&]() {
@@ -337,55 +354,65 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
             *reinterpret_cast<uint4 *>(&As[1][0][lr][lk]) = ra1h;
             *reinterpret_cast<uint4 *>(&As[1][1][lr][lk]) = ra1l;
         }
-        *reinterpret_cast<uint4 *>(&Bs[0][lr][lk]) = rb;
-        if (split) *reinterpret_cast<uint4 *>(&Bs[1][lr][lk]) = rbl;
+        *reinterpret_cast<uint4 *>(&Bs[0][lr][lk]) = rb0;
+        *reinterpret_cast<uint4 *>(&Bs[0][lr + 64][lk]) = rb1;
+        if (split) {
+            *reinterpret_cast<uint4 *>(&Bs[1][lr][lk]) = rl0;
+            *reinterpret_cast<uint4 *>(&Bs[1][lr + 64][lk]) = rl1;
+        }
     };
+    // fragment of a 16 x 32 block: lane l holds row (l & 15), k 8 (l >> 4) .. + 7
+    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    auto mfma_part = [&](auto part) {
+        constexpr int q = decltype(part)::value;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const bf16x8_ hi = lds_frag(&As[q][0][wm + a * 16 + fr][fk]);
+            const bf16x8_ lo = lds_frag(&As[q][1][wm + a * 16 + fr][fk]);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const bf16x8_ fb = lds_frag(&Bs[0][wn + b * 16 + fr][fk]);
+                if (split) {
+                    const bf16x8_ fl = lds_frag(&Bs[1][wn + b * 16 + fr][fk]);
+                    acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, fl, acc[q][a][b], 0, 0, 0);
+                }
+                acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, fb, acc[q][a][b], 0, 0, 0);
+                acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, fb, acc[q][a][b], 0, 0, 0);
+            }
+        }
+    };
+    // K slabs of 32 go global -> registers -> LDS; the next slab's loads are
+    // issued before the current slab's MFMAs so their latency overlaps them
     load_slab(0);
     store_slab();
     __syncthreads();
     for (int k0 = 0; k0 < Pp; k0 += 32) {
         const bool more = k0 + 32 < Pp;
         if (more) load_slab(k0 + 32);
-#pragma unroll
-        for (int ks = 0; ks < 32; ks += 16) {
-            short4_ b4[2], b4l[2];
-            for (int b = 0; b < 2; ++b) {
-                b4[b] = *reinterpret_cast<const short4_ *>(&Bs[0][wn + b * 16 + (lane & 15)][ks + 4 * (lane >> 4)]);
-                if (split)
-                    b4l[b] = *reinterpret_cast<const short4_ *>(&Bs[1][wn + b * 16 + (lane & 15)][ks + 4 * (lane >> 4)]);
-            }
-            for (int q = 0; q < nparts; ++q)
-                for (int a = 0; a < 2; ++a) {
-                    const int row = wm + a * 16 + (lane & 15);
-                    const short4_ hi = *reinterpret_cast<const short4_ *>(&As[q][0][row][ks + 4 * (lane >> 4)]);
-                    const short4_ lo = *reinterpret_cast<const short4_ *>(&As[q][1][row][ks + 4 * (lane >> 4)]);
-                    for (int b = 0; b < 2; ++b) {
-                        if (split)
-                            acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(hi, b4l[b], acc[q][a][b], 0, 0, 0);
-                        acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lo, b4[b], acc[q][a][b], 0, 0, 0);
-                        acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(hi, b4[b], acc[q][a][b], 0, 0, 0);
-                    }
-                }
-        }
+        mfma_part(std::integral_constant<int, 0>{});
+        if (two) mfma_part(std::integral_constant<int, 1>{});
         __syncthreads();
         if (more) {
             store_slab();
             __syncthreads();
         }
     }
+    // 16 x 16 result layout: lane holds rows 4 (lane >> 4) + i of column lane & 15
     const int kind0 = it.parts & 0xff;
-    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
         const int lev = lev0 + wn + b * 16 + (lane & 15);
         if (lev >= L) continue;
         const float x = (float)lv.xd[lev], y = (float)lv.yd[lev];
         const float s0 = (kind0 == 1) ? x : y;
+#pragma unroll
         for (int a = 0; a < 2; ++a) {
             const int r = row0 + wm + a * 16 + 4 * (lane >> 4);
             if (r >= Pp) continue;
             float4 v;
             v.x = s0 * acc[0][a][b][0]; v.y = s0 * acc[0][a][b][1];
             v.z = s0 * acc[0][a][b][2]; v.w = s0 * acc[0][a][b][3];
-            if (nparts > 1) {
+            if (two) {
                 v.x += y * acc[1][a][b][0]; v.y += y * acc[1][a][b][1];
                 v.z += y * acc[1][a][b][2]; v.w += y * acc[1][a][b][3];
             }
