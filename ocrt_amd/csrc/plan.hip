@@ -438,8 +438,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 const int parts = (I.aer ? 1 : 0) + ((m <= 2 && I.ray) ? 1 : 0);
                 const int K = std::max(parts, 1) * P;
                 b += 3.0 * (I.lmax + 1) * (2 * n + 1) * 8 + 6.0 * (n + 1) * (2 * n + 1) * 8 +
-                     (double)Mp * K * ae * (mixed ? 2 : 1) + (double)L * P * (fe + je + 8) +
-                     (mixed ? 8.0 * P : 0.0) + ((mixed && m < kSplitModes) ? 2.0 * L * P : 0.0) + 6.0 * n * 8;
+                     (double)Mp * K * ae * (mixed ? 2 : 1) + (double)L * P * (fe + je + (mixed ? 4 : 8)) +
+                     (mixed ? 24.0 * P : 0.0) + ((mixed && m < kSplitModes) ? 2.0 * L * P : 0.0) + 6.0 * n * 8;
                 if (adjoint)   // transposed operator, checkpoints, segment fields, lambda, second parts
                     b += (double)Mp * K * ae +
                          (double)L * P * 8 * ((I.max_orders - 1) / every + 1 + (every - 1) + 2);
@@ -594,8 +594,13 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     rc |= dmalloc(p, (void **)&P.A, mx_A * ae);
     rc |= dmalloc(p, (void **)&P.F, mx_F * fe);
     rc |= dmalloc(p, (void **)&P.J, mx_F * je);
-    rc |= dmalloc(p, (void **)&P.tot, mx_F * 8);
-    if (mixed) rc |= dmalloc(p, (void **)&P.comp, std::max(mx_cmp, 1LL) * 4);
+    if (mixed) {
+        rc |= dmalloc(p, (void **)&P.tot32, mx_F * 4);
+        rc |= dmalloc(p, (void **)&P.comp, std::max(mx_cmp, 1LL) * 4);
+        rc |= dmalloc(p, (void **)&P.tend, std::max(mx_cmp, 1LL) * 8);
+    } else {
+        rc |= dmalloc(p, (void **)&P.tot, mx_F * 8);
+    }
     if (mixed) rc |= dmalloc(p, (void **)&P.Flo, std::max(mx_flo, 1LL) * 2);
     rc |= dmalloc(p, (void **)&P.bc, mx_bc * 8);
     if (adjoint) {

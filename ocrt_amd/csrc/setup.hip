@@ -573,56 +573,82 @@ __device__ inline double pair_molecular(const DevInst &I, int m, const double *p
 // (scaled by ydel) block.
 // TRANS packs the transposed operator (A_part^T of every part) in the same
 // layout, for the adjoint's M^T products.
+//
+// Each wavefront takes whole rows (wave-uniform row index, so the row's
+// selectors sit in scalar registers); its lanes take CW adjacent columns each
+// (CW = 2 for bf16: one 32-bit store per half), with the column selectors
+// decoded once per lane and pass.  The bf16 pair A = hi + lo is formed in
+// registers: lo = bf16(v - hi).
+__device__ inline double operator_entry(const DevInst &I, const DevItem &it, const Pools &P, int kind, int ro,
+                                        int ci)
+{
+    const int n = it.n, dirs = 2 * n + 1;
+    const int to = col_sel_type(ro, n), o = ro - (ro / n) * n + 1;
+    const int ti = col_sel_type(ci, n), i = ci - (ci / n) * n + 1;
+    const double w = P.ring[I.off_ring + n + (i - 1)];
+    if (w == 0.0) return 0.0;
+    const int q = c_sel_q[to][ti];
+    const int s = c_sel_s[to][ti];
+    double e;
+    if (kind == 1) {
+        e = pair_particle(P.tab + it.off_tab, (long long)(n + 1) * dirs, dirs, n, q, o, i);
+    } else {
+        const double *plm = P.basis + it.off_basis;
+        const double *plm2 = plm + 2LL * dirs;
+        const double *rrl2 = plm + (long long)(it.lmax + 1) * dirs + 2LL * dirs;
+        const double *rtl2 = plm + 2LL * (it.lmax + 1) * dirs + 2LL * dirs;
+        e = pair_molecular(I, it.m, plm2, rrl2, rtl2, n, q, o, i);
+    }
+    return w * (double)s * e;
+}
+
 template <typename AT, bool TRANS>
-__global__ void pack_operator(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P,
-                              int n_items, int blocks_per_item)
+__global__ __launch_bounds__(256) void pack_operator(const DevInst *__restrict__ inst,
+                                                     const DevItem *__restrict__ items, Pools P, int n_items,
+                                                     int blocks_per_item)
 {
     int item, bx;
     xcd_item_block(blocks_per_item, item, bx);   // an item's tables stay in one XCD's L2
     if (item >= n_items) return;
     const DevItem &it = items[item];
     const DevInst &I = inst[it.inst];
-    const long long total = (long long)it.Mp * it.K;
-    const int n = it.n, dirs = 2 * n + 1, six_n = 6 * n;
-    AT *A = reinterpret_cast<AT *>(P.A) + it.off_A;
-    // the item's blocks stride over its entries
-    for (long long idx = (long long)bx * blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)blocks_per_item * blockDim.x) {
-        const int r = (int)(idx / it.K), c = (int)(idx % it.K);
-        const int part = c / it.P, cc = c % it.P;
-        // fp64: A[Mp][K], parts side by side along K.  bf16: [part][hi, lo][Mp][P].
-        const long long dst = (sizeof(AT) == 2) ? ((long long)(part * 2) * it.Mp + r) * it.P + cc : idx;
-        const long long lo_off = (long long)it.Mp * it.P;
-        double v = 0.0;
-        if (r < six_n && cc < six_n) {
-            const int ro = TRANS ? cc : r, ci = TRANS ? r : cc;   // (outgoing, incoming) state columns
-            const int to = col_sel_type(ro, n), o = ro % n + 1;
-            const int ti = col_sel_type(ci, n), i = ci % n + 1;
-            const double w = P.ring[I.off_ring + n + (i - 1)];
-            if (w != 0.0) {
-                const int q = c_sel_q[to][ti];
-                const int s = c_sel_s[to][ti];
-                const int kind = (it.parts >> (8 * part)) & 0xff;   // 1 particle, 2 molecular
-                double e;
-                if (kind == 1) {
-                    const long long cnt = (long long)(n + 1) * dirs;
-                    e = pair_particle(P.tab + it.off_tab, cnt, dirs, n, q, o, i);
-                } else {
-                    const double *plm = P.basis + it.off_basis;
-                    const double *plm2 = plm + 2LL * dirs;
-                    const double *rrl2 = plm + (long long)(it.lmax + 1) * dirs + 2LL * dirs;
-                    const double *rtl2 = plm + 2LL * (it.lmax + 1) * dirs + 2LL * dirs;
-                    e = pair_molecular(I, it.m, plm2, rrl2, rtl2, n, q, o, i);
+    constexpr bool BF = sizeof(AT) == 2;
+    constexpr int CW = BF ? 2 : 1;
+    const int six_n = 6 * it.n, Pp = it.P, Mp = it.Mp, K = it.K;
+    const int nparts = K / Pp;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int waves = blocks_per_item * (blockDim.x >> 6);
+    for (int r = __builtin_amdgcn_readfirstlane(bx * (int)(blockDim.x >> 6) + wave); r < Mp; r += waves) {
+        for (int part = 0; part < nparts; ++part) {
+            const int kind = (it.parts >> (8 * part)) & 0xff;   // 1 particle, 2 molecular
+            for (int cc = CW * lane; cc < Pp; cc += CW * 64) {
+                double v[CW];
+#pragma unroll
+                for (int e = 0; e < CW; ++e) {
+                    const int c = cc + e;
+                    v[e] = (r < six_n && c < six_n) ? operator_entry(I, it, P, kind, TRANS ? c : r, TRANS ? r : c)
+                                                    : 0.0;
                 }
-                v = w * (double)s * e;
+                if constexpr (BF) {
+                    // fp32 rounding first, as every bf16 conversion of the solver
+                    unsigned short hi[CW], lo[CW];
+#pragma unroll
+                    for (int e = 0; e < CW; ++e) {
+                        const __hip_bfloat16 h = __float2bfloat16((float)v[e]);
+                        const __hip_bfloat16 l = __float2bfloat16((float)(v[e] - (double)__bfloat162float(h)));
+                        hi[e] = __builtin_bit_cast(unsigned short, h);
+                        lo[e] = __builtin_bit_cast(unsigned short, l);
+                    }
+                    // [part][hi, lo][Mp][P]
+                    unsigned *dst = reinterpret_cast<unsigned *>(reinterpret_cast<unsigned short *>(P.A) + it.off_A +
+                                                                 ((long long)(part * 2) * Mp + r) * Pp + cc);
+                    dst[0] = (unsigned)hi[0] | ((unsigned)hi[1] << 16);
+                    dst[(long long)Mp * Pp / 2] = (unsigned)lo[0] | ((unsigned)lo[1] << 16);
+                } else {
+                    // [Mp][K], parts side by side along K
+                    reinterpret_cast<double *>(P.A)[it.off_A + (long long)r * K + part * Pp + cc] = v[0];
+                }
             }
-        }
-        A[dst] = to_storage<AT>(v);
-        if (sizeof(AT) == 2) {
-            // bf16 operator carried as hi + lo: the rounding of a fixed operator is
-            // systematic across orders, the residual term removes it to ~2^-16
-            const double hi = (double)__bfloat162float(*reinterpret_cast<const __hip_bfloat16 *>(&A[dst]));
-            A[dst + lo_off] = to_storage<AT>(v - hi);
         }
     }
 }

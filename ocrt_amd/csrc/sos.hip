@@ -27,10 +27,10 @@ template <typename SrcFn, typename EmitFn>
 __device__ inline void sweep_column_f64(int hemi, int nt, double mu, double bc, const double *h,
                                         const double *T, int n, int o, SrcFn S, EmitFn emit)
 {
-    // levels are taken in blocks of 8: the block's sources, depths and
+    // levels are taken in blocks of 4: the block's sources, depths and
     // transmittances are gathered first (clamped indices, no branches), so
     // their loads overlap, then the recurrence runs over them
-    constexpr int kB = 8;
+    constexpr int kB = 4;
     const int step = hemi ? 1 : -1;
     const int k_first = hemi ? 0 : nt;
     double run = hemi ? 0.0 : bc;
@@ -92,8 +92,11 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (tix >= n_tiles) return;
     const ColTile ct = tiles[tix];
-    if (ct.item < 0) return;
-    const DevItem &it = items[ct.item];
+    // the tile is wavefront-uniform: its item, and so every per-level array
+    // read of the scan, stays in scalar registers
+    const int item = __builtin_amdgcn_readfirstlane(ct.item);
+    if (item < 0) return;
+    const DevItem &it = items[item];
     const DevInst &I = inst[it.inst];
     const int n = it.n, nt = it.nt, Pp = it.P, m = it.m;
     const int col = ct.col0 + lane;
@@ -106,11 +109,12 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
         s.residual = 0.0;
         s.max_cur = 0ull;
         s.max_tot = 0ull;
-        st[ct.item] = s;
+        st[item] = s;
     }
     if (col >= ct.col_end) return;
     FT *F = reinterpret_cast<FT *>(P.F) + it.off_F;
-    double *tot = P.tot + it.off_tot;
+    double *tot = MIXED ? nullptr : P.tot + it.off_tot;
+    float *tot32 = MIXED ? P.tot32 + it.off_tot : nullptr;
     if (MIXED) {
         P.comp[it.off_cmp + col] = 0.f;
         P.comp[it.off_cmp + Pp + col] = 0.f;
@@ -120,13 +124,16 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
         for (int k = 0; k <= nt; ++k) {
             F[(long long)k * Pp + col] = to_storage<FT>(0.0);
             if (Flo) Flo[(long long)k * Pp + col] = __float2bfloat16(0.f);
-            tot[(long long)k * Pp + col] = 0.0;
+            if (MIXED) tot32[(long long)k * Pp + col] = 0.f;
+            else tot[(long long)k * Pp + col] = 0.0;
         }
+        if (MIXED) P.tend[it.off_cmp + col] = P.tend[it.off_cmp + Pp + col] = 0.0;
         return;
     }
     const LevelView lv(P, I);
     int c, hemi, o;
     col_decode(col, n, c, hemi, o);
+    hemi = __builtin_amdgcn_readfirstlane(hemi);   // column tiles are direction-pure
     const double *mu_ring = P.ring + I.off_ring;
     const double mu = mu_ring[o];
     const double *T = P.T + I.off_T;
@@ -145,11 +152,14 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
                      [&](int k, double v) {
                          if (rough_seed) v += rsun * (FS * (down * exp(-(tau - lv.h[k]) / mu)));
                          const long long ix = (long long)k * Pp + col;
-                         tot[ix] = v;
-                         if constexpr (MIXED)
+                         if constexpr (MIXED) {
+                             tot32[ix] = (float)v;
+                             if (k == 0 || k == nt) P.tend[it.off_cmp + (k == 0 ? 0 : Pp) + col] = v;
                              store_field_pair(reinterpret_cast<__hip_bfloat16 *>(F), Flo, ix, v);
-                         else
+                         } else {
+                             tot[ix] = v;
                              F[ix] = to_storage<FT>(v);
+                         }
                          if (hemi == 1 && k == nt) P.bc[it.off_bc + 3 * n + c * n + o] = v;
                      });
 }
@@ -430,9 +440,11 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
 // exposing it once per level.  U and W do not change the arithmetic (every
 // variant gives the same bits); the plan's autotuner picks them.
 //
-// The order sum is fp64 over the whole field (it feeds the convergence test).
-// In the mixed form the two rows that become outputs, k = 0 (TOA) and k = nt
-// (BOA), also carry an fp32 Neumaier compensation term.
+// The order sum of the fp64 form is fp64 over the whole field.  In the mixed
+// form it is fp32 over the whole field, where its interior rows only feed the
+// max-norm of the convergence test; the two rows that become outputs, k = 0
+// (TOA) and k = nt (BOA), are summed in fp64 with an fp32 Neumaier
+// compensation term.
 __device__ inline int clamp_level(int k, int nt) { return k < 0 ? 0 : (k > nt ? nt : k); }
 __device__ inline int clamp_layer(int l, int nt) { return l < 0 ? 0 : (l > nt - 1 ? nt - 1 : l); }
 
@@ -461,20 +473,27 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
         const LevelView lv(P, I);
         int c, hemi, o;
         col_decode(col, n, c, hemi, o);
+        hemi = __builtin_amdgcn_readfirstlane(hemi);   // column tiles are direction-pure
         const JT *__restrict__ J = reinterpret_cast<const JT *>(P.J) + off_F + col;
         FT *__restrict__ F = reinterpret_cast<FT *>(P.F) + off_F + col;
         __hip_bfloat16 *__restrict__ Flo =
             (MIXED && it.fsplit) ? reinterpret_cast<__hip_bfloat16 *>(P.Flo) + it.off_Flo + col : nullptr;
-        double *__restrict__ tot = P.tot + it.off_tot + col;
+        double *__restrict__ tot = MIXED ? nullptr : P.tot + it.off_tot + col;
+        float *__restrict__ tot32 = MIXED ? P.tot32 + it.off_tot + col : nullptr;
         double *__restrict__ bottom = P.bc + off_bc + 3 * n + c * n + o;
         float *__restrict__ cmp_top = MIXED ? P.comp + off_cmp + col : nullptr;
         float *__restrict__ cmp_bot = MIXED ? P.comp + off_cmp + Pp + col : nullptr;
+        double *__restrict__ end_top = MIXED ? P.tend + off_cmp + col : nullptr;
+        double *__restrict__ end_bot = MIXED ? P.tend + off_cmp + Pp + col : nullptr;
         const double mu = P.ring[I.off_ring + o];
         const double bc = (I.surf != 0 && hemi == 0) ? P.bc[off_bc + c * n + o] : 0.0;
         // level order of the scan and the layer crossed when stepping to level k
         const int k_first = hemi ? 0 : nt;
         const int step = hemi ? 1 : -1;
         auto layer_of = [&](int k) { return hemi ? k - 1 : k; };
+        // t_old: the order sum before this order at level k (fp64 form; the
+        // mixed form passes its fp32 interior value and reads the output rows
+        // from their fp64 copies here)
         auto emit = [&](int k, double v, double t_old) {
             const long long ix = (long long)k * Pp;
             if constexpr (MIXED)
@@ -484,16 +503,30 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
             mcur = fmax(mcur, fabs(v));
             if (hemi == 1 && k == nt) *bottom = v;
             if (!accumulate) return;
-            const double tsum = t_old + v;
-            tot[ix] = tsum;
-            mtot = fmax(mtot, fabs(tsum));
-            if (MIXED && (k == 0 || k == nt)) {
-                float *cp = k == 0 ? cmp_top : cmp_bot;
-                const double e = (fabs(t_old) >= fabs(v)) ? ((t_old - tsum) + v) : ((v - tsum) + t_old);
-                *cp = (float)((double)*cp + e);
+            if constexpr (MIXED) {
+                const float s32 = (float)t_old + (float)v;
+                tot32[ix] = s32;
+                if (k == 0 || k == nt) {
+                    double *te = k == 0 ? end_top : end_bot;
+                    float *cp = k == 0 ? cmp_top : cmp_bot;
+                    const double t64 = *te, tsum = t64 + v;
+                    *te = tsum;
+                    mtot = fmax(mtot, fabs(tsum));
+                    const double e = (fabs(t64) >= fabs(v)) ? ((t64 - tsum) + v) : ((v - tsum) + t64);
+                    *cp = (float)((double)*cp + e);
+                } else {
+                    mtot = fmax(mtot, fabs((double)s32));
+                }
+            } else {
+                const double tsum = t_old + v;
+                tot[ix] = tsum;
+                mtot = fmax(mtot, fabs(tsum));
             }
         };
-        emit(k_first, hemi ? 0.0 : bc, accumulate ? tot[(long long)k_first * Pp] : 0.0);
+        if constexpr (MIXED)
+            emit(k_first, hemi ? 0.0 : bc, accumulate ? (double)tot32[(long long)k_first * Pp] : 0.0);
+        else
+            emit(k_first, hemi ? 0.0 : bc, accumulate ? tot[(long long)k_first * Pp] : 0.0);
         if (!MIXED) {
             const double *T = P.T + I.off_T;
             double run = hemi ? 0.0 : bc;
@@ -534,8 +567,7 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
             float run = hemi ? 0.f : (float)bc;
             float s_in = (float)J[(long long)k_first * Pp];
             for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * U) {
-                float sj[U], om[U], idt[U];
-                double to[U];
+                float sj[U], om[U], idt[U], to[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     // clamped indices keep the block's loads unconditional, so
@@ -545,7 +577,7 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                     sj[u] = (float)J[(long long)k * Pp];
                     om[u] = OM[(long long)l * n + o];
                     idt[u] = IDT[l];
-                    to[u] = tot[(long long)k * Pp];
+                    to[u] = tot32[(long long)k * Pp];
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -559,6 +591,13 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                 }
             }
         }
+    } else if (!MIXED && col < ct.col_end) {
+        // pad columns (6n <= col < P) of the fp64 field are written too:
+        // adjoint plans sweep every order into a buffer of its own, and the
+        // next scattering integral multiplies these columns by zero operator
+        // entries, which must not meet leftover NaN or Inf bit patterns
+        FT *__restrict__ F = reinterpret_cast<FT *>(P.F) + off_F + col;
+        for (int k = 0; k <= nt; ++k) F[(long long)k * Pp] = to_storage<FT>(0.0);
     }
     if (!accumulate) return;
     const double bm = wave_max(mcur);
@@ -570,13 +609,15 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
 }
 
 // ---------------------------------------------------------------------------
-// Order sum at level k of column col; the output rows k = 0 and k = nt of the
-// mixed form add their compensation term.
+// Order sum at an output row (k = 0 or k = nt) of column col: the mixed form
+// reads the fp64 copy of the row plus its compensation term.
 __device__ inline double total_at(const Pools &P, const DevItem &it, bool mixed, int k, int col)
 {
-    const double t = P.tot[it.off_tot + (long long)k * it.P + col];
-    if (mixed && (k == 0 || k == it.nt)) return t + (double)P.comp[it.off_cmp + (k == 0 ? 0 : it.P) + col];
-    return t;
+    if (mixed) {
+        const long long e = it.off_cmp + (k == 0 ? 0 : it.P) + col;
+        return P.tend[e] + (double)P.comp[e];
+    }
+    return P.tot[it.off_tot + (long long)k * it.P + col];
 }
 
 // Per-item outputs after the order loop (rt_solver.c:1010-1061).

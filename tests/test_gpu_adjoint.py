@@ -198,3 +198,28 @@ def test_autograd_function(ocrt, oracle):
         gl = layer_gradients(lay, gh, gx, gy)
         assert close(layers.grad[i].numpy(), gl, 1e-9)
         assert abs(wind.grad[i].item() - gw) <= 1e-6 * abs(gw)
+
+
+def test_adjoint_forward_on_reused_nan_memory(ocrt):
+    """Checkpoint buffers of an adjoint plan may start with any bit pattern:
+    device memory filled with NaN and released before the plan is built does
+    not reach the results (the fp64 sweeps write the pad columns of every
+    order's field, which the next scattering integral reads)."""
+    import torch
+    cases, gk = layered_cases(seed=5)
+    descs = [make_instance(layers=lay, **kw) for lay, kw in cases]
+    a = ocrt.solve(descs, precision="fp64")
+    for sparse in (False, True):
+        junk = torch.full((1 << 28,), float("nan"), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        del junk
+        torch.cuda.empty_cache()
+        p = ocrt.Plan(descs, precision="fp64", adjoint=True, ckpt_sparse=sparse)
+        p.execute()
+        b = p.results()
+        adj = p.adjoint(np.ones((len(descs), 3)))
+        p.close()
+        for f in ("rho_I", "rho_Q", "rho_U", "n_orders_used"):
+            assert np.array_equal(a[f], b[f]), (sparse, f)
+        for key in ("grad_h", "grad_x", "grad_y", "grad_wind"):
+            assert np.all(np.isfinite(adj[key])), (sparse, key)
