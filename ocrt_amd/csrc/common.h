@@ -65,7 +65,10 @@ struct DevItem {
     int inst, m, n, nt, L, P, K, parts;   // parts: bit0 particle, bit1 molecular
     int Mp, tiles_m, tiles_n, lmax;
     int fsplit;           // mixed form: field stored as a bf16 pair hi + lo (low Fourier modes)
-    int pad_;
+    int op_owner;         // 1: computes its basis, phase tables and operator; 0: shares another item's
+    // items of instances with identical ring, sun, phase-matrix expansion and
+    // Rayleigh coefficients (the bands of one geometry and aerosol model) share
+    // one basis, one set of phase tables and one operator per Fourier mode
     long long off_basis;  // pool_basis: 3 x [(lmax+1)][2n+1]
     long long off_tab;    // pool_tab: 6 x [(n+1)][2n+1]
     long long off_A;      // pool_A: [Mp][K]

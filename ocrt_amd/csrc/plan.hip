@@ -121,7 +121,12 @@ struct HostInst {
     std::vector<double> mu, w;
     int status;
     long long lev_n, T_n, R_n, greek_n;
+    int op_src = -1;     // an earlier instance with the same operator inputs, -1: none
 };
+
+// Window of earlier instances searched for one whose surface kernels or
+// scattering operator an instance can share.
+constexpr int kShareWindow = 64;
 
 struct Chunk {
     int i0, i1, it0, it1;       // instance and item ranges
@@ -411,6 +416,25 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         h.R_n = (I.surf == 2) ? (long long)(I.mmax + 1) * n * n * 9 + (long long)(I.mmax + 1) * n * 9 : 0;
         p->mmax_all = std::max(p->mmax_all, I.mmax);
     }
+    // Instances whose basis, phase tables and scattering operator are equal
+    // bit for bit: same ring (nodes, weights, view node), sun, Fourier range,
+    // phase-matrix expansion and Rayleigh coefficients.  Only their optical
+    // depths and scattering fractions differ, which enter per level.
+    auto same_operator = [&](int a, int b) {
+        const HostInst &ha = p->inst[a], &hb = p->inst[b];
+        const DevInst &A = ha.d, &B = hb.d;
+        if (A.n != B.n || A.mus != B.mus || A.mmax != B.mmax || A.lmax != B.lmax || A.aer != B.aer ||
+            A.ray != B.ray || A.La != B.La || A.beta0 != B.beta0 || A.beta2 != B.beta2 || A.gamma2 != B.gamma2 ||
+            A.alpha2 != B.alpha2 || ha.mu != hb.mu || ha.w != hb.w)
+            return false;
+        if (A.aer)
+            for (int e = 0; e < 4 * (A.La + 1); ++e)
+                if (in[a].greek[e] != in[b].greek[e]) return false;
+        return true;
+    };
+    for (int i = 0; i < n_inst && !p->per_inst; ++i)
+        for (int j = std::max(0, i - kShareWindow); j < i; ++j)
+            if (p->inst[j].op_src < 0 && same_operator(i, j)) { p->inst[i].op_src = j; break; }
     // ---- chunking: instances grouped so one chunk's working set stays bounded
     const double budget = mixed ? 48e9 : 48e9;
     const size_t fe = mixed ? 2 : 8, je = mixed ? 4 : 8, ae = mixed ? 2 : 8;
@@ -434,15 +458,17 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             const DevInst &I = h.d;
             double b = (double)(h.lev_n + 2 * h.T_n + h.R_n) * 8.0;
             const int n = I.n, L = I.nt + 1, P = roundup(6 * n, Palign), Mp = roundup(6 * n, 64);
+            const bool shared_op = h.op_src >= c.i0;   // its source is in this chunk
             for (int m = 0; m <= I.mmax; ++m) {
                 const int parts = (I.aer ? 1 : 0) + ((m <= 2 && I.ray) ? 1 : 0);
                 const int K = std::max(parts, 1) * P;
-                b += 3.0 * (I.lmax + 1) * (2 * n + 1) * 8 + 6.0 * (n + 1) * (2 * n + 1) * 8 +
-                     (double)Mp * K * ae * (mixed ? 2 : 1) + (double)L * P * (fe + je + (mixed ? 4 : 8)) +
+                if (!shared_op)
+                    b += 3.0 * (I.lmax + 1) * (2 * n + 1) * 8 + 6.0 * (n + 1) * (2 * n + 1) * 8 +
+                         (double)Mp * K * ae * (mixed ? 2 : 1) * (adjoint ? 2 : 1);
+                b += (double)L * P * (fe + je + (mixed ? 4 : 8)) +
                      (mixed ? 24.0 * P : 0.0) + ((mixed && m < kSplitModes) ? 2.0 * L * P : 0.0) + 6.0 * n * 8;
-                if (adjoint)   // transposed operator, checkpoints, segment fields, lambda, second parts
-                    b += (double)Mp * K * ae +
-                         (double)L * P * 8 * ((I.max_orders - 1) / every + 1 + (every - 1) + 2);
+                if (adjoint)   // checkpoints, segment fields, lambda, second parts
+                    b += (double)L * P * 8 * ((I.max_orders - 1) / every + 1 + (every - 1) + 2);
             }
             if (adjoint) b += (double)h.R_n * 8;
             const int items_after = (item_global - c.it0) + I.mmax + 1;
@@ -465,17 +491,27 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     // ---- per-chunk layout
     for (Chunk &c : p->chunks) {
         long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0, cmp = 0, flo = 0;
-        std::vector<std::vector<GemmTile>> gemm_of_item;
+        // items grouped by the operator they use: class of each owner item and
+        // the member items of each class
+        std::vector<int> class_of_item;
+        std::vector<std::vector<int>> class_items;
+        std::vector<char> op_owner(c.i1 - c.i0, 0);
         for (int ii = c.i0; ii < c.i1; ++ii) {
             HostInst &h = p->inst[ii];
             DevInst I = h.d;
+            // share the operator of an earlier owner of this chunk with equal inputs
+            int op_from = -1;
+            if (h.op_src >= c.i0)
+                for (int jj = std::max(c.i0, ii - kShareWindow); jj < ii && op_from < 0; ++jj)
+                    if (op_owner[jj - c.i0] && same_operator(ii, jj)) op_from = jj - c.i0;
+            op_owner[ii - c.i0] = op_from < 0;
             I.off_lev = lev; lev += h.lev_n;
             I.off_T = T; T += h.T_n;
             if (I.surf == 2) {
                 // share the surface kernels with an earlier instance of this chunk
                 // that has the same ring, sun and surface
                 int owner = -1;
-                for (int jj = std::max(c.i0, ii - 64); jj < ii && owner < 0; ++jj) {
+                for (int jj = std::max(c.i0, ii - kShareWindow); jj < ii && owner < 0; ++jj) {
                     const HostInst &g = p->inst[jj];
                     const DevInst &J = c.hinst[jj - c.i0];
                     if (J.surf == 2 && J.surf_owner && J.n == I.n && J.mmax == I.mmax && J.mus == I.mus &&
@@ -517,9 +553,21 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 const int tile_lev = mixed ? kGemmLevels : kGemmLevelsF64;
                 it.Mp = Mp; it.tiles_m = Mp / 64; it.tiles_n = (L + tile_lev - 1) / tile_lev;
                 it.lmax = I.lmax;
-                it.off_basis = basis; basis += 3LL * (I.lmax + 1) * (2 * n + 1);
-                it.off_tab = tab; tab += 6LL * (n + 1) * (2 * n + 1);
-                it.off_A = A; A += (long long)Mp * it.K * (mixed ? 2 : 1);
+                int cls;
+                if (op_from >= 0) {
+                    const int src = c.hinst[op_from].item0 + m;
+                    const DevItem &o = c.hitem[src];
+                    it.off_basis = o.off_basis; it.off_tab = o.off_tab; it.off_A = o.off_A;
+                    it.op_owner = 0;
+                    cls = class_of_item[src];
+                } else {
+                    it.off_basis = basis; basis += 3LL * (I.lmax + 1) * (2 * n + 1);
+                    it.off_tab = tab; tab += 6LL * (n + 1) * (2 * n + 1);
+                    it.off_A = A; A += (long long)Mp * it.K * (mixed ? 2 : 1);
+                    it.op_owner = 1;
+                    cls = (int)class_items.size();
+                    class_items.emplace_back();
+                }
                 it.off_F = F; it.off_tot = F; F += (long long)L * Pp;
                 it.off_bc = bc; bc += 6LL * n;
                 it.off_cmp = cmp; cmp += mixed ? 2LL * Pp : 0;
@@ -529,10 +577,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 it.off_Flo = flo; flo += it.fsplit ? (long long)L * Pp : 0;
                 c.max_pack = std::max(c.max_pack, (long long)Mp * it.K);
                 const int item_local = (int)c.hitem.size();
-                std::vector<GemmTile> gt;
-                for (int tm = 0; tm < it.tiles_m; ++tm)
-                    for (int tn = 0; tn < it.tiles_n; ++tn) gt.push_back(GemmTile{item_local, tm, tn, 0});
-                gemm_of_item.push_back(std::move(gt));
+                class_of_item.push_back(cls);
+                class_items[cls].push_back(item_local);
                 // direction-pure column tiles: upward half, downward half, padding
                 const int bounds[4] = {0, 3 * n, 6 * n, Pp};
                 for (int h = 0; h < 3; ++h)
@@ -543,7 +589,17 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             if (I.ipss) c.hjobs.push_back(IpssJob{ii - c.i0, 0, I.vza, I.raa});
             c.hinst.push_back(I);
         }
-        c.htiles = xcd_tile_order(gemm_of_item);
+        // the GEMM tiles of one class stay on one XCD, ordered (row tile, item,
+        // level tile) so consecutive tiles reuse the class's operator rows in L2
+        std::vector<std::vector<GemmTile>> gemm_of_class(class_items.size());
+        for (size_t k = 0; k < class_items.size(); ++k) {
+            const DevItem &o = c.hitem[class_items[k][0]];
+            for (int tm = 0; tm < o.tiles_m; ++tm)
+                for (int il : class_items[k])
+                    for (int tn = 0; tn < c.hitem[il].tiles_n; ++tn)
+                        gemm_of_class[k].push_back(GemmTile{il, tm, tn, 0});
+        }
+        c.htiles = xcd_tile_order(gemm_of_class);
         c.n_jobs = (int)c.hjobs.size();
         c.lev = lev; c.T = T; c.R = R; c.basis = basis; c.tab = tab; c.A = A; c.F = F; c.bc = bc;
         c.cmp = cmp; c.flo = flo;
@@ -589,7 +645,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     rc |= dmalloc(p, (void **)&P.Tf, mx_T * 4);
     rc |= dmalloc(p, (void **)&P.idt, mx_lev * 4);
     rc |= dmalloc(p, (void **)&P.R, mx_R * 8);
-    rc |= dmalloc(p, (void **)&P.basis, mx_basis * 8);
+    // padded: the phase tables read the rows of a full group past the last row
+    rc |= dmalloc(p, (void **)&P.basis, (mx_basis + kTabRows) * 8);
     rc |= dmalloc(p, (void **)&P.tab, mx_tab * 8);
     rc |= dmalloc(p, (void **)&P.A, mx_A * ae);
     rc |= dmalloc(p, (void **)&P.F, mx_F * fe);
@@ -751,7 +808,7 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             hipLaunchKernelGGL(coxmunk_fourier<false>, dim3(c.max_pairs, ni), dim3(256), 0, s, c.dinst, P);
     }
     hipLaunchKernelGGL(angular_basis, dim3(nit), dim3(128), 0, s, c.dinst, c.ditem, P);
-    hipLaunchKernelGGL(phase_tables, dim3(xcd_item_grid(nit, c.tab_groups)), dim3(256), 0, s, c.dinst, c.ditem, P,
+    hipLaunchKernelGGL(phase_tables, dim3(xcd_item_grid(nit, c.tab_groups)), dim3(64), 0, s, c.dinst, c.ditem, P,
                        nit, c.tab_groups);
     // a few blocks per item striding over its operator: an item's table reads
     // stay in one L2 and the launch stays far below 2^31 work-items

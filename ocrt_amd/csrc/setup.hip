@@ -11,6 +11,7 @@
 //       the Stokes sign pattern folded in (rt_sos_operator.c:50-126, 353-407)
 #include "common.h"
 #include "device_math.h"
+#include "wave.h"
 
 namespace ocrt {
 
@@ -260,14 +261,21 @@ __global__ void fourier_weights(double *w, int mmax)
 // DERIV also writes d R^m / d sigma^2 (pool dR, the layout of pool R) for the
 // adjoint: every entry is proportional to the brdf, so its derivative is the
 // entry times d ln(brdf) / d sigma^2.
+//
+// Thread t evaluates the Mueller matrix at azimuths p = t and t + 256 and keeps
+// both in registers.  The Fourier sums are then taken in batches of 8 rows
+// (8 modes, or 4 modes x {value, derivative}) x 9 entries: every thread reads
+// its two samples of the cos / sin rows of the batch (consecutive azimuths
+// over the lanes), forms its products, and a transposed wavefront reduction
+// leaves each row's totals in one group of eight lanes; the four wavefronts'
+// totals are added through LDS.
 template <bool DERIV>
 __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict__ inst, Pools P)
 {
     constexpr int kHalf = kNPhi / 2;
-    // rows padded by one entry: the nine rows are read together at one phi
-    // and must not fall on one LDS bank
-    __shared__ double R[9][kHalf + 1];
-    __shared__ double DL[DERIV ? kHalf : 1];
+    static_assert(kHalf == 2 * 256, "two azimuths per thread");
+    constexpr int kRowModes = DERIV ? 4 : 8;   // Fourier modes per batch
+    __shared__ double red[4][8][9];
     const DevInst &I = inst[blockIdx.y];
     if (I.surf != 2 || !I.surf_owner) return;
     const int n = I.n;
@@ -294,36 +302,59 @@ __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict
         o = pair - n * n; i = 0;
         mo = mu[o]; mi = I.mus;
     }
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const double dphi = 2.0 * OCRT_PI / (double)kNPhi;
     const double s2v = slope_variance(I.wind, I.sigma_type);
     const double shadow = sancer_shadow(mo, mi, s2v);
     const double dls = DERIV ? sancer_dlog(mo, mi, s2v, shadow) : 0.0;
-    for (int p = threadIdx.x; p < kHalf; p += blockDim.x) {
-        const double phi = (p + 0.5) * dphi;
-        double r[9], tb2;
-        coxmunk_R_core(mo, mi, cos(phi), sin(phi), s2v, shadow, I.n_water, I.q_conv, r, &tb2);
-        for (int e = 0; e < 9; ++e) R[e][p] = r[e];
-        if (DERIV) DL[p] = (tb2 >= 0.0) ? tb2 / (s2v * s2v) - 1.0 / s2v + dls : 0.0;
+    double r[2][9], dl[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const double phi = (t + 256 * q + 0.5) * dphi;
+        double tb2;
+        coxmunk_R_core(mo, mi, cos(phi), sin(phi), s2v, shadow, I.n_water, I.q_conv, r[q], &tb2);
+        dl[q] = (DERIV && tb2 >= 0.0) ? tb2 / (s2v * s2v) - 1.0 / s2v + dls : 0.0;
     }
-    __syncthreads();
-    const int nout = (I.mmax + 1) * 9;
-    for (int t = threadIdx.x; t < nout; t += blockDim.x) {
-        const int m = t / 9, kl = t % 9;
-        const bool use_cos = (kl == 0 || kl == 1 || kl == 3 || kl == 4 || kl == 8);
-        const double *W = P.cm_w + ((long long)m * 2 + (use_cos ? 0 : 1)) * kNPhi;
-        double acc = 0.0, accd = 0.0;
-        for (int p = 0; p < kHalf; ++p) {
-            const double rw = R[kl][p] * W[p];
-            acc += rw;
-            if (DERIV) accd += rw * DL[p];
+    const int M = I.mmax;
+    for (int m0 = 0; m0 <= M; m0 += kRowModes) {
+        double v[8 * 9];   // [row][entry]
+#pragma unroll
+        for (int j = 0; j < kRowModes; ++j) {
+            const int m = m0 + j <= M ? m0 + j : M;   // rows past the last mode are computed, not written
+            const double *wc = P.cm_w + (long long)m * 2 * kNPhi, *ws = wc + kNPhi;
+            const double c0 = wc[t], c1 = wc[t + 256], s0 = ws[t], s1 = ws[t + 256];
+#pragma unroll
+            for (int kl = 0; kl < 9; ++kl) {
+                const bool use_cos = (kl == 0 || kl == 1 || kl == 3 || kl == 4 || kl == 8);
+                const double a0 = r[0][kl] * (use_cos ? c0 : s0), a1 = r[1][kl] * (use_cos ? c1 : s1);
+                if (DERIV) {
+                    v[(2 * j) * 9 + kl] = a0 + a1;
+                    v[(2 * j + 1) * 9 + kl] = a0 * dl[0] + a1 * dl[1];
+                } else {
+                    v[j * 9 + kl] = a0 + a1;
+                }
+            }
         }
-        const double nrm = (m == 0) ? 1.0 / (2.0 * OCRT_PI) : 1.0 / OCRT_PI;
-        double v = 2.0 * acc * (dphi * nrm), vd = 2.0 * accd * (dphi * nrm);
-        if (m > 0 && (kl == 2 || kl == 5)) { v = -v; vd = -vd; }
-        const long long at = !solar ? I.off_R + (((long long)m * n + o) * n + i) * 9 + kl
-                                    : I.off_Rsun + ((long long)m * n + o) * 9 + kl;
-        P.R[at] = v;
-        if (DERIV) P.dR[at] = vd;
+        wave_sum_8xC<9>(v, lane);
+        if ((lane & 7) == 0)
+#pragma unroll
+            for (int kl = 0; kl < 9; ++kl) red[wave][lane >> 3][kl] = v[kl];
+        __syncthreads();
+        if (t < 72) {
+            const int row = t / 9, kl = t % 9;
+            const double tot = (red[0][row][kl] + red[1][row][kl]) + (red[2][row][kl] + red[3][row][kl]);
+            const int m = m0 + (DERIV ? row >> 1 : row);
+            if (m <= M) {
+                const double nrm = (m == 0) ? 1.0 / (2.0 * OCRT_PI) : 1.0 / OCRT_PI;
+                double val = 2.0 * tot * (dphi * nrm);
+                if (m > 0 && (kl == 2 || kl == 5)) val = -val;
+                const long long at = !solar ? I.off_R + (((long long)m * n + o) * n + i) * 9 + kl
+                                            : I.off_Rsun + ((long long)m * n + o) * 9 + kl;
+                if (DERIV && (row & 1)) P.dR[at] = val;
+                else P.R[at] = val;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -344,6 +375,7 @@ __device__ inline double dir_mu(const double *ring_mu, int n, double mus, int d)
 __global__ void angular_basis(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P)
 {
     const DevItem &it = items[blockIdx.x];
+    if (!it.op_owner) return;   // shares the basis of an identical item
     const DevInst &I = inst[it.inst];
     const int n = it.n, dirs = 2 * n + 1, m = it.m, lmax = it.lmax;
     double *plm = P.basis + it.off_basis;
@@ -409,84 +441,86 @@ __global__ void angular_basis(const DevInst *__restrict__ inst, const DevItem *_
 //   ARR       = sum_l rtl_j rtl_k zeta_l + rrl_j rrl_k alpha_l
 //   ART       = sum_l rtl_j rrl_k alpha_l + rrl_j rtl_k zeta_l
 // for rows j = 0..n (directions j + n) and every column direction k.  One
-// workgroup per (item, group of kTabRows rows), the groups of an item on one
-// XCD; the rows' basis values and the Greek coefficients are staged in LDS
-// in chunks of kTabL orders and every thread accumulates its column for all
-// rows of the group.
-constexpr int kTabL = 64;
-
-__global__ __launch_bounds__(256) void phase_tables(const DevInst *__restrict__ inst,
-                                                    const DevItem *__restrict__ items, Pools P, int n_items,
-                                                    int groups)
+// wavefront per (item, group of kTabRows rows), the groups of an item on one
+// XCD.  The rows' basis values and the Greek coefficients of each order l are
+// wavefront-uniform (scalar loads); the lanes take the column directions in
+// passes of 64 and each accumulates its column for all rows of the group,
+// with the next order's column values loaded before the current order's
+// multiply-adds.
+__global__ __launch_bounds__(64) void phase_tables(const DevInst *__restrict__ inst,
+                                                   const DevItem *__restrict__ items, Pools P, int n_items,
+                                                   int groups)
 {
     int item, g;
     xcd_item_block(groups, item, g);
     if (item >= n_items) return;
     const DevItem &it = items[item];
+    if (!it.op_owner) return;   // shares the tables of an identical item
     const DevInst &I = inst[it.inst];
     const int n = it.n, dirs = 2 * n + 1, m = it.m, lmax = it.lmax;
     const int j0 = g * kTabRows;
     if (j0 > n) return;
     const int nr = min(kTabRows, n + 1 - j0);
     const long long cnt = (long long)(n + 1) * dirs;
-    double *tab = P.tab + it.off_tab;
+    double *__restrict__ tab = P.tab + it.off_tab;
+    const int lane = threadIdx.x;
     if (!I.aer) {
-        for (int e = threadIdx.x; e < nr * dirs; e += blockDim.x)
+        for (int e = lane; e < nr * dirs; e += 64)
             for (int t = 0; t < 6; ++t) tab[t * cnt + (long long)j0 * dirs + e] = 0.0;
         return;
     }
-    const double *plm = P.basis + it.off_basis;
-    const double *rrl = plm + (long long)(lmax + 1) * dirs;
-    const double *rtl = rrl + (long long)(lmax + 1) * dirs;
-    const double *gk = P.greek + I.off_greek;
+    const double *__restrict__ plm = P.basis + it.off_basis;
+    const double *__restrict__ rrl = plm + (long long)(lmax + 1) * dirs;
+    const double *__restrict__ rtl = rrl + (long long)(lmax + 1) * dirs;
+    const double *__restrict__ gk = P.greek + I.off_greek;
     const int q = I.La + 1;
     const int lend = lmax < I.La ? lmax : I.La;
-    __shared__ double sp[kTabL][kTabRows], sr[kTabL][kTabRows], st[kTabL][kTabRows];
-    __shared__ double cb[kTabL], cg[kTabL], ca[kTabL], cz[kTabL];
-    for (int k0 = 0; k0 < dirs; k0 += blockDim.x) {
-        const int k = k0 + threadIdx.x;
-        const bool live = k < dirs;
+    // the rows' values through the constant address space: wavefront-uniform
+    // addresses there become scalar loads
+    typedef const __attribute__((address_space(4))) double cdouble;
+    cdouble *plm_s = (cdouble *)plm, *rrl_s = (cdouble *)rrl, *rtl_s = (cdouble *)rtl, *gk_s = (cdouble *)gk;
+    for (int k0 = 0; k0 < dirs; k0 += 64) {
+        const int k = k0 + lane;
+        const int kc = k < dirs ? k : dirs - 1;   // clamped column: loads stay in bounds
         double acc[kTabRows][6];
 #pragma unroll
         for (int r = 0; r < kTabRows; ++r)
 #pragma unroll
             for (int t = 0; t < 6; ++t) acc[r][t] = 0.0;
-        for (int l0 = m; l0 <= lend; l0 += kTabL) {
-            const int nl = min(kTabL, lend - l0 + 1);
-            __syncthreads();
-            for (int e = threadIdx.x; e < nl * kTabRows; e += blockDim.x) {
-                const int li = e / kTabRows, r = e % kTabRows;
-                const long long b = (long long)(l0 + li) * dirs + j0 + r + n;
-                const bool in = r < nr;
-                sp[li][r] = in ? plm[b] : 0.0;
-                sr[li][r] = in ? rrl[b] : 0.0;
-                st[li][r] = in ? rtl[b] : 0.0;
-            }
-            for (int li = threadIdx.x; li < nl; li += blockDim.x) {
-                const int l = l0 + li;
-                cb[li] = gk[l]; cg[li] = gk[q + l]; ca[li] = gk[2 * q + l]; cz[li] = gk[3 * q + l];
-            }
-            __syncthreads();
-            if (live) {
-                for (int li = 0; li < nl; ++li) {
-                    const long long b = (long long)(l0 + li) * dirs + k;
-                    const double pk = plm[b], rk = rrl[b], tk = rtl[b];
-                    const double pkb = pk * cb[li], rkg = rk * cg[li], tkg = tk * cg[li];
-                    const double tka = tk * ca[li], tkz = tk * cz[li], rka = rk * ca[li], rkz = rk * cz[li];
+        // one order's terms for every row of the group
+        auto term = [&](int l, double pk, double rk, double tk) {
+            const long long bj = (long long)l * dirs + n + j0;
+            const double cb = gk_s[l], cg = gk_s[q + l], ca = gk_s[2 * q + l], cz = gk_s[3 * q + l];
+            const double pkb = pk * cb, rkg = rk * cg, tkg = tk * cg;
+            const double tka = tk * ca, tkz = tk * cz, rka = rk * ca, rkz = rk * cz;
 #pragma unroll
-                    for (int r = 0; r < kTabRows; ++r) {
-                        const double pj = sp[li][r], rj = sr[li][r], tj = st[li][r];
-                        acc[r][0] += pj * pkb;
-                        acc[r][1] += pj * rkg;
-                        acc[r][2] += pj * tkg;
-                        acc[r][3] += tj * tkz + rj * rka;
-                        acc[r][4] += tj * rka + rj * tkz;
-                        acc[r][5] += tj * tka + rj * rkz;
-                    }
-                }
+            for (int r = 0; r < kTabRows; ++r) {
+                // rows past the group's end read on into the next order's row (or
+                // the pool's padding); their sums are never stored
+                const double pj = plm_s[bj + r], rj = rrl_s[bj + r], tj = rtl_s[bj + r];
+                acc[r][0] += pj * pkb;
+                acc[r][1] += pj * rkg;
+                acc[r][2] += pj * tkg;
+                acc[r][3] += tj * tkz + rj * rka;
+                acc[r][4] += tj * rka + rj * tkz;
+                acc[r][5] += tj * tka + rj * rkz;
             }
+        };
+        // orders in pairs: both orders' column values are loaded before the
+        // first order's multiply-adds, so the second load overlaps them
+        int l = m;
+        for (; l + 1 <= lend; l += 2) {
+            const long long b0 = (long long)l * dirs + kc, b1 = b0 + dirs;
+            const double p0 = plm[b0], r0 = rrl[b0], t0 = rtl[b0];
+            const double p1 = plm[b1], r1 = rrl[b1], t1 = rtl[b1];
+            term(l, p0, r0, t0);
+            term(l + 1, p1, r1, t1);
         }
-        if (live) {
+        if (l <= lend) {
+            const long long b0 = (long long)l * dirs + kc;
+            term(l, plm[b0], rrl[b0], rtl[b0]);
+        }
+        if (k < dirs) {
 #pragma unroll
             for (int r = 0; r < kTabRows; ++r) {
                 if (r >= nr) break;
@@ -611,6 +645,7 @@ __global__ __launch_bounds__(256) void pack_operator(const DevInst *__restrict__
     xcd_item_block(blocks_per_item, item, bx);   // an item's tables stay in one XCD's L2
     if (item >= n_items) return;
     const DevItem &it = items[item];
+    if (!it.op_owner) return;   // shares the operator of an identical item
     const DevInst &I = inst[it.inst];
     constexpr bool BF = sizeof(AT) == 2;
     constexpr int CW = BF ? 2 : 1;

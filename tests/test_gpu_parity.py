@@ -127,6 +127,26 @@ def test_one_instance_per_launch_equals_batched(ocrt, golden_descs):
         assert np.array_equal(a[f], b[f]), f
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_shared_operators_equal_separate_solves(ocrt, precision):
+    """Bands of one geometry have equal ring, sun and phase-matrix expansion, so
+    a batch solves them with one basis, one set of phase tables and one
+    operator per mode; every result equals the instance solved alone, bit for
+    bit (both forms), and the order counts match."""
+    from ocrt_amd import synthetic as S
+    descs, keep = S.batch_config(n_geom=3, seed=17, layers=(40, 90), streams=(8, 20))
+    # a lone instance between the groups, with another phase function
+    lone, keep2 = S.batch_config(n_geom=1, bands=(555.0,), seed=5, layers=(40, 90), streams=(8, 20))
+    descs = descs[:8] + lone + descs[8:]
+    batch = ocrt.solve(descs, precision=precision, modes=True)
+    for i, d in enumerate(descs):
+        one = ocrt.solve([d], precision=precision, modes=True)
+        for f in ("rho_I", "rho_Q", "rho_U", "n_orders_used"):
+            assert np.array_equal(batch[f][i], one[f][0]), (precision, i, f)
+        m = d.m_max + 1
+        assert np.array_equal(batch["toa_modes"][i][:m], one["toa_modes"][0][:m]), (precision, i)
+
+
 def test_mixed_vs_fp64_full_size_batch(ocrt):
     """The whole batched workload (8 bands x 1024 geometries, 16..64 streams,
     50..400 layers, rough ocean, modes 0..16): every instance of the mixed form
