@@ -222,12 +222,14 @@ __global__ void surface_adj(const DevInst *__restrict__ inst, const DevItem *__r
         } else if (w[i] != 0.0) {
             const double pw = W * mu[i] * w[i];
             for (int op = 0; op < n; ++op) {
-                const double *R = P.R + I.off_R + (((long long)m * n + op) * n + i) * 9;
-                const double *dR = P.dR + I.off_R + (((long long)m * n + op) * n + i) * 9;
+                // entry kl of R^m[op][i] at R[kl * n]
+                const double *R = P.R + r_index(I, m, op, i, 0);
+                const double *dR = P.dR + r_index(I, m, op, i, 0);
                 const double b0 = bb[op], b1 = bb[n + op], b2 = bb[2 * n + op];
                 for (int c2 = 0; c2 < 3; ++c2) {
-                    acc[c2] += pw * (b0 * R[c2] + b1 * R[3 + c2] + b2 * R[6 + c2]);
-                    gw += pw * (b0 * dR[c2] + b1 * dR[3 + c2] + b2 * dR[6 + c2]) * Ub[col_dn(n, c2, i)];
+                    acc[c2] += pw * (b0 * R[c2 * n] + b1 * R[(3 + c2) * n] + b2 * R[(6 + c2) * n]);
+                    gw += pw * (b0 * dR[c2 * n] + b1 * dR[(3 + c2) * n] + b2 * dR[(6 + c2) * n]) *
+                          Ub[col_dn(n, c2, i)];
                 }
             }
         }

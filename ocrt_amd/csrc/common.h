@@ -42,7 +42,7 @@ struct DevInst {
     long long off_ring;  // pool_ring: mu[n], w[n]
     long long off_T;     // pool_T (double) / pool_Tf (float): [nt][n]
     long long off_greek; // pool_greek: [4][La+1]
-    long long off_R;     // pool_R: [(mmax+1)][n][n][9]
+    long long off_R;     // pool_R: [(mmax+1)][n incoming][9 entries][n outgoing] (see r_index)
     long long off_Rsun;  // pool_R: [(mmax+1)][n][9]
     long long off_layers;// pool_layers: [4][nt] Rayleigh tau, aerosol tau, aerosol ssa, absorption tau
     long long off_grad;  // pool_grad: gH[L], gX[L], gY[L], gD[nt], gW[1] (adjoint)
@@ -136,6 +136,17 @@ __host__ inline unsigned xcd_item_grid(int items, int X)
 {
     return (unsigned)(((items + kXcdCount - 1) / kXcdCount) * kXcdCount * X);
 }
+
+// Entry kl (row-major 3 x 3) of the rough-surface kernel R^m[o][i] for
+// outgoing node o and incoming node i.  The outgoing node is the fastest
+// index, so threads over the outgoing nodes read consecutive addresses.
+__host__ __device__ inline long long r_index(const DevInst &I, int m, int o, int i, int kl)
+{
+    return I.off_R + (((long long)m * I.n + i) * 9 + kl) * I.n + o;
+}
+
+// Threads of a Cox-Munk Fourier workgroup: two azimuths of the half circle each.
+constexpr int kCoxMunkThreads = 256;
 
 // Rows of the phase tables computed by one workgroup.
 constexpr int kTabRows = 8;

@@ -211,6 +211,101 @@ __device__ inline void coxmunk_R_core(double mu_out, double mu_in_dn, double cph
         }
 }
 
+// coxmunk_R_core for one (outgoing, incoming) pair of zenith cosines at many
+// azimuths.  Everything independent of the azimuth is formed once; the two
+// meridian/facet-plane rotations enter through cos 2i and sin 2i taken as
+// (x^2 - y^2) / (x^2 + y^2) and 2 x y / (x^2 + y^2) of the unnormalised
+// direction components, which is the normalise-then-double-angle step of
+// coxmunk_R_core without its square roots and divisions; the Fresnel
+// amplitudes share one reciprocal, and R = L(i2) . MF . L(i1) . brdf is
+// written out with the zeros of MF and L dropped.  Same branches, limits and
+// thresholds as coxmunk_R_core; the results agree to rounding.
+struct CoxMunkPair {
+    double mo, mia, s1, s2, mm, ss, msum, inv_s1, inv_s2, ps_pref, inv_s2v, fac, nw, inv_nw2;
+    int qc;
+
+    __device__ CoxMunkPair(double mu_out, double mu_in_dn, double s2v, double shadow, double n_water, int q_conv)
+    {
+        mo = mu_out;
+        mia = -mu_in_dn;
+        s1 = sqrt(fmax(0.0, 1.0 - mu_out * mu_out));
+        s2 = sqrt(fmax(0.0, 1.0 - mia * mia));
+        mm = mu_out * mia;
+        ss = s1 * s2;
+        msum = mu_out + mu_in_dn;
+        inv_s1 = 1.0 / fmax(1e-12, s1);
+        inv_s2 = 1.0 / fmax(1e-12, s2);
+        ps_pref = 1.0 / (OCRT_PI * s2v);
+        inv_s2v = 1.0 / s2v;
+        fac = shadow / (4.0 * mu_in_dn * mu_out);
+        nw = n_water;
+        inv_nw2 = (1.0 / n_water) * (1.0 / n_water);
+        qc = q_conv;
+    }
+
+    // cos 2i, sin 2i of one rotation from the unnormalised (x, y)
+    __device__ static void double_angle(double x, double y, double &c, double &s)
+    {
+        const double qq = x * x + y * y;
+        if (qq < 1e-24) { c = 1.0; s = 0.0; return; }
+        const double iq = 1.0 / qq;
+        c = (x * x - y * y) * iq;
+        s = 2.0 * x * y * iq;
+    }
+
+    __device__ void eval(double cphi, double sphi, double *R, double &tb2) const
+    {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) R[i] = 0.0;
+        tb2 = -1.0;
+        double cT = mm + ss * cphi;
+        cT = fmin(1.0, fmax(-1.0, cT));
+        const double co = sqrt(fmax(0.0, 0.5 * (1.0 - cT)));
+        if (co <= 1e-6) return;
+        const double cb = msum / (2.0 * co);
+        if (cb <= 1e-6) return;
+        const double rden = 1.0 / fmax(1e-12, sqrt(fmax(0.0, 1.0 - cT * cT)));
+        double c1, sn1, c2, sn2;
+        if (s2 <= 1.0e-8) {
+            const double ci = -copysign(1.0, mia) * cphi;
+            c1 = ci * ci - sphi * sphi; sn1 = 2.0 * ci * sphi;
+        } else {
+            double_angle((mo - mia * cT) * rden * inv_s2, s1 * sphi * rden, c1, sn1);
+        }
+        if (s1 <= 1.0e-8) {
+            const double ci = -copysign(1.0, mo) * cphi;
+            c2 = ci * ci - sphi * sphi; sn2 = 2.0 * ci * sphi;
+        } else {
+            double_angle((mia - mo * cT) * rden * inv_s1, s2 * sphi * rden, c2, sn2);
+        }
+        // Fresnel reflection at the facet (fresnel_R at mu = co)
+        double A = 1.0, q = 0.0, m8 = 1.0;
+        const double st2 = inv_nw2 * fmax(0.0, 1.0 - co * co);
+        if (st2 < 1.0) {
+            const double mt = sqrt(1.0 - st2);
+            const double a = co - nw * mt, b = co + nw * mt, c = nw * co - mt, d = nw * co + mt;
+            const double inv = 1.0 / (b * d);
+            const double rs = a * d * inv, rp = c * b * inv;
+            const double Rs = rs * rs, Rp = rp * rp;
+            A = 0.5 * (Rs + Rp);
+            q = (qc == 1) ? 0.5 * (Rp - Rs) : 0.5 * (Rs - Rp);
+            m8 = rs * rp;
+        }
+        const double cb2 = cb * cb;
+        tb2 = (1.0 - cb2) / fmax(cb2, 1e-8);
+        const double brdf = ps_pref * exp(-tb2 * inv_s2v) * fac / (cb2 * cb2);
+        R[0] = A * brdf;
+        R[1] = q * c1 * brdf;
+        R[2] = -q * sn1 * brdf;
+        R[3] = c2 * q * brdf;
+        R[4] = (c2 * A * c1 - sn2 * m8 * sn1) * brdf;
+        R[5] = (-c2 * A * sn1 - sn2 * m8 * c1) * brdf;
+        R[6] = sn2 * q * brdf;
+        R[7] = (sn2 * A * c1 + c2 * m8 * sn1) * brdf;
+        R[8] = (-sn2 * A * sn1 + c2 * m8 * c1) * brdf;
+    }
+};
+
 __device__ inline void coxmunk_R(double mu_out, double mu_in_dn, double cphi, double sphi, double ws,
                                  int sigma_type, double nw, int qc, double *R)
 {

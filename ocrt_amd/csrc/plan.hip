@@ -22,7 +22,8 @@ __global__ void medium_fractions(const DevInst *, Pools);
 __global__ void medium_gas(const DevInst *, Pools);
 __global__ void medium_transmit(const DevInst *, Pools);
 __global__ void fourier_weights(double *, int);
-template <bool DERIV> __global__ void coxmunk_fourier(const DevInst *, Pools);
+template <bool DERIV>
+__global__ void coxmunk_fourier(const DevInst *, Pools, const int *, const int *, int, int);
 __global__ void angular_basis(const DevInst *, const DevItem *, Pools);
 __global__ void phase_tables(const DevInst *, const DevItem *, Pools, int, int);
 template <typename AT, bool TRANS>
@@ -142,6 +143,10 @@ struct Chunk {
     DevItem *ditem = nullptr;
     GemmTile *dtiles = nullptr;
     ColTile *dcols = nullptr;
+    // rough-surface kernel owners of the chunk and the prefix sums of their
+    // (outgoing, incoming-or-sun) pair counts: one Cox-Munk workgroup per pair
+    std::vector<int> hcm_owner, hcm_first;
+    int *dcm_owner = nullptr, *dcm_first = nullptr;
     IpssJob *djobs = nullptr;
     int *djob_of = nullptr;
     int max_L = 0, max_P = 0, max_pairs = 0, max_tab = 0, tab_groups = 1, maxit = 1, any_rough = 0, n_jobs = 0;
@@ -528,6 +533,9 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                     I.off_Rsun = R + (long long)(I.mmax + 1) * I.n * I.n * 9;
                     R += h.R_n;
                     I.surf_owner = 1;
+                    if (c.hcm_first.empty()) c.hcm_first.push_back(0);
+                    c.hcm_owner.push_back(ii - c.i0);
+                    c.hcm_first.push_back(c.hcm_first.back() + I.n * I.n + I.n);
                 }
             }
             I.item0 = p->item_of_inst[ii] - c.it0;
@@ -633,7 +641,9 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     for (const ocrt_gas *g : gas_used) hgas.push_back(pack_gas(*g, hgas_pool));
     rc |= dmalloc(p, (void **)&p->gas_tab, std::max<size_t>(hgas.size(), 1) * sizeof(DevGas));
     rc |= dmalloc(p, (void **)&p->gas_pool, std::max<size_t>(hgas_pool.size(), 1) * 8);
-    rc |= dmalloc(p, (void **)&p->cm_w, (size_t)(p->mmax_all + 1) * 2 * kNPhi * 8);
+    // cos / sin (m phi) for m = 0 .. max(mmax, 1): the m = 1 row also gives the
+    // azimuths of the Cox-Munk quadrature
+    rc |= dmalloc(p, (void **)&p->cm_w, (size_t)(std::max(p->mmax_all, 1) + 1) * 2 * kNPhi * 8);
     rc |= dmalloc(p, (void **)&p->state, (size_t)p->n_items * sizeof(ItemState));
     rc |= dmalloc(p, (void **)&p->iout, (size_t)p->n_items * sizeof(ItemOut));
     rc |= dmalloc(p, (void **)&p->ipss, (size_t)n_inst * sizeof(InstOut));
@@ -719,7 +729,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                       al(std::max<size_t>(c.htiles.size(), 1) * sizeof(GemmTile)) +
                       al(std::max<size_t>(c.hcols.size(), 1) * sizeof(ColTile)) +
                       al(std::max<size_t>(c.hjobs.size(), 1) * sizeof(IpssJob)) +
-                      al((size_t)(c.i1 - c.i0) * sizeof(int));
+                      al((size_t)(c.i1 - c.i0) * sizeof(int)) + al(std::max<size_t>(c.hcm_owner.size(), 1) * 4) +
+                      al(std::max<size_t>(c.hcm_first.size(), 1) * 4);
     std::vector<unsigned char> hdesc(desc_bytes, 0);
     unsigned char *ddesc = nullptr;
     if (dmalloc(p, (void **)&ddesc, desc_bytes)) { ocrt_plan_destroy(p); return OCRT_ERR_ALLOC; }
@@ -742,13 +753,15 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         c.djobs = (IpssJob *)put(c.hjobs.data(), c.hjobs.size() * sizeof(IpssJob),
                                  std::max<size_t>(c.hjobs.size(), 1) * sizeof(IpssJob));
         c.djob_of = (int *)put(job_of.data(), job_of.size() * sizeof(int), job_of.size() * sizeof(int));
+        c.dcm_owner = (int *)put(c.hcm_owner.data(), c.hcm_owner.size() * 4, std::max<size_t>(c.hcm_owner.size(), 1) * 4);
+        c.dcm_first = (int *)put(c.hcm_first.data(), c.hcm_first.size() * 4, std::max<size_t>(c.hcm_first.size(), 1) * 4);
     }
     if (hipMemcpy(ddesc, hdesc.data(), desc_bytes, hipMemcpyHostToDevice) != hipSuccess) {
         ocrt_plan_destroy(p);
         return OCRT_ERR_DEVICE;
     }
-    hipLaunchKernelGGL(fourier_weights, dim3((kNPhi + 255) / 256, p->mmax_all + 1), dim3(256), 0, 0, p->cm_w,
-                       p->mmax_all);
+    const int cm_rows = std::max(p->mmax_all, 1);
+    hipLaunchKernelGGL(fourier_weights, dim3((kNPhi + 255) / 256, cm_rows + 1), dim3(256), 0, 0, p->cm_w, cm_rows);
     if (hipDeviceSynchronize() != hipSuccess) { ocrt_plan_destroy(p); return OCRT_ERR_DEVICE; }
     *out = p;
     return OCRT_OK;
@@ -801,11 +814,20 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     hipLaunchKernelGGL(medium_fractions, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
     if (c.any_gas) hipLaunchKernelGGL(medium_gas, dim3(ni), dim3(256), 0, s, c.dinst, P);
     hipLaunchKernelGGL(medium_transmit, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
-    if (c.any_rough) {
-        if (p->adjoint)
-            hipLaunchKernelGGL(coxmunk_fourier<true>, dim3(c.max_pairs, ni), dim3(256), 0, s, c.dinst, P);
-        else
-            hipLaunchKernelGGL(coxmunk_fourier<false>, dim3(c.max_pairs, ni), dim3(256), 0, s, c.dinst, P);
+    if (!c.hcm_owner.empty()) {
+        // one workgroup per pair, in launches of at most 2^22 workgroups (far
+        // below 2^31 work-items each)
+        const int total = c.hcm_first.back(), no = (int)c.hcm_owner.size();
+        const dim3 blk(kCoxMunkThreads);
+        for (int base = 0; base < total; base += 1 << 22) {
+            const dim3 grid((unsigned)std::min(total - base, 1 << 22));
+            if (p->adjoint)
+                hipLaunchKernelGGL(coxmunk_fourier<true>, grid, blk, 0, s, c.dinst, P, c.dcm_owner, c.dcm_first, no,
+                                   base);
+            else
+                hipLaunchKernelGGL(coxmunk_fourier<false>, grid, blk, 0, s, c.dinst, P, c.dcm_owner, c.dcm_first, no,
+                                   base);
+        }
     }
     hipLaunchKernelGGL(angular_basis, dim3(nit), dim3(128), 0, s, c.dinst, c.ditem, P);
     hipLaunchKernelGGL(phase_tables, dim3(xcd_item_grid(nit, c.tab_groups)), dim3(64), 0, s, c.dinst, c.ditem, P,

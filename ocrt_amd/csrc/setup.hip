@@ -9,6 +9,8 @@
 //   phase_tables    : phase-matrix Fourier tables (rt_angular_basis.c:278-358)
 //   pack_operator   : dense scattering operator with quadrature weights and
 //       the Stokes sign pattern folded in (rt_sos_operator.c:50-126, 353-407)
+#include <type_traits>
+
 #include "common.h"
 #include "device_math.h"
 #include "wave.h"
@@ -250,7 +252,8 @@ __global__ void fourier_weights(double *w, int mmax)
 
 // ---------------------------------------------------------------------------
 // Rough-ocean Fourier kernels.  One workgroup per (instance, outgoing node,
-// incoming node or the sun).  The isotropic-slope BRDF is even in the
+// incoming node or the sun), over the pairs of the instances that own their
+// kernels only.  The isotropic-slope BRDF is even in the
 // relative azimuth for the II IQ QI QQ UU entries and odd for the others, so
 // the midpoint rule over [0, 2 pi) folds onto its first half (phi_p and
 // 2 pi - phi_p pair up) and only kNPhi / 2 azimuths are evaluated.  The
@@ -262,25 +265,22 @@ __global__ void fourier_weights(double *w, int mmax)
 // adjoint: every entry is proportional to the brdf, so its derivative is the
 // entry times d ln(brdf) / d sigma^2.
 //
-// Thread t evaluates the Mueller matrix at azimuths p = t and t + 256 and keeps
-// both in registers.  The Fourier sums are then taken in batches of 8 rows
-// (8 modes, or 4 modes x {value, derivative}) x 9 entries: every thread reads
-// its two samples of the cos / sin rows of the batch (consecutive azimuths
-// over the lanes), forms its products, and a transposed wavefront reduction
-// leaves each row's totals in one group of eight lanes; the four wavefronts'
-// totals are added through LDS.
+// Thread t of the 256 evaluates the Mueller matrix at azimuths t and t + 256
+// and keeps both in registers.  The Fourier sums are then taken in batches of
+// 8 rows (8 modes, or 4 modes x {value, derivative}) x 9 entries: every thread
+// reads its two samples of the cos / sin rows of the batch (consecutive
+// azimuths over the lanes), forms its products, and transposed wavefront
+// reductions (three entries at a time) leave each row's totals in one group of
+// eight lanes; the four wavefronts' totals are added through LDS.
 template <bool DERIV>
-__global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict__ inst, Pools P)
+__device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
 {
     constexpr int kHalf = kNPhi / 2;
-    static_assert(kHalf == 2 * 256, "two azimuths per thread");
+    static_assert(kHalf == 2 * kCoxMunkThreads, "two azimuths per thread");
+    constexpr int kWaves = kCoxMunkThreads / 64;
     constexpr int kRowModes = DERIV ? 4 : 8;   // Fourier modes per batch
-    __shared__ double red[4][8][9];
-    const DevInst &I = inst[blockIdx.y];
-    if (I.surf != 2 || !I.surf_owner) return;
+    __shared__ double red[kWaves][8][9];
     const int n = I.n;
-    const int pair = blockIdx.x;
-    if (pair >= n * n + n) return;
     const double *mu = P.ring + I.off_ring;
     const double *w = mu + n;
     int o, i;
@@ -292,8 +292,8 @@ __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict
             const int nout = (I.mmax + 1) * 9;
             for (int t = threadIdx.x; t < nout; t += blockDim.x) {
                 const int m = t / 9, kl = t % 9;
-                P.R[I.off_R + (((long long)m * n + o) * n + i) * 9 + kl] = 0.0;
-                if (DERIV) P.dR[I.off_R + (((long long)m * n + o) * n + i) * 9 + kl] = 0.0;
+                P.R[r_index(I, m, o, i, kl)] = 0.0;
+                if (DERIV) P.dR[r_index(I, m, o, i, kl)] = 0.0;
             }
             return;
         }
@@ -307,48 +307,62 @@ __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict
     const double s2v = slope_variance(I.wind, I.sigma_type);
     const double shadow = sancer_shadow(mo, mi, s2v);
     const double dls = DERIV ? sancer_dlog(mo, mi, s2v, shadow) : 0.0;
+    const CoxMunkPair cm(mo, mi, s2v, shadow, I.n_water, I.q_conv);
     double r[2][9], dl[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        const double phi = (t + 256 * q + 0.5) * dphi;
+        // azimuth (p + 1/2) dphi: its cosine and sine are the m = 1 row of the
+        // Fourier weight table
+        const int p = t + kCoxMunkThreads * q;
         double tb2;
-        coxmunk_R_core(mo, mi, cos(phi), sin(phi), s2v, shadow, I.n_water, I.q_conv, r[q], &tb2);
+        cm.eval(P.cm_w[2 * kNPhi + p], P.cm_w[3 * kNPhi + p], r[q], tb2);
         dl[q] = (DERIV && tb2 >= 0.0) ? tb2 / (s2v * s2v) - 1.0 / s2v + dls : 0.0;
     }
     const int M = I.mmax;
-    for (int m0 = 0; m0 <= M; m0 += kRowModes) {
-        double v[8 * 9];   // [row][entry]
+    // one reduction over the entries [E0, E0 + C) of a batch's eight rows
+    auto reduce_entries = [&](int m0, auto e0, auto cnt) {
+        constexpr int E0 = decltype(e0)::value, C = decltype(cnt)::value;
+        double v[8 * C];   // [row][entry - E0]
 #pragma unroll
         for (int j = 0; j < kRowModes; ++j) {
             const int m = m0 + j <= M ? m0 + j : M;   // rows past the last mode are computed, not written
             const double *wc = P.cm_w + (long long)m * 2 * kNPhi, *ws = wc + kNPhi;
-            const double c0 = wc[t], c1 = wc[t + 256], s0 = ws[t], s1 = ws[t + 256];
+            const int t1 = t + kCoxMunkThreads;
+            const double c0 = wc[t], c1 = wc[t1], s0 = ws[t], s1 = ws[t1];
 #pragma unroll
-            for (int kl = 0; kl < 9; ++kl) {
+            for (int e = 0; e < C; ++e) {
+                const int kl = E0 + e;
                 const bool use_cos = (kl == 0 || kl == 1 || kl == 3 || kl == 4 || kl == 8);
                 const double a0 = r[0][kl] * (use_cos ? c0 : s0), a1 = r[1][kl] * (use_cos ? c1 : s1);
                 if (DERIV) {
-                    v[(2 * j) * 9 + kl] = a0 + a1;
-                    v[(2 * j + 1) * 9 + kl] = a0 * dl[0] + a1 * dl[1];
+                    v[(2 * j) * C + e] = a0 + a1;
+                    v[(2 * j + 1) * C + e] = a0 * dl[0] + a1 * dl[1];
                 } else {
-                    v[j * 9 + kl] = a0 + a1;
+                    v[j * C + e] = a0 + a1;
                 }
             }
         }
-        wave_sum_8xC<9>(v, lane);
+        wave_sum_8xC<C>(v, lane);
         if ((lane & 7) == 0)
 #pragma unroll
-            for (int kl = 0; kl < 9; ++kl) red[wave][lane >> 3][kl] = v[kl];
+            for (int e = 0; e < C; ++e) red[wave][lane >> 3][E0 + e] = v[e];
+    };
+    for (int m0 = 0; m0 <= M; m0 += kRowModes) {
+        reduce_entries(m0, std::integral_constant<int, 0>{}, std::integral_constant<int, 3>{});
+        reduce_entries(m0, std::integral_constant<int, 3>{}, std::integral_constant<int, 3>{});
+        reduce_entries(m0, std::integral_constant<int, 6>{}, std::integral_constant<int, 3>{});
         __syncthreads();
         if (t < 72) {
             const int row = t / 9, kl = t % 9;
-            const double tot = (red[0][row][kl] + red[1][row][kl]) + (red[2][row][kl] + red[3][row][kl]);
+            double tot = 0.0;
+#pragma unroll
+            for (int wv = 0; wv < kWaves; ++wv) tot += red[wv][row][kl];
             const int m = m0 + (DERIV ? row >> 1 : row);
             if (m <= M) {
                 const double nrm = (m == 0) ? 1.0 / (2.0 * OCRT_PI) : 1.0 / OCRT_PI;
                 double val = 2.0 * tot * (dphi * nrm);
                 if (m > 0 && (kl == 2 || kl == 5)) val = -val;
-                const long long at = !solar ? I.off_R + (((long long)m * n + o) * n + i) * 9 + kl
+                const long long at = !solar ? r_index(I, m, o, i, kl)
                                             : I.off_Rsun + ((long long)m * n + o) * 9 + kl;
                 if (DERIV && (row & 1)) P.dR[at] = val;
                 else P.R[at] = val;
@@ -358,8 +372,27 @@ __global__ __launch_bounds__(256) void coxmunk_fourier(const DevInst *__restrict
     }
 }
 
-template __global__ void coxmunk_fourier<false>(const DevInst *, Pools);
-template __global__ void coxmunk_fourier<true>(const DevInst *, Pools);
+template <bool DERIV>
+// three workgroups per CU: the register budget that keeps the Mueller
+// evaluations free of spills (measured faster than two or four)
+__global__ __launch_bounds__(kCoxMunkThreads, 3) void coxmunk_fourier(const DevInst *__restrict__ inst, Pools P,
+                                                                   const int *__restrict__ owners,
+                                                                   const int *__restrict__ first, int n_owners,
+                                                                   int pair_base)
+{
+    // pair b of the chunk belongs to owner k with first[k] <= b < first[k + 1]
+    const int b = pair_base + blockIdx.x;
+    if (b >= first[n_owners]) return;
+    int lo = 0, hi = n_owners;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (first[mid] <= b) lo = mid; else hi = mid;
+    }
+    coxmunk_pair<DERIV>(inst[owners[lo]], P, b - first[lo]);
+}
+
+template __global__ void coxmunk_fourier<false>(const DevInst *, Pools, const int *, const int *, int, int);
+template __global__ void coxmunk_fourier<true>(const DevInst *, Pools, const int *, const int *, int, int);
 
 // ---------------------------------------------------------------------------
 // Signed direction cosine of direction index d = j + n, j in [-n, n]; j = 0 is

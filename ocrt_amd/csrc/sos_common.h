@@ -118,9 +118,9 @@ __device__ inline void surface_bc_item(const DevInst &I, const DevItem &it, cons
     const double *mu = P.ring + I.off_ring, *w = mu + n;
     double *bc = P.bc + it.off_bc;
     const double W = (I.surf == 1) ? 2.0 * OCRT_PI : ((m == 0) ? 2.0 * OCRT_PI : OCRT_PI);
-    for (int o = threadIdx.x; o < n; o += blockDim.x) {
-        double si = 0.0, sq = 0.0, su = 0.0;
-        if (I.surf == 1) {
+    if (I.surf == 1) {
+        for (int o = threadIdx.x; o < n; o += blockDim.x) {
+            double si, sq, su;
             double Fr[9];
             fresnel_R(mu[o], I.n_water, I.q_conv, Fr);
             const double a = bot[o], b = bot[n + o], cc = bot[2 * n + o];
@@ -137,17 +137,24 @@ __device__ inline void surface_bc_item(const DevInst &I, const DevItem &it, cons
                 sq = pw * (R[3] * a + R[4] * b + R[5] * cc);
                 su = pw * (R[6] * a + R[7] * b + R[8] * cc);
             }
-        } else {
-            const double *R0 = P.R + I.off_R + ((long long)m * n + o) * n * 9;
-            for (int i = 0; i < n; ++i) {
-                if (w[i] == 0.0) continue;
-                const double a = bot[i], b = bot[n + i], cc = bot[2 * n + i];
-                const double pw = W * mu[i] * w[i];
-                const double *R = R0 + (long long)i * 9;
-                si += pw * (R[0] * a + R[1] * b + R[2] * cc);
-                sq += pw * (R[3] * a + R[4] * b + R[5] * cc);
-                su += pw * (R[6] * a + R[7] * b + R[8] * cc);
-            }
+            bc[o] = si;
+            bc[n + o] = sq;
+            bc[2 * n + o] = su;
+        }
+        return;
+    }
+    // rough surface, threads over the outgoing nodes: entry kl of R^m[o][i]
+    // sits at R[kl n] from the node's own base, consecutive over the lanes
+    for (int o = threadIdx.x; o < n; o += blockDim.x) {
+        double si = 0.0, sq = 0.0, su = 0.0;
+        for (int i = 0; i < n; ++i) {
+            if (w[i] == 0.0) continue;
+            const double a = bot[i], b = bot[n + i], cc = bot[2 * n + i];
+            const double pw = W * mu[i] * w[i];
+            const double *R = P.R + r_index(I, m, o, i, 0);
+            si += pw * (R[0] * a + R[n] * b + R[2 * n] * cc);
+            sq += pw * (R[3 * n] * a + R[4 * n] * b + R[5 * n] * cc);
+            su += pw * (R[6 * n] * a + R[7 * n] * b + R[8 * n] * cc);
         }
         bc[o] = si;
         bc[n + o] = sq;
