@@ -184,10 +184,15 @@ enum {
 };
 
 /* Launch shape chosen for the mixed-form sweep (levels per block, wavefronts
- * per workgroup) and, after autotuning, the summed sweep time of every shape
- * tried (ms_per_shape[n_shapes], may be NULL). */
+ * per workgroup) and, after autotuning, the summed order-loop time
+ * (scattering integral and sweep) of every shape tried (ms_per_shape[n_shapes],
+ * may be NULL).  All shapes give the same bits. */
 int ocrt_plan_tuning(const ocrt_plan *plan, int *sweep_unroll, int *sweep_waves, double *ms_per_shape,
                      int *n_shapes);
+/* Selects the mixed-form sweep's launch shape by hand (unroll 4, 8 or 16 with
+ * waves 2 or 4); it replaces autotuning.  No counterpart in the reference.
+ * -1 for a pair that is not a shape. */
+int ocrt_plan_set_tuning(ocrt_plan *plan, int sweep_unroll, int sweep_waves);
 
 /* ocrt_plan_create with option flags.  No counterpart in the reference; the
  * adjoint is the Jacobian path of the batched solver (ocrt_adjoint_batch). */

@@ -72,6 +72,7 @@ def library():
         lib.ocrt_gas_destroy.restype = None
         lib.ocrt_plan_lut_extras.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_plan_tuning.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ocrt_plan_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.ocrt_plan_residuals.argtypes = [C.c_void_p, C.c_void_p]
         _lib = lib
     return _lib
@@ -138,9 +139,15 @@ class Plan:
             raise OcrtError(f"ocrt_plan_create failed with code {rc}")
         self.mmax = self.lib.ocrt_plan_mmax(self._handle)
 
+    def set_tuning(self, unroll, waves):
+        """Select the mixed-form sweep's launch shape by hand."""
+        rc = self.lib.ocrt_plan_set_tuning(self._handle, int(unroll), int(waves))
+        if rc != 0:
+            raise OcrtError(f"no launch shape unroll {unroll} waves {waves}")
+
     def tuning(self):
         """(levels per block, wavefronts per workgroup) of the mixed-form sweep
-        and, after autotuning, the summed sweep ms of every shape tried."""
+        and, after autotuning, the summed order-loop ms of every shape tried."""
         u, w, n = C.c_int(), C.c_int(), C.c_int()
         ms = (C.c_double * 16)()
         self.lib.ocrt_plan_tuning(self._handle, C.byref(u), C.byref(w), ms, C.byref(n))

@@ -10,10 +10,8 @@
 //   per item     : one (instance, Fourier mode m) pair.  Angular basis
 //                  plm/rrl/rtl [(lmax+1)][2n+1], six phase tables
 //                  [(n+1)][2n+1], the dense scattering operator A[Mp][K]
-//                  and the fields F (previous order), J (source) and the
-//                  order sum: fp64 in the fp64 form; in the mixed form fp32
-//                  over the whole field, with the two output rows (TOA, BOA)
-//                  also kept in fp64 plus an fp32 compensation term.
+//                  and the fields F (previous order), J (source), total
+//                  (fp64 order sum) and comp (compensation, mixed form).
 //   Field row k (level) holds 6n state columns ordered
 //     [I_up(n) Q_up(n) U_up(n) I_dn(n) Q_dn(n) U_dn(n)], node o = 0..n-1
 //   padded to the row pitch P; pad columns stay zero.
@@ -75,7 +73,7 @@ struct DevItem {
     long long off_F;      // pool_F / pool_J: [L][P]
     long long off_tot;    // pool_tot: [L][P]
     long long off_bc;     // pool_bc: [3][n] boundary condition, then [3][n] bottom downward field
-    long long off_cmp;    // pool_comp, pool_tend: [2][P] order sum (mixed form) at k = 0 and k = nt
+    long long off_cmp;    // pool_comp: [2][P] compensation of the order sum at k = 0 and k = nt
     long long off_Flo;    // pool_Flo: [L][P] low half of the bf16 field pair (fsplit items)
     long long off_nodes;  // pool_nodes (LUT plans, global over the plan): [4][n]
 };
@@ -107,8 +105,6 @@ struct Pools {
     double *grad;         // adjoint: per-instance accumulators (see DevInst::off_grad)
     const double *gout;  // adjoint: upstream gradient [instance][3] of (rho_I, rho_Q, rho_U)
     float *comp;          // fp32 compensation of the mixed-form order sum on the two output rows
-    double *tend;         // mixed form: fp64 order sum on the two output rows, [2][P] per item like comp
-    float *tot32;         // mixed form: fp32 order sum [L][P] (interior rows: convergence test only)
     float *Tf;            // fp32 1 - T for the mixed-form sweeps
     float *idt;           // fp32 1 / dtau per layer, indexed like lev (mixed-form sweeps)
     void *A, *F, *J;      // double or bf16/float depending on the form

@@ -470,8 +470,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 if (!shared_op)
                     b += 3.0 * (I.lmax + 1) * (2 * n + 1) * 8 + 6.0 * (n + 1) * (2 * n + 1) * 8 +
                          (double)Mp * K * ae * (mixed ? 2 : 1) * (adjoint ? 2 : 1);
-                b += (double)L * P * (fe + je + (mixed ? 4 : 8)) +
-                     (mixed ? 24.0 * P : 0.0) + ((mixed && m < kSplitModes) ? 2.0 * L * P : 0.0) + 6.0 * n * 8;
+                b += (double)L * P * (fe + je + 8) + (mixed ? 8.0 * P : 0.0) + ((mixed && m < kSplitModes) ? 2.0 * L * P : 0.0) + 6.0 * n * 8;
                 if (adjoint)   // checkpoints, segment fields, lambda, second parts
                     b += (double)L * P * 8 * ((I.max_orders - 1) / every + 1 + (every - 1) + 2);
             }
@@ -661,13 +660,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     rc |= dmalloc(p, (void **)&P.A, mx_A * ae);
     rc |= dmalloc(p, (void **)&P.F, mx_F * fe);
     rc |= dmalloc(p, (void **)&P.J, mx_F * je);
-    if (mixed) {
-        rc |= dmalloc(p, (void **)&P.tot32, mx_F * 4);
-        rc |= dmalloc(p, (void **)&P.comp, std::max(mx_cmp, 1LL) * 4);
-        rc |= dmalloc(p, (void **)&P.tend, std::max(mx_cmp, 1LL) * 8);
-    } else {
-        rc |= dmalloc(p, (void **)&P.tot, mx_F * 8);
-    }
+    rc |= dmalloc(p, (void **)&P.tot, mx_F * 8);
+    if (mixed) rc |= dmalloc(p, (void **)&P.comp, std::max(mx_cmp, 1LL) * 4);
     if (mixed) rc |= dmalloc(p, (void **)&P.Flo, std::max(mx_flo, 1LL) * 2);
     rc |= dmalloc(p, (void **)&P.bc, mx_bc * 8);
     if (adjoint) {
@@ -777,6 +771,8 @@ static void drop_graph(ocrt_plan *p);
 
 // Launch shapes of the mixed-form sweep the autotuner chooses from: levels
 // per block U and wavefronts per workgroup W (all give the same bits).
+// Launch shapes of the mixed-form sweep: levels per load block U and
+// wavefronts per workgroup W.
 struct SweepShape { int U, W; };
 constexpr SweepShape kSweepShapes[] = {{8, 4}, {4, 4}, {16, 4}, {8, 2}, {4, 2}, {16, 2}};
 constexpr int kSweepShapeCount = sizeof(kSweepShapes) / sizeof(kSweepShapes[0]);
@@ -1104,10 +1100,10 @@ static int execute_eager(ocrt_plan *p, hipStream_t s)
     return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
 }
 
-// Autotuner: one warm-up execute, then one execute per sweep shape with the
-// sweeps bracketed by events; the shape with the least summed sweep time is
-// kept.  Every shape computes the same bits, so the results of the plan do
-// not depend on the choice.
+// Autotuner: one warm-up execute, then one execute per sweep launch shape
+// with the order loop's kernels bracketed by events; the shape with the least
+// summed time is kept.  Every shape computes the same bits, so the
+// results of the plan do not depend on the choice.
 static int autotune(ocrt_plan *p, hipStream_t s)
 {
     const int saved = p->profile;
@@ -1120,17 +1116,18 @@ static int autotune(ocrt_plan *p, hipStream_t s)
     for (int v = 0; v < kSweepShapeCount && !rc; ++v) {
         p->sweep_variant = v;
         rc = execute_eager(p, s);
-        double sweep_ms = 0.0;
-        if (!rc) rc = ocrt_plan_kernel_times(p, nullptr, nullptr, &sweep_ms, nullptr);
-        p->tune_ms[v] = sweep_ms;
-        if (sweep_ms < p->tune_ms[best]) best = v;
+        // the whole order loop: scattering integral and sweep
+        double gemm_ms = 0.0, sweep_ms = 0.0;
+        if (!rc) rc = ocrt_plan_kernel_times(p, &gemm_ms, nullptr, &sweep_ms, nullptr);
+        p->tune_ms[v] = gemm_ms + sweep_ms;
+        if (p->tune_ms[v] < p->tune_ms[best]) best = v;
     }
     p->profile = saved;
     p->sweep_variant = best;
     p->tuned = 1;
     if (getenv("OCRT_DEBUG_PLAN"))
         for (int v = 0; v < kSweepShapeCount; ++v)
-            fprintf(stderr, "ocrt autotune: sweep U %d W %d: %.3f ms%s\n", kSweepShapes[v].U, kSweepShapes[v].W,
+            fprintf(stderr, "ocrt autotune: order loop U %d W %d: %.3f ms%s\n", kSweepShapes[v].U, kSweepShapes[v].W,
                     p->tune_ms[v], v == best ? " (chosen)" : "");
     return rc;
 }
@@ -1188,6 +1185,19 @@ extern "C" int ocrt_plan_execute(ocrt_plan *p, void *stream)
         return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
     }
     return execute_eager(p, s);
+}
+
+extern "C" int ocrt_plan_set_tuning(ocrt_plan *p, int sweep_unroll, int sweep_waves)
+{
+    if (!p) return OCRT_ERR_ARGS;
+    for (int v = 0; v < kSweepShapeCount; ++v)
+        if (kSweepShapes[v].U == sweep_unroll && kSweepShapes[v].W == sweep_waves) {
+            drop_graph(p);   // a captured graph holds the previous shape's launches
+            p->sweep_variant = v;
+            p->tuned = 1;
+            return OCRT_OK;
+        }
+    return OCRT_ERR_ARGS;
 }
 
 extern "C" int ocrt_plan_tuning(const ocrt_plan *p, int *sweep_unroll, int *sweep_waves, double *ms_per_shape,

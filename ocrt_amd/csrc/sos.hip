@@ -113,8 +113,7 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     }
     if (col >= ct.col_end) return;
     FT *F = reinterpret_cast<FT *>(P.F) + it.off_F;
-    double *tot = MIXED ? nullptr : P.tot + it.off_tot;
-    float *tot32 = MIXED ? P.tot32 + it.off_tot : nullptr;
+    double *tot = P.tot + it.off_tot;
     if (MIXED) {
         P.comp[it.off_cmp + col] = 0.f;
         P.comp[it.off_cmp + Pp + col] = 0.f;
@@ -124,10 +123,8 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
         for (int k = 0; k <= nt; ++k) {
             F[(long long)k * Pp + col] = to_storage<FT>(0.0);
             if (Flo) Flo[(long long)k * Pp + col] = __float2bfloat16(0.f);
-            if (MIXED) tot32[(long long)k * Pp + col] = 0.f;
-            else tot[(long long)k * Pp + col] = 0.0;
+            tot[(long long)k * Pp + col] = 0.0;
         }
-        if (MIXED) P.tend[it.off_cmp + col] = P.tend[it.off_cmp + Pp + col] = 0.0;
         return;
     }
     const LevelView lv(P, I);
@@ -152,14 +149,11 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
                      [&](int k, double v) {
                          if (rough_seed) v += rsun * (FS * (down * exp(-(tau - lv.h[k]) / mu)));
                          const long long ix = (long long)k * Pp + col;
-                         if constexpr (MIXED) {
-                             tot32[ix] = (float)v;
-                             if (k == 0 || k == nt) P.tend[it.off_cmp + (k == 0 ? 0 : Pp) + col] = v;
+                         tot[ix] = v;
+                         if constexpr (MIXED)
                              store_field_pair(reinterpret_cast<__hip_bfloat16 *>(F), Flo, ix, v);
-                         } else {
-                             tot[ix] = v;
+                         else
                              F[ix] = to_storage<FT>(v);
-                         }
                          if (hemi == 1 && k == nt) P.bc[it.off_bc + 3 * n + c * n + o] = v;
                      });
 }
@@ -303,28 +297,28 @@ __device__ inline bf16x8_ lds_frag(const unsigned short *p)
     return __builtin_bit_cast(bf16x8_, *reinterpret_cast<const uint4 *>(p));
 }
 
-__global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ inst,
-                                                 const DevItem *__restrict__ items, Pools P,
-                                                 const GemmTile *__restrict__ tiles,
-                                                 const ItemState *__restrict__ st, int order)
+// LDS staging of one K slab of a tile: [part][hi, lo][row][k] of the operator
+// and [hi, lo][level][k] of the field, rows padded to 40 entries.
+struct GemmLds {
+    unsigned short A[2][2][64][40];
+    unsigned short B[2][kGemmLevels][40];
+};
+
+// One tile of the bf16 scattering integral: operator rows row0 .. row0 + 63
+// and levels lev0 .. lev0 + kGemmLevels - 1 of one item, four waves of 32 x 64.
+// Rows past the operator's Mp rows load its last row (their results are never
+// used).  The fp32 results, scaled by the level fractions, go to
+// out(lev, r, v) for the four rows r .. r + 3 of v, at levels < L only.
+template <typename Out>
+__device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, const unsigned short *A,
+                                      const unsigned short *F, const unsigned short *Flo, int row0, int lev0,
+                                      GemmLds &S, Out out)
 {
-    const GemmTile tl = tiles[blockIdx.x];
-    if (tl.item < 0 || !item_runs(st[tl.item], order)) return;
-    const DevItem &it = items[tl.item];
-    const DevInst &I = inst[it.inst];
     const int Pp = it.P, L = it.L, Mp = it.Mp;
     const int nparts = it.K / Pp;
-    const unsigned short *A = reinterpret_cast<const unsigned short *>(P.A) + it.off_A;
-    const unsigned short *F = reinterpret_cast<const unsigned short *>(P.F) + it.off_F;
     const bool split = it.fsplit != 0;
-    const unsigned short *Flo = reinterpret_cast<const unsigned short *>(P.Flo) + it.off_Flo;
-    float *J = reinterpret_cast<float *>(P.J) + it.off_F;
-    const LevelView lv(P, I);
-    __shared__ unsigned short As[2][2][64][40];   // [part][hi, lo][row][k], rows padded to 40
-    __shared__ unsigned short Bs[2][kGemmLevels][40];   // [hi, lo][level][k]
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
-    const int row0 = tl.tm * 64, lev0 = tl.tn * kGemmLevels;
     float4_ acc[2][2][4];
 #pragma unroll
     for (int q = 0; q < 2; ++q)
@@ -337,9 +331,10 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
     const int lr = t >> 2, lk = (t & 3) * 8;
     const long long slab = (long long)Mp * Pp;
     const bool two = nparts > 1;
+    const int arow = min(row0 + lr, Mp - 1);
     uint4 ra0h, ra0l, ra1h, ra1l, rb0, rb1, rl0, rl1;
     auto load_slab = [&](int k0) {
-        const long long ro = (long long)(row0 + lr) * Pp + k0 + lk;
+        const long long ro = (long long)arow * Pp + k0 + lk;
         ra0h = *reinterpret_cast<const uint4 *>(A + ro);
         ra0l = *reinterpret_cast<const uint4 *>(A + slab + ro);
         if (two) {
@@ -358,17 +353,17 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
         }
     };
     auto store_slab = [&]() {
-        *reinterpret_cast<uint4 *>(&As[0][0][lr][lk]) = ra0h;
-        *reinterpret_cast<uint4 *>(&As[0][1][lr][lk]) = ra0l;
+        *reinterpret_cast<uint4 *>(&S.A[0][0][lr][lk]) = ra0h;
+        *reinterpret_cast<uint4 *>(&S.A[0][1][lr][lk]) = ra0l;
         if (two) {
-            *reinterpret_cast<uint4 *>(&As[1][0][lr][lk]) = ra1h;
-            *reinterpret_cast<uint4 *>(&As[1][1][lr][lk]) = ra1l;
+            *reinterpret_cast<uint4 *>(&S.A[1][0][lr][lk]) = ra1h;
+            *reinterpret_cast<uint4 *>(&S.A[1][1][lr][lk]) = ra1l;
         }
-        *reinterpret_cast<uint4 *>(&Bs[0][lr][lk]) = rb0;
-        *reinterpret_cast<uint4 *>(&Bs[0][lr + 64][lk]) = rb1;
+        *reinterpret_cast<uint4 *>(&S.B[0][lr][lk]) = rb0;
+        *reinterpret_cast<uint4 *>(&S.B[0][lr + 64][lk]) = rb1;
         if (split) {
-            *reinterpret_cast<uint4 *>(&Bs[1][lr][lk]) = rl0;
-            *reinterpret_cast<uint4 *>(&Bs[1][lr + 64][lk]) = rl1;
+            *reinterpret_cast<uint4 *>(&S.B[1][lr][lk]) = rl0;
+            *reinterpret_cast<uint4 *>(&S.B[1][lr + 64][lk]) = rl1;
         }
     };
     // fragment of a 16 x 32 block: lane l holds row (l & 15), k 8 (l >> 4) .. + 7
@@ -377,13 +372,13 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
         constexpr int q = decltype(part)::value;
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            const bf16x8_ hi = lds_frag(&As[q][0][wm + a * 16 + fr][fk]);
-            const bf16x8_ lo = lds_frag(&As[q][1][wm + a * 16 + fr][fk]);
+            const bf16x8_ hi = lds_frag(&S.A[q][0][wm + a * 16 + fr][fk]);
+            const bf16x8_ lo = lds_frag(&S.A[q][1][wm + a * 16 + fr][fk]);
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const bf16x8_ fb = lds_frag(&Bs[0][wn + b * 16 + fr][fk]);
+                const bf16x8_ fb = lds_frag(&S.B[0][wn + b * 16 + fr][fk]);
                 if (split) {
-                    const bf16x8_ fl = lds_frag(&Bs[1][wn + b * 16 + fr][fk]);
+                    const bf16x8_ fl = lds_frag(&S.B[1][wn + b * 16 + fr][fk]);
                     acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, fl, acc[q][a][b], 0, 0, 0);
                 }
                 acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, fb, acc[q][a][b], 0, 0, 0);
@@ -418,17 +413,37 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             const int r = row0 + wm + a * 16 + 4 * (lane >> 4);
-            if (r >= Pp) continue;
             float4 v;
             v.x = s0 * acc[0][a][b][0]; v.y = s0 * acc[0][a][b][1];
             v.z = s0 * acc[0][a][b][2]; v.w = s0 * acc[0][a][b][3];
             if (two) {
-                v.x += y * acc[1][a][b][0]; v.y += y * acc[1][a][b][1];
-                v.z += y * acc[1][a][b][2]; v.w += y * acc[1][a][b][3];
+                v.x = fmaf(y, acc[1][a][b][0], v.x); v.y = fmaf(y, acc[1][a][b][1], v.y);
+                v.z = fmaf(y, acc[1][a][b][2], v.z); v.w = fmaf(y, acc[1][a][b][3], v.w);
             }
-            *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v;
+            out(lev, r, v);
         }
     }
+}
+
+__global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ inst,
+                                                 const DevItem *__restrict__ items, Pools P,
+                                                 const GemmTile *__restrict__ tiles,
+                                                 const ItemState *__restrict__ st, int order)
+{
+    const GemmTile tl = tiles[blockIdx.x];
+    if (tl.item < 0 || !item_runs(st[tl.item], order)) return;
+    const DevItem &it = items[tl.item];
+    const DevInst &I = inst[it.inst];
+    const int Pp = it.P;
+    const unsigned short *A = reinterpret_cast<const unsigned short *>(P.A) + it.off_A;
+    const unsigned short *F = reinterpret_cast<const unsigned short *>(P.F) + it.off_F;
+    const unsigned short *Flo = reinterpret_cast<const unsigned short *>(P.Flo) + it.off_Flo;
+    float *J = reinterpret_cast<float *>(P.J) + it.off_F;
+    const LevelView lv(P, I);
+    __shared__ GemmLds S;
+    gemm_tile_bf16(it, lv, A, F, Flo, tl.tm * 64, tl.tn * kGemmLevels, S, [&](int lev, int r, float4 v) {
+        if (r < Pp) *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v;
+    });
 }
 
 // Formal solution of the current order for every column, fused with the order
@@ -440,13 +455,69 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
 // exposing it once per level.  U and W do not change the arithmetic (every
 // variant gives the same bits); the plan's autotuner picks them.
 //
-// The order sum of the fp64 form is fp64 over the whole field.  In the mixed
-// form it is fp32 over the whole field, where its interior rows only feed the
-// max-norm of the convergence test; the two rows that become outputs, k = 0
-// (TOA) and k = nt (BOA), are summed in fp64 with an fp32 Neumaier
-// compensation term.
+// The order sum is fp64 over the whole field (it feeds the convergence test).
+// In the mixed form the two rows that become outputs, k = 0 (TOA) and k = nt
+// (BOA), also carry an fp32 Neumaier compensation term.
 __device__ inline int clamp_level(int k, int nt) { return k < 0 ? 0 : (k > nt ? nt : k); }
 __device__ inline int clamp_layer(int l, int nt) { return l < 0 ? 0 : (l > nt - 1 ? nt - 1 : l); }
+
+// One level of the mixed-form scan in fp32, written with om = 1 - T (accurate
+// from expm1):
+//   run <- run - om run + ((S_out - S_in) + om (S_in + slope mu_signed)) / 2
+// the linear-source recurrence with no 1 - T cancellation; slope is the source
+// gradient in optical depth along the scan.  The multiply-adds are explicit,
+// so every kernel that scans computes the same bits.
+__device__ inline float mixed_step(float run, float s_in, float s_out, float om, float idt, float smu, bool down)
+{
+    const float a = (down ? (s_out - s_in) : (s_in - s_out)) * idt;
+    return fmaf(0.5f, fmaf(om, fmaf(a, smu, s_in), s_out - s_in), fmaf(-om, run, run));
+}
+
+// One column of the mixed form: its field pair (bf16, plus the bf16 residual
+// for the items that keep it), its fp64 order sum with the fp32 compensation
+// of the two output rows, and the column's two max-norms.
+struct MixedColumn {
+    __hip_bfloat16 *F, *Flo;   // level stride P; Flo null unless the item keeps the pair
+    double *tot, *bottom;
+    float *cmp_top, *cmp_bot;
+    long long Pp;
+    int nt, down;
+    double mcur, mtot;
+
+    __device__ MixedColumn(const Pools &P, const DevItem &it, int col, int c, int o, int down_)
+    {
+        const int n = it.n;
+        F = reinterpret_cast<__hip_bfloat16 *>(P.F) + it.off_F + col;
+        Flo = it.fsplit ? reinterpret_cast<__hip_bfloat16 *>(P.Flo) + it.off_Flo + col : nullptr;
+        tot = P.tot + it.off_tot + col;
+        bottom = P.bc + it.off_bc + 3 * n + c * n + o;
+        cmp_top = P.comp + it.off_cmp + col;
+        cmp_bot = P.comp + it.off_cmp + it.P + col;
+        Pp = it.P;
+        nt = it.nt;
+        down = down_;
+        mcur = mtot = 0.0;
+    }
+
+    // the field value v at level k, with t_old the order sum there before
+    // this order
+    __device__ void emit(int k, double v, double t_old)
+    {
+        const long long ix = (long long)k * Pp;
+        store_field_pair(F, Flo, ix, v);
+        mcur = fmax(mcur, fabs(v));
+        if (down && k == nt) *bottom = v;
+        const double tsum = t_old + v;
+        tot[ix] = tsum;
+        mtot = fmax(mtot, fabs(tsum));
+        if (k == 0 || k == nt) {
+            // Neumaier: the rounding error of this addition, kept in fp32
+            float *cp = k == 0 ? cmp_top : cmp_bot;
+            const double e = (fabs(t_old) >= fabs(v)) ? ((t_old - tsum) + v) : ((v - tsum) + t_old);
+            *cp = (float)((double)*cp + e);
+        }
+    }
+};
 
 template <typename FT, typename JT, bool MIXED, int U, int W>
 __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst,
@@ -475,59 +546,61 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
         col_decode(col, n, c, hemi, o);
         hemi = __builtin_amdgcn_readfirstlane(hemi);   // column tiles are direction-pure
         const JT *__restrict__ J = reinterpret_cast<const JT *>(P.J) + off_F + col;
-        FT *__restrict__ F = reinterpret_cast<FT *>(P.F) + off_F + col;
-        __hip_bfloat16 *__restrict__ Flo =
-            (MIXED && it.fsplit) ? reinterpret_cast<__hip_bfloat16 *>(P.Flo) + it.off_Flo + col : nullptr;
-        double *__restrict__ tot = MIXED ? nullptr : P.tot + it.off_tot + col;
-        float *__restrict__ tot32 = MIXED ? P.tot32 + it.off_tot + col : nullptr;
-        double *__restrict__ bottom = P.bc + off_bc + 3 * n + c * n + o;
-        float *__restrict__ cmp_top = MIXED ? P.comp + off_cmp + col : nullptr;
-        float *__restrict__ cmp_bot = MIXED ? P.comp + off_cmp + Pp + col : nullptr;
-        double *__restrict__ end_top = MIXED ? P.tend + off_cmp + col : nullptr;
-        double *__restrict__ end_bot = MIXED ? P.tend + off_cmp + Pp + col : nullptr;
         const double mu = P.ring[I.off_ring + o];
         const double bc = (I.surf != 0 && hemi == 0) ? P.bc[off_bc + c * n + o] : 0.0;
         // level order of the scan and the layer crossed when stepping to level k
         const int k_first = hemi ? 0 : nt;
         const int step = hemi ? 1 : -1;
         auto layer_of = [&](int k) { return hemi ? k - 1 : k; };
-        // t_old: the order sum before this order at level k (fp64 form; the
-        // mixed form passes its fp32 interior value and reads the output rows
-        // from their fp64 copies here)
-        auto emit = [&](int k, double v, double t_old) {
-            const long long ix = (long long)k * Pp;
-            if constexpr (MIXED)
-                store_field_pair(reinterpret_cast<__hip_bfloat16 *>(F), Flo, ix, v);
-            else
-                F[ix] = to_storage<FT>(v);
-            mcur = fmax(mcur, fabs(v));
-            if (hemi == 1 && k == nt) *bottom = v;
-            if (!accumulate) return;
-            if constexpr (MIXED) {
-                const float s32 = (float)t_old + (float)v;
-                tot32[ix] = s32;
-                if (k == 0 || k == nt) {
-                    double *te = k == 0 ? end_top : end_bot;
-                    float *cp = k == 0 ? cmp_top : cmp_bot;
-                    const double t64 = *te, tsum = t64 + v;
-                    *te = tsum;
-                    mtot = fmax(mtot, fabs(tsum));
-                    const double e = (fabs(t64) >= fabs(v)) ? ((t64 - tsum) + v) : ((v - tsum) + t64);
-                    *cp = (float)((double)*cp + e);
-                } else {
-                    mtot = fmax(mtot, fabs((double)s32));
+        if constexpr (MIXED) {
+            const float *OM = P.Tf + I.off_T;
+            const float *IDT = P.idt + I.off_lev;
+            MixedColumn mc(P, it, col, c, o, hemi);
+            const float smu = hemi ? -(float)mu : (float)mu;
+            float run = hemi ? 0.f : (float)bc;
+            float s_in = (float)J[(long long)k_first * Pp];
+            mc.emit(k_first, hemi ? 0.0 : bc, mc.tot[(long long)k_first * Pp]);
+            for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * U) {
+                float sj[U], om[U], idt[U];
+                double to[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    // clamped indices keep the block's loads unconditional, so
+                    // they are all in flight before the first one is used
+                    const int k = clamp_level(k0 + step * u, nt);
+                    const int l = clamp_layer(layer_of(k0 + step * u), nt);
+                    sj[u] = (float)J[(long long)k * Pp];
+                    om[u] = OM[(long long)l * n + o];
+                    idt[u] = IDT[l];
+                    to[u] = mc.tot[(long long)k * Pp];
                 }
-            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int k = k0 + step * u;
+                    if (k < 0 || k > nt) break;
+                    run = mixed_step(run, s_in, sj[u], om[u], idt[u], smu, hemi);
+                    mc.emit(k, (double)run, to[u]);
+                    s_in = sj[u];
+                }
+            }
+            mcur = mc.mcur;
+            mtot = mc.mtot;
+        } else {
+            FT *__restrict__ F = reinterpret_cast<FT *>(P.F) + off_F + col;
+            double *__restrict__ tot = P.tot + it.off_tot + col;
+            double *__restrict__ bottom = P.bc + off_bc + 3 * n + c * n + o;
+            // t_old: the order sum before this order at level k
+            auto emit = [&](int k, double v, double t_old) {
+                const long long ix = (long long)k * Pp;
+                F[ix] = to_storage<FT>(v);
+                mcur = fmax(mcur, fabs(v));
+                if (hemi == 1 && k == nt) *bottom = v;
+                if (!accumulate) return;
                 const double tsum = t_old + v;
                 tot[ix] = tsum;
                 mtot = fmax(mtot, fabs(tsum));
-            }
-        };
-        if constexpr (MIXED)
-            emit(k_first, hemi ? 0.0 : bc, accumulate ? (double)tot32[(long long)k_first * Pp] : 0.0);
-        else
+            };
             emit(k_first, hemi ? 0.0 : bc, accumulate ? tot[(long long)k_first * Pp] : 0.0);
-        if (!MIXED) {
             const double *T = P.T + I.off_T;
             double run = hemi ? 0.0 : bc;
             double s_in = (double)J[(long long)k_first * Pp];
@@ -557,39 +630,6 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                     s_in = s_out;
                 }
             }
-        } else {
-            // fp32 scan written with om = 1 - T (accurate from expm1):
-            //   run <- run - om run + (S_out - S_in + om (S_in + slope mu_signed)) / 2
-            // which is the linear-source recurrence with no 1 - T cancellation
-            const float *OM = P.Tf + I.off_T;
-            const float *IDT = P.idt + I.off_lev;
-            const float smu = hemi ? -(float)mu : (float)mu;
-            float run = hemi ? 0.f : (float)bc;
-            float s_in = (float)J[(long long)k_first * Pp];
-            for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * U) {
-                float sj[U], om[U], idt[U], to[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    // clamped indices keep the block's loads unconditional, so
-                    // they are all in flight before the first one is used
-                    const int k = clamp_level(k0 + step * u, nt);
-                    const int l = clamp_layer(layer_of(k0 + step * u), nt);
-                    sj[u] = (float)J[(long long)k * Pp];
-                    om[u] = OM[(long long)l * n + o];
-                    idt[u] = IDT[l];
-                    to[u] = tot32[(long long)k * Pp];
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int k = k0 + step * u;
-                    if (k < 0 || k > nt) break;
-                    const float s_out = sj[u];
-                    const float a = hemi ? (s_out - s_in) * idt[u] : (s_in - s_out) * idt[u];
-                    run = run - om[u] * run + 0.5f * ((s_out - s_in) + om[u] * (s_in + a * smu));
-                    emit(k, (double)run, to[u]);
-                    s_in = s_out;
-                }
-            }
         }
     } else if (!MIXED && col < ct.col_end) {
         // pad columns (6n <= col < P) of the fp64 field are written too:
@@ -610,14 +650,12 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
 
 // ---------------------------------------------------------------------------
 // Order sum at an output row (k = 0 or k = nt) of column col: the mixed form
-// reads the fp64 copy of the row plus its compensation term.
+// adds the row's compensation term.
 __device__ inline double total_at(const Pools &P, const DevItem &it, bool mixed, int k, int col)
 {
-    if (mixed) {
-        const long long e = it.off_cmp + (k == 0 ? 0 : it.P) + col;
-        return P.tend[e] + (double)P.comp[e];
-    }
-    return P.tot[it.off_tot + (long long)k * it.P + col];
+    const double t = P.tot[it.off_tot + (long long)k * it.P + col];
+    if (mixed) return t + (double)P.comp[it.off_cmp + (k == 0 ? 0 : it.P) + col];
+    return t;
 }
 
 // Per-item outputs after the order loop (rt_solver.c:1010-1061).
