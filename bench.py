@@ -35,6 +35,28 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X fp64 matrix
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "sos_traffic_8192inst_mixed.md")
+
+
+def measured_traffic(geoms):
+    """HBM bytes per launch of gemm_bf16 and of the sweep, measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-calibrated) on this workload and
+    committed with the profile summary; None for another workload size."""
+    try:
+        lines = open(TRAFFIC_PROFILE).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        if ln.startswith("bench-traffic-bytes-per-launch:"):
+            tok = ln.split(":", 1)[1].split()
+            kv = dict(zip(tok[0::2], tok[1::2]))
+            if kv.get("workload") != f"geoms={geoms}":
+                return None
+            return {"gemm_bf16": float(kv["gemm_bf16"]), "sweep": float(kv["sweep"])}
+    return None
+
+
 def log(msg):
     sys.stderr.write(f"[bench] {msg}\n")
     sys.stderr.flush()
@@ -206,11 +228,14 @@ def main():
     sweep_s = kt["sweep_ms"] / 1e3
     gemm_tf = work["gemm_flops"] / gemm_s / 1e12 if gemm_s > 0 else 0.0
     sweep_gbs = work["sweep_bytes"] / sweep_s / 1e9 if sweep_s > 0 else 0.0
+    traffic = measured_traffic(args.geoms)
     rl_gemm = {"kernel": "gemm_bf16", "bound": "mfma", "achieved": gemm_tf, "peak": BF16_PEAK_TFLOPS,
-               "unit": "TFLOP/s", "frac": gemm_tf / BF16_PEAK_TFLOPS, "traffic": None,
+               "unit": "TFLOP/s", "frac": gemm_tf / BF16_PEAK_TFLOPS,
+               "traffic": traffic["gemm_bf16"] if traffic else None,
                "ms_total": kt["gemm_ms"], "launches": kt["gemm_launches"]}
     rl_sweep = {"kernel": "sweep<float,true>", "bound": "hbm", "achieved": sweep_gbs, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS,
+                "traffic": traffic["sweep"] if traffic else None,
                 "ms_total": kt["sweep_ms"], "launches": kt["sweep_launches"]}
     roofline = rl_gemm if kt["gemm_ms"] >= kt["sweep_ms"] else rl_sweep
 

@@ -156,9 +156,16 @@ int ocrt_plan_results(ocrt_plan *plan, ocrt_result_t *out, double *toa_modes,
 int  ocrt_plan_mmax(const ocrt_plan *plan);
 /* Bytes of device memory the plan holds. */
 size_t ocrt_plan_device_bytes(const ocrt_plan *plan);
-/* Algorithmic work of the last execute: scattering-integral flops and
- * layer-sweep bytes summed over the orders actually run (useful to compute
- * roofline fractions); either pointer may be NULL. */
+/* Algorithmic work of the last execute, summed over the orders actually run
+ * (useful to compute roofline fractions); either pointer may be NULL:
+ *   gemm_flops  : 72 n n_in (nt + 1) per item and order for the particle
+ *                 operator, plus 144 n (nt + 1) for the rank <= 6 Rayleigh part
+ *                 of modes m <= 2 (n_in: nodes of non-zero weight);
+ *   sweep_bytes : per element (6 n (nt + 1) per item and order) the source read
+ *                 (4 B mixed, 8 B fp64), the field written (2 B, 4 B for the
+ *                 field pairs of modes m < 3; 8 B fp64) and the fp64 order sum
+ *                 read and written (16 B), plus 1 - T read for every layer and
+ *                 node of both hemispheres (2 n nt x 4 or 8 B). */
 int  ocrt_plan_work(const ocrt_plan *plan, double *gemm_flops, double *sweep_bytes,
                     double *gemm_launches);
 /* Per-kernel timing for roofline reports: when enabled, every scattering-

@@ -1273,7 +1273,13 @@ extern "C" int ocrt_plan_results(ocrt_plan *p, ocrt_result_t *out, double *toa_m
             // dense particle operator 2 (6 n)(6 n_in) per level; the molecular
             // operator has rank <= 6 and counts as 2 * 2 * (6 n) * 6 per level
             alg += ord * L * ((I.aer ? 72.0 * I.n * n_in : 0.0) + ((m <= 2 && I.ray) ? 144.0 * I.n : 0.0));
-            bytes += ord * (2.0 * 3.0 * 2.0 * I.n * L + 2.0 * I.n * I.nt) * (mixed ? 4.0 : 8.0);
+            // per element and order: read the source, write the field (bf16,
+            // plus its residual for the low modes, in the mixed form), read and
+            // write the fp64 order sum; per layer and node of each hemisphere:
+            // read 1 - T
+            const double el = 6.0 * I.n * L;
+            const double fb = mixed ? (m < kSplitModes ? 4.0 : 2.0) : 8.0;
+            bytes += ord * (el * ((mixed ? 4.0 : 8.0) + fb + 16.0) + 2.0 * I.n * I.nt * (mixed ? 4.0 : 8.0));
         }
         if (toa_modes)
             for (int m = I.mmax + 1; m < M; ++m)
