@@ -210,30 +210,52 @@ __global__ void surface_adj(const DevInst *__restrict__ inst, const DevItem *__r
     const double *Ub = reinterpret_cast<const double *>(P.F) + it.off_F + (long long)nt * it.P;
     const double W = (I.surf == 1) ? 2.0 * OCRT_PI : ((m == 0) ? 2.0 * OCRT_PI : OCRT_PI);
     double gw = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double acc[3] = {0.0, 0.0, 0.0};
-        if (I.surf == 1) {
+    if (I.surf == 1) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            double acc[3];
             double Fr[9];
             fresnel_R(mu[i], I.n_water, I.q_conv, Fr);
             const double f = (w[i] == 0.0) ? 1.0
                              : (W * mu[i] * w[i]) * (1.0 / (2.0 * OCRT_PI * mu[i] * w[i]));
             for (int c2 = 0; c2 < 3; ++c2)
                 acc[c2] = f * (bb[i] * Fr[c2] + bb[n + i] * Fr[3 + c2] + bb[2 * n + i] * Fr[6 + c2]);
-        } else if (w[i] != 0.0) {
-            const double pw = W * mu[i] * w[i];
-            for (int op = 0; op < n; ++op) {
-                // entry kl of R^m[op][i] at R[kl * n]
-                const double *R = P.R + r_index(I, m, op, i, 0);
-                const double *dR = P.dR + r_index(I, m, op, i, 0);
-                const double b0 = bb[op], b1 = bb[n + op], b2 = bb[2 * n + op];
-                for (int c2 = 0; c2 < 3; ++c2) {
-                    acc[c2] += pw * (b0 * R[c2 * n] + b1 * R[(3 + c2) * n] + b2 * R[(6 + c2) * n]);
-                    gw += pw * (b0 * dR[c2 * n] + b1 * dR[(3 + c2) * n] + b2 * dR[(6 + c2) * n]) *
-                          Ub[col_dn(n, c2, i)];
-                }
-            }
+            for (int c2 = 0; c2 < 3; ++c2) lamb[col_dn(n, c2, i)] += acc[c2];
         }
-        for (int c2 = 0; c2 < 3; ++c2) lamb[col_dn(n, c2, i)] += acc[c2];
+    } else {
+        // rough surface, one wavefront: for every incoming node i the lanes
+        // take the outgoing nodes op, whose kernel entries R^m[op][i] are
+        // consecutive in op; eight nodes' three sums are reduced together
+        const int lane = threadIdx.x & 63;
+        for (int i0 = 0; i0 < n; i0 += 8) {
+            double v[8 * 3];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int i = i0 + j < n ? i0 + j : n - 1;
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+                if (i0 + j < n && w[i] != 0.0) {
+                    const double pw = W * mu[i] * w[i];
+                    const double u0 = Ub[col_dn(n, 0, i)], u1 = Ub[col_dn(n, 1, i)], u2 = Ub[col_dn(n, 2, i)];
+                    // entry kl of R^m[op][i] at R[kl n + op]
+                    const double *R = P.R + r_index(I, m, 0, i, 0);
+                    const double *dR = P.dR + r_index(I, m, 0, i, 0);
+                    for (int op = lane; op < n; op += 64) {
+                        const double b0 = bb[op], b1 = bb[n + op], b2 = bb[2 * n + op];
+                        a0 += b0 * R[op] + b1 * R[3 * n + op] + b2 * R[6 * n + op];
+                        a1 += b0 * R[n + op] + b1 * R[4 * n + op] + b2 * R[7 * n + op];
+                        a2 += b0 * R[2 * n + op] + b1 * R[5 * n + op] + b2 * R[8 * n + op];
+                        gw += pw * ((b0 * dR[op] + b1 * dR[3 * n + op] + b2 * dR[6 * n + op]) * u0 +
+                                    (b0 * dR[n + op] + b1 * dR[4 * n + op] + b2 * dR[7 * n + op]) * u1 +
+                                    (b0 * dR[2 * n + op] + b1 * dR[5 * n + op] + b2 * dR[8 * n + op]) * u2);
+                    }
+                    a0 *= pw; a1 *= pw; a2 *= pw;
+                }
+                v[3 * j] = a0; v[3 * j + 1] = a1; v[3 * j + 2] = a2;
+            }
+            wave_sum_8xC<3>(v, lane);
+            const int i = i0 + (lane >> 3);
+            if ((lane & 7) == 0 && i < n)
+                for (int c2 = 0; c2 < 3; ++c2) lamb[col_dn(n, c2, i)] += v[c2];
+        }
     }
     if (I.surf == 2) {
         gw = wave_sum(gw);

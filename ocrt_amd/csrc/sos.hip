@@ -305,23 +305,24 @@ struct GemmLds {
 };
 
 // One tile of the bf16 scattering integral: operator rows row0 .. row0 + 63
-// and levels lev0 .. lev0 + kGemmLevels - 1 of one item, four waves of 32 x 64.
+// and levels lev0 .. lev0 + kGemmLevels - 1 of one item with NP operator parts
+// (the item's K is NP P), four waves of 32 x 64.  NP is a template parameter
+// so that single-part items hold half the accumulators (higher occupancy).
 // Rows past the operator's Mp rows load its last row (their results are never
 // used).  The fp32 results, scaled by the level fractions, go to
 // out(lev, r, v) for the four rows r .. r + 3 of v, at levels < L only.
-template <typename Out>
+template <int NP, typename Out>
 __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, const unsigned short *A,
                                       const unsigned short *F, const unsigned short *Flo, int row0, int lev0,
                                       GemmLds &S, Out out)
 {
     const int Pp = it.P, L = it.L, Mp = it.Mp;
-    const int nparts = it.K / Pp;
     const bool split = it.fsplit != 0;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
-    float4_ acc[2][2][4];
+    float4_ acc[NP][2][4];
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
+    for (int q = 0; q < NP; ++q)
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -330,14 +331,14 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     // and part, two of the field (levels lr and lr + 64)
     const int lr = t >> 2, lk = (t & 3) * 8;
     const long long slab = (long long)Mp * Pp;
-    const bool two = nparts > 1;
+    constexpr bool two = NP > 1;
     const int arow = min(row0 + lr, Mp - 1);
     uint4 ra0h, ra0l, ra1h, ra1l, rb0, rb1, rl0, rl1;
     auto load_slab = [&](int k0) {
         const long long ro = (long long)arow * Pp + k0 + lk;
         ra0h = *reinterpret_cast<const uint4 *>(A + ro);
         ra0l = *reinterpret_cast<const uint4 *>(A + slab + ro);
-        if (two) {
+        if constexpr (two) {
             ra1h = *reinterpret_cast<const uint4 *>(A + 2 * slab + ro);
             ra1l = *reinterpret_cast<const uint4 *>(A + 3 * slab + ro);
         }
@@ -355,7 +356,7 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     auto store_slab = [&]() {
         *reinterpret_cast<uint4 *>(&S.A[0][0][lr][lk]) = ra0h;
         *reinterpret_cast<uint4 *>(&S.A[0][1][lr][lk]) = ra0l;
-        if (two) {
+        if constexpr (two) {
             *reinterpret_cast<uint4 *>(&S.A[1][0][lr][lk]) = ra1h;
             *reinterpret_cast<uint4 *>(&S.A[1][1][lr][lk]) = ra1l;
         }
@@ -395,7 +396,7 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
         const bool more = k0 + 32 < Pp;
         if (more) load_slab(k0 + 32);
         mfma_part(std::integral_constant<int, 0>{});
-        if (two) mfma_part(std::integral_constant<int, 1>{});
+        if constexpr (two) mfma_part(std::integral_constant<int, 1>{});
         __syncthreads();
         if (more) {
             store_slab();
@@ -416,15 +417,18 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
             float4 v;
             v.x = s0 * acc[0][a][b][0]; v.y = s0 * acc[0][a][b][1];
             v.z = s0 * acc[0][a][b][2]; v.w = s0 * acc[0][a][b][3];
-            if (two) {
-                v.x = fmaf(y, acc[1][a][b][0], v.x); v.y = fmaf(y, acc[1][a][b][1], v.y);
-                v.z = fmaf(y, acc[1][a][b][2], v.z); v.w = fmaf(y, acc[1][a][b][3], v.w);
+            if constexpr (two) {
+                v.x = fmaf(y, acc[NP - 1][a][b][0], v.x); v.y = fmaf(y, acc[NP - 1][a][b][1], v.y);
+                v.z = fmaf(y, acc[NP - 1][a][b][2], v.z); v.w = fmaf(y, acc[NP - 1][a][b][3], v.w);
             }
             out(lev, r, v);
         }
     }
 }
 
+// NP: operator parts of every item in the tile list (the plan keeps one list
+// per part count).
+template <int NP>
 __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ inst,
                                                  const DevItem *__restrict__ items, Pools P,
                                                  const GemmTile *__restrict__ tiles,
@@ -441,7 +445,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
     float *J = reinterpret_cast<float *>(P.J) + it.off_F;
     const LevelView lv(P, I);
     __shared__ GemmLds S;
-    gemm_tile_bf16(it, lv, A, F, Flo, tl.tm * 64, tl.tn * kGemmLevels, S, [&](int lev, int r, float4 v) {
+    gemm_tile_bf16<NP>(it, lv, A, F, Flo, tl.tm * 64, tl.tn * kGemmLevels, S, [&](int lev, int r, float4 v) {
         if (r < Pp) *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v;
     });
 }
@@ -813,6 +817,10 @@ OCRT_SWEEP_INST(__hip_bfloat16, float, true, 8, 4)
 OCRT_SWEEP_INST(__hip_bfloat16, float, true, 16, 2)
 OCRT_SWEEP_INST(__hip_bfloat16, float, true, 16, 4)
 #undef OCRT_SWEEP_INST
+template __global__ void gemm_bf16<1>(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *,
+                                      int);
+template __global__ void gemm_bf16<2>(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *,
+                                      int);
 template __global__ void gemm_f64<kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                             const ItemState *, int);
 template __global__ void gemm_f64<kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
