@@ -47,7 +47,8 @@ enum {
     OCRT_ERR_ARGS = -1,
     OCRT_ERR_NONFINITE = -2,
     OCRT_ERR_ALLOC = -3,
-    OCRT_ERR_DEVICE = -4
+    OCRT_ERR_DEVICE = -4,
+    OCRT_ERR_FLAT = -5      /* ocrt_interface_fourier: T_aw of a flat surface */
 };
 
 /* Arithmetic forms of the solve.
@@ -253,6 +254,48 @@ int ocrt_plan_lut_extras(ocrt_plan *plan, double *raw, double *view_modes);
 /* Final relative residual of every (instance, mode) of the last execute,
  * [n_inst][m_max_all + 1] (sos_residual_per_m of rt_result_t, rt_types.h:511). */
 int ocrt_plan_residuals(ocrt_plan *plan, double *residuals);
+
+/* ---------------------------------------------------------------------------
+ * Rough air-water interface matrices of the coupled ocean (Cox-Munk slopes,
+ * Fresnel facets, Sancer shadowing for the reflections and T_wa):
+ *   OCRT_IFACE_R_AA  air-side reflection; mu_o upward (air), mu_i the
+ *                    incident downward |mu|: surface_coxmunk_fourier_kernel
+ *                    (code/OCRT_C/src/shared/surface.h:222, surface.c:839-1134)
+ *   OCRT_IFACE_R_WW  water-side internal reflection; mu_o reflected downward,
+ *                    mu_i incident upward |mu| in water:
+ *                    surface_R_ww_coxmunk_fourier_kernel (surface.h:246, surface.c:1386-1447)
+ *   OCRT_IFACE_T_WA  water -> air transmission; mu_o air, mu_i water:
+ *                    surface_T_wa_coxmunk_fourier_kernel (surface.h:271, surface.c:1731-1811)
+ *   OCRT_IFACE_T_AW  air -> water transmission; mu_o water, mu_i air:
+ *                    surface_T_aw_coxmunk_fourier_kernel (surface.h:264, surface.c:1583-1721)
+ */
+enum { OCRT_IFACE_R_AA = 0, OCRT_IFACE_R_WW = 1, OCRT_IFACE_T_WA = 2, OCRT_IFACE_T_AW = 3 };
+#define OCRT_IFACE_MAX_MODES 256     /* m_max + 1 */
+#define OCRT_IFACE_MAX_PHI 65536
+
+/* Azimuthal Fourier kernels of one interface matrix for every mode
+ * m = 0..m_max in one launch: out[((m * n_o + o) * n_i + i) * 9 + 3k + l]
+ * (host memory), mode m's slab equal to what the reference's per-mode
+ * function writes for m.  Midpoint azimuth rule with n_phi points, cos for
+ * the II IQ QI QQ UU entries and sin for the others, 1/(2 pi) at m = 0 and
+ * 1/pi above; IU and QU negated for m > 0 except in R_WW.  Cosines in
+ * (0, 1]; n_o * n_i < 2^23; sigma_type 1 is the OCRT slope law, any other
+ * value Cox-Munk's sigma = 0.0731 sqrt(W) (surface.c:57-64).  Returns
+ * OCRT_OK, OCRT_ERR_ARGS, OCRT_ERR_FLAT (T_AW with slope variance <= 1e-10,
+ * where the reference returns -2), OCRT_ERR_DEVICE.  Blocks on hip_stream
+ * (NULL: the default stream) until the result is in out. */
+int ocrt_interface_fourier(int kind, const double *mu_o, int n_o, const double *mu_i, int n_i, int m_max,
+                           int n_phi, double wind, int sigma_type, double n_water, int q_convention,
+                           double *out, void *hip_stream);
+
+/* Slope-integrated direct irradiance terms for n cosines (host memory):
+ * OCRT_IFACE_T_AW the air -> water transmittance of the solar beam,
+ * surface_T_aw_coxmunk_direct (surface.h:43, surface.c:138-198);
+ * OCRT_IFACE_R_WW the water-side reflectance of an upwelling beam,
+ * surface_R_ww_coxmunk_direct in its default first-encounter form
+ * (surface.h:61, surface.c:229-368). */
+int ocrt_interface_direct(int kind, const double *mu, int n, double wind, int sigma_type, double n_water,
+                          double *out, void *hip_stream);
 
 /* One-call form: plan, execute, copy results, destroy. */
 int ocrt_solve_batch(const ocrt_instance_t *inst, int n_inst, int precision,

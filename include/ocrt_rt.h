@@ -318,6 +318,30 @@ double ocrt_rayleigh_tau_bodhaine1999(double wavelength_nm, double pressure_hpa,
 int ocrt_rt_solve_batch(const rt_case_t *cs, const rt_options_t *opts,
                         const rt_aerosol_input_t *aer, int n, rt_result_t *out, int *rc);
 
+/* The reference's rough-interface surface functions, same names, arguments
+ * and return codes (code/OCRT_C/src/shared/surface.h:40-61, 222-281), served
+ * by the GPU kernels of ocrt_interface_fourier / ocrt_interface_direct.  The
+ * first call for a node set and surface computes every mode
+ * 0..max(m, build cap) at once and keeps them in a per-thread cache; modes
+ * above 255 are refused (-1).  surface_fkc_set_build_mmax sets the build cap
+ * (0..64; other values restore 64), as in surface.c:830-838. */
+int surface_coxmunk_fourier_kernel(const double *mu_o, int n_o, const double *mu_i, int n_i, int m,
+                                   int n_phi_quad, double ws, int sigma_type, double n_water,
+                                   int q_convention, double *R_m);
+int surface_R_ww_coxmunk_fourier_kernel(const double *mu_o, int n_o, const double *mu_i, int n_i, int m,
+                                        int n_phi_quad, double ws, int sigma_type, double n_water,
+                                        int q_convention, double *R_m);
+int surface_T_wa_coxmunk_fourier_kernel(const double *mu_o, int n_o, const double *mu_i, int n_i, int m,
+                                        int n_phi_quad, double ws, int sigma_type, double n_water,
+                                        int q_convention, double *T_m);
+int surface_T_aw_coxmunk_fourier_kernel(const double *mu_o, int n_o, const double *mu_i, int n_i, int m,
+                                        int n_phi_quad, double ws, int sigma_type, double n_water,
+                                        int q_convention, double *T_m);
+void surface_fkc_set_build_mmax(int m_max);
+/* NaN when the GPU call fails */
+double surface_T_aw_coxmunk_direct(double mu_sun_air, double n_water, double wind_speed, int sigma_type);
+double surface_R_ww_coxmunk_direct(double mu_up_water, double n_water, double wind_speed, int sigma_type);
+
 #ifdef __cplusplus
 }
 #endif

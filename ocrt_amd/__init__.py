@@ -74,6 +74,10 @@ def library():
         lib.ocrt_plan_tuning.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_plan_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.ocrt_plan_residuals.argtypes = [C.c_void_p, C.c_void_p]
+        lib.ocrt_interface_fourier.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                               C.c_double, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
+        lib.ocrt_interface_direct.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double,
+                                              C.c_void_p, C.c_void_p]
         _lib = lib
     return _lib
 
@@ -82,6 +86,41 @@ def _require_gpu():
     import torch
     if not torch.cuda.is_available():
         raise OcrtError("the OCRT MI355X solver needs a ROCm GPU; none is visible")
+
+
+INTERFACE_KINDS = {"R_aa": 0, "R_ww": 1, "T_wa": 2, "T_aw": 3}
+
+
+def interface_fourier(kind, mu_o, mu_i, m_max, n_phi=1024, wind=5.0, sigma_type=1, n_water=1.34, q_convention=1):
+    """Azimuthal Fourier kernels of a rough-interface Mueller matrix
+    (ocrt_interface_fourier) for modes 0..m_max: array [m_max+1, n_o, n_i, 3, 3].
+    kind: "R_aa", "R_ww", "T_wa" or "T_aw" (see include/ocrt_amd.h for the
+    meaning of mu_o and mu_i in each)."""
+    _require_gpu()
+    lib = library()
+    mo = np.ascontiguousarray(mu_o, dtype=np.float64)
+    mi = np.ascontiguousarray(mu_i, dtype=np.float64)
+    out = np.zeros((m_max + 1, mo.size, mi.size, 3, 3))
+    rc = lib.ocrt_interface_fourier(INTERFACE_KINDS[kind], mo.ctypes.data, mo.size, mi.ctypes.data, mi.size, m_max,
+                                    n_phi, wind, sigma_type, n_water, q_convention, out.ctypes.data, None)
+    if rc != 0:
+        raise OcrtError(f"ocrt_interface_fourier failed with code {rc}")
+    return out
+
+
+def interface_direct(kind, mu, wind=5.0, sigma_type=1, n_water=1.34):
+    """Slope-integrated direct terms (ocrt_interface_direct): "T_aw" the air ->
+    water transmittance of a beam at cosine mu, "R_ww" the water-side
+    reflectance of an upwelling beam."""
+    _require_gpu()
+    lib = library()
+    m = np.ascontiguousarray(mu, dtype=np.float64)
+    out = np.zeros(m.size)
+    rc = lib.ocrt_interface_direct(INTERFACE_KINDS[kind], m.ctypes.data, m.size, wind, sigma_type, n_water,
+                                   out.ctypes.data, None)
+    if rc != 0:
+        raise OcrtError(f"ocrt_interface_direct failed with code {rc}")
+    return out
 
 
 PROFILES = {"usstd76": 0, "tropical": 1, "mlsumm": 2, "mlwint": 3, "sasumm": 4, "sawint": 5, "userdef": 6}

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libocrt_amd.so")
 CLI = os.path.join(HERE, "ocrt")
-SOURCES = ["plan.hip", "setup.hip", "sos.hip", "ipss.hip", "adjoint.hip", "lut.hip", "rt_compat.hip", "gas.hip"]
+SOURCES = ["plan.hip", "setup.hip", "sos.hip", "ipss.hip", "adjoint.hip", "lut.hip", "rt_compat.hip", "gas.hip", "interface.hip"]
 ARCH = os.environ.get("OCRT_OFFLOAD_ARCH", "gfx950")
 
 # -packed-fp32-ops off: hipcc -O3 must not emit v_pk_{add,mul,fma}_f32 on this target.

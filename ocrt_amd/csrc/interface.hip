@@ -1,0 +1,616 @@
+// Rough air-water interface matrices of the coupled ocean: azimuthal Fourier
+// kernels of the Cox-Munk Mueller matrices for
+//   R_aa  air-side reflection               (code/OCRT_C/src/shared/surface.c:514-646, 839-1134)
+//   R_ww  water-side internal reflection    (surface.c:1136-1249, 1386-1447)
+//   T_wa  water -> air transmission         (surface.c:648-773, 1731-1811)
+//   T_aw  air -> water transmission         (surface.c:80-136, 1476-1721)
+// and the slope-integrated direct terms T_aw(mu_sun) and R_ww(mu_up)
+// (surface.c:138-198, 229-368).
+//
+// One workgroup per (outgoing, incoming) node pair evaluates the 3x3 Mueller
+// matrix at 256 azimuths at a time into LDS; thread m then projects the block
+// onto cos(m phi) / sin(m phi) (weights from a table built once per call), so
+// one pass over the azimuth rule serves every mode.
+// Everything is fp64.  The entry points take host arrays, like the
+// reference's functions, and own their device buffers.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "device_math.h"
+#include "ocrt_amd.h"
+#include "ocrt_rt.h"
+#include "wave.h"
+
+namespace ocrt {
+namespace {
+
+constexpr int kIfThreads = 256;       // azimuths per block and threads per pair
+constexpr int kIfMaxModes = 256;      // m_max + 1 <= threads: one mode per thread
+constexpr int kDirectThreads = 128;   // >= the 101 slope rows of the direct integrals
+constexpr long long kIfMaxPairs = (1LL << 31) / kIfThreads - 1;   // work-items of one launch < 2^31
+
+// Symmetric Fresnel-form matrix [[A q 0] [q A 0] [0 0 d]] rotated into the
+// meridian planes: L(c2, s2) . M . L(c1, s1), with L = diag(1, [[c -s] [s c]])
+// and (c, s) = (cos 2i, sin 2i) of each rotation angle.
+__device__ inline void rotated_mueller(double A, double q, double d, double c1, double s1, double c2, double s2,
+                                       double *R)
+{
+    R[0] = A;
+    R[1] = q * c1;
+    R[2] = -q * s1;
+    R[3] = c2 * q;
+    R[4] = c2 * A * c1 - s2 * d * s1;
+    R[5] = -c2 * A * s1 - s2 * d * c1;
+    R[6] = s2 * q;
+    R[7] = s2 * A * c1 + c2 * d * s1;
+    R[8] = -s2 * A * s1 + c2 * d * c1;
+}
+
+// cos 2i and sin 2i of the angle between the meridian plane of a direction
+// (sine s_self, cosine mu_self) and the scattering plane.  Away from the pole
+// the angle's (cos, sin) are (x, y) normalised (x = num / (den s_self),
+// y = s_other sin(phi) / den; identity when |(x, y)| < 1e-12); at the pole
+// (s_self <= 1e-8) they are (-sign(mu_self) cos phi, sin phi)
+// (surface.c:731-755).
+__device__ inline void plane_rotation(double s_self, double mu_self, double x, double y, double cphi, double sphi,
+                                      double &c2, double &s2)
+{
+    if (s_self <= 1.0e-8) {
+        const double ci = -copysign(1.0, mu_self) * cphi;
+        c2 = ci * ci - sphi * sphi;
+        s2 = 2.0 * ci * sphi;
+        return;
+    }
+    CoxMunkPair::double_angle(x, y, c2, s2);
+}
+
+// Water-side internal reflection: the air-side geometry with the facet
+// Fresnel from water (n) to air (1), identity beyond the critical angle.
+struct RwwPair {
+    double mo, mia, s1, s2, mm, ss, msum, inv_s1, inv_s2, ps_pref, inv_s2v, fac, nw;
+    int qc;
+    __device__ RwwPair(double mu_out, double mu_in, double s2v, double n_water, int q_conv)
+    {
+        mo = mu_out;
+        mia = -mu_in;
+        s1 = sqrt(fmax(0.0, 1.0 - mu_out * mu_out));
+        s2 = sqrt(fmax(0.0, 1.0 - mia * mia));
+        mm = mu_out * mia;
+        ss = s1 * s2;
+        msum = mu_out + mu_in;
+        inv_s1 = 1.0 / fmax(1e-12, s1);
+        inv_s2 = 1.0 / fmax(1e-12, s2);
+        ps_pref = 1.0 / (OCRT_PI * s2v);
+        inv_s2v = 1.0 / s2v;
+        fac = sancer_shadow(mu_out, mu_in, s2v) / (4.0 * mu_in * mu_out);
+        nw = n_water;
+        qc = q_conv;
+    }
+    __device__ void eval(double cphi, double sphi, double *R) const
+    {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) R[k] = 0.0;
+        const double cT = fmin(1.0, fmax(-1.0, mm + ss * cphi));
+        const double co = sqrt(fmax(0.0, 0.5 * (1.0 - cT)));
+        if (co <= 1e-6) return;
+        const double cb = msum / (2.0 * co);
+        if (cb <= 1e-6) return;
+        const double rden = 1.0 / fmax(1e-12, sqrt(fmax(0.0, 1.0 - cT * cT)));
+        double c1, n1, c2, n2;
+        plane_rotation(s2, mia, (mo - mia * cT) * rden * inv_s2, s1 * sphi * rden, cphi, sphi, c1, n1);
+        plane_rotation(s1, mo, (mia - mo * cT) * rden * inv_s1, s2 * sphi * rden, cphi, sphi, c2, n2);
+        double A = 1.0, q = 0.0, d = 1.0;
+        const double st2 = nw * nw * fmax(0.0, 1.0 - co * co);
+        if (st2 < 1.0) {
+            const double mt = sqrt(1.0 - st2);
+            const double rs = (nw * co - mt) / (nw * co + mt), rp = (co - nw * mt) / (co + nw * mt);
+            const double Rs = rs * rs, Rp = rp * rp;
+            A = 0.5 * (Rs + Rp);
+            q = (qc == 1) ? 0.5 * (Rp - Rs) : 0.5 * (Rs - Rp);
+            d = rs * rp;
+        }
+        const double cb2 = cb * cb;
+        const double tb2 = (1.0 - cb2) / fmax(cb2, 1e-8);
+        const double brdf = ps_pref * exp(-tb2 * inv_s2v) * fac / (cb2 * cb2);
+        rotated_mueller(A * brdf, q * brdf, d * brdf, c1, n1, c2, n2, R);
+    }
+};
+
+// Water -> air Walter BTDF with Cox-Munk slopes and Sancer shadowing: the
+// refraction half vector N = n p_water - p_air has |N|^2 = n^2 + 1 - 2 n cos(Psi),
+// the facet tilt is (n mu_w - mu_a) / |N| and the in-water facet incidence
+// (n - cos Psi) / |N|; zero beyond the critical angle.
+struct TwaPair {
+    double m1, m2, s1, s2, mm, ss, inv_s1, inv_s2, ps_pref, inv_s2v, pref, n;
+    int qc;
+    bool live;
+    __device__ TwaPair(double mu_air, double mu_water, double s2v, double n_water, int q_conv)
+    {
+        m1 = mu_water;
+        m2 = mu_air;
+        live = m1 > 1e-9 && m2 > 1e-9;
+        s1 = sqrt(fmax(0.0, 1.0 - m1 * m1));
+        s2 = sqrt(fmax(0.0, 1.0 - m2 * m2));
+        mm = m1 * m2;
+        ss = s1 * s2;
+        inv_s1 = 1.0 / fmax(1e-12, s1);
+        inv_s2 = 1.0 / fmax(1e-12, s2);
+        ps_pref = 1.0 / (OCRT_PI * s2v);
+        inv_s2v = 1.0 / s2v;
+        n = n_water;
+        pref = live ? sancer_shadow(m1, m2, s2v) / (n * m1 * m2) : 0.0;
+        qc = q_conv;
+    }
+    __device__ void eval(double cphi, double sphi, double *R) const
+    {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) R[k] = 0.0;
+        if (!live) return;
+        const double cP = fmin(1.0, fmax(-1.0, mm + ss * cphi));
+        const double N2 = n * n + 1.0 - 2.0 * n * cP;
+        if (N2 <= 1e-12) return;
+        const double iN = 1.0 / sqrt(N2);
+        const double cb = (n * m1 - m2) * iN, cw = (n - cP) * iN;
+        if (cb <= 1e-6 || cw <= 1e-6) return;
+        const double st2 = n * n * fmax(0.0, 1.0 - cw * cw);
+        if (st2 >= 1.0) return;
+        const double ma = sqrt(1.0 - st2);
+        const double rs = (n * cw - ma) / (n * cw + ma), rp = (cw - n * ma) / (cw + n * ma);
+        const double ts = 1.0 + rs, tp = n * (1.0 + rp);
+        const double cb2 = cb * cb;
+        const double tb2 = (1.0 - cb2) / fmax(cb2, 1e-8);
+        const double common = (ma * ma) / N2 * pref * ps_pref * exp(-tb2 * inv_s2v) / (cb2 * cb2);
+        const double fs = common * ts * ts, fp = common * tp * tp;
+        const double rden = 1.0 / fmax(1e-12, sqrt(fmax(0.0, 1.0 - cP * cP)));
+        double c1, n1, c2, n2;
+        plane_rotation(s1, m1, (m2 - m1 * cP) * rden * inv_s1, s2 * sphi * rden, cphi, sphi, c1, n1);
+        plane_rotation(s2, m2, (m1 - m2 * cP) * rden * inv_s2, s1 * sphi * rden, cphi, sphi, c2, n2);
+        rotated_mueller(0.5 * (fs + fp), (qc == 1) ? 0.5 * (fp - fs) : 0.5 * (fs - fp), common * ts * tp, c1, n1,
+                        c2, n2, R);
+    }
+};
+
+// Air -> water microfacet transmission without shadowing (Walter et al. 2007,
+// Beckmann density with the Cox-Munk variance).  Directions point away from
+// the interface: i = (s_i, 0, mu_i) in air, o = (s_o cos phi, s_o sin phi,
+// -mu_o) in water, h = -(i + n o) normalised into the upper hemisphere.  The
+// Stokes rotations follow the photon path: both signed cosines negative and
+// the azimuth shifted by pi (surface.c:1512-1524).
+struct TawPair {
+    double mi, mo, si, so, n, inv_s2v, dpref, inv_si, inv_so, ab;
+    int qc;
+    bool live;
+    __device__ TawPair(double mu_water, double mu_air, double s2v, double n_water, int q_conv)
+    {
+        mi = mu_air;
+        mo = mu_water;
+        live = s2v > 1e-10 && mi > 1e-9 && mo > 1e-9;
+        si = sqrt(fmax(0.0, 1.0 - mi * mi));
+        so = sqrt(fmax(0.0, 1.0 - mo * mo));
+        n = n_water;
+        inv_s2v = 1.0 / s2v;
+        dpref = 1.0 / (OCRT_PI * s2v);
+        inv_si = 1.0 / fmax(1e-12, si);
+        inv_so = 1.0 / fmax(1e-12, so);
+        ab = mi * mo;   // (-mu_i)(-mu_o)
+        qc = q_conv;
+    }
+    __device__ void eval(double cphi, double sphi, double *R) const
+    {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) R[k] = 0.0;
+        if (!live) return;
+        const double ox = so * cphi, oy = so * sphi;
+        double hx = -(si + n * ox), hy = -(n * oy), hz = -(mi - n * mo);
+        const double hn = sqrt(hx * hx + hy * hy + hz * hz);
+        if (hn < 1e-12) return;
+        const double ihn = (hz < 0.0 ? -1.0 : 1.0) / hn;
+        hx *= ihn; hy *= ihn; hz *= ihn;
+        const double ih = si * hx + mi * hz;
+        const double oh = ox * hx + oy * hy - mo * hz;
+        if (ih <= 0.0 || hz <= 1e-9) return;
+        const double hz2 = hz * hz;
+        const double D = exp(-((1.0 - hz2) / hz2) * inv_s2v) * dpref / (hz2 * hz2);
+        const double dh = (ih + n * oh) * (ih + n * oh);
+        if (dh < 1e-12) return;
+        const double stt = sqrt(fmax(0.0, 1.0 - ih * ih)) / n;
+        double Ts = 0.0, Tp = 0.0;
+        if (stt < 1.0) {
+            const double ctt = sqrt(fmax(0.0, 1.0 - stt * stt));
+            const double rs = (ih - n * ctt) / (ih + n * ctt), rp = (n * ih - ctt) / (n * ih + ctt);
+            Ts = 1.0 - rs * rs;
+            Tp = 1.0 - rp * rp;
+        }
+        const double common = (fabs(ih) * fabs(oh)) / (mi * mo) * (n * n * D) / dh;
+        const double fs = common * Ts, fp = common * Tp;
+        const double cr = -cphi, sr = -sphi;
+        const double cP = fmin(1.0, fmax(-1.0, ab + si * so * cr));
+        const double rden = 1.0 / fmax(1e-12, sqrt(fmax(0.0, 1.0 - cP * cP)));
+        double c1, n1, c2, n2;
+        plane_rotation(si, -mi, (-mo + mi * cP) * rden * inv_si, so * sr * rden, cr, sr, c1, n1);
+        plane_rotation(so, -mo, (-mi + mo * cP) * rden * inv_so, si * sr * rden, cr, sr, c2, n2);
+        rotated_mueller(0.5 * (fs + fp), (qc == 1) ? 0.5 * (fp - fs) : 0.5 * (fs - fp),
+                        common * sqrt(fmax(0.0, Ts * Tp)), c1, n1, c2, n2, R);
+    }
+};
+
+// Air-side reflection, through the Cox-Munk evaluator of the atmosphere's
+// boundary kernels.
+struct RaaPair {
+    CoxMunkPair cm;
+    __device__ RaaPair(double mu_out, double mu_in, double s2v, double n_water, int q_conv)
+        : cm(mu_out, mu_in, s2v, sancer_shadow(mu_out, mu_in, s2v), n_water, q_conv)
+    {
+    }
+    __device__ void eval(double cphi, double sphi, double *R) const
+    {
+        double tb2;
+        cm.eval(cphi, sphi, R, tb2);
+    }
+};
+
+template <int KIND> struct PairOf;
+template <> struct PairOf<OCRT_IFACE_R_AA> { using type = RaaPair; };
+template <> struct PairOf<OCRT_IFACE_R_WW> { using type = RwwPair; };
+template <> struct PairOf<OCRT_IFACE_T_WA> { using type = TwaPair; };
+template <> struct PairOf<OCRT_IFACE_T_AW> { using type = TawPair; };
+
+// W[2m][p] = cos(m phi_p), W[2m + 1][p] = sin(m phi_p), phi_p = (p + 1/2) 2 pi / n_phi
+__global__ void interface_weights(int n_phi, int n_rows, double *__restrict__ W)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n_rows * n_phi) return;
+    const int m = (int)(e / n_phi), p = (int)(e % n_phi);
+    const double phi = (p + 0.5) * (2.0 * OCRT_PI / (double)n_phi);
+    W[(2LL * m) * n_phi + p] = cos(m * phi);
+    W[(2LL * m + 1) * n_phi + p] = sin(m * phi);
+}
+
+// One workgroup per node pair (blockIdx.x = o * n_i + i); output
+// out[m][o][i][9] for m = 0..m_max.  Thread m projects onto mode m, summing
+// the azimuths in their natural order, so every mode's bits are the same
+// whatever m_max the launch covers.
+template <int KIND>
+__global__ __launch_bounds__(kIfThreads) void interface_fourier(const double *__restrict__ mu_o, int n_o,
+                                                                const double *__restrict__ mu_i, int n_i,
+                                                                int m_max, int n_phi, double s2v, double nw,
+                                                                int qc, const double *__restrict__ W,
+                                                                double *__restrict__ out)
+{
+    __shared__ double tab[9][kIfThreads];
+    const int pair = blockIdx.x, t = threadIdx.x;
+    const int o = pair / n_i, i = pair % n_i;
+    const typename PairOf<KIND>::type pr(mu_o[o], mu_i[i], s2v, nw, qc);
+    const bool owner = t <= m_max;
+    double acc[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+    for (int p0 = 0; p0 < n_phi; p0 += kIfThreads) {
+        const int p = p0 + t;
+        double R[9];
+        if (p < n_phi) {
+            // cos and sin of the azimuth itself are the m = 1 rows of W
+            pr.eval(W[2LL * n_phi + p], W[3LL * n_phi + p], R);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) R[k] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) tab[k][t] = R[k];
+        __syncthreads();
+        if (owner) {
+            const int len = min(kIfThreads, n_phi - p0);
+            const double *wc = W + (2LL * t) * n_phi + p0, *ws = wc + n_phi;
+            for (int q = 0; q < len; ++q) {
+                const double c = wc[q], s = ws[q];
+                acc[0] = fma(tab[0][q], c, acc[0]);
+                acc[1] = fma(tab[1][q], c, acc[1]);
+                acc[2] = fma(tab[2][q], s, acc[2]);
+                acc[3] = fma(tab[3][q], c, acc[3]);
+                acc[4] = fma(tab[4][q], c, acc[4]);
+                acc[5] = fma(tab[5][q], s, acc[5]);
+                acc[6] = fma(tab[6][q], s, acc[6]);
+                acc[7] = fma(tab[7][q], s, acc[7]);
+                acc[8] = fma(tab[8][q], c, acc[8]);
+            }
+        }
+        __syncthreads();
+    }
+    if (!owner) return;
+    const double dphi = 2.0 * OCRT_PI / (double)n_phi;
+    const double nrm = (t == 0) ? 1.0 / (2.0 * OCRT_PI) : 1.0 / OCRT_PI;
+    double *dst = out + (long long)t * n_o * n_i * 9 + (long long)pair * 9;
+#pragma unroll
+    for (int kl = 0; kl < 9; ++kl) {
+        double v = acc[kl] * (dphi * nrm);
+        // incoming-U column in the sine basis of the reference's Fourier fields
+        if (KIND != OCRT_IFACE_R_WW && t > 0 && (kl == 2 || kl == 5)) v = -v;
+        dst[kl] = v;
+    }
+}
+
+// Facet term of the direct integrals at local incidence cosine c: air -> water
+// transmittance 1 - R (T_aw) or the water -> air reflectance, 1 beyond the
+// critical angle (R_ww).
+template <int KIND> __device__ inline double direct_facet(double c, double nw)
+{
+    if (KIND == OCRT_IFACE_T_AW) {
+        const double stt = sqrt(fmax(0.0, 1.0 - c * c)) / nw, ctt = sqrt(fmax(0.0, 1.0 - stt * stt));
+        const double rs = (c - nw * ctt) / (c + nw * ctt), rp = (nw * c - ctt) / (nw * c + ctt);
+        return 1.0 - 0.5 * (rs * rs + rp * rp);
+    }
+    const double sa = nw * sqrt(fmax(0.0, 1.0 - c * c));
+    if (sa >= 1.0) return 1.0;
+    const double ca = sqrt(fmax(0.0, 1.0 - sa * sa));
+    const double rs = (nw * c - ca) / (nw * c + ca), rp = (c - nw * ca) / (c + nw * ca);
+    return 0.5 * (rs * rs + rp * rp);
+}
+
+// Slope-integrated direct term for one cosine per workgroup: trapezoid rule
+// on 101 x 101 facet slopes over +-6 sigma, one slope row per thread.
+template <int KIND>
+__global__ __launch_bounds__(kDirectThreads) void interface_direct(const double *__restrict__ mu, double s2v,
+                                                                   double nw, double *__restrict__ out)
+{
+    __shared__ double red[kDirectThreads / 64];
+    const double m = mu[blockIdx.x];
+    const int t = threadIdx.x;
+    if (m <= 1e-9 || s2v <= 1e-10) {
+        if (t == 0) out[blockIdx.x] = m <= 1e-9 ? (KIND == OCRT_IFACE_T_AW ? 0.0 : 1.0) : direct_facet<KIND>(m, nw);
+        return;
+    }
+    constexpr int N = 101;
+    const double sn = sqrt(fmax(0.0, 1.0 - m * m));
+    const double L = 6.0 * sqrt(s2v), dz = 2.0 * L / (double)(N - 1);
+    const double dens = 1.0 / (OCRT_PI * s2v);
+    double acc = 0.0;
+    if (t < N) {
+        const double zx = -L + t * dz;
+        const double wx = (t == 0 || t == N - 1) ? 0.5 : 1.0;
+        const double proj = m + zx * sn;
+        if (proj > 0.0)
+            for (int j = 0; j < N; ++j) {
+                const double zy = -L + j * dz;
+                const double wy = (j == 0 || j == N - 1) ? 0.5 : 1.0;
+                const double r2 = zx * zx + zy * zy;
+                acc += wx * wy * (dens * exp(-r2 / s2v)) * proj * direct_facet<KIND>(proj / sqrt(1.0 + r2), nw);
+            }
+    }
+    acc = wave_sum(acc);
+    if ((t & 63) == 0) red[t >> 6] = acc;
+    __syncthreads();
+    if (t == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kDirectThreads / 64; ++w) s += red[w];
+        const double v = s * dz * dz / m;
+        out[blockIdx.x] = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+    }
+}
+
+double host_slope_variance(double ws, int sigma_type)
+{
+    if (sigma_type == 1) return 0.003 + 0.00512 * std::fmax(0.01, ws);
+    const double s = 0.0731 * std::sqrt(std::fmax(0.0, ws));
+    return std::fmax(s * s, 1e-10);
+}
+
+bool cosines_ok(const double *mu, int n)
+{
+    for (int k = 0; k < n; ++k)
+        if (!(mu[k] > 0.0 && mu[k] <= 1.0)) return false;
+    return true;
+}
+
+// Device buffers of one call, released on every exit path.
+struct DevBufs {
+    std::vector<void *> p;
+    ~DevBufs()
+    {
+        for (void *q : p) (void)hipFree(q);
+    }
+    template <typename T> hipError_t get(T **ptr, size_t bytes)
+    {
+        void *q = nullptr;
+        const hipError_t e = hipMalloc(&q, bytes ? bytes : 256);
+        if (e == hipSuccess) p.push_back(q);
+        *ptr = (T *)q;
+        return e;
+    }
+};
+
+#define IFCHK(x)                                                                          \
+    do {                                                                                  \
+        const hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess) {                                                           \
+            fprintf(stderr, "ocrt_amd: %s failed: %s\n", #x, hipGetErrorString(e_));      \
+            return OCRT_ERR_DEVICE;                                                       \
+        }                                                                                 \
+    } while (0)
+
+}  // namespace
+}  // namespace ocrt
+
+using namespace ocrt;
+
+extern "C" int ocrt_interface_fourier(int kind, const double *mu_o, int n_o, const double *mu_i, int n_i, int m_max,
+                                      int n_phi, double wind, int sigma_type, double n_water, int q_convention,
+                                      double *out, void *hip_stream)
+{
+    if (kind < OCRT_IFACE_R_AA || kind > OCRT_IFACE_T_AW || !mu_o || !mu_i || !out) return OCRT_ERR_ARGS;
+    if (n_o < 1 || n_i < 1 || (long long)n_o * n_i > kIfMaxPairs) return OCRT_ERR_ARGS;
+    if (m_max < 0 || m_max + 1 > kIfMaxModes || n_phi < 4 || n_phi > OCRT_IFACE_MAX_PHI) return OCRT_ERR_ARGS;
+    if (!std::isfinite(wind) || !(n_water > 0.0) || !std::isfinite(n_water)) return OCRT_ERR_ARGS;
+    if (!cosines_ok(mu_o, n_o) || !cosines_ok(mu_i, n_i)) return OCRT_ERR_ARGS;
+    const double s2v = host_slope_variance(wind, sigma_type);
+    if (kind == OCRT_IFACE_T_AW && s2v <= 1e-10) return OCRT_ERR_FLAT;
+    hipStream_t st = (hipStream_t)hip_stream;
+    const int w_rows = m_max > 1 ? m_max + 1 : 2;   // the m = 1 rows carry cos/sin of the azimuth
+    const size_t n_out = (size_t)(m_max + 1) * n_o * n_i * 9;
+    DevBufs B;
+    double *d_mo, *d_mi, *d_w, *d_out;
+    IFCHK(B.get(&d_mo, sizeof(double) * n_o));
+    IFCHK(B.get(&d_mi, sizeof(double) * n_i));
+    IFCHK(B.get(&d_w, sizeof(double) * 2 * (size_t)w_rows * n_phi));
+    IFCHK(B.get(&d_out, sizeof(double) * n_out));
+    IFCHK(hipMemcpyAsync(d_mo, mu_o, sizeof(double) * n_o, hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_mi, mu_i, sizeof(double) * n_i, hipMemcpyHostToDevice, st));
+    const long long n_w = (long long)w_rows * n_phi;
+    interface_weights<<<(unsigned)((n_w + 255) / 256), 256, 0, st>>>(n_phi, w_rows, d_w);
+    IFCHK(hipGetLastError());
+    const unsigned grid = (unsigned)((long long)n_o * n_i);
+    switch (kind) {
+    case OCRT_IFACE_R_AA:
+        interface_fourier<OCRT_IFACE_R_AA><<<grid, kIfThreads, 0, st>>>(d_mo, n_o, d_mi, n_i, m_max, n_phi, s2v,
+                                                                         n_water, q_convention, d_w, d_out);
+        break;
+    case OCRT_IFACE_R_WW:
+        interface_fourier<OCRT_IFACE_R_WW><<<grid, kIfThreads, 0, st>>>(d_mo, n_o, d_mi, n_i, m_max, n_phi, s2v,
+                                                                         n_water, q_convention, d_w, d_out);
+        break;
+    case OCRT_IFACE_T_WA:
+        interface_fourier<OCRT_IFACE_T_WA><<<grid, kIfThreads, 0, st>>>(d_mo, n_o, d_mi, n_i, m_max, n_phi, s2v,
+                                                                         n_water, q_convention, d_w, d_out);
+        break;
+    default:
+        interface_fourier<OCRT_IFACE_T_AW><<<grid, kIfThreads, 0, st>>>(d_mo, n_o, d_mi, n_i, m_max, n_phi, s2v,
+                                                                         n_water, q_convention, d_w, d_out);
+        break;
+    }
+    IFCHK(hipGetLastError());
+    IFCHK(hipMemcpyAsync(out, d_out, sizeof(double) * n_out, hipMemcpyDeviceToHost, st));
+    IFCHK(hipStreamSynchronize(st));
+    return OCRT_OK;
+}
+
+extern "C" int ocrt_interface_direct(int kind, const double *mu, int n, double wind, int sigma_type, double n_water,
+                                     double *out, void *hip_stream)
+{
+    if ((kind != OCRT_IFACE_T_AW && kind != OCRT_IFACE_R_WW) || !mu || !out || n < 1 || n >= (1 << 24))
+        return OCRT_ERR_ARGS;
+    if (!std::isfinite(wind) || !(n_water > 0.0) || !std::isfinite(n_water)) return OCRT_ERR_ARGS;
+    for (int k = 0; k < n; ++k)
+        if (!std::isfinite(mu[k]) || mu[k] > 1.0) return OCRT_ERR_ARGS;
+    const double s2v = host_slope_variance(wind, sigma_type);
+    hipStream_t st = (hipStream_t)hip_stream;
+    DevBufs B;
+    double *d_mu, *d_out;
+    IFCHK(B.get(&d_mu, sizeof(double) * n));
+    IFCHK(B.get(&d_out, sizeof(double) * n));
+    IFCHK(hipMemcpyAsync(d_mu, mu, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    if (kind == OCRT_IFACE_T_AW)
+        interface_direct<OCRT_IFACE_T_AW><<<n, kDirectThreads, 0, st>>>(d_mu, s2v, n_water, d_out);
+    else
+        interface_direct<OCRT_IFACE_R_WW><<<n, kDirectThreads, 0, st>>>(d_mu, s2v, n_water, d_out);
+    IFCHK(hipGetLastError());
+    IFCHK(hipMemcpyAsync(out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    IFCHK(hipStreamSynchronize(st));
+    return OCRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The reference's per-mode surface functions (shared/surface.h:40-61, 222-281),
+// same names, arguments and return codes.  The first call for a node set and
+// surface computes every mode 0..max(m, build cap) on the GPU in one launch
+// and keeps them in a per-thread cache; later modes of the same key are
+// copied from it, as the reference's all-mode cache does (surface.c:1263-1380).
+namespace {
+
+constexpr int kFkcModes = 64;   // default build cap, the reference's OCRT_FKC_MMAX
+thread_local int g_build_cap = kFkcModes;
+
+struct ModeCache {
+    bool valid = false;
+    int n_o = 0, n_i = 0, n_phi = 0, st = 0, qc = 0, built = -1;
+    double ws = 0.0, nw = 0.0;
+    std::vector<double> mo, mi, coef;
+};
+thread_local ModeCache g_cache[4];
+
+int serve_mode(int kind, const double *mu_o, int n_o, const double *mu_i, int n_i, int m, int n_phi, double ws,
+               int st, double nw, int qc, double *R_m)
+{
+    if (!mu_o || !mu_i || !R_m) return -1;
+    if (n_o < 1 || n_i < 1 || m < 0 || n_phi < 4) return -1;
+    if (kind == OCRT_IFACE_T_AW && host_slope_variance(ws, st) <= 1e-10) return -2;
+    ModeCache &C = g_cache[kind];
+    const size_t slab = (size_t)n_o * n_i * 9;
+    const bool hit = C.valid && C.n_o == n_o && C.n_i == n_i && C.n_phi == n_phi && C.st == st && C.qc == qc &&
+                     C.ws == ws && C.nw == nw && m <= C.built &&
+                     std::memcmp(C.mo.data(), mu_o, sizeof(double) * n_o) == 0 &&
+                     std::memcmp(C.mi.data(), mu_i, sizeof(double) * n_i) == 0;
+    if (!hit) {
+        const int top = std::min(kIfMaxModes - 1, std::max(m, g_build_cap));
+        if (m > top) return -1;
+        C.valid = false;
+        C.coef.assign((size_t)(top + 1) * slab, 0.0);
+        const int rc = ocrt_interface_fourier(kind, mu_o, n_o, mu_i, n_i, top, n_phi, ws, st, nw, qc,
+                                              C.coef.data(), nullptr);
+        if (rc != OCRT_OK) return rc == OCRT_ERR_ARGS ? -1 : -3;
+        C.mo.assign(mu_o, mu_o + n_o);
+        C.mi.assign(mu_i, mu_i + n_i);
+        C.n_o = n_o; C.n_i = n_i; C.n_phi = n_phi; C.st = st; C.qc = qc; C.ws = ws; C.nw = nw;
+        C.built = top;
+        C.valid = true;
+    }
+    std::memcpy(R_m, C.coef.data() + (size_t)m * slab, sizeof(double) * slab);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" void surface_fkc_set_build_mmax(int m_max)
+{
+    g_build_cap = (m_max >= 0 && m_max <= kFkcModes) ? m_max : kFkcModes;
+}
+
+extern "C" int surface_coxmunk_fourier_kernel(const double *mu_o, int n_o, const double *mu_i, int n_i, int m,
+                                              int n_phi_quad, double ws, int sigma_type, double n_water,
+                                              int q_convention, double *R_m)
+{
+    return serve_mode(OCRT_IFACE_R_AA, mu_o, n_o, mu_i, n_i, m, n_phi_quad, ws, sigma_type, n_water, q_convention,
+                      R_m);
+}
+
+extern "C" int surface_R_ww_coxmunk_fourier_kernel(const double *mu_o, int n_o, const double *mu_i, int n_i, int m,
+                                                   int n_phi_quad, double ws, int sigma_type, double n_water,
+                                                   int q_convention, double *R_m)
+{
+    return serve_mode(OCRT_IFACE_R_WW, mu_o, n_o, mu_i, n_i, m, n_phi_quad, ws, sigma_type, n_water, q_convention,
+                      R_m);
+}
+
+extern "C" int surface_T_wa_coxmunk_fourier_kernel(const double *mu_o, int n_o, const double *mu_i, int n_i, int m,
+                                                   int n_phi_quad, double ws, int sigma_type, double n_water,
+                                                   int q_convention, double *T_m)
+{
+    return serve_mode(OCRT_IFACE_T_WA, mu_o, n_o, mu_i, n_i, m, n_phi_quad, ws, sigma_type, n_water, q_convention,
+                      T_m);
+}
+
+extern "C" int surface_T_aw_coxmunk_fourier_kernel(const double *mu_o, int n_o, const double *mu_i, int n_i, int m,
+                                                   int n_phi_quad, double ws, int sigma_type, double n_water,
+                                                   int q_convention, double *T_m)
+{
+    return serve_mode(OCRT_IFACE_T_AW, mu_o, n_o, mu_i, n_i, m, n_phi_quad, ws, sigma_type, n_water, q_convention,
+                      T_m);
+}
+
+extern "C" double surface_T_aw_coxmunk_direct(double mu_sun_air, double n_water, double wind_speed, int sigma_type)
+{
+    double v = 0.0;
+    if (ocrt_interface_direct(OCRT_IFACE_T_AW, &mu_sun_air, 1, wind_speed, sigma_type, n_water, &v, nullptr) != 0)
+        return NAN;
+    return v;
+}
+
+extern "C" double surface_R_ww_coxmunk_direct(double mu_up_water, double n_water, double wind_speed, int sigma_type)
+{
+    double v = 0.0;
+    if (ocrt_interface_direct(OCRT_IFACE_R_WW, &mu_up_water, 1, wind_speed, sigma_type, n_water, &v, nullptr) != 0)
+        return NAN;
+    return v;
+}

@@ -451,14 +451,215 @@ void oracle_coxmunk_R(double mu_out, double mu_in_dn, double cphi, double sphi, 
     for (int i = 0; i < 9; ++i) R[i] = T2[i] * brdf;
 }
 
-/* Azimuthal Fourier coefficients of the Cox-Munk Mueller kernel for every
- * mode at once (surface.c:839-938 per mode; 1263-1380 all modes): midpoint
- * phi rule, cos for II IQ QI QQ UU and sin for the other entries,
- * normalisation 1/(2 pi) at m = 0 and 1/pi above, IU/QU negated for m > 0. */
-int oracle_coxmunk_fourier(const double *mu_o, int n_o, const double *mu_i, int n_i, int m_max,
-                           int n_phi, double ws, int st, double nw, int qc, double *out)
+/* ------------------------------------------------------------------------ */
+/* Rough-interface Mueller kernels of the coupled ocean (shared/surface.c).
+ * kind 0 R_aa: air-side reflection (:514-646), oracle_coxmunk_R above;
+ * kind 1 R_ww: water-side internal reflection (:1136-1249);
+ * kind 2 T_wa: water -> air transmission (:648-773);
+ * kind 3 T_aw: air -> water transmission (:80-136, :1476-1566).
+ * Arguments follow the reference's Fourier-kernel layout: mu_o is the
+ * outgoing and mu_i the incident zenith cosine of the pair (for T_aw mu_o is
+ * the in-water and mu_i the air cosine). */
+
+/* Flat-interface reflection between indices n1 (incidence) and n2, meridian
+ * basis; identity under total internal reflection (surface.c:385-430). */
+static void fresnel_R_general(double mu_i, double n1, double n2, int qc, double *M)
+{
+    for (int i = 0; i < 9; ++i) M[i] = 0.0;
+    const double sin2 = fmax(0.0, 1.0 - mu_i * mu_i);
+    const double st2 = (n1 / n2) * (n1 / n2) * sin2;
+    if (st2 >= 1.0) { M[0] = M[4] = M[8] = 1.0; return; }
+    const double mt = sqrt(1.0 - st2);
+    const double rs = (n1 * mu_i - n2 * mt) / (n1 * mu_i + n2 * mt);
+    const double rp = (n2 * mu_i - n1 * mt) / (n2 * mu_i + n1 * mt);
+    const double Rs = rs * rs, Rp = rp * rp;
+    const double q = (qc == 1) ? 0.5 * (Rp - Rs) : 0.5 * (Rs - Rp);
+    M[0] = 0.5 * (Rs + Rp); M[1] = q; M[3] = q; M[4] = 0.5 * (Rs + Rp); M[8] = rs * rp;
+}
+
+/* Sancer bistatic shadowing 1 / (1 + Lambda_a + Lambda_b) (surface.c:591-612). */
+static double sancer(double mu_a, double mu_b, double s2v)
+{
+    const double sl = sqrt(s2v);
+    double lam[2] = {0.0, 0.0};
+    const double mus[2] = {mu_a, mu_b};
+    for (int q = 0; q < 2; ++q)
+        if (mus[q] < 1.0 - 1e-10) {
+            const double sn = sqrt(fmax(1e-20, 1.0 - mus[q] * mus[q]));
+            const double nu = (mus[q] / sn) / sl;
+            if (nu <= 20.0) lam[q] = 0.5 * (exp(-nu * nu) / (nu * sqrt(PI)) - erfc(nu));
+        }
+    return 1.0 / (1.0 + lam[0] + lam[1]);
+}
+
+/* cos/sin of the meridian-to-scattering-plane angle at one end of the path;
+ * the pole limit and the normalisation follow surface.c:731-755. */
+static void meridian_angle(double s_self, double mu_self, double num, double s_other_sphi, double den,
+                           double cphi, double sphi, double *c, double *s)
+{
+    if (s_self <= 1.0e-8) { *c = -copysign(1.0, mu_self) * cphi; *s = sphi; return; }
+    *c = num / (den * fmax(1e-12, s_self));
+    *s = s_other_sphi / den;
+    const double q = sqrt(*c * *c + *s * *s);
+    if (q < 1e-12) { *c = 1.0; *s = 0.0; } else { *c /= q; *s /= q; }
+}
+
+/* L(2 i_out) . M . L(2 i_in) */
+static void rotate_pair(const double *M, double c_in, double s_in, double c_out, double s_out, double *out)
+{
+    double L1[9], L2[9], T1[9];
+    rot_L(c_in, s_in, L1);
+    rot_L(c_out, s_out, L2);
+    mul3(M, L1, T1);
+    mul3(L2, T1, out);
+}
+
+static void rww_trig(double mu_out, double mu_in, double cphi, double sphi, double ws, int st, double nw,
+                     int qc, double *R)
+{
+    for (int i = 0; i < 9; ++i) R[i] = 0.0;
+    const double s2v = slope_variance(ws, st);
+    const double mia = -mu_in;
+    const double s1 = sqrt(fmax(0.0, 1.0 - mu_out * mu_out));
+    const double s2 = sqrt(fmax(0.0, 1.0 - mia * mia));
+    double cT = mu_out * mia + s1 * s2 * cphi;
+    if (cT > 1.0) cT = 1.0; else if (cT < -1.0) cT = -1.0;
+    const double co = sqrt(fmax(0.0, 0.5 * (1.0 - cT)));
+    if (co <= 1e-6) return;
+    const double cb = (mu_out + mu_in) / (2.0 * co);
+    if (cb <= 1e-6) return;
+    const double den = fmax(1e-12, sqrt(fmax(0.0, 1.0 - cT * cT)));
+    double c1, n1, c2, n2;
+    meridian_angle(s2, mia, mu_out - mia * cT, s1 * sphi, den, cphi, sphi, &c1, &n1);
+    meridian_angle(s1, mu_out, mia - mu_out * cT, s2 * sphi, den, cphi, sphi, &c2, &n2);
+    double MF[9];
+    fresnel_R_general(co, nw, 1.0, qc, MF);   /* water -> air facet, TIR included */
+    const double cb2 = cb * cb;
+    const double tb2 = (1.0 - cb2) / fmax(cb2, 1e-8);
+    const double Pslope = (1.0 / (PI * s2v)) * exp(-tb2 / s2v);
+    const double brdf = Pslope * sancer(mu_in, mu_out, s2v) / (4.0 * mu_in * mu_out * cb2 * cb2);
+    double T[9];
+    rotate_pair(MF, c1, n1, c2, n2, T);
+    for (int i = 0; i < 9; ++i) R[i] = T[i] * brdf;
+}
+
+/* Water -> air Walter BTDF with Cox-Munk slopes and Sancer shadowing; the
+ * refraction half vector N = n p1 - p2 gives |N|^2 = n^2 + 1 - 2 n cos(Psi). */
+static void twa_trig(double mu_air, double mu_water, double cphi, double sphi, double ws, int st, double nw,
+                     int qc, double *T)
+{
+    for (int i = 0; i < 9; ++i) T[i] = 0.0;
+    const double m1 = mu_water, m2 = mu_air, n = nw;
+    if (m1 <= 1e-9 || m2 <= 1e-9) return;
+    const double s2v = slope_variance(ws, st);
+    const double s1 = sqrt(fmax(0.0, 1.0 - m1 * m1)), s2 = sqrt(fmax(0.0, 1.0 - m2 * m2));
+    double cP = m1 * m2 + s1 * s2 * cphi;
+    if (cP > 1.0) cP = 1.0; else if (cP < -1.0) cP = -1.0;
+    const double N2 = n * n + 1.0 - 2.0 * n * cP;
+    if (N2 <= 1e-12) return;
+    const double N = sqrt(N2);
+    const double cb = (n * m1 - m2) / N, cw = (n - cP) / N;
+    if (cb <= 1e-6 || cw <= 1e-6) return;
+    /* facet transmission water(n) -> air(1) at incidence cw */
+    const double st2 = n * n * fmax(0.0, 1.0 - cw * cw);
+    if (st2 >= 1.0) return;
+    const double ma = sqrt(1.0 - st2);
+    const double rs = (n * cw - ma) / (n * cw + ma);
+    const double rp = (cw - n * ma) / (cw + n * ma);
+    const double ts = 1.0 + rs, tp = n * (1.0 + rp);
+    const double cb2 = cb * cb;
+    const double tb2 = (1.0 - cb2) / fmax(cb2, 1e-8);
+    const double Pslope = (1.0 / (PI * s2v)) * exp(-tb2 / s2v);
+    const double common = (ma * ma) / (n * m1 * m2 * N2) * (Pslope / (cb2 * cb2)) * sancer(m1, m2, s2v);
+    const double fs = common * ts * ts, fp = common * tp * tp;
+    double MT[9] = {0};
+    const double q = (qc == 1) ? 0.5 * (fp - fs) : 0.5 * (fs - fp);
+    MT[0] = MT[4] = 0.5 * (fs + fp);
+    MT[1] = MT[3] = q;
+    MT[8] = common * ts * tp;
+    const double den = fmax(1e-12, sqrt(fmax(0.0, 1.0 - cP * cP)));
+    double c1, n1, c2, n2;
+    meridian_angle(s1, m1, m2 - m1 * cP, s2 * sphi, den, cphi, sphi, &c1, &n1);
+    meridian_angle(s2, m2, m1 - m2 * cP, s1 * sphi, den, cphi, sphi, &c2, &n2);
+    rotate_pair(MT, c1, n1, c2, n2, T);
+}
+
+/* Air -> water microfacet transmission without shadowing (Walter et al. 2007,
+ * Beckmann density with the Cox-Munk variance).  Directions point away from
+ * the interface: i = (s_i, 0, mu_i) in air, o = (s_o cos phi, s_o sin phi,
+ * -mu_o) in water, h = -(i + n o) normalised to the upper hemisphere. */
+static void taw_trig(double mu_water, double mu_air, double cphi, double sphi, double ws, int st, double nw,
+                     int qc, double *T)
+{
+    for (int i = 0; i < 9; ++i) T[i] = 0.0;
+    const double s2v = slope_variance(ws, st);
+    const double mi = mu_air, mo = mu_water, n = nw;
+    if (s2v <= 1e-10 || mi <= 1e-9 || mo <= 1e-9) return;
+    const double si = sqrt(fmax(0.0, 1.0 - mi * mi)), so = sqrt(fmax(0.0, 1.0 - mo * mo));
+    const double ox = so * cphi, oy = so * sphi;
+    double h[3] = {-(si + n * ox), -(n * oy), -(mi - n * mo)};
+    const double hn = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    if (hn < 1e-12) return;
+    for (int k = 0; k < 3; ++k) h[k] /= hn;
+    if (h[2] < 0.0) for (int k = 0; k < 3; ++k) h[k] = -h[k];
+    const double ih = si * h[0] + mi * h[2];
+    const double oh = ox * h[0] + oy * h[1] - mo * h[2];
+    if (ih <= 0.0 || h[2] <= 1e-9) return;
+    const double tan2 = (1.0 - h[2] * h[2]) / (h[2] * h[2]);
+    const double D = exp(-tan2 / s2v) / (PI * s2v * h[2] * h[2] * h[2] * h[2]);
+    const double dh = (ih + n * oh) * (ih + n * oh);
+    /* facet Fresnel air(1) -> water(n) at incidence ih */
+    const double sti = sqrt(fmax(0.0, 1.0 - ih * ih)), stt = sti / n;
+    double Ts = 0.0, Tp = 0.0;
+    if (stt < 1.0) {
+        const double ctt = sqrt(fmax(0.0, 1.0 - stt * stt));
+        const double rs = (ih - n * ctt) / (ih + n * ctt), rp = (n * ih - ctt) / (n * ih + ctt);
+        Ts = 1.0 - rs * rs;
+        Tp = 1.0 - rp * rp;
+    }
+    if (dh < 1e-12) return;
+    const double common = (fabs(ih) * fabs(oh)) / (mi * mo) * (n * n * D) / dh;
+    const double fs = common * Ts, fp = common * Tp;
+    double MT[9] = {0};
+    const double q = (qc == 1) ? 0.5 * (fp - fs) : 0.5 * (fs - fp);
+    MT[0] = MT[4] = 0.5 * (fs + fp);
+    MT[1] = MT[3] = q;
+    MT[8] = common * sqrt(fmax(0.0, Ts * Tp));
+    /* rotations along the photon path: both signed cosines negative and the
+     * azimuth shifted by pi (surface.c:1512-1524) */
+    const double cr = -cphi, sr = -sphi, a = -mi, b = -mo;
+    double cP = a * b + si * so * cr;
+    if (cP > 1.0) cP = 1.0; else if (cP < -1.0) cP = -1.0;
+    const double den = fmax(1e-12, sqrt(fmax(0.0, 1.0 - cP * cP)));
+    double c1, n1, c2, n2;
+    meridian_angle(si, a, b - a * cP, so * sr, den, cr, sr, &c1, &n1);
+    meridian_angle(so, b, a - b * cP, si * sr, den, cr, sr, &c2, &n2);
+    rotate_pair(MT, c1, n1, c2, n2, T);
+}
+
+void oracle_interface_trig(int kind, double mu_o, double mu_i, double cphi, double sphi, double ws,
+                           int st, double nw, int qc, double *M)
+{
+    switch (kind) {
+    case 0: oracle_coxmunk_R(mu_o, mu_i, cphi, sphi, ws, st, nw, qc, M); break;
+    case 1: rww_trig(mu_o, mu_i, cphi, sphi, ws, st, nw, qc, M); break;
+    case 2: twa_trig(mu_o, mu_i, cphi, sphi, ws, st, nw, qc, M); break;
+    default: taw_trig(mu_o, mu_i, cphi, sphi, ws, st, nw, qc, M); break;
+    }
+}
+
+/* Azimuthal Fourier coefficients of one interface kernel for every mode at
+ * once (surface.c:839-938, 1386-1447, 1583-1721, 1731-1811 per mode; the
+ * all-mode form :1263-1380): midpoint phi rule, cos for II IQ QI QQ UU and
+ * sin for the other entries, normalisation 1/(2 pi) at m = 0 and 1/pi above;
+ * IU and QU negated for m > 0 except for R_ww.  T_aw of a flat surface
+ * (slope variance <= 1e-10) has no rough kernel: -2. */
+int oracle_interface_fourier(int kind, const double *mu_o, int n_o, const double *mu_i, int n_i, int m_max,
+                             int n_phi, double ws, int st, double nw, int qc, double *out)
 {
     static const int cos_entry[9] = {1, 1, 0, 1, 1, 0, 0, 0, 1};
+    if (n_o < 1 || n_i < 1 || m_max < 0 || n_phi < 4) return -1;
+    if (kind == 3 && slope_variance(ws, st) <= 1e-10) return -2;
     const double dphi = 2.0 * PI / (double)n_phi;
     const size_t slab = (size_t)n_o * n_i * 9;
     double *W = (double *)malloc(sizeof(double) * 2 * (size_t)(m_max + 1) * n_phi);
@@ -476,7 +677,7 @@ int oracle_coxmunk_fourier(const double *mu_o, int n_o, const double *mu_i, int 
             for (int p = 0; p < n_phi; ++p) {
                 const double phi = (p + 0.5) * dphi;
                 double R[9];
-                oracle_coxmunk_R(mu_o[o], mu_i[i], cos(phi), sin(phi), ws, st, nw, qc, R);
+                oracle_interface_trig(kind, mu_o[o], mu_i[i], cos(phi), sin(phi), ws, st, nw, qc, R);
                 for (int m = 0; m <= m_max; ++m) {
                     const double cm = W[(2 * (size_t)m) * n_phi + p];
                     const double sm = W[(2 * (size_t)m + 1) * n_phi + p];
@@ -487,12 +688,67 @@ int oracle_coxmunk_fourier(const double *mu_o, int n_o, const double *mu_i, int 
                 const double nrm = (m == 0) ? 1.0 / (2.0 * PI) : 1.0 / PI;
                 double *d = out + (size_t)m * slab + ((size_t)o * n_i + i) * 9;
                 for (int kl = 0; kl < 9; ++kl) d[kl] = acc[m * 9 + kl] * (dphi * nrm);
-                if (m > 0) { d[2] = -d[2]; d[5] = -d[5]; }
+                if (m > 0 && kind != 1) { d[2] = -d[2]; d[5] = -d[5]; }
             }
         }
     free(W);
     free(acc);
     return 0;
+}
+
+int oracle_coxmunk_fourier(const double *mu_o, int n_o, const double *mu_i, int n_i, int m_max,
+                           int n_phi, double ws, int st, double nw, int qc, double *out)
+{
+    return oracle_interface_fourier(0, mu_o, n_o, mu_i, n_i, m_max, n_phi, ws, st, nw, qc, out);
+}
+
+/* Slope-integrated direct-beam irradiance terms (no shadowing), trapezoid rule
+ * on 101 x 101 slopes over +-6 sigma:
+ *   kind 3: air -> water transmittance T_aw(mu_sun) (surface.c:138-198);
+ *   kind 1: water-side reflectance R_ww(mu_up), the first-encounter form
+ *           (surface.c:229-238, 302-368). */
+/* facet Fresnel term of the direct integrals at local incidence cosine c:
+ * air -> water transmittance 1 - R (kind 3) or the water -> air reflectance,
+ * 1 beyond the critical angle (kind 1) */
+static double direct_facet(int kind, double c, double nw)
+{
+    if (kind == 3) {
+        const double stt = sqrt(fmax(0.0, 1.0 - c * c)) / nw, ctt = sqrt(fmax(0.0, 1.0 - stt * stt));
+        const double rs = (c - nw * ctt) / (c + nw * ctt), rp = (nw * c - ctt) / (nw * c + ctt);
+        return 1.0 - 0.5 * (rs * rs + rp * rp);
+    }
+    const double sa = nw * sqrt(fmax(0.0, 1.0 - c * c));
+    if (sa >= 1.0) return 1.0;
+    const double ca = sqrt(fmax(0.0, 1.0 - sa * sa));
+    const double rs = (nw * c - ca) / (nw * c + ca), rp = (c - nw * ca) / (c + nw * ca);
+    return 0.5 * (rs * rs + rp * rp);
+}
+
+double oracle_interface_direct(int kind, double mu, double ws, int st, double nw)
+{
+    if (mu <= 1e-9) return kind == 3 ? 0.0 : 1.0;
+    const double s2 = slope_variance(ws, st);
+    const double sn = sqrt(fmax(0.0, 1.0 - mu * mu));
+    if (s2 <= 1e-10) return direct_facet(kind, mu, nw);
+    double acc = 0.0;
+    const double sig = sqrt(s2), L = 6.0 * sig;
+    const int N = 101;
+    const double dz = 2.0 * L / (double)(N - 1);
+    for (int i = 0; i < N; ++i) {
+        const double zx = -L + i * dz;
+        const double wx = (i == 0 || i == N - 1) ? 0.5 : 1.0;
+        const double proj = mu + zx * sn;
+        if (proj <= 0.0) continue;
+        for (int j = 0; j < N; ++j) {
+            const double zy = -L + j * dz;
+            const double wy = (j == 0 || j == N - 1) ? 0.5 : 1.0;
+            const double r2 = zx * zx + zy * zy;
+            const double F = direct_facet(kind, proj / sqrt(1.0 + r2), nw);
+            acc += wx * wy * ((1.0 / (PI * s2)) * exp(-r2 / s2)) * proj * F;
+        }
+    }
+    const double v = acc * dz * dz / mu;
+    return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
 }
 
 /* Direct sunglint (surface.c:1813-1907). */

@@ -71,6 +71,20 @@ void oracle_flat_fresnel(double mu, double n_water, int q_convention, double *MF
 int oracle_coxmunk_fourier(const double *mu_o, int n_o, const double *mu_i, int n_i,
                            int m_max, int n_phi, double ws, int sigma_type,
                            double n_water, int q_convention, double *out);
+/* Rough-interface kernels of the coupled ocean, kind 0 R_aa (air-side
+ * reflection), 1 R_ww (water-side reflection), 2 T_wa (water -> air), 3 T_aw
+ * (air -> water); mu_o outgoing, mu_i incident zenith cosine
+ * (shared/surface.c:80-136, 514-803, 1136-1249, 1476-1566). */
+void oracle_interface_trig(int kind, double mu_o, double mu_i, double cos_phi, double sin_phi,
+                           double ws, int sigma_type, double n_water, int q_convention, double *M);
+/* their Fourier kernels for all m <= m_max: out [(m_max+1)][n_o][n_i][9];
+ * -2 for T_aw on a flat surface */
+int oracle_interface_fourier(int kind, const double *mu_o, int n_o, const double *mu_i, int n_i,
+                             int m_max, int n_phi, double ws, int sigma_type, double n_water,
+                             int q_convention, double *out);
+/* slope-integrated direct terms: kind 3 T_aw(mu_sun), kind 1 R_ww(mu_up)
+ * (surface.c:138-198, 229-368) */
+double oracle_interface_direct(int kind, double mu, double ws, int sigma_type, double n_water);
 /* IPSS kappa for an instance at (vza, raa) (rt_ipss.c:607-644) */
 double oracle_ipss_kappa(const ocrt_instance_t *in, double vza_deg, double raa_deg);
 double oracle_ipss_tau_dir_dn(const ocrt_instance_t *in);

@@ -87,3 +87,17 @@ def oracle_solve(lib, d):
     rc = lib.oracle_solve(C.byref(d), C.byref(r), modes.ctypes.data_as(C.c_void_p),
                           orders.ctypes.data_as(C.c_void_p), None)
     return rc, r, modes.reshape(-1, 3), orders
+
+
+@pytest.fixture(scope="session")
+def orc(oracle):
+    """The oracle with the argument types of its interface-kernel functions."""
+    D = C.POINTER(C.c_double)
+    oracle.oracle_interface_fourier.argtypes = [C.c_int, D, C.c_int, D, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                C.c_int, C.c_double, C.c_int, D]
+    oracle.oracle_interface_trig.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                             C.c_int, C.c_double, C.c_int, D]
+    oracle.oracle_interface_trig.restype = None
+    oracle.oracle_interface_direct.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double]
+    oracle.oracle_interface_direct.restype = C.c_double
+    return oracle

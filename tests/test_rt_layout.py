@@ -65,7 +65,7 @@ def declared(header):
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     # static inline helpers live in the header, not the library
     text = re.sub(r"static inline[^{]*\{.*?\n\}", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:rt|ocrt)_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:rt|ocrt|surface)_[a-zA-Z_0-9]+)\s*\(", text)))
 
 
 def test_library_exports_reference_entry_points():
