@@ -259,15 +259,16 @@ template <> struct PairOf<OCRT_IFACE_R_WW> { using type = RwwPair; };
 template <> struct PairOf<OCRT_IFACE_T_WA> { using type = TwaPair; };
 template <> struct PairOf<OCRT_IFACE_T_AW> { using type = TawPair; };
 
-// W[2m][p] = cos(m phi_p), W[2m + 1][p] = sin(m phi_p), phi_p = (p + 1/2) 2 pi / n_phi
-__global__ void interface_weights(int n_phi, int n_rows, double *__restrict__ W)
+// W[p][m] = (cos(m phi_p), sin(m phi_p)), phi_p = (p + 1/2) 2 pi / n_phi:
+// mode-fastest, so the threads of a wavefront (consecutive modes) read one
+// contiguous run per azimuth.
+__global__ void interface_weights(int n_phi, int n_rows, double2 *__restrict__ W)
 {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (long long)n_rows * n_phi) return;
-    const int m = (int)(e / n_phi), p = (int)(e % n_phi);
+    const int p = (int)(e / n_rows), m = (int)(e % n_rows);
     const double phi = (p + 0.5) * (2.0 * OCRT_PI / (double)n_phi);
-    W[(2LL * m) * n_phi + p] = cos(m * phi);
-    W[(2LL * m + 1) * n_phi + p] = sin(m * phi);
+    W[e] = make_double2(cos(m * phi), sin(m * phi));
 }
 
 // One workgroup per node pair (blockIdx.x = o * n_i + i); output
@@ -278,7 +279,7 @@ template <int KIND>
 __global__ __launch_bounds__(kIfThreads) void interface_fourier(const double *__restrict__ mu_o, int n_o,
                                                                 const double *__restrict__ mu_i, int n_i,
                                                                 int m_max, int n_phi, double s2v, double nw,
-                                                                int qc, const double *__restrict__ W,
+                                                                int qc, const double2 *__restrict__ W, int w_rows,
                                                                 double *__restrict__ out)
 {
     __shared__ double tab[9][kIfThreads];
@@ -293,8 +294,9 @@ __global__ __launch_bounds__(kIfThreads) void interface_fourier(const double *__
         const int p = p0 + t;
         double R[9];
         if (p < n_phi) {
-            // cos and sin of the azimuth itself are the m = 1 rows of W
-            pr.eval(W[2LL * n_phi + p], W[3LL * n_phi + p], R);
+            // cos and sin of the azimuth itself are the m = 1 entries of W
+            const double2 cs = W[(long long)p * w_rows + 1];
+            pr.eval(cs.x, cs.y, R);
         } else {
 #pragma unroll
             for (int k = 0; k < 9; ++k) R[k] = 0.0;
@@ -304,9 +306,10 @@ __global__ __launch_bounds__(kIfThreads) void interface_fourier(const double *__
         __syncthreads();
         if (owner) {
             const int len = min(kIfThreads, n_phi - p0);
-            const double *wc = W + (2LL * t) * n_phi + p0, *ws = wc + n_phi;
+            const double2 *wt = W + (long long)p0 * w_rows + t;
             for (int q = 0; q < len; ++q) {
-                const double c = wc[q], s = ws[q];
+                const double2 w = wt[(long long)q * w_rows];
+                const double c = w.x, s = w.y;
                 acc[0] = fma(tab[0][q], c, acc[0]);
                 acc[1] = fma(tab[1][q], c, acc[1]);
                 acc[2] = fma(tab[2][q], s, acc[2]);
@@ -451,10 +454,11 @@ extern "C" int ocrt_interface_fourier(int kind, const double *mu_o, int n_o, con
     const int w_rows = m_max > 1 ? m_max + 1 : 2;   // the m = 1 rows carry cos/sin of the azimuth
     const size_t n_out = (size_t)(m_max + 1) * n_o * n_i * 9;
     DevBufs B;
-    double *d_mo, *d_mi, *d_w, *d_out;
+    double *d_mo, *d_mi, *d_out;
+    double2 *d_w;
     IFCHK(B.get(&d_mo, sizeof(double) * n_o));
     IFCHK(B.get(&d_mi, sizeof(double) * n_i));
-    IFCHK(B.get(&d_w, sizeof(double) * 2 * (size_t)w_rows * n_phi));
+    IFCHK(B.get(&d_w, sizeof(double2) * (size_t)w_rows * n_phi));
     IFCHK(B.get(&d_out, sizeof(double) * n_out));
     IFCHK(hipMemcpyAsync(d_mo, mu_o, sizeof(double) * n_o, hipMemcpyHostToDevice, st));
     IFCHK(hipMemcpyAsync(d_mi, mu_i, sizeof(double) * n_i, hipMemcpyHostToDevice, st));
@@ -465,19 +469,19 @@ extern "C" int ocrt_interface_fourier(int kind, const double *mu_o, int n_o, con
     switch (kind) {
     case OCRT_IFACE_R_AA:
         interface_fourier<OCRT_IFACE_R_AA><<<grid, kIfThreads, 0, st>>>(d_mo, n_o, d_mi, n_i, m_max, n_phi, s2v,
-                                                                         n_water, q_convention, d_w, d_out);
+                                                                         n_water, q_convention, d_w, w_rows, d_out);
         break;
     case OCRT_IFACE_R_WW:
         interface_fourier<OCRT_IFACE_R_WW><<<grid, kIfThreads, 0, st>>>(d_mo, n_o, d_mi, n_i, m_max, n_phi, s2v,
-                                                                         n_water, q_convention, d_w, d_out);
+                                                                         n_water, q_convention, d_w, w_rows, d_out);
         break;
     case OCRT_IFACE_T_WA:
         interface_fourier<OCRT_IFACE_T_WA><<<grid, kIfThreads, 0, st>>>(d_mo, n_o, d_mi, n_i, m_max, n_phi, s2v,
-                                                                         n_water, q_convention, d_w, d_out);
+                                                                         n_water, q_convention, d_w, w_rows, d_out);
         break;
     default:
         interface_fourier<OCRT_IFACE_T_AW><<<grid, kIfThreads, 0, st>>>(d_mo, n_o, d_mi, n_i, m_max, n_phi, s2v,
-                                                                         n_water, q_convention, d_w, d_out);
+                                                                         n_water, q_convention, d_w, w_rows, d_out);
         break;
     }
     IFCHK(hipGetLastError());
