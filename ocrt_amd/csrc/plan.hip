@@ -33,7 +33,7 @@ __global__ void order_one(const DevInst *, const DevItem *, Pools, ItemState *, 
 template <typename FT> __global__ void order_status(const DevInst *, const DevItem *, Pools, ItemState *, int);
 template <int MODE>
 __global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
-template <int NP>
+template <int NP, bool SPLIT>
 __global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <typename FT, typename JT, bool MIXED, int U, int W>
 __global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int, int);
@@ -134,8 +134,11 @@ struct Chunk {
     int i0, i1, it0, it1;       // instance and item ranges
     std::vector<DevInst> hinst;
     std::vector<DevItem> hitem;
-    std::vector<GemmTile> htiles;   // single-part items' tiles [0, ntiles1), then two-part ones
-    int ntiles1 = 0;
+    // GEMM tiles: single-part items with a single bf16 field [0, ntiles1),
+    // single-part items with a field pair [ntiles1, ntiles2), two-part items
+    // [ntiles2, size)
+    std::vector<GemmTile> htiles;
+    int ntiles1 = 0, ntiles2 = 0;
     std::vector<ColTile> hcols;
     std::vector<IpssJob> hjobs;
     std::vector<IpssJob> hjobs_lut;   // IPSS factors of every LUT direction
@@ -600,19 +603,23 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         }
         // the GEMM tiles of one class stay on one XCD, ordered (row tile, item,
         // level tile) so consecutive tiles reuse the class's operator rows in L2;
-        // single-part items first, then two-part ones (one kernel instance each)
-        std::vector<std::vector<GemmTile>> gemm_of_class[2];
+        // single-part unsplit items first, then single-part split ones, then
+        // two-part ones (one kernel instance each)
+        std::vector<std::vector<GemmTile>> gemm_of_class[3];
         for (size_t k = 0; k < class_items.size(); ++k) {
             const DevItem &o = c.hitem[class_items[k][0]];
             std::vector<GemmTile> gt;
             for (int tm = 0; tm < o.tiles_m; ++tm)
                 for (int il : class_items[k])
                     for (int tn = 0; tn < c.hitem[il].tiles_n; ++tn) gt.push_back(GemmTile{il, tm, tn, 0});
-            gemm_of_class[o.K > o.P ? 1 : 0].push_back(std::move(gt));
+            gemm_of_class[o.K > o.P ? 2 : (o.fsplit ? 1 : 0)].push_back(std::move(gt));
         }
         c.htiles = xcd_tile_order(gemm_of_class[0]);
         c.ntiles1 = (int)c.htiles.size();
-        const std::vector<GemmTile> t2 = xcd_tile_order(gemm_of_class[1]);
+        const std::vector<GemmTile> t1 = xcd_tile_order(gemm_of_class[1]);
+        c.htiles.insert(c.htiles.end(), t1.begin(), t1.end());
+        c.ntiles2 = (int)c.htiles.size();
+        const std::vector<GemmTile> t2 = xcd_tile_order(gemm_of_class[2]);
         c.htiles.insert(c.htiles.end(), t2.begin(), t2.end());
         c.n_jobs = (int)c.hjobs.size();
         c.lev = lev; c.T = T; c.R = R; c.basis = basis; c.tab = tab; c.A = A; c.F = F; c.bc = bc;
@@ -883,11 +890,14 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             hipLaunchKernelGGL(order_status<__hip_bfloat16>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
             if (eg) (void)hipEventRecord(eg[0], s);
             if (c.ntiles1 > 0)
-                hipLaunchKernelGGL(gemm_bf16<1>, dim3((unsigned)c.ntiles1), dim3(256), 0, s, c.dinst, c.ditem, P,
-                                   c.dtiles, st, 0);
-            if (ntiles > (unsigned)c.ntiles1)
-                hipLaunchKernelGGL(gemm_bf16<2>, dim3(ntiles - (unsigned)c.ntiles1), dim3(256), 0, s, c.dinst,
-                                   c.ditem, P, c.dtiles + c.ntiles1, st, 0);
+                hipLaunchKernelGGL((gemm_bf16<1, false>), dim3((unsigned)c.ntiles1), dim3(256), 0, s, c.dinst,
+                                   c.ditem, P, c.dtiles, st, 0);
+            if (c.ntiles2 > c.ntiles1)
+                hipLaunchKernelGGL((gemm_bf16<1, true>), dim3((unsigned)(c.ntiles2 - c.ntiles1)), dim3(256), 0, s,
+                                   c.dinst, c.ditem, P, c.dtiles + c.ntiles1, st, 0);
+            if (ntiles > (unsigned)c.ntiles2)
+                hipLaunchKernelGGL((gemm_bf16<2, true>), dim3(ntiles - (unsigned)c.ntiles2), dim3(256), 0, s,
+                                   c.dinst, c.ditem, P, c.dtiles + c.ntiles2, st, 0);
             if (eg) (void)hipEventRecord(eg[1], s);
             if (es) (void)hipEventRecord(es[0], s);
             launch_sweep_mixed(p->sweep_variant, ncols, s, c.dinst, c.ditem, P, st, c.dcols);

@@ -299,25 +299,27 @@ __device__ inline bf16x8_ lds_frag(const unsigned short *p)
 
 // LDS staging of one K slab of a tile: [part][hi, lo][row][k] of the operator
 // and [hi, lo][level][k] of the field, rows padded to 40 entries.
-struct GemmLds {
-    unsigned short A[2][2][64][40];
-    unsigned short B[2][kGemmLevels][40];
+template <int NP, bool SPLIT> struct GemmLds {
+    unsigned short A[NP][2][64][40];
+    unsigned short B[SPLIT ? 2 : 1][kGemmLevels][40];
 };
 
 // One tile of the bf16 scattering integral: operator rows row0 .. row0 + 63
 // and levels lev0 .. lev0 + kGemmLevels - 1 of one item with NP operator parts
 // (the item's K is NP P), four waves of 32 x 64.  NP is a template parameter
-// so that single-part items hold half the accumulators (higher occupancy).
+// so that single-part items hold half the accumulators (higher occupancy);
+// SPLIT = false compiles out the bf16 pair field (its registers and LDS plane)
+// for tiles whose items keep a single bf16 field.
 // Rows past the operator's Mp rows load its last row (their results are never
 // used).  The fp32 results, scaled by the level fractions, go to
 // out(lev, r, v) for the four rows r .. r + 3 of v, at levels < L only.
-template <int NP, typename Out>
+template <int NP, bool SPLIT, typename Out>
 __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, const unsigned short *A,
                                       const unsigned short *F, const unsigned short *Flo, int row0, int lev0,
-                                      GemmLds &S, Out out)
+                                      GemmLds<NP, SPLIT> &S, Out out)
 {
     const int Pp = it.P, L = it.L, Mp = it.Mp;
-    const bool split = it.fsplit != 0;
+    const bool split = SPLIT && it.fsplit != 0;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
     float4_ acc[NP][2][4];
@@ -362,10 +364,11 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
         }
         *reinterpret_cast<uint4 *>(&S.B[0][lr][lk]) = rb0;
         *reinterpret_cast<uint4 *>(&S.B[0][lr + 64][lk]) = rb1;
-        if (split) {
-            *reinterpret_cast<uint4 *>(&S.B[1][lr][lk]) = rl0;
-            *reinterpret_cast<uint4 *>(&S.B[1][lr + 64][lk]) = rl1;
-        }
+        if constexpr (SPLIT)
+            if (split) {
+                *reinterpret_cast<uint4 *>(&S.B[SPLIT ? 1 : 0][lr][lk]) = rl0;
+                *reinterpret_cast<uint4 *>(&S.B[SPLIT ? 1 : 0][lr + 64][lk]) = rl1;
+            }
     };
     // fragment of a 16 x 32 block: lane l holds row (l & 15), k 8 (l >> 4) .. + 7
     const int fr = lane & 15, fk = 8 * (lane >> 4);
@@ -378,10 +381,11 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const bf16x8_ fb = lds_frag(&S.B[0][wn + b * 16 + fr][fk]);
-                if (split) {
-                    const bf16x8_ fl = lds_frag(&S.B[1][wn + b * 16 + fr][fk]);
-                    acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, fl, acc[q][a][b], 0, 0, 0);
-                }
+                if constexpr (SPLIT)
+                    if (split) {
+                        const bf16x8_ fl = lds_frag(&S.B[SPLIT ? 1 : 0][wn + b * 16 + fr][fk]);
+                        acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, fl, acc[q][a][b], 0, 0, 0);
+                    }
                 acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, fb, acc[q][a][b], 0, 0, 0);
                 acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, fb, acc[q][a][b], 0, 0, 0);
             }
@@ -426,10 +430,12 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     }
 }
 
-// NP: operator parts of every item in the tile list (the plan keeps one list
-// per part count).
-template <int NP>
-__global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ inst,
+// NP: operator parts of every item in the tile list; SPLIT: whether its items
+// may keep their field as a bf16 pair (the plan keeps one list per kind).
+// Single-part unsplit tiles fit 96 registers and 20 KB of LDS: five waves per
+// SIMD.
+template <int NP, bool SPLIT>
+__global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(const DevInst *__restrict__ inst,
                                                  const DevItem *__restrict__ items, Pools P,
                                                  const GemmTile *__restrict__ tiles,
                                                  const ItemState *__restrict__ st, int order)
@@ -444,8 +450,8 @@ __global__ __launch_bounds__(256) void gemm_bf16(const DevInst *__restrict__ ins
     const unsigned short *Flo = reinterpret_cast<const unsigned short *>(P.Flo) + it.off_Flo;
     float *J = reinterpret_cast<float *>(P.J) + it.off_F;
     const LevelView lv(P, I);
-    __shared__ GemmLds S;
-    gemm_tile_bf16<NP>(it, lv, A, F, Flo, tl.tm * 64, tl.tn * kGemmLevels, S, [&](int lev, int r, float4 v) {
+    __shared__ GemmLds<NP, SPLIT> S;
+    gemm_tile_bf16<NP, SPLIT>(it, lv, A, F, Flo, tl.tm * 64, tl.tn * kGemmLevels, S, [&](int lev, int r, float4 v) {
         if (r < Pp) *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v;
     });
 }
@@ -817,10 +823,12 @@ OCRT_SWEEP_INST(__hip_bfloat16, float, true, 8, 4)
 OCRT_SWEEP_INST(__hip_bfloat16, float, true, 16, 2)
 OCRT_SWEEP_INST(__hip_bfloat16, float, true, 16, 4)
 #undef OCRT_SWEEP_INST
-template __global__ void gemm_bf16<1>(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *,
-                                      int);
-template __global__ void gemm_bf16<2>(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *,
-                                      int);
+template __global__ void gemm_bf16<1, false>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                             const ItemState *, int);
+template __global__ void gemm_bf16<1, true>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                            const ItemState *, int);
+template __global__ void gemm_bf16<2, true>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                            const ItemState *, int);
 template __global__ void gemm_f64<kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                             const ItemState *, int);
 template __global__ void gemm_f64<kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
