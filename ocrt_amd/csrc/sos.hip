@@ -140,14 +140,22 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     // once, rt_surface_boundary.c:210-428) share it; over the rough ocean the
     // direct beam reflected into the upward nodes is added level by level
     // (rt_surface_boundary.c:430-493)
+    // The seed's path attenuation exp(-(tau - h[k]) / mu) from the surface up
+    // to level k is carried as the running product of the layers' Beer-Lambert
+    // factors, which the upward scan visits in order (k = nt .. 0).
     const bool rough_seed = I.surf == 2 && hemi == 0;
     const double tau = lv.h[nt], down = exp(-tau / I.mus);
     const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
     const double rsun = rough_seed ? P.R[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c] : 0.0;
+    const double seed0 = rsun * (FS * down);
+    double att = 1.0;
     sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o,
                      [&](int k) { return src.primary(lv, k) + src.flat_value(lv, I.mus, k); },
                      [&](int k, double v) {
-                         if (rough_seed) v += rsun * (FS * (down * exp(-(tau - lv.h[k]) / mu)));
+                         if (rough_seed) {
+                             if (k < nt) att *= T[(long long)k * n + o];
+                             v += seed0 * att;
+                         }
                          const long long ix = (long long)k * Pp + col;
                          tot[ix] = v;
                          if constexpr (MIXED)
