@@ -164,3 +164,31 @@ def test_mixed_vs_fp64_full_size_batch(ocrt):
     assert ei.max() <= MIXED_TOL_I and eq.max() <= MIXED_TOL_QU
     # the order count may differ by one where the residual sits at the tolerance
     assert np.abs(a["n_orders_used"] - b["n_orders_used"]).max() <= 1
+
+
+def test_extreme_sizes_in_one_ragged_batch(ocrt, oracle):
+    """The largest Gauss rule the reference allows (256 streams plus the view
+    node, rough ocean, aerosol), the smallest instance (2 streams, 2 layers,
+    m = 0, black) and a thin-stream deep one (3 streams, 400 layers) in one
+    batch: FP64 against the oracle instance by instance, the mixed form
+    within its budget of the FP64 form."""
+    from ocrt_amd import synthetic as S
+    from ocrt_amd.abi import make_instance
+    gk = S.hg_greek(0.7, 20)
+    descs = [make_instance(surface="black_fresnel_ocean", wind=7.0, tau_a=0.2, ssa_a=0.9, greek=gk, n_mu=256,
+                           n_layers=12, m_max=3, sza=35.0, vza=20.0, raa=70.0),
+             make_instance(surface="black", n_mu=2, n_layers=2, m_max=0, sza=10.0, vza=5.0, raa=0.0),
+             make_instance(surface="flat", tau_a=0.05, ssa_a=0.99, greek=gk, n_mu=3, n_layers=400, m_max=5,
+                           sza=60.0, vza=65.0, raa=150.0)]
+    print("[gpu] ragged batch: 256/2/3 streams, 12/2/400 layers", flush=True)
+    a = ocrt.solve(descs, precision="fp64", modes=True)
+    b = ocrt.solve(descs, precision="mixed")
+    for i, d in enumerate(descs):
+        rc, r, modes, orders = oracle_solve(oracle, d)
+        assert rc == 0 and a["status"][i] == 0 and b["status"][i] == 0
+        for f in ("rho_I", "rho_Q", "rho_U"):
+            assert abs(a[f][i] - getattr(r, f)) <= FP64_TOL * abs(r.rho_I), (i, f)
+        assert np.max(np.abs(a["toa_modes"][i, : d.m_max + 1] - modes)) <= FP64_TOL * np.max(np.abs(modes))
+        assert list(a["orders_per_m"][i, : d.m_max + 1]) == list(orders)
+        assert abs(b["rho_I"][i] - a["rho_I"][i]) <= MIXED_TOL_I * abs(a["rho_I"][i])
+        assert max(abs(b["rho_Q"][i] - a["rho_Q"][i]), abs(b["rho_U"][i] - a["rho_U"][i])) <= MIXED_TOL_QU
