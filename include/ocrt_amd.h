@@ -251,6 +251,19 @@ int ocrt_plan_lut_results(ocrt_plan *plan, double *grid, double *scalars);
  *               instance's m_max; m_max_all = ocrt_plan_mmax(plan). */
 int ocrt_plan_lut_extras(ocrt_plan *plan, double *raw, double *view_modes);
 
+/* Per-node Fourier fields, the boa_I/Q/U and per-m TOA outputs of the Python
+ * batch solver (code/OCRT_Python/ocrt_py/atmos_batch.py:990-1013).
+ * ocrt_plan_export_nodes enables them (call before the execute whose fields
+ * are wanted; LUT plans have them already).  ocrt_plan_node_fields copies the
+ * last execute's fields (synchronises), either pointer may be NULL:
+ *   toa_up  [n_inst][m_max_all + 1][3][ring_pitch]: TOA upward I, Q, U at
+ *           every ring node (ascending mu; rough direct-beam seed removed)
+ *   boa_dn  [n_inst][m_max_all + 1][3][ring_pitch]: BOA downward I, Q, U at
+ *           every ring node (ascending |mu|)
+ * zero past an instance's n_ring or m_max; ring_pitch >= every n_ring. */
+int ocrt_plan_export_nodes(ocrt_plan *plan);
+int ocrt_plan_node_fields(ocrt_plan *plan, int ring_pitch, double *toa_up, double *boa_dn);
+
 /* Final relative residual of every (instance, mode) of the last execute,
  * [n_inst][m_max_all + 1] (sos_residual_per_m of rt_result_t, rt_types.h:511). */
 int ocrt_plan_residuals(ocrt_plan *plan, double *residuals);

@@ -74,6 +74,8 @@ def library():
         lib.ocrt_plan_tuning.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_plan_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.ocrt_plan_residuals.argtypes = [C.c_void_p, C.c_void_p]
+        lib.ocrt_plan_export_nodes.argtypes = [C.c_void_p]
+        lib.ocrt_plan_node_fields.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         lib.ocrt_interface_fourier.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                                C.c_double, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
         lib.ocrt_interface_direct.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double,
@@ -241,6 +243,24 @@ class Plan:
             out["grad_y"].append(lev[off + 2 * L:off + 3 * L].copy())
             off += 3 * L
         return out
+
+    def export_nodes(self):
+        """Keep the per-node TOA and BOA Fourier fields of the following
+        executes (ocrt_plan_export_nodes)."""
+        rc = self.lib.ocrt_plan_export_nodes(self._handle)
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_export_nodes failed with code {rc}")
+
+    def node_fields(self, ring_pitch):
+        """(toa_up, boa_dn) of the last execute, each [n_inst, m_max_all + 1, 3,
+        ring_pitch] (ocrt_plan_node_fields)."""
+        shape = (self.n, self.mmax + 1, 3, ring_pitch)
+        up = np.zeros(shape)
+        dn = np.zeros(shape)
+        rc = self.lib.ocrt_plan_node_fields(self._handle, ring_pitch, up.ctypes.data, dn.ctypes.data)
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_node_fields failed with code {rc}")
+        return up, dn
 
     def execute(self, stream=None):
         ptr = None

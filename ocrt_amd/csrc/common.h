@@ -75,7 +75,7 @@ struct DevItem {
     long long off_bc;     // pool_bc: [3][n] boundary condition, then [3][n] bottom downward field
     long long off_cmp;    // pool_comp: [2][P] compensation of the order sum at k = 0 and k = nt
     long long off_Flo;    // pool_Flo: [L][P] low half of the bf16 field pair (fsplit items)
-    long long off_nodes;  // pool_nodes (LUT plans, global over the plan): [4][n]
+    long long off_nodes;  // pool_nodes (LUT or node-export plans, global over the plan): [6][n]
 };
 
 struct ItemState {
@@ -111,7 +111,8 @@ struct Pools {
     void *J2;             // second output of a parts-split scattering integral (adjoint)
     void *Flo;            // bf16 low half of the field pair (mixed form, fsplit items)
     double *cm_w;         // cos/sin(m phi) table [(Mall+1)][2][kNPhi]
-    double *nodes;        // LUT plans: per item [4][n] TOA upward I, Q, U and BOA downward I at every node
+    double *nodes;        // LUT / node-export plans: per item [6][n] TOA upward I, Q, U and BOA downward
+                          // I, Q, U at every node
     const DevGas *gas_tab;    // gas absorption tables of the plan
     const double *gas_pool;   // their arrays
 };

@@ -584,7 +584,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 it.off_bc = bc; bc += 6LL * n;
                 it.off_cmp = cmp; cmp += mixed ? 2LL * Pp : 0;
                 it.off_nodes = p->nodes_n;
-                p->nodes_n += 4LL * n;
+                p->nodes_n += 6LL * n;
                 it.fsplit = (mixed && m < kSplitModes) ? 1 : 0;
                 it.off_Flo = flo; flo += it.fsplit ? (long long)L * Pp : 0;
                 c.max_pack = std::max(c.max_pack, (long long)Mp * it.K);
@@ -924,7 +924,7 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     else
         hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, c.maxit + 1);
     Pools Pe = P;
-    Pe.nodes = p->lut ? p->nodes : nullptr;
+    Pe.nodes = p->nodes;   // allocated only for LUT and node-export plans
     hipLaunchKernelGGL(extract, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pe, st, io, mixed ? 1 : 0);
     if (c.n_jobs > 0)
         hipLaunchKernelGGL(ipss_kappa, dim3(c.n_jobs), dim3(256), 0, s, c.dinst, c.djobs, P,
@@ -1038,7 +1038,7 @@ extern "C" int ocrt_plan_set_lut(ocrt_plan *p, int n_vza, const double *vza_deg,
         if (!std::isfinite(raa_deg[i])) return OCRT_ERR_ARGS;
     const long long grid = (long long)n_vza * n_raa;
     int rc = 0;
-    rc |= dmalloc(p, (void **)&p->nodes, p->nodes_n * 8);
+    if (!p->nodes) rc |= dmalloc(p, (void **)&p->nodes, p->nodes_n * 8);
     rc |= dmalloc(p, (void **)&p->lut_vza, (size_t)n_vza * 8);
     rc |= dmalloc(p, (void **)&p->lut_raa, (size_t)n_raa * 8);
     rc |= dmalloc(p, (void **)&p->lut_grid, (size_t)p->n_inst * grid * 6 * 8);
@@ -1093,6 +1093,40 @@ extern "C" int ocrt_plan_lut_extras(ocrt_plan *p, double *raw, double *view_mode
     const size_t v = (size_t)p->n_inst * p->lut_nv * (p->mmax_all + 1) * 3;
     if (raw) HIPCHK(hipMemcpy(raw, p->lut_raw, g * 8, hipMemcpyDeviceToHost));
     if (view_modes) HIPCHK(hipMemcpy(view_modes, p->lut_vpm, v * 8, hipMemcpyDeviceToHost));
+    return OCRT_OK;
+}
+
+extern "C" int ocrt_plan_export_nodes(ocrt_plan *p)
+{
+    if (!p) return OCRT_ERR_ARGS;
+    if (p->nodes) return OCRT_OK;
+    drop_graph(p);   // extract's arguments change
+    return dmalloc(p, (void **)&p->nodes, p->nodes_n * 8) ? OCRT_ERR_ALLOC : OCRT_OK;
+}
+
+extern "C" int ocrt_plan_node_fields(ocrt_plan *p, int ring_pitch, double *toa_up, double *boa_dn)
+{
+    if (!p || !p->nodes || ring_pitch < 1) return OCRT_ERR_ARGS;
+    for (const Chunk &c : p->chunks)
+        for (const DevItem &it : c.hitem)
+            if (it.n > ring_pitch) return OCRT_ERR_ARGS;
+    HIPCHK(hipDeviceSynchronize());
+    std::vector<double> h((size_t)p->nodes_n);
+    HIPCHK(hipMemcpy(h.data(), p->nodes, (size_t)p->nodes_n * 8, hipMemcpyDeviceToHost));
+    const int M = p->mmax_all + 1;
+    const size_t total = (size_t)p->n_inst * M * 3 * ring_pitch;
+    if (toa_up) std::fill(toa_up, toa_up + total, 0.0);
+    if (boa_dn) std::fill(boa_dn, boa_dn + total, 0.0);
+    for (const Chunk &c : p->chunks)
+        for (const DevItem &it : c.hitem) {
+            const size_t gi = (size_t)(c.i0 + it.inst), n = (size_t)it.n;
+            for (size_t s = 0; s < 3; ++s)
+                for (size_t o = 0; o < n; ++o) {
+                    const size_t at = ((gi * M + (size_t)it.m) * 3 + s) * (size_t)ring_pitch + o;
+                    if (toa_up) toa_up[at] = h[(size_t)it.off_nodes + s * n + o];
+                    if (boa_dn) boa_dn[at] = h[(size_t)it.off_nodes + (3 + s) * n + o];
+                }
+        }
     return OCRT_OK;
 }
 

@@ -719,9 +719,10 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
     }
     if (P.nodes) {
         // every node of the TOA upward field (rough seed removed) and of the
-        // BOA downward intensity, for the LUT's view sampling
+        // BOA downward field, [6][n]: up I Q U, down I Q U (the LUT's view
+        // sampling reads the first four rows)
         double *dst = P.nodes + it.off_nodes;
-        for (int q = threadIdx.x; q < 4 * n; q += blockDim.x) {
+        for (int q = threadIdx.x; q < 6 * n; q += blockDim.x) {
             const int c = q / n, o = q % n;
             double val;
             if (c < 3) {
@@ -733,7 +734,7 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
                            (FS * (down * exp(-(tau - lv.h[0]) / mu[o])));
                 }
             } else {
-                val = total_at(P, it, mixed, nt, col_dn(n, 0, o));
+                val = total_at(P, it, mixed, nt, col_dn(n, c - 3, o));
             }
             dst[q] = val;
         }
