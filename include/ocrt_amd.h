@@ -310,6 +310,26 @@ int ocrt_interface_fourier(int kind, const double *mu_o, int n_o, const double *
 int ocrt_interface_direct(int kind, const double *mu, int n, double wind, int sigma_type, double n_water,
                           double *out, void *hip_stream);
 
+/* Atmosphere -> water coupling of the coupled ocean
+ * (code/OCRT_C/src/rt_air_water_coupling.c:244-478): the atmosphere's BOA
+ * downward Fourier field I/Q/U_a [(m_max+1) * n_a] at the air nodes mu_a
+ * (host arrays) mapped onto the in-water inbound field I/Q/U_w
+ * [(m_max+1) * n_w] at the water nodes mu_w.
+ *   w_a != NULL and a rough surface (wind > 0, slope variance > 1e-10): the
+ *     Cox-Munk air -> water Fourier transmission (ocrt_interface_fourier
+ *     kind T_AW, 128 azimuths) contracted with 2 pi (m = 0) or pi times
+ *     mu_j w_j;
+ *   otherwise: Snell refraction, the flat radiance transmission Mueller and
+ *     linear interpolation in the air nodes (zero inside the total-reflection
+ *     cone); with wind > 0 the m = 0 intensity is replaced by the BTDF
+ *     transmission of the m = 0 skylight (192 air nodes, 96 azimuths).
+ * n_water > 1, n_a >= 2, m_max <= 255.  Blocks until the outputs are in
+ * host memory. */
+int ocrt_couple_atm_to_water(int m_max, int n_a, const double *mu_a, const double *w_a, const double *I_a,
+                             const double *Q_a, const double *U_a, double n_water, int q_convention,
+                             const double *mu_w, int n_w, double wind, int sigma_type, double *I_w,
+                             double *Q_w, double *U_w, void *hip_stream);
+
 /* One-call form: plan, execute, copy results, destroy. */
 int ocrt_solve_batch(const ocrt_instance_t *inst, int n_inst, int precision,
                      ocrt_result_t *out, void *hip_stream);

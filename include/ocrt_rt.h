@@ -342,6 +342,39 @@ void surface_fkc_set_build_mmax(int m_max);
 double surface_T_aw_coxmunk_direct(double mu_sun_air, double n_water, double wind_speed, int sigma_type);
 double surface_R_ww_coxmunk_direct(double mu_up_water, double n_water, double wind_speed, int sigma_type);
 
+/* Atmosphere BOA export and in-water inbound field of the coupled ocean,
+ * field for field as rt_atm_boa_export_t (code/OCRT_C/src/rt_types.h:874-885)
+ * and rt_aw_coupled_field_t (rt_air_water_coupling.h:26-33); fields are
+ * [(m_max+1) * n] with index m * n + j. */
+typedef struct {
+    int     m_max;
+    int     n_mu;
+    double *mu_quad_pos;        /* [n_mu] positive air nodes */
+    double *I_per_m;            /* BOA downward diffuse I^m */
+    double *Q_per_m;
+    double *U_per_m;
+    double  tau_atm_total;
+    double  mu_sun_air;
+    int     allocated;
+} rt_atm_boa_export_t;
+
+typedef struct {
+    int     m_max;
+    int     n_mu_water;
+    double *mu_water_pos;       /* [n_mu_water] */
+    double *I_inwater_per_m;    /* in-water inbound I^m */
+    double *Q_inwater_per_m;
+    double *U_inwater_per_m;
+} rt_aw_coupled_field_t;
+
+/* The reference's coupling entry point and helpers (rt_air_water_coupling.h:35-68),
+ * served by ocrt_couple_atm_to_water; -1 bad arguments, -3 GPU failure. */
+int  rt_aw_coupled_field_alloc(rt_aw_coupled_field_t *f, int m_max, int n_mu_water);
+void rt_aw_coupled_field_free(rt_aw_coupled_field_t *f);
+int  rt_air_water_couple_atm_to_water(const rt_atm_boa_export_t *boa_export, double n_water, int q_convention,
+                                      const double *mu_water_pos, int n_mu_water, const double *w_atm_pos,
+                                      double wind_speed, int sigma_type, rt_aw_coupled_field_t *coupled);
+
 #ifdef __cplusplus
 }
 #endif

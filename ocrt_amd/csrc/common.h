@@ -188,4 +188,14 @@ struct IpssJob { int inst; int pad; double vza, raa; };
 
 __host__ __device__ inline int roundup(int x, int a) { return (x + a - 1) / a * a; }
 
+// Positive half of the 2n-point Gauss-Legendre rule (host; plan.hip).
+int gauss_legendre_pos(int n, double *mu, double *w);
+
+// Fourier kernels of a rough-interface matrix for modes 0..m_max into device
+// memory d_out[m][o][i][9] from device node arrays, on stream st
+// (interface.hip; the arguments are checked by the caller).
+int interface_fourier_device(int kind, const double *d_mo, int n_o, const double *d_mi, int n_i, int m_max,
+                             int n_phi, double wind, int sigma_type, double n_water, int q_convention,
+                             double *d_out, hipStream_t st);
+
 }  // namespace ocrt

@@ -439,29 +439,17 @@ struct DevBufs {
 
 using namespace ocrt;
 
-extern "C" int ocrt_interface_fourier(int kind, const double *mu_o, int n_o, const double *mu_i, int n_i, int m_max,
-                                      int n_phi, double wind, int sigma_type, double n_water, int q_convention,
-                                      double *out, void *hip_stream)
+namespace ocrt {
+
+int interface_fourier_device(int kind, const double *d_mo, int n_o, const double *d_mi, int n_i, int m_max,
+                             int n_phi, double wind, int sigma_type, double n_water, int q_convention,
+                             double *d_out, hipStream_t st)
 {
-    if (kind < OCRT_IFACE_R_AA || kind > OCRT_IFACE_T_AW || !mu_o || !mu_i || !out) return OCRT_ERR_ARGS;
-    if (n_o < 1 || n_i < 1 || (long long)n_o * n_i > kIfMaxPairs) return OCRT_ERR_ARGS;
-    if (m_max < 0 || m_max + 1 > kIfMaxModes || n_phi < 4 || n_phi > OCRT_IFACE_MAX_PHI) return OCRT_ERR_ARGS;
-    if (!std::isfinite(wind) || !(n_water > 0.0) || !std::isfinite(n_water)) return OCRT_ERR_ARGS;
-    if (!cosines_ok(mu_o, n_o) || !cosines_ok(mu_i, n_i)) return OCRT_ERR_ARGS;
     const double s2v = host_slope_variance(wind, sigma_type);
-    if (kind == OCRT_IFACE_T_AW && s2v <= 1e-10) return OCRT_ERR_FLAT;
-    hipStream_t st = (hipStream_t)hip_stream;
-    const int w_rows = m_max > 1 ? m_max + 1 : 2;   // the m = 1 rows carry cos/sin of the azimuth
-    const size_t n_out = (size_t)(m_max + 1) * n_o * n_i * 9;
+    const int w_rows = m_max > 1 ? m_max + 1 : 2;   // the m = 1 entries carry cos/sin of the azimuth
     DevBufs B;
-    double *d_mo, *d_mi, *d_out;
     double2 *d_w;
-    IFCHK(B.get(&d_mo, sizeof(double) * n_o));
-    IFCHK(B.get(&d_mi, sizeof(double) * n_i));
     IFCHK(B.get(&d_w, sizeof(double2) * (size_t)w_rows * n_phi));
-    IFCHK(B.get(&d_out, sizeof(double) * n_out));
-    IFCHK(hipMemcpyAsync(d_mo, mu_o, sizeof(double) * n_o, hipMemcpyHostToDevice, st));
-    IFCHK(hipMemcpyAsync(d_mi, mu_i, sizeof(double) * n_i, hipMemcpyHostToDevice, st));
     const long long n_w = (long long)w_rows * n_phi;
     interface_weights<<<(unsigned)((n_w + 255) / 256), 256, 0, st>>>(n_phi, w_rows, d_w);
     IFCHK(hipGetLastError());
@@ -485,6 +473,35 @@ extern "C" int ocrt_interface_fourier(int kind, const double *mu_o, int n_o, con
         break;
     }
     IFCHK(hipGetLastError());
+    // the weight table is released when this returns: finish with it first
+    IFCHK(hipStreamSynchronize(st));
+    return OCRT_OK;
+}
+
+}  // namespace ocrt
+
+extern "C" int ocrt_interface_fourier(int kind, const double *mu_o, int n_o, const double *mu_i, int n_i, int m_max,
+                                      int n_phi, double wind, int sigma_type, double n_water, int q_convention,
+                                      double *out, void *hip_stream)
+{
+    if (kind < OCRT_IFACE_R_AA || kind > OCRT_IFACE_T_AW || !mu_o || !mu_i || !out) return OCRT_ERR_ARGS;
+    if (n_o < 1 || n_i < 1 || (long long)n_o * n_i > kIfMaxPairs) return OCRT_ERR_ARGS;
+    if (m_max < 0 || m_max + 1 > kIfMaxModes || n_phi < 4 || n_phi > OCRT_IFACE_MAX_PHI) return OCRT_ERR_ARGS;
+    if (!std::isfinite(wind) || !(n_water > 0.0) || !std::isfinite(n_water)) return OCRT_ERR_ARGS;
+    if (!cosines_ok(mu_o, n_o) || !cosines_ok(mu_i, n_i)) return OCRT_ERR_ARGS;
+    if (kind == OCRT_IFACE_T_AW && host_slope_variance(wind, sigma_type) <= 1e-10) return OCRT_ERR_FLAT;
+    hipStream_t st = (hipStream_t)hip_stream;
+    const size_t n_out = (size_t)(m_max + 1) * n_o * n_i * 9;
+    DevBufs B;
+    double *d_mo, *d_mi, *d_out;
+    IFCHK(B.get(&d_mo, sizeof(double) * n_o));
+    IFCHK(B.get(&d_mi, sizeof(double) * n_i));
+    IFCHK(B.get(&d_out, sizeof(double) * n_out));
+    IFCHK(hipMemcpyAsync(d_mo, mu_o, sizeof(double) * n_o, hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_mi, mu_i, sizeof(double) * n_i, hipMemcpyHostToDevice, st));
+    const int rc = interface_fourier_device(kind, d_mo, n_o, d_mi, n_i, m_max, n_phi, wind, sigma_type, n_water,
+                                            q_convention, d_out, st);
+    if (rc != OCRT_OK) return rc;
     IFCHK(hipMemcpyAsync(out, d_out, sizeof(double) * n_out, hipMemcpyDeviceToHost, st));
     IFCHK(hipStreamSynchronize(st));
     return OCRT_OK;
@@ -617,4 +634,261 @@ extern "C" double surface_R_ww_coxmunk_direct(double mu_up_water, double n_water
     if (ocrt_interface_direct(OCRT_IFACE_R_WW, &mu_up_water, 1, wind_speed, sigma_type, n_water, &v, nullptr) != 0)
         return NAN;
     return v;
+}
+
+// ---------------------------------------------------------------------------
+// Atmosphere -> water coupling of the coupled ocean: the atmosphere's BOA
+// downward Fourier field mapped onto the in-water nodes
+// (code/OCRT_C/src/rt_air_water_coupling.c:244-478).
+//   rough surface with air weights: S_w^m(i) = C_m sum_j mu_j w_j T_aw^m[i][j] S_a^m(j),
+//     C_0 = 2 pi, C_m = pi, T_aw from 128 azimuths (:268-315);
+//   otherwise the flat map: Snell refraction to mu_air, the flat radiance
+//     transmission Mueller and linear interpolation of S_a in mu_air (zero in
+//     the total-reflection cone, :317-396); with wind > 0 the m = 0 intensity
+//     is then replaced by the Cox-Munk BTDF transmission of the m = 0
+//     skylight on a 192-node air rule with 96 azimuths (:398-476).
+namespace ocrt {
+namespace {
+
+constexpr int kTawPhi = 128;      // azimuths of the rough transmission kernel
+constexpr int kFineAir = 192;     // air nodes of the m = 0 BTDF integral
+constexpr int kFineAzimuth = 96;  // its azimuths, dphi = 2 pi p / 96
+
+// Linear interpolation on an ascending, duplicate-free grid with linear
+// extrapolation from the end pairs (rt_air_water_coupling.c:102-114).
+__device__ inline double interp_ascending(const double *ms, const double *v, const int *src, int w, double x)
+{
+    if (w < 2) return (w == 1) ? v[src[0]] : 0.0;
+    int hi = 1;
+    while (hi < w && ms[hi] < x) ++hi;
+    int lo = hi - 1;
+    if (hi >= w) { hi = w - 1; lo = w - 2; }
+    const double t = (x - ms[lo]) / (ms[hi] - ms[lo]);
+    return (1.0 - t) * v[src[lo]] + t * v[src[hi]];
+}
+
+// Rough path: one thread per (mode, water node).  field [3][m_max + 1][n_a].
+__global__ void couple_rough(const double *__restrict__ T, const double *__restrict__ field,
+                             const double *__restrict__ mu_a, const double *__restrict__ w_a, int n_w, int n_a,
+                             int m_max, double *__restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (m_max + 1) * n_w) return;
+    const int m = e / n_w, iw = e % n_w;
+    const double az = (m == 0) ? 2.0 * OCRT_PI : OCRT_PI;
+    const long long M = m_max + 1;
+    const double *Ia = field + (long long)m * n_a, *Qa = Ia + M * n_a, *Ua = Qa + M * n_a;
+    double sI = 0.0, sQ = 0.0, sU = 0.0;
+    for (int j = 0; j < n_a; ++j) {
+        const double wj = az * mu_a[j] * w_a[j];
+        if (wj == 0.0) continue;
+        const double *t = T + (((long long)m * n_w + iw) * n_a + j) * 9;
+        sI += wj * (t[0] * Ia[j] + t[1] * Qa[j] + t[2] * Ua[j]);
+        sQ += wj * (t[3] * Ia[j] + t[4] * Qa[j] + t[5] * Ua[j]);
+        sU += wj * (t[6] * Ia[j] + t[7] * Qa[j] + t[8] * Ua[j]);
+    }
+    out[(0 * M + m) * n_w + iw] = sI;
+    out[(1 * M + m) * n_w + iw] = sQ;
+    out[(2 * M + m) * n_w + iw] = sU;
+}
+
+// Flat path: one thread per (mode, water node).  ms/src: the air nodes
+// sorted ascending with duplicates (within 1e-12) dropped, and the source
+// index of each.
+__global__ void couple_flat(const double *__restrict__ field, const double *__restrict__ ms,
+                            const int *__restrict__ src, int w, const double *__restrict__ mu_w, int n_w, int n_a,
+                            int m_max, double nw, int qc, double *__restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (m_max + 1) * n_w) return;
+    const int m = e / n_w, iw = e % n_w;
+    const long long M = m_max + 1;
+    double r[3] = {0.0, 0.0, 0.0};
+    const double mw = mu_w[iw];
+    const double s2a = nw * nw * (1.0 - mw * mw);
+    if (mw > 0.0 && mw <= 1.0 && s2a < 1.0) {
+        const double ma = sqrt(1.0 - s2a);
+        // flat radiance transmission air (1) -> water (n) at incidence ma
+        // (surface.c:431-460): t_s = 1 + r_s, t_p = (1 / n)(1 + r_p), n^2 law
+        const double st2 = (1.0 / nw) * (1.0 / nw) * fmax(0.0, 1.0 - ma * ma);
+        const double mt = sqrt(1.0 - st2);
+        const double rs = (ma - nw * mt) / (ma + nw * mt), rp = (nw * ma - mt) / (nw * ma + mt);
+        const double ts = 1.0 + rs, tp = (1.0 / nw) * (1.0 + rp);
+        const double rad = (nw * nw) * ((nw * mt) / ma);
+        const double Ts = rad * ts * ts, Tp = rad * tp * tp;
+        const double A = 0.5 * (Ts + Tp), q = (qc == 1) ? 0.5 * (Tp - Ts) : 0.5 * (Ts - Tp), d = rad * ts * tp;
+        const double *Ia = field + (long long)m * n_a, *Qa = Ia + M * n_a, *Ua = Qa + M * n_a;
+        const double I = interp_ascending(ms, Ia, src, w, ma), Q = interp_ascending(ms, Qa, src, w, ma),
+                     U = interp_ascending(ms, Ua, src, w, ma);
+        r[0] = A * I + q * Q;
+        r[1] = q * I + A * Q;
+        r[2] = d * U;
+    }
+    for (int c = 0; c < 3; ++c) out[(c * M + m) * n_w + iw] = r[c];
+}
+
+// The m = 0 intensity under a rough surface: one workgroup per water node,
+// one fine air node per thread; each thread sums its 96 azimuths in order.
+__global__ __launch_bounds__(256) void couple_btdf_m0(const double *__restrict__ field, const double *__restrict__ ms,
+                                                      const int *__restrict__ src, int w,
+                                                      const double *__restrict__ mu_w, const double *__restrict__ fa_mu,
+                                                      const double *__restrict__ fa_w, double s2v, double nw,
+                                                      double *__restrict__ out)
+{
+    __shared__ double red[4];
+    const int iw = blockIdx.x, t = threadIdx.x;
+    const double mw = mu_w[iw];
+    if (!(mw > 1e-9 && mw <= 1.0)) return;   // keeps the flat value
+    double term = 0.0;
+    if (t < kFineAir) {
+        const double mua = fa_mu[t];
+        const double L0 = interp_ascending(ms, field, src, w, mua);
+        const TawPair pr(mw, mua, s2v, nw, 1);
+        const double dph = 2.0 * OCRT_PI / (double)kFineAzimuth;
+        double K0 = 0.0;
+        for (int p = 0; p < kFineAzimuth; ++p) {
+            const double phi = dph * (double)p;
+            double R[9];
+            pr.eval(cos(phi), sin(phi), R);
+            K0 += fmax(R[0], 0.0);
+        }
+        K0 *= dph;
+        term = fa_w[t] * mua * K0 * L0;
+    }
+    term = wave_sum(term);
+    if ((t & 63) == 0) red[t >> 6] = term;
+    __syncthreads();
+    if (t == 0) out[iw] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+}  // namespace
+}  // namespace ocrt
+
+extern "C" int ocrt_couple_atm_to_water(int m_max, int n_a, const double *mu_a, const double *w_a, const double *I_a,
+                                        const double *Q_a, const double *U_a, double n_water, int q_convention,
+                                        const double *mu_w, int n_w, double wind, int sigma_type, double *I_w,
+                                        double *Q_w, double *U_w, void *hip_stream)
+{
+    if (!mu_a || !I_a || !Q_a || !U_a || !mu_w || !I_w || !Q_w || !U_w) return OCRT_ERR_ARGS;
+    if (!(n_water > 1.0) || n_a < 2 || n_w < 1 || m_max < 0 || m_max + 1 > kIfMaxModes) return OCRT_ERR_ARGS;
+    if ((long long)n_w * n_a > kIfMaxPairs || !std::isfinite(wind)) return OCRT_ERR_ARGS;
+    hipStream_t st = (hipStream_t)hip_stream;
+    const long long M = m_max + 1;
+    const size_t nf = (size_t)M * n_a, no = (size_t)M * n_w;
+    DevBufs B;
+    double *d_field, *d_ma, *d_mw, *d_out;
+    IFCHK(B.get(&d_field, sizeof(double) * 3 * nf));
+    IFCHK(B.get(&d_ma, sizeof(double) * n_a));
+    IFCHK(B.get(&d_mw, sizeof(double) * n_w));
+    IFCHK(B.get(&d_out, sizeof(double) * 3 * no));
+    IFCHK(hipMemcpyAsync(d_field, I_a, sizeof(double) * nf, hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_field + nf, Q_a, sizeof(double) * nf, hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_field + 2 * nf, U_a, sizeof(double) * nf, hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_ma, mu_a, sizeof(double) * n_a, hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_mw, mu_w, sizeof(double) * n_w, hipMemcpyHostToDevice, st));
+    const double s2v = host_slope_variance(wind, sigma_type);
+    const unsigned grid = (unsigned)((no + 255) / 256);
+    if (w_a && wind > 0.0 && s2v > 1e-10) {
+        double *d_T, *d_wa;
+        IFCHK(B.get(&d_T, sizeof(double) * (size_t)M * n_w * n_a * 9));
+        IFCHK(B.get(&d_wa, sizeof(double) * n_a));
+        IFCHK(hipMemcpyAsync(d_wa, w_a, sizeof(double) * n_a, hipMemcpyHostToDevice, st));
+        const int rc = interface_fourier_device(OCRT_IFACE_T_AW, d_mw, n_w, d_ma, n_a, m_max, kTawPhi, wind,
+                                                sigma_type, n_water, q_convention, d_T, st);
+        if (rc != OCRT_OK) return rc;
+        couple_rough<<<grid, 256, 0, st>>>(d_T, d_field, d_ma, d_wa, n_w, n_a, m_max, d_out);
+        IFCHK(hipGetLastError());
+    } else {
+        // the air nodes sorted ascending, duplicates within 1e-12 dropped
+        // (rt_air_water_coupling.c:149-209)
+        std::vector<int> ord(n_a);
+        for (int a = 0; a < n_a; ++a) ord[a] = a;
+        for (int a = 1; a < n_a; ++a) {
+            const int k = ord[a];
+            int b = a - 1;
+            while (b >= 0 && mu_a[ord[b]] > mu_a[k]) { ord[b + 1] = ord[b]; --b; }
+            ord[b + 1] = k;
+        }
+        std::vector<double> ms;
+        std::vector<int> src;
+        for (int a = 0; a < n_a; ++a) {
+            const double m = mu_a[ord[a]];
+            if (!ms.empty() && std::fabs(m - ms.back()) < 1e-12) continue;
+            ms.push_back(m);
+            src.push_back(ord[a]);
+        }
+        double *d_ms;
+        int *d_src;
+        IFCHK(B.get(&d_ms, sizeof(double) * ms.size()));
+        IFCHK(B.get(&d_src, sizeof(int) * src.size()));
+        IFCHK(hipMemcpyAsync(d_ms, ms.data(), sizeof(double) * ms.size(), hipMemcpyHostToDevice, st));
+        IFCHK(hipMemcpyAsync(d_src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice, st));
+        couple_flat<<<grid, 256, 0, st>>>(d_field, d_ms, d_src, (int)ms.size(), d_mw, n_w, n_a, m_max, n_water,
+                                          q_convention, d_out);
+        IFCHK(hipGetLastError());
+        if (wind > 0.0) {
+            std::vector<double> fa(2 * kFineAir);
+            if (gauss_legendre_pos(kFineAir, fa.data(), fa.data() + kFineAir) != 0) return OCRT_ERR_ARGS;
+            double *d_fa;
+            IFCHK(B.get(&d_fa, sizeof(double) * fa.size()));
+            IFCHK(hipMemcpyAsync(d_fa, fa.data(), sizeof(double) * fa.size(), hipMemcpyHostToDevice, st));
+            couple_btdf_m0<<<n_w, 256, 0, st>>>(d_field, d_ms, d_src, (int)ms.size(), d_mw, d_fa, d_fa + kFineAir,
+                                                s2v, n_water, d_out);
+            IFCHK(hipGetLastError());
+        }
+    }
+    IFCHK(hipMemcpyAsync(I_w, d_out, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    IFCHK(hipMemcpyAsync(Q_w, d_out + no, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    IFCHK(hipMemcpyAsync(U_w, d_out + 2 * no, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    IFCHK(hipStreamSynchronize(st));
+    return OCRT_OK;
+}
+
+// The reference's coupling entry point and its output helpers, same names,
+// structs and return codes (rt_air_water_coupling.h:26-68).
+extern "C" int rt_aw_coupled_field_alloc(rt_aw_coupled_field_t *f, int m_max, int n_mu_water)
+{
+    if (!f || m_max < 0 || n_mu_water < 1) return -1;
+    std::memset(f, 0, sizeof(*f));
+    f->m_max = m_max;
+    f->n_mu_water = n_mu_water;
+    const size_t sz = (size_t)(m_max + 1) * (size_t)n_mu_water;
+    f->mu_water_pos = (double *)std::calloc((size_t)n_mu_water, sizeof(double));
+    f->I_inwater_per_m = (double *)std::calloc(sz, sizeof(double));
+    f->Q_inwater_per_m = (double *)std::calloc(sz, sizeof(double));
+    f->U_inwater_per_m = (double *)std::calloc(sz, sizeof(double));
+    if (!f->mu_water_pos || !f->I_inwater_per_m || !f->Q_inwater_per_m || !f->U_inwater_per_m) {
+        rt_aw_coupled_field_free(f);
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" void rt_aw_coupled_field_free(rt_aw_coupled_field_t *f)
+{
+    if (!f) return;
+    std::free(f->mu_water_pos);
+    std::free(f->I_inwater_per_m);
+    std::free(f->Q_inwater_per_m);
+    std::free(f->U_inwater_per_m);
+    std::memset(f, 0, sizeof(*f));
+}
+
+extern "C" int rt_air_water_couple_atm_to_water(const rt_atm_boa_export_t *boa, double n_water, int q_convention,
+                                                const double *mu_water_pos, int n_mu_water, const double *w_atm_pos,
+                                                double wind_speed, int sigma_type, rt_aw_coupled_field_t *coupled)
+{
+    if (!boa || !mu_water_pos || !coupled) return -1;
+    if (n_water <= 1.0 || boa->n_mu < 2) return -1;
+    if (coupled->n_mu_water != n_mu_water || coupled->m_max != boa->m_max) return -1;
+    if (!coupled->I_inwater_per_m || !coupled->Q_inwater_per_m || !coupled->U_inwater_per_m ||
+        !coupled->mu_water_pos)
+        return -1;
+    const int rc = ocrt_couple_atm_to_water(boa->m_max, boa->n_mu, boa->mu_quad_pos, w_atm_pos, boa->I_per_m,
+                                            boa->Q_per_m, boa->U_per_m, n_water, q_convention, mu_water_pos,
+                                            n_mu_water, wind_speed, sigma_type, coupled->I_inwater_per_m,
+                                            coupled->Q_inwater_per_m, coupled->U_inwater_per_m, nullptr);
+    if (rc != OCRT_OK) return rc == OCRT_ERR_ARGS ? -1 : -3;
+    std::memcpy(coupled->mu_water_pos, mu_water_pos, sizeof(double) * (size_t)n_mu_water);
+    return 0;
 }

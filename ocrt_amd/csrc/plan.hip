@@ -53,11 +53,6 @@ __global__ void glint_adj(const DevInst *, int, Pools);
 // lut.hip
 __global__ void lut_eval(const DevInst *, const DevItem *, Pools, const InstOut *, const int *, const double *, int,
                          const double *, int, const double *, double *, double *, double *, double *, int);
-}  // namespace ocrt
-
-using namespace ocrt;
-
-namespace {
 
 // Positive half of the 2n-point Gauss-Legendre rule, ascending, weights summing
 // to one: Newton iteration from Tricomi's estimate on the three-term
@@ -94,6 +89,13 @@ int gauss_legendre_pos(int n, double *mu, double *w)
     }
     return 0;
 }
+
+}  // namespace ocrt
+
+using namespace ocrt;
+
+namespace {
+
 
 // Insert the view cosine at its sorted position with zero weight unless a
 // node lies within 1e-12 (rt_angle_grid.c:21-73).  Returns the ring size.
