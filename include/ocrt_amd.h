@@ -330,6 +330,22 @@ int ocrt_couple_atm_to_water(int m_max, int n_a, const double *mu_a, const doubl
                              const double *mu_w, int n_w, double wind, int sigma_type, double *I_w,
                              double *Q_w, double *U_w, void *hip_stream);
 
+/* Water -> atmosphere coupling (rt_air_water_coupling.c:598-780): the
+ * in-water upwelling Fourier field I/Q/U_w [(m_max+1) * n_w] at the water
+ * nodes mu_w mapped onto the air nodes mu_a, I/Q/U_a [(m_max+1) * n_a].
+ *   wind > 0: per air node, the Cox-Munk water -> air Fourier transmission
+ *     (kind T_WA, 128 azimuths) over 1024 trapezoid points within 0.3 of the
+ *     refracted cosine plus 96 Gauss nodes outside, contracted with the
+ *     linearly interpolated water field (2 pi at m = 0, pi above); air nodes
+ *     outside (0, 1] give zero;
+ *   otherwise: reverse Snell, the flat water -> air radiance transmission
+ *     Mueller and the interpolated water field.
+ * The reference's opt-in multi-encounter re-escape term (environment switch,
+ * off by default) is not added.  n_water > 1, n_w >= 2, m_max <= 255. */
+int ocrt_couple_water_to_atm(int m_max, int n_w, const double *mu_w, const double *I_w, const double *Q_w,
+                             const double *U_w, double n_water, int q_convention, double wind, int sigma_type,
+                             int n_a, const double *mu_a, double *I_a, double *Q_a, double *U_a, void *hip_stream);
+
 /* One-call form: plan, execute, copy results, destroy. */
 int ocrt_solve_batch(const ocrt_instance_t *inst, int n_inst, int precision,
                      ocrt_result_t *out, void *hip_stream);

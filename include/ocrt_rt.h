@@ -374,6 +374,15 @@ void rt_aw_coupled_field_free(rt_aw_coupled_field_t *f);
 int  rt_air_water_couple_atm_to_water(const rt_atm_boa_export_t *boa_export, double n_water, int q_convention,
                                       const double *mu_water_pos, int n_mu_water, const double *w_atm_pos,
                                       double wind_speed, int sigma_type, rt_aw_coupled_field_t *coupled);
+/* Reverse coupling (rt_air_water_coupling.h:70-88), served by
+ * ocrt_couple_water_to_atm; the weights enter only the reference's opt-in
+ * re-escape term and are not used. */
+int  rt_air_water_couple_water_to_atm(const double *I_water_per_m, const double *Q_water_per_m,
+                                      const double *U_water_per_m, const double *mu_water_pos,
+                                      const double *w_water_pos, int n_mu_water, int m_max, double n_water,
+                                      int q_convention, double wind_speed, int sigma_type,
+                                      const double *mu_atm_pos, int n_mu_atm, const double *w_atm_pos,
+                                      double *I_air_per_m, double *Q_air_per_m, double *U_air_per_m);
 
 #ifdef __cplusplus
 }

@@ -892,3 +892,218 @@ extern "C" int rt_air_water_couple_atm_to_water(const rt_atm_boa_export_t *boa, 
     std::memcpy(coupled->mu_water_pos, mu_water_pos, sizeof(double) * (size_t)n_mu_water);
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Water -> atmosphere coupling: the in-water upwelling Fourier field just
+// below the surface mapped onto the air nodes (rt_air_water_coupling.c:598-780).
+//   rough (wind > 0): per air node mu_a, the water -> air Cox-Munk Fourier
+//     transmission (128 azimuths) over a composite water grid: 1024 trapezoid
+//     points within 0.3 of the refracted cosine plus the 96-node Gauss rule
+//     outside that window, contracted with the water field interpolated
+//     linearly onto the grid, times 2 pi (m = 0) or pi;
+//   flat: reverse Snell to mu_w, the flat water -> air radiance transmission
+//     Mueller and the interpolated water field.
+// The reference's multi-encounter re-escape term is opt-in there (an
+// environment switch, off by default) and is not added here.
+namespace ocrt {
+namespace {
+
+constexpr int kWinPoints = 1024;     // trapezoid window around the refracted cosine
+constexpr int kGloPoints = 96;       // Gauss nodes on the rest of (0, 1)
+constexpr double kWinHalf = 0.30;
+constexpr int kGridPoints = kWinPoints + kGloPoints;
+
+// Composite water grid of one air node: xi, wi [kGridPoints] (:640-660).
+__global__ void water_grid(const double *__restrict__ mu_a, int n_a, double n2, const double *__restrict__ gl,
+                           double *__restrict__ xi, double *__restrict__ wi)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_a * kGridPoints) return;
+    const int ja = e / kGridPoints, i = e % kGridPoints;
+    const double ma = mu_a[ja];
+    const double mws = sqrt(fmax(0.0, 1.0 - (1.0 - ma * ma) / n2));
+    double lo = mws - kWinHalf, hi = mws + kWinHalf;
+    if (lo < 1e-6) lo = 1e-6;
+    if (hi > 1.0 - 1e-9) hi = 1.0 - 1e-9;
+    const double h = (hi - lo) / (double)(kWinPoints - 1);
+    double x, w;
+    if (i < kWinPoints) {
+        x = lo + h * (double)i;
+        w = ((i == 0 || i == kWinPoints - 1) ? 0.5 : 1.0) * h;
+    } else {
+        x = gl[i - kWinPoints];
+        w = (x > lo && x < hi) ? 0.0 : gl[kGloPoints + i - kWinPoints];
+    }
+    xi[e] = x;
+    wi[e] = w;
+}
+
+// Rough contraction of one air node: one workgroup per mode.  T [m][1][grid][9],
+// water field [3][m_max + 1][n_w] on the sorted node map (ms, src, w).
+__global__ __launch_bounds__(256) void couple_rough_up(const double *__restrict__ T, const double *__restrict__ xi,
+                                                       const double *__restrict__ wi,
+                                                       const double *__restrict__ field, const double *__restrict__ ms,
+                                                       const int *__restrict__ src, int w, int n_w, int m_max, int ja,
+                                                       int n_a, double *__restrict__ out)
+{
+    __shared__ double red[3][4];
+    const int m = blockIdx.x, t = threadIdx.x;
+    const long long M = m_max + 1;
+    const double az = (m == 0) ? 2.0 * OCRT_PI : OCRT_PI;
+    const double *Iw = field + (long long)m * n_w, *Qw = Iw + M * n_w, *Uw = Qw + M * n_w;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int i = t; i < kGridPoints; i += blockDim.x) {
+        if (wi[i] == 0.0) continue;
+        const double wj = az * xi[i] * wi[i];
+        const double *tr = T + ((long long)m * kGridPoints + i) * 9;
+        const double I = interp_ascending(ms, Iw, src, w, xi[i]), Q = interp_ascending(ms, Qw, src, w, xi[i]),
+                     U = interp_ascending(ms, Uw, src, w, xi[i]);
+        s[0] += wj * (tr[0] * I + tr[1] * Q + tr[2] * U);
+        s[1] += wj * (tr[3] * I + tr[4] * Q + tr[5] * U);
+        s[2] += wj * (tr[6] * I + tr[7] * Q + tr[8] * U);
+    }
+    for (int c = 0; c < 3; ++c) {
+        const double v = wave_sum(s[c]);
+        if ((t & 63) == 0) red[c][t >> 6] = v;
+    }
+    __syncthreads();
+    if (t < 3) out[((long long)t * M + m) * n_a + ja] = (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
+}
+
+// Flat map: one thread per (mode, air node).
+__global__ void couple_flat_up(const double *__restrict__ field, const double *__restrict__ ms,
+                               const int *__restrict__ src, int w, const double *__restrict__ mu_a, int n_a, int n_w,
+                               int m_max, double nw, int qc, double *__restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (m_max + 1) * n_a) return;
+    const int m = e / n_a, ja = e % n_a;
+    const long long M = m_max + 1;
+    const double ma = mu_a[ja];
+    const double mw = sqrt(fmax(0.0, 1.0 - (1.0 - ma * ma) / (nw * nw)));
+    // flat radiance transmission water (n) -> air (1) at incidence mw
+    // (surface.c:431-460): t_s = 1 + r_s, t_p = n (1 + r_p), zero beyond the
+    // critical angle
+    double A = 0.0, q = 0.0, d = 0.0;
+    const double st2 = nw * nw * fmax(0.0, 1.0 - mw * mw);
+    if (st2 < 1.0) {
+        const double mt = sqrt(1.0 - st2);
+        const double rs = (nw * mw - mt) / (nw * mw + mt), rp = (mw - nw * mt) / (mw + nw * mt);
+        const double ts = 1.0 + rs, tp = nw * (1.0 + rp);
+        const double rad = (1.0 / (nw * nw)) * (mt / (nw * mw));
+        const double Ts = rad * ts * ts, Tp = rad * tp * tp;
+        A = 0.5 * (Ts + Tp);
+        q = (qc == 1) ? 0.5 * (Tp - Ts) : 0.5 * (Ts - Tp);
+        d = rad * ts * tp;
+    }
+    const double *Iw = field + (long long)m * n_w, *Qw = Iw + M * n_w, *Uw = Qw + M * n_w;
+    const double I = interp_ascending(ms, Iw, src, w, mw), Q = interp_ascending(ms, Qw, src, w, mw),
+                 U = interp_ascending(ms, Uw, src, w, mw);
+    out[(0 * M + m) * n_a + ja] = A * I + q * Q;
+    out[(1 * M + m) * n_a + ja] = q * I + A * Q;
+    out[(2 * M + m) * n_a + ja] = d * U;
+}
+
+// Ascending, duplicate-free view of a node set (insertion sort, 1e-12 dedup;
+// rt_air_water_coupling.c:149-209).
+void sorted_nodes(const double *mu, int n, std::vector<double> &ms, std::vector<int> &src)
+{
+    std::vector<int> ord(n);
+    for (int a = 0; a < n; ++a) ord[a] = a;
+    for (int a = 1; a < n; ++a) {
+        const int k = ord[a];
+        int b = a - 1;
+        while (b >= 0 && mu[ord[b]] > mu[k]) { ord[b + 1] = ord[b]; --b; }
+        ord[b + 1] = k;
+    }
+    ms.clear();
+    src.clear();
+    for (int a = 0; a < n; ++a) {
+        const double m = mu[ord[a]];
+        if (!ms.empty() && std::fabs(m - ms.back()) < 1e-12) continue;
+        ms.push_back(m);
+        src.push_back(ord[a]);
+    }
+}
+
+}  // namespace
+}  // namespace ocrt
+
+extern "C" int ocrt_couple_water_to_atm(int m_max, int n_w, const double *mu_w, const double *I_w, const double *Q_w,
+                                        const double *U_w, double n_water, int q_convention, double wind,
+                                        int sigma_type, int n_a, const double *mu_a, double *I_a, double *Q_a,
+                                        double *U_a, void *hip_stream)
+{
+    if (!I_w || !Q_w || !U_w || !mu_w || !mu_a || !I_a || !Q_a || !U_a) return OCRT_ERR_ARGS;
+    if (!(n_water > 1.0) || n_w < 2 || n_a < 1 || m_max < 0 || m_max + 1 > kIfMaxModes) return OCRT_ERR_ARGS;
+    if (!std::isfinite(wind)) return OCRT_ERR_ARGS;
+    hipStream_t st = (hipStream_t)hip_stream;
+    const long long M = m_max + 1;
+    const size_t nf = (size_t)M * n_w, no = (size_t)M * n_a;
+    std::vector<double> ms;
+    std::vector<int> src;
+    sorted_nodes(mu_w, n_w, ms, src);
+    DevBufs B;
+    double *d_field, *d_ma, *d_out, *d_ms;
+    int *d_src;
+    IFCHK(B.get(&d_field, sizeof(double) * 3 * nf));
+    IFCHK(B.get(&d_ma, sizeof(double) * n_a));
+    IFCHK(B.get(&d_out, sizeof(double) * 3 * no));
+    IFCHK(B.get(&d_ms, sizeof(double) * ms.size()));
+    IFCHK(B.get(&d_src, sizeof(int) * src.size()));
+    IFCHK(hipMemcpyAsync(d_field, I_w, sizeof(double) * nf, hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_field + nf, Q_w, sizeof(double) * nf, hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_field + 2 * nf, U_w, sizeof(double) * nf, hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_ma, mu_a, sizeof(double) * n_a, hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_ms, ms.data(), sizeof(double) * ms.size(), hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice, st));
+    if (wind > 0.0) {
+        std::vector<double> gl(2 * kGloPoints);
+        if (gauss_legendre_pos(kGloPoints, gl.data(), gl.data() + kGloPoints) != 0) return OCRT_ERR_ARGS;
+        double *d_gl, *d_xi, *d_wi, *d_T;
+        IFCHK(B.get(&d_gl, sizeof(double) * gl.size()));
+        IFCHK(B.get(&d_xi, sizeof(double) * (size_t)n_a * kGridPoints));
+        IFCHK(B.get(&d_wi, sizeof(double) * (size_t)n_a * kGridPoints));
+        IFCHK(B.get(&d_T, sizeof(double) * (size_t)M * kGridPoints * 9));
+        IFCHK(hipMemcpyAsync(d_gl, gl.data(), sizeof(double) * gl.size(), hipMemcpyHostToDevice, st));
+        IFCHK(hipMemsetAsync(d_out, 0, sizeof(double) * 3 * no, st));
+        const long long ng = (long long)n_a * kGridPoints;
+        water_grid<<<(unsigned)((ng + 255) / 256), 256, 0, st>>>(d_ma, n_a, n_water * n_water, d_gl, d_xi, d_wi);
+        IFCHK(hipGetLastError());
+        for (int ja = 0; ja < n_a; ++ja) {
+            if (!(mu_a[ja] > 0.0) || mu_a[ja] > 1.0) continue;   // outside (0, 1]: stays zero, as the reference
+            const double *xi = d_xi + (size_t)ja * kGridPoints, *wi = d_wi + (size_t)ja * kGridPoints;
+            const int rc = interface_fourier_device(OCRT_IFACE_T_WA, d_ma + ja, 1, xi, kGridPoints, m_max, kTawPhi,
+                                                    wind, sigma_type, n_water, q_convention, d_T, st);
+            if (rc != OCRT_OK) return rc;
+            couple_rough_up<<<(unsigned)M, 256, 0, st>>>(d_T, xi, wi, d_field, d_ms, d_src, (int)ms.size(), n_w,
+                                                         m_max, ja, n_a, d_out);
+            IFCHK(hipGetLastError());
+        }
+    } else {
+        couple_flat_up<<<(unsigned)((no + 255) / 256), 256, 0, st>>>(d_field, d_ms, d_src, (int)ms.size(), d_ma, n_a,
+                                                                      n_w, m_max, n_water, q_convention, d_out);
+        IFCHK(hipGetLastError());
+    }
+    IFCHK(hipMemcpyAsync(I_a, d_out, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    IFCHK(hipMemcpyAsync(Q_a, d_out + no, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    IFCHK(hipMemcpyAsync(U_a, d_out + 2 * no, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    IFCHK(hipStreamSynchronize(st));
+    return OCRT_OK;
+}
+
+extern "C" int rt_air_water_couple_water_to_atm(const double *I_water_per_m, const double *Q_water_per_m,
+                                                const double *U_water_per_m, const double *mu_water_pos,
+                                                const double *w_water_pos, int n_mu_water, int m_max,
+                                                double n_water, int q_convention, double wind_speed,
+                                                int sigma_type, const double *mu_atm_pos, int n_mu_atm,
+                                                const double *w_atm_pos, double *I_air_per_m,
+                                                double *Q_air_per_m, double *U_air_per_m)
+{
+    (void)w_water_pos;   // weights enter only the reference's opt-in re-escape term
+    (void)w_atm_pos;
+    const int rc = ocrt_couple_water_to_atm(m_max, n_mu_water, mu_water_pos, I_water_per_m, Q_water_per_m,
+                                            U_water_per_m, n_water, q_convention, wind_speed, sigma_type, n_mu_atm,
+                                            mu_atm_pos, I_air_per_m, Q_air_per_m, U_air_per_m, nullptr);
+    return rc == OCRT_OK ? 0 : (rc == OCRT_ERR_ARGS ? -1 : -3);
+}

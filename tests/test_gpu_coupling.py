@@ -87,3 +87,40 @@ def test_coupling_argument_checks(libs):
         c = Coupled(1, 3, _p(np.zeros(3)), _p(np.zeros(6)), _p(np.zeros(6)), _p(np.zeros(6)))
         assert fn(C.byref(b), 1.34, 1, _p(mu_a), 3, None, 0.0, 1, C.byref(c)) == -1   # m_max mismatch
         assert fn(C.byref(b), 1.0, 1, _p(mu_a), 3, None, 0.0, 1, C.byref(c)) == -1    # n_water <= 1
+
+
+def _bind_up(lib):
+    f = lib.rt_air_water_couple_water_to_atm
+    f.argtypes = [D, D, D, D, D, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, D, C.c_int, D, D, D, D]
+    f.restype = C.c_int
+    return f
+
+
+@pytest.mark.parametrize("wind", [5.0, 0.0])
+def test_reverse_coupling_equals_reference(ref_lib, wind):
+    """Water -> atmosphere: the rough composite-grid transmission (wind 5) and
+    the flat reverse-Snell map (wind 0) against the reference build."""
+    import ocrt_amd
+    ours, ref = _bind_up(ocrt_amd.library()), _bind_up(ref_lib)
+    m_max = 3
+    xw, ww = np.polynomial.legendre.leggauss(40)
+    mu_w, w_w = np.ascontiguousarray(xw[20:]), np.ascontiguousarray(ww[20:])
+    n_w = mu_w.size
+    rng = np.random.default_rng(3)
+    I = np.concatenate([np.exp(-2.0 * (1 - mu_w)) * 0.02 * 0.6 ** m for m in range(m_max + 1)])
+    Q = I * rng.uniform(-0.2, 0.2, I.size)
+    U = I * rng.uniform(-0.1, 0.1, I.size)
+    xa, wa = np.polynomial.legendre.leggauss(24)
+    mu_a = np.ascontiguousarray(np.concatenate([xa[12:], [0.43]]))
+    w_a = np.ascontiguousarray(np.concatenate([wa[12:], [0.0]]))
+    print(f"[gpu] reverse coupling wind {wind}", flush=True)
+    res = []
+    for fn in (ours, ref):
+        out = [np.zeros((m_max + 1) * mu_a.size) for _ in range(3)]
+        rc = fn(_p(I), _p(Q), _p(U), _p(mu_w), _p(w_w), n_w, m_max, 1.34, 1, wind, 1, _p(mu_a), mu_a.size, _p(w_a),
+                _p(out[0]), _p(out[1]), _p(out[2]))
+        assert rc == 0
+        res.append(out)
+    for s in range(3):
+        scale = max(np.abs(res[1][s]).max(), 1e-300)
+        assert np.abs(res[0][s] - res[1][s]).max() <= 1e-12 * scale, (wind, s)
