@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
             rho = lv_[u] + T * rho;
             pk0 = q0[u]; pk1 = q1[u]; sk = sp;
         }
-        wave_sum_8xC<3>(red, lane);
+        wave_sum_8xC_lds<3>(red, lane);
         const int u = lane >> 3, t = t0 + u;
         if ((lane & 7) == 0 && t < nt) {
             const int k = hemi ? nt - t : t;
@@ -251,7 +251,7 @@ __global__ void surface_adj(const DevInst *__restrict__ inst, const DevItem *__r
                 }
                 v[3 * j] = a0; v[3 * j + 1] = a1; v[3 * j + 2] = a2;
             }
-            wave_sum_8xC<3>(v, lane);
+            wave_sum_8xC_lds<3>(v, lane);
             const int i = i0 + (lane >> 3);
             if ((lane & 7) == 0 && i < n)
                 for (int c2 = 0; c2 < 3; ++c2) lamb[col_dn(n, c2, i)] += v[c2];
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
             lam_k = lv_[u];
             sk = sp;
         }
-        wave_sum_8xC<4>(red, lane);
+        wave_sum_8xC_lds<4>(red, lane);
         const int u = lane >> 3, t = t0 + u;
         if ((lane & 7) == 0 && t < nt) {
             const int k = hemi ? nt - t : t;
