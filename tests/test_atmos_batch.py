@@ -120,3 +120,33 @@ def test_solve_atm_batch_equals_oracle(oracle):
         assert abs(out["T_diff_dn_hemi"][b] - r.T_diff_dn_hemi) <= 1e-11 * abs(r.T_diff_dn_hemi), b
         assert out["n_orders"][b] == max(orders)
         assert np.array_equal(out["mu_quad"][b], atms[b].rm[n_mu + 1:])
+
+
+@pytest.mark.gpu
+def test_node_fields_agree_with_the_view_modes(oracle):
+    """The per-node TOA export at the view node is the per-mode TOA value the
+    results report (rough-ocean seed removed in both), bit for bit, and every
+    field is zero past the instance's ring and modes."""
+    import ocrt_amd
+    from ocrt_amd import synthetic as S
+    from ocrt_amd.abi import make_instance
+    gk = S.hg_greek(0.7, 30)
+    descs = [make_instance(surface="black_fresnel_ocean", wind=7.0, tau_a=0.2, ssa_a=0.9, greek=gk, n_mu=12,
+                           n_layers=30, m_max=5, sza=40.0, vza=25.0, raa=80.0),
+             make_instance(surface="flat", tau_a=0.1, ssa_a=0.95, greek=gk, n_mu=20, n_layers=45, m_max=3,
+                           sza=55.0, vza=62.0, raa=130.0)]
+    plan = ocrt_amd.Plan(descs, precision="mixed")
+    plan.export_nodes()
+    plan.execute()
+    res = plan.results(modes=True)
+    pitch = 24
+    up, dn = plan.node_fields(pitch)
+    plan.close()
+    for b, d in enumerate(descs):
+        _, _, vi = ring(oracle, d.n_mu, math.cos(math.radians(d.vza_deg)))
+        n_ring = int(res["n_ring"][b])
+        for m in range(d.m_max + 1):
+            assert np.array_equal(up[b, m, :, vi], res["toa_modes"][b, m]), (b, m)
+        assert not up[b, d.m_max + 1:].any() and not dn[b, d.m_max + 1:].any()
+        assert not up[b, :, :, n_ring:].any() and not dn[b, :, :, n_ring:].any()
+        assert np.all(dn[b, 0, 0, :n_ring] > 0.0)   # diffuse skylight reaches every node at m = 0
