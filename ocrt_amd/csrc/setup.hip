@@ -347,7 +347,12 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
 #pragma unroll
             for (int e = 0; e < C; ++e) red[wave][lane >> 3][E0 + e] = v[e];
     };
-    for (int m0 = 0; m0 <= M; m0 += kRowModes) {
+    // the last one or two modes of the value form fit one reduction: its 24
+    // slots hold (mode, entry) pairs q = 9 (m - m0) + kl instead of 8 rows of
+    // three entries, so the tail costs one reduction instead of three
+    const int tail = (!DERIV && (M + 1) % kRowModes != 0 && (M + 1) % kRowModes <= 2) ? (M + 1) % kRowModes : 0;
+    const int m_tail = M + 1 - tail;
+    for (int m0 = 0; m0 < m_tail; m0 += kRowModes) {
         reduce_entries(m0, std::integral_constant<int, 0>{}, std::integral_constant<int, 3>{});
         reduce_entries(m0, std::integral_constant<int, 3>{}, std::integral_constant<int, 3>{});
         reduce_entries(m0, std::integral_constant<int, 6>{}, std::integral_constant<int, 3>{});
@@ -369,6 +374,34 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
             }
         }
         __syncthreads();
+    }
+    if (tail > 0) {
+        double v[24];
+        const int t1 = t + kCoxMunkThreads;
+#pragma unroll
+        for (int q = 0; q < 24; ++q) {
+            const int mq = m_tail + (q < 9 * tail ? q / 9 : 0), kl = q % 9;
+            const double *wc = P.cm_w + (long long)mq * 2 * kNPhi, *ws = wc + kNPhi;
+            const bool use_cos = (kl == 0 || kl == 1 || kl == 3 || kl == 4 || kl == 8);
+            v[q] = (q < 9 * tail) ? r[0][kl] * (use_cos ? wc[t] : ws[t]) + r[1][kl] * (use_cos ? wc[t1] : ws[t1])
+                                  : 0.0;
+        }
+        wave_sum_8xC<3>(v, lane);
+        if ((lane & 7) == 0)
+#pragma unroll
+            for (int e = 0; e < 3; ++e) red[wave][lane >> 3][e] = v[e];
+        __syncthreads();
+        if (t < 9 * tail) {
+            const int row = t / 3, e = t % 3, m = m_tail + t / 9, kl = t % 9;
+            double tot = 0.0;
+#pragma unroll
+            for (int wv = 0; wv < kWaves; ++wv) tot += red[wv][row][e];
+            const double nrm = (m == 0) ? 1.0 / (2.0 * OCRT_PI) : 1.0 / OCRT_PI;
+            double val = 2.0 * tot * (dphi * nrm);
+            if (m > 0 && (kl == 2 || kl == 5)) val = -val;
+            const long long at = !solar ? r_index(I, m, o, i, kl) : I.off_Rsun + ((long long)m * n + o) * 9 + kl;
+            P.R[at] = val;
+        }
     }
 }
 
