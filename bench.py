@@ -74,30 +74,70 @@ def timed_steps(plan, steps, warmup, sync):
     return (time.perf_counter() - t0) / steps
 
 
-def cpu_baseline(descs, n_sample, threads):
-    """Reference solver (oracle/_ref, built from the reference sources) on the
-    first n_sample instances, OpenMP over instances as rt_io_run_batch does;
-    falls back to the C restatement when the reference build is absent."""
-    from ocrt_amd.abi import Result, instance_array
-    sample = descs[:n_sample]
-    arr = instance_array(sample)
-    out = (Result * len(sample))()
+def _cpu_lib():
+    """The reference solver built from its sources (oracle/_ref), else the C
+    restatement; both are the CPU baseline, never the measured GPU path."""
     ref = os.path.join(ROOT, "oracle", "_ref", "libocrt_ref.so")
     if os.path.exists(ref):
-        lib, fn, kind = C.CDLL(ref), "ref_solve_batch", "reference"
-    else:
-        path = os.path.join(ROOT, "oracle", "liboracle.so")
-        if not os.path.exists(path):
-            import subprocess
-            subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
-                           stdout=subprocess.DEVNULL)
-        lib, fn, kind = C.CDLL(path), "oracle_solve_batch", "port"
+        return C.CDLL(ref), "ref_solve_batch", "reference"
+    path = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(path):
+        import subprocess
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
+                       stdout=subprocess.DEVNULL)
+    return C.CDLL(path), "oracle_solve_batch", "port"
+
+
+def spread_sample(descs, k):
+    """k instances spread evenly over the workload (every len/k-th), so the
+    sample's mix of layers, streams and bands follows the whole batch."""
+    step = max(1, len(descs) // k)
+    return [descs[i] for i in range(0, len(descs), step)][:k]
+
+
+def cpu_baseline(descs, n_sample, threads, n_single):
+    """Reference solver on a spread sample of the workload, OpenMP dynamic over
+    instances as rt_io_run_batch does (rt_io.c:199); then single-thread ms per
+    instance on a smaller spread sample."""
+    from ocrt_amd.abi import Result, instance_array
+    lib, fn, kind = _cpu_lib()
+    sample = spread_sample(descs, n_sample)
+    arr = instance_array(sample)
+    out = (Result * len(sample))()
     t0 = time.perf_counter()
     failed = getattr(lib, fn)(arr, len(sample), out, threads)
     dt = time.perf_counter() - t0
+    one = spread_sample(descs, n_single)
+    arr1 = instance_array(one)
+    out1 = (Result * len(one))()
+    t0 = time.perf_counter()
+    getattr(lib, fn)(arr1, len(one), out1, 1)
+    dt1 = time.perf_counter() - t0
     return {"value": len(sample) / dt, "unit": "solves/s", "cores": threads, "kind": kind,
-            "sample": f"first {len(sample)} instances of the workload ({failed} failed), "
-                      f"OpenMP dynamic over instances, {dt:.1f} s wall"}
+            "nproc": os.cpu_count(),
+            "single_thread_ms_per_instance": dt1 / len(one) * 1e3,
+            "sample": f"{len(sample)} instances spread over the workload (every {len(descs) // len(sample)}-th; "
+                      f"{failed} failed), OpenMP dynamic over instances on {threads} threads "
+                      f"(the host's CPU share for one GPU; nproc {os.cpu_count()}), {dt:.1f} s wall; "
+                      f"single thread: {len(one)} spread instances, {dt1:.1f} s"}
+
+
+def survey_work(descs, orders, n_ring):
+    """Algorithmic work of the orders >= 2 (the ones the scattering-integral
+    and sweep kernels run), by the per-unit figures of the survey:
+      F = 72 n_out n_in (nt + 1) flop per order and mode (n_out = ring size,
+          n_in = Gauss nodes of non-zero weight);
+      B = (2 * 3 * 2 n (nt + 1) + 2 n nt) * 4 bytes per order and mode (fp32
+          source read and field written for I, Q, U in both hemispheres, plus
+          the transmission table)."""
+    F = B = 0.0
+    for i, d in enumerate(descs):
+        n, nin, nt = int(n_ring[i]), d.n_mu, d.n_layers
+        runs = float(np.maximum(orders[i, : d.m_max + 1] - 1, 0).sum())
+        if d.L_max > 0:
+            F += runs * 72.0 * n * nin * (nt + 1)
+        B += runs * (2 * 3 * 2 * n * (nt + 1) + 2 * n * nt) * 4.0
+    return F, B
 
 
 def main():
@@ -107,7 +147,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--geoms", type=int, default=1024, help="geometries per band (workload size)")
     ap.add_argument("--plain-sample", type=int, default=64)
-    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--cpu-single", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -149,7 +190,7 @@ def main():
     pm.profile(True)
     ms_mixed = timed_steps(pm, args.steps, args.warmup, sync) * 1e3
     kt = pm.kernel_times()
-    res_m = pm.results()
+    res_m = pm.results(modes=True)
     work = pm.work()
     log(f"mixed: {ms_mixed:.1f} ms/step, kernels {kt}")
     pg = ocrt_amd.Plan(descs, precision="mixed", graph=True)
@@ -166,7 +207,7 @@ def main():
     log("fp64 batched plan")
     pf = ocrt_amd.Plan(descs, precision="fp64")
     ms_fp64 = timed_steps(pf, 1, 1, sync) * 1e3
-    res_f = pf.results()
+    res_f = pf.results(modes=True)
     pf.close()
     log(f"fp64 batched: {ms_fp64:.1f} ms/step")
 
@@ -178,11 +219,12 @@ def main():
     worst = int(np.argmax(np.where(ok, rel_I, 0.0)))
     log(f"mixed vs fp64: max rel I {err_I:.3e} (instance {worst}), max abs Q/U {err_QU:.3e}")
 
-    log(f"fp64 one instance per launch on {args.plain_sample} instances")
-    pp = ocrt_amd.Plan(descs[: args.plain_sample], precision="fp64", one_per_launch=True)
+    log(f"fp64 one instance per launch on {args.plain_sample} instances spread over the workload")
+    plain = spread_sample(descs, args.plain_sample)
+    pp = ocrt_amd.Plan(plain, precision="fp64", one_per_launch=True)
     ms_plain = timed_steps(pp, 1, 1, sync) * 1e3
     pp.close()
-    plain_rate = args.plain_sample / (ms_plain / 1e3)
+    plain_rate = len(plain) / (ms_plain / 1e3)
 
     adjoint = None
     if not args.no_adjoint:
@@ -201,13 +243,29 @@ def main():
             sync()
             t_adj += (time.perf_counter() - t0) / 3
         pa.close()
+        # the workload's scheme: checkpoints every 4 orders, the Jacobian call
+        # runs its own checkpointing forward, then the backward recursion with
+        # recomputation between checkpoints
+        ps = ocrt_amd.Plan(d3, precision="fp64", adjoint=True, ckpt_sparse=True)
+        ps.adjoint(g3)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            ps.adjoint(g3)
+        sync()
+        t_jac = (time.perf_counter() - t0) / 3
+        ps.close()
         pm3 = ocrt_amd.Plan(d3, precision="mixed")
         t_mix = timed_steps(pm3, 3, 1, sync)
         pm3.close()
         adjoint = {"instances": len(d3), "fp64_forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3,
-                   "mixed_forward_ms": t_mix * 1e3, "adjoint_over_fp64_forward": t_adj / t_fwd,
+                   "mixed_forward_ms": t_mix * 1e3,
+                   "jacobian_ckpt4_ms": t_jac * 1e3,
+                   "jacobian_ckpt4_over_mixed_forward": t_jac / t_mix,
+                   "jacobian_ckpt4_over_fp64_forward": t_jac / t_fwd,
+                   "adjoint_over_fp64_forward": t_adj / t_fwd,
                    "adjoint_over_mixed_forward": t_adj / t_mix,
-                   "jacobian_rows_per_s": len(d3) / t_adj}
+                   "jacobian_rows_per_s": len(d3) / t_jac}
         log(f"adjoint: {adjoint}")
 
     single = None
@@ -226,24 +284,35 @@ def main():
     value = n / (ms_mixed / 1e3)
     gemm_s = kt["gemm_ms"] / 1e3
     sweep_s = kt["sweep_ms"] / 1e3
-    gemm_tf = work["gemm_flops"] / gemm_s / 1e12 if gemm_s > 0 else 0.0
-    sweep_gbs = work["sweep_bytes"] / sweep_s / 1e9 if sweep_s > 0 else 0.0
+    F_alg, B_alg = survey_work(descs, res_m["orders_per_m"], res_m["n_ring"])
+    gemm_tf = F_alg / gemm_s / 1e12 if gemm_s > 0 else 0.0
+    sweep_gbs = B_alg / sweep_s / 1e9 if sweep_s > 0 else 0.0
     traffic = measured_traffic(args.geoms)
+    gl, sl = max(kt["gemm_launches"], 1), max(kt["sweep_launches"], 1)
     rl_gemm = {"kernel": "gemm_bf16", "bound": "mfma", "achieved": gemm_tf, "peak": BF16_PEAK_TFLOPS,
                "unit": "TFLOP/s", "frac": gemm_tf / BF16_PEAK_TFLOPS,
                "traffic": traffic["gemm_bf16"] if traffic else None,
-               "ms_total": kt["gemm_ms"], "launches": kt["gemm_launches"]}
+               "algorithmic_per_launch": F_alg / gl, "avg_launch_ms": kt["gemm_ms"] / gl,
+               "ms_total": kt["gemm_ms"], "launches": kt["gemm_launches"],
+               "issued_model_tflops": work["gemm_flops"] / gemm_s / 1e12 if gemm_s > 0 else 0.0}
     rl_sweep = {"kernel": "sweep<float,true>", "bound": "hbm", "achieved": sweep_gbs, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS,
                 "traffic": traffic["sweep"] if traffic else None,
-                "ms_total": kt["sweep_ms"], "launches": kt["sweep_launches"]}
+                "algorithmic_per_launch": B_alg / sl, "avg_launch_ms": kt["sweep_ms"] / sl,
+                "ms_total": kt["sweep_ms"], "launches": kt["sweep_launches"],
+                "moved_model_gbs": work["sweep_bytes"] / sweep_s / 1e9 if sweep_s > 0 else 0.0}
     roofline = rl_gemm if kt["gemm_ms"] >= kt["sweep_ms"] else rl_sweep
+    om = res_m["orders_per_m"].astype(np.float64)
+    of = res_f["orders_per_m"].astype(np.float64)
+    orders_per_mode = {"mixed_mean": [round(v, 3) for v in om.mean(axis=0).tolist()],
+                       "fp64_mean": [round(v, 3) for v in of.mean(axis=0).tolist()],
+                       "max_abs_diff": int(np.abs(om - of).max())}
 
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        log(f"cpu baseline on {args.cpu_sample} instances, {threads} threads")
-        cpu = cpu_baseline(descs, args.cpu_sample, threads)
+        log(f"cpu baseline on {args.cpu_sample} spread instances, {threads} threads (nproc {os.cpu_count()})")
+        cpu = cpu_baseline(descs, args.cpu_sample, threads, args.cpu_single)
 
     line = {
         "metric": METRIC,
@@ -271,7 +340,8 @@ def main():
         "max_rel_err_I_vs_fp64": err_I,
         "max_abs_err_QU_vs_fp64": err_QU,
         "worst_instance_I": worst,
-        "adjoint_fwd_ratio": adjoint["adjoint_over_fp64_forward"] if adjoint else None,
+        "orders_per_mode": orders_per_mode,
+        "adjoint_fwd_ratio": adjoint["jacobian_ckpt4_over_mixed_forward"] if adjoint else None,
         "adjoint": adjoint,
         "single_instance_config1": single,
         "roofline": roofline,

@@ -192,3 +192,33 @@ def test_extreme_sizes_in_one_ragged_batch(ocrt, oracle):
         assert list(a["orders_per_m"][i, : d.m_max + 1]) == list(orders)
         assert abs(b["rho_I"][i] - a["rho_I"][i]) <= MIXED_TOL_I * abs(a["rho_I"][i])
         assert max(abs(b["rho_Q"][i] - a["rho_Q"][i]), abs(b["rho_U"][i] - a["rho_U"][i])) <= MIXED_TOL_QU
+
+
+def test_config1_full_size_all_modes(ocrt):
+    """The full single-instance Rayleigh + aerosol case: 48 streams, 400 layers,
+    180 Greek terms, rough ocean, IPSS, Fourier modes 0..95 (the reference
+    stops at 32; the record is the oracle's, its modes 0..32 pinned to the
+    reference build).  FP64 within 1e-11 in rho and in every mode, equal
+    order counts; the mixed form within its budget."""
+    import json
+    import os
+    from conftest import GOLDEN
+    from ocrt_amd import synthetic as S
+    with open(os.path.join(GOLDEN, "oracle_config1_m95.json")) as f:
+        rec = json.load(f)
+    ds, gk = S.config1(m_max=95)
+    print("[gpu] config 1 full size: n 48, nt 400, L 180, m 0..95, IPSS", flush=True)
+    a = ocrt.solve(ds, precision="fp64", modes=True)
+    b = ocrt.solve(ds, precision="mixed", modes=True)
+    ref = rec["result"]
+    assert a["status"][0] == 0 and b["status"][0] == 0
+    for f in ("rho_I", "rho_Q", "rho_U"):
+        assert abs(a[f][0] - ref[f]) <= FP64_TOL * abs(ref["rho_I"]), (f, a[f][0], ref[f])
+    rm = np.array(rec["toa_modes"]).reshape(-1, 3)
+    assert np.max(np.abs(a["toa_modes"][0, :96] - rm)) <= FP64_TOL * np.max(np.abs(rm[:, 0]))
+    assert list(a["orders_per_m"][0, :96]) == rec["orders_per_m"]
+    assert abs(a["kappa"][0] - ref["kappa"]) <= FP64_TOL * ref["kappa"]
+    ei = abs(b["rho_I"][0] - ref["rho_I"]) / abs(ref["rho_I"])
+    eq = max(abs(b["rho_Q"][0] - ref["rho_Q"]), abs(b["rho_U"][0] - ref["rho_U"]))
+    print(f"[gpu] config 1 mixed: rel I {ei:.2e}, abs Q/U {eq:.2e}", flush=True)
+    assert ei <= MIXED_TOL_I and eq <= MIXED_TOL_QU
