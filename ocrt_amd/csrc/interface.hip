@@ -441,18 +441,28 @@ using namespace ocrt;
 
 namespace ocrt {
 
-int interface_fourier_device(int kind, const double *d_mo, int n_o, const double *d_mi, int n_i, int m_max,
-                             int n_phi, double wind, int sigma_type, double n_water, int q_convention,
-                             double *d_out, hipStream_t st)
+// Rows of the azimuth weight table for modes 0..m_max: the m = 1 entries
+// carry cos/sin of the azimuth itself, so at least two rows.
+int interface_weight_rows(int m_max) { return m_max > 1 ? m_max + 1 : 2; }
+
+// Fill the azimuth weight table W[n_phi][interface_weight_rows(m_max)]
+// (caller-owned device memory) on st.
+hipError_t interface_weights_launch(int n_phi, int m_max, double2 *d_w, hipStream_t st)
 {
-    const double s2v = host_slope_variance(wind, sigma_type);
-    const int w_rows = m_max > 1 ? m_max + 1 : 2;   // the m = 1 entries carry cos/sin of the azimuth
-    DevBufs B;
-    double2 *d_w;
-    IFCHK(B.get(&d_w, sizeof(double2) * (size_t)w_rows * n_phi));
+    const int w_rows = interface_weight_rows(m_max);
     const long long n_w = (long long)w_rows * n_phi;
     interface_weights<<<(unsigned)((n_w + 255) / 256), 256, 0, st>>>(n_phi, w_rows, d_w);
-    IFCHK(hipGetLastError());
+    return hipGetLastError();
+}
+
+// Launch the Fourier projection of one interface matrix over every node pair
+// against a weight table made by interface_weights_launch for the same m_max
+// and n_phi.  Asynchronous: no allocation, no synchronisation.
+hipError_t interface_fourier_launch(int kind, const double *d_mo, int n_o, const double *d_mi, int n_i, int m_max,
+                                    int n_phi, double s2v, double n_water, int q_convention, const double2 *d_w,
+                                    double *d_out, hipStream_t st)
+{
+    const int w_rows = interface_weight_rows(m_max);
     const unsigned grid = (unsigned)((long long)n_o * n_i);
     switch (kind) {
     case OCRT_IFACE_R_AA:
@@ -472,7 +482,20 @@ int interface_fourier_device(int kind, const double *d_mo, int n_o, const double
                                                                          n_water, q_convention, d_w, w_rows, d_out);
         break;
     }
-    IFCHK(hipGetLastError());
+    return hipGetLastError();
+}
+
+int interface_fourier_device(int kind, const double *d_mo, int n_o, const double *d_mi, int n_i, int m_max,
+                             int n_phi, double wind, int sigma_type, double n_water, int q_convention,
+                             double *d_out, hipStream_t st)
+{
+    const double s2v = host_slope_variance(wind, sigma_type);
+    DevBufs B;
+    double2 *d_w;
+    IFCHK(B.get(&d_w, sizeof(double2) * (size_t)interface_weight_rows(m_max) * n_phi));
+    IFCHK(interface_weights_launch(n_phi, m_max, d_w, st));
+    IFCHK(interface_fourier_launch(kind, d_mo, n_o, d_mi, n_i, m_max, n_phi, s2v, n_water, q_convention, d_w, d_out,
+                                   st));
     // the weight table is released when this returns: finish with it first
     IFCHK(hipStreamSynchronize(st));
     return OCRT_OK;
@@ -1070,12 +1093,17 @@ extern "C" int ocrt_couple_water_to_atm(int m_max, int n_w, const double *mu_w, 
         const long long ng = (long long)n_a * kGridPoints;
         water_grid<<<(unsigned)((ng + 255) / 256), 256, 0, st>>>(d_ma, n_a, n_water * n_water, d_gl, d_xi, d_wi);
         IFCHK(hipGetLastError());
+        // one azimuth weight table serves every air node; the launches below
+        // follow each other on st, so d_T is reused without synchronising
+        double2 *d_w;
+        IFCHK(B.get(&d_w, sizeof(double2) * (size_t)interface_weight_rows(m_max) * kTawPhi));
+        IFCHK(interface_weights_launch(kTawPhi, m_max, d_w, st));
+        const double s2v = host_slope_variance(wind, sigma_type);
         for (int ja = 0; ja < n_a; ++ja) {
             if (!(mu_a[ja] > 0.0) || mu_a[ja] > 1.0) continue;   // outside (0, 1]: stays zero, as the reference
             const double *xi = d_xi + (size_t)ja * kGridPoints, *wi = d_wi + (size_t)ja * kGridPoints;
-            const int rc = interface_fourier_device(OCRT_IFACE_T_WA, d_ma + ja, 1, xi, kGridPoints, m_max, kTawPhi,
-                                                    wind, sigma_type, n_water, q_convention, d_T, st);
-            if (rc != OCRT_OK) return rc;
+            IFCHK(interface_fourier_launch(OCRT_IFACE_T_WA, d_ma + ja, 1, xi, kGridPoints, m_max, kTawPhi, s2v,
+                                           n_water, q_convention, d_w, d_T, st));
             couple_rough_up<<<(unsigned)M, 256, 0, st>>>(d_T, xi, wi, d_field, d_ms, d_src, (int)ms.size(), n_w,
                                                          m_max, ja, n_a, d_out);
             IFCHK(hipGetLastError());
