@@ -301,6 +301,22 @@ int ocrt_interface_fourier(int kind, const double *mu_o, int n_o, const double *
                            int n_phi, double wind, int sigma_type, double n_water, int q_convention,
                            double *out, void *hip_stream);
 
+/* Allocation-free form of ocrt_interface_fourier.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Inputs and output are caller-owned DEVICE memory on the current device:
+ * d_mu_o[n_o], d_mu_i[n_i] (cosines in (0, 1], not checked here), d_out with
+ * the layout of ocrt_interface_fourier.  The azimuth weight table lives in a
+ * fixed 4 MiB device array per GPU, so (max(m_max, 1) + 1) * n_phi must not
+ * exceed 262144 (m_max 255 at n_phi 1024; OCRT_ERR_ARGS and a message
+ * otherwise).  The call allocates and frees nothing and does not
+ * synchronise: the result is in d_out when hip_stream (NULL: default stream)
+ * reaches it.  Calls that overlap on one GPU share the table and race; order
+ * them on one stream.  Same bits as ocrt_interface_fourier. */
+int ocrt_interface_fourier_noalloc(int kind, const double *d_mu_o, int n_o, const double *d_mu_i, int n_i,
+                                   int m_max, int n_phi, double wind, int sigma_type, double n_water,
+                                   int q_convention, double *d_out, void *hip_stream);
+
 /* Slope-integrated direct irradiance terms for n cosines (host memory):
  * OCRT_IFACE_T_AW the air -> water transmittance of the solar beam,
  * surface_T_aw_coxmunk_direct (surface.h:43, surface.c:138-198);

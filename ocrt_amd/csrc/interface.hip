@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -527,6 +528,73 @@ extern "C" int ocrt_interface_fourier(int kind, const double *mu_o, int n_o, con
     if (rc != OCRT_OK) return rc;
     IFCHK(hipMemcpyAsync(out, d_out, sizeof(double) * n_out, hipMemcpyDeviceToHost, st));
     IFCHK(hipStreamSynchronize(st));
+    return OCRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form of ocrt_interface_fourier.  Its one scratch buffer, the
+// azimuth weight table, lives in a file-scope device array: 2^18 (cos, sin)
+// pairs, 4 MiB on every GPU the library runs on, held until the process ends.
+// Host code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address); it uses the address hipGetSymbolAddress
+// returns on each device, looked up once per device at the first call there.
+namespace ocrt {
+namespace {
+constexpr long long kIfScratchPairs = 1LL << 18;
+constexpr int kScratchDevices = 64;
+__device__ __align__(256) double2 scratch[kIfScratchPairs];
+std::mutex g_scratch_mu;
+double2 *g_scratch_addr[kScratchDevices];
+
+// The scratch table's device address on the current device (nullptr and a
+// message on failure).
+double2 *scratch_table()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices) {
+        fprintf(stderr, "ocrt_amd: no usable current device for the scratch table\n");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "ocrt_amd: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+                    hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_addr[dev] = (double2 *)p;
+    }
+    return g_scratch_addr[dev];
+}
+}  // namespace
+}  // namespace ocrt
+
+extern "C" int ocrt_interface_fourier_noalloc(int kind, const double *d_mu_o, int n_o, const double *d_mu_i,
+                                              int n_i, int m_max, int n_phi, double wind, int sigma_type,
+                                              double n_water, int q_convention, double *d_out, void *hip_stream)
+{
+    if (kind < OCRT_IFACE_R_AA || kind > OCRT_IFACE_T_AW || !d_mu_o || !d_mu_i || !d_out) return OCRT_ERR_ARGS;
+    if (n_o < 1 || n_i < 1 || (long long)n_o * n_i > kIfMaxPairs) return OCRT_ERR_ARGS;
+    if (m_max < 0 || m_max + 1 > kIfMaxModes || n_phi < 4 || n_phi > OCRT_IFACE_MAX_PHI) return OCRT_ERR_ARGS;
+    if (!std::isfinite(wind) || !(n_water > 0.0) || !std::isfinite(n_water)) return OCRT_ERR_ARGS;
+    if ((long long)interface_weight_rows(m_max) * n_phi > kIfScratchPairs) {
+        fprintf(stderr,
+                "ocrt_interface_fourier_noalloc: (m_max + 1) x n_phi = %lld exceeds the %lld-entry scratch table "
+                "(e.g. m_max 255 with n_phi 1024)\n",
+                (long long)interface_weight_rows(m_max) * n_phi, kIfScratchPairs);
+        return OCRT_ERR_ARGS;
+    }
+    const double s2v = host_slope_variance(wind, sigma_type);
+    if (kind == OCRT_IFACE_T_AW && s2v <= 1e-10) return OCRT_ERR_FLAT;
+    double2 *w = scratch_table();
+    if (!w) return OCRT_ERR_DEVICE;
+    hipStream_t st = (hipStream_t)hip_stream;
+    // every entry of the table is rewritten before the projection reads it
+    IFCHK(interface_weights_launch(n_phi, m_max, w, st));
+    IFCHK(interface_fourier_launch(kind, d_mu_o, n_o, d_mu_i, n_i, m_max, n_phi, s2v, n_water, q_convention, w,
+                                   d_out, st));
     return OCRT_OK;
 }
 
