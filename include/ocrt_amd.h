@@ -98,6 +98,18 @@ typedef struct {
     /* Optional gas absorption table (ocrt_gas_create); NULL: none.  Applies
      * to the column generator above (with layers, absorption is layers[3]). */
     const struct ocrt_gas *gas;
+    /* Optional angle-space aerosol phase function, the value kernel of
+     * rt_aerosol_input_t (code/OCRT_C/src/rt_solver.h:102-113, used at
+     * rt_solver.c:190-240 and rt_ipss.c:134-146): n_ang_phase >= 2 ascending
+     * scattering angles theta_phase[] in degrees with P11 values p11_phase[]
+     * (normalised as beta_0 = 1; host memory).  With an aerosol it replaces
+     * the Legendre sum of beta in the aerosol I -> I phase table, which is then
+     * the azimuth mean of the linearly interpolated table over 720 midpoints,
+     * and in the IPSS aerosol phase.  gamma, alpha and zeta still come from
+     * greek.  0 / NULL: the moment kernel. */
+    int n_ang_phase;
+    const double *theta_phase;
+    const double *p11_phase;
 } ocrt_instance_t;
 
 /* Gas absorption of an AFGL standard atmosphere with layered HITRAN cross

@@ -27,12 +27,15 @@
  *     depolarisation 0.0279 (rt_solver.c:646);
  *   - n_mu, n_layers, fourier_m_max (0..32), sos.max_iterations,
  *     sos.tolerance, pssa (IPSS), aer_h_km from rt_options_t;
- *   - aerosol: tau_a > 0 with its Greek coefficients (L_max >= 2).
+ *   - aerosol: tau_a > 0 with its Greek coefficients (L_max >= 2); with
+ *     use_value_kernel and a P11 angle table (n_ang_phase >= 2, theta_phase
+ *     ascending in degrees) the aerosol I -> I phase table and the IPSS
+ *     aerosol phase come from that table (rt_solver.c:190-240, 922-925;
+ *     rt_ipss.c:81-146), as in the reference (P12/P33 tables unread there).
  * Requests this build does not serve return -1 and print one line on stderr:
  * integration_method CONSTANT (a diagnostic of the reference), view_as_node
  * 0 (its diagnostic interpolation route), the ocean-coupling bottom source
- * (ext_bottom_*, bottom_source_only), the aerosol value kernel
- * (use_value_kernel) and surface OCEAN.  Gas absorption is served: abs_state
+ * (ext_bottom_*, bottom_source_only) and surface OCEAN.  Gas absorption is served: abs_state
  * is an rt_absorption_t (below).
  *
  * Every call solves one case in FP64 on the current HIP device, synchronously.

@@ -41,6 +41,9 @@ class Instance(C.Structure):
         ("tolerance", C.c_double),
         ("layers", C.POINTER(C.c_double)),
         ("gas", C.c_void_p),
+        ("n_ang_phase", C.c_int),
+        ("theta_phase", C.POINTER(C.c_double)),
+        ("p11_phase", C.POINTER(C.c_double)),
     ]
 
 
@@ -68,13 +71,16 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
                   surface="flat", n_water=1.34, wind=0.0, sigma_type=1,
                   q_convention=1, decouple_sunglint=0, tau_R=0.0941, depol=0.0279,
                   tau_a=0.0, ssa_a=1.0, aer_h_km=2.0, greek=None, n_mu=24,
-                  n_layers=50, m_max=2, max_orders=20, tolerance=1e-7, ipss=0, layers=None, gas=None):
+                  n_layers=50, m_max=2, max_orders=20, tolerance=1e-7, ipss=0, layers=None, gas=None,
+                  phase_table=None):
     """Fill one descriptor.  `greek` is a float64 array [4, L+1] (beta, gamma,
     alpha, zeta) or None.  `layers` is None (column generator) or a float64
     array [4, n_layers] of per-layer Rayleigh tau, aerosol tau, aerosol SSA and
     absorption tau, top down; n_layers then follows its shape.  `gas` is an
-    ocrt_gas handle (GasTable) or None.  Converted copies are kept alive by the
-    descriptor."""
+    ocrt_gas handle (GasTable) or None.  `phase_table` is None (moment kernel)
+    or a pair (theta_deg, P11) of equal-length arrays, ascending in theta: the
+    angle-space aerosol phase function of the value kernel.  Converted copies
+    are kept alive by the descriptor."""
     d = Instance()
     d.sza_deg, d.vza_deg, d.raa_deg, d.wavelength_nm = sza, vza, raa, wavelength
     d.surface = SURFACE_NAMES[surface] if isinstance(surface, str) else int(surface)
@@ -106,6 +112,19 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
         d.layers = None
     d.gas = gas.handle if gas is not None else None
     d._gas_keepalive = gas
+    if phase_table is not None:
+        th = np.ascontiguousarray(phase_table[0], dtype=np.float64)
+        p11 = np.ascontiguousarray(phase_table[1], dtype=np.float64)
+        if th.ndim != 1 or th.shape != p11.shape or th.size < 2:
+            raise ValueError("phase_table must be two equal 1-d arrays of at least 2 angles")
+        d.n_ang_phase = th.size
+        d.theta_phase = th.ctypes.data_as(C.POINTER(C.c_double))
+        d.p11_phase = p11.ctypes.data_as(C.POINTER(C.c_double))
+        d._phase_keepalive = (th, p11)
+    else:
+        d.n_ang_phase = 0
+        d.theta_phase = None
+        d.p11_phase = None
     return d
 
 

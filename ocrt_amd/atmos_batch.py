@@ -22,9 +22,13 @@ xdel[k+1] dtau_k (single-scattering albedo 1) and the rest of dtau_k as
 absorption, so the solver sees the same optical depths and scattering
 fractions; level 0 keeps the case's own fractions.  The cases run as one
 uneven batch on the GPU (FP64 by default, like the reference's float64
-arrays).  Not served (ValueError): the coupled-ocean bottom source, the
-aerosol value kernel, azimuth rules other than 1024 points, and the
-degenerate wind <= 0 with the Cox-Munk sigma law.
+arrays).  With use_value_kernel and aer_p11 (per case a pair of ascending
+scattering angles in degrees and normalised P11 values) the aerosol I -> I
+phase table is the azimuth mean of that table, as the reference's batched
+value kernel (code/OCRT_Python/ocrt_py/atmos_batch.py:929-947,
+aerosol.py:530-549).  Not served (ValueError): the coupled-ocean bottom
+source, azimuth rules other than 1024 points, and the degenerate wind <= 0
+with the Cox-Munk sigma law.
 """
 import math
 
@@ -78,12 +82,13 @@ def solve_atm_batch(atms, vjs, mu_suns, dphis, wind_list, n_mu, nt, m_max, max_i
         raise ValueError(f"mode {mode!r} is not a batched atmosphere mode")
     if bottom_source is not None:
         raise ValueError("the coupled-ocean bottom source is not served")
-    if use_value_kernel:
-        raise ValueError("the aerosol value kernel is not served (aerosol enters as Greek coefficients)")
     if n_phi_quad != 1024:
         raise ValueError("the Cox-Munk azimuth rule has 1024 points")
     greek_mode = mode == 'aerosol_black_fresnel_ocean'
     B = len(atms)
+    value_kernel = greek_mode and use_value_kernel and aer_p11 is not None
+    if value_kernel and len(aer_p11) != B:
+        raise ValueError("one (theta, P11) table per case")
     if not (len(vjs) == len(mu_suns) == len(dphis) == len(wind_list) == B):
         raise ValueError("one view index, solar cosine, azimuth and wind per case")
     descs, mu_quad, w_quad = [], np.zeros((B, n_mu)), np.zeros((B, n_mu))
@@ -128,7 +133,8 @@ def solve_atm_batch(atms, vjs, mu_suns, dphis, wind_list, n_mu, nt, m_max, max_i
                           decouple_sunglint=1, tau_R=tau_R,
                           depol=_depolarisation(atm.beta2) if hasattr(atm, "beta2") else _DEPOL_DEFAULT,
                           tau_a=float(lay[1].sum()), ssa_a=1.0, greek=greek, n_mu=n_gl, m_max=m_max,
-                          max_orders=max_iterations, tolerance=tolerance, ipss=0, layers=lay)
+                          max_orders=max_iterations, tolerance=tolerance, ipss=0, layers=lay,
+                          phase_table=(aer_p11[b][0], aer_p11[b][1]) if value_kernel else None)
         descs.append(d)
     plan = Plan(descs, precision=precision)
     plan.export_nodes()

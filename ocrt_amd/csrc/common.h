@@ -45,6 +45,9 @@ struct DevInst {
     long long off_layers;// pool_layers: [4][nt] Rayleigh tau, aerosol tau, aerosol ssa, absorption tau
     long long off_grad;  // pool_grad: gH[L], gX[L], gY[L], gD[nt], gW[1] (adjoint)
     double wl;           // wavelength (nm)
+    long long off_ang;   // pool_greek: value-kernel P11 table, theta[n_ang] (deg) then P11[n_ang]
+    int n_ang;           // entries of that table; 0: the moment kernel
+    int pad_ang;
 };
 
 // One gas absorption table in the gas pool: AFGL levels z[n_lev] (km,
@@ -147,6 +150,29 @@ constexpr int kCoxMunkThreads = 256;
 
 // Rows of the phase tables computed by one workgroup.
 constexpr int kTabRows = 8;
+
+// Azimuth midpoints of the aerosol value kernel (rt_solver.c:922-925) and the
+// half of them the even integrand is evaluated at.
+constexpr int kVkPhi = 720;
+constexpr int kVkHalf = kVkPhi / 2;
+
+// P11 of an ascending angle table at scattering-angle cosine cth: linear in
+// the angle (degrees), clamped to the end values (rt_solver.c:190-202,
+// rt_ipss.c:134-146).
+__host__ __device__ inline double p11_table_at(int n, const double *th, const double *p11, double cth)
+{
+    cth = cth > 1.0 ? 1.0 : (cth < -1.0 ? -1.0 : cth);
+    const double theta = acos(cth) * 180.0 / OCRT_PI;
+    if (theta <= th[0]) return p11[0];
+    if (theta >= th[n - 1]) return p11[n - 1];
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (th[mid] <= theta) lo = mid; else hi = mid;
+    }
+    const double t = (theta - th[lo]) / (th[hi] - th[lo]);
+    return p11[lo] + t * (p11[hi] - p11[lo]);
+}
 
 struct GemmTile { int item, tm, tn, pad; };   // item < 0: idle slot
 // Level width of a scattering-integral tile: bf16 form (gemm_bf16) and fp64

@@ -83,7 +83,11 @@ __global__ __launch_bounds__(256) void ipss_kappa(const DevInst *__restrict__ in
         const double b2 = (1.0 - dp) / (2.0 + dp);
         geo[0] = 1.0 + b2 * 0.5 * (3.0 * ct * ct - 1.0);
         double pa = 1.0;
-        if (I.aer) {
+        if (I.n_ang > 1) {
+            // value-kernel table: interpolated, not clipped (rt_ipss.c:134-146)
+            const double *th = P.greek + I.off_ang;
+            pa = p11_table_at(I.n_ang, th, th + I.n_ang, ct);
+        } else if (I.aer) {
             const double *g = P.greek + I.off_greek;
             double p0 = 1.0, p1 = ct, s = g[0];
             if (I.La >= 1) s += g[1] * p1;

@@ -26,6 +26,9 @@ template <bool DERIV>
 __global__ void coxmunk_fourier(const DevInst *, Pools, const int *, const int *, int, int);
 __global__ void angular_basis(const DevInst *, const DevItem *, Pools);
 __global__ void phase_tables(const DevInst *, const DevItem *, Pools, int, int);
+__global__ void value_weights(double *, int);
+__global__ void value_phase(const DevInst *, const DevItem *, Pools, const int *, const int *, int, int,
+                            const double *, int);
 template <typename AT, bool TRANS>
 __global__ void pack_operator(const DevInst *, const DevItem *, Pools, int, int);
 template <typename FT, bool MIXED>
@@ -154,6 +157,11 @@ struct Chunk {
     // (outgoing, incoming-or-sun) pair counts: one Cox-Munk workgroup per pair
     std::vector<int> hcm_owner, hcm_first;
     int *dcm_owner = nullptr, *dcm_first = nullptr;
+    // instances of the chunk whose aerosol I -> I table comes from the value
+    // kernel (operator owners only) and the prefix sums of their (row,
+    // column) entry counts: one value_phase workgroup per entry
+    std::vector<int> hvk_owner, hvk_first;
+    int *dvk_owner = nullptr, *dvk_first = nullptr;
     IpssJob *djobs = nullptr;
     int *djob_of = nullptr;
     int max_L = 0, max_P = 0, max_pairs = 0, max_tab = 0, tab_groups = 1, maxit = 1, any_rough = 0, n_jobs = 0;
@@ -217,6 +225,7 @@ struct ocrt_plan {
     std::vector<int> item_of_inst;   // first global item of each instance
     // global device arrays
     double *ring = nullptr, *greek = nullptr, *cm_w = nullptr, *layers = nullptr;
+    double *vk_w = nullptr;   // cos(m phi_q) of the value kernel, [360][mmax_all + 1]; null without one
     ItemState *state = nullptr;
     ItemOut *iout = nullptr;
     InstOut *ipss = nullptr;
@@ -334,6 +343,15 @@ static int validate(const ocrt_instance_t &d)
         if (d.tau_a < 0.0 || d.tau_R < 0.0) return OCRT_ERR_ARGS;
         if (!(d.tau_R + (aer ? d.tau_a : 0.0) > 0.0)) return OCRT_ERR_ARGS;
     }
+    if (d.n_ang_phase != 0) {
+        // the value kernel's P11 table: ascending angles in [0, 180], finite values
+        if (d.n_ang_phase < 2 || !d.theta_phase || !d.p11_phase) return OCRT_ERR_ARGS;
+        for (int a = 0; a < d.n_ang_phase; ++a) {
+            if (!std::isfinite(d.theta_phase[a]) || !std::isfinite(d.p11_phase[a])) return OCRT_ERR_ARGS;
+            if (d.theta_phase[a] < 0.0 || d.theta_phase[a] > 180.0) return OCRT_ERR_ARGS;
+            if (a > 0 && d.theta_phase[a] < d.theta_phase[a - 1]) return OCRT_ERR_ARGS;
+        }
+    }
     if (d.surface < 0 || d.surface > 3) return OCRT_ERR_ARGS;
     if ((d.surface == OCRT_SURFACE_FLAT || d.surface == OCRT_SURFACE_BLACK_FRESNEL_OCEAN) && !(d.n_water > 0.0))
         return OCRT_ERR_ARGS;
@@ -410,6 +428,9 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         ring_n += 2 * n;
         I.off_greek = greek_n;
         greek_n += I.aer ? 4LL * (I.La + 1) : 0;
+        I.n_ang = (I.aer && d.n_ang_phase >= 2) ? d.n_ang_phase : 0;
+        I.off_ang = greek_n;
+        greek_n += 2LL * I.n_ang;
         I.off_layers = layers_n;
         layers_n += I.layered ? 4LL * I.nt : 0;
         I.gidx = i;
@@ -442,6 +463,10 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         if (A.aer)
             for (int e = 0; e < 4 * (A.La + 1); ++e)
                 if (in[a].greek[e] != in[b].greek[e]) return false;
+        if (A.n_ang != B.n_ang) return false;
+        for (int e = 0; e < A.n_ang; ++e)
+            if (in[a].theta_phase[e] != in[b].theta_phase[e] || in[a].p11_phase[e] != in[b].p11_phase[e])
+                return false;
         return true;
     };
     for (int i = 0; i < n_inst && !p->per_inst; ++i)
@@ -601,6 +626,11 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 c.hitem.push_back(it);
             }
             if (I.ipss) c.hjobs.push_back(IpssJob{ii - c.i0, 0, I.vza, I.raa});
+            if (I.n_ang > 1 && op_from < 0) {
+                if (c.hvk_first.empty()) c.hvk_first.push_back(0);
+                c.hvk_owner.push_back(ii - c.i0);
+                c.hvk_first.push_back(c.hvk_first.back() + (n + 1) * (2 * n + 1));
+            }
             c.hinst.push_back(I);
         }
         // the GEMM tiles of one class stay on one XCD, ordered (row tile, item,
@@ -659,6 +689,9 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     // cos / sin (m phi) for m = 0 .. max(mmax, 1): the m = 1 row also gives the
     // azimuths of the Cox-Munk quadrature
     rc |= dmalloc(p, (void **)&p->cm_w, (size_t)(std::max(p->mmax_all, 1) + 1) * 2 * kNPhi * 8);
+    bool any_vk = false;
+    for (const Chunk &c : p->chunks) any_vk |= !c.hvk_owner.empty();
+    if (any_vk) rc |= dmalloc(p, (void **)&p->vk_w, (size_t)kVkHalf * (p->mmax_all + 1) * 8);
     rc |= dmalloc(p, (void **)&p->state, (size_t)p->n_items * sizeof(ItemState));
     rc |= dmalloc(p, (void **)&p->iout, (size_t)p->n_items * sizeof(ItemOut));
     rc |= dmalloc(p, (void **)&p->ipss, (size_t)n_inst * sizeof(InstOut));
@@ -722,6 +755,10 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             const int q = h.d.La + 1;
             for (int e = 0; e < 4 * q; ++e) hgreek[h.d.off_greek + e] = in[ii].greek[e];
         }
+        for (int e = 0; e < h.d.n_ang; ++e) {
+            hgreek[h.d.off_ang + e] = in[ii].theta_phase[e];
+            hgreek[h.d.off_ang + h.d.n_ang + e] = in[ii].p11_phase[e];
+        }
         if (h.d.layered)
             for (long long e = 0; e < 4LL * h.d.nt; ++e) hlayers[h.d.off_layers + e] = in[ii].layers[e];
     }
@@ -740,7 +777,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                       al(std::max<size_t>(c.hcols.size(), 1) * sizeof(ColTile)) +
                       al(std::max<size_t>(c.hjobs.size(), 1) * sizeof(IpssJob)) +
                       al((size_t)(c.i1 - c.i0) * sizeof(int)) + al(std::max<size_t>(c.hcm_owner.size(), 1) * 4) +
-                      al(std::max<size_t>(c.hcm_first.size(), 1) * 4);
+                      al(std::max<size_t>(c.hcm_first.size(), 1) * 4) +
+                      al(std::max<size_t>(c.hvk_owner.size(), 1) * 4) + al(std::max<size_t>(c.hvk_first.size(), 1) * 4);
     std::vector<unsigned char> hdesc(desc_bytes, 0);
     unsigned char *ddesc = nullptr;
     if (dmalloc(p, (void **)&ddesc, desc_bytes)) { ocrt_plan_destroy(p); return OCRT_ERR_ALLOC; }
@@ -765,6 +803,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         c.djob_of = (int *)put(job_of.data(), job_of.size() * sizeof(int), job_of.size() * sizeof(int));
         c.dcm_owner = (int *)put(c.hcm_owner.data(), c.hcm_owner.size() * 4, std::max<size_t>(c.hcm_owner.size(), 1) * 4);
         c.dcm_first = (int *)put(c.hcm_first.data(), c.hcm_first.size() * 4, std::max<size_t>(c.hcm_first.size(), 1) * 4);
+        c.dvk_owner = (int *)put(c.hvk_owner.data(), c.hvk_owner.size() * 4, std::max<size_t>(c.hvk_owner.size(), 1) * 4);
+        c.dvk_first = (int *)put(c.hvk_first.data(), c.hvk_first.size() * 4, std::max<size_t>(c.hvk_first.size(), 1) * 4);
     }
     if (hipMemcpy(ddesc, hdesc.data(), desc_bytes, hipMemcpyHostToDevice) != hipSuccess) {
         ocrt_plan_destroy(p);
@@ -772,6 +812,9 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     }
     const int cm_rows = std::max(p->mmax_all, 1);
     hipLaunchKernelGGL(fourier_weights, dim3((kNPhi + 255) / 256, cm_rows + 1), dim3(256), 0, 0, p->cm_w, cm_rows);
+    if (p->vk_w)
+        hipLaunchKernelGGL(value_weights, dim3((kVkHalf * (p->mmax_all + 1) + 255) / 256), dim3(256), 0, 0, p->vk_w,
+                           p->mmax_all + 1);
     if (hipDeviceSynchronize() != hipSuccess) { ocrt_plan_destroy(p); return OCRT_ERR_DEVICE; }
     *out = p;
     return OCRT_OK;
@@ -844,6 +887,14 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     hipLaunchKernelGGL(angular_basis, dim3(nit), dim3(128), 0, s, c.dinst, c.ditem, P);
     hipLaunchKernelGGL(phase_tables, dim3(xcd_item_grid(nit, c.tab_groups)), dim3(64), 0, s, c.dinst, c.ditem, P,
                        nit, c.tab_groups);
+    if (!c.hvk_owner.empty()) {
+        // value-kernel instances replace their I -> I table, one workgroup per
+        // entry, in launches of at most 2^22 workgroups
+        const int total = c.hvk_first.back(), no = (int)c.hvk_owner.size();
+        for (int base = 0; base < total; base += 1 << 22)
+            hipLaunchKernelGGL(value_phase, dim3((unsigned)std::min(total - base, 1 << 22)), dim3(256), 0, s,
+                               c.dinst, c.ditem, P, c.dvk_owner, c.dvk_first, no, base, p->vk_w, p->mmax_all + 1);
+    }
     // a few blocks per item striding over its operator: an item's table reads
     // stay in one L2 and the launch stays far below 2^31 work-items
     const int pack_blocks = (int)std::min<long long>(16, (c.max_pack + 255) / 256);

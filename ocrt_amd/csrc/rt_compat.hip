@@ -112,7 +112,10 @@ int prepare(const rt_case_t *cs, const rt_options_t *opts, const rt_aerosol_inpu
     }
     if (opts->rayleigh_model != RT_RAYLEIGH_MODEL_BODHAINE_1999) return OCRT_ERR_ARGS;
     const bool aerosol = aer && aer->tau_a > 0.0;
-    if (aerosol && aer->use_value_kernel) { say("the aerosol value kernel"); return OCRT_ERR_ARGS; }
+    // value kernel: the reference reads only the P11 angle table on this path
+    // (rt_solver.c:922-925 with :190-240, rt_ipss.c:81-146)
+    const bool value_kernel = aerosol && aer->use_value_kernel && aer->P11_phase;
+    if (value_kernel && (aer->n_ang_phase < 2 || !aer->theta_phase)) return OCRT_ERR_ARGS;
     // Rayleigh optical depth (rt_solver.c:648-658)
     if (cs->rayleigh_on == 0) {
         p.tau_R = 0.0;
@@ -161,6 +164,11 @@ int prepare(const rt_case_t *cs, const rt_options_t *opts, const rt_aerosol_inpu
         d.ssa_a = aer->ssa_a;
         d.L_max = Lm;
         d.greek = p.greek.data();
+        if (value_kernel) {
+            d.n_ang_phase = aer->n_ang_phase;
+            d.theta_phase = aer->theta_phase;
+            d.p11_phase = aer->P11_phase;
+        }
     } else {
         d.tau_a = 0.0;
         d.ssa_a = 1.0;

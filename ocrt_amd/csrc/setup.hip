@@ -250,6 +250,17 @@ __global__ void fourier_weights(double *w, int mmax)
     w[((long long)m * 2 + 1) * kNPhi + p] = sin(m * phi);
 }
 
+// cos(m phi_q) on the value kernel's half circle, [q][w_rows] (mode fastest):
+// phi_q = (q + 1/2) 2 pi / 720, q < 360.
+__global__ void value_weights(double *w, int w_rows)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kVkHalf * w_rows) return;
+    const int q = e / w_rows, m = e % w_rows;
+    const double phi = 2.0 * OCRT_PI * ((double)q + 0.5) / (double)kVkPhi;
+    w[e] = cos((double)m * phi);
+}
+
 // ---------------------------------------------------------------------------
 // Rough-ocean Fourier kernels.  One workgroup per (instance, outgoing node,
 // incoming node or the sun), over the pairs of the instances that own their
@@ -436,6 +447,50 @@ __device__ inline double dir_mu(const double *ring_mu, int n, double mus, int d)
     if (j == 0) return -mus;
     return j > 0 ? ring_mu[j - 1] : -ring_mu[-j - 1];
 }
+
+// Value kernel of the aerosol I -> I phase table (rt_solver.c:204-240,
+// selected at :922-925), for every Fourier mode of an instance:
+//   P^m[j][k] = (1/720) sum_q P11(mu_j mu_k + s_j s_k cos phi_q) cos(m phi_q)
+// over the 720 azimuth midpoints, P11 from the instance's angle table.  One
+// workgroup per (owner instance, row j = 0..n, column direction k): the
+// integrand is even in phi (phi_q and 2 pi - phi_q pair up), so the 360
+// azimuths of the half circle are evaluated once into LDS and thread m folds
+// them onto cos(m phi_q) for mode m, writing the entry of that mode's item.
+// Runs after phase_tables and replaces its P table.
+__global__ __launch_bounds__(256) void value_phase(const DevInst *__restrict__ inst, const DevItem *__restrict__ items,
+                                                   Pools P, const int *__restrict__ owners,
+                                                   const int *__restrict__ first, int n_owners, int pair_base,
+                                                   const double *__restrict__ vw, int w_rows)
+{
+    __shared__ double val[kVkHalf];
+    const int b = pair_base + blockIdx.x;
+    if (b >= first[n_owners]) return;
+    int lo = 0, hi = n_owners;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (first[mid] <= b) lo = mid; else hi = mid;
+    }
+    const DevInst &I = inst[owners[lo]];
+    const int n = I.n, dirs = 2 * n + 1, pair = b - first[lo];
+    const int j = pair / dirs, kd = pair % dirs;
+    const double *ring = P.ring + I.off_ring;
+    const double mj = dir_mu(ring, n, I.mus, j + n), mk = dir_mu(ring, n, I.mus, kd);
+    const double sj = sqrt(fmax(0.0, 1.0 - mj * mj)), sk = sqrt(fmax(0.0, 1.0 - mk * mk));
+    const double *th = P.greek + I.off_ang, *p11 = th + I.n_ang;
+    for (int q = threadIdx.x; q < kVkHalf; q += blockDim.x) {
+        const double phi = 2.0 * OCRT_PI * ((double)q + 0.5) / (double)kVkPhi;
+        val[q] = p11_table_at(I.n_ang, th, p11, mj * mk + sj * sk * cos(phi));
+    }
+    __syncthreads();
+    const long long cnt = (long long)(n + 1) * dirs;
+    for (int m = threadIdx.x; m <= I.mmax; m += blockDim.x) {
+        double acc = 0.0;
+        for (int q = 0; q < kVkHalf; ++q) acc = fma(val[q], vw[(long long)q * w_rows + m], acc);
+        const DevItem &it = items[I.item0 + m];
+        P.tab[it.off_tab + 0 * cnt + (long long)j * dirs + kd] = 2.0 * acc / (double)kVkPhi;
+    }
+}
+
 
 // Basis of one item: one thread per direction runs the three recurrences in l.
 __global__ void angular_basis(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P)

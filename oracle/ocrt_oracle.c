@@ -829,6 +829,8 @@ typedef struct {
     double Re, H, mu_sun, depol;
     int L_aer;
     const double *betal;
+    int n_ang;                       /* value-kernel P11 table, 0: none */
+    const double *th_ang, *p11_ang;
     double *ts, *los_t, *los_tau;
     int *los_lay;
     int los_n;
@@ -914,8 +916,23 @@ static int ipss_init(ipss_t *G, int nt, const double *h, const double *xdel, con
 
 static double ipss_src(const ipss_t *G, int j) { return G->w_src ? G->w_src[j] : G->omega[j]; }
 
+/* Linear interpolation in scattering angle on an ascending P11 table,
+ * clamped to the end values (rt_solver.c:190-202, rt_ipss.c:134-146). */
+static double p11_table_at(int n, const double *th, const double *p11, double cth)
+{
+    if (cth > 1.0) cth = 1.0; else if (cth < -1.0) cth = -1.0;
+    const double theta = acos(cth) * 180.0 / PI;
+    if (theta <= th[0]) return p11[0];
+    if (theta >= th[n - 1]) return p11[n - 1];
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (th[mid] <= theta) lo = mid; else hi = mid; }
+    const double t = (theta - th[lo]) / (th[hi] - th[lo]);
+    return p11[lo] + t * (p11[hi] - p11[lo]);
+}
+
 static double ipss_phase_aer(const ipss_t *G, double ct)
 {
+    if (G->n_ang > 1) return p11_table_at(G->n_ang, G->th_ang, G->p11_ang, ct);
     if (G->L_aer >= 0 && G->betal) {
         double p0 = 1.0, p1 = ct, s = G->betal[0];
         if (G->L_aer >= 1) s += G->betal[1] * p1;
@@ -1114,6 +1131,8 @@ typedef struct {
     double mus, muv, tau_tot;
     double beta0, beta2, gamma2, alpha2;
     const double *gb_, *gg_, *ga_, *gz_;   /* greek beta, gamma, alpha, zeta */
+    int n_ang;                              /* value-kernel P11 table, 0: moment kernel */
+    const double *th_ang, *p11_ang;
     double *rm_s, *gb_s, *rm, *gb;         /* signed ring, offset pointers [-n..n] */
     double *h, *ch, *xd, *yd, *z;
     double *T;                              /* [n][nt] Beer-Lambert factors */
@@ -1157,6 +1176,29 @@ static void build_tables(const med_t *M, int m, ang_t *A)
             A->P[ix] = p; A->GR[ix] = gr; A->GT[ix] = gt;
             A->ARR[ix] = rr; A->ART[ix] = rt; A->ATT[ix] = tt;
         }
+}
+
+/* Value kernel of the aerosol I -> I table (rt_solver.c:204-240): the
+ * azimuth mean of the interpolated P11 table over 720 midpoints,
+ * P[j][k] = (1/720) sum_q P11(mu_j mu_k + s_j s_k cos phi_q) cos(m phi_q),
+ * rows j = 0..n (row 0 the solar direction -mu_s) and every column k. */
+static void value_table(const med_t *M, int m, ang_t *A)
+{
+    const int n = M->n, dirs = M->dirs, nphi = 720;
+    const double two_pi = 2.0 * PI;
+    for (int j = 0; j <= n; ++j) {
+        const double mj = M->rm[j], sj = sqrt(fmax(0.0, 1.0 - mj * mj));
+        for (int k = -n; k <= n; ++k) {
+            const double mk = M->rm[k], sk = sqrt(fmax(0.0, 1.0 - mk * mk));
+            double acc = 0.0;
+            for (int q = 0; q < nphi; ++q) {
+                const double phi = two_pi * ((double)q + 0.5) / (double)nphi;
+                const double cth = mj * mk + sj * sk * cos(phi);
+                acc += p11_table_at(M->n_ang, M->th_ang, M->p11_ang, cth) * cos((double)m * phi);
+            }
+            A->P[(size_t)j * dirs + (size_t)(k + n)] = acc / (double)nphi;
+        }
+    }
 }
 
 #define TB(tab, j, k) (tab)[(size_t)(j) * dirs + (size_t)((k) + n)]
@@ -1585,6 +1627,8 @@ static int solve_core(const ocrt_instance_t *in, solve_t *s)
     M->La = aer ? in->L_max : 0;
     M->m_max = in->m_max;
     if (aer) { const int q = in->L_max + 1; M->gb_ = in->greek; M->gg_ = in->greek + q; M->ga_ = in->greek + 2 * q; M->gz_ = in->greek + 3 * q; }
+    if (in->n_ang_phase > 0 && (in->n_ang_phase < 2 || !in->theta_phase || !in->p11_phase)) return -1;
+    if (aer && in->n_ang_phase >= 2) { M->n_ang = in->n_ang_phase; M->th_ang = in->theta_phase; M->p11_ang = in->p11_phase; }
     double mu[700], w[700];
     if (in->n_mu > 639) return -1;
     int vi;
@@ -1664,6 +1708,7 @@ static int solve_core(const ocrt_instance_t *in, solve_t *s)
     for (int m = 0; m <= mm; ++m) {
         oracle_basis(M->lmax, m, dirs, M->rm_s, A.plm, A.rrl, A.rtl);
         build_tables(M, m, &A);
+        if (M->n_ang > 1) value_table(M, m, &A);
         if (S.kind == 2) {
             memcpy(S.R, Rall + (size_t)m * n * n * 9, sizeof(double) * (size_t)n * n * 9);
             S.Rsun = Rsun + (size_t)m * n * 9;
@@ -1692,7 +1737,11 @@ static int solve_core(const ocrt_instance_t *in, solve_t *s)
 
 static int ipss_for(const ocrt_instance_t *in, const med_t *M, ipss_t *G)
 {
-    return ipss_init(G, M->nt, M->h, M->xd, M->yd, M->z, M->mus, 0.0279, M->aer, M->La, M->gb_);
+    const int rc = ipss_init(G, M->nt, M->h, M->xd, M->yd, M->z, M->mus, 0.0279, M->aer, M->La, M->gb_);
+    if (rc == 0 && G->w_src && M->n_ang > 1) {   /* the table has priority (rt_ipss.c:81-99, 134) */
+        G->n_ang = M->n_ang; G->th_ang = M->th_ang; G->p11_ang = M->p11_ang;
+    }
+    return rc;
 }
 
 int oracle_solve(const ocrt_instance_t *in, ocrt_result_t *out, double *toa_modes, int *orders,
@@ -2122,6 +2171,7 @@ int oracle_adjoint(const ocrt_instance_t *in, const double *gout, double *g_h, d
     for (int m = 0; m <= mm && rc == 0; ++m) {
         oracle_basis(M->lmax, m, dirs, M->rm_s, A->plm, A->rrl, A->rtl);
         build_tables(M, m, A);
+        if (M->n_ang > 1) value_table(M, m, A);
         const double W = (kind == 1) ? 2.0 * PI : ((m == 0) ? 2.0 * PI : PI);
         const double *Rm = kind == 2 ? Rall + (size_t)m * n * n * 9 : NULL;
         const double *dRm = kind == 2 ? dRall + (size_t)m * n * n * 9 : NULL;

@@ -69,6 +69,14 @@ static void fill_case(const ocrt_instance_t *in, rt_case_t *cs, rt_options_t *op
         aer->alphal_aer = in->greek + 2 * n;
         aer->zetal_aer = in->greek + 3 * n;
         aer->use_value_kernel = 0;
+        if (in->n_ang_phase >= 2 && in->theta_phase && in->p11_phase) {
+            /* value kernel: the solver reads only the P11 table (rt_solver.c:190-240,
+             * rt_ipss.c:81-146); P12 and P33 are borrowed and unused on this path */
+            aer->use_value_kernel = 1;
+            aer->n_ang_phase = in->n_ang_phase;
+            aer->theta_phase = in->theta_phase;
+            aer->P11_phase = in->p11_phase;
+        }
         cs->aerosol_on = 1;
     }
 }

@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import ocrt_amd
@@ -102,3 +103,17 @@ def test_aerosol_ssa_out_of_range_rejected():
     d = make_instance(tau_a=0.1, ssa_a=1.2, greek=S.hg_greek(0.7, 20))
     plan = C.c_void_p()
     assert lib.ocrt_plan_create(C.byref(d), 1, 0, 0, C.byref(plan)) == -1
+
+
+@pytest.mark.parametrize("table", [
+    (np.array([0.0, 90.0, 45.0]), np.ones(3)),          # not ascending
+    (np.array([0.0, 200.0]), np.ones(2)),               # past 180 degrees
+    (np.array([0.0, 180.0]), np.array([1.0, np.nan])),  # non-finite value
+])
+def test_bad_value_kernel_table_rejected(table):
+    from ocrt_amd import synthetic as S
+    lib = ocrt_amd.library()
+    d = make_instance(tau_a=0.1, greek=S.hg_greek(0.7, 20), phase_table=table)
+    plan = C.c_void_p()
+    assert lib.ocrt_plan_create(C.byref(d), 1, 0, 0, C.byref(plan)) == -1
+    assert not plan.value

@@ -75,7 +75,8 @@ def test_layers_reproduce_the_medium():
     assert (x0, y0) == (0.0, 1.0)
 
 
-@pytest.mark.parametrize("kw", [dict(bottom_source=(0, 0, 0)), dict(use_value_kernel=True),
+@pytest.mark.parametrize("kw", [dict(bottom_source=(0, 0, 0)),
+                                dict(use_value_kernel=True, aer_p11=[], mode="aerosol_coxmunk"),
                                 dict(n_phi_quad=512), dict(mode="flat")])
 def test_unserved_options_are_refused(kw):
     from ocrt_amd.atmos_batch import solve_atm_batch
@@ -85,12 +86,18 @@ def test_unserved_options_are_refused(kw):
 
 
 @pytest.mark.gpu
-def test_solve_atm_batch_equals_oracle(oracle):
+@pytest.mark.parametrize("value_kernel", [False, True])
+def test_solve_atm_batch_equals_oracle(oracle, value_kernel):
     from ocrt_amd import synthetic as S
     from ocrt_amd.abi import make_instance
     from ocrt_amd.atmos_batch import solve_atm_batch
     nt, n_gl, m_max = 40, 16, 6
     gk = S.hg_greek(0.72, 40)
+    # the value kernel's table: HG with a backscatter lobe on a 1-degree grid
+    th = np.arange(0.0, 180.5, 1.0)
+    cth = np.cos(np.radians(th))
+    p11 = (1 - 0.72 ** 2) / (1 + 0.72 ** 2 - 2 * 0.72 * cth) ** 1.5 + 0.5 * np.exp(-((180.0 - th) / 5.0) ** 2)
+    table = (th, p11) if value_kernel else None
     geo = [(35.0, 20.0, 60.0, 4.0, 0.9), (62.0, 48.0, 150.0, 9.0, 0.95), (15.0, 5.0, 10.0, 1.5, 0.85)]
     atms, vjs, mus, dphis, winds, descs = [], [], [], [], [], []
     for b, (sza, vza, raa, ws, ssa) in enumerate(geo):
@@ -105,11 +112,12 @@ def test_solve_atm_batch_equals_oracle(oracle):
         descs.append(make_instance(surface="black_fresnel_ocean", wind=ws, decouple_sunglint=1, sza=sza, vza=vza,
                                    raa=raa, tau_R=float(ray.sum()), tau_a=float(aer.sum()), ssa_a=ssa, greek=gk,
                                    n_mu=n_gl, m_max=m_max, layers=np.stack([ray, aer, np.full(nt, ssa),
-                                                                            np.zeros(nt)])))
+                                                                            np.zeros(nt)]), phase_table=table))
     n_mu = n_gl + 1
     print("[gpu] solve_atm_batch: 3 cases, 16 streams + view node, 40 layers, m 0..6", flush=True)
     out = solve_atm_batch(atms, vjs, mus, dphis, winds, n_mu, nt, m_max, 20, 1e-7,
-                          mode="aerosol_black_fresnel_ocean", L_max=40)
+                          mode="aerosol_black_fresnel_ocean", L_max=40, use_value_kernel=value_kernel,
+                          aer_p11=[table] * 3 if value_kernel else None)
     assert out["boa_I"].shape == (3, m_max + 1, n_mu)
     for b, d in enumerate(descs):
         rc, r, modes, orders = oracle_solve(oracle, d)
