@@ -48,7 +48,8 @@ enum {
     OCRT_ERR_NONFINITE = -2,
     OCRT_ERR_ALLOC = -3,
     OCRT_ERR_DEVICE = -4,
-    OCRT_ERR_FLAT = -5      /* ocrt_interface_fourier: T_aw of a flat surface */
+    OCRT_ERR_FLAT = -5,     /* ocrt_interface_fourier: T_aw of a flat surface */
+    OCRT_ERR_BOTTOM_SHAPE = -7  /* bottom source shape differs from the ring (rt_solver.h:11) */
 };
 
 /* Arithmetic forms of the solve.
@@ -110,7 +111,31 @@ typedef struct {
     int n_ang_phase;
     const double *theta_phase;
     const double *p11_phase;
+    /* Optional external upward source at the lower boundary: the coupled
+     * ocean's water-leaving Fourier field just above the surface (the
+     * ext_bottom_per_m_* options, code/OCRT_C/src/rt_types.h:722-747, applied
+     * by rt_surface_boundary.c:495-527).  Host [3][bottom_m_max + 1][bottom_n_mu]
+     * (I block, Q block, U block; mode m's values at the ring's upward nodes,
+     * ascending mu).  bottom_n_mu must equal the instance's ring size
+     * (ocrt_ring; OCRT_ERR_BOTTOM_SHAPE otherwise); modes above bottom_m_max
+     * carry no source.  The source, attenuated to every level, joins the order
+     * sum and the order-1 field that seeds the scattering.  It acts with the
+     * FLAT and BLACK_FRESNEL_OCEAN surfaces only, as in the reference (its
+     * atmosphere-only solver ignores it).  bottom_source_only = 1 drops the
+     * solar single scattering and the surface's direct-beam seeds, so only the
+     * bottom source drives the field.  NULL / 0: none.  Not with the adjoint. */
+    const double *bottom_source;
+    int bottom_n_mu, bottom_m_max;
+    int bottom_source_only;
 } ocrt_instance_t;
+
+/* Size and nodes of the ring a plan builds for n_mu Gauss nodes per hemisphere
+ * and a view zenith angle: the positive half of the 2 n_mu-point
+ * Gauss-Legendre rule with the view cosine inserted at its sorted position
+ * with weight zero (none within 1e-12 of it; rt_angle_grid.c:21-73).  mu and w
+ * (either may be NULL) receive the ascending cosines and weights, room for
+ * n_mu + 1.  Returns the ring size, or OCRT_ERR_ARGS. */
+int ocrt_ring(int n_mu, double vza_deg, double *mu, double *w);
 
 /* Gas absorption of an AFGL standard atmosphere with layered HITRAN cross
  * sections (rt_absorption.c, rt_atm.c:437-548): made once on the host from

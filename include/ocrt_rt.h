@@ -31,12 +31,15 @@
  *     use_value_kernel and a P11 angle table (n_ang_phase >= 2, theta_phase
  *     ascending in degrees) the aerosol I -> I phase table and the IPSS
  *     aerosol phase come from that table (rt_solver.c:190-240, 922-925;
- *     rt_ipss.c:81-146), as in the reference (P12/P33 tables unread there).
+ *     rt_ipss.c:81-146), as in the reference (P12/P33 tables unread there);
+ *   - the ocean-coupling bottom source: ext_bottom_per_m_I/Q/U with
+ *     ext_bottom_n_mu equal to the ring size and ext_bottom_m_max >= 0, and
+ *     bottom_source_only (rt_types.h:722-747; rt_solver.c:687-706, 949-977;
+ *     rt_surface_boundary.c:495-567); a shape mismatch returns -7.
  * Requests this build does not serve return -1 and print one line on stderr:
  * integration_method CONSTANT (a diagnostic of the reference), view_as_node
- * 0 (its diagnostic interpolation route), the ocean-coupling bottom source
- * (ext_bottom_*, bottom_source_only) and surface OCEAN.  Gas absorption is served: abs_state
- * is an rt_absorption_t (below).
+ * 0 (its diagnostic interpolation route) and surface OCEAN.  Gas absorption
+ * is served: abs_state is an rt_absorption_t (below).
  *
  * Every call solves one case in FP64 on the current HIP device, synchronously.
  * Calls from several host threads are safe; each builds its own plan.

@@ -44,6 +44,9 @@ class Instance(C.Structure):
         ("n_ang_phase", C.c_int),
         ("theta_phase", C.POINTER(C.c_double)),
         ("p11_phase", C.POINTER(C.c_double)),
+        ("bottom_source", C.POINTER(C.c_double)),
+        ("bottom_n_mu", C.c_int), ("bottom_m_max", C.c_int),
+        ("bottom_source_only", C.c_int),
     ]
 
 
@@ -72,15 +75,18 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
                   q_convention=1, decouple_sunglint=0, tau_R=0.0941, depol=0.0279,
                   tau_a=0.0, ssa_a=1.0, aer_h_km=2.0, greek=None, n_mu=24,
                   n_layers=50, m_max=2, max_orders=20, tolerance=1e-7, ipss=0, layers=None, gas=None,
-                  phase_table=None):
+                  phase_table=None, bottom_source=None, bottom_source_only=0):
     """Fill one descriptor.  `greek` is a float64 array [4, L+1] (beta, gamma,
     alpha, zeta) or None.  `layers` is None (column generator) or a float64
     array [4, n_layers] of per-layer Rayleigh tau, aerosol tau, aerosol SSA and
     absorption tau, top down; n_layers then follows its shape.  `gas` is an
     ocrt_gas handle (GasTable) or None.  `phase_table` is None (moment kernel)
     or a pair (theta_deg, P11) of equal-length arrays, ascending in theta: the
-    angle-space aerosol phase function of the value kernel.  Converted copies
-    are kept alive by the descriptor."""
+    angle-space aerosol phase function of the value kernel.  `bottom_source`
+    is None or an array [3, m_src + 1, n_ring] of the upward I, Q, U Fourier
+    field injected at the bottom (n_ring = ring_nodes(n_mu, vza)[0].size);
+    `bottom_source_only` drops the solar source.  Converted copies are kept
+    alive by the descriptor."""
     d = Instance()
     d.sza_deg, d.vza_deg, d.raa_deg, d.wavelength_nm = sza, vza, raa, wavelength
     d.surface = SURFACE_NAMES[surface] if isinstance(surface, str) else int(surface)
@@ -125,6 +131,19 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
         d.n_ang_phase = 0
         d.theta_phase = None
         d.p11_phase = None
+    if bottom_source is not None:
+        b = np.ascontiguousarray(bottom_source, dtype=np.float64)
+        if b.ndim != 3 or b.shape[0] != 3:
+            raise ValueError("bottom_source must have shape [3, m_src + 1, n_ring]")
+        d.bottom_source = b.ctypes.data_as(C.POINTER(C.c_double))
+        d.bottom_m_max = b.shape[1] - 1
+        d.bottom_n_mu = b.shape[2]
+        d._bottom_keepalive = b
+    else:
+        d.bottom_source = None
+        d.bottom_m_max = -1
+        d.bottom_n_mu = 0
+    d.bottom_source_only = int(bottom_source_only)
     return d
 
 

@@ -47,8 +47,15 @@ struct DevInst {
     double wl;           // wavelength (nm)
     long long off_ang;   // pool_greek: value-kernel P11 table, theta[n_ang] (deg) then P11[n_ang]
     int n_ang;           // entries of that table; 0: the moment kernel
-    int pad_ang;
+    int bsrc_mmax;       // external bottom source: highest mode carried, -1: none
+    long long off_bsrc;  // pool_greek: bottom source [3][bsrc_mmax + 1][n] (upward I, Q, U per mode)
+    int bsrc_only;       // 1: no solar single scattering or direct-beam surface seeds
+    int pad_bsrc;
 };
+
+// Whether the rough surface's direct-beam seed enters the order-1 field
+// (rt_surface_boundary.c:553-567): rough surface and a solar source.
+__host__ __device__ inline bool rough_seed_on(const DevInst &I) { return I.surf == 2 && !I.bsrc_only; }
 
 // One gas absorption table in the gas pool: AFGL levels z[n_lev] (km,
 // ascending) and number densities n[6][n_lev] (cm^-3); per gas a

@@ -372,7 +372,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     if (adjoint && precision != OCRT_PREC_FP64) return OCRT_ERR_ARGS;
     if (adjoint)
         for (int i = 0; i < n_inst; ++i)
-            if (in[i].ipss) return OCRT_ERR_ARGS;
+            if (in[i].ipss || in[i].bottom_source || in[i].bottom_source_only) return OCRT_ERR_ARGS;
     *out = nullptr;
     ocrt_plan *p = new ocrt_plan();
     p->n_inst = n_inst;
@@ -431,6 +431,15 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         I.n_ang = (I.aer && d.n_ang_phase >= 2) ? d.n_ang_phase : 0;
         I.off_ang = greek_n;
         greek_n += 2LL * I.n_ang;
+        // external bottom source: shaped on this ring (rt_solver.c:687-706)
+        I.bsrc_mmax = -1;
+        I.bsrc_only = d.bottom_source_only ? 1 : 0;
+        I.off_bsrc = greek_n;
+        if (d.bottom_source) {
+            if (d.bottom_n_mu != n || d.bottom_m_max < 0) { delete p; return OCRT_ERR_BOTTOM_SHAPE; }
+            I.bsrc_mmax = d.bottom_m_max;
+            greek_n += 3LL * (d.bottom_m_max + 1) * n;
+        }
         I.off_layers = layers_n;
         layers_n += I.layered ? 4LL * I.nt : 0;
         I.gidx = i;
@@ -759,6 +768,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             hgreek[h.d.off_ang + e] = in[ii].theta_phase[e];
             hgreek[h.d.off_ang + h.d.n_ang + e] = in[ii].p11_phase[e];
         }
+        for (long long e = 0; e < 3LL * (h.d.bsrc_mmax + 1) * n; ++e)
+            hgreek[h.d.off_bsrc + e] = in[ii].bottom_source[e];
         if (h.d.layered)
             for (long long e = 0; e < 4LL * h.d.nt; ++e) hlayers[h.d.off_layers + e] = in[ii].layers[e];
     }
@@ -1473,6 +1484,20 @@ extern "C" int ocrt_adjoint_batch(const ocrt_instance_t *in, int n, int precisio
     if (!rc && out) rc = ocrt_plan_results(p, out, nullptr, nullptr);
     ocrt_plan_destroy(p);
     return rc;
+}
+
+extern "C" int ocrt_ring(int n_mu, double vza_deg, double *mu, double *w)
+{
+    if (n_mu < 2 || n_mu > kMaxRing - 1 || !(vza_deg >= 0.0 && vza_deg < 90.0)) return OCRT_ERR_ARGS;
+    std::vector<double> rm, rw;
+    int vidx = 0;
+    const int n = build_ring(n_mu, cos(vza_deg * OCRT_PI / 180.0), rm, rw, vidx);
+    if (n < 2) return OCRT_ERR_ARGS;
+    for (int o = 0; o < n; ++o) {
+        if (mu) mu[o] = rm[o];
+        if (w) w[o] = rw[o];
+    }
+    return n;
 }
 
 extern "C" const char *ocrt_version(void) { return "ocrt_amd 0.1 gfx950"; }

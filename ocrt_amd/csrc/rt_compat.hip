@@ -79,6 +79,7 @@ using GasPtr = std::unique_ptr<ocrt_gas, GasDeleter>;
 struct Prep {
     ocrt_instance_t in{};
     std::vector<double> greek;
+    std::vector<double> bsrc;   // the bottom source, [3][m + 1][n]
     double tau_R = 0.0;
     bool zero_out = false;   // refused after the reference has cleared its output
 };
@@ -105,11 +106,11 @@ int prepare(const rt_case_t *cs, const rt_options_t *opts, const rt_aerosol_inpu
     if (opts->integration_method != RT_INTEGRATION_METHOD_LINEAR) { say("integration_method CONSTANT"); return OCRT_ERR_ARGS; }
     if (!opts->view_as_node) { say("view_as_node 0"); return OCRT_ERR_ARGS; }
     if (opts->abs_state && !gas) return OCRT_ERR_ARGS;
-    if (opts->ext_bottom_per_m_I || opts->ext_bottom_per_m_Q || opts->ext_bottom_per_m_U ||
-        opts->bottom_source_only || opts->sos.ext_bottom_I || opts->sos.bottom_source_only) {
-        say("the ocean-coupling bottom source");
-        return OCRT_ERR_ARGS;
-    }
+    // the ocean-coupling bottom source (rt_solver.c:687-706, 949-977); the
+    // slices in opts->sos are never read there (the driver overwrites them)
+    const bool ext_any = opts->ext_bottom_per_m_I || opts->ext_bottom_per_m_Q || opts->ext_bottom_per_m_U;
+    const bool ext_all = opts->ext_bottom_per_m_I && opts->ext_bottom_per_m_Q && opts->ext_bottom_per_m_U;
+    if (ext_any && !ext_all) { p.zero_out = true; return OCRT_ERR_BOTTOM_SHAPE; }
     if (opts->rayleigh_model != RT_RAYLEIGH_MODEL_BODHAINE_1999) return OCRT_ERR_ARGS;
     const bool aerosol = aer && aer->tau_a > 0.0;
     // value kernel: the reference reads only the P11 angle table on this path
@@ -153,6 +154,24 @@ int prepare(const rt_case_t *cs, const rt_options_t *opts, const rt_aerosol_inpu
     d.ipss = opts->pssa ? 1 : 0;
     d.layers = nullptr;
     d.gas = gas;
+    d.bottom_source_only = opts->bottom_source_only ? 1 : 0;
+    if (ext_all) {
+        // one [3][m + 1][n] block from the three per-component arrays
+        const int nb = opts->ext_bottom_n_mu, mb = opts->ext_bottom_m_max;
+        // the source must be shaped on the solve's ring before it is read
+        if (nb < 1 || mb < 0 || nb != ocrt_ring(d.n_mu, d.vza_deg, nullptr, nullptr)) {
+            p.zero_out = true;
+            return OCRT_ERR_BOTTOM_SHAPE;
+        }
+        const size_t blk = (size_t)(mb + 1) * nb;
+        p.bsrc.resize(3 * blk);
+        std::memcpy(&p.bsrc[0], opts->ext_bottom_per_m_I, blk * sizeof(double));
+        std::memcpy(&p.bsrc[blk], opts->ext_bottom_per_m_Q, blk * sizeof(double));
+        std::memcpy(&p.bsrc[2 * blk], opts->ext_bottom_per_m_U, blk * sizeof(double));
+        d.bottom_source = p.bsrc.data();
+        d.bottom_n_mu = nb;
+        d.bottom_m_max = mb;
+    }
     if (aerosol) {
         const int Lm = aer->L_max;
         p.greek.resize(4 * (size_t)(Lm + 1));

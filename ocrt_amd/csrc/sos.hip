@@ -143,18 +143,25 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     // The seed's path attenuation exp(-(tau - h[k]) / mu) from the surface up
     // to level k is carried as the running product of the layers' Beer-Lambert
     // factors, which the upward scan visits in order (k = nt .. 0).
-    const bool rough_seed = I.surf == 2 && hemi == 0;
+    // An external bottom source (the coupled ocean's water-leaving field,
+    // rt_surface_boundary.c:495-527) enters the same way: its value at the
+    // node, attenuated to level k.  bsrc_only drops the solar source.
+    const bool rough_seed = rough_seed_on(I) && hemi == 0;
+    const bool ext = I.surf != 0 && m <= I.bsrc_mmax && hemi == 0;
+    const bool solar = !I.bsrc_only;
     const double tau = lv.h[nt], down = exp(-tau / I.mus);
     const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
     const double rsun = rough_seed ? P.R[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c] : 0.0;
     const double seed0 = rsun * (FS * down);
+    const double ext0 = ext ? P.greek[I.off_bsrc + ((long long)c * (I.bsrc_mmax + 1) + m) * n + o] : 0.0;
     double att = 1.0;
     sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o,
-                     [&](int k) { return src.primary(lv, k) + src.flat_value(lv, I.mus, k); },
+                     [&](int k) { return solar ? src.primary(lv, k) + src.flat_value(lv, I.mus, k) : 0.0; },
                      [&](int k, double v) {
-                         if (rough_seed) {
+                         if (rough_seed || ext) {
                              if (k < nt) att *= T[(long long)k * n + o];
-                             v += seed0 * att;
+                             if (rough_seed) v += seed0 * att;
+                             if (ext) v += ext0 * att;
                          }
                          const long long ix = (long long)k * Pp + col;
                          tot[ix] = v;
@@ -686,13 +693,7 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
     const int n = it.n, nt = it.nt, Pp = it.P, m = it.m, v = I.vidx;
     const double *mu = P.ring + I.off_ring, *w = mu + n;
     const LevelView lv(P, I);
-    __shared__ double red[8];
-    double hs = 0.0;
-    for (int o = threadIdx.x; o < n; o += blockDim.x)
-        hs += mu[o] * w[o] * total_at(P, it, mixed, nt, col_dn(n, 0, o));
     // ordered sum over nodes for the hemispheric flux
-    red[0] = 0.0;
-    __syncthreads();
     if (threadIdx.x == 0) {
         double F = 0.0;
         for (int o = 0; o < n; ++o) F += mu[o] * w[o] * total_at(P, it, mixed, nt, col_dn(n, 0, o));
@@ -701,7 +702,7 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
         const double tau = lv.h[nt];
         for (int c = 0; c < 3; ++c) {
             double val = total_at(P, it, mixed, 0, col_up(n, c, v));
-            if (I.surf == 2) {
+            if (rough_seed_on(I)) {
                 const double down = exp(-tau / I.mus);
                 const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
                 const double rk = P.R[I.off_Rsun + ((long long)m * n + v) * 9 + 3 * c];
@@ -715,7 +716,6 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
         r.converged = s.converged;
         r.residual = s.residual;
         out[item] = r;
-        (void)hs;
     }
     if (P.nodes) {
         // every node of the TOA upward field (rough seed removed) and of the
@@ -727,7 +727,7 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
             double val;
             if (c < 3) {
                 val = total_at(P, it, mixed, 0, col_up(n, c, o));
-                if (I.surf == 2) {
+                if (rough_seed_on(I)) {
                     const double tau = lv.h[nt], down = exp(-tau / I.mus);
                     const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
                     val -= P.R[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c] *

@@ -1396,13 +1396,17 @@ static int sos_loop(const med_t *M, int m, const ang_t *A, const surf_t *S, cons
     double *cI = dU + cnt, *cQ = cI + cnt, *cU = cQ + cnt;
     double *bcI = cU + cnt, *bcQ = bcI + n, *bcU = bcQ + n;
     int rc = 0;
+    const int bso = in->bottom_source_only != 0;
     primary_source(M, m, A, sI, sQ, sU);
     if (sweep(M, sI, NULL, pI) || sweep(M, sQ, NULL, pQ) || sweep(M, sU, NULL, pU)) { rc = -2; goto done; }
+    if (bso) {   /* only the bottom source drives the field (rt_solver.c:949-954) */
+        memset(pI, 0, cnt * sizeof(double)); memset(pQ, 0, cnt * sizeof(double)); memset(pU, 0, cnt * sizeof(double));
+    }
     memcpy(tI, pI, cnt * sizeof(double));
     memcpy(tQ, pQ, cnt * sizeof(double));
     memcpy(tU, pU, cnt * sizeof(double));
-    const int rough = (S->kind == 2);
-    if (S->kind == 1) {
+    const int rough = (S->kind == 2) && !bso;
+    if (S->kind == 1 && !bso) {
         /* reflected direct beam scattered once (rt_surface_boundary.c:210-428) */
         const double mus = M->mus, nw = in->n_water, n2 = nw * nw;
         const double tn = sqrt(n2 - 1.0 + mus * mus);
@@ -1463,6 +1467,24 @@ static int sos_loop(const med_t *M, int m, const ang_t *A, const surf_t *S, cons
                 tI[ix] += vI; tQ[ix] += vQ; tU[ix] += vU;
                 o1I[ix] += vI; o1Q[ix] += vQ; o1U[ix] += vU;
                 dI[ix] = vI; dQ[ix] = vQ; dU[ix] = vU;
+            }
+        }
+    }
+    if (S->kind != 0 && in->bottom_source && m <= in->bottom_m_max) {
+        /* external upward source at the bottom, attenuated to every level
+         * (rt_surface_boundary.c:495-527) */
+        const int Mb = in->bottom_m_max + 1;
+        const double *bI = in->bottom_source + (size_t)m * n;
+        const double *bQ = bI + (size_t)Mb * n, *bU = bQ + (size_t)Mb * n;
+        const double tau = M->h[nt];
+        for (int k = 0; k <= nt; ++k) {
+            const double up = tau - M->h[k];
+            for (int o = 0; o < n; ++o) {
+                const double a = exp(-up / M->rm[o + 1]);
+                const double vI = bI[o] * a, vQ = bQ[o] * a, vU = bU[o] * a;
+                const size_t ix = FI(k, o + 1);
+                tI[ix] += vI; tQ[ix] += vQ; tU[ix] += vU;
+                o1I[ix] += vI; o1Q[ix] += vQ; o1U[ix] += vU;
             }
         }
     }
@@ -1628,6 +1650,7 @@ static int solve_core(const ocrt_instance_t *in, solve_t *s)
     M->m_max = in->m_max;
     if (aer) { const int q = in->L_max + 1; M->gb_ = in->greek; M->gg_ = in->greek + q; M->ga_ = in->greek + 2 * q; M->gz_ = in->greek + 3 * q; }
     if (in->n_ang_phase > 0 && (in->n_ang_phase < 2 || !in->theta_phase || !in->p11_phase)) return -1;
+    if (in->bottom_source && in->bottom_m_max < 0) return -7;
     if (aer && in->n_ang_phase >= 2) { M->n_ang = in->n_ang_phase; M->th_ang = in->theta_phase; M->p11_ang = in->p11_phase; }
     double mu[700], w[700];
     if (in->n_mu > 639) return -1;
@@ -1635,6 +1658,7 @@ static int solve_core(const ocrt_instance_t *in, solve_t *s)
     const int n = oracle_ring(in->n_mu, M->muv, mu, w, &vi);
     if (n < 2) return -1;
     M->n = n; M->nt = in->n_layers; M->L = M->nt + 1; M->dirs = 2 * n + 1; M->vidx = vi + 1;
+    if (in->bottom_source && in->bottom_n_mu != n) return -7;   /* rt_solver.c:698-706 */
     M->lmax = in->m_max > 2 ? in->m_max : 2;
     if (aer && in->L_max > M->lmax) M->lmax = in->L_max;
     const double d = 0.0279;   /* fixed in the reference driver (rt_solver.c:646) */
@@ -2113,6 +2137,7 @@ static void glint3(const ocrt_instance_t *in, const med_t *M, double ws, double 
 int oracle_adjoint(const ocrt_instance_t *in, const double *gout, double *g_h, double *g_x, double *g_y,
                    double *g_wind)
 {
+    if (in->bottom_source || in->bottom_source_only) return -1;   /* forward-only option */
     if (in->ipss) return -1;
     solve_t s;
     int rc = solve_core(in, &s);

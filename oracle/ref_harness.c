@@ -56,6 +56,17 @@ static void fill_case(const ocrt_instance_t *in, rt_case_t *cs, rt_options_t *op
     opts->pssa = in->ipss;
     opts->view_as_node = 1;
     opts->aer_h_km = in->aer_h_km;
+    if (in->bottom_source) {
+        /* the coupled ocean's water-leaving field as the pass-2 bottom source
+         * (rt_types.h:722-747) */
+        const size_t blk = (size_t)(in->bottom_m_max + 1) * (size_t)in->bottom_n_mu;
+        opts->ext_bottom_per_m_I = in->bottom_source;
+        opts->ext_bottom_per_m_Q = in->bottom_source + blk;
+        opts->ext_bottom_per_m_U = in->bottom_source + 2 * blk;
+        opts->ext_bottom_n_mu = in->bottom_n_mu;
+        opts->ext_bottom_m_max = in->bottom_m_max;
+    }
+    opts->bottom_source_only = in->bottom_source_only;
 
     memset(aer, 0, sizeof(*aer));
     *use_aer = (in->L_max > 0 && in->tau_a > 0.0 && in->greek != NULL);

@@ -117,3 +117,21 @@ def test_bad_value_kernel_table_rejected(table):
     plan = C.c_void_p()
     assert lib.ocrt_plan_create(C.byref(d), 1, 0, 0, C.byref(plan)) == -1
     assert not plan.value
+
+
+def test_bottom_source_shape_rejected():
+    from ocrt_amd import synthetic as S
+    lib = ocrt_amd.library()
+    mu = np.zeros(10)
+    n = lib.ocrt_ring(8, C.c_double(30.0), mu.ctypes.data_as(C.c_void_p), None)
+    assert n == 9 and np.all(np.diff(mu[:n]) > 0)
+    plan = C.c_void_p()
+    for shape in ((3, 2, n - 1), (3, 2, n + 1)):
+        d = make_instance(surface="flat", tau_a=0.1, greek=S.hg_greek(0.7, 20), n_mu=8, vza=30.0,
+                          bottom_source=np.zeros(shape), bottom_source_only=1)
+        assert lib.ocrt_plan_create(C.byref(d), 1, 0, 0, C.byref(plan)) == -7
+        assert not plan.value
+    # the adjoint does not take a bottom source
+    d = make_instance(surface="flat", tau_a=0.1, greek=S.hg_greek(0.7, 20), n_mu=8, vza=30.0,
+                      bottom_source=np.zeros((3, 2, n)))
+    assert lib.ocrt_plan_create_ex(C.byref(d), 1, 0, 2, C.byref(plan)) == -1
