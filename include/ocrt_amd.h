@@ -127,6 +127,34 @@ typedef struct {
     const double *bottom_source;
     int bottom_n_mu, bottom_m_max;
     int bottom_source_only;
+    /* In-water column of the coupled ocean (water_tau > 0 selects it): the
+     * water body as the reference builds it in atmosphere form
+     * (code/OCRT_C/src/rt_water_rt.c:2104-2236) -- n_layers layers uniform in
+     * optical depth over water_tau, every level carrying the pure-water
+     * scattering albedo water_omega_w in the molecular slot (depolarisation
+     * depol, 0.039 for water) and the particle albedo water_omega_p in the
+     * aerosol slot (Greek coefficients greek / L_max; value-kernel table
+     * allowed).  sza_deg and vza_deg are the in-water (refracted) sun and
+     * view angles; the solar single scattering is scaled by beam_scale (the
+     * transmitted direct beam, 1 = the reference's F_sun = pi normalisation).
+     * tau_R, tau_a, ssa_a, aer_h_km, layers, gas and ipss are not used
+     * (layers, gas and ipss must be NULL / 0); surface must be BLACK (the deep
+     * black bottom) and no bottom source is taken.
+     *   top_reflect  0: none; 1: the flat water-side Fresnel reflection of each
+     *                node (total internal reflection: identity), fed back every
+     *                order as the downward boundary condition at the top
+     *                (rt_solver.c:2290-2733); 2: the rough Cox-Munk water-side
+     *                reflection (wind_speed, sigma_type, n_water, q_convention;
+     *                128-azimuth Fourier kernels, rt_water_rt.c:5605), contracted
+     *                with 2 pi mu w (m = 0) or pi mu w (rt_solver.c:2734-2922).
+     *   top_source   optional host [3][top_m_max + 1][top_n_mu]: diffuse downward
+     *                I, Q, U Fourier field entering below the surface (the
+     *                transmitted skylight) at the ring nodes, attenuated to every
+     *                level into the order sum and the order-1 field;
+     *                top_n_mu must equal the ring size. */
+    double water_tau, water_omega_w, water_omega_p, beam_scale;
+    int top_reflect, top_n_mu, top_m_max, pad_water;
+    const double *top_source;
 } ocrt_instance_t;
 
 /* Size and nodes of the ring a plan builds for n_mu Gauss nodes per hemisphere

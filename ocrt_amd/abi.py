@@ -47,6 +47,10 @@ class Instance(C.Structure):
         ("bottom_source", C.POINTER(C.c_double)),
         ("bottom_n_mu", C.c_int), ("bottom_m_max", C.c_int),
         ("bottom_source_only", C.c_int),
+        ("water_tau", C.c_double), ("water_omega_w", C.c_double),
+        ("water_omega_p", C.c_double), ("beam_scale", C.c_double),
+        ("top_reflect", C.c_int), ("top_n_mu", C.c_int), ("top_m_max", C.c_int), ("pad_water", C.c_int),
+        ("top_source", C.POINTER(C.c_double)),
     ]
 
 
@@ -75,7 +79,8 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
                   q_convention=1, decouple_sunglint=0, tau_R=0.0941, depol=0.0279,
                   tau_a=0.0, ssa_a=1.0, aer_h_km=2.0, greek=None, n_mu=24,
                   n_layers=50, m_max=2, max_orders=20, tolerance=1e-7, ipss=0, layers=None, gas=None,
-                  phase_table=None, bottom_source=None, bottom_source_only=0):
+                  phase_table=None, bottom_source=None, bottom_source_only=0, water=None,
+                  top_reflect=0, top_source=None, beam_scale=1.0):
     """Fill one descriptor.  `greek` is a float64 array [4, L+1] (beta, gamma,
     alpha, zeta) or None.  `layers` is None (column generator) or a float64
     array [4, n_layers] of per-layer Rayleigh tau, aerosol tau, aerosol SSA and
@@ -85,8 +90,12 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
     angle-space aerosol phase function of the value kernel.  `bottom_source`
     is None or an array [3, m_src + 1, n_ring] of the upward I, Q, U Fourier
     field injected at the bottom (n_ring = ring_nodes(n_mu, vza)[0].size);
-    `bottom_source_only` drops the solar source.  Converted copies are kept
-    alive by the descriptor."""
+    `bottom_source_only` drops the solar source.  `water` = (tau, omega_w,
+    omega_p) selects an in-water column (sza/vza then in-water angles, depol
+    0.039 for water, surface "black"); `top_reflect` 1 flat or 2 rough
+    water-side reflection at the top; `top_source` [3, m_src + 1, n_ring] the
+    diffuse field entering at the top; `beam_scale` the direct beam's weight.
+    Converted copies are kept alive by the descriptor."""
     d = Instance()
     d.sza_deg, d.vza_deg, d.raa_deg, d.wavelength_nm = sza, vza, raa, wavelength
     d.surface = SURFACE_NAMES[surface] if isinstance(surface, str) else int(surface)
@@ -144,6 +153,24 @@ def make_instance(sza=40.0, vza=55.0, raa=90.0, wavelength=555.0,
         d.bottom_m_max = -1
         d.bottom_n_mu = 0
     d.bottom_source_only = int(bottom_source_only)
+    if water is not None:
+        d.water_tau, d.water_omega_w, d.water_omega_p = (float(v) for v in water)
+    else:
+        d.water_tau = d.water_omega_w = d.water_omega_p = 0.0
+    d.beam_scale = beam_scale
+    d.top_reflect = int(top_reflect)
+    if top_source is not None:
+        t = np.ascontiguousarray(top_source, dtype=np.float64)
+        if t.ndim != 3 or t.shape[0] != 3:
+            raise ValueError("top_source must have shape [3, m_src + 1, n_ring]")
+        d.top_source = t.ctypes.data_as(C.POINTER(C.c_double))
+        d.top_m_max = t.shape[1] - 1
+        d.top_n_mu = t.shape[2]
+        d._top_keepalive = t
+    else:
+        d.top_source = None
+        d.top_m_max = -1
+        d.top_n_mu = 0
     return d
 
 

@@ -50,7 +50,12 @@ struct DevInst {
     int bsrc_mmax;       // external bottom source: highest mode carried, -1: none
     long long off_bsrc;  // pool_greek: bottom source [3][bsrc_mmax + 1][n] (upward I, Q, U per mode)
     int bsrc_only;       // 1: no solar single scattering or direct-beam surface seeds
-    int pad_bsrc;
+    int water;           // 1: in-water column (uniform levels over wtau, fixed albedos)
+    double wtau, womega_w, womega_p, beam_scale;
+    int top_kind;        // water-side reflection at the top: 0 none, 1 flat Fresnel, 2 rough Cox-Munk
+    int tsrc_mmax;       // diffuse top source: highest mode carried, -1: none
+    long long off_tsrc;  // pool_greek: top source [3][tsrc_mmax + 1][n] (downward I, Q, U per mode)
+    long long off_rtop;  // pool_rtop (top_kind 2): [(mmax+1)][n out][n in][9] water-side Fourier kernels
 };
 
 // Whether the rough surface's direct-beam seed enters the order-1 field
@@ -82,7 +87,8 @@ struct DevItem {
     long long off_A;      // pool_A: [Mp][K]
     long long off_F;      // pool_F / pool_J: [L][P]
     long long off_tot;    // pool_tot: [L][P]
-    long long off_bc;     // pool_bc: [3][n] boundary condition, then [3][n] bottom downward field
+    long long off_bc;     // pool_bc: [3][n] bottom boundary condition, [3][n] bottom downward field,
+                          // [3][n] top boundary condition, [3][n] top upward field
     long long off_cmp;    // pool_comp: [2][P] compensation of the order sum at k = 0 and k = nt
     long long off_Flo;    // pool_Flo: [L][P] low half of the bf16 field pair (fsplit items)
     long long off_nodes;  // pool_nodes (LUT or node-export plans, global over the plan): [6][n]
@@ -121,6 +127,7 @@ struct Pools {
     void *J2;             // second output of a parts-split scattering integral (adjoint)
     void *Flo;            // bf16 low half of the field pair (mixed form, fsplit items)
     double *cm_w;         // cos/sin(m phi) table [(Mall+1)][2][kNPhi]
+    const double *rtop;   // water-side reflection kernels of in-water columns (DevInst::off_rtop)
     double *nodes;        // LUT / node-export plans: per item [6][n] TOA upward I, Q, U and BOA downward
                           // I, Q, U at every node
     const DevGas *gas_tab;    // gas absorption tables of the plan

@@ -110,6 +110,24 @@ __device__ inline void fresnel_R(double mu_i, double n_water, int qc, double *M)
 // rotations, including the exact pole limits (surface.c:514-646).
 // Sancer bistatic shadowing S = 1 / (1 + Lambda_i + Lambda_v) (surface.c:591-612);
 // it depends on the two zenith cosines only.
+// Fresnel reflection Mueller [3][3] of light inside the water meeting the
+// surface at cosine mu_w (n1 = n_water, n2 = 1), meridian basis; total
+// internal reflection gives the identity (shared/surface.c:385-429,
+// rt_air_water.c:70-74).
+__device__ inline void fresnel_R_water_side(double mu_w, double n_water, int qc, double *M)
+{
+    for (int e = 0; e < 9; ++e) M[e] = 0.0;
+    const double sin2 = fmax(0.0, 1.0 - mu_w * mu_w);
+    const double st2 = n_water * n_water * sin2;
+    if (st2 >= 1.0) { M[0] = M[4] = M[8] = 1.0; return; }
+    const double mt = sqrt(1.0 - st2);
+    const double rs = (n_water * mu_w - mt) / (n_water * mu_w + mt);
+    const double rp = (mu_w - n_water * mt) / (mu_w + n_water * mt);
+    const double Rs = rs * rs, Rp = rp * rp;
+    const double q = (qc == 1) ? 0.5 * (Rp - Rs) : 0.5 * (Rs - Rp);
+    M[0] = 0.5 * (Rs + Rp); M[1] = q; M[3] = q; M[4] = 0.5 * (Rs + Rp); M[8] = rs * rp;
+}
+
 __device__ inline double sancer_shadow(double mu_out, double mu_in_dn, double s2v)
 {
     const double sl = sqrt(s2v);

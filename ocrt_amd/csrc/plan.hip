@@ -226,6 +226,8 @@ struct ocrt_plan {
     // global device arrays
     double *ring = nullptr, *greek = nullptr, *cm_w = nullptr, *layers = nullptr;
     double *vk_w = nullptr;   // cos(m phi_q) of the value kernel, [360][mmax_all + 1]; null without one
+    double *rtop = nullptr;   // water-side Cox-Munk kernels of rough in-water columns (DevInst::off_rtop)
+    std::vector<int> rtop_owner;   // instances that compute a kernel set (the others share one)
     ItemState *state = nullptr;
     ItemOut *iout = nullptr;
     InstOut *ipss = nullptr;
@@ -315,6 +317,7 @@ static double layer_sum(const ocrt_instance_t &d, int row)
 // Aerosol scattering enters only with a phase matrix and a positive column.
 static bool has_aerosol(const ocrt_instance_t &d)
 {
+    if (d.water_tau > 0.0) return d.L_max > 0 && d.greek && d.water_omega_p > 0.0;
     if (d.layers) return d.L_max > 0 && d.greek && layer_sum(d, 1) > 0.0;
     return d.L_max > 0 && d.tau_a > 0.0 && d.greek;
 }
@@ -328,7 +331,22 @@ static int validate(const ocrt_instance_t &d)
     if (d.n_mu < 2 || d.n_mu > kMaxRing - 1 || d.n_layers < 2) return OCRT_ERR_ARGS;
     if (d.max_orders < 1 || !(d.tolerance > 0.0)) return OCRT_ERR_ARGS;
     const bool aer = has_aerosol(d);
-    if (d.layers) {
+    if (d.water_tau > 0.0) {
+        // in-water column: albedos in [0, 1], a black bottom, no layering, gas,
+        // IPSS or bottom source
+        if (!(std::isfinite(d.water_tau) && d.water_omega_w >= 0.0 && d.water_omega_p >= 0.0 &&
+              d.water_omega_w + d.water_omega_p <= 1.0 && d.beam_scale >= 0.0 && std::isfinite(d.beam_scale)))
+            return OCRT_ERR_ARGS;
+        if (d.layers || d.gas || d.ipss || d.bottom_source || d.bottom_source_only) return OCRT_ERR_ARGS;
+        if (d.surface != OCRT_SURFACE_BLACK) return OCRT_ERR_ARGS;
+        if (d.top_reflect < 0 || d.top_reflect > 2) return OCRT_ERR_ARGS;
+        if (d.top_reflect && !(d.n_water > 1.0)) return OCRT_ERR_ARGS;
+        // the rough kernels come from the all-mode interface launch (m <= 255)
+        if (d.top_reflect == 2 && (!(d.wind_speed > 0.0) || d.m_max > 255)) return OCRT_ERR_ARGS;
+        if (aer && d.L_max < 2) return OCRT_ERR_ARGS;
+    } else if (d.top_reflect || d.top_source) {
+        return OCRT_ERR_ARGS;   // top reflection and top source belong to in-water columns
+    } else if (d.layers) {
         // every layer needs a positive extinction and an albedo in [0, 1]
         const int nt = d.n_layers;
         const double *R = d.layers, *A = R + nt, *W = A + nt, *G = W + nt;
@@ -372,7 +390,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     if (adjoint && precision != OCRT_PREC_FP64) return OCRT_ERR_ARGS;
     if (adjoint)
         for (int i = 0; i < n_inst; ++i)
-            if (in[i].ipss || in[i].bottom_source || in[i].bottom_source_only) return OCRT_ERR_ARGS;
+            if (in[i].ipss || in[i].bottom_source || in[i].bottom_source_only || in[i].water_tau > 0.0)
+                return OCRT_ERR_ARGS;
     *out = nullptr;
     ocrt_plan *p = new ocrt_plan();
     p->n_inst = n_inst;
@@ -398,7 +417,14 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         I.muv = cos(d.vza_deg * OCRT_PI / 180.0);
         I.aer = has_aerosol(d) ? 1 : 0;
         I.layered = d.layers ? 1 : 0;
-        if (d.layers) {
+        I.water = d.water_tau > 0.0 ? 1 : 0;
+        I.wtau = d.water_tau; I.womega_w = d.water_omega_w; I.womega_p = d.water_omega_p;
+        I.beam_scale = d.beam_scale;
+        I.top_kind = I.water ? d.top_reflect : 0;
+        if (I.water) {
+            I.tauR = d.water_omega_w * d.water_tau;
+            I.taua = I.aer ? d.water_omega_p * d.water_tau : 0.0;
+        } else if (d.layers) {
             I.tauR = layer_sum(d, 0);
             I.taua = I.aer ? layer_sum(d, 1) : 0.0;
         } else {
@@ -440,6 +466,14 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             I.bsrc_mmax = d.bottom_m_max;
             greek_n += 3LL * (d.bottom_m_max + 1) * n;
         }
+        // diffuse top source of an in-water column, shaped on this ring
+        I.tsrc_mmax = -1;
+        I.off_tsrc = greek_n;
+        if (d.top_source) {
+            if (d.top_n_mu != n || d.top_m_max < 0) { delete p; return OCRT_ERR_BOTTOM_SHAPE; }
+            I.tsrc_mmax = d.top_m_max;
+            greek_n += 3LL * (d.top_m_max + 1) * n;
+        }
         I.off_layers = layers_n;
         layers_n += I.layered ? 4LL * I.nt : 0;
         I.gidx = i;
@@ -457,6 +491,31 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         h.T_n = (long long)I.nt * n;
         h.R_n = (I.surf == 2) ? (long long)(I.mmax + 1) * n * n * 9 + (long long)(I.mmax + 1) * n * 9 : 0;
         p->mmax_all = std::max(p->mmax_all, I.mmax);
+    }
+    // Water-side reflection kernels of rough in-water columns, shared by the
+    // instances with the same ring, modes and surface.
+    long long rtop_n = 0;
+    for (int i = 0; i < n_inst; ++i) {
+        HostInst &h = p->inst[i];
+        DevInst &I = h.d;
+        if (I.top_kind != 2) continue;
+        int owner = -1;
+        for (int j : p->rtop_owner) {
+            const HostInst &g = p->inst[j];
+            const DevInst &J = g.d;
+            if (J.n == I.n && J.mmax == I.mmax && J.wind == I.wind && J.sigma_type == I.sigma_type &&
+                J.n_water == I.n_water && J.q_conv == I.q_conv && g.mu == h.mu && g.w == h.w) {
+                owner = j;
+                break;
+            }
+        }
+        if (owner >= 0) {
+            I.off_rtop = p->inst[owner].d.off_rtop;
+        } else {
+            I.off_rtop = rtop_n;
+            rtop_n += (long long)(I.mmax + 1) * I.n * I.n * 9;
+            p->rtop_owner.push_back(i);
+        }
     }
     // Instances whose basis, phase tables and scattering operator are equal
     // bit for bit: same ring (nodes, weights, view node), sun, Fourier range,
@@ -617,7 +676,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                     class_items.emplace_back();
                 }
                 it.off_F = F; it.off_tot = F; F += (long long)L * Pp;
-                it.off_bc = bc; bc += 6LL * n;
+                it.off_bc = bc; bc += 12LL * n;
                 it.off_cmp = cmp; cmp += mixed ? 2LL * Pp : 0;
                 it.off_nodes = p->nodes_n;
                 p->nodes_n += 6LL * n;
@@ -701,6 +760,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     bool any_vk = false;
     for (const Chunk &c : p->chunks) any_vk |= !c.hvk_owner.empty();
     if (any_vk) rc |= dmalloc(p, (void **)&p->vk_w, (size_t)kVkHalf * (p->mmax_all + 1) * 8);
+    if (rtop_n > 0) rc |= dmalloc(p, (void **)&p->rtop, (size_t)rtop_n * 8);
     rc |= dmalloc(p, (void **)&p->state, (size_t)p->n_items * sizeof(ItemState));
     rc |= dmalloc(p, (void **)&p->iout, (size_t)p->n_items * sizeof(ItemOut));
     rc |= dmalloc(p, (void **)&p->ipss, (size_t)n_inst * sizeof(InstOut));
@@ -743,6 +803,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     P.greek = p->greek;
     P.layers = p->layers;
     P.cm_w = p->cm_w;
+    P.rtop = p->rtop;
     P.nodes = nullptr;
     P.gas_tab = p->gas_tab;
     P.gas_pool = p->gas_pool;
@@ -770,6 +831,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         }
         for (long long e = 0; e < 3LL * (h.d.bsrc_mmax + 1) * n; ++e)
             hgreek[h.d.off_bsrc + e] = in[ii].bottom_source[e];
+        for (long long e = 0; e < 3LL * (h.d.tsrc_mmax + 1) * n; ++e)
+            hgreek[h.d.off_tsrc + e] = in[ii].top_source[e];
         if (h.d.layered)
             for (long long e = 0; e < 4LL * h.d.nt; ++e) hlayers[h.d.off_layers + e] = in[ii].layers[e];
     }
@@ -778,6 +841,16 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         (layers_n > 0 && hipMemcpy(p->layers, hlayers.data(), layers_n * 8, hipMemcpyHostToDevice) != hipSuccess)) {
         ocrt_plan_destroy(p);
         return OCRT_ERR_DEVICE;
+    }
+    // water-side Cox-Munk kernels of the rough in-water columns, every mode in
+    // one launch per owner, on the ring nodes just uploaded (128 azimuths,
+    // rt_water_rt.c:5605)
+    for (int i : p->rtop_owner) {
+        const DevInst &I = p->inst[i].d;
+        const double *d_mu = p->ring + I.off_ring;
+        const int rcw = interface_fourier_device(OCRT_IFACE_R_WW, d_mu, I.n, d_mu, I.n, I.mmax, 128, I.wind,
+                                                 I.sigma_type, I.n_water, I.q_conv, p->rtop + I.off_rtop, 0);
+        if (rcw != OCRT_OK) { ocrt_plan_destroy(p); return rcw; }
     }
     // chunk descriptors: one device buffer, sub-allocated 256-byte aligned
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };

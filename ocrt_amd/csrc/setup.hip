@@ -30,6 +30,18 @@ __global__ void medium_levels(const DevInst *__restrict__ inst, Pools P)
     if (k > nt) return;
     const int L = nt + 1;
     double *h = P.lev + I.off_lev, *ch = h + L, *xd = ch + L, *yd = xd + L, *z = yd + L;
+    if (I.water) {
+        // in-water column (rt_water_rt.c:2156-2176): levels uniform in optical
+        // depth, fixed particle and pure-water albedos at every level; the
+        // solar source carries the transmitted beam's weight
+        const double hk = (double)k * I.wtau / (double)nt;
+        h[k] = hk;
+        ch[k] = 0.5 * I.beam_scale * exp(-hk / I.mus);
+        xd[k] = I.aer ? I.womega_p : 0.0;
+        yd[k] = I.womega_w;
+        z[k] = 1.0;
+        return;
+    }
     if (I.layered) {
         // running extinction sum from the top, in layer order, so every level
         // carries exactly the sequential prefix sum
@@ -86,7 +98,7 @@ __global__ void medium_fractions(const DevInst *__restrict__ inst, Pools P)
     const int L = nt + 1;
     const double *h = P.lev + I.off_lev, *z = h + 4 * L;
     double *xd = P.lev + I.off_lev + 2 * L, *yd = xd + L;
-    if (I.aer && !I.layered) {
+    if (I.aer && !I.layered && !I.water) {
         if (k == 0) {
             if (I.tauR == 0.0 && I.taua > 0.0) { xd[0] = I.ssa; yd[0] = 0.0; }
             else { xd[0] = 0.0; yd[0] = 1.0; }

@@ -146,20 +146,28 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     // An external bottom source (the coupled ocean's water-leaving field,
     // rt_surface_boundary.c:495-527) enters the same way: its value at the
     // node, attenuated to level k.  bsrc_only drops the solar source.
+    // The diffuse field entering an in-water column at the top (the
+    // transmitted skylight) enters the downward columns alike, attenuated
+    // from the top to level k.
     const bool rough_seed = rough_seed_on(I) && hemi == 0;
-    const bool ext = I.surf != 0 && m <= I.bsrc_mmax && hemi == 0;
+    const bool ext = (I.surf != 0 && m <= I.bsrc_mmax && hemi == 0) || (m <= I.tsrc_mmax && hemi == 1);
     const bool solar = !I.bsrc_only;
     const double tau = lv.h[nt], down = exp(-tau / I.mus);
     const double FS = ((m == 0) ? OCRT_PI : 0.5 * OCRT_PI) * I.mus;
     const double rsun = rough_seed ? P.R[I.off_Rsun + ((long long)m * n + o) * 9 + 3 * c] : 0.0;
     const double seed0 = rsun * (FS * down);
-    const double ext0 = ext ? P.greek[I.off_bsrc + ((long long)c * (I.bsrc_mmax + 1) + m) * n + o] : 0.0;
+    const bool top_up = I.top_kind != 0;
+    const double ext0 = !ext ? 0.0
+                        : hemi == 0 ? P.greek[I.off_bsrc + ((long long)c * (I.bsrc_mmax + 1) + m) * n + o]
+                                    : P.greek[I.off_tsrc + ((long long)c * (I.tsrc_mmax + 1) + m) * n + o];
     double att = 1.0;
     sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o,
                      [&](int k) { return solar ? src.primary(lv, k) + src.flat_value(lv, I.mus, k) : 0.0; },
                      [&](int k, double v) {
                          if (rough_seed || ext) {
-                             if (k < nt) att *= T[(long long)k * n + o];
+                             // the upward scan visits k = nt .. 0, the downward one 0 .. nt
+                             if (hemi == 0 && k < nt) att *= T[(long long)k * n + o];
+                             if (hemi == 1 && k > 0) att *= T[(long long)(k - 1) * n + o];
                              if (rough_seed) v += seed0 * att;
                              if (ext) v += ext0 * att;
                          }
@@ -170,6 +178,7 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
                          else
                              F[ix] = to_storage<FT>(v);
                          if (hemi == 1 && k == nt) P.bc[it.off_bc + 3 * n + c * n + o] = v;
+                         if (top_up && hemi == 0 && k == 0) P.bc[it.off_bc + 9 * n + c * n + o] = v;
                      });
 }
 
@@ -202,8 +211,9 @@ __global__ void order_status(const DevInst *__restrict__ inst, const DevItem *__
         act = s.active;
     }
     __syncthreads();
-    if (!act || I.surf == 0) return;
-    surface_bc_item(I, it, P);
+    if (!act) return;
+    if (I.surf != 0) surface_bc_item(I, it, P);
+    if (I.top_kind != 0) top_bc_item(I, it, P);
 }
 
 // ---------------------------------------------------------------------------
@@ -503,7 +513,7 @@ __device__ inline float mixed_step(float run, float s_in, float s_out, float om,
 // of the two output rows, and the column's two max-norms.
 struct MixedColumn {
     __hip_bfloat16 *F, *Flo;   // level stride P; Flo null unless the item keeps the pair
-    double *tot, *bottom;
+    double *tot, *bottom, *top;   // top: upward field at the top of an in-water column, else null
     float *cmp_top, *cmp_bot;
     long long Pp;
     int nt, down;
@@ -516,6 +526,7 @@ struct MixedColumn {
         Flo = it.fsplit ? reinterpret_cast<__hip_bfloat16 *>(P.Flo) + it.off_Flo + col : nullptr;
         tot = P.tot + it.off_tot + col;
         bottom = P.bc + it.off_bc + 3 * n + c * n + o;
+        top = nullptr;
         cmp_top = P.comp + it.off_cmp + col;
         cmp_bot = P.comp + it.off_cmp + it.P + col;
         Pp = it.P;
@@ -532,6 +543,7 @@ struct MixedColumn {
         store_field_pair(F, Flo, ix, v);
         mcur = fmax(mcur, fabs(v));
         if (down && k == nt) *bottom = v;
+        if (top && k == 0) *top = v;
         const double tsum = t_old + v;
         tot[ix] = tsum;
         mtot = fmax(mtot, fabs(tsum));
@@ -572,7 +584,12 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
         hemi = __builtin_amdgcn_readfirstlane(hemi);   // column tiles are direction-pure
         const JT *__restrict__ J = reinterpret_cast<const JT *>(P.J) + off_F + col;
         const double mu = P.ring[I.off_ring + o];
-        const double bc = (I.surf != 0 && hemi == 0) ? P.bc[off_bc + c * n + o] : 0.0;
+        // boundary values: the surface's at the bottom of the upward columns,
+        // the water-side reflection's at the top of an in-water column's
+        // downward ones
+        const double bc = hemi == 0 ? (I.surf != 0 ? P.bc[off_bc + c * n + o] : 0.0)
+                                    : (I.top_kind != 0 ? P.bc[off_bc + 6 * n + c * n + o] : 0.0);
+        double *top_up = (I.top_kind != 0 && hemi == 0) ? P.bc + off_bc + 9 * n + c * n + o : nullptr;
         // level order of the scan and the layer crossed when stepping to level k
         const int k_first = hemi ? 0 : nt;
         const int step = hemi ? 1 : -1;
@@ -581,10 +598,11 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
             const float *OM = P.Tf + I.off_T;
             const float *IDT = P.idt + I.off_lev;
             MixedColumn mc(P, it, col, c, o, hemi);
+            mc.top = top_up;
             const float smu = hemi ? -(float)mu : (float)mu;
-            float run = hemi ? 0.f : (float)bc;
+            float run = (float)bc;
             float s_in = (float)J[(long long)k_first * Pp];
-            mc.emit(k_first, hemi ? 0.0 : bc, mc.tot[(long long)k_first * Pp]);
+            mc.emit(k_first, bc, mc.tot[(long long)k_first * Pp]);
             for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * U) {
                 float sj[U], om[U], idt[U];
                 double to[U];
@@ -620,14 +638,15 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                 F[ix] = to_storage<FT>(v);
                 mcur = fmax(mcur, fabs(v));
                 if (hemi == 1 && k == nt) *bottom = v;
+                if (top_up && k == 0) *top_up = v;
                 if (!accumulate) return;
                 const double tsum = t_old + v;
                 tot[ix] = tsum;
                 mtot = fmax(mtot, fabs(tsum));
             };
-            emit(k_first, hemi ? 0.0 : bc, accumulate ? tot[(long long)k_first * Pp] : 0.0);
+            emit(k_first, bc, accumulate ? tot[(long long)k_first * Pp] : 0.0);
             const double *T = P.T + I.off_T;
-            double run = hemi ? 0.0 : bc;
+            double run = bc;
             double s_in = (double)J[(long long)k_first * Pp];
             const double smu = hemi ? -mu : mu;
             for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * U) {

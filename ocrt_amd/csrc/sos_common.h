@@ -106,6 +106,48 @@ struct ColumnSource {
     }
 };
 
+// Downward boundary condition at the top of an in-water column for the next
+// order, from the upward field there of the order just finished
+// (off_bc + 9n), threads over the reflected nodes:
+//   flat  : the node's own water-side Fresnel matrix (rt_solver.c:2594-2606);
+//   rough : sum_k C mu_k w_k K^m[j][k] up(k), C = 2 pi (m = 0) or pi
+//           (rt_solver.c:2797-2822), K the water-side Cox-Munk kernel.
+__device__ inline void top_bc_item(const DevInst &I, const DevItem &it, const Pools &P)
+{
+    const int n = it.n, m = it.m;
+    const double *up = P.bc + it.off_bc + 9 * n;   // [3][n] upward field at the top
+    double *tb = P.bc + it.off_bc + 6 * n;
+    const double *mu = P.ring + I.off_ring, *w = mu + n;
+    if (I.top_kind == 1) {
+        for (int o = threadIdx.x; o < n; o += blockDim.x) {
+            double M[9];
+            fresnel_R_water_side(mu[o], I.n_water, I.q_conv, M);
+            const double a = up[o], b = up[n + o], cc = up[2 * n + o];
+            tb[o] = M[0] * a + M[1] * b + M[2] * cc;
+            tb[n + o] = M[3] * a + M[4] * b + M[5] * cc;
+            tb[2 * n + o] = M[6] * a + M[7] * b + M[8] * cc;
+        }
+        return;
+    }
+    const double C = (m == 0) ? 2.0 * OCRT_PI : OCRT_PI;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        double aI = 0.0, aQ = 0.0, aU = 0.0;
+        const double *K = P.rtop + I.off_rtop + ((long long)m * n + j) * n * 9;
+        for (int k = 0; k < n; ++k) {
+            if (w[k] == 0.0) continue;
+            const double a = up[k], b = up[n + k], cc = up[2 * n + k];
+            const double wk = C * mu[k] * w[k];
+            const double *Kk = K + 9LL * k;
+            aI += wk * (Kk[0] * a + Kk[1] * b + Kk[2] * cc);
+            aQ += wk * (Kk[3] * a + Kk[4] * b + Kk[5] * cc);
+            aU += wk * (Kk[6] * a + Kk[7] * b + Kk[8] * cc);
+        }
+        tb[j] = aI;
+        tb[n + j] = aQ;
+        tb[2 * n + j] = aU;
+    }
+}
+
 // Lower boundary condition of order `it` for the upward columns from the
 // bottom downward field of the previous order (bc + 3n), threads over the
 // outgoing nodes (rt_surface_boundary.c:137-208):

@@ -193,6 +193,7 @@ static double layers_sum(const ocrt_instance_t *in, int row)
 
 static int has_aerosol(const ocrt_instance_t *in)
 {
+    if (in->water_tau > 0.0) return in->L_max > 0 && in->greek != NULL && in->water_omega_p > 0.0;
     if (in->layers) return in->L_max > 0 && in->greek != NULL && layers_sum(in, 1) > 0.0;
     return in->L_max > 0 && in->tau_a > 0.0 && in->greek != NULL;
 }
@@ -237,6 +238,19 @@ int oracle_medium(const ocrt_instance_t *in, double *h, double *ch, double *xdel
     const int nt = in->n_layers;
     const double mus = cos(in->sza_deg * PI / 180.0);
     const int aer = has_aerosol(in);
+    if (in->water_tau > 0.0) {
+        /* in-water column (rt_water_rt.c:2156-2176): uniform levels, fixed
+         * albedos; the solar source scaled by the transmitted beam's weight */
+        for (int k = 0; k <= nt; ++k) {
+            const double hk = (double)k * in->water_tau / (double)nt;
+            h[k] = hk;
+            ch[k] = 0.5 * in->beam_scale * exp(-hk / mus);
+            xdel[k] = aer ? in->water_omega_p : 0.0;
+            ydel[k] = in->water_omega_w;
+            z_km[k] = 1.0;
+        }
+        return 0;
+    }
     if (in->layers) return medium_layers(in, aer, mus, h, ch, xdel, ydel, z_km);
     const double tauR = in->tau_R > 0.0 ? in->tau_R : 0.0;
     const double taua = aer ? in->tau_a : 0.0;
@@ -1227,7 +1241,9 @@ static void primary_source(const med_t *M, int m, const ang_t *A, double *SI, do
 }
 
 /* Formal solution with a linear-in-tau source (rt_layer_transport.c:89-208). */
-static int sweep(const med_t *M, const double *S, const double *bc, double *F)
+/* Formal solution with boundary values: bc at the bottom of the upward
+ * columns, tb at the top of the downward ones (either may be NULL). */
+static int sweep_bc(const med_t *M, const double *S, const double *bc, const double *tb, double *F)
 {
     const int n = M->n, dirs = M->dirs, nt = M->nt;
     const double *tau = M->h;
@@ -1248,7 +1264,7 @@ static int sweep(const med_t *M, const double *S, const double *bc, double *F)
                 F[FI(k, j)] = run;
             }
         } else {
-            double run = 0.0;
+            double run = tb ? tb[-j - 1] : 0.0;
             F[FI(0, j)] = run;
             for (int k = 1; k <= nt; ++k) {
                 const double so = S[FI(k, j)], si = S[FI(k - 1, j)];
@@ -1261,6 +1277,11 @@ static int sweep(const med_t *M, const double *S, const double *bc, double *F)
         }
     }
     return 0;
+}
+
+static int sweep(const med_t *M, const double *S, const double *bc, double *F)
+{
+    return sweep_bc(M, S, bc, NULL, F);
 }
 
 /* Scattering integral (rt_sos_operator.c:206-424). */
@@ -1376,6 +1397,8 @@ typedef struct {
     double *R;         /* [n][n][9] for this m */
     double *Rsun;      /* [n][9] for this m (rough) */
     double W;          /* Fourier weight */
+    int top;           /* in-water column: water-side reflection at the top, 0 none, 1 flat, 2 rough */
+    const double *Rtop;/* [n][n][9] rough water-side kernel for this m */
 } surf_t;
 
 /* Surface-coupled order loop with first-order seeds (rt_solver.c:2923-3151,
@@ -1387,7 +1410,7 @@ static int sos_loop(const med_t *M, int m, const ang_t *A, const surf_t *S, cons
 {
     const int n = M->n, dirs = M->dirs, nt = M->nt;
     const size_t cnt = (size_t)(nt + 1) * dirs;
-    double *buf = (double *)calloc(cnt * 15 + 3 * (size_t)n, sizeof(double));
+    double *buf = (double *)calloc(cnt * 15 + 6 * (size_t)n, sizeof(double));
     if (!buf) return -3;
     double *sI = buf, *sQ = sI + cnt, *sU = sQ + cnt;
     double *pI = sU + cnt, *pQ = pI + cnt, *pU = pQ + cnt;
@@ -1488,6 +1511,21 @@ static int sos_loop(const med_t *M, int m, const ang_t *A, const surf_t *S, cons
             }
         }
     }
+    if (in->top_source && m <= in->top_m_max) {
+        /* diffuse field entering an in-water column at the top, attenuated
+         * down to every level */
+        const int Mt = in->top_m_max + 1;
+        const double *aI = in->top_source + (size_t)m * n;
+        const double *aQ = aI + (size_t)Mt * n, *aU = aQ + (size_t)Mt * n;
+        for (int k = 0; k <= nt; ++k)
+            for (int o = 0; o < n; ++o) {
+                const double a = exp(-M->h[k] / M->rm[o + 1]);
+                const double vI = aI[o] * a, vQ = aQ[o] * a, vU = aU[o] * a;
+                const size_t ix = FI(k, -(o + 1));
+                tI[ix] += vI; tQ[ix] += vQ; tU[ix] += vU;
+                o1I[ix] += vI; o1Q[ix] += vQ; o1U[ix] += vU;
+            }
+    }
     *orders = 1; *residual = 0.0; *converged = 1;
     if (in->max_orders == 1) goto finish;
     for (size_t i = 0; i < cnt; ++i) { pI[i] += o1I[i]; pQ[i] += o1Q[i]; pU[i] += o1U[i]; }
@@ -1522,7 +1560,39 @@ static int sos_loop(const med_t *M, int m, const ang_t *A, const surf_t *S, cons
             }
             bI = bcI; bQ = bcQ; bU = bcU;
         }
-        if (sweep(M, sI, bI, cI) || sweep(M, sQ, bQ, cQ) || sweep(M, sU, bU, cU)) { rc = -2; goto done; }
+        const double *tbI = NULL, *tbQ = NULL, *tbU = NULL;
+        if (S->top) {
+            /* water-side reflection of the previous order's upward field at the
+             * top (rt_solver.c:2594-2606 flat, 2797-2822 rough) */
+            double *tb = bcI + 3 * (size_t)n;
+            for (int j = 1; j <= n; ++j) {
+                double aI = 0, aQ = 0, aU = 0;
+                if (S->top == 1) {
+                    double F[9];
+                    fresnel_R_general(M->rm[j], in->n_water, 1.0, in->q_convention, F);
+                    const double a = pI[FI(0, j)], b = pQ[FI(0, j)], c = pU[FI(0, j)];
+                    aI = F[0] * a + F[1] * b + F[2] * c;
+                    aQ = F[3] * a + F[4] * b + F[5] * c;
+                    aU = F[6] * a + F[7] * b + F[8] * c;
+                } else {
+                    const double C = (m == 0) ? 2.0 * PI : PI;
+                    for (int k = 1; k <= n; ++k) {
+                        const double wk = C * M->rm[k] * M->gb[k];
+                        const double a = pI[FI(0, k)], b = pQ[FI(0, k)], c = pU[FI(0, k)];
+                        const double *K = S->Rtop + ((size_t)(j - 1) * n + (k - 1)) * 9;
+                        aI += wk * (K[0] * a + K[1] * b + K[2] * c);
+                        aQ += wk * (K[3] * a + K[4] * b + K[5] * c);
+                        aU += wk * (K[6] * a + K[7] * b + K[8] * c);
+                    }
+                }
+                tb[j - 1] = aI; tb[n + j - 1] = aQ; tb[2 * n + j - 1] = aU;
+            }
+            tbI = tb; tbQ = tb + n; tbU = tb + 2 * n;
+        }
+        if (sweep_bc(M, sI, bI, tbI, cI) || sweep_bc(M, sQ, bQ, tbQ, cQ) || sweep_bc(M, sU, bU, tbU, cU)) {
+            rc = -2;
+            goto done;
+        }
         for (int k = 0; k <= nt; ++k)
             for (int d = -n; d <= n; ++d) {
                 if (d == 0) continue;
@@ -1616,6 +1686,7 @@ typedef struct {
     double *modes;      /* [m_max+1][3] TOA at view node */
     double *boa_dn;     /* [m_max+1][n] I downward at BOA */
     double *toa_up;     /* [m_max+1][3][n] TOA upward at every node (LUT sampling) */
+    double *boa_dn3;    /* [m_max+1][3][n] BOA downward I, Q, U at every node */
     int *orders;
     double *resid;
     int max_orders, all_conv;
@@ -1624,7 +1695,7 @@ typedef struct {
 static void solve_free(solve_t *s)
 {
     free(s->M.rm_s); free(s->M.gb_s); free(s->M.h); free(s->M.ch); free(s->M.xd); free(s->M.yd);
-    free(s->M.z); free(s->M.T); free(s->modes); free(s->boa_dn); free(s->toa_up);
+    free(s->M.z); free(s->M.T); free(s->modes); free(s->boa_dn); free(s->toa_up); free(s->boa_dn3);
     free(s->orders); free(s->resid);
     memset(s, 0, sizeof(*s));
 }
@@ -1651,6 +1722,14 @@ static int solve_core(const ocrt_instance_t *in, solve_t *s)
     if (aer) { const int q = in->L_max + 1; M->gb_ = in->greek; M->gg_ = in->greek + q; M->ga_ = in->greek + 2 * q; M->gz_ = in->greek + 3 * q; }
     if (in->n_ang_phase > 0 && (in->n_ang_phase < 2 || !in->theta_phase || !in->p11_phase)) return -1;
     if (in->bottom_source && in->bottom_m_max < 0) return -7;
+    if (in->water_tau > 0.0) {
+        if (in->layers || in->ipss || in->bottom_source || in->surface != OCRT_SURFACE_BLACK) return -1;
+        if (in->water_omega_w < 0.0 || in->water_omega_p < 0.0 || in->water_omega_w + in->water_omega_p > 1.0) return -1;
+        if (in->top_reflect < 0 || in->top_reflect > 2 || (in->top_reflect && !(in->n_water > 1.0))) return -1;
+        if (in->top_reflect == 2 && !(in->wind_speed > 0.0)) return -1;
+    } else if (in->top_reflect || in->top_source) {
+        return -1;
+    }
     if (aer && in->n_ang_phase >= 2) { M->n_ang = in->n_ang_phase; M->th_ang = in->theta_phase; M->p11_ang = in->p11_phase; }
     double mu[700], w[700];
     if (in->n_mu > 639) return -1;
@@ -1659,9 +1738,12 @@ static int solve_core(const ocrt_instance_t *in, solve_t *s)
     if (n < 2) return -1;
     M->n = n; M->nt = in->n_layers; M->L = M->nt + 1; M->dirs = 2 * n + 1; M->vidx = vi + 1;
     if (in->bottom_source && in->bottom_n_mu != n) return -7;   /* rt_solver.c:698-706 */
+    if (in->top_source && (in->top_n_mu != n || in->top_m_max < 0)) return -7;
     M->lmax = in->m_max > 2 ? in->m_max : 2;
     if (aer && in->L_max > M->lmax) M->lmax = in->L_max;
-    const double d = 0.0279;   /* fixed in the reference driver (rt_solver.c:646) */
+    /* fixed in the reference's atmosphere driver (rt_solver.c:646); the water
+     * body's own (0.039 for water, rt_water_rt.c:2122) in an in-water column */
+    const double d = (in->water_tau > 0.0) ? in->depol : 0.0279;
     const double an = 2.0 * (1.0 - d) / (2.0 + d);
     M->beta0 = 1.0; M->beta2 = 0.5 * an; M->gamma2 = -an * sqrt(1.5); M->alpha2 = 3.0 * an;
     const int nt = M->nt, dirs = M->dirs;
@@ -1673,10 +1755,11 @@ static int solve_core(const ocrt_instance_t *in, solve_t *s)
     s->modes = calloc((size_t)3 * (mm + 1), sizeof(double));
     s->boa_dn = calloc((size_t)(mm + 1) * n, sizeof(double));
     s->toa_up = calloc((size_t)(mm + 1) * 3 * n, sizeof(double));
+    s->boa_dn3 = calloc((size_t)(mm + 1) * 3 * n, sizeof(double));
     s->orders = calloc((size_t)mm + 1, sizeof(int));
     s->resid = calloc((size_t)mm + 1, sizeof(double));
     if (!M->rm_s || !M->gb_s || !M->h || !M->ch || !M->xd || !M->yd || !M->z || !M->T || !s->modes ||
-        !s->boa_dn || !s->toa_up || !s->orders || !s->resid) return -3;
+        !s->boa_dn || !s->toa_up || !s->boa_dn3 || !s->orders || !s->resid) return -3;
     M->rm = M->rm_s + n; M->gb = M->gb_s + n;
     M->rm[0] = -M->mus; M->gb[0] = 0.0;
     for (int j = 1; j <= n; ++j) { M->rm[j] = mu[j - 1]; M->rm[-j] = -mu[j - 1]; M->gb[j] = M->gb[-j] = w[j - 1]; }
@@ -1727,9 +1810,19 @@ static int solve_core(const ocrt_instance_t *in, solve_t *s)
             S.W = 2.0 * PI;
         }
     }
+    double *Rtop = NULL;
+    S.top = in->water_tau > 0.0 ? in->top_reflect : 0;
+    if (S.top == 2) {
+        /* water-side Cox-Munk kernels of every mode, 128 azimuths (rt_water_rt.c:5605) */
+        Rtop = malloc(sizeof(double) * (size_t)(mm + 1) * n * n * 9);
+        if (!Rtop) { free(abuf); free(S.R); free(Rall); free(Rsun); return -3; }
+        oracle_interface_fourier(1, &M->rm[1], n, &M->rm[1], n, mm, 128, in->wind_speed, in->sigma_type,
+                                 in->n_water, in->q_convention, Rtop);
+    }
     int rc = 0;
     s->max_orders = 0; s->all_conv = 1;
     for (int m = 0; m <= mm; ++m) {
+        if (Rtop) S.Rtop = Rtop + (size_t)m * n * n * 9;
         oracle_basis(M->lmax, m, dirs, M->rm_s, A.plm, A.rrl, A.rtl);
         build_tables(M, m, &A);
         if (M->n_ang > 1) value_table(M, m, &A);
@@ -1753,9 +1846,12 @@ static int solve_core(const ocrt_instance_t *in, solve_t *s)
             s->toa_up[((size_t)m * 3 + 0) * n + jp - 1] = tI[FI(0, jp)];
             s->toa_up[((size_t)m * 3 + 1) * n + jp - 1] = tQ[FI(0, jp)];
             s->toa_up[((size_t)m * 3 + 2) * n + jp - 1] = tU[FI(0, jp)];
+            s->boa_dn3[((size_t)m * 3 + 0) * n + jp - 1] = tI[FI(nt, -jp)];
+            s->boa_dn3[((size_t)m * 3 + 1) * n + jp - 1] = tQ[FI(nt, -jp)];
+            s->boa_dn3[((size_t)m * 3 + 2) * n + jp - 1] = tU[FI(nt, -jp)];
         }
     }
-    free(abuf); free(S.R); free(Rall); free(Rsun);
+    free(abuf); free(S.R); free(Rall); free(Rsun); free(Rtop);
     return rc;
 }
 
@@ -1904,6 +2000,24 @@ int oracle_solve_lut(const ocrt_instance_t *in, int n_vza, const double *vza, in
     if (ip) ipss_free(&G);
     solve_free(&s);
     return 0;
+}
+
+/* Per-node Fourier fields of one solve: top_up [m_max+1][3][n] upward I, Q, U
+ * at the top (k = 0; rough direct seed not removed), bot_dn [m_max+1][3][n]
+ * downward at the bottom, orders [m_max+1]; n = ring size (returned). */
+int oracle_solve_nodes(const ocrt_instance_t *in, double *top_up, double *bot_dn, int *orders)
+{
+    solve_t s;
+    int rc = solve_core(in, &s);
+    if (rc == 0) {
+        const size_t cnt = (size_t)(in->m_max + 1) * 3 * s.M.n;
+        if (top_up) memcpy(top_up, s.toa_up, cnt * sizeof(double));
+        if (bot_dn) memcpy(bot_dn, s.boa_dn3, cnt * sizeof(double));
+        if (orders) memcpy(orders, s.orders, (size_t)(in->m_max + 1) * sizeof(int));
+        rc = s.M.n;
+    }
+    solve_free(&s);
+    return rc;
 }
 
 int oracle_solve_batch(const ocrt_instance_t *in, int n, ocrt_result_t *out, int n_threads)
@@ -2137,7 +2251,7 @@ static void glint3(const ocrt_instance_t *in, const med_t *M, double ws, double 
 int oracle_adjoint(const ocrt_instance_t *in, const double *gout, double *g_h, double *g_x, double *g_y,
                    double *g_wind)
 {
-    if (in->bottom_source || in->bottom_source_only) return -1;   /* forward-only option */
+    if (in->bottom_source || in->bottom_source_only || in->water_tau > 0.0) return -1;   /* forward-only options */
     if (in->ipss) return -1;
     solve_t s;
     int rc = solve_core(in, &s);

@@ -48,6 +48,9 @@ int oracle_adjoint(const ocrt_instance_t *in, const double *gout, double *g_h, d
 
 /* OpenMP batch, one instance per thread (rt_io.c:199). */
 int oracle_solve_batch(const ocrt_instance_t *in, int n, ocrt_result_t *out, int n_threads);
+/* Per-node Fourier fields (top upward, bottom downward, [m_max+1][3][n]) and
+ * order counts of one solve; returns the ring size n or a negative code. */
+int oracle_solve_nodes(const ocrt_instance_t *in, double *top_up, double *bot_dn, int *orders);
 
 /* ---- component entry points used by the unit tests ---- */
 int oracle_gauss_legendre_pos(int n, double *mu, double *w);

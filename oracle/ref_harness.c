@@ -24,6 +24,12 @@
 
 #include "rt_solver.h"
 #include "rt_types.h"
+#include "rt_atm.h"
+#include "rt_kernel.h"
+#include "rt_sos_operator.h"
+#include "rt_quadrature.h"
+#include "rt_air_water.h"
+#include "shared/surface.h"
 #include "ocrt_amd.h"
 
 static void fill_case(const ocrt_instance_t *in, rt_case_t *cs, rt_options_t *opts,
@@ -188,3 +194,131 @@ int ref_solve_batch(const ocrt_instance_t *in, int n, ocrt_result_t *out, int n_
 }
 
 int ref_max_threads(void) { return omp_get_max_threads(); }
+
+/* In-water SOS of the reference's water body, per Fourier mode, for an
+ * ocrt_instance_t in-water column (water_tau > 0, top_reflect 0/1/2, no top
+ * source).  The medium is filled as the reference fills its in-water
+ * atmosphere-form medium (rt_water_rt.c:2104-2236): uniform levels, fixed
+ * particle / pure-water albedos, water depolarisation in the molecular slot,
+ * GL ring of n_mu nodes with the refracted sun in the solar slot.  Per mode
+ * the angular basis, phase tables, operator and primary source are built as
+ * in the case driver (rt_solver.c:904-960) and the order loop is the
+ * reference's own rt_solver_sos_pol (top_reflect 0), rt_solver_sos_pol_intrefl
+ * (1, per-node rt_air_water_R_ww) or rt_solver_sos_pol_intrefl_rough (2,
+ * surface_R_ww_coxmunk_fourier_kernel with 128 azimuths), as
+ * rt_water_rt.c:5966-6034 calls them.  Outputs (optional):
+ *   top_up  [m_max+1][3][n_mu]  upward I, Q, U at the top (k = 0)
+ *   bot_dn  [m_max+1][3][n_mu]  downward I, Q, U at the bottom (k = nt)
+ *   orders  [m_max+1]
+ * Returns 0, or a negative code. */
+int ref_water_solve(const ocrt_instance_t *in, double *top_up, double *bot_dn, int *orders)
+{
+    if (!in || !(in->water_tau > 0.0) || in->top_source) return -1;
+    const int nt = in->n_layers, n = in->n_mu, mm = in->m_max;
+    const int aer = in->L_max > 0 && in->greek && in->water_omega_p > 0.0;
+    const int L = aer ? in->L_max : 0;
+    int l_max = mm > 2 ? mm : 2;
+    if (L > l_max) l_max = L;
+    const double mus = cos(in->sza_deg * 3.14159265358979323846 / 180.0);
+    rt_atm_t atm;
+    memset(&atm, 0, sizeof(atm));
+    rt_legendre_workspace_t ws;
+    memset(&ws, 0, sizeof(ws));
+    if (rt_atm_alloc(&atm, nt, n) != 0) return -3;
+    const double d = in->depol;
+    const double ron = 2.0 * (1.0 - d) / (2.0 + d);
+    atm.tau_total = in->water_tau;
+    atm.depol = d;
+    atm.ssa = in->water_omega_w + in->water_omega_p;
+    atm.mu_sun = mus;
+    atm.rayleigh_model = RT_RAYLEIGH_MODEL_BODHAINE_1999;
+    atm.beta0 = 1.0;
+    atm.beta2 = 0.5 * ron;
+    atm.gamma2 = -ron * sqrt(1.5);
+    atm.alpha2 = 3.0 * ron;
+    for (int k = 0; k <= nt; ++k) {
+        const double hk = (double)k * in->water_tau / (double)nt;
+        atm.h[k] = hk;
+        atm.ch[k] = 0.5 * in->beam_scale * exp(-hk / mus);
+        atm.xdel[k] = aer ? in->water_omega_p : 0.0;
+        atm.ydel[k] = in->water_omega_w;
+        if (atm.z_km_level) atm.z_km_level[k] = 1.0;
+        if (k < nt && atm.tau_abs_layer) atm.tau_abs_layer[k] = 0.0;
+    }
+    double muq[640], wq[640];
+    if (n > 639 || rt_quadrature_gauss_legendre_pos(n, muq, wq) != 0) { rt_atm_free(&atm); return -1; }
+    atm.rm[0] = -mus;
+    atm.gb[0] = 0.0;
+    for (int j = 1; j <= n; ++j) {
+        atm.rm[j] = muq[j - 1]; atm.rm[-j] = -muq[j - 1];
+        atm.gb[j] = atm.gb[-j] = wq[j - 1];
+    }
+    if (aer) {
+        const size_t q = (size_t)L + 1;
+        atm.betal_aer = calloc(q, sizeof(double));
+        atm.gammal_aer = calloc(q, sizeof(double));
+        atm.alphal_aer = calloc(q, sizeof(double));
+        atm.zetal_aer = calloc(q, sizeof(double));
+        if (!atm.betal_aer || !atm.gammal_aer || !atm.alphal_aer || !atm.zetal_aer) { rt_atm_free(&atm); return -3; }
+        memcpy(atm.betal_aer, in->greek, q * sizeof(double));
+        memcpy(atm.gammal_aer, in->greek + q, q * sizeof(double));
+        memcpy(atm.alphal_aer, in->greek + 2 * q, q * sizeof(double));
+        memcpy(atm.zetal_aer, in->greek + 3 * q, q * sizeof(double));
+        atm.aerosol_active = 1;
+        atm.L_max = L;
+    }
+    if (rt_legendre_workspace_alloc(&ws, n, l_max) != 0) { rt_atm_free(&atm); return -3; }
+    const int dirs = 2 * n + 1;
+    const size_t cnt = (size_t)(nt + 1) * dirs;
+    double *buf = calloc(9 * cnt + (size_t)n * n * 9, sizeof(double));
+    if (!buf) { rt_legendre_workspace_free(&ws); rt_atm_free(&atm); return -3; }
+    double *si = buf, *sq = si + cnt, *su = sq + cnt, *pi_ = su + cnt, *pq = pi_ + cnt, *pu = pq + cnt;
+    double *ti = pu + cnt, *tq = ti + cnt, *tu = tq + cnt, *R = tu + cnt;
+    rt_solver_sos_options_t so = rt_solver_sos_options_default();
+    so.max_iterations = in->max_orders;
+    so.tolerance = in->tolerance;
+    int rc = 0;
+    for (int m = 0; m <= mm && rc == 0; ++m) {
+        if (rt_legendre_compute_pol(&ws, &atm, m) != 0 ||
+            rt_kernel_phase_fourier_pol(&ws, m, aer ? atm.gammal_aer : NULL) != 0 ||
+            rt_kernel_phase_fourier_aerosol_full(&ws, m, aer ? atm.alphal_aer : NULL, aer ? atm.zetal_aer : NULL) != 0 ||
+            rt_kernel_phase_fourier(&ws, m, aer ? atm.betal_aer : NULL) != 0 ||
+            rt_sos_operator_prepare(&atm, m, &ws) != 0 ||
+            rt_solver_primary_source(&atm, m, &ws, si) != 0 ||
+            rt_solver_primary_source_pol(&atm, m, &ws, sq, su) != 0) { rc = -1; break; }
+        if (rt_solver_integrate(&atm, m, si, RT_INTEGRATION_METHOD_LINEAR, pi_) != 0 ||
+            rt_solver_integrate(&atm, m, sq, RT_INTEGRATION_METHOD_LINEAR, pq) != 0 ||
+            rt_solver_integrate(&atm, m, su, RT_INTEGRATION_METHOD_LINEAR, pu) != 0) { rc = -2; break; }
+        rt_solver_sos_result_t res;
+        memset(&res, 0, sizeof(res));
+        if (in->top_reflect == 1) {
+            for (int j = 1; j <= n; ++j) rt_air_water_R_ww(atm.rm[j], in->n_water, in->q_convention, R + (size_t)(j - 1) * 9);
+            rc = rt_solver_sos_pol_intrefl(&atm, m, &ws, pi_, pq, pu, R, &so, ti, tq, tu, &res);
+        } else if (in->top_reflect == 2) {
+            rc = surface_R_ww_coxmunk_fourier_kernel(&atm.rm[1], n, &atm.rm[1], n, m, 128, in->wind_speed,
+                                                     in->sigma_type, in->n_water, in->q_convention, R);
+            if (rc == 0) rc = rt_solver_sos_pol_intrefl_rough(&atm, m, &ws, pi_, pq, pu, R, &so, ti, tq, tu, &res);
+        } else {
+            rc = rt_solver_sos_pol(&atm, m, &ws, pi_, pq, pu, &so, ti, tq, tu, &res);
+        }
+        if (rc) break;
+        if (orders) orders[m] = res.n_orders_used;
+        for (int j = 1; j <= n; ++j) {
+            const size_t up = (size_t)(j + n), dn = (size_t)nt * dirs + (size_t)(-j + n);
+            if (top_up) {
+                top_up[((size_t)m * 3 + 0) * n + j - 1] = ti[up];
+                top_up[((size_t)m * 3 + 1) * n + j - 1] = tq[up];
+                top_up[((size_t)m * 3 + 2) * n + j - 1] = tu[up];
+            }
+            if (bot_dn) {
+                bot_dn[((size_t)m * 3 + 0) * n + j - 1] = ti[dn];
+                bot_dn[((size_t)m * 3 + 1) * n + j - 1] = tq[dn];
+                bot_dn[((size_t)m * 3 + 2) * n + j - 1] = tu[dn];
+            }
+        }
+    }
+    free(buf);
+    rt_legendre_workspace_free(&ws);
+    rt_atm_free(&atm);
+    return rc;
+}
