@@ -142,3 +142,79 @@ def config2(n_geom=1024, **kw):
 def config3(n_inst=256, seed=3, **kw):
     return batch_config(n_geom=n_inst, bands=(555.0,), seed=seed, fixed_layers=200,
                         fixed_streams=32, **kw)
+
+
+# ---------------------------------------------------------------------------
+# Coupled atmosphere-ocean workload: 100 wavelengths over 330-1100 nm, water
+# bodies with chlorophyll-driven optical properties and a strongly
+# forward-peaked 1000-term hydrosol.
+
+def pure_water_absorption(wl):
+    """Smooth synthetic pure-water absorption (m^-1): low in the blue, rising
+    through the red to strong near-infrared absorption."""
+    wl = np.asarray(wl, dtype=np.float64)
+    return (0.005 + 0.05 * np.exp((wl - 550.0) / 80.0) + 2.0 / (1.0 + np.exp(-(wl - 740.0) / 25.0))
+            + 20.0 / (1.0 + np.exp(-(wl - 1000.0) / 40.0)))
+
+
+def pure_water_scattering(wl):
+    """Pure-water scattering coefficient (m^-1), lambda^-4.32 law."""
+    return 0.0038 * (400.0 / np.asarray(wl, dtype=np.float64)) ** 4.32
+
+
+def phyto_shape(wl):
+    """Smooth synthetic phytoplankton absorption shape: blue and red peaks."""
+    wl = np.asarray(wl, dtype=np.float64)
+    return np.exp(-((wl - 440.0) / 40.0) ** 2) + 0.5 * np.exp(-((wl - 675.0) / 15.0) ** 2) + 0.1
+
+
+def water_iops(wl, chl):
+    """(a, b_w, b_p) in m^-1: absorption a = a_w + 0.06 Chl^0.65 shape(wl),
+    pure-water scattering b_w, particle scattering b_p = 0.3 Chl^0.62 (550/wl)."""
+    a = pure_water_absorption(wl) + 0.06 * chl ** 0.65 * phyto_shape(wl)
+    return a, pure_water_scattering(wl), 0.3 * chl ** 0.62 * (550.0 / wl)
+
+
+def hydrosol_greek(L=1000, g1=0.95, g2=-0.3, w=0.98, dolp=0.3):
+    """Greek set of a two-term HG hydrosol (weight w on g1), with Rayleigh-like
+    polarisation of amplitude dolp (P12 = -dolp (1 - mu^2) P11)."""
+    return np.ascontiguousarray(w * hg_greek(g1, L, dolp=dolp) + (1.0 - w) * hg_greek(g2, L, dolp=dolp))
+
+
+def config4(n_inst=512, seed=4, n_mu=64, n_layers=300, water_layers=100, m_max=127, depth_m=200.0,
+            tau_cap=30.0, L_hyd=1000, sza=40.0, vza=30.0, raa=90.0, wind=5.0, chl=None, aod_scale=1.0):
+    """Coupled atmosphere-ocean instances: wavelength i mod 100 on a uniform
+    330-1100 nm grid, Chl ~ logU(0.01, 10) mg m^-3 per instance (or the
+    given array), atmosphere as the batched workload (Rayleigh tau lambda^-4.08,
+    AOD(865) ~ logU(0.01, 0.5) with an Angstrom law, HG g 0.7 aerosol with 80
+    Greek terms, SSA ~ U(0.85, 1)) over a rough sea (wind 5 m/s, n 1.34) in
+    300 layers; water 200 m deep in 100 layers (optical depth capped at
+    tau_cap: the black bottom lies below), 64 streams both sides, modes 0..127.
+    Returns (atmosphere descriptors, Water list, per-instance dict)."""
+    from .ocean import Water
+    rng = np.random.default_rng(seed)
+    wls = np.linspace(330.0, 1100.0, 100)
+    hyd = hydrosol_greek(L_hyd)
+    gk = hg_greek(0.70, 80)
+    descs, waters = [], []
+    meta = {"wavelength": [], "chl": [], "aod865": [], "angstrom": []}
+    for i in range(n_inst):
+        wl = float(wls[i % 100])
+        c_chl = float(chl[i]) if chl is not None else math.exp(rng.uniform(math.log(0.01), math.log(10.0)))
+        aod865 = math.exp(rng.uniform(math.log(0.01), math.log(0.5)))
+        ang = rng.uniform(0.0, 2.0)
+        ssa = rng.uniform(0.85, 1.0)
+        tau_r = RAYLEIGH_TAU_555 * (wl / 555.0) ** -4.08
+        aod = aod_scale * aod865 * (wl / 865.0) ** -ang
+        descs.append(make_instance(sza=sza, vza=vza, raa=raa, wavelength=wl, surface="black_fresnel_ocean",
+                                   wind=wind, tau_R=tau_r, tau_a=aod, ssa_a=ssa, aer_h_km=2.0, greek=gk,
+                                   n_mu=n_mu, n_layers=n_layers, m_max=m_max))
+        a, bw, bp = water_iops(wl, c_chl)
+        c = a + bw + bp
+        waters.append(Water(min(c * depth_m, tau_cap), bw / c, bp / c, greek=hyd, n_layers=water_layers,
+                            n_mu=n_mu))
+        meta["wavelength"].append(wl)
+        meta["chl"].append(c_chl)
+        meta["aod865"].append(aod865)
+        meta["angstrom"].append(ang)
+    return descs, waters, {k: np.array(v) for k, v in meta.items()}
