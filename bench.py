@@ -152,6 +152,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-ocean", action="store_true")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time only the mixed form (for counter-collection runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -268,6 +269,43 @@ def main():
                    "jacobian_rows_per_s": len(d3) / t_jac}
         log(f"adjoint: {adjoint}")
 
+    ocean_leg = None
+    if not args.no_ocean:
+        from ocrt_amd import ocean
+        log("coupled atmosphere-ocean workload: 512 instances, 100 wavelengths, 64 streams, modes 0..127")
+        d4, w4, meta4 = S.config4()
+        ocean.coupled_solve(d4[:8], w4[:8], precision="mixed")   # warm-up
+        sync()
+        st_m, st_f = {}, {}
+        t0 = time.perf_counter()
+        om = ocean.coupled_solve(d4, w4, precision="mixed", stage_times=st_m)
+        sync()
+        t_m = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        of = ocean.coupled_solve(d4, w4, precision="fp64", stage_times=st_f)
+        sync()
+        t_f = time.perf_counter() - t0
+        ei4 = np.abs(om["rho_I"] - of["rho_I"]) / np.abs(of["rho_I"])
+        eq4 = np.maximum(np.abs(om["rho_Q"] - of["rho_Q"]), np.abs(om["rho_U"] - of["rho_U"]))
+        nj = 64
+
+        def make(c, a):
+            d, w, _ = S.config4(n_inst=nj, chl=c, aod_scale=a)
+            return d, w
+        t0 = time.perf_counter()
+        ocean.coupled_jacobian(make, meta4["chl"][:nj], rel=1e-3)
+        sync()
+        t_j = time.perf_counter() - t0
+        ocean_leg = {"instances": len(d4), "mixed_s": t_m, "fp64_s": t_f,
+                     "mixed_solves_per_s": len(d4) / t_m, "fp64_solves_per_s": len(d4) / t_f,
+                     "stages_mixed_s": st_m, "stages_fp64_s": st_f,
+                     "max_rel_err_I_mixed_vs_fp64": float(ei4.max()), "max_abs_err_QU_mixed_vs_fp64": float(eq4.max()),
+                     "water_orders_max": int(of["orders_water"].max()),
+                     "jacobian_instances": nj, "jacobian_fp64_s": t_j,
+                     "jacobian_note": "d(rho_I,Q,U)/d ln Chl and d ln AOD by central differences (5 coupled solves "
+                                      "per instance in one batch, fp64)"}
+        log(f"coupled ocean: {ocean_leg}")
+
     single = None
     if not args.no_single:
         log("single instance: Rayleigh + aerosol, 400 layers, 48 streams, modes 0..95, IPSS")
@@ -344,6 +382,7 @@ def main():
         "adjoint_fwd_ratio": adjoint["jacobian_ckpt4_over_mixed_forward"] if adjoint else None,
         "adjoint": adjoint,
         "single_instance_config1": single,
+        "coupled_ocean": ocean_leg,
         "roofline": roofline,
         "roofline_other": rl_sweep if roofline is rl_gemm else rl_gemm,
         "cpu_baseline": cpu,

@@ -56,6 +56,7 @@ struct DevInst {
     int tsrc_mmax;       // diffuse top source: highest mode carried, -1: none
     long long off_tsrc;  // pool_greek: top source [3][tsrc_mmax + 1][n] (downward I, Q, U per mode)
     long long off_rtop;  // pool_rtop (top_kind 2): [(mmax+1)][n out][n in][9] water-side Fourier kernels
+    float jscale, jinv;  // mixed form: the scattering source is kept in fp16 as J * jscale (a power of two)
 };
 
 // Whether the rough surface's direct-beam seed enters the order-1 field
@@ -121,6 +122,9 @@ struct Pools {
     double *grad;         // adjoint: per-instance accumulators (see DevInst::off_grad)
     const double *gout;  // adjoint: upstream gradient [instance][3] of (rho_I, rho_Q, rho_U)
     float *comp;          // fp32 compensation of the mixed-form order sum on the two output rows
+    double *trow;         // mixed form: fp64 order sum of the two output rows (k = 0, k = nt), [2][P] per item
+                          // at off_cmp; the interior rows keep a compensated fp32 + fp16 pair in pool_tot
+    float *Jf;            // mixed form: fp32 source of the items that keep their field as a pair, [L][P] at off_Flo
     float *Tf;            // fp32 1 - T for the mixed-form sweeps
     float *idt;           // fp32 1 / dtau per layer, indexed like lev (mixed-form sweeps)
     void *A, *F, *J;      // double or bf16/float depending on the form

@@ -474,6 +474,24 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             I.tsrc_mmax = d.top_m_max;
             greek_n += 3LL * (d.top_m_max + 1) * n;
         }
+        // fp16 source scale of the mixed form: the largest source magnitude S
+        // (the direct beam's pi, times the refracted beam's scale in water, and
+        // the external sources) maps to [2^5, 2^6).  Fields stay below about S
+        // and the scattering integral below a few times the field, so the
+        // scaled source keeps clear of fp16's largest value (65504) while
+        // everything above 2^-21 S stays in its normal range.
+        {
+            double smax = I.bsrc_only ? 0.0 : OCRT_PI * (I.water ? d.beam_scale : 1.0);
+            for (long long e = 0; d.bottom_source && e < 3LL * (I.bsrc_mmax + 1) * n; ++e)
+                smax = std::max(smax, std::fabs(d.bottom_source[e]));
+            for (long long e = 0; d.top_source && e < 3LL * (I.tsrc_mmax + 1) * n; ++e)
+                smax = std::max(smax, std::fabs(d.top_source[e]));
+            int ex = 0;
+            if (smax > 0.0 && std::isfinite(smax)) frexp(smax, &ex);
+            const int js = std::min(std::max(6 - ex, -100), 100);
+            I.jscale = ldexpf(1.0f, js);
+            I.jinv = ldexpf(1.0f, -js);
+        }
         I.off_layers = layers_n;
         layers_n += I.layered ? 4LL * I.nt : 0;
         I.gidx = i;
@@ -542,7 +560,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             if (p->inst[j].op_src < 0 && same_operator(i, j)) { p->inst[i].op_src = j; break; }
     // ---- chunking: instances grouped so one chunk's working set stays bounded
     const double budget = mixed ? 48e9 : 48e9;
-    const size_t fe = mixed ? 2 : 8, je = mixed ? 4 : 8, ae = mixed ? 2 : 8;
+    const size_t fe = mixed ? 2 : 8, je = mixed ? 2 : 8, ae = mixed ? 2 : 8;
     int item_global = 0;
     p->item_of_inst.resize(n_inst);
     // Adjoint plans keep the field of every ckpt-th order and recompute the
@@ -780,7 +798,9 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     rc |= dmalloc(p, (void **)&P.J, mx_F * je);
     rc |= dmalloc(p, (void **)&P.tot, mx_F * 8);
     if (mixed) rc |= dmalloc(p, (void **)&P.comp, std::max(mx_cmp, 1LL) * 4);
+    if (mixed) rc |= dmalloc(p, (void **)&P.trow, std::max(mx_cmp, 1LL) * 8);
     if (mixed) rc |= dmalloc(p, (void **)&P.Flo, std::max(mx_flo, 1LL) * 2);
+    if (mixed) rc |= dmalloc(p, (void **)&P.Jf, std::max(mx_flo, 1LL) * 4);
     rc |= dmalloc(p, (void **)&P.bc, mx_bc * 8);
     if (adjoint) {
         int maxit = 1;
@@ -927,7 +947,7 @@ static void launch_sweep_mixed(int v, int ncols, hipStream_t s, const DevInst *d
     const dim3 grid((unsigned)((ncols + W - 1) / W)), block((unsigned)(64 * W));
 #define OCRT_SWEEP_CASE(I, U_, W_)                                                                              \
     case I:                                                                                                      \
-        hipLaunchKernelGGL((sweep<__hip_bfloat16, float, true, U_, W_>), grid, block, 0, s, di, dt, P, st, tiles, \
+        hipLaunchKernelGGL((sweep<__hip_bfloat16, _Float16, true, U_, W_>), grid, block, 0, s, di, dt, P, st, tiles, \
                            ncols, 0);                                                                            \
         break;
     switch (v) {
@@ -1466,13 +1486,15 @@ extern "C" int ocrt_plan_results(ocrt_plan *p, ocrt_result_t *out, double *toa_m
             // dense particle operator 2 (6 n)(6 n_in) per level; the molecular
             // operator has rank <= 6 and counts as 2 * 2 * (6 n) * 6 per level
             alg += ord * L * ((I.aer ? 72.0 * I.n * n_in : 0.0) + ((m <= 2 && I.ray) ? 144.0 * I.n : 0.0));
-            // per element and order: read the source, write the field (bf16,
-            // plus its residual for the low modes, in the mixed form), read and
-            // write the fp64 order sum; per layer and node of each hemisphere:
-            // read 1 - T
+            // per element and order: read the source (mixed form: fp16, fp32
+            // for the low modes), write the field (bf16, plus its residual for
+            // the low modes, in the mixed form), read and write the order sum
+            // (fp64; the mixed form's interior rows as a 6-byte compensated
+            // pair); per layer and node of each hemisphere: read 1 - T
             const double el = 6.0 * I.n * L;
             const double fb = mixed ? (m < kSplitModes ? 4.0 : 2.0) : 8.0;
-            bytes += ord * (el * ((mixed ? 4.0 : 8.0) + fb + 16.0) + 2.0 * I.n * I.nt * (mixed ? 4.0 : 8.0));
+            const double jb = mixed ? (m < kSplitModes ? 4.0 : 2.0) : 8.0;
+            bytes += ord * (el * (jb + fb + (mixed ? 12.0 : 16.0)) + 2.0 * I.n * I.nt * (mixed ? 4.0 : 8.0));
         }
         if (toa_modes)
             for (int m = I.mmax + 1; m < M; ++m)

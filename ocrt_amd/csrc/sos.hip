@@ -20,6 +20,34 @@
 
 namespace ocrt {
 
+// Interior order sum of the mixed form as a compensated pair: the fp32 value
+// hi and the remainder (sum - hi) in units of 2^(e - 25), in fp16, where e is
+// the binary exponent of hi (hi = f 2^e, 0.5 <= |f| < 1) clamped below at
+// -125.  For a normal hi the remainder is at most half an fp32 ulp, 2^(e - 25);
+// fp32 subnormals (and zero) have the fixed spacing 2^-149, so their remainder
+// is at most 2^-150 = 2^(-125 - 25): with the clamp the scaled remainder stays
+// within [-1, 1] for every finite sum.  24 + 11 significant bits, a relative
+// precision of 2^-35.  The pair takes 6 bytes where an fp64 sum takes 8 (12
+// instead of 16 read and written per element and order); the two output rows
+// keep an fp64 sum plus an fp32 Neumaier term.  The convergence test's
+// max-norm is formed in fp64 from the pair's value.
+__device__ inline int pair_exponent(float hi)
+{
+    int e;
+    frexpf(hi, &e);
+    return e < -125 ? -125 : e;
+}
+__device__ inline double pair_value(float hi, _Float16 q)
+{
+    return (double)hi + ldexp((double)q, pair_exponent(hi) - 25);
+}
+__device__ inline void pair_store(double t, float *hi, _Float16 *q)
+{
+    const float h = (float)t;
+    *hi = h;
+    *q = (_Float16)ldexp(t - (double)h, 25 - pair_exponent(h));
+}
+
 // Formal solution of one column with the reference's linear-source closed form
 // (rt_layer_transport.c:89-99, 144-203), in fp64.  S(k) gives the source at
 // level k; emit(k, value) receives the field.
@@ -114,6 +142,9 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     if (col >= ct.col_end) return;
     FT *F = reinterpret_cast<FT *>(P.F) + it.off_F;
     double *tot = P.tot + it.off_tot;
+    // mixed form: interior order sum as a compensated pair, output rows fp64
+    float *thi = reinterpret_cast<float *>(tot);
+    _Float16 *tlo = reinterpret_cast<_Float16 *>(thi + (long long)it.L * Pp);
     if (MIXED) {
         P.comp[it.off_cmp + col] = 0.f;
         P.comp[it.off_cmp + Pp + col] = 0.f;
@@ -123,7 +154,7 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
         for (int k = 0; k <= nt; ++k) {
             F[(long long)k * Pp + col] = to_storage<FT>(0.0);
             if (Flo) Flo[(long long)k * Pp + col] = __float2bfloat16(0.f);
-            tot[(long long)k * Pp + col] = 0.0;
+            if (!MIXED) tot[(long long)k * Pp + col] = 0.0;
         }
         return;
     }
@@ -172,11 +203,14 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
                              if (ext) v += ext0 * att;
                          }
                          const long long ix = (long long)k * Pp + col;
-                         tot[ix] = v;
-                         if constexpr (MIXED)
+                         if constexpr (MIXED) {
+                             if (k == 0 || k == nt) P.trow[it.off_cmp + (k == 0 ? 0 : Pp) + col] = v;
+                             else pair_store(v, thi + ix, tlo + ix);
                              store_field_pair(reinterpret_cast<__hip_bfloat16 *>(F), Flo, ix, v);
-                         else
+                         } else {
+                             tot[ix] = v;
                              F[ix] = to_storage<FT>(v);
+                         }
                          if (hemi == 1 && k == nt) P.bc[it.off_bc + 3 * n + c * n + o] = v;
                          if (top_up && hemi == 0 && k == 0) P.bc[it.off_bc + 9 * n + c * n + o] = v;
                      });
@@ -201,7 +235,7 @@ __global__ void order_status(const DevInst *__restrict__ inst, const DevItem *__
             const double res = (mt > 0.0) ? mc / mt : 0.0;
             s.orders = it_next - 1;
             s.residual = res;
-            if (!(mc == mc) || !(mt == mt)) { s.status = -2; s.active = 0; }
+            if (!isfinite(mc) || !isfinite(mt)) { s.status = -2; s.active = 0; }
             else if (res < I.tol) { s.converged = 1; s.active = 0; }
             else if (it_next - 1 >= I.max_orders) s.active = 0;
         }
@@ -473,11 +507,25 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(co
     const unsigned short *A = reinterpret_cast<const unsigned short *>(P.A) + it.off_A;
     const unsigned short *F = reinterpret_cast<const unsigned short *>(P.F) + it.off_F;
     const unsigned short *Flo = reinterpret_cast<const unsigned short *>(P.Flo) + it.off_Flo;
-    float *J = reinterpret_cast<float *>(P.J) + it.off_F;
+    // J is stored in fp16 (11 significant bits), scaled by the instance's
+    // power of two jscale: half the bytes of the write here and of the sweep's
+    // read.  The items that keep their field as a pair (16 significant bits)
+    // keep an fp32 source.
+    _Float16 *J = reinterpret_cast<_Float16 *>(P.J) + it.off_F;
+    float *Jf = P.Jf + it.off_Flo;
+    const float js = I.jscale;
     const LevelView lv(P, I);
     __shared__ GemmLds<NP, SPLIT> S;
     gemm_tile_bf16<NP, SPLIT>(it, lv, A, F, Flo, tl.tm * 64, tl.tn * kGemmLevels, S, [&](int lev, int r, float4 v) {
-        if (r < Pp) *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v;
+        if (r >= Pp) return;
+        if constexpr (SPLIT) {
+            *reinterpret_cast<float4 *>(Jf + (long long)lev * Pp + r) = v;
+        } else {
+            typedef _Float16 half4_ __attribute__((ext_vector_type(4)));
+            const half4_ h = {(_Float16)(v.x * js), (_Float16)(v.y * js), (_Float16)(v.z * js),
+                              (_Float16)(v.w * js)};
+            *reinterpret_cast<half4_ *>(J + (long long)lev * Pp + r) = h;
+        }
     });
 }
 
@@ -513,7 +561,10 @@ __device__ inline float mixed_step(float run, float s_in, float s_out, float om,
 // of the two output rows, and the column's two max-norms.
 struct MixedColumn {
     __hip_bfloat16 *F, *Flo;   // level stride P; Flo null unless the item keeps the pair
-    double *tot, *bottom, *top;   // top: upward field at the top of an in-water column, else null
+    float *thi;                   // interior order sum: fp32 value and fp16 remainder (pair_value)
+    _Float16 *tlo;
+    double *row_top, *row_bot;    // fp64 order sum of the output rows k = 0 and k = nt
+    double *bottom, *top;         // top: upward field at the top of an in-water column, else null
     float *cmp_top, *cmp_bot;
     long long Pp;
     int nt, down;
@@ -524,7 +575,11 @@ struct MixedColumn {
         const int n = it.n;
         F = reinterpret_cast<__hip_bfloat16 *>(P.F) + it.off_F + col;
         Flo = it.fsplit ? reinterpret_cast<__hip_bfloat16 *>(P.Flo) + it.off_Flo + col : nullptr;
-        tot = P.tot + it.off_tot + col;
+        thi = reinterpret_cast<float *>(P.tot + it.off_tot);
+        tlo = reinterpret_cast<_Float16 *>(thi + (long long)it.L * it.P) + col;
+        thi += col;
+        row_top = P.trow + it.off_cmp + col;
+        row_bot = row_top + it.P;
         bottom = P.bc + it.off_bc + 3 * n + c * n + o;
         top = nullptr;
         cmp_top = P.comp + it.off_cmp + col;
@@ -533,6 +588,14 @@ struct MixedColumn {
         nt = it.nt;
         down = down_;
         mcur = mtot = 0.0;
+    }
+
+    // the order sum at level k before this order
+    __device__ double old_sum(int k) const
+    {
+        if (k == 0) return *row_top;
+        if (k == nt) return *row_bot;
+        return pair_value(thi[(long long)k * Pp], tlo[(long long)k * Pp]);
     }
 
     // the field value v at level k, with t_old the order sum there before
@@ -545,9 +608,11 @@ struct MixedColumn {
         if (down && k == nt) *bottom = v;
         if (top && k == 0) *top = v;
         const double tsum = t_old + v;
-        tot[ix] = tsum;
         mtot = fmax(mtot, fabs(tsum));
-        if (k == 0 || k == nt) {
+        if (k != 0 && k != nt) {
+            pair_store(tsum, thi + ix, tlo + ix);
+        } else {
+            *(k == 0 ? row_top : row_bot) = tsum;
             // Neumaier: the rounding error of this addition, kept in fp32
             float *cp = k == 0 ? cmp_top : cmp_bot;
             const double e = (fabs(t_old) >= fabs(v)) ? ((t_old - tsum) + v) : ((v - tsum) + t_old);
@@ -600,32 +665,42 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
             MixedColumn mc(P, it, col, c, o, hemi);
             mc.top = top_up;
             const float smu = hemi ? -(float)mu : (float)mu;
-            float run = (float)bc;
-            float s_in = (float)J[(long long)k_first * Pp];
-            mc.emit(k_first, bc, mc.tot[(long long)k_first * Pp]);
-            for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * U) {
-                float sj[U], om[U], idt[U];
-                double to[U];
+            // the source: fp32 for the items that keep their field as a pair,
+            // else fp16 scaled by jscale
+            const float ji = I.jinv;
+            const float *__restrict__ Jf = P.Jf + it.off_Flo + col;
+            auto scan = [&](auto jload) {
+                float run = (float)bc;
+                float s_in = jload(k_first);
+                mc.emit(k_first, bc, mc.old_sum(k_first));
+                for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * U) {
+                    float sj[U], om[U], idt[U];
+                    double to[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    // clamped indices keep the block's loads unconditional, so
-                    // they are all in flight before the first one is used
-                    const int k = clamp_level(k0 + step * u, nt);
-                    const int l = clamp_layer(layer_of(k0 + step * u), nt);
-                    sj[u] = (float)J[(long long)k * Pp];
-                    om[u] = OM[(long long)l * n + o];
-                    idt[u] = IDT[l];
-                    to[u] = mc.tot[(long long)k * Pp];
-                }
+                    for (int u = 0; u < U; ++u) {
+                        // clamped indices keep the block's loads unconditional, so
+                        // they are all in flight before the first one is used
+                        const int k = clamp_level(k0 + step * u, nt);
+                        const int l = clamp_layer(layer_of(k0 + step * u), nt);
+                        sj[u] = jload(k);
+                        om[u] = OM[(long long)l * n + o];
+                        idt[u] = IDT[l];
+                        to[u] = mc.old_sum(k);
+                    }
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int k = k0 + step * u;
-                    if (k < 0 || k > nt) break;
-                    run = mixed_step(run, s_in, sj[u], om[u], idt[u], smu, hemi);
-                    mc.emit(k, (double)run, to[u]);
-                    s_in = sj[u];
+                    for (int u = 0; u < U; ++u) {
+                        const int k = k0 + step * u;
+                        if (k < 0 || k > nt) break;
+                        run = mixed_step(run, s_in, sj[u], om[u], idt[u], smu, hemi);
+                        mc.emit(k, (double)run, to[u]);
+                        s_in = sj[u];
+                    }
                 }
-            }
+            };
+            if (it.fsplit)
+                scan([&](int k) { return Jf[(long long)k * Pp]; });
+            else
+                scan([&](int k) { return (float)J[(long long)k * Pp] * ji; });
             mcur = mc.mcur;
             mtot = mc.mtot;
         } else {
@@ -697,9 +772,11 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
 // adds the row's compensation term.
 __device__ inline double total_at(const Pools &P, const DevItem &it, bool mixed, int k, int col)
 {
-    const double t = P.tot[it.off_tot + (long long)k * it.P + col];
-    if (mixed) return t + (double)P.comp[it.off_cmp + (k == 0 ? 0 : it.P) + col];
-    return t;
+    if (mixed) {
+        const long long r = it.off_cmp + (k == 0 ? 0 : it.P) + col;
+        return P.trow[r] + (double)P.comp[r];
+    }
+    return P.tot[it.off_tot + (long long)k * it.P + col];
 }
 
 // Per-item outputs after the order loop (rt_solver.c:1010-1061).
@@ -844,12 +921,12 @@ template __global__ void order_status<__hip_bfloat16>(const DevInst *, const Dev
     template __global__ void sweep<FT, JT, MX, U, W>(const DevInst *, const DevItem *, Pools, ItemState *,    \
                                                      const ColTile *, int, int);
 OCRT_SWEEP_INST(double, double, false, 8, 4)
-OCRT_SWEEP_INST(__hip_bfloat16, float, true, 4, 2)
-OCRT_SWEEP_INST(__hip_bfloat16, float, true, 4, 4)
-OCRT_SWEEP_INST(__hip_bfloat16, float, true, 8, 2)
-OCRT_SWEEP_INST(__hip_bfloat16, float, true, 8, 4)
-OCRT_SWEEP_INST(__hip_bfloat16, float, true, 16, 2)
-OCRT_SWEEP_INST(__hip_bfloat16, float, true, 16, 4)
+OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 4, 2)
+OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 4, 4)
+OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 8, 2)
+OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 8, 4)
+OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 16, 2)
+OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 16, 4)
 #undef OCRT_SWEEP_INST
 template __global__ void gemm_bf16<1, false>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                              const ItemState *, int);
