@@ -87,7 +87,7 @@ struct DevItem {
     long long off_tab;    // pool_tab: 6 x [(n+1)][2n+1]
     long long off_A;      // pool_A: [Mp][K]
     long long off_F;      // pool_F / pool_J: [L][P]
-    long long off_tot;    // pool_tot: [L][P]
+    long long off_tot;    // pool_tot: [L][P] (fp64; fp32 in the mixed form)
     long long off_bc;     // pool_bc: [3][n] bottom boundary condition, [3][n] bottom downward field,
                           // [3][n] top boundary condition, [3][n] top upward field
     long long off_cmp;    // pool_comp: [2][P] compensation of the order sum at k = 0 and k = nt
@@ -123,7 +123,7 @@ struct Pools {
     const double *gout;  // adjoint: upstream gradient [instance][3] of (rho_I, rho_Q, rho_U)
     float *comp;          // fp32 compensation of the mixed-form order sum on the two output rows
     double *trow;         // mixed form: fp64 order sum of the two output rows (k = 0, k = nt), [2][P] per item
-                          // at off_cmp; the interior rows keep a compensated fp32 + fp16 pair in pool_tot
+                          // at off_cmp; the interior rows keep an fp32 sum in pool_tot
     float *Jf;            // mixed form: fp32 source of the items that keep their field as a pair, [L][P] at off_Flo
     float *Tf;            // fp32 1 - T for the mixed-form sweeps
     float *idt;           // fp32 1 / dtau per layer, indexed like lev (mixed-form sweeps)

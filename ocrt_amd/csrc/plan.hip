@@ -796,7 +796,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     rc |= dmalloc(p, (void **)&P.A, mx_A * ae);
     rc |= dmalloc(p, (void **)&P.F, mx_F * fe);
     rc |= dmalloc(p, (void **)&P.J, mx_F * je);
-    rc |= dmalloc(p, (void **)&P.tot, mx_F * 8);
+    rc |= dmalloc(p, (void **)&P.tot, mx_F * (mixed ? 4 : 8));
     if (mixed) rc |= dmalloc(p, (void **)&P.comp, std::max(mx_cmp, 1LL) * 4);
     if (mixed) rc |= dmalloc(p, (void **)&P.trow, std::max(mx_cmp, 1LL) * 8);
     if (mixed) rc |= dmalloc(p, (void **)&P.Flo, std::max(mx_flo, 1LL) * 2);
@@ -1489,12 +1489,12 @@ extern "C" int ocrt_plan_results(ocrt_plan *p, ocrt_result_t *out, double *toa_m
             // per element and order: read the source (mixed form: fp16, fp32
             // for the low modes), write the field (bf16, plus its residual for
             // the low modes, in the mixed form), read and write the order sum
-            // (fp64; the mixed form's interior rows as a 6-byte compensated
-            // pair); per layer and node of each hemisphere: read 1 - T
+            // (fp64; fp32 in the mixed form's interior rows); per layer and
+            // node of each hemisphere: read 1 - T
             const double el = 6.0 * I.n * L;
             const double fb = mixed ? (m < kSplitModes ? 4.0 : 2.0) : 8.0;
             const double jb = mixed ? (m < kSplitModes ? 4.0 : 2.0) : 8.0;
-            bytes += ord * (el * (jb + fb + (mixed ? 12.0 : 16.0)) + 2.0 * I.n * I.nt * (mixed ? 4.0 : 8.0));
+            bytes += ord * (el * (jb + fb + (mixed ? 8.0 : 16.0)) + 2.0 * I.n * I.nt * (mixed ? 4.0 : 8.0));
         }
         if (toa_modes)
             for (int m = I.mmax + 1; m < M; ++m)
