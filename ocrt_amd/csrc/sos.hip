@@ -519,7 +519,9 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(co
 //
 // The order sum is fp64 over the whole field (it feeds the convergence test).
 // In the mixed form the two rows that become outputs, k = 0 (TOA) and k = nt
-// (BOA), also carry an fp32 Neumaier compensation term.
+// (BOA), keep it in fp64 with an fp32 Neumaier compensation term and the
+// interior levels in fp32; the scan takes the boundary row, the interior
+// levels without per-level row tests, then the far row.
 __device__ inline int clamp_level(int k, int nt) { return k < 0 ? 0 : (k > nt ? nt : k); }
 __device__ inline int clamp_layer(int l, int nt) { return l < 0 ? 0 : (l > nt - 1 ? nt - 1 : l); }
 
@@ -535,9 +537,10 @@ __device__ inline float mixed_step(float run, float s_in, float s_out, float om,
     return fmaf(0.5f, fmaf(om, fmaf(a, smu, s_in), s_out - s_in), fmaf(-om, run, run));
 }
 
-// One column of the mixed form: its field pair (bf16, plus the bf16 residual
-// for the items that keep it), its fp64 order sum with the fp32 compensation
-// of the two output rows, and the column's two max-norms.
+// One column of the mixed form: its field (bf16, plus the bf16 residual for
+// the items that keep the pair), the order sum of its two output rows k = 0
+// and k = nt (fp64 with an fp32 Neumaier term) and the column's two
+// max-norms.  The sweep walks the interior levels itself, branch-free.
 struct MixedColumn {
     __hip_bfloat16 *F, *Flo;   // level stride P; Flo null unless the item keeps the pair
     float *tin;                   // interior order sum (fp32)
@@ -546,7 +549,7 @@ struct MixedColumn {
     float *cmp_top, *cmp_bot;
     long long Pp;
     int nt, down;
-    double mcur, mtot;
+    float mcur, mtot;
 
     __device__ MixedColumn(const Pools &P, const DevItem &it, int col, int c, int o, int down_)
     {
@@ -563,37 +566,25 @@ struct MixedColumn {
         Pp = it.P;
         nt = it.nt;
         down = down_;
-        mcur = mtot = 0.0;
+        mcur = mtot = 0.f;
     }
 
-    // the order sum at level k before this order
-    __device__ double old_sum(int k) const
-    {
-        if (k == 0) return *row_top;
-        if (k == nt) return *row_bot;
-        return (double)tin[(long long)k * Pp];
-    }
-
-    // the field value v at level k, with t_old the order sum there before
-    // this order
-    __device__ void emit(int k, double v, double t_old)
+    // the field value v at the output row k (0 or nt)
+    __device__ void emit_row(int k, double v)
     {
         const long long ix = (long long)k * Pp;
         store_field_pair(F, Flo, ix, v);
-        mcur = fmax(mcur, fabs(v));
+        mcur = fmaxf(mcur, fabsf((float)v));
         if (down && k == nt) *bottom = v;
         if (top && k == 0) *top = v;
-        const double tsum = t_old + v;
-        mtot = fmax(mtot, fabs(tsum));
-        if (k != 0 && k != nt) {
-            tin[ix] = (float)tsum;
-        } else {
-            *(k == 0 ? row_top : row_bot) = tsum;
-            // Neumaier: the rounding error of this addition, kept in fp32
-            float *cp = k == 0 ? cmp_top : cmp_bot;
-            const double e = (fabs(t_old) >= fabs(v)) ? ((t_old - tsum) + v) : ((v - tsum) + t_old);
-            *cp = (float)((double)*cp + e);
-        }
+        double *rs = k == 0 ? row_top : row_bot;
+        const double t_old = *rs, tsum = t_old + v;
+        *rs = tsum;
+        mtot = fmaxf(mtot, fabsf((float)tsum));
+        // Neumaier: the rounding error of this addition, kept in fp32
+        float *cp = k == 0 ? cmp_top : cmp_bot;
+        const double e = (fabs(t_old) >= fabs(v)) ? ((t_old - tsum) + v) : ((v - tsum) + t_old);
+        *cp = (float)((double)*cp + e);
     }
 };
 
@@ -645,38 +636,61 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
             // else fp16 scaled by jscale
             const float ji = I.jinv;
             const float *__restrict__ Jf = P.Jf + it.off_Flo + col;
-            auto scan = [&](auto jload) {
+            auto scan = [&](auto split_tag) {
+                constexpr bool SPLIT = decltype(split_tag)::value;
+                auto jload = [&](int k) -> float {
+                    if constexpr (SPLIT) return Jf[(long long)k * Pp];
+                    else return (float)J[(long long)k * Pp] * ji;
+                };
+                __hip_bfloat16 *__restrict__ F = mc.F;
+                __hip_bfloat16 *__restrict__ Flo = mc.Flo;
+                float *__restrict__ tin = mc.tin;
                 float run = (float)bc;
                 float s_in = jload(k_first);
-                mc.emit(k_first, bc, mc.old_sum(k_first));
-                for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * U) {
-                    float sj[U], om[U], idt[U];
-                    double to[U];
+                mc.emit_row(k_first, bc);
+                float cur = 0.f, tot = 0.f;
+                // interior levels, i = 1 .. nt - 1 steps along the scan
+                for (int i0 = 1; i0 < nt; i0 += U) {
+                    float sj[U], om[U], idt[U], to[U];
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
-                        // clamped indices keep the block's loads unconditional, so
-                        // they are all in flight before the first one is used
-                        const int k = clamp_level(k0 + step * u, nt);
-                        const int l = clamp_layer(layer_of(k0 + step * u), nt);
+                        // clamped indices keep the block's loads unconditional,
+                        // so they are all in flight before the first one is used
+                        const int i = min(i0 + u, nt - 1);
+                        const int k = hemi ? i : nt - i;
+                        const int l = layer_of(k);
                         sj[u] = jload(k);
                         om[u] = OM[(long long)l * n + o];
                         idt[u] = IDT[l];
-                        to[u] = mc.old_sum(k);
+                        to[u] = tin[(long long)k * Pp];
                     }
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
-                        const int k = k0 + step * u;
-                        if (k < 0 || k > nt) break;
+                        const int i = i0 + u;
+                        if (i >= nt) break;
+                        const long long ix = (long long)(hemi ? i : nt - i) * Pp;
                         run = mixed_step(run, s_in, sj[u], om[u], idt[u], smu, hemi);
-                        mc.emit(k, (double)run, to[u]);
+                        const __hip_bfloat16 h = __float2bfloat16(run);
+                        F[ix] = h;
+                        if constexpr (SPLIT) Flo[ix] = __float2bfloat16(run - __bfloat162float(h));
+                        const float ts = to[u] + run;
+                        tin[ix] = ts;
+                        cur = fmaxf(cur, fabsf(run));
+                        tot = fmaxf(tot, fabsf(ts));
                         s_in = sj[u];
                     }
                 }
+                // the far output row
+                const int kl = hemi ? nt : 0, ll = layer_of(kl);
+                run = mixed_step(run, s_in, jload(kl), OM[(long long)ll * n + o], IDT[ll], smu, hemi);
+                mc.mcur = fmaxf(mc.mcur, cur);
+                mc.mtot = fmaxf(mc.mtot, tot);
+                mc.emit_row(kl, (double)run);
             };
             if (it.fsplit)
-                scan([&](int k) { return Jf[(long long)k * Pp]; });
+                scan(std::true_type{});
             else
-                scan([&](int k) { return (float)J[(long long)k * Pp] * ji; });
+                scan(std::false_type{});
             mcur = mc.mcur;
             mtot = mc.mtot;
         } else {
