@@ -642,43 +642,63 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                     if constexpr (SPLIT) return Jf[(long long)k * Pp];
                     else return (float)J[(long long)k * Pp] * ji;
                 };
-                __hip_bfloat16 *__restrict__ F = mc.F;
-                __hip_bfloat16 *__restrict__ Flo = mc.Flo;
-                float *__restrict__ tin = mc.tin;
                 float run = (float)bc;
                 float s_in = jload(k_first);
                 mc.emit_row(k_first, bc);
                 float cur = 0.f, tot = 0.f;
-                // interior levels, i = 1 .. nt - 1 steps along the scan
-                for (int i0 = 1; i0 < nt; i0 += U) {
+                // interior levels, i = 1 .. nt - 1 steps along the scan: running
+                // pointers at the level (source, field, order sum) and at the
+                // layer crossed to reach it (1 - T, 1 / dtau)
+                const long long dP = hemi ? (long long)Pp : -(long long)Pp;
+                const int dL = hemi ? 1 : -1;
+                const int k1 = k_first + dL, l1 = layer_of(k1);
+                const long long ix1 = (long long)k1 * Pp;
+                const _Float16 *__restrict__ pJ = J + ix1;
+                const float *__restrict__ pJf = Jf + ix1;
+                __hip_bfloat16 *__restrict__ pF = mc.F + ix1;
+                __hip_bfloat16 *__restrict__ pFlo = SPLIT ? mc.Flo + ix1 : nullptr;
+                float *__restrict__ pT = mc.tin + ix1;
+                const float *__restrict__ pOM = OM + (long long)l1 * n + o;
+                const float *__restrict__ pIDT = IDT + l1;
+                auto level = [&](long long e, float sj, float om, float idt, float to) {
+                    run = mixed_step(run, s_in, sj, om, idt, smu, hemi);
+                    const __hip_bfloat16 h = __float2bfloat16(run);
+                    pF[e] = h;
+                    if constexpr (SPLIT) pFlo[e] = __float2bfloat16(run - __bfloat162float(h));
+                    const float ts = to + run;
+                    pT[e] = ts;
+                    cur = fmaxf(cur, fabsf(run));
+                    tot = fmaxf(tot, fabsf(ts));
+                    s_in = sj;
+                };
+                auto src = [&](long long e) -> float {
+                    if constexpr (SPLIT) return pJf[e];
+                    else return (float)pJ[e] * ji;
+                };
+                int i = 1;
+                // whole blocks of U levels: every load of the block is issued
+                // before its dependent steps
+                for (; i + U <= nt; i += U) {
                     float sj[U], om[U], idt[U], to[U];
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
-                        // clamped indices keep the block's loads unconditional,
-                        // so they are all in flight before the first one is used
-                        const int i = min(i0 + u, nt - 1);
-                        const int k = hemi ? i : nt - i;
-                        const int l = layer_of(k);
-                        sj[u] = jload(k);
-                        om[u] = OM[(long long)l * n + o];
-                        idt[u] = IDT[l];
-                        to[u] = tin[(long long)k * Pp];
+                        sj[u] = src(u * dP);
+                        om[u] = pOM[u * dL * n];
+                        idt[u] = pIDT[u * dL];
+                        to[u] = pT[u * dP];
                     }
 #pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const int i = i0 + u;
-                        if (i >= nt) break;
-                        const long long ix = (long long)(hemi ? i : nt - i) * Pp;
-                        run = mixed_step(run, s_in, sj[u], om[u], idt[u], smu, hemi);
-                        const __hip_bfloat16 h = __float2bfloat16(run);
-                        F[ix] = h;
-                        if constexpr (SPLIT) Flo[ix] = __float2bfloat16(run - __bfloat162float(h));
-                        const float ts = to[u] + run;
-                        tin[ix] = ts;
-                        cur = fmaxf(cur, fabsf(run));
-                        tot = fmaxf(tot, fabsf(ts));
-                        s_in = sj[u];
-                    }
+                    for (int u = 0; u < U; ++u) level(u * dP, sj[u], om[u], idt[u], to[u]);
+                    pJ += U * dP; pJf += U * dP; pF += U * dP; pT += U * dP;
+                    if constexpr (SPLIT) pFlo += U * dP;
+                    pOM += U * dL * n; pIDT += U * dL;
+                }
+                // the remaining interior levels one at a time
+                for (; i < nt; ++i) {
+                    level(0, src(0), pOM[0], pIDT[0], pT[0]);
+                    pJ += dP; pJf += dP; pF += dP; pT += dP;
+                    if constexpr (SPLIT) pFlo += dP;
+                    pOM += dL * n; pIDT += dL;
                 }
                 // the far output row
                 const int kl = hemi ? nt : 0, ll = layer_of(kl);
