@@ -118,6 +118,7 @@ struct Pools {
     double *lev, *ring, *T, *greek, *R, *basis, *tab, *tot, *bc;
     double *layers;       // explicit layering of the instances that carry one
     double *dR;           // adjoint: d R^m / d sigma^2, layout of R
+    float *Rf;            // mixed form: fp32 copy of pool_R read by the per-order rough boundary condition
     double *lam;          // adjoint: [L][P] per item, adjoint of the current order's field
     double *grad;         // adjoint: per-instance accumulators (see DevInst::off_grad)
     const double *gout;  // adjoint: upstream gradient [instance][3] of (rho_I, rho_Q, rho_U)

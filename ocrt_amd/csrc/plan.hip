@@ -790,6 +790,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     rc |= dmalloc(p, (void **)&P.Tf, mx_T * 4);
     rc |= dmalloc(p, (void **)&P.idt, mx_lev * 4);
     rc |= dmalloc(p, (void **)&P.R, mx_R * 8);
+    if (mixed) rc |= dmalloc(p, (void **)&P.Rf, std::max(mx_R, 1LL) * 4);
     // padded: the phase tables read the rows of a full group past the last row
     rc |= dmalloc(p, (void **)&P.basis, (mx_basis + kTabRows) * 8);
     rc |= dmalloc(p, (void **)&P.tab, mx_tab * 8);

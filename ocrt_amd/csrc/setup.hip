@@ -316,6 +316,7 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
             for (int t = threadIdx.x; t < nout; t += blockDim.x) {
                 const int m = t / 9, kl = t % 9;
                 P.R[r_index(I, m, o, i, kl)] = 0.0;
+                if (P.Rf) P.Rf[r_index(I, m, o, i, kl)] = 0.f;
                 if (DERIV) P.dR[r_index(I, m, o, i, kl)] = 0.0;
             }
             return;
@@ -392,8 +393,12 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
                 if (m > 0 && (kl == 2 || kl == 5)) val = -val;
                 const long long at = !solar ? r_index(I, m, o, i, kl)
                                             : I.off_Rsun + ((long long)m * n + o) * 9 + kl;
-                if (DERIV && (row & 1)) P.dR[at] = val;
-                else P.R[at] = val;
+                if (DERIV && (row & 1)) {
+                    P.dR[at] = val;
+                } else {
+                    P.R[at] = val;
+                    if (P.Rf && !solar) P.Rf[at] = (float)val;
+                }
             }
         }
         __syncthreads();
@@ -424,6 +429,7 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
             if (m > 0 && (kl == 2 || kl == 5)) val = -val;
             const long long at = !solar ? r_index(I, m, o, i, kl) : I.off_Rsun + ((long long)m * n + o) * 9 + kl;
             P.R[at] = val;
+            if (P.Rf && !solar) P.Rf[at] = (float)val;
         }
     }
 }

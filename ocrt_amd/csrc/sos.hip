@@ -30,53 +30,70 @@ namespace ocrt {
 
 // Formal solution of one column with the reference's linear-source closed form
 // (rt_layer_transport.c:89-99, 144-203), in fp64.  S(k) gives the source at
-// level k; emit(k, value) receives the field.
+// level k; emit(k, value, tr, where) receives the field at level k, with tr
+// the Beer-Lambert factor of the layer just crossed and where = 0 for the
+// starting row, 1 for an interior level, 2 for the far row (a compile-time
+// constant, so the interior levels carry no row tests).
 template <typename SrcFn, typename EmitFn>
 __device__ inline void sweep_column_f64(int hemi, int nt, double mu, double bc, const double *h,
                                         const double *T, int n, int o, SrcFn S, EmitFn emit)
 {
-    // levels are taken in blocks of 4: the block's sources, depths and
-    // transmittances are gathered first (clamped indices, no branches), so
-    // their loads overlap, then the recurrence runs over them
-    constexpr int kB = 4;
-    const int step = hemi ? 1 : -1;
+    using first_t = std::integral_constant<int, 0>;
+    using inner_t = std::integral_constant<int, 1>;
+    using far_t = std::integral_constant<int, 2>;
+    const int dL = hemi ? 1 : -1;
     const int k_first = hemi ? 0 : nt;
     double run = hemi ? 0.0 : bc;
-    emit(k_first, run);
+    emit(k_first, run, 1.0, first_t{});
     double s_in = S(k_first);
-    for (int k0 = k_first + step; hemi ? (k0 <= nt) : (k0 >= 0); k0 += step * kB) {
-        double sv[kB], hk[kB], hu[kB], tr[kB];
+    double hu = h[k_first];   // depth of the upstream level
+    auto step = [&](double s_out, double hk, double tr) {
+        if (hemi == 0) {
+            const double dt = hu - hk;
+            const double a = (s_in - s_out) / dt;
+            const double b = s_out - a * hk;
+            run = tr * run + ((1.0 - tr) * (b + a * mu) + a * (hk - hu * tr)) * 0.5;
+        } else {
+            const double dt = hk - hu;
+            const double a = (s_out - s_in) / dt;
+            const double b = s_out - a * hk;
+            run = tr * run + ((1.0 - tr) * (b - a * mu) + a * (hk - hu * tr)) * 0.5;
+        }
+        s_in = s_out;
+        hu = hk;
+    };
+    // level i steps along the scan: k = k_first + dL i, crossing layer
+    // hemi ? k - 1 : k.  Interior levels i = 1 .. nt - 1 in blocks of 4 whose
+    // sources, depths and transmittances are gathered before the recurrence
+    // runs over them, then one at a time, then the far row.
+    constexpr int kB = 4;
+    auto layer_of = [&](int k) { return hemi ? k - 1 : k; };
+    int i = 1;
+    for (; i + kB <= nt; i += kB) {
+        double sv[kB], hk[kB], tr[kB];
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-            int k = k0 + step * u;
-            k = k < 0 ? 0 : (k > nt ? nt : k);
-            const int ku = hemi ? (k > 0 ? k - 1 : 0) : (k < nt ? k + 1 : nt);   // upstream level
-            const int layer = hemi ? ku : k;
+            const int k = k_first + dL * (i + u);
             sv[u] = S(k);
             hk[u] = h[k];
-            hu[u] = h[ku];
-            tr[u] = T[(long long)(layer < nt ? layer : nt - 1) * n + o];
+            tr[u] = T[(long long)layer_of(k) * n + o];
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-            const int k = k0 + step * u;
-            if (k < 0 || k > nt) break;
-            const double s_out = sv[u];
-            if (hemi == 0) {
-                const double dt = hu[u] - hk[u];
-                const double a = (s_in - s_out) / dt;
-                const double b = s_out - a * hk[u];
-                run = tr[u] * run + ((1.0 - tr[u]) * (b + a * mu) + a * (hk[u] - hu[u] * tr[u])) * 0.5;
-            } else {
-                const double dt = hk[u] - hu[u];
-                const double a = (s_out - s_in) / dt;
-                const double b = s_out - a * hk[u];
-                run = tr[u] * run + ((1.0 - tr[u]) * (b - a * mu) + a * (hk[u] - hu[u] * tr[u])) * 0.5;
-            }
-            emit(k, run);
-            s_in = s_out;
+            step(sv[u], hk[u], tr[u]);
+            emit(k_first + dL * (i + u), run, tr[u], inner_t{});
         }
     }
+    for (; i < nt; ++i) {
+        const int k = k_first + dL * i;
+        const double tr = T[(long long)layer_of(k) * n + o];
+        step(S(k), h[k], tr);
+        emit(k, run, tr, inner_t{});
+    }
+    const int kl = k_first + dL * nt;
+    const double tr = T[(long long)layer_of(kl) * n + o];
+    step(S(kl), h[kl], tr);
+    emit(kl, run, tr, far_t{});
 }
 
 // Field store of the mixed form: bf16, plus the bf16 residual v - hi for the
@@ -173,25 +190,27 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
     double att = 1.0;
     sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o,
                      [&](int k) { return solar ? src.primary(lv, k) + src.flat_value(lv, I.mus, k) : 0.0; },
-                     [&](int k, double v) {
+                     [&](int k, double v, double tr, auto where) {
+                         constexpr int kWhere = decltype(where)::value;   // 0 starting row, 1 interior, 2 far row
                          if (rough_seed || ext) {
-                             // the upward scan visits k = nt .. 0, the downward one 0 .. nt
-                             if (hemi == 0 && k < nt) att *= T[(long long)k * n + o];
-                             if (hemi == 1 && k > 0) att *= T[(long long)(k - 1) * n + o];
+                             // the running product of the layers crossed since the start
+                             if constexpr (kWhere != 0) att *= tr;
                              if (rough_seed) v += seed0 * att;
                              if (ext) v += ext0 * att;
                          }
                          const long long ix = (long long)k * Pp + col;
                          if constexpr (MIXED) {
-                             if (k == 0 || k == nt) P.trow[it.off_cmp + (k == 0 ? 0 : Pp) + col] = v;
-                             else tin[ix] = (float)v;
+                             if constexpr (kWhere == 1) tin[ix] = (float)v;
+                             else P.trow[it.off_cmp + (k == 0 ? 0 : Pp) + col] = v;
                              store_field_pair(reinterpret_cast<__hip_bfloat16 *>(F), Flo, ix, v);
                          } else {
                              tot[ix] = v;
                              F[ix] = to_storage<FT>(v);
                          }
-                         if (hemi == 1 && k == nt) P.bc[it.off_bc + 3 * n + c * n + o] = v;
-                         if (top_up && hemi == 0 && k == 0) P.bc[it.off_bc + 9 * n + c * n + o] = v;
+                         if constexpr (kWhere != 1) {
+                             if (hemi == 1 && k == nt) P.bc[it.off_bc + 3 * n + c * n + o] = v;
+                             if (top_up && hemi == 0 && k == 0) P.bc[it.off_bc + 9 * n + c * n + o] = v;
+                         }
                      });
 }
 

@@ -186,7 +186,27 @@ __device__ inline void surface_bc_item(const DevInst &I, const DevItem &it, cons
         return;
     }
     // rough surface, threads over the outgoing nodes: entry kl of R^m[o][i]
-    // sits at R[kl n] from the node's own base, consecutive over the lanes
+    // sits at R[kl n] from the node's own base, consecutive over the lanes.
+    // Mixed-form plans read the fp32 copy of the kernels (half the bytes of
+    // this per-order product), accumulating in fp64.
+    if (P.Rf) {
+        for (int o = threadIdx.x; o < n; o += blockDim.x) {
+            double si = 0.0, sq = 0.0, su = 0.0;
+            for (int i = 0; i < n; ++i) {
+                if (w[i] == 0.0) continue;
+                const double a = bot[i], b = bot[n + i], cc = bot[2 * n + i];
+                const double pw = W * mu[i] * w[i];
+                const float *R = P.Rf + r_index(I, m, o, i, 0);
+                si += pw * ((double)R[0] * a + (double)R[n] * b + (double)R[2 * n] * cc);
+                sq += pw * ((double)R[3 * n] * a + (double)R[4 * n] * b + (double)R[5 * n] * cc);
+                su += pw * ((double)R[6 * n] * a + (double)R[7 * n] * b + (double)R[8 * n] * cc);
+            }
+            bc[o] = si;
+            bc[n + o] = sq;
+            bc[2 * n + o] = su;
+        }
+        return;
+    }
     for (int o = threadIdx.x; o < n; o += blockDim.x) {
         double si = 0.0, sq = 0.0, su = 0.0;
         for (int i = 0; i < n; ++i) {
