@@ -252,8 +252,11 @@ enum {
                                       chunk, else every 2nd, else every 4th) */
     OCRT_PLAN_AUTOTUNE = 8,        /* mixed form: the first execute times every launch shape of the layer
                                       sweep on this plan's work and keeps the fastest (ocrt_plan_tuning) */
-    OCRT_PLAN_GRAPH = 16           /* the first execute captures the whole forward launch sequence into a
+    OCRT_PLAN_GRAPH = 16,          /* the first execute captures the whole forward launch sequence into a
                                       hipGraph; later executes replay it (not with profiling or adjoint) */
+    OCRT_PLAN_DEVICE_INPUTS = 32   /* the per-layer optical depths and wind speeds may be replaced from
+                                      device memory between executes (ocrt_plan_set_inputs): no two
+                                      instances share their surface kernels */
 };
 
 /* Launch shape chosen for the mixed-form sweep (levels per block, wavefronts
@@ -287,6 +290,33 @@ int ocrt_plan_create_ex(const ocrt_instance_t *inst, int n_inst, int precision, 
  * chain rule through the layer-to-level map (ocrt_amd/adjoint.py). */
 int ocrt_plan_adjoint(ocrt_plan *plan, const double *grad_out, double *grad_levels, double *grad_wind,
                       void *hip_stream);
+
+/* Device-memory inputs and outputs of a plan, for callers whose data already
+ * lives on the GPU (the torch.autograd.Function of ocrt_amd/torch_ops.py): no
+ * host copies, every copy and kernel on hip_stream, no synchronisation.  No
+ * counterpart in the reference, whose solver takes host structs only.
+ *
+ * ocrt_plan_set_inputs: the layers of every instance, packed in instance
+ * order as [4][n_layers] blocks (the layout of ocrt_instance_t.layers), from
+ * d_layers; and the wind speeds [n_inst] from d_wind (plans made with
+ * OCRT_PLAN_DEVICE_INPUTS).  Either may be NULL.  The structure of the plan
+ * stays as made: an instance whose new values break it (a negative or
+ * non-finite optical depth, an albedo outside [0, 1], a layer with no
+ * extinction, an aerosol column appearing or vanishing, a rough surface
+ * without wind or a flat one with wind) reports status OCRT_ERR_ARGS from the
+ * next execute.  OCRT_ERR_ARGS when an instance is not layered (d_layers) or
+ * the plan lacks OCRT_PLAN_DEVICE_INPUTS (d_wind). */
+int ocrt_plan_set_inputs(ocrt_plan *plan, const double *d_layers, const double *d_wind, void *hip_stream);
+/* After an execute: d_rho [n_inst][3] = (rho_I, rho_Q, rho_U), NaN for an
+ * instance whose status is not OCRT_OK; d_status [n_inst] (may be NULL). */
+int ocrt_plan_rho(ocrt_plan *plan, double *d_rho, int *d_status, void *hip_stream);
+/* ocrt_plan_adjoint with device-memory operands and the per-layer chain rule
+ * on the device: d_grad_out [n_inst][3]; d_grad_layers packed like d_layers
+ * above, d/d(Rayleigh tau, aerosol tau, aerosol albedo, absorption tau) of
+ * every layer (may be NULL); d_grad_wind [n_inst] (may be NULL).  Every
+ * instance must be layered when d_grad_layers is given. */
+int ocrt_plan_adjoint_layers(ocrt_plan *plan, const double *d_grad_out, double *d_grad_layers, double *d_grad_wind,
+                             void *hip_stream);
 
 /* One-call Jacobian form (the boundary's ocrt_adjoint_batch): plan, forward,
  * adjoint, copy results and gradients, destroy.  precision must be FP64. */

@@ -22,6 +22,7 @@ PLAN_ADJOINT = 2
 PLAN_CKPT_SPARSE = 4
 PLAN_AUTOTUNE = 8
 PLAN_GRAPH = 16
+PLAN_DEVICE_INPUTS = 32
 
 
 class OcrtError(RuntimeError):
@@ -63,6 +64,9 @@ def library():
         lib.ocrt_plan_lut_results.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_plan_create_ex.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         lib.ocrt_plan_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ocrt_plan_set_inputs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ocrt_plan_rho.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ocrt_plan_adjoint_layers.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_adjoint_batch.argtypes = [C.POINTER(Instance), C.c_int, C.c_int, C.c_void_p, C.POINTER(Result),
                                            C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_gas_create.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]
@@ -165,7 +169,7 @@ class Plan:
     another (the plain one-instance-per-launch form)."""
 
     def __init__(self, descs, precision="mixed", one_per_launch=False, adjoint=False, ckpt_sparse=False,
-                 autotune=False, graph=False):
+                 autotune=False, graph=False, device_inputs=False):
         _require_gpu()
         self.lib = library()
         self.n = len(descs)
@@ -174,7 +178,7 @@ class Plan:
         self.n_layers = [d.n_layers for d in descs]
         flags = ((PLAN_ONE_PER_LAUNCH if one_per_launch else 0) | (PLAN_ADJOINT if adjoint else 0) |
                  (PLAN_CKPT_SPARSE if ckpt_sparse else 0) | (PLAN_AUTOTUNE if autotune else 0) |
-                 (PLAN_GRAPH if graph else 0))
+                 (PLAN_GRAPH if graph else 0) | (PLAN_DEVICE_INPUTS if device_inputs else 0))
         rc = self.lib.ocrt_plan_create_ex(self._arr, self.n, PRECISIONS[precision], flags, C.byref(self._handle))
         if rc != 0:
             raise OcrtError(f"ocrt_plan_create failed with code {rc}")
@@ -243,6 +247,38 @@ class Plan:
             out["grad_y"].append(lev[off + 2 * L:off + 3 * L].copy())
             off += 3 * L
         return out
+
+    @staticmethod
+    def _stream_ptr(stream):
+        if stream is None:
+            return None
+        return C.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+    def set_inputs(self, d_layers=0, d_wind=0, stream=None):
+        """Replace the layers (device address of the packed [4][n_layers]
+        blocks) and/or the wind speeds (device address of [n]) from device
+        memory, on `stream` (ocrt_plan_set_inputs)."""
+        rc = self.lib.ocrt_plan_set_inputs(self._handle, C.c_void_p(d_layers or None), C.c_void_p(d_wind or None),
+                                           self._stream_ptr(stream))
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_set_inputs failed with code {rc}")
+
+    def rho_device(self, d_rho, d_status=0, stream=None):
+        """(rho_I, rho_Q, rho_U) [n][3] of the last execute into device memory
+        (ocrt_plan_rho)."""
+        rc = self.lib.ocrt_plan_rho(self._handle, C.c_void_p(d_rho), C.c_void_p(d_status or None),
+                                    self._stream_ptr(stream))
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_rho failed with code {rc}")
+
+    def adjoint_layers(self, d_grad_out, d_grad_layers=0, d_grad_wind=0, stream=None):
+        """Adjoint with device-memory operands and the per-layer chain rule on
+        the device (ocrt_plan_adjoint_layers)."""
+        rc = self.lib.ocrt_plan_adjoint_layers(self._handle, C.c_void_p(d_grad_out),
+                                               C.c_void_p(d_grad_layers or None), C.c_void_p(d_grad_wind or None),
+                                               self._stream_ptr(stream))
+        if rc != 0:
+            raise OcrtError(f"ocrt_plan_adjoint_layers failed with code {rc}")
 
     def export_nodes(self):
         """Keep the per-node TOA and BOA Fourier fields of the following

@@ -16,9 +16,11 @@ SOURCES = ["plan.hip", "setup.hip", "sos.hip", "ipss.hip", "adjoint.hip", "lut.h
 ARCH = os.environ.get("OCRT_OFFLOAD_ARCH", "gfx950")
 
 # -packed-fp32-ops off: hipcc -O3 must not emit v_pk_{add,mul,fma}_f32 on this target.
+# Multiply-adds fuse everywhere, as with -ffp-contract=fast, except in the
+# blocks that say `#pragma clang fp contract(off)`.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-fno-gpu-rdc", "-ffp-contract=fast", "-Wno-unused-result"]
+         "-fno-gpu-rdc", "-ffp-contract=fast-honor-pragmas", "-Wno-unused-result"]
 
 
 def build(verbose=False, jobs=4):

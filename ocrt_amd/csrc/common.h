@@ -57,6 +57,8 @@ struct DevInst {
     long long off_tsrc;  // pool_greek: top source [3][tsrc_mmax + 1][n] (downward I, Q, U per mode)
     long long off_rtop;  // pool_rtop (top_kind 2): [(mmax+1)][n out][n in][9] water-side Fourier kernels
     float jscale, jinv;  // mixed form: the scattering source is kept in fp16 as J * jscale (a power of two)
+    int aer_able;        // a phase matrix is given (aerosol scattering possible with a positive column)
+    int input_bad;       // inputs replaced from device memory broke the plan's structure: status OCRT_ERR_ARGS
 };
 
 // Whether the rough surface's direct-beam seed enters the order-1 field

@@ -260,6 +260,7 @@ struct ocrt_plan {
     // u_{1+2e}, ... for interval e), the e - 1 recomputed fields of a segment,
     // per-instance accumulators
     int adjoint = 0;
+    int dev_inputs = 0;               // OCRT_PLAN_DEVICE_INPUTS: no shared surface kernels
     int fwd_ready = 0;                // the last execute left this plan's checkpoints intact
     int ckpt_every = 1;
     void *At = nullptr;
@@ -383,7 +384,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     if (!in || n_inst < 1 || !out || (precision != OCRT_PREC_FP64 && precision != OCRT_PREC_MIXED))
         return OCRT_ERR_ARGS;
     if (flags & ~(OCRT_PLAN_ONE_PER_LAUNCH | OCRT_PLAN_ADJOINT | OCRT_PLAN_CKPT_SPARSE | OCRT_PLAN_AUTOTUNE |
-                  OCRT_PLAN_GRAPH))
+                  OCRT_PLAN_GRAPH | OCRT_PLAN_DEVICE_INPUTS))
         return OCRT_ERR_ARGS;
     const bool adjoint = (flags & OCRT_PLAN_ADJOINT) != 0;
     // the adjoint runs the fp64 form and needs the plane-parallel solve (no IPSS)
@@ -400,6 +401,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     p->autotune = (flags & OCRT_PLAN_AUTOTUNE) ? 1 : 0;
     p->use_graph = (flags & OCRT_PLAN_GRAPH) && !adjoint ? 1 : 0;
     p->adjoint = adjoint ? 1 : 0;
+    p->dev_inputs = (flags & OCRT_PLAN_DEVICE_INPUTS) ? 1 : 0;
     const bool mixed = precision == OCRT_PREC_MIXED;
     const int Palign = mixed ? 32 : 16;
     p->inst.resize(n_inst);
@@ -416,6 +418,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         I.mus = cos(d.sza_deg * OCRT_PI / 180.0);
         I.muv = cos(d.vza_deg * OCRT_PI / 180.0);
         I.aer = has_aerosol(d) ? 1 : 0;
+        I.aer_able = (d.L_max > 0 && d.greek) ? 1 : 0;
         I.layered = d.layers ? 1 : 0;
         I.water = d.water_tau > 0.0 ? 1 : 0;
         I.wtau = d.water_tau; I.womega_w = d.water_omega_w; I.womega_p = d.water_omega_p;
@@ -631,9 +634,10 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             I.off_T = T; T += h.T_n;
             if (I.surf == 2) {
                 // share the surface kernels with an earlier instance of this chunk
-                // that has the same ring, sun and surface
+                // that has the same ring, sun and surface (not when the wind
+                // speeds may change later)
                 int owner = -1;
-                for (int jj = std::max(c.i0, ii - kShareWindow); jj < ii && owner < 0; ++jj) {
+                for (int jj = std::max(c.i0, ii - kShareWindow); jj < ii && owner < 0 && !p->dev_inputs; ++jj) {
                     const HostInst &g = p->inst[jj];
                     const DevInst &J = c.hinst[jj - c.i0];
                     if (J.surf == 2 && J.surf_owner && J.n == I.n && J.mmax == I.mmax && J.mus == I.mus &&
@@ -1305,10 +1309,10 @@ static int execute_eager(ocrt_plan *p, hipStream_t s)
 {
     p->gemm_launches = 0.0;
     p->ev_used = 0;
-    for (Chunk &c : p->chunks) {
+    // chunks follow one another in stream order (they share the pools); the
+    // host does not wait
+    for (Chunk &c : p->chunks)
         if (forward_chunk(p, c, s) != 0) return OCRT_ERR_DEVICE;
-        HIPCHK(hipStreamSynchronize(s));
-    }
     // a single-chunk adjoint plan keeps the checkpoints of this forward pass
     p->fwd_ready = (p->adjoint && p->chunks.size() == 1) ? 1 : 0;
     return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
@@ -1435,9 +1439,9 @@ extern "C" int ocrt_plan_adjoint(ocrt_plan *p, const double *grad_out, double *g
     for (Chunk &c : p->chunks) {
         if (!p->fwd_ready && forward_chunk(p, c, s) != 0) return OCRT_ERR_DEVICE;
         if (backward_chunk(p, c, s) != 0) return OCRT_ERR_DEVICE;
-        HIPCHK(hipStreamSynchronize(s));
     }
     p->fwd_ready = (p->chunks.size() == 1) ? 1 : 0;
+    HIPCHK(hipStreamSynchronize(s));   // host outputs: the call returns when they are complete
     if (!grad_levels && !grad_wind) return OCRT_OK;
     std::vector<double> g(p->grad_n);
     HIPCHK(hipMemcpy(g.data(), p->grad, (size_t)p->grad_n * 8, hipMemcpyDeviceToHost));
@@ -1459,6 +1463,153 @@ extern "C" int ocrt_plan_adjoint(ocrt_plan *p, const double *grad_out, double *g
         off += 3LL * L;
     }
     return OCRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device-memory inputs and outputs (callers whose data already lives on the
+// GPU).  Every copy and kernel goes on the caller's stream.
+
+// New layer optical depths and wind speeds of one chunk's instances: the
+// column totals the medium kernels read, and the checks the host-side
+// validation makes at plan creation, against the structure the plan was made
+// with.
+__global__ void refresh_inputs(DevInst *__restrict__ inst, int n_inst, const double *__restrict__ layers,
+                               const double *__restrict__ wind, int set_layers)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    DevInst &I = inst[i];
+    int bad = 0;
+    if (set_layers && I.layered) {
+        const int nt = I.nt;
+        const double *R = layers + I.off_layers, *A = R + nt, *W = A + nt, *G = W + nt;
+        double sr = 0.0, sa = 0.0;
+        for (int l = 0; l < nt; ++l) {
+            if (!(R[l] >= 0.0 && A[l] >= 0.0 && G[l] >= 0.0 && W[l] >= 0.0 && W[l] <= 1.0)) bad = 1;
+            if (!(R[l] + A[l] + G[l] > 0.0) || !isfinite(R[l] + A[l] + G[l])) bad = 1;
+            sr += R[l];
+            sa += A[l];
+        }
+        // an aerosol column appearing or vanishing changes the operator parts
+        if (I.aer ? !(sa > 0.0) : (I.aer_able && sa > 0.0)) bad = 1;
+        I.tauR = sr;
+        I.taua = I.aer ? sa : 0.0;
+    }
+    if (wind) {
+        const double v = wind[I.gidx];
+        if (!isfinite(v)) bad = 1;
+        // rough kernels need a positive wind; a flat sea was made from a calm one
+        if (I.surface_kind == OCRT_SURFACE_BLACK_FRESNEL_OCEAN && (I.surf == 2) != (v > 0.0)) bad = 1;
+        I.wind = v;
+    }
+    I.input_bad = bad;
+}
+
+extern "C" int ocrt_plan_set_inputs(ocrt_plan *p, const double *d_layers, const double *d_wind, void *stream)
+{
+    if (!p) return OCRT_ERR_ARGS;
+    if (d_wind && !p->dev_inputs) return OCRT_ERR_ARGS;
+    long long total = 0;
+    for (const HostInst &h : p->inst) {
+        if (d_layers && !h.d.layered) return OCRT_ERR_ARGS;
+        total += h.d.layered ? 4LL * h.d.nt : 0;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    // every instance layered: the plan's layer pool is the packed concatenation
+    if (d_layers && total > 0)
+        HIPCHK(hipMemcpyAsync(p->layers, d_layers, (size_t)total * 8, hipMemcpyDeviceToDevice, s));
+    for (Chunk &c : p->chunks) {
+        const int ni = c.i1 - c.i0;
+        hipLaunchKernelGGL(refresh_inputs, dim3((unsigned)((ni + 63) / 64)), dim3(64), 0, s, c.dinst, ni, p->layers,
+                           d_wind, d_layers ? 1 : 0);
+    }
+    p->fwd_ready = 0;
+    return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
+}
+
+__global__ void rho_out(const ocrt_result_t *__restrict__ res, int n, double *__restrict__ rho, int *__restrict__ status)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ocrt_result_t &r = res[i];
+    const bool ok = r.status == OCRT_OK;
+    const double nan = __builtin_nan("");
+    rho[3 * i] = ok ? r.rho_I : nan;
+    rho[3 * i + 1] = ok ? r.rho_Q : nan;
+    rho[3 * i + 2] = ok ? r.rho_U : nan;
+    if (status) status[i] = r.status;
+}
+
+extern "C" int ocrt_plan_rho(ocrt_plan *p, double *d_rho, int *d_status, void *stream)
+{
+    if (!p || !d_rho) return OCRT_ERR_ARGS;
+    hipLaunchKernelGGL(rho_out, dim3((unsigned)((p->n_inst + 63) / 64)), dim3(64), 0, (hipStream_t)stream, p->res,
+                       p->n_inst, d_rho, d_status);
+    return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
+}
+
+// Per-layer gradients of one chunk's layered instances from the adjoint's
+// per-level accumulators, by the chain rule through the layer-to-level map
+// (rt_atm.c:343-360):
+//   h[k] = sum_{l<k} dtau_l,  dtau_l = R_l + A_l + G_l,
+//   xdel[l+1] = W_l A_l / dtau_l,  ydel[l+1] = R_l / dtau_l,
+//   level 0: (0, 1) with Rayleigh present, else (W_0, 0).
+// The level gradient of h is the one ocrt_plan_adjoint reports, d/dh[k] =
+// gH[k] + gD[k-1] - gD[k]; the sums over the levels below a layer run from
+// the bottom up.  Contraction is off so the arithmetic is the plain sequence
+// of roundings the host-side map (ocrt_amd/adjoint.py) performs.
+__global__ void layer_grads(const DevInst *__restrict__ inst, int n_inst, const double *__restrict__ layers,
+                            const double *__restrict__ grad, double *__restrict__ gl, double *__restrict__ gw)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    const DevInst &I = inst[i];
+    const int nt = I.nt, L = nt + 1;
+    const double *gH = grad + I.off_grad, *gX = gH + L, *gY = gX + L, *gD = gY + L;
+    if (gw) gw[I.gidx] = gH[4 * L - 1];
+    if (!gl) return;
+    const double *R = layers + I.off_layers, *A = R + nt, *W = A + nt, *G = W + nt;
+    double *oR = gl + I.off_layers, *oA = oR + nt, *oW = oA + nt, *oG = oW + nt;
+    const bool aer = I.aer != 0;
+    double suf = 0.0, rsum = 0.0;
+    for (int l = nt - 1; l >= 0; --l) {
+        const int k = l + 1;
+        const double dh = gH[k] + gD[k - 1] - (k < nt ? gD[k] : 0.0);
+        suf = suf + dh;   // d/d dtau_l: every level below layer l
+        const double dt = R[l] + A[l] + G[l];
+        const double gx1 = aer ? gX[k] : 0.0, gy1 = gY[k];
+        const double x1 = aer ? W[l] * A[l] / dt : 0.0, y1 = R[l] / dt;
+        const double common = suf - (gx1 * x1 + gy1 * y1) / dt;
+        oR[l] = common + gy1 / dt;
+        oA[l] = common + (aer ? gx1 * W[l] / dt : 0.0);
+        oW[l] = aer ? gx1 * A[l] / dt : 0.0;
+        oG[l] = common;
+        rsum += R[l];
+    }
+    // without Rayleigh the top level's fractions are (W_0, 0)
+    if (aer && !(rsum > 0.0)) oW[0] += gX[0];
+}
+
+extern "C" int ocrt_plan_adjoint_layers(ocrt_plan *p, const double *d_grad_out, double *d_grad_layers,
+                                        double *d_grad_wind, void *stream)
+{
+    if (!p || !d_grad_out || !p->adjoint) return OCRT_ERR_ARGS;
+    if (d_grad_layers)
+        for (const HostInst &h : p->inst)
+            if (!h.d.layered) return OCRT_ERR_ARGS;
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(hipMemcpyAsync(p->gout, d_grad_out, (size_t)p->n_inst * 3 * 8, hipMemcpyDeviceToDevice, s));
+    for (Chunk &c : p->chunks) {
+        if (!p->fwd_ready && forward_chunk(p, c, s) != 0) return OCRT_ERR_DEVICE;
+        if (backward_chunk(p, c, s) != 0) return OCRT_ERR_DEVICE;
+        const int ni = c.i1 - c.i0;
+        if (d_grad_layers || d_grad_wind)
+            hipLaunchKernelGGL(layer_grads, dim3((unsigned)((ni + 63) / 64)), dim3(64), 0, s, c.dinst, ni, p->layers,
+                               p->grad, d_grad_layers, d_grad_wind);
+    }
+    p->fwd_ready = (p->chunks.size() == 1) ? 1 : 0;
+    return hipGetLastError() == hipSuccess ? OCRT_OK : OCRT_ERR_DEVICE;
 }
 
 extern "C" int ocrt_plan_results(ocrt_plan *p, ocrt_result_t *out, double *toa_modes, int *orders)

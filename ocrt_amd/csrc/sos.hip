@@ -934,6 +934,7 @@ __global__ void finalize(const DevInst *__restrict__ inst, int n_inst, const Ite
     }
     if (!(o.rho_I == o.rho_I) || !(o.rho_Q == o.rho_Q) || !(o.rho_U == o.rho_U) || !isfinite(o.rho_I))
         status = -2;
+    if (I.input_bad) status = OCRT_ERR_ARGS;
     o.status = status;
     o.n_ring = I.n;
     res[i] = o;

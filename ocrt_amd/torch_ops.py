@@ -11,13 +11,39 @@ absorption tau, top down.  `base` is a list of B keyword dicts for
 ocrt_amd.abi.make_instance (geometry, surface, Greek coefficients, streams,
 modes, ...); their layers and wind are replaced by the tensors.  The solve
 runs the FP64 form with IPSS off (the adjoint's domain).
+
+Two ways through the boundary:
+
+* device_io (the default for CUDA tensors): the tensors stay on the device.
+  A plan per shape signature (base descriptors, batch, layers) is made once,
+  from the first call's values, and kept; every call hands the plan the
+  layers' and winds' data_ptr()s (ocrt_plan_set_inputs), runs it on
+  torch.cuda.current_stream(), and has the results and the gradients written
+  straight into device tensors (ocrt_plan_rho, ocrt_plan_adjoint_layers).
+  Nothing is copied to the host after the plan exists and nothing
+  synchronises.  The plan keeps the structure it was made with: an instance
+  whose aerosol column appears or vanishes, or whose sea turns calm or rough,
+  comes back as NaN (status OCRT_ERR_ARGS) -- make a plan for that structure
+  by calling with it first, or use the host path.
+* host path (device_io=False): descriptors from host copies and a fresh plan
+  per call, results and gradients through host memory.
+
+A cached plan serves every forward of its signature; a backward whose
+forward is no longer the plan's last one reruns that forward from its saved
+inputs before the adjoint, so interleaved forwards and repeated backwards
+(retain_graph) give the gradients of their own inputs.
 """
+from collections import OrderedDict
+
 import numpy as np
 import torch
 
 from . import Plan
 from .abi import make_instance
 from .adjoint import layer_gradients
+
+_CACHE = OrderedDict()
+_CACHE_MAX = 4
 
 
 def _descriptors(layers, wind, base):
@@ -31,9 +57,66 @@ def _descriptors(layers, wind, base):
     return descs
 
 
+def _hashable(v):
+    if isinstance(v, np.ndarray):
+        return ("nd", v.dtype.str, v.shape, v.tobytes())
+    if isinstance(v, (list, tuple)):
+        return tuple(_hashable(x) for x in v)
+    if isinstance(v, dict):
+        return tuple(sorted((k, _hashable(x)) for k, x in v.items()))
+    return v
+
+
+def _signature(layers, base):
+    kws = tuple(_hashable({k: v for k, v in kw.items() if k not in ("layers", "wind")}) for kw in base)
+    return (tuple(layers.shape), str(layers.device), kws)
+
+
+class _CachedPlan:
+    """An adjoint plan with device inputs, and the forward it last ran."""
+
+    def __init__(self, layers, wind, base):
+        descs = _descriptors(layers.detach().cpu().numpy(), wind.detach().cpu().numpy(), base)
+        self.plan = Plan(descs, precision="fp64", adjoint=True, device_inputs=True)
+        self.keep = descs
+        self.last = 0      # token of the forward whose state the plan holds
+        self.issued = 0
+
+    def run(self, layers, wind, stream):
+        self.plan.set_inputs(layers.data_ptr(), wind.data_ptr(), stream)
+        self.plan.execute(stream)
+        self.issued += 1
+        self.last = self.issued
+        return self.last
+
+
+def _cached_plan(layers, wind, base):
+    key = _signature(layers, base)
+    entry = _CACHE.get(key)
+    if entry is None:
+        entry = _CachedPlan(layers, wind, base)
+        _CACHE[key] = entry
+        while len(_CACHE) > _CACHE_MAX:
+            _CACHE.popitem(last=False)   # the plan lives on while a graph still holds it
+    else:
+        _CACHE.move_to_end(key)
+    return entry
+
+
 class TOAStokes(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, layers, wind, base):
+    def forward(ctx, layers, wind, base, device_io):
+        if device_io:
+            lay = layers.detach().to(torch.float64).contiguous()
+            wnd = wind.detach().to(device=lay.device, dtype=torch.float64).contiguous()
+            entry = _cached_plan(lay, wnd, base)
+            stream = torch.cuda.current_stream(lay.device)
+            token = entry.run(lay, wnd, stream)
+            out = torch.empty((lay.shape[0], 3), dtype=torch.float64, device=lay.device)
+            entry.plan.rho_device(out.data_ptr(), 0, stream)
+            ctx.entry, ctx.token, ctx.device_io = entry, token, True
+            ctx.save_for_backward(lay, wnd)
+            return out
         lay = layers.detach().to("cpu", torch.float64).numpy()
         wnd = wind.detach().to("cpu", torch.float64).numpy()
         descs = _descriptors(lay, wnd, base)
@@ -47,20 +130,35 @@ class TOAStokes(torch.autograd.Function):
         ctx.layers = lay
         ctx.aerosol = [d.L_max > 0 and d.greek for d in descs]
         ctx.keep = descs
+        ctx.device_io = False
         out = np.stack([res["rho_I"], res["rho_Q"], res["rho_U"]], axis=1)
         return torch.as_tensor(out, dtype=torch.float64, device=layers.device)
 
     @staticmethod
     def backward(ctx, grad_out):
+        if ctx.device_io:
+            lay, wnd = ctx.saved_tensors
+            entry = ctx.entry
+            stream = torch.cuda.current_stream(lay.device)
+            if entry.last != ctx.token:
+                # another forward ran on this plan since: redo this one's
+                ctx.token = entry.run(lay, wnd, stream)
+            g = grad_out.detach().to(device=lay.device, dtype=torch.float64).contiguous()
+            gl = torch.empty_like(lay)
+            gw = torch.empty_like(wnd)
+            entry.plan.adjoint_layers(g.data_ptr(), gl.data_ptr(), gw.data_ptr(), stream)
+            return gl, gw, None, None
         g = grad_out.detach().to("cpu", torch.float64).numpy()
         adj = ctx.plan.adjoint(g)
-        ctx.plan.close()
         gl = np.stack([layer_gradients(ctx.layers[b], adj["grad_h"][b], adj["grad_x"][b], adj["grad_y"][b],
                                        aerosol=bool(ctx.aerosol[b])) for b in range(len(ctx.layers))])
         dev = grad_out.device
         return (torch.as_tensor(gl, dtype=torch.float64, device=dev),
-                torch.as_tensor(adj["grad_wind"], dtype=torch.float64, device=dev), None)
+                torch.as_tensor(adj["grad_wind"], dtype=torch.float64, device=dev), None, None)
 
 
-def toa_stokes(layers, wind, base):
-    return TOAStokes.apply(layers, wind, base)
+def toa_stokes(layers, wind, base, device_io=None):
+    """TOA (rho_I, rho_Q, rho_U) [B, 3]; device_io defaults to layers.is_cuda."""
+    if device_io is None:
+        device_io = bool(layers.is_cuda)
+    return TOAStokes.apply(layers, wind, base, bool(device_io))
