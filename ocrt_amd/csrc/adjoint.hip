@@ -80,6 +80,23 @@ __global__ void adj_surface_bc(const DevInst *__restrict__ inst, const DevItem *
     surface_bc_item(I, it, P);
 }
 
+// Adjoint plans: the coefficients of one scan step over layer k at node o
+// (step_coefs: Beer-Lambert factor, the linear-source weights of the two end
+// levels and their derivatives by the layer depth), once per layer and node
+// instead of once per column and order.
+__global__ void adj_step_table(const DevInst *__restrict__ inst, Pools P)
+{
+    const DevInst &I = inst[blockIdx.y];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= I.nt) return;
+    const double *h = P.lev + I.off_lev;
+    const double *mu = P.ring + I.off_ring;
+    const double dt = fabs(h[k] - h[k + 1]);
+    double *c = P.sc + 6 * (I.off_T + (long long)k * I.n);
+    for (int o = 0; o < I.n; ++o)
+        step_coefs(dt, mu[o], c[6 * o], c[6 * o + 1], c[6 * o + 2], c[6 * o + 3], c[6 * o + 4], c[6 * o + 5]);
+}
+
 // Adjoint sweep of order `order` >= 2.  P.F holds u_o, P.J / P.J2 the
 // unscaled operator parts A_0 u_{o-1} / A_1 u_{o-1}, P.lam lambda_o.  Writes
 // Sbar_o over P.J, bbar_o over the boundary slots of P.bc, and accumulates
@@ -113,7 +130,6 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
     const double *__restrict__ J1 = reinterpret_cast<const double *>(P.J2) + off_F + col;
     const double *__restrict__ lam = P.lam + off_F + col;
     double *gX = P.grad + I.off_grad + L, *gY = gX + L, *gD = gY + L;
-    const double amu = P.ring[I.off_ring + o];
     const bool two = nparts > 1;
     auto source = [&](int k, double p0, double p1) {
         const double s0 = (kind0 == 1) ? lv.xd[k] : lv.yd[k];
@@ -161,8 +177,9 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
             const int k = hemi ? nt - t : t;
             const int kp = hemi ? k - 1 : k + 1;
             const double sp = source(kp, q0[u], q1[u]);
-            double T, al, be, dT, dal, dbe;
-            step_coefs(fabs(lv.h[k] - lv.h[kp]), amu, T, al, be, dT, dal, dbe);
+            // the step's coefficients over the layer between k and kp (table)
+            const double *cf = P.sc + 6 * (I.off_T + (long long)(k < kp ? k : kp) * n + o);
+            const double T = cf[0], al = cf[1], be = cf[2], dT = cf[3], dal = cf[4], dbe = cf[5];
             const double sbar = pending + al * rho;
             pending = be * rho;
             if (live) {
@@ -354,8 +371,9 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
             const int k = hemi ? nt - t : t;
             const int kp = hemi ? k - 1 : k + 1;
             const double sp = S(kp);
-            double T, al, be, dT, dal, dbe;
-            step_coefs(fabs(lv.h[k] - lv.h[kp]), amu, T, al, be, dT, dal, dbe);
+            // the step's coefficients over the layer between k and kp (table)
+            const double *cf = P.sc + 6 * (I.off_T + (long long)(k < kp ? k : kp) * n + o);
+            const double T = cf[0], al = cf[1], be = cf[2], dT = cf[3], dal = cf[4], dbe = cf[5];
             const double sbar = pending + al * rho;
             pending = be * rho;
             const double rk = live ? uv[u] - (rough ? rsun * seed_f(kp) : 0.0) : 0.0;

@@ -122,6 +122,8 @@ struct Pools {
     double *dR;           // adjoint: d R^m / d sigma^2, layout of R
     float *Rf;            // mixed form: fp32 copy of pool_R read by the per-order rough boundary condition
     double *lam;          // adjoint: [L][P] per item, adjoint of the current order's field
+    double *sc;           // adjoint: per layer and node the scan-step coefficients (T, al, be, dT, dal, dbe),
+                          // [nt][n][6] at 6 off_T
     double *grad;         // adjoint: per-instance accumulators (see DevInst::off_grad)
     const double *gout;  // adjoint: upstream gradient [instance][3] of (rho_I, rho_Q, rho_U)
     float *comp;          // fp32 compensation of the mixed-form order sum on the two output rows

@@ -21,6 +21,7 @@ __global__ void medium_levels(const DevInst *, Pools);
 __global__ void medium_fractions(const DevInst *, Pools);
 __global__ void medium_gas(const DevInst *, Pools);
 __global__ void medium_transmit(const DevInst *, Pools);
+__global__ void adj_step_table(const DevInst *, Pools);
 __global__ void fourier_weights(double *, int);
 template <bool DERIV>
 __global__ void coxmunk_fourier(const DevInst *, Pools, const int *, const int *, int, int);
@@ -38,6 +39,8 @@ template <int MODE>
 __global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <int NP, bool SPLIT>
 __global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
+template <int NP, int MODE>
+__global__ void gemm_x3(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <typename FT, typename JT, bool MIXED, int U, int W>
 __global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int, int);
 __global__ void extract(const DevInst *, const DevItem *, Pools, const ItemState *, ItemOut *, int);
@@ -260,6 +263,7 @@ struct ocrt_plan {
     // u_{1+2e}, ... for interval e), the e - 1 recomputed fields of a segment,
     // per-instance accumulators
     int adjoint = 0;
+    int adj_mixed = 0;                // adjoint plan with split-bf16 operator products (gemm_x3)
     int dev_inputs = 0;               // OCRT_PLAN_DEVICE_INPUTS: no shared surface kernels
     int fwd_ready = 0;                // the last execute left this plan's checkpoints intact
     int ckpt_every = 1;
@@ -387,8 +391,10 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                   OCRT_PLAN_GRAPH | OCRT_PLAN_DEVICE_INPUTS))
         return OCRT_ERR_ARGS;
     const bool adjoint = (flags & OCRT_PLAN_ADJOINT) != 0;
-    // the adjoint runs the fp64 form and needs the plane-parallel solve (no IPSS)
-    if (adjoint && precision != OCRT_PREC_FP64) return OCRT_ERR_ARGS;
+    // the adjoint keeps fp64 fields and needs the plane-parallel solve (no
+    // IPSS); with precision MIXED its operator products take the split-bf16
+    // MFMA form (gemm_x3) instead of fp64 MFMA
+    const bool adj_mixed = adjoint && precision == OCRT_PREC_MIXED;
     if (adjoint)
         for (int i = 0; i < n_inst; ++i)
             if (in[i].ipss || in[i].bottom_source || in[i].bottom_source_only || in[i].water_tau > 0.0)
@@ -396,14 +402,16 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     *out = nullptr;
     ocrt_plan *p = new ocrt_plan();
     p->n_inst = n_inst;
-    p->prec = precision;
+    p->prec = adj_mixed ? OCRT_PREC_FP64 : precision;
+    p->adj_mixed = adj_mixed ? 1 : 0;
     p->per_inst = (flags & OCRT_PLAN_ONE_PER_LAUNCH) ? 1 : 0;
     p->autotune = (flags & OCRT_PLAN_AUTOTUNE) ? 1 : 0;
     p->use_graph = (flags & OCRT_PLAN_GRAPH) && !adjoint ? 1 : 0;
     p->adjoint = adjoint ? 1 : 0;
     p->dev_inputs = (flags & OCRT_PLAN_DEVICE_INPUTS) ? 1 : 0;
-    const bool mixed = precision == OCRT_PREC_MIXED;
-    const int Palign = mixed ? 32 : 16;
+    const bool mixed = p->prec == OCRT_PREC_MIXED;
+    const bool gemm_bf = mixed || adj_mixed;   // bf16-pair operator, 32-wide K slabs, 128-level tiles
+    const int Palign = gemm_bf ? 32 : 16;
     p->inst.resize(n_inst);
     long long ring_n = 0, greek_n = 0, layers_n = 0;
     std::vector<const ocrt_gas *> gas_used;
@@ -563,7 +571,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             if (p->inst[j].op_src < 0 && same_operator(i, j)) { p->inst[i].op_src = j; break; }
     // ---- chunking: instances grouped so one chunk's working set stays bounded
     const double budget = mixed ? 48e9 : 48e9;
-    const size_t fe = mixed ? 2 : 8, je = mixed ? 2 : 8, ae = mixed ? 2 : 8;
+    const size_t fe = mixed ? 2 : 8, je = mixed ? 2 : 8, ae = gemm_bf ? 2 : 8;
     int item_global = 0;
     p->item_of_inst.resize(n_inst);
     // Adjoint plans keep the field of every ckpt-th order and recompute the
@@ -590,7 +598,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 const int K = std::max(parts, 1) * P;
                 if (!shared_op)
                     b += 3.0 * (I.lmax + 1) * (2 * n + 1) * 8 + 6.0 * (n + 1) * (2 * n + 1) * 8 +
-                         (double)Mp * K * ae * (mixed ? 2 : 1) * (adjoint ? 2 : 1);
+                         (double)Mp * K * ae * (gemm_bf ? 2 : 1) * (adjoint ? 2 : 1);
                 b += (double)L * P * (fe + je + 8) + (mixed ? 8.0 * P : 0.0) + ((mixed && m < kSplitModes) ? 2.0 * L * P : 0.0) + 6.0 * n * 8;
                 if (adjoint)   // checkpoints, segment fields, lambda, second parts
                     b += (double)L * P * 8 * ((I.max_orders - 1) / every + 1 + (every - 1) + 2);
@@ -679,7 +687,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 if (np == 0) kinds[np++] = 1;
                 it.parts = kinds[0] | (np > 1 ? (kinds[1] << 8) : 0);
                 it.K = np * Pp;
-                const int tile_lev = mixed ? kGemmLevels : kGemmLevelsF64;
+                const int tile_lev = gemm_bf ? kGemmLevels : kGemmLevelsF64;
                 it.Mp = Mp; it.tiles_m = Mp / 64; it.tiles_n = (L + tile_lev - 1) / tile_lev;
                 it.lmax = I.lmax;
                 int cls;
@@ -692,7 +700,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 } else {
                     it.off_basis = basis; basis += 3LL * (I.lmax + 1) * (2 * n + 1);
                     it.off_tab = tab; tab += 6LL * (n + 1) * (2 * n + 1);
-                    it.off_A = A; A += (long long)Mp * it.K * (mixed ? 2 : 1);
+                    it.off_A = A; A += (long long)Mp * it.K * (gemm_bf ? 2 : 1);
                     it.op_owner = 1;
                     cls = (int)class_items.size();
                     class_items.emplace_back();
@@ -702,7 +710,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 it.off_cmp = cmp; cmp += mixed ? 2LL * Pp : 0;
                 it.off_nodes = p->nodes_n;
                 p->nodes_n += 6LL * n;
-                it.fsplit = (mixed && m < kSplitModes) ? 1 : 0;
+                it.fsplit = ((mixed && m < kSplitModes) || adj_mixed) ? 1 : 0;
                 it.off_Flo = flo; flo += it.fsplit ? (long long)L * Pp : 0;
                 c.max_pack = std::max(c.max_pack, (long long)Mp * it.K);
                 const int item_local = (int)c.hitem.size();
@@ -816,6 +824,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         for (void *&q : p->seg) rc |= dmalloc(p, &q, mx_F * 8);
         rc |= dmalloc(p, &p->At, mx_A * ae);
         rc |= dmalloc(p, (void **)&P.lam, mx_F * 8);
+        rc |= dmalloc(p, (void **)&P.sc, mx_T * 6 * 8);
         rc |= dmalloc(p, (void **)&P.J2, mx_F * 8);
         rc |= dmalloc(p, (void **)&P.dR, std::max(mx_R, 1LL) * 8);
         rc |= dmalloc(p, (void **)&p->grad, p->grad_n * 8);
@@ -945,6 +954,29 @@ struct SweepShape { int U, W; };
 constexpr SweepShape kSweepShapes[] = {{8, 4}, {4, 4}, {16, 4}, {8, 2}, {4, 2}, {16, 2}};
 constexpr int kSweepShapeCount = sizeof(kSweepShapes) / sizeof(kSweepShapes[0]);
 
+// Operator product of an adjoint plan's split-bf16 form over the chunk's tile
+// list: the single-part tiles (the first two classes), then the two-part ones.
+static void launch_x3(int mode, const Chunk &c, hipStream_t s, const Pools &P, const ItemState *st, int order)
+{
+    const unsigned nt = (unsigned)c.htiles.size(), n1 = (unsigned)c.ntiles2;
+    if (n1 > 0) {
+        if (mode == kGemmMix)
+            hipLaunchKernelGGL((gemm_x3<1, kGemmMix>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st,
+                               order);
+        else
+            hipLaunchKernelGGL((gemm_x3<1, kGemmParts>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st,
+                               order);
+    }
+    if (nt > n1) {
+        if (mode == kGemmMix)
+            hipLaunchKernelGGL((gemm_x3<2, kGemmMix>), dim3(nt - n1), dim3(256), 0, s, c.dinst, c.ditem, P,
+                               c.dtiles + n1, st, order);
+        else
+            hipLaunchKernelGGL((gemm_x3<2, kGemmParts>), dim3(nt - n1), dim3(256), 0, s, c.dinst, c.ditem, P,
+                               c.dtiles + n1, st, order);
+    }
+}
+
 static void launch_sweep_mixed(int v, int ncols, hipStream_t s, const DevInst *di, const DevItem *dt, Pools P,
                                ItemState *st, const ColTile *tiles)
 {
@@ -978,6 +1010,8 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     hipLaunchKernelGGL(medium_fractions, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
     if (c.any_gas) hipLaunchKernelGGL(medium_gas, dim3(ni), dim3(256), 0, s, c.dinst, P);
     hipLaunchKernelGGL(medium_transmit, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
+    if (p->adjoint)
+        hipLaunchKernelGGL(adj_step_table, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
     if (!c.hcm_owner.empty()) {
         // one workgroup per pair, in launches of at most 2^22 workgroups (far
         // below 2^31 work-items each)
@@ -1008,7 +1042,7 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     // stay in one L2 and the launch stays far below 2^31 work-items
     const int pack_blocks = (int)std::min<long long>(16, (c.max_pack + 255) / 256);
     const dim3 pack_grid(xcd_item_grid(nit, pack_blocks));
-    if (mixed)
+    if (mixed || p->adj_mixed)
         hipLaunchKernelGGL((pack_operator<__hip_bfloat16, false>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem, P,
                            nit, pack_blocks);
     else
@@ -1017,8 +1051,12 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     if (p->adjoint) {
         Pools Pt = P;
         Pt.A = p->At;
-        hipLaunchKernelGGL((pack_operator<double, true>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem, Pt, nit,
-                           pack_blocks);
+        if (p->adj_mixed)
+            hipLaunchKernelGGL((pack_operator<__hip_bfloat16, true>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem,
+                               Pt, nit, pack_blocks);
+        else
+            hipLaunchKernelGGL((pack_operator<double, true>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem, Pt, nit,
+                               pack_blocks);
     }
     // adjoint plans keep u_1, u_{1+e}, u_{1+2e}, ... of every item.  With
     // e = 1 the sweeps write straight into the checkpoint of their order and
@@ -1070,8 +1108,11 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             Pg.F = field_of(it - 1);
             Ps.F = field_of(it);
             if (eg) (void)hipEventRecord(eg[0], s);
-            hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pg, c.dtiles, st,
-                               0);
+            if (p->adj_mixed)
+                launch_x3(kGemmMix, c, s, Pg, st, 0);
+            else
+                hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pg, c.dtiles,
+                                   st, 0);
             if (eg) (void)hipEventRecord(eg[1], s);
             if (es) (void)hipEventRecord(es[0], s);
             hipLaunchKernelGGL((sweep<double, double, false, 8, 4>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st,
@@ -1126,8 +1167,11 @@ static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             Pin.F = slot(sg, j - 1);
             hipLaunchKernelGGL(adj_bottom, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pin, st, o);
             hipLaunchKernelGGL(adj_surface_bc, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pin, st, o);
-            hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pin, c.dtiles, st,
-                               o);
+            if (p->adj_mixed)
+                launch_x3(kGemmMix, c, s, Pin, st, o);
+            else
+                hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pin, c.dtiles,
+                                   st, o);
             Pools Pout = P0;
             Pout.F = slot(sg, j);
             hipLaunchKernelGGL((sweep<double, double, false, 8, 4>), cols, dim3(256), 0, s, c.dinst, c.ditem, Pout, st,
@@ -1139,8 +1183,11 @@ static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             void *u_prev = slot(sg, o - 1 - base);
             Pools Pg = P0;
             Pg.F = u_prev;
-            hipLaunchKernelGGL(gemm_f64<kGemmParts>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pg, c.dtiles, st,
-                               o);
+            if (p->adj_mixed)
+                launch_x3(kGemmParts, c, s, Pg, st, o);
+            else
+                hipLaunchKernelGGL(gemm_f64<kGemmParts>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pg,
+                                   c.dtiles, st, o);
             Pools Ps = P0;
             Ps.F = u_o;
             hipLaunchKernelGGL(sweep_adj, cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st, c.dcols, ncols, o);
@@ -1148,8 +1195,11 @@ static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             Pt.A = p->At;
             Pt.F = P0.J;
             Pt.J = P0.lam;
-            hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pt, c.dtiles, st,
-                               o);
+            if (p->adj_mixed)
+                launch_x3(kGemmMix, c, s, Pt, st, o);
+            else
+                hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pt, c.dtiles,
+                                   st, o);
             hipLaunchKernelGGL(surface_adj, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pg, st, o);
         }
     }

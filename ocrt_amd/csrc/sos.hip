@@ -366,17 +366,19 @@ template <int NP, bool SPLIT> struct GemmLds {
 // (the item's K is NP P), four waves of 32 x 64.  NP is a template parameter
 // so that single-part items hold half the accumulators (higher occupancy);
 // SPLIT = false compiles out the bf16 pair field (its registers and LDS plane)
-// for tiles whose items keep a single bf16 field.
+// for tiles whose items keep a single bf16 field.  F64IN reads an fp64 field
+// (F64) instead and splits every value into its bf16 pair on the way into LDS.
 // Rows past the operator's Mp rows load its last row (their results are never
-// used).  The fp32 results, scaled by the level fractions, go to
-// out(lev, r, v) for the four rows r .. r + 3 of v, at levels < L only.
-template <int NP, bool SPLIT, typename Out>
+// used).  EPI = 0: the fp32 results, scaled by the level fractions, go to
+// out(lev, r, v) for the four rows r .. r + 3 of v; EPI = 1: the parts go
+// unscaled to out(lev, r, v0, v1) (v1 zero for one part); levels < L only.
+template <int NP, bool SPLIT, bool F64IN = false, int EPI = 0, typename Out>
 __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, const unsigned short *A,
-                                      const unsigned short *F, const unsigned short *Flo, int row0, int lev0,
-                                      GemmLds<NP, SPLIT> &S, Out out)
+                                      const unsigned short *F, const unsigned short *Flo, const double *F64,
+                                      int row0, int lev0, GemmLds<NP, SPLIT> &S, Out out)
 {
     const int Pp = it.P, L = it.L, Mp = it.Mp;
-    const bool split = SPLIT && it.fsplit != 0;
+    const bool split = SPLIT && (F64IN || it.fsplit != 0);
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
     float4_ acc[NP][2][4];
@@ -403,6 +405,26 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
         }
         const int l0 = lev0 + lr, l1 = l0 + 64;
         rb0 = rb1 = rl0 = rl1 = make_uint4(0u, 0u, 0u, 0u);
+        if constexpr (F64IN) {
+            // eight fp64 values -> their bf16 pairs (hi of the fp32 rounding,
+            // lo of the remainder), as every bf16 pair of the solver
+            auto split8 = [&](const double *src, uint4 &hi, uint4 &lo) {
+                unsigned h[8], l[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const double v = src[e];
+                    const __hip_bfloat16 bh = __float2bfloat16((float)v);
+                    const __hip_bfloat16 bl = __float2bfloat16((float)(v - (double)__bfloat162float(bh)));
+                    h[e] = __builtin_bit_cast(unsigned short, bh);
+                    l[e] = __builtin_bit_cast(unsigned short, bl);
+                }
+                hi = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+                lo = make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16), l[6] | (l[7] << 16));
+            };
+            if (l0 < L) split8(F64 + (long long)l0 * Pp + k0 + lk, rb0, rl0);
+            if (l1 < L) split8(F64 + (long long)l1 * Pp + k0 + lk, rb1, rl1);
+            return;
+        }
         if (l0 < L) {
             rb0 = *reinterpret_cast<const uint4 *>(F + (long long)l0 * Pp + k0 + lk);
             if (split) rl0 = *reinterpret_cast<const uint4 *>(Flo + (long long)l0 * Pp + k0 + lk);
@@ -465,24 +487,41 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
         }
     }
     // 16 x 16 result layout: lane holds rows 4 (lane >> 4) + i of column lane & 15
-    const int kind0 = it.parts & 0xff;
+    if constexpr (EPI == 1) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int lev = lev0 + wn + b * 16 + (lane & 15);
-        if (lev >= L) continue;
-        const float x = (float)lv.xd[lev], y = (float)lv.yd[lev];
-        const float s0 = (kind0 == 1) ? x : y;
+        for (int b = 0; b < 4; ++b) {
+            const int lev = lev0 + wn + b * 16 + (lane & 15);
+            if (lev >= L) continue;
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const int r = row0 + wm + a * 16 + 4 * (lane >> 4);
-            float4 v;
-            v.x = s0 * acc[0][a][b][0]; v.y = s0 * acc[0][a][b][1];
-            v.z = s0 * acc[0][a][b][2]; v.w = s0 * acc[0][a][b][3];
-            if constexpr (two) {
-                v.x = fmaf(y, acc[NP - 1][a][b][0], v.x); v.y = fmaf(y, acc[NP - 1][a][b][1], v.y);
-                v.z = fmaf(y, acc[NP - 1][a][b][2], v.z); v.w = fmaf(y, acc[NP - 1][a][b][3], v.w);
+            for (int a = 0; a < 2; ++a) {
+                const int r = row0 + wm + a * 16 + 4 * (lane >> 4);
+                const float4 v0 = {acc[0][a][b][0], acc[0][a][b][1], acc[0][a][b][2], acc[0][a][b][3]};
+                float4 v1 = {0.f, 0.f, 0.f, 0.f};
+                if constexpr (two) v1 = {acc[NP - 1][a][b][0], acc[NP - 1][a][b][1], acc[NP - 1][a][b][2],
+                                         acc[NP - 1][a][b][3]};
+                out(lev, r, v0, v1);
             }
-            out(lev, r, v);
+        }
+    } else {
+        const int kind0 = it.parts & 0xff;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int lev = lev0 + wn + b * 16 + (lane & 15);
+            if (lev >= L) continue;
+            const float x = (float)lv.xd[lev], y = (float)lv.yd[lev];
+            const float s0 = (kind0 == 1) ? x : y;
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int r = row0 + wm + a * 16 + 4 * (lane >> 4);
+                float4 v;
+                v.x = s0 * acc[0][a][b][0]; v.y = s0 * acc[0][a][b][1];
+                v.z = s0 * acc[0][a][b][2]; v.w = s0 * acc[0][a][b][3];
+                if constexpr (two) {
+                    v.x = fmaf(y, acc[NP - 1][a][b][0], v.x); v.y = fmaf(y, acc[NP - 1][a][b][1], v.y);
+                    v.z = fmaf(y, acc[NP - 1][a][b][2], v.z); v.w = fmaf(y, acc[NP - 1][a][b][3], v.w);
+                }
+                out(lev, r, v);
+            }
         }
     }
 }
@@ -514,7 +553,8 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(co
     const float js = I.jscale;
     const LevelView lv(P, I);
     __shared__ GemmLds<NP, SPLIT> S;
-    gemm_tile_bf16<NP, SPLIT>(it, lv, A, F, Flo, tl.tm * 64, tl.tn * kGemmLevels, S, [&](int lev, int r, float4 v) {
+    gemm_tile_bf16<NP, SPLIT>(it, lv, A, F, Flo, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
+                              [&](int lev, int r, float4 v) {
         if (r >= Pp) return;
         if constexpr (SPLIT) {
             *reinterpret_cast<float4 *>(Jf + (long long)lev * Pp + r) = v;
@@ -525,6 +565,51 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(co
             *reinterpret_cast<half4_ *>(J + (long long)lev * Pp + r) = h;
         }
     });
+}
+
+// Scattering integral of the adjoint plans' split-bf16 form: the fp64 field
+// (the forward u or the adjoint lambda) split into bf16 pairs on the way into
+// LDS, the bf16-pair operator (A, or A^T for the adjoint), three MFMAs per
+// block (A_hi F_hi + A_lo F_hi + A_hi F_lo), fp64 results:
+//   MODE kGemmMix   J[lev][r] = s0(lev) (A_0 F)[r][lev] + ydel(lev) (A_1 F)[r][lev]
+//   MODE kGemmParts J[lev][r] = (A_0 F)[r][lev], J2[lev][r] = (A_1 F)[r][lev]
+// It replaces gemm_f64 in plans that trade the fp64 operator product for
+// about 16 significant bits per factor.
+template <int NP, int MODE>
+__global__ __launch_bounds__(256, 1) void gemm_x3(const DevInst *__restrict__ inst, const DevItem *__restrict__ items,
+                                                  Pools P, const GemmTile *__restrict__ tiles,
+                                                  const ItemState *__restrict__ st, int order)
+{
+    const GemmTile tl = tiles[blockIdx.x];
+    if (tl.item < 0 || !item_runs(st[tl.item], order)) return;
+    const DevItem &it = items[tl.item];
+    const DevInst &I = inst[it.inst];
+    const int Pp = it.P;
+    const unsigned short *A = reinterpret_cast<const unsigned short *>(P.A) + it.off_A;
+    const double *F = reinterpret_cast<const double *>(P.F) + it.off_F;
+    double *J = reinterpret_cast<double *>(P.J) + it.off_F;
+    double *J2 = MODE == kGemmParts ? reinterpret_cast<double *>(P.J2) + it.off_F : nullptr;
+    const LevelView lv(P, I);
+    __shared__ GemmLds<NP, true> S;
+    if constexpr (MODE == kGemmMix) {
+        gemm_tile_bf16<NP, true, true, 0>(it, lv, A, nullptr, nullptr, F, tl.tm * 64, tl.tn * kGemmLevels, S,
+                                          [&](int lev, int r, float4 v) {
+                                              if (r >= Pp) return;
+                                              double *d = J + (long long)lev * Pp + r;
+                                              d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+                                          });
+    } else {
+        gemm_tile_bf16<NP, true, true, 1>(it, lv, A, nullptr, nullptr, F, tl.tm * 64, tl.tn * kGemmLevels, S,
+                                          [&](int lev, int r, float4 v0, float4 v1) {
+                                              if (r >= Pp) return;
+                                              double *d = J + (long long)lev * Pp + r;
+                                              d[0] = v0.x; d[1] = v0.y; d[2] = v0.z; d[3] = v0.w;
+                                              if (NP > 1) {
+                                                  double *d2 = J2 + (long long)lev * Pp + r;
+                                                  d2[0] = v1.x; d2[1] = v1.y; d2[2] = v1.z; d2[3] = v1.w;
+                                              }
+                                          });
+    }
 }
 
 // Formal solution of the current order for every column, fused with the order
@@ -960,6 +1045,14 @@ OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 16, 4)
 #undef OCRT_SWEEP_INST
 template __global__ void gemm_bf16<1, false>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                              const ItemState *, int);
+template __global__ void gemm_x3<1, kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                              const ItemState *, int);
+template __global__ void gemm_x3<2, kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                              const ItemState *, int);
+template __global__ void gemm_x3<1, kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                                const ItemState *, int);
+template __global__ void gemm_x3<2, kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                                const ItemState *, int);
 template __global__ void gemm_bf16<1, true>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                             const ItemState *, int);
 template __global__ void gemm_bf16<2, true>(const DevInst *, const DevItem *, Pools, const GemmTile *,
