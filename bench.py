@@ -187,6 +187,8 @@ def main():
     pm.execute()            # autotuning: one warm-up and one execute per sweep shape
     sync()
     tuning = pm.tuning()
+    gemm_tuning = pm.gemm_tuning()
+    log(f"GEMM tile shape per operator class: {gemm_tuning}")
     log(f"autotune: {tuning}")
     pm.profile(True)
     ms_mixed = timed_steps(pm, args.steps, args.warmup, sync) * 1e3
@@ -395,6 +397,7 @@ def main():
         "cpu_baseline": cpu,
         "device_bytes": pm.device_bytes(),
         "sweep_autotune": tuning,
+        "gemm_autotune": gemm_tuning,
         "graph_replay_ms_per_step": ms_graph,
     }
     pm.close()

@@ -77,6 +77,8 @@ def library():
         lib.ocrt_plan_lut_extras.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_plan_tuning.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_plan_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.ocrt_plan_set_gemm_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.ocrt_plan_gemm_tuning.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_plan_residuals.argtypes = [C.c_void_p, C.c_void_p]
         lib.ocrt_plan_export_nodes.argtypes = [C.c_void_p]
         lib.ocrt_plan_node_fields.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
@@ -197,6 +199,25 @@ class Plan:
         ms = (C.c_double * 16)()
         self.lib.ocrt_plan_tuning(self._handle, C.byref(u), C.byref(w), ms, C.byref(n))
         return {"unroll": u.value, "waves": w.value, "ms_per_shape": list(ms[: n.value])}
+
+    GEMM_SHAPES = ((64, 128), (64, 64), (128, 64))   # operator rows x levels per tile
+
+    def set_gemm_shape(self, op_class, shape):
+        """Tile shape (index into GEMM_SHAPES) of one operator class of the
+        mixed form's scattering integral (0: one part, single field; 1: one
+        part, field pair; 2: two parts)."""
+        rc = self.lib.ocrt_plan_set_gemm_shape(self._handle, int(op_class), int(shape))
+        if rc != 0:
+            raise OcrtError(f"no GEMM class {op_class} / shape {shape}")
+
+    def gemm_tuning(self):
+        """Shape chosen per operator class and, after autotuning, the class's
+        GEMM ms per execute with every shape ([class][shape])."""
+        sh = (C.c_int * 3)()
+        ms = (C.c_double * 9)()
+        n = self.lib.ocrt_plan_gemm_tuning(self._handle, sh, ms)
+        return {"shape_of_class": [self.GEMM_SHAPES[k] for k in sh],
+                "ms": [[ms[3 * k + j] for j in range(3)] for k in range(3)] if n else None}
 
     def set_lut(self, vza_deg, raa_deg):
         """Sample every following execute on the grid vza x raa (full-grid / LUT mode)."""

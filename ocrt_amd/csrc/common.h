@@ -198,6 +198,12 @@ __host__ __device__ inline double p11_table_at(int n, const double *th, const do
 }
 
 struct GemmTile { int item, tm, tn, pad; };   // item < 0: idle slot
+// Tile shapes (operator rows x levels) of the bf16 scattering integral; the
+// plan keeps one tile list per shape and runs each operator class with the
+// shape its autotuner chose (all shapes give the same bits).
+struct GemmShape { int tm, tn; };
+constexpr GemmShape kGemmShapes[] = {{64, 128}, {64, 64}, {128, 64}};
+constexpr int kGemmShapeCount = 3;
 // Level width of a scattering-integral tile: bf16 form (gemm_bf16) and fp64
 // form (gemm_f64); the row width is 64 in both.
 constexpr int kGemmLevels = 128;

@@ -37,7 +37,7 @@ __global__ void order_one(const DevInst *, const DevItem *, Pools, ItemState *, 
 template <typename FT> __global__ void order_status(const DevInst *, const DevItem *, Pools, ItemState *, int);
 template <int MODE>
 __global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
-template <int NP, bool SPLIT>
+template <int NP, bool SPLIT, int TM, int TN>
 __global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <int NP, int MODE>
 __global__ void gemm_x3(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
@@ -147,6 +147,11 @@ struct Chunk {
     // [ntiles2, size)
     std::vector<GemmTile> htiles;
     int ntiles1 = 0, ntiles2 = 0;
+    // mixed form: the same three classes tiled with every shape of
+    // kGemmShapes, class k at [cls_off[s][k], cls_off[s][k + 1]) of list s
+    std::vector<GemmTile> htiles_s[kGemmShapeCount];
+    int cls_off[kGemmShapeCount][4] = {};
+    GemmTile *dtiles_s[kGemmShapeCount] = {};
     std::vector<ColTile> hcols;
     std::vector<IpssJob> hjobs;
     std::vector<IpssJob> hjobs_lut;   // IPSS factors of every LUT direction
@@ -250,6 +255,12 @@ struct ocrt_plan {
     int sweep_variant = 0;
     int autotune = 0, tuned = 0;
     std::vector<double> tune_ms;
+    // tile shape (kGemmShapes) of each operator class of the bf16 scattering
+    // integral: single-part single-field, single-part pair, two-part; and the
+    // autotuner's per-class GEMM time with each shape
+    int gemm_shape[3] = {0, 0, 0};
+    int gemm_tuned = 0, tune_classes = 0;
+    double gemm_ms[3][kGemmShapeCount] = {};
     // replay of the captured forward launch sequence (hipGraph)
     int use_graph = 0;
     hipGraph_t graph = nullptr;
@@ -281,20 +292,26 @@ struct ocrt_plan {
 };
 
 
-static hipEvent_t *ev_pair(ocrt_plan *p, int kind)
+// An event pair (start, stop) of kind `kind` from the plan's pool, by value:
+// growing the pool must not invalidate pairs handed out before.
+struct EvPair {
+    hipEvent_t e[2];
+    bool ok;
+};
+static EvPair ev_pair(ocrt_plan *p, int kind)
 {
     if (p->ev_used + 2 > p->ev.size()) {
         for (int q = 0; q < 64; ++q) {
             hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return EvPair{{nullptr, nullptr}, false};
             p->ev.push_back(e);
         }
         p->ev_kind.resize(p->ev.size() / 2, 0);
     }
-    hipEvent_t *e = &p->ev[p->ev_used];
+    const EvPair r{{p->ev[p->ev_used], p->ev[p->ev_used + 1]}, true};
     p->ev_kind[p->ev_used / 2] = kind;
     p->ev_used += 2;
-    return e;
+    return r;
 }
 
 static int dmalloc(ocrt_plan *p, void **ptr, size_t bytes)
@@ -744,6 +761,29 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                     for (int tn = 0; tn < c.hitem[il].tiles_n; ++tn) gt.push_back(GemmTile{il, tm, tn, 0});
             gemm_of_class[o.K > o.P ? 2 : (o.fsplit ? 1 : 0)].push_back(std::move(gt));
         }
+        if (mixed) {
+            // the bf16 form's tile list for every shape
+            for (int sh = 0; sh < kGemmShapeCount; ++sh) {
+                const int TM = kGemmShapes[sh].tm, TN = kGemmShapes[sh].tn;
+                std::vector<std::vector<GemmTile>> by_class[3];
+                for (size_t k = 0; k < class_items.size(); ++k) {
+                    const DevItem &o = c.hitem[class_items[k][0]];
+                    std::vector<GemmTile> gt;
+                    for (int tm = 0; tm < (o.Mp + TM - 1) / TM; ++tm)
+                        for (int il : class_items[k])
+                            for (int tn = 0; tn < (c.hitem[il].L + TN - 1) / TN; ++tn)
+                                gt.push_back(GemmTile{il, tm, tn, 0});
+                    by_class[o.K > o.P ? 2 : (o.fsplit ? 1 : 0)].push_back(std::move(gt));
+                }
+                std::vector<GemmTile> &lst = c.htiles_s[sh];
+                for (int k = 0; k < 3; ++k) {
+                    c.cls_off[sh][k] = (int)lst.size();
+                    const std::vector<GemmTile> tk = xcd_tile_order(by_class[k]);
+                    lst.insert(lst.end(), tk.begin(), tk.end());
+                }
+                c.cls_off[sh][3] = (int)lst.size();
+            }
+        }
         c.htiles = xcd_tile_order(gemm_of_class[0]);
         c.ntiles1 = (int)c.htiles.size();
         const std::vector<GemmTile> t1 = xcd_tile_order(gemm_of_class[1]);
@@ -892,6 +932,9 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     for (const Chunk &c : p->chunks)
         desc_bytes += al(c.hinst.size() * sizeof(DevInst)) + al(c.hitem.size() * sizeof(DevItem)) +
                       al(std::max<size_t>(c.htiles.size(), 1) * sizeof(GemmTile)) +
+                      al(std::max<size_t>(c.htiles_s[0].size(), 1) * sizeof(GemmTile)) +
+                      al(std::max<size_t>(c.htiles_s[1].size(), 1) * sizeof(GemmTile)) +
+                      al(std::max<size_t>(c.htiles_s[2].size(), 1) * sizeof(GemmTile)) +
                       al(std::max<size_t>(c.hcols.size(), 1) * sizeof(ColTile)) +
                       al(std::max<size_t>(c.hjobs.size(), 1) * sizeof(IpssJob)) +
                       al((size_t)(c.i1 - c.i0) * sizeof(int)) + al(std::max<size_t>(c.hcm_owner.size(), 1) * 4) +
@@ -914,6 +957,9 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         c.ditem = (DevItem *)put(c.hitem.data(), c.hitem.size() * sizeof(DevItem), c.hitem.size() * sizeof(DevItem));
         c.dtiles = (GemmTile *)put(c.htiles.data(), c.htiles.size() * sizeof(GemmTile),
                                    std::max<size_t>(c.htiles.size(), 1) * sizeof(GemmTile));
+        for (int sh = 0; sh < kGemmShapeCount; ++sh)
+            c.dtiles_s[sh] = (GemmTile *)put(c.htiles_s[sh].data(), c.htiles_s[sh].size() * sizeof(GemmTile),
+                                             std::max<size_t>(c.htiles_s[sh].size(), 1) * sizeof(GemmTile));
         c.dcols = (ColTile *)put(c.hcols.data(), c.hcols.size() * sizeof(ColTile),
                                  std::max<size_t>(c.hcols.size(), 1) * sizeof(ColTile));
         c.djobs = (IpssJob *)put(c.hjobs.data(), c.hjobs.size() * sizeof(IpssJob),
@@ -953,6 +999,36 @@ static void drop_graph(ocrt_plan *p);
 struct SweepShape { int U, W; };
 constexpr SweepShape kSweepShapes[] = {{8, 4}, {4, 4}, {16, 4}, {8, 2}, {4, 2}, {16, 2}};
 constexpr int kSweepShapeCount = sizeof(kSweepShapes) / sizeof(kSweepShapes[0]);
+
+// The bf16 scattering integral of one order: each operator class with its
+// tile shape; while the autotuner times the classes, an event pair brackets
+// each class's launch (kinds 2 + class).
+template <int NP, bool SP>
+static void launch_gemm_class(int sh, unsigned n, const GemmTile *tiles, const Chunk &c, hipStream_t s,
+                              const Pools &P, const ItemState *st)
+{
+    switch (sh) {
+    case 0: hipLaunchKernelGGL((gemm_bf16<NP, SP, 64, 128>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
+    case 1: hipLaunchKernelGGL((gemm_bf16<NP, SP, 64, 64>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
+    default: hipLaunchKernelGGL((gemm_bf16<NP, SP, 128, 64>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
+    }
+}
+
+static void launch_gemm_mixed(ocrt_plan *p, const Chunk &c, hipStream_t s, const Pools &P, const ItemState *st)
+{
+    for (int k = 0; k < 3; ++k) {
+        const int sh = p->gemm_shape[k];
+        const unsigned n = (unsigned)(c.cls_off[sh][k + 1] - c.cls_off[sh][k]);
+        if (n == 0) continue;
+        const GemmTile *tiles = c.dtiles_s[sh] + c.cls_off[sh][k];
+        const EvPair e = p->tune_classes ? ev_pair(p, 2 + k) : EvPair{{nullptr, nullptr}, false};
+        if (e.ok) (void)hipEventRecord(e.e[0], s);
+        if (k == 0) launch_gemm_class<1, false>(sh, n, tiles, c, s, P, st);
+        else if (k == 1) launch_gemm_class<1, true>(sh, n, tiles, c, s, P, st);
+        else launch_gemm_class<2, true>(sh, n, tiles, c, s, P, st);
+        if (e.ok) (void)hipEventRecord(e.e[1], s);
+    }
+}
 
 // Operator product of an adjoint plan's split-bf16 form over the chunk's tile
 // list: the single-part tiles (the first two classes), then the two-part ones.
@@ -1084,40 +1160,32 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     if (checkpoint(1)) return OCRT_ERR_DEVICE;
     const unsigned ntiles = (unsigned)c.htiles.size();
     for (int it = 2; it <= c.maxit; ++it) {
-        hipEvent_t *eg = p->profile ? ev_pair(p, 0) : nullptr;
-        hipEvent_t *es = p->profile ? ev_pair(p, 1) : nullptr;
+        const EvPair eg = p->profile ? ev_pair(p, 0) : EvPair{{nullptr, nullptr}, false};
+        const EvPair es = p->profile ? ev_pair(p, 1) : EvPair{{nullptr, nullptr}, false};
         if (mixed) {
             hipLaunchKernelGGL(order_status<__hip_bfloat16>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
-            if (eg) (void)hipEventRecord(eg[0], s);
-            if (c.ntiles1 > 0)
-                hipLaunchKernelGGL((gemm_bf16<1, false>), dim3((unsigned)c.ntiles1), dim3(256), 0, s, c.dinst,
-                                   c.ditem, P, c.dtiles, st, 0);
-            if (c.ntiles2 > c.ntiles1)
-                hipLaunchKernelGGL((gemm_bf16<1, true>), dim3((unsigned)(c.ntiles2 - c.ntiles1)), dim3(256), 0, s,
-                                   c.dinst, c.ditem, P, c.dtiles + c.ntiles1, st, 0);
-            if (ntiles > (unsigned)c.ntiles2)
-                hipLaunchKernelGGL((gemm_bf16<2, true>), dim3(ntiles - (unsigned)c.ntiles2), dim3(256), 0, s,
-                                   c.dinst, c.ditem, P, c.dtiles + c.ntiles2, st, 0);
-            if (eg) (void)hipEventRecord(eg[1], s);
-            if (es) (void)hipEventRecord(es[0], s);
+            if (eg.ok) (void)hipEventRecord(eg.e[0], s);
+            launch_gemm_mixed(p, c, s, P, st);
+            if (eg.ok) (void)hipEventRecord(eg.e[1], s);
+            if (es.ok) (void)hipEventRecord(es.e[0], s);
             launch_sweep_mixed(p->sweep_variant, ncols, s, c.dinst, c.ditem, P, st, c.dcols);
-            if (es) (void)hipEventRecord(es[1], s);
+            if (es.ok) (void)hipEventRecord(es.e[1], s);
         } else {
             hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
             Pools Pg = P, Ps = P;
             Pg.F = field_of(it - 1);
             Ps.F = field_of(it);
-            if (eg) (void)hipEventRecord(eg[0], s);
+            if (eg.ok) (void)hipEventRecord(eg.e[0], s);
             if (p->adj_mixed)
                 launch_x3(kGemmMix, c, s, Pg, st, 0);
             else
                 hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pg, c.dtiles,
                                    st, 0);
-            if (eg) (void)hipEventRecord(eg[1], s);
-            if (es) (void)hipEventRecord(es[0], s);
+            if (eg.ok) (void)hipEventRecord(eg.e[1], s);
+            if (es.ok) (void)hipEventRecord(es.e[0], s);
             hipLaunchKernelGGL((sweep<double, double, false, 8, 4>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st,
                                c.dcols, ncols, 0);
-            if (es) (void)hipEventRecord(es[1], s);
+            if (es.ok) (void)hipEventRecord(es.e[1], s);
         }
         p->gemm_launches += 1.0;
         if (checkpoint(it)) return OCRT_ERR_DEVICE;
@@ -1372,6 +1440,22 @@ static int execute_eager(ocrt_plan *p, hipStream_t s)
 // with the order loop's kernels bracketed by events; the shape with the least
 // summed time is kept.  Every shape computes the same bits, so the
 // results of the plan do not depend on the choice.
+// Summed time of each operator class's GEMM launches over the last execute
+// (the autotuner's per-class event pairs).
+static int class_times(ocrt_plan *p, double out[3])
+{
+    HIPCHK(hipDeviceSynchronize());
+    out[0] = out[1] = out[2] = 0.0;
+    for (size_t q = 0; q + 1 < p->ev_used; q += 2) {
+        const int k = p->ev_kind[q / 2];
+        if (k < 2) continue;
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev[q], p->ev[q + 1]));
+        out[k - 2] += ms;
+    }
+    return OCRT_OK;
+}
+
 static int autotune(ocrt_plan *p, hipStream_t s)
 {
     const int saved = p->profile;
@@ -1379,6 +1463,24 @@ static int autotune(ocrt_plan *p, hipStream_t s)
     p->sweep_variant = 0;
     int rc = execute_eager(p, s);
     p->profile = 1;
+    // GEMM tile shape per operator class: one execute per shape, every class
+    // timed with that shape; each class keeps its fastest
+    p->tune_classes = 1;
+    for (int sh = 0; sh < kGemmShapeCount && !rc; ++sh) {
+        for (int k = 0; k < 3; ++k) p->gemm_shape[k] = sh;
+        rc = execute_eager(p, s);
+        double t[3];
+        if (!rc) rc = class_times(p, t);
+        for (int k = 0; k < 3; ++k) p->gemm_ms[k][sh] = t[k];
+    }
+    p->tune_classes = 0;
+    for (int k = 0; k < 3; ++k) {
+        int b = 0;
+        for (int sh = 1; sh < kGemmShapeCount; ++sh)
+            if (p->gemm_ms[k][sh] < p->gemm_ms[k][b]) b = sh;
+        p->gemm_shape[k] = b;
+    }
+    p->gemm_tuned = 1;
     p->tune_ms.assign(kSweepShapeCount, 0.0);
     int best = 0;
     for (int v = 0; v < kSweepShapeCount && !rc; ++v) {
@@ -1478,6 +1580,26 @@ extern "C" int ocrt_plan_tuning(const ocrt_plan *p, int *sweep_unroll, int *swee
     if (ms_per_shape && p->tuned)
         for (int v = 0; v < kSweepShapeCount; ++v) ms_per_shape[v] = p->tune_ms[v];
     return OCRT_OK;
+}
+
+extern "C" int ocrt_plan_set_gemm_shape(ocrt_plan *p, int op_class, int shape)
+{
+    if (!p || op_class < 0 || op_class > 2 || shape < 0 || shape >= kGemmShapeCount) return OCRT_ERR_ARGS;
+    drop_graph(p);   // a captured graph holds the previous shape's launches
+    p->gemm_shape[op_class] = shape;
+    return OCRT_OK;
+}
+
+extern "C" int ocrt_plan_gemm_tuning(const ocrt_plan *p, int *shape_of_class, double *ms)
+{
+    if (!p) return OCRT_ERR_ARGS;
+    for (int k = 0; k < 3; ++k) {
+        if (shape_of_class) shape_of_class[k] = p->gemm_shape[k];
+        if (ms)
+            for (int sh = 0; sh < kGemmShapeCount; ++sh)
+                ms[k * kGemmShapeCount + sh] = p->gemm_tuned ? p->gemm_ms[k][sh] : 0.0;
+    }
+    return p->gemm_tuned ? kGemmShapeCount : 0;
 }
 
 extern "C" int ocrt_plan_adjoint(ocrt_plan *p, const double *grad_out, double *grad_levels, double *grad_wind,
@@ -1737,10 +1859,12 @@ extern "C" int ocrt_plan_kernel_times(ocrt_plan *p, double *gemm_ms, int *gemm_l
     double t[2] = {0.0, 0.0};
     int cnt[2] = {0, 0};
     for (size_t q = 0; q + 1 < p->ev_used; q += 2) {
+        const int k = p->ev_kind[q / 2];
+        if (k > 1) continue;   // per-class GEMM pairs of the autotuner
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, p->ev[q], p->ev[q + 1]));
-        t[p->ev_kind[q / 2]] += ms;
-        cnt[p->ev_kind[q / 2]] += 1;
+        t[k] += ms;
+        cnt[k] += 1;
     }
     if (gemm_ms) *gemm_ms = t[0];
     if (gemm_launches) *gemm_launches = cnt[0];

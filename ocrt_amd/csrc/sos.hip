@@ -356,97 +356,117 @@ __device__ inline bf16x8_ lds_frag(const unsigned short *p)
 
 // LDS staging of one K slab of a tile: [part][hi, lo][row][k] of the operator
 // and [hi, lo][level][k] of the field, rows padded to 40 entries.
-template <int NP, bool SPLIT> struct GemmLds {
-    unsigned short A[NP][2][64][40];
-    unsigned short B[SPLIT ? 2 : 1][kGemmLevels][40];
+template <int NP, bool SPLIT, int TM = 64, int TN = kGemmLevels> struct GemmLds {
+    unsigned short A[NP][2][TM][40];
+    unsigned short B[SPLIT ? 2 : 1][TN][40];
 };
 
-// One tile of the bf16 scattering integral: operator rows row0 .. row0 + 63
-// and levels lev0 .. lev0 + kGemmLevels - 1 of one item with NP operator parts
-// (the item's K is NP P), four waves of 32 x 64.  NP is a template parameter
-// so that single-part items hold half the accumulators (higher occupancy);
-// SPLIT = false compiles out the bf16 pair field (its registers and LDS plane)
-// for tiles whose items keep a single bf16 field.  F64IN reads an fp64 field
-// (F64) instead and splits every value into its bf16 pair on the way into LDS.
+// One tile of the bf16 scattering integral: operator rows row0 .. row0 + TM - 1
+// and levels lev0 .. lev0 + TN - 1 of one item with NP operator parts (the
+// item's K is NP P), four waves of TM/2 x TN/2.  The tile shapes (64 x 128,
+// 64 x 64, 128 x 64) give the same bits: every output takes the same K slabs
+// in the same order.  NP is a template parameter so that single-part items
+// hold half the accumulators (higher occupancy); SPLIT = false compiles out
+// the bf16 pair field (its registers and LDS plane) for tiles whose items keep
+// a single bf16 field.  F64IN reads an fp64 field (F64) instead and splits
+// every value into its bf16 pair on the way into LDS.
 // Rows past the operator's Mp rows load its last row (their results are never
 // used).  EPI = 0: the fp32 results, scaled by the level fractions, go to
 // out(lev, r, v) for the four rows r .. r + 3 of v; EPI = 1: the parts go
 // unscaled to out(lev, r, v0, v1) (v1 zero for one part); levels < L only.
-template <int NP, bool SPLIT, bool F64IN = false, int EPI = 0, typename Out>
+template <int NP, bool SPLIT, bool F64IN = false, int EPI = 0, int TM = 64, int TN = kGemmLevels, typename Out>
 __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, const unsigned short *A,
                                       const unsigned short *F, const unsigned short *Flo, const double *F64,
-                                      int row0, int lev0, GemmLds<NP, SPLIT> &S, Out out)
+                                      int row0, int lev0, GemmLds<NP, SPLIT, TM, TN> &S, Out out)
 {
+    constexpr int AB = TM / 32, BB = TN / 32;   // 16 x 16 blocks per wave along rows and levels
+    constexpr int AR = TM / 64, FR = TN / 64;   // operator rows and field levels staged per thread
     const int Pp = it.P, L = it.L, Mp = it.Mp;
     const bool split = SPLIT && (F64IN || it.fsplit != 0);
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
-    float4_ acc[NP][2][4];
+    const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
+    float4_ acc[NP][AB][BB];
 #pragma unroll
     for (int q = 0; q < NP; ++q)
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < AB; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[q][a][b] = (float4_){0.f, 0.f, 0.f, 0.f};
-    // global -> LDS staging: one 8-wide k chunk of one operator row per thread
-    // and part, two of the field (levels lr and lr + 64)
+            for (int b = 0; b < BB; ++b) acc[q][a][b] = (float4_){0.f, 0.f, 0.f, 0.f};
+    // global -> LDS staging: one 8-wide k chunk of operator rows lr + 64 j per
+    // thread and part, and of field levels lr + 64 j
     const int lr = t >> 2, lk = (t & 3) * 8;
     const long long slab = (long long)Mp * Pp;
     constexpr bool two = NP > 1;
-    const int arow = min(row0 + lr, Mp - 1);
-    uint4 ra0h, ra0l, ra1h, ra1l, rb0, rb1, rl0, rl1;
-    auto load_slab = [&](int k0) {
-        const long long ro = (long long)arow * Pp + k0 + lk;
-        ra0h = *reinterpret_cast<const uint4 *>(A + ro);
-        ra0l = *reinterpret_cast<const uint4 *>(A + slab + ro);
-        if constexpr (two) {
-            ra1h = *reinterpret_cast<const uint4 *>(A + 2 * slab + ro);
-            ra1l = *reinterpret_cast<const uint4 *>(A + 3 * slab + ro);
-        }
-        const int l0 = lev0 + lr, l1 = l0 + 64;
-        rb0 = rb1 = rl0 = rl1 = make_uint4(0u, 0u, 0u, 0u);
+    // staged operator chunks a<part><hi/lo><row block> and field chunks
+    // rb/rl<level block>, as named registers (no indexed arrays, which the
+    // compiler would place in scratch)
+    uint4 a0h0, a0l0, a1h0, a1l0, a0h1, a0l1, a1h1, a1l1, rb0, rb1, rl0, rl1;
+    auto ld = [](const unsigned short *q) { return *reinterpret_cast<const uint4 *>(q); };
+    auto load_field = [&](int l, int k0, uint4 &hi, uint4 &lo) {
+        hi = lo = make_uint4(0u, 0u, 0u, 0u);
+        if (l >= L) return;
         if constexpr (F64IN) {
             // eight fp64 values -> their bf16 pairs (hi of the fp32 rounding,
             // lo of the remainder), as every bf16 pair of the solver
-            auto split8 = [&](const double *src, uint4 &hi, uint4 &lo) {
-                unsigned h[8], l[8];
+            const double *src = F64 + (long long)l * Pp + k0 + lk;
+            unsigned h[8], w[8];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const double v = src[e];
-                    const __hip_bfloat16 bh = __float2bfloat16((float)v);
-                    const __hip_bfloat16 bl = __float2bfloat16((float)(v - (double)__bfloat162float(bh)));
-                    h[e] = __builtin_bit_cast(unsigned short, bh);
-                    l[e] = __builtin_bit_cast(unsigned short, bl);
-                }
-                hi = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
-                lo = make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16), l[6] | (l[7] << 16));
-            };
-            if (l0 < L) split8(F64 + (long long)l0 * Pp + k0 + lk, rb0, rl0);
-            if (l1 < L) split8(F64 + (long long)l1 * Pp + k0 + lk, rb1, rl1);
-            return;
-        }
-        if (l0 < L) {
-            rb0 = *reinterpret_cast<const uint4 *>(F + (long long)l0 * Pp + k0 + lk);
-            if (split) rl0 = *reinterpret_cast<const uint4 *>(Flo + (long long)l0 * Pp + k0 + lk);
-        }
-        if (l1 < L) {
-            rb1 = *reinterpret_cast<const uint4 *>(F + (long long)l1 * Pp + k0 + lk);
-            if (split) rl1 = *reinterpret_cast<const uint4 *>(Flo + (long long)l1 * Pp + k0 + lk);
+            for (int e = 0; e < 8; ++e) {
+                const double v = src[e];
+                const __hip_bfloat16 bh = __float2bfloat16((float)v);
+                const __hip_bfloat16 bl = __float2bfloat16((float)(v - (double)__bfloat162float(bh)));
+                h[e] = __builtin_bit_cast(unsigned short, bh);
+                w[e] = __builtin_bit_cast(unsigned short, bl);
+            }
+            hi = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+            lo = make_uint4(w[0] | (w[1] << 16), w[2] | (w[3] << 16), w[4] | (w[5] << 16), w[6] | (w[7] << 16));
+        } else {
+            hi = ld(F + (long long)l * Pp + k0 + lk);
+            if (split) lo = ld(Flo + (long long)l * Pp + k0 + lk);
         }
     };
-    auto store_slab = [&]() {
-        *reinterpret_cast<uint4 *>(&S.A[0][0][lr][lk]) = ra0h;
-        *reinterpret_cast<uint4 *>(&S.A[0][1][lr][lk]) = ra0l;
+    auto load_slab = [&](int k0) {
+        const long long ro0 = (long long)min(row0 + lr, Mp - 1) * Pp + k0 + lk;
+        a0h0 = ld(A + ro0);
+        a0l0 = ld(A + slab + ro0);
         if constexpr (two) {
-            *reinterpret_cast<uint4 *>(&S.A[1][0][lr][lk]) = ra1h;
-            *reinterpret_cast<uint4 *>(&S.A[1][1][lr][lk]) = ra1l;
+            a1h0 = ld(A + 2 * slab + ro0);
+            a1l0 = ld(A + 3 * slab + ro0);
         }
-        *reinterpret_cast<uint4 *>(&S.B[0][lr][lk]) = rb0;
-        *reinterpret_cast<uint4 *>(&S.B[0][lr + 64][lk]) = rb1;
+        if constexpr (AR > 1) {
+            const long long ro1 = (long long)min(row0 + lr + 64, Mp - 1) * Pp + k0 + lk;
+            a0h1 = ld(A + ro1);
+            a0l1 = ld(A + slab + ro1);
+            if constexpr (two) {
+                a1h1 = ld(A + 2 * slab + ro1);
+                a1l1 = ld(A + 3 * slab + ro1);
+            }
+        }
+        load_field(lev0 + lr, k0, rb0, rl0);
+        if constexpr (FR > 1) load_field(lev0 + lr + 64, k0, rb1, rl1);
+    };
+    auto st = [](unsigned short *q, uint4 v) { *reinterpret_cast<uint4 *>(q) = v; };
+    auto store_slab = [&]() {
+        st(&S.A[0][0][lr][lk], a0h0);
+        st(&S.A[0][1][lr][lk], a0l0);
+        if constexpr (two) {
+            st(&S.A[1][0][lr][lk], a1h0);
+            st(&S.A[1][1][lr][lk], a1l0);
+        }
+        if constexpr (AR > 1) {
+            st(&S.A[0][0][lr + 64][lk], a0h1);
+            st(&S.A[0][1][lr + 64][lk], a0l1);
+            if constexpr (two) {
+                st(&S.A[1][0][lr + 64][lk], a1h1);
+                st(&S.A[1][1][lr + 64][lk], a1l1);
+            }
+        }
+        st(&S.B[0][lr][lk], rb0);
+        if constexpr (FR > 1) st(&S.B[0][lr + 64][lk], rb1);
         if constexpr (SPLIT)
             if (split) {
-                *reinterpret_cast<uint4 *>(&S.B[SPLIT ? 1 : 0][lr][lk]) = rl0;
-                *reinterpret_cast<uint4 *>(&S.B[SPLIT ? 1 : 0][lr + 64][lk]) = rl1;
+                st(&S.B[SPLIT ? 1 : 0][lr][lk], rl0);
+                if constexpr (FR > 1) st(&S.B[SPLIT ? 1 : 0][lr + 64][lk], rl1);
             }
     };
     // fragment of a 16 x 32 block: lane l holds row (l & 15), k 8 (l >> 4) .. + 7
@@ -454,11 +474,11 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     auto mfma_part = [&](auto part) {
         constexpr int q = decltype(part)::value;
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
+        for (int a = 0; a < AB; ++a) {
             const bf16x8_ hi = lds_frag(&S.A[q][0][wm + a * 16 + fr][fk]);
             const bf16x8_ lo = lds_frag(&S.A[q][1][wm + a * 16 + fr][fk]);
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
+            for (int b = 0; b < BB; ++b) {
                 const bf16x8_ fb = lds_frag(&S.B[0][wn + b * 16 + fr][fk]);
                 if constexpr (SPLIT)
                     if (split) {
@@ -489,11 +509,11 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     // 16 x 16 result layout: lane holds rows 4 (lane >> 4) + i of column lane & 15
     if constexpr (EPI == 1) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < BB; ++b) {
             const int lev = lev0 + wn + b * 16 + (lane & 15);
             if (lev >= L) continue;
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
+            for (int a = 0; a < AB; ++a) {
                 const int r = row0 + wm + a * 16 + 4 * (lane >> 4);
                 const float4 v0 = {acc[0][a][b][0], acc[0][a][b][1], acc[0][a][b][2], acc[0][a][b][3]};
                 float4 v1 = {0.f, 0.f, 0.f, 0.f};
@@ -505,13 +525,13 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     } else {
         const int kind0 = it.parts & 0xff;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < BB; ++b) {
             const int lev = lev0 + wn + b * 16 + (lane & 15);
             if (lev >= L) continue;
             const float x = (float)lv.xd[lev], y = (float)lv.yd[lev];
             const float s0 = (kind0 == 1) ? x : y;
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
+            for (int a = 0; a < AB; ++a) {
                 const int r = row0 + wm + a * 16 + 4 * (lane >> 4);
                 float4 v;
                 v.x = s0 * acc[0][a][b][0]; v.y = s0 * acc[0][a][b][1];
@@ -530,7 +550,7 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
 // may keep their field as a bf16 pair (the plan keeps one list per kind).
 // Single-part unsplit tiles fit 96 registers and 20 KB of LDS: five waves per
 // SIMD.
-template <int NP, bool SPLIT>
+template <int NP, bool SPLIT, int TM, int TN>
 __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(const DevInst *__restrict__ inst,
                                                  const DevItem *__restrict__ items, Pools P,
                                                  const GemmTile *__restrict__ tiles,
@@ -552,9 +572,9 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(co
     float *Jf = P.Jf + it.off_Flo;
     const float js = I.jscale;
     const LevelView lv(P, I);
-    __shared__ GemmLds<NP, SPLIT> S;
-    gemm_tile_bf16<NP, SPLIT>(it, lv, A, F, Flo, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
-                              [&](int lev, int r, float4 v) {
+    __shared__ GemmLds<NP, SPLIT, TM, TN> S;
+    gemm_tile_bf16<NP, SPLIT, false, 0, TM, TN>(it, lv, A, F, Flo, nullptr, tl.tm * TM, tl.tn * TN, S,
+                                                [&](int lev, int r, float4 v) {
         if (r >= Pp) return;
         if constexpr (SPLIT) {
             *reinterpret_cast<float4 *>(Jf + (long long)lev * Pp + r) = v;
@@ -1043,8 +1063,19 @@ OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 8, 4)
 OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 16, 2)
 OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 16, 4)
 #undef OCRT_SWEEP_INST
-template __global__ void gemm_bf16<1, false>(const DevInst *, const DevItem *, Pools, const GemmTile *,
-                                             const ItemState *, int);
+#define OCRT_GEMM_INST(NP, SP, TM, TN)                                                                     \
+    template __global__ void gemm_bf16<NP, SP, TM, TN>(const DevInst *, const DevItem *, Pools, const GemmTile *, \
+                                                       const ItemState *, int);
+OCRT_GEMM_INST(1, false, 64, 128)
+OCRT_GEMM_INST(1, true, 64, 128)
+OCRT_GEMM_INST(2, true, 64, 128)
+OCRT_GEMM_INST(1, false, 64, 64)
+OCRT_GEMM_INST(1, true, 64, 64)
+OCRT_GEMM_INST(2, true, 64, 64)
+OCRT_GEMM_INST(1, false, 128, 64)
+OCRT_GEMM_INST(1, true, 128, 64)
+OCRT_GEMM_INST(2, true, 128, 64)
+#undef OCRT_GEMM_INST
 template __global__ void gemm_x3<1, kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                               const ItemState *, int);
 template __global__ void gemm_x3<2, kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
@@ -1053,10 +1084,6 @@ template __global__ void gemm_x3<1, kGemmParts>(const DevInst *, const DevItem *
                                                 const ItemState *, int);
 template __global__ void gemm_x3<2, kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                                 const ItemState *, int);
-template __global__ void gemm_bf16<1, true>(const DevInst *, const DevItem *, Pools, const GemmTile *,
-                                            const ItemState *, int);
-template __global__ void gemm_bf16<2, true>(const DevInst *, const DevItem *, Pools, const GemmTile *,
-                                            const ItemState *, int);
 template __global__ void gemm_f64<kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                             const ItemState *, int);
 template __global__ void gemm_f64<kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,

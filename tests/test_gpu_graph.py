@@ -4,9 +4,10 @@
   every replay, the same bits as an eager plan over the same instances; the
   replays are interleaved with eager executes and synchronisations (mixed and
   FP64 forms, and a LUT plan).
-- Every launch shape of the mixed-form sweep gives the same bits, so an
-  autotuned plan equals the default one; the tuner reports the shape it chose
-  and the time of every shape it tried.
+- Every launch shape of the mixed-form sweep and every tile shape of its
+  scattering integral (per operator class) gives the same bits, so an
+  autotuned plan equals the default one; the tuner reports the shapes it
+  chose and the time of every shape it tried.
 """
 import numpy as np
 import pytest
@@ -79,6 +80,11 @@ def test_autotuned_plan_equals_default(ocrt):
     print(f"[gpu] autotune chose {info}", flush=True)
     assert (info["unroll"], info["waves"]) in SHAPES
     assert len(info["ms_per_shape"]) == len(SHAPES) and min(info["ms_per_shape"]) > 0.0
+    gt = t.gemm_tuning()
+    print(f"[gpu] GEMM tiles chosen {gt}", flush=True)
+    assert all(sh in ocrt.Plan.GEMM_SHAPES for sh in gt["shape_of_class"])
+    # every class with tiles was timed with every shape
+    assert gt["ms"] is not None and any(min(row) > 0.0 for row in gt["ms"])
     t.execute()
     same(t.results(), want)
     t.close()
@@ -105,4 +111,30 @@ def test_every_launch_shape_gives_the_same_bits(ocrt, graph):
         for _ in range(2):
             p.execute()
             same(p.results(), want)
+        p.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_every_gemm_tile_shape_gives_the_same_bits(ocrt, graph):
+    """Every tile shape of the scattering integral (64 x 128, 64 x 64,
+    128 x 64), for every operator class at once and class by class, gives the
+    results of the default shape bit for bit, over repeated executes, also
+    replayed from a graph; the batch has all three classes (modes 0..8: field
+    pairs below m = 3, Rayleigh parts up to m = 2)."""
+    descs, keep = S.config2(n_geom=6, seed=9, m_max=8)
+    a = ocrt.Plan(descs, precision="mixed")
+    a.execute()
+    want = a.results(modes=True)
+    a.close()
+    print(f"[gpu] GEMM tile shapes, {len(descs)} instances, graph={graph}", flush=True)
+    cases = [(sh, sh, sh) for sh in range(3)] + [(1, 2, 0), (2, 0, 1)]
+    for shapes in cases:
+        p = ocrt.Plan(descs, precision="mixed", graph=graph)
+        for k, sh in enumerate(shapes):
+            p.set_gemm_shape(k, sh)
+        for _ in range(2):
+            p.execute()
+            got = p.results(modes=True)
+            same(got, want)
+            assert np.array_equal(got["toa_modes"], want["toa_modes"]), shapes
         p.close()
