@@ -95,6 +95,13 @@ struct DevItem {
     long long off_cmp;    // pool_comp: [2][P] compensation of the order sum at k = 0 and k = nt
     long long off_Flo;    // pool_Flo: [L][P] low half of the bf16 field pair (fsplit items)
     long long off_nodes;  // pool_nodes (LUT or node-export plans, global over the plan): [6][n]
+    // mixed form, m <= 2 with aerosol and Rayleigh: the molecular operator
+    // (rank 3) runs outside the dense GEMM.  The operator gets 64 more rows
+    // from pbase: rows pbase .. pbase + 2 hold its input factors G, so the
+    // GEMM's projection tile yields W = G F per level (pool_W [L][4]); the
+    // sweep adds ydel (U W) with the output factors U = E C (pool_lr [3][P]).
+    int lowrank, pbase;
+    long long off_lr, off_W;
 };
 
 struct ItemState {
@@ -130,6 +137,8 @@ struct Pools {
     double *trow;         // mixed form: fp64 order sum of the two output rows (k = 0, k = nt), [2][P] per item
                           // at off_cmp; the interior rows keep an fp32 sum in pool_tot
     float *Jf;            // mixed form: fp32 source of the items that keep their field as a pair, [L][P] at off_Flo
+    float *lr;            // mixed form: output factors U of the rank-3 molecular operator, [3][P] at off_lr
+    float *W;             // mixed form: its projections W = G F per level, [L][4] at off_W
     float *Tf;            // fp32 1 - T for the mixed-form sweeps
     float *idt;           // fp32 1 / dtau per layer, indexed like lev (mixed-form sweeps)
     void *A, *F, *J;      // double or bf16/float depending on the form

@@ -22,6 +22,7 @@ __global__ void medium_fractions(const DevInst *, Pools);
 __global__ void medium_gas(const DevInst *, Pools);
 __global__ void medium_transmit(const DevInst *, Pools);
 __global__ void adj_step_table(const DevInst *, Pools);
+__global__ void lowrank_factors(const DevInst *, const DevItem *, Pools);
 __global__ void fourier_weights(double *, int);
 template <bool DERIV>
 __global__ void coxmunk_fourier(const DevInst *, Pools, const int *, const int *, int, int);
@@ -175,7 +176,7 @@ struct Chunk {
     int max_L = 0, max_P = 0, max_pairs = 0, max_tab = 0, tab_groups = 1, maxit = 1, any_rough = 0, n_jobs = 0;
     int any_gas = 0;
     long long max_pack = 0;
-    long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0, cmp = 0, flo = 0;
+    long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0, cmp = 0, flo = 0, lr = 0, wlr = 0;
 };
 
 // Mixed form: Fourier modes m < kSplitModes keep their field as a bf16 pair
@@ -640,7 +641,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     p->n_items = item_global;
     // ---- per-chunk layout
     for (Chunk &c : p->chunks) {
-        long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0, cmp = 0, flo = 0;
+        long long lev = 0, T = 0, R = 0, basis = 0, tab = 0, A = 0, F = 0, bc = 0, cmp = 0, flo = 0, lr = 0, wlr = 0;
         // items grouped by the operator they use: class of each owner item and
         // the member items of each class
         std::vector<int> class_of_item;
@@ -699,25 +700,31 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 memset(&it, 0, sizeof(it));
                 it.inst = ii - c.i0; it.m = m; it.n = n; it.nt = I.nt; it.L = L; it.P = Pp;
                 int kinds[2], np = 0;
+                // the mixed form takes the molecular operator of an item with
+                // aerosol out of the dense product (rank 3, see DevItem::lowrank)
+                it.lowrank = (mixed && m <= 2 && I.ray && I.aer) ? 1 : 0;
                 if (I.aer) kinds[np++] = 1;
-                if (m <= 2 && I.ray) kinds[np++] = 2;
+                if (m <= 2 && I.ray && !it.lowrank) kinds[np++] = 2;
                 if (np == 0) kinds[np++] = 1;
                 it.parts = kinds[0] | (np > 1 ? (kinds[1] << 8) : 0);
                 it.K = np * Pp;
                 const int tile_lev = gemm_bf ? kGemmLevels : kGemmLevelsF64;
-                it.Mp = Mp; it.tiles_m = Mp / 64; it.tiles_n = (L + tile_lev - 1) / tile_lev;
+                it.pbase = Mp;
+                const int Mop = Mp + (it.lowrank ? 64 : 0);   // operator rows, with the projection tile
+                it.Mp = Mop; it.tiles_m = Mop / 64; it.tiles_n = (L + tile_lev - 1) / tile_lev;
                 it.lmax = I.lmax;
                 int cls;
                 if (op_from >= 0) {
                     const int src = c.hinst[op_from].item0 + m;
                     const DevItem &o = c.hitem[src];
-                    it.off_basis = o.off_basis; it.off_tab = o.off_tab; it.off_A = o.off_A;
+                    it.off_basis = o.off_basis; it.off_tab = o.off_tab; it.off_A = o.off_A; it.off_lr = o.off_lr;
                     it.op_owner = 0;
                     cls = class_of_item[src];
                 } else {
                     it.off_basis = basis; basis += 3LL * (I.lmax + 1) * (2 * n + 1);
                     it.off_tab = tab; tab += 6LL * (n + 1) * (2 * n + 1);
-                    it.off_A = A; A += (long long)Mp * it.K * (gemm_bf ? 2 : 1);
+                    it.off_A = A; A += (long long)Mop * it.K * (gemm_bf ? 2 : 1);
+                    it.off_lr = lr; lr += it.lowrank ? 3LL * Pp : 0;
                     it.op_owner = 1;
                     cls = (int)class_items.size();
                     class_items.emplace_back();
@@ -725,6 +732,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 it.off_F = F; it.off_tot = F; F += (long long)L * Pp;
                 it.off_bc = bc; bc += 12LL * n;
                 it.off_cmp = cmp; cmp += mixed ? 2LL * Pp : 0;
+                it.off_W = wlr; wlr += it.lowrank ? 4LL * L : 0;
                 it.off_nodes = p->nodes_n;
                 p->nodes_n += 6LL * n;
                 it.fsplit = ((mixed && m < kSplitModes) || adj_mixed) ? 1 : 0;
@@ -793,17 +801,19 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         c.htiles.insert(c.htiles.end(), t2.begin(), t2.end());
         c.n_jobs = (int)c.hjobs.size();
         c.lev = lev; c.T = T; c.R = R; c.basis = basis; c.tab = tab; c.A = A; c.F = F; c.bc = bc;
-        c.cmp = cmp; c.flo = flo;
+        c.cmp = cmp; c.flo = flo; c.lr = lr; c.wlr = wlr;
     }
     // ---- device allocations
     long long mx_lev = 0, mx_T = 0, mx_R = 0, mx_basis = 0, mx_tab = 0, mx_A = 0, mx_F = 0, mx_bc = 0,
-              mx_cmp = 0, mx_flo = 0;
+              mx_cmp = 0, mx_flo = 0, mx_lr = 0, mx_wlr = 0;
     for (const Chunk &c : p->chunks) {
         mx_lev = std::max(mx_lev, c.lev); mx_T = std::max(mx_T, c.T); mx_R = std::max(mx_R, c.R);
         mx_basis = std::max(mx_basis, c.basis); mx_tab = std::max(mx_tab, c.tab);
         mx_A = std::max(mx_A, c.A); mx_F = std::max(mx_F, c.F); mx_bc = std::max(mx_bc, c.bc);
         mx_cmp = std::max(mx_cmp, c.cmp);
         mx_flo = std::max(mx_flo, c.flo);
+        mx_lr = std::max(mx_lr, c.lr);
+        mx_wlr = std::max(mx_wlr, c.wlr);
     }
     if (getenv("OCRT_DEBUG_PLAN"))
         fprintf(stderr, "ocrt plan: %zu chunks items %d ring %lld greek %lld lev %lld T %lld R %lld basis %lld tab %lld "
@@ -854,6 +864,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     if (mixed) rc |= dmalloc(p, (void **)&P.trow, std::max(mx_cmp, 1LL) * 8);
     if (mixed) rc |= dmalloc(p, (void **)&P.Flo, std::max(mx_flo, 1LL) * 2);
     if (mixed) rc |= dmalloc(p, (void **)&P.Jf, std::max(mx_flo, 1LL) * 4);
+    if (mixed) rc |= dmalloc(p, (void **)&P.lr, std::max(mx_lr, 1LL) * 4);
+    if (mixed) rc |= dmalloc(p, (void **)&P.W, std::max(mx_wlr, 1LL) * 4);
     rc |= dmalloc(p, (void **)&P.bc, mx_bc * 8);
     if (adjoint) {
         int maxit = 1;
@@ -1118,6 +1130,7 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     // stay in one L2 and the launch stays far below 2^31 work-items
     const int pack_blocks = (int)std::min<long long>(16, (c.max_pack + 255) / 256);
     const dim3 pack_grid(xcd_item_grid(nit, pack_blocks));
+    if (mixed) hipLaunchKernelGGL(lowrank_factors, dim3(nit), dim3(256), 0, s, c.dinst, c.ditem, P);
     if (mixed || p->adj_mixed)
         hipLaunchKernelGGL((pack_operator<__hip_bfloat16, false>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem, P,
                            nit, pack_blocks);
@@ -1805,8 +1818,10 @@ extern "C" int ocrt_plan_results(ocrt_plan *p, ocrt_result_t *out, double *toa_m
             if (orders) orders[(long long)i * M + m] = r.orders;
             const int ord = std::max(r.orders - 1, 0);
             const double L = I.nt + 1.0;
-            const int parts = (I.aer ? 1 : 0) + ((m <= 2 && I.ray) ? 1 : 0);
-            flops += ord * 2.0 * (6.0 * I.n) * (6.0 * I.n * std::max(parts, 1)) * L;
+            const bool lowrank = mixed && m <= 2 && I.ray && I.aer;
+            const int parts = (I.aer ? 1 : 0) + ((m <= 2 && I.ray && !lowrank) ? 1 : 0);
+            // issued: the dense parts, and a rank-3 item's 16 projection rows
+            flops += ord * 2.0 * (6.0 * I.n + (lowrank ? 16.0 : 0.0)) * (6.0 * I.n * std::max(parts, 1)) * L;
             // dense particle operator 2 (6 n)(6 n_in) per level; the molecular
             // operator has rank <= 6 and counts as 2 * 2 * (6 n) * 6 per level
             alg += ord * L * ((I.aer ? 72.0 * I.n * n_in : 0.0) + ((m <= 2 && I.ray) ? 144.0 * I.n : 0.0));

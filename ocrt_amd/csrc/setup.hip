@@ -740,6 +740,58 @@ __device__ inline double pair_molecular(const DevInst &I, int m, const double *p
     }
 }
 
+// The rank-3 molecular operator of Fourier mode m <= 2 (the reference's
+// outer-product channels, rt_sos_operator.c:102-119): over the state columns
+// c = (type t, node o), t in [I_up I_dn Q_up Q_dn U_up U_dn] with component
+// t / 2 and hemisphere h = t % 2, and the l = 2 basis p, r, t at +o, with the
+// parities of the generalised spherical functions (eps_p = eps_r = (-1)^m,
+// eps_t = -(-1)^m) folding the mirrored directions in:
+//   out factors  e1 = [I]  e2 = [I] eps_p^h p  e3 = [Q] eps_r^h r - [U] eps_t^h t
+//   in factors   g_b = w_o x (the same functions)
+//   A_mol = E C G^T,  C = [[beta0 (m = 0), 0, 0], [0, beta2, gamma2], [0, gamma2, alpha2]]
+// (the selector signs of the dense form, c_sel_s, are reproduced by these
+// parities; the dense operator of pack_operator equals E C G^T to rounding).
+// U = E C are the output factors the sweep applies; G the input factors the
+// operator's projection rows carry.
+__device__ inline void molecular_factors(const DevInst &I, const DevItem &it, const Pools &P, int c, double E[3],
+                                         double G[3])
+{
+    E[0] = E[1] = E[2] = G[0] = G[1] = G[2] = 0.0;
+    const int n = it.n;
+    if (c >= 6 * n) return;
+    const int t = col_sel_type(c, n), comp = t / 2, h = t % 2, o = c - (c / n) * n + 1;
+    const int dirs = 2 * n + 1;
+    const double *plm = P.basis + it.off_basis;
+    const double p2 = plm[2LL * dirs + n + o];
+    const double r2 = plm[(long long)(it.lmax + 1) * dirs + 2LL * dirs + n + o];
+    const double t2 = plm[2LL * (it.lmax + 1) * dirs + 2LL * dirs + n + o];
+    const double ep = (it.m & 1) ? -1.0 : 1.0, et = -ep;
+    const double w = P.ring[I.off_ring + n + (o - 1)];
+    const double sp = h ? ep : 1.0, sr = h ? ep : 1.0, st = h ? et : 1.0;
+    if (comp == 0) { E[0] = 1.0; E[1] = sp * p2; }
+    else if (comp == 1) E[2] = sr * r2;
+    else E[2] = -st * t2;
+    if (w == 0.0) return;
+    G[0] = w * E[0]; G[1] = w * E[1]; G[2] = w * E[2];
+}
+
+// Output factors U = E C of every rank-3 item that owns its operator, fp32.
+__global__ void lowrank_factors(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P)
+{
+    const DevItem &it = items[blockIdx.x];
+    if (!it.lowrank || !it.op_owner) return;
+    const DevInst &I = inst[it.inst];
+    const double b0 = (it.m == 0) ? I.beta0 : 0.0;
+    float *U = P.lr + it.off_lr;
+    for (int c = threadIdx.x; c < it.P; c += blockDim.x) {
+        double E[3], G[3];
+        molecular_factors(I, it, P, c, E, G);
+        U[c] = (float)(b0 * E[0]);
+        U[it.P + c] = (float)(I.beta2 * E[1] + I.gamma2 * E[2]);
+        U[2 * it.P + c] = (float)(I.gamma2 * E[1] + I.alpha2 * E[2]);
+    }
+}
+
 // Dense operator A[Mp][K] of one item; element (r, c) with row r = outgoing
 // state column and c = (part, incoming state column).  Part 0/1 of the
 // column range is the particle (scaled by xdel at apply time) or molecular
@@ -800,8 +852,16 @@ __global__ __launch_bounds__(256) void pack_operator(const DevInst *__restrict__
 #pragma unroll
                 for (int e = 0; e < CW; ++e) {
                     const int c = cc + e;
-                    v[e] = (r < six_n && c < six_n) ? operator_entry(I, it, P, kind, TRANS ? c : r, TRANS ? r : c)
-                                                    : 0.0;
+                    if (it.lowrank && r >= it.pbase) {
+                        // projection rows: the rank-3 molecular input factors G
+                        double E[3], G[3];
+                        molecular_factors(I, it, P, c, E, G);
+                        v[e] = (r - it.pbase < 3) ? G[r - it.pbase] : 0.0;
+                    } else {
+                        v[e] = (r < six_n && c < six_n)
+                                   ? operator_entry(I, it, P, kind, TRANS ? c : r, TRANS ? r : c)
+                                   : 0.0;
+                    }
                 }
                 if constexpr (BF) {
                     // fp32 rounding first, as every bf16 conversion of the solver

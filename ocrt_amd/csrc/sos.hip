@@ -377,7 +377,7 @@ template <int NP, bool SPLIT, int TM = 64, int TN = kGemmLevels> struct GemmLds 
 template <int NP, bool SPLIT, bool F64IN = false, int EPI = 0, int TM = 64, int TN = kGemmLevels, typename Out>
 __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, const unsigned short *A,
                                       const unsigned short *F, const unsigned short *Flo, const double *F64,
-                                      int row0, int lev0, GemmLds<NP, SPLIT, TM, TN> &S, Out out)
+                                      float *Wp, int row0, int lev0, GemmLds<NP, SPLIT, TM, TN> &S, Out out)
 {
     constexpr int AB = TM / 32, BB = TN / 32;   // 16 x 16 blocks per wave along rows and levels
     constexpr int AR = TM / 64, FR = TN / 64;   // operator rows and field levels staged per thread
@@ -532,7 +532,15 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
             const float s0 = (kind0 == 1) ? x : y;
 #pragma unroll
             for (int a = 0; a < AB; ++a) {
-                const int r = row0 + wm + a * 16 + 4 * (lane >> 4);
+                const int rb = row0 + wm + a * 16, r = rb + 4 * (lane >> 4);
+                if (Wp && rb >= it.pbase) {
+                    // projection rows of a rank-3 molecular item: W = G F,
+                    // unscaled, rows pbase .. pbase + 2 (and a zero fourth)
+                    if (rb == it.pbase && (lane >> 4) == 0)
+                        *reinterpret_cast<float4 *>(Wp + 4LL * lev) =
+                            make_float4(acc[0][a][b][0], acc[0][a][b][1], acc[0][a][b][2], acc[0][a][b][3]);
+                    continue;
+                }
                 float4 v;
                 v.x = s0 * acc[0][a][b][0]; v.y = s0 * acc[0][a][b][1];
                 v.z = s0 * acc[0][a][b][2]; v.w = s0 * acc[0][a][b][3];
@@ -573,7 +581,8 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(co
     const float js = I.jscale;
     const LevelView lv(P, I);
     __shared__ GemmLds<NP, SPLIT, TM, TN> S;
-    gemm_tile_bf16<NP, SPLIT, false, 0, TM, TN>(it, lv, A, F, Flo, nullptr, tl.tm * TM, tl.tn * TN, S,
+    float *Wp = (SPLIT && it.lowrank) ? P.W + it.off_W : nullptr;
+    gemm_tile_bf16<NP, SPLIT, false, 0, TM, TN>(it, lv, A, F, Flo, nullptr, Wp, tl.tm * TM, tl.tn * TN, S,
                                                 [&](int lev, int r, float4 v) {
         if (r >= Pp) return;
         if constexpr (SPLIT) {
@@ -612,14 +621,14 @@ __global__ __launch_bounds__(256, 1) void gemm_x3(const DevInst *__restrict__ in
     const LevelView lv(P, I);
     __shared__ GemmLds<NP, true> S;
     if constexpr (MODE == kGemmMix) {
-        gemm_tile_bf16<NP, true, true, 0>(it, lv, A, nullptr, nullptr, F, tl.tm * 64, tl.tn * kGemmLevels, S,
+        gemm_tile_bf16<NP, true, true, 0>(it, lv, A, nullptr, nullptr, F, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
                                           [&](int lev, int r, float4 v) {
                                               if (r >= Pp) return;
                                               double *d = J + (long long)lev * Pp + r;
                                               d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
                                           });
     } else {
-        gemm_tile_bf16<NP, true, true, 1>(it, lv, A, nullptr, nullptr, F, tl.tm * 64, tl.tn * kGemmLevels, S,
+        gemm_tile_bf16<NP, true, true, 1>(it, lv, A, nullptr, nullptr, F, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
                                           [&](int lev, int r, float4 v0, float4 v1) {
                                               if (r >= Pp) return;
                                               double *d = J + (long long)lev * Pp + r;
@@ -760,10 +769,27 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
             // else fp16 scaled by jscale
             const float ji = I.jinv;
             const float *__restrict__ Jf = P.Jf + it.off_Flo + col;
-            auto scan = [&](auto split_tag) {
-                constexpr bool SPLIT = decltype(split_tag)::value;
+            // rank-3 molecular items add ydel(k) (U W[k]) to the dense source:
+            // the column's three output factors, the item's projections
+            const float *__restrict__ Wl = P.W + it.off_W;
+            const double *__restrict__ YD = lv.yd;
+            float u0 = 0.f, u1 = 0.f, u2 = 0.f;
+            if (it.lowrank) {
+                const float *Ul = P.lr + it.off_lr + col;
+                u0 = Ul[0]; u1 = Ul[Pp]; u2 = Ul[2LL * Pp];
+            }
+            auto mol = [&](int k) -> float {
+                const float4 w = *reinterpret_cast<const float4 *>(Wl + 4LL * k);
+                return (float)YD[k] * fmaf(u2, w.z, fmaf(u1, w.y, u0 * w.x));
+            };
+            // kind 0: fp16 source; 1: fp32 source; 2: fp32 source plus the
+            // rank-3 molecular term
+            auto scan = [&](auto kind_tag) {
+                constexpr int KIND = decltype(kind_tag)::value;
+                constexpr bool SPLIT = KIND >= 1;
                 auto jload = [&](int k) -> float {
-                    if constexpr (SPLIT) return Jf[(long long)k * Pp];
+                    if constexpr (KIND == 2) return Jf[(long long)k * Pp] + mol(k);
+                    else if constexpr (SPLIT) return Jf[(long long)k * Pp];
                     else return (float)J[(long long)k * Pp] * ji;
                 };
                 float run = (float)bc;
@@ -795,8 +821,9 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                     tot = fmaxf(tot, fabsf(ts));
                     s_in = sj;
                 };
-                auto src = [&](long long e) -> float {
-                    if constexpr (SPLIT) return pJf[e];
+                auto src = [&](long long e, int k) -> float {
+                    if constexpr (KIND == 2) return pJf[e] + mol(k);
+                    else if constexpr (SPLIT) return pJf[e];
                     else return (float)pJ[e] * ji;
                 };
                 int i = 1;
@@ -806,7 +833,7 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                     float sj[U], om[U], idt[U], to[U];
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
-                        sj[u] = src(u * dP);
+                        sj[u] = src(u * dP, k1 + dL * (i - 1 + u));
                         om[u] = pOM[u * dL * n];
                         idt[u] = pIDT[u * dL];
                         to[u] = pT[u * dP];
@@ -819,7 +846,7 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                 }
                 // the remaining interior levels one at a time
                 for (; i < nt; ++i) {
-                    level(0, src(0), pOM[0], pIDT[0], pT[0]);
+                    level(0, src(0, k1 + dL * (i - 1)), pOM[0], pIDT[0], pT[0]);
                     pJ += dP; pJf += dP; pF += dP; pT += dP;
                     if constexpr (SPLIT) pFlo += dP;
                     pOM += dL * n; pIDT += dL;
@@ -831,10 +858,12 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                 mc.mtot = fmaxf(mc.mtot, tot);
                 mc.emit_row(kl, (double)run);
             };
-            if (it.fsplit)
-                scan(std::true_type{});
+            if (it.lowrank)
+                scan(std::integral_constant<int, 2>{});
+            else if (it.fsplit)
+                scan(std::integral_constant<int, 1>{});
             else
-                scan(std::false_type{});
+                scan(std::integral_constant<int, 0>{});
             mcur = mc.mcur;
             mtot = mc.mtot;
         } else {
