@@ -469,6 +469,21 @@ int ocrt_couple_water_to_atm(int m_max, int n_w, const double *mu_w, const doubl
                              const double *U_w, double n_water, int q_convention, double wind, int sigma_type,
                              int n_a, const double *mu_a, double *I_a, double *Q_a, double *U_a, void *hip_stream);
 
+/* Batched forms of the two couplings (no counterpart in the reference, which
+ * couples one case per call): n_inst instances that share the node sets, the
+ * wind, the slope model, the refractive index, the Q convention and m_max.
+ * `fields` holds each instance's I, Q and U blocks back to back,
+ * [n_inst][3][(m_max+1) * n_in], and `out` receives
+ * [n_inst][3][(m_max+1) * n_out] in the same order.  The transmission kernels
+ * are computed once for the whole batch; each instance's output is bit for bit
+ * what the one-instance call above gives it.  1 <= n_inst <= 65535. */
+int ocrt_couple_atm_to_water_batch(int n_inst, int m_max, int n_a, const double *mu_a, const double *w_a,
+                                   const double *fields, double n_water, int q_convention, const double *mu_w,
+                                   int n_w, double wind, int sigma_type, double *out, void *hip_stream);
+int ocrt_couple_water_to_atm_batch(int n_inst, int m_max, int n_w, const double *mu_w, const double *fields,
+                                   double n_water, int q_convention, double wind, int sigma_type, int n_a,
+                                   const double *mu_a, double *out, void *hip_stream);
+
 /* One-call form: plan, execute, copy results, destroy. */
 int ocrt_solve_batch(const ocrt_instance_t *inst, int n_inst, int precision,
                      ocrt_result_t *out, void *hip_stream);

@@ -763,6 +763,9 @@ __global__ void couple_rough(const double *__restrict__ T, const double *__restr
                              const double *__restrict__ mu_a, const double *__restrict__ w_a, int n_w, int n_a,
                              int m_max, double *__restrict__ out)
 {
+    // instance blockIdx.y of a batch sharing nodes and surface
+    field += (long long)blockIdx.y * 3 * (m_max + 1) * n_a;
+    out += (long long)blockIdx.y * 3 * (m_max + 1) * n_w;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (m_max + 1) * n_w) return;
     const int m = e / n_w, iw = e % n_w;
@@ -790,6 +793,8 @@ __global__ void couple_flat(const double *__restrict__ field, const double *__re
                             const int *__restrict__ src, int w, const double *__restrict__ mu_w, int n_w, int n_a,
                             int m_max, double nw, int qc, double *__restrict__ out)
 {
+    field += (long long)blockIdx.y * 3 * (m_max + 1) * n_a;
+    out += (long long)blockIdx.y * 3 * (m_max + 1) * n_w;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (m_max + 1) * n_w) return;
     const int m = e / n_w, iw = e % n_w;
@@ -824,9 +829,12 @@ __global__ __launch_bounds__(256) void couple_btdf_m0(const double *__restrict__
                                                       const int *__restrict__ src, int w,
                                                       const double *__restrict__ mu_w, const double *__restrict__ fa_mu,
                                                       const double *__restrict__ fa_w, double s2v, double nw,
-                                                      double *__restrict__ out)
+                                                      double *__restrict__ out, long long in_stride,
+                                                      long long out_stride)
 {
     __shared__ double red[4];
+    field += (long long)blockIdx.y * in_stride;
+    out += (long long)blockIdx.y * out_stride;
     const int iw = blockIdx.x, t = threadIdx.x;
     const double mw = mu_w[iw];
     if (!(mw > 1e-9 && mw <= 1.0)) return;   // keeps the flat value
@@ -855,12 +863,12 @@ __global__ __launch_bounds__(256) void couple_btdf_m0(const double *__restrict__
 }  // namespace
 }  // namespace ocrt
 
-extern "C" int ocrt_couple_atm_to_water(int m_max, int n_a, const double *mu_a, const double *w_a, const double *I_a,
-                                        const double *Q_a, const double *U_a, double n_water, int q_convention,
-                                        const double *mu_w, int n_w, double wind, int sigma_type, double *I_w,
-                                        double *Q_w, double *U_w, void *hip_stream)
+extern "C" int ocrt_couple_atm_to_water_batch(int n_inst, int m_max, int n_a, const double *mu_a, const double *w_a,
+                                              const double *fields, double n_water, int q_convention,
+                                              const double *mu_w, int n_w, double wind, int sigma_type, double *out,
+                                              void *hip_stream)
 {
-    if (!mu_a || !I_a || !Q_a || !U_a || !mu_w || !I_w || !Q_w || !U_w) return OCRT_ERR_ARGS;
+    if (n_inst < 1 || n_inst > 65535 || !mu_a || !fields || !mu_w || !out) return OCRT_ERR_ARGS;
     if (!(n_water > 1.0) || n_a < 2 || n_w < 1 || m_max < 0 || m_max + 1 > kIfMaxModes) return OCRT_ERR_ARGS;
     if ((long long)n_w * n_a > kIfMaxPairs || !std::isfinite(wind)) return OCRT_ERR_ARGS;
     hipStream_t st = (hipStream_t)hip_stream;
@@ -868,18 +876,17 @@ extern "C" int ocrt_couple_atm_to_water(int m_max, int n_a, const double *mu_a, 
     const size_t nf = (size_t)M * n_a, no = (size_t)M * n_w;
     DevBufs B;
     double *d_field, *d_ma, *d_mw, *d_out;
-    IFCHK(B.get(&d_field, sizeof(double) * 3 * nf));
+    IFCHK(B.get(&d_field, sizeof(double) * 3 * nf * n_inst));
     IFCHK(B.get(&d_ma, sizeof(double) * n_a));
     IFCHK(B.get(&d_mw, sizeof(double) * n_w));
-    IFCHK(B.get(&d_out, sizeof(double) * 3 * no));
-    IFCHK(hipMemcpyAsync(d_field, I_a, sizeof(double) * nf, hipMemcpyHostToDevice, st));
-    IFCHK(hipMemcpyAsync(d_field + nf, Q_a, sizeof(double) * nf, hipMemcpyHostToDevice, st));
-    IFCHK(hipMemcpyAsync(d_field + 2 * nf, U_a, sizeof(double) * nf, hipMemcpyHostToDevice, st));
+    IFCHK(B.get(&d_out, sizeof(double) * 3 * no * n_inst));
+    IFCHK(hipMemcpyAsync(d_field, fields, sizeof(double) * 3 * nf * n_inst, hipMemcpyHostToDevice, st));
     IFCHK(hipMemcpyAsync(d_ma, mu_a, sizeof(double) * n_a, hipMemcpyHostToDevice, st));
     IFCHK(hipMemcpyAsync(d_mw, mu_w, sizeof(double) * n_w, hipMemcpyHostToDevice, st));
     const double s2v = host_slope_variance(wind, sigma_type);
-    const unsigned grid = (unsigned)((no + 255) / 256);
+    const dim3 grid((unsigned)((no + 255) / 256), (unsigned)n_inst);
     if (w_a && wind > 0.0 && s2v > 1e-10) {
+        // one set of transmission kernels serves every instance of the batch
         double *d_T, *d_wa;
         IFCHK(B.get(&d_T, sizeof(double) * (size_t)M * n_w * n_a * 9));
         IFCHK(B.get(&d_wa, sizeof(double) * n_a));
@@ -923,15 +930,35 @@ extern "C" int ocrt_couple_atm_to_water(int m_max, int n_a, const double *mu_a, 
             double *d_fa;
             IFCHK(B.get(&d_fa, sizeof(double) * fa.size()));
             IFCHK(hipMemcpyAsync(d_fa, fa.data(), sizeof(double) * fa.size(), hipMemcpyHostToDevice, st));
-            couple_btdf_m0<<<n_w, 256, 0, st>>>(d_field, d_ms, d_src, (int)ms.size(), d_mw, d_fa, d_fa + kFineAir,
-                                                s2v, n_water, d_out);
+            couple_btdf_m0<<<dim3((unsigned)n_w, (unsigned)n_inst), 256, 0, st>>>(
+                d_field, d_ms, d_src, (int)ms.size(), d_mw, d_fa, d_fa + kFineAir, s2v, n_water, d_out,
+                (long long)(3 * nf), (long long)(3 * no));
             IFCHK(hipGetLastError());
         }
     }
-    IFCHK(hipMemcpyAsync(I_w, d_out, sizeof(double) * no, hipMemcpyDeviceToHost, st));
-    IFCHK(hipMemcpyAsync(Q_w, d_out + no, sizeof(double) * no, hipMemcpyDeviceToHost, st));
-    IFCHK(hipMemcpyAsync(U_w, d_out + 2 * no, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    IFCHK(hipMemcpyAsync(out, d_out, sizeof(double) * 3 * no * n_inst, hipMemcpyDeviceToHost, st));
     IFCHK(hipStreamSynchronize(st));
+    return OCRT_OK;
+}
+
+extern "C" int ocrt_couple_atm_to_water(int m_max, int n_a, const double *mu_a, const double *w_a, const double *I_a,
+                                        const double *Q_a, const double *U_a, double n_water, int q_convention,
+                                        const double *mu_w, int n_w, double wind, int sigma_type, double *I_w,
+                                        double *Q_w, double *U_w, void *hip_stream)
+{
+    if (!mu_a || !I_a || !Q_a || !U_a || !mu_w || !I_w || !Q_w || !U_w || m_max < 0 || n_a < 1 || n_w < 1)
+        return OCRT_ERR_ARGS;
+    const size_t nf = (size_t)(m_max + 1) * n_a, no = (size_t)(m_max + 1) * n_w;
+    std::vector<double> f(3 * nf), o(3 * no);
+    std::copy(I_a, I_a + nf, f.begin());
+    std::copy(Q_a, Q_a + nf, f.begin() + nf);
+    std::copy(U_a, U_a + nf, f.begin() + 2 * nf);
+    const int rc = ocrt_couple_atm_to_water_batch(1, m_max, n_a, mu_a, w_a, f.data(), n_water, q_convention, mu_w,
+                                                  n_w, wind, sigma_type, o.data(), hip_stream);
+    if (rc != OCRT_OK) return rc;
+    std::copy(o.begin(), o.begin() + no, I_w);
+    std::copy(o.begin() + no, o.begin() + 2 * no, Q_w);
+    std::copy(o.begin() + 2 * no, o.end(), U_w);
     return OCRT_OK;
 }
 
@@ -1038,6 +1065,8 @@ __global__ __launch_bounds__(256) void couple_rough_up(const double *__restrict_
                                                        int n_a, double *__restrict__ out)
 {
     __shared__ double red[3][4];
+    field += (long long)blockIdx.y * 3 * (m_max + 1) * n_w;
+    out += (long long)blockIdx.y * 3 * (m_max + 1) * n_a;
     const int m = blockIdx.x, t = threadIdx.x;
     const long long M = m_max + 1;
     const double az = (m == 0) ? 2.0 * OCRT_PI : OCRT_PI;
@@ -1066,6 +1095,8 @@ __global__ void couple_flat_up(const double *__restrict__ field, const double *_
                                const int *__restrict__ src, int w, const double *__restrict__ mu_a, int n_a, int n_w,
                                int m_max, double nw, int qc, double *__restrict__ out)
 {
+    field += (long long)blockIdx.y * 3 * (m_max + 1) * n_w;
+    out += (long long)blockIdx.y * 3 * (m_max + 1) * n_a;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (m_max + 1) * n_a) return;
     const int m = e / n_a, ja = e % n_a;
@@ -1120,12 +1151,12 @@ void sorted_nodes(const double *mu, int n, std::vector<double> &ms, std::vector<
 }  // namespace
 }  // namespace ocrt
 
-extern "C" int ocrt_couple_water_to_atm(int m_max, int n_w, const double *mu_w, const double *I_w, const double *Q_w,
-                                        const double *U_w, double n_water, int q_convention, double wind,
-                                        int sigma_type, int n_a, const double *mu_a, double *I_a, double *Q_a,
-                                        double *U_a, void *hip_stream)
+extern "C" int ocrt_couple_water_to_atm_batch(int n_inst, int m_max, int n_w, const double *mu_w,
+                                              const double *fields, double n_water, int q_convention, double wind,
+                                              int sigma_type, int n_a, const double *mu_a, double *out,
+                                              void *hip_stream)
 {
-    if (!I_w || !Q_w || !U_w || !mu_w || !mu_a || !I_a || !Q_a || !U_a) return OCRT_ERR_ARGS;
+    if (n_inst < 1 || n_inst > 65535 || !fields || !mu_w || !mu_a || !out) return OCRT_ERR_ARGS;
     if (!(n_water > 1.0) || n_w < 2 || n_a < 1 || m_max < 0 || m_max + 1 > kIfMaxModes) return OCRT_ERR_ARGS;
     if (!std::isfinite(wind)) return OCRT_ERR_ARGS;
     hipStream_t st = (hipStream_t)hip_stream;
@@ -1137,14 +1168,12 @@ extern "C" int ocrt_couple_water_to_atm(int m_max, int n_w, const double *mu_w, 
     DevBufs B;
     double *d_field, *d_ma, *d_out, *d_ms;
     int *d_src;
-    IFCHK(B.get(&d_field, sizeof(double) * 3 * nf));
+    IFCHK(B.get(&d_field, sizeof(double) * 3 * nf * n_inst));
     IFCHK(B.get(&d_ma, sizeof(double) * n_a));
-    IFCHK(B.get(&d_out, sizeof(double) * 3 * no));
+    IFCHK(B.get(&d_out, sizeof(double) * 3 * no * n_inst));
     IFCHK(B.get(&d_ms, sizeof(double) * ms.size()));
     IFCHK(B.get(&d_src, sizeof(int) * src.size()));
-    IFCHK(hipMemcpyAsync(d_field, I_w, sizeof(double) * nf, hipMemcpyHostToDevice, st));
-    IFCHK(hipMemcpyAsync(d_field + nf, Q_w, sizeof(double) * nf, hipMemcpyHostToDevice, st));
-    IFCHK(hipMemcpyAsync(d_field + 2 * nf, U_w, sizeof(double) * nf, hipMemcpyHostToDevice, st));
+    IFCHK(hipMemcpyAsync(d_field, fields, sizeof(double) * 3 * nf * n_inst, hipMemcpyHostToDevice, st));
     IFCHK(hipMemcpyAsync(d_ma, mu_a, sizeof(double) * n_a, hipMemcpyHostToDevice, st));
     IFCHK(hipMemcpyAsync(d_ms, ms.data(), sizeof(double) * ms.size(), hipMemcpyHostToDevice, st));
     IFCHK(hipMemcpyAsync(d_src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice, st));
@@ -1157,12 +1186,13 @@ extern "C" int ocrt_couple_water_to_atm(int m_max, int n_w, const double *mu_w, 
         IFCHK(B.get(&d_wi, sizeof(double) * (size_t)n_a * kGridPoints));
         IFCHK(B.get(&d_T, sizeof(double) * (size_t)M * kGridPoints * 9));
         IFCHK(hipMemcpyAsync(d_gl, gl.data(), sizeof(double) * gl.size(), hipMemcpyHostToDevice, st));
-        IFCHK(hipMemsetAsync(d_out, 0, sizeof(double) * 3 * no, st));
+        IFCHK(hipMemsetAsync(d_out, 0, sizeof(double) * 3 * no * n_inst, st));
         const long long ng = (long long)n_a * kGridPoints;
         water_grid<<<(unsigned)((ng + 255) / 256), 256, 0, st>>>(d_ma, n_a, n_water * n_water, d_gl, d_xi, d_wi);
         IFCHK(hipGetLastError());
-        // one azimuth weight table serves every air node; the launches below
-        // follow each other on st, so d_T is reused without synchronising
+        // one azimuth weight table serves every air node, and one kernel row
+        // per air node every instance of the batch; the launches below follow
+        // each other on st, so d_T is reused without synchronising
         double2 *d_w;
         IFCHK(B.get(&d_w, sizeof(double2) * (size_t)interface_weight_rows(m_max) * kTawPhi));
         IFCHK(interface_weights_launch(kTawPhi, m_max, d_w, st));
@@ -1172,19 +1202,38 @@ extern "C" int ocrt_couple_water_to_atm(int m_max, int n_w, const double *mu_w, 
             const double *xi = d_xi + (size_t)ja * kGridPoints, *wi = d_wi + (size_t)ja * kGridPoints;
             IFCHK(interface_fourier_launch(OCRT_IFACE_T_WA, d_ma + ja, 1, xi, kGridPoints, m_max, kTawPhi, s2v,
                                            n_water, q_convention, d_w, d_T, st));
-            couple_rough_up<<<(unsigned)M, 256, 0, st>>>(d_T, xi, wi, d_field, d_ms, d_src, (int)ms.size(), n_w,
-                                                         m_max, ja, n_a, d_out);
+            couple_rough_up<<<dim3((unsigned)M, (unsigned)n_inst), 256, 0, st>>>(
+                d_T, xi, wi, d_field, d_ms, d_src, (int)ms.size(), n_w, m_max, ja, n_a, d_out);
             IFCHK(hipGetLastError());
         }
     } else {
-        couple_flat_up<<<(unsigned)((no + 255) / 256), 256, 0, st>>>(d_field, d_ms, d_src, (int)ms.size(), d_ma, n_a,
-                                                                      n_w, m_max, n_water, q_convention, d_out);
+        couple_flat_up<<<dim3((unsigned)((no + 255) / 256), (unsigned)n_inst), 256, 0, st>>>(
+            d_field, d_ms, d_src, (int)ms.size(), d_ma, n_a, n_w, m_max, n_water, q_convention, d_out);
         IFCHK(hipGetLastError());
     }
-    IFCHK(hipMemcpyAsync(I_a, d_out, sizeof(double) * no, hipMemcpyDeviceToHost, st));
-    IFCHK(hipMemcpyAsync(Q_a, d_out + no, sizeof(double) * no, hipMemcpyDeviceToHost, st));
-    IFCHK(hipMemcpyAsync(U_a, d_out + 2 * no, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    IFCHK(hipMemcpyAsync(out, d_out, sizeof(double) * 3 * no * n_inst, hipMemcpyDeviceToHost, st));
     IFCHK(hipStreamSynchronize(st));
+    return OCRT_OK;
+}
+
+extern "C" int ocrt_couple_water_to_atm(int m_max, int n_w, const double *mu_w, const double *I_w, const double *Q_w,
+                                        const double *U_w, double n_water, int q_convention, double wind,
+                                        int sigma_type, int n_a, const double *mu_a, double *I_a, double *Q_a,
+                                        double *U_a, void *hip_stream)
+{
+    if (!I_w || !Q_w || !U_w || !mu_w || !mu_a || !I_a || !Q_a || !U_a || m_max < 0 || n_w < 1 || n_a < 1)
+        return OCRT_ERR_ARGS;
+    const size_t nf = (size_t)(m_max + 1) * n_w, no = (size_t)(m_max + 1) * n_a;
+    std::vector<double> f(3 * nf), o(3 * no);
+    std::copy(I_w, I_w + nf, f.begin());
+    std::copy(Q_w, Q_w + nf, f.begin() + nf);
+    std::copy(U_w, U_w + nf, f.begin() + 2 * nf);
+    const int rc = ocrt_couple_water_to_atm_batch(1, m_max, n_w, mu_w, f.data(), n_water, q_convention, wind,
+                                                  sigma_type, n_a, mu_a, o.data(), hip_stream);
+    if (rc != OCRT_OK) return rc;
+    std::copy(o.begin(), o.begin() + no, I_a);
+    std::copy(o.begin() + no, o.begin() + 2 * no, Q_a);
+    std::copy(o.begin() + 2 * no, o.end(), U_a);
     return OCRT_OK;
 }
 

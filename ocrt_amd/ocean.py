@@ -9,14 +9,16 @@ structure, on batched plans:
   1. atmosphere pass 1 (ocrt_plan, the sea surface as the lower boundary):
      TOA Stokes vector and the diffuse downward field at the bottom of the
      atmosphere at every ring node (rt_solve_case_pol_for_ocean, :1468);
-  2. air -> water (ocrt_couple_atm_to_water, rt_air_water_coupling.c:244-478):
-     that field mapped onto the water ring, just below the surface;
+  2. air -> water (ocrt_couple_atm_to_water_batch, rt_air_water_coupling.c:
+     244-478): that field mapped onto the water ring, just below the surface,
+     for every instance that shares rings and surface in one call;
   3. in-water SOS (in-water columns of an ocrt_plan: rt_water_rt.c:2104-2236,
      5966-6034): the refracted direct beam, weighted by the atmosphere's
      direct transmittance and the surface's direct transmission, plus the
      transmitted skylight as a diffuse top source, with the water-side
      reflection of the surface fed back every order;
-  4. water -> air (ocrt_couple_water_to_atm, rt_air_water_coupling.c:598-780):
+  4. water -> air (ocrt_couple_water_to_atm_batch,
+     rt_air_water_coupling.c:598-780):
      the upwelling field just below the surface mapped onto the atmosphere's
      ring just above it, the water-leaving radiance;
   5. atmosphere pass 2 (ocrt_plan with the bottom source, bottom_source_only,
@@ -79,33 +81,37 @@ def flat_transmittance(mu_air, n_water):
     return 1.0 - 0.5 * (rs * rs + rp * rp)
 
 
+def _groups(keys):
+    """Indices grouped by equal key, in first-seen order."""
+    g = {}
+    for i, k in enumerate(keys):
+        g.setdefault(k, []).append(i)
+    return list(g.values())
+
+
 def _couple_down(lib, m_max, mu_a, w_a, fields, n_water, q_conv, mu_w, wind, sigma_type):
-    """ocrt_couple_atm_to_water for one instance: fields [M][3][n_a] -> [3][M][n_w]."""
-    M, n_a, n_w = m_max + 1, mu_a.size, mu_w.size
-    I = np.ascontiguousarray(fields[:, 0, :n_a])
-    Q = np.ascontiguousarray(fields[:, 1, :n_a])
-    U = np.ascontiguousarray(fields[:, 2, :n_a])
-    out = np.zeros((3, M, n_w))
-    rc = lib.ocrt_couple_atm_to_water(m_max, n_a, _p(mu_a), _p(w_a), _p(I), _p(Q), _p(U), C.c_double(n_water),
-                                      q_conv, _p(mu_w), n_w, C.c_double(wind), sigma_type, _p(out[0]), _p(out[1]),
-                                      _p(out[2]), None)
+    """ocrt_couple_atm_to_water_batch for instances sharing nodes and surface:
+    fields [B][M][3][>= n_a] -> [B][3][M][n_w]."""
+    B, M, n_a, n_w = len(fields), m_max + 1, mu_a.size, mu_w.size
+    f = np.ascontiguousarray(np.stack([np.transpose(x[:, :, :n_a], (1, 0, 2)) for x in fields]))
+    out = np.zeros((B, 3, M, n_w))
+    rc = lib.ocrt_couple_atm_to_water_batch(B, m_max, n_a, _p(mu_a), _p(w_a), _p(f), C.c_double(n_water), q_conv,
+                                            _p(mu_w), n_w, C.c_double(wind), sigma_type, _p(out), None)
     if rc != 0:
-        raise RuntimeError(f"ocrt_couple_atm_to_water failed with code {rc}")
+        raise RuntimeError(f"ocrt_couple_atm_to_water_batch failed with code {rc}")
     return out
 
 
 def _couple_up(lib, m_max, mu_w, fields, n_water, q_conv, wind, sigma_type, mu_a):
-    """ocrt_couple_water_to_atm for one instance: fields [M][3][n_w] -> [3][M][n_a]."""
-    M, n_w, n_a = m_max + 1, mu_w.size, mu_a.size
-    I = np.ascontiguousarray(fields[:, 0, :n_w])
-    Q = np.ascontiguousarray(fields[:, 1, :n_w])
-    U = np.ascontiguousarray(fields[:, 2, :n_w])
-    out = np.zeros((3, M, n_a))
-    rc = lib.ocrt_couple_water_to_atm(m_max, n_w, _p(mu_w), _p(I), _p(Q), _p(U), C.c_double(n_water), q_conv,
-                                      C.c_double(wind), sigma_type, n_a, _p(mu_a), _p(out[0]), _p(out[1]),
-                                      _p(out[2]), None)
+    """ocrt_couple_water_to_atm_batch for instances sharing nodes and surface:
+    fields [B][M][3][>= n_w] -> [B][3][M][n_a]."""
+    B, M, n_w, n_a = len(fields), m_max + 1, mu_w.size, mu_a.size
+    f = np.ascontiguousarray(np.stack([np.transpose(x[:, :, :n_w], (1, 0, 2)) for x in fields]))
+    out = np.zeros((B, 3, M, n_a))
+    rc = lib.ocrt_couple_water_to_atm_batch(B, m_max, n_w, _p(mu_w), _p(f), C.c_double(n_water), q_conv,
+                                            C.c_double(wind), sigma_type, n_a, _p(mu_a), _p(out), None)
     if rc != 0:
-        raise RuntimeError(f"ocrt_couple_water_to_atm failed with code {rc}")
+        raise RuntimeError(f"ocrt_couple_water_to_atm_batch failed with code {rc}")
     return out
 
 
@@ -159,20 +165,31 @@ def coupled_solve(atm_descs, waters, precision="mixed", max_orders_water=150, to
     _, boa = pa.node_fields(pitch_a)
     pa.close()
     t1 = time.perf_counter()
-    # 2. air -> water, and the in-water columns
-    wdescs, keep, rings_a, rings_w = [], [], [], []
+    # 2. air -> water, and the in-water columns; instances that share their
+    # rings and surface are coupled in one batched call
+    keep, rings_a, rings_w, winds, mus_w = [], [], [], [], []
     beams = np.zeros(N)
     for i, (d, w) in enumerate(zip(atm_descs, waters)):
-        mu_a, w_a = ring(d.n_mu, d.vza_deg)
+        rings_a.append(ring(d.n_mu, d.vza_deg))
+        mu_vw = snell_water(math.cos(math.radians(d.vza_deg)), d.n_water)
+        rings_w.append(ring(w.n_mu, math.degrees(math.acos(mu_vw)))[0])
+        winds.append(d.wind_speed if d.surface == 2 else 0.0)
+    skies = [None] * N
+    keys = [(d.m_max, rings_a[i][0].tobytes(), rings_a[i][1].tobytes(), rings_w[i].tobytes(), d.n_water,
+             d.q_convention, winds[i], d.sigma_type) for i, d in enumerate(atm_descs)]
+    for g in _groups(keys):
+        d = atm_descs[g[0]]
+        mu_a, w_a = rings_a[g[0]]
+        sky = _couple_down(lib, d.m_max, mu_a, w_a, [boa[i, : d.m_max + 1] for i in g], d.n_water, d.q_convention,
+                           rings_w[g[0]], winds[g[0]], d.sigma_type)
+        for j, i in enumerate(g):
+            skies[i] = np.ascontiguousarray(sky[j])
+    wdescs = []
+    for i, (d, w) in enumerate(zip(atm_descs, waters)):
         mus = math.cos(math.radians(d.sza_deg))
-        muv = math.cos(math.radians(d.vza_deg))
         mu_sw = snell_water(mus, d.n_water)
-        mu_vw = snell_water(muv, d.n_water)
-        vza_w = math.degrees(math.acos(mu_vw))
-        mu_w, w_w = ring(w.n_mu, vza_w)
-        wind = d.wind_speed if d.surface == 2 else 0.0
-        sky = _couple_down(lib, d.m_max, mu_a, w_a, boa[i, : d.m_max + 1], d.n_water, d.q_convention, mu_w, wind,
-                           d.sigma_type)
+        vza_w = math.degrees(math.acos(snell_water(math.cos(math.radians(d.vza_deg)), d.n_water)))
+        wind = winds[i]
         T_dir = direct_transmittance(lib, mus, wind, d.sigma_type, d.n_water)
         beams[i] = ra["T_dir_dn"][i] * mus * T_dir / mu_sw
         greek = w.greek if (w.greek is not None and w.omega_p > 0.0) else None
@@ -181,11 +198,9 @@ def coupled_solve(atm_descs, waters, precision="mixed", max_orders_water=150, to
                            q_convention=d.q_convention, depol=w.depol, greek=greek, n_mu=w.n_mu,
                            n_layers=w.n_layers, m_max=d.m_max, max_orders=max_orders_water, tolerance=tolerance,
                            water=(w.tau, w.omega_w, w.omega_p), top_reflect=2 if wind > 0.0 else 1,
-                           top_source=sky, beam_scale=beams[i])
+                           top_source=skies[i], beam_scale=beams[i])
         wdescs.append(wd)
-        keep.append(sky)
-        rings_a.append(mu_a)
-        rings_w.append(mu_w)
+        keep.append(skies[i])
     t2 = time.perf_counter()
     pw = Plan(wdescs, precision=precision)
     pw.export_nodes()
@@ -195,22 +210,29 @@ def coupled_solve(atm_descs, waters, precision="mixed", max_orders_water=150, to
     up_w, _ = pw.node_fields(pitch_w)
     pw.close()
     t3 = time.perf_counter()
-    # 4. water -> air, and 5. atmosphere pass 2 with the water-leaving source
+    # 4. water -> air, batched like step 2, and 5. atmosphere pass 2 with the
+    # water-leaving source
+    wls = [None] * N
+    keys = [(d.m_max, rings_w[i].tobytes(), d.n_water, d.q_convention, winds[i], d.sigma_type,
+             rings_a[i][0].tobytes()) for i, d in enumerate(atm_descs)]
+    for g in _groups(keys):
+        d = atm_descs[g[0]]
+        n_w = rings_w[g[0]].size
+        wl = _couple_up(lib, d.m_max, rings_w[g[0]], [up_w[i, : d.m_max + 1, :, :n_w] for i in g], d.n_water,
+                        d.q_convention, winds[g[0]], d.sigma_type, rings_a[g[0]][0])
+        for j, i in enumerate(g):
+            wls[i] = np.ascontiguousarray(wl[j])
     bdescs = []
     for i, d in enumerate(atm_descs):
-        wind = d.wind_speed if d.surface == 2 else 0.0
-        n_w = rings_w[i].size
-        wl = _couple_up(lib, d.m_max, rings_w[i], up_w[i, : d.m_max + 1, :, :n_w], d.n_water, d.q_convention, wind,
-                        d.sigma_type, rings_a[i])
         # the same atmosphere (every field, its aerosol tables included), with
         # the water-leaving field as its only source
         bd = type(d).from_buffer_copy(d)
         bd.decouple_sunglint = 1
-        bd.bottom_source = wl.ctypes.data_as(D)
-        bd.bottom_m_max, bd.bottom_n_mu = d.m_max, rings_a[i].size
+        bd.bottom_source = wls[i].ctypes.data_as(D)
+        bd.bottom_m_max, bd.bottom_n_mu = d.m_max, rings_a[i][0].size
         bd.bottom_source_only = 1
         bdescs.append(bd)
-        keep.append(wl)
+        keep.append(wls[i])
     t4 = time.perf_counter()
     pb = Plan(bdescs, precision=precision)
     pb.execute()

@@ -124,3 +124,75 @@ def test_reverse_coupling_equals_reference(ref_lib, wind):
     for s in range(3):
         scale = max(np.abs(res[1][s]).max(), 1e-300)
         assert np.abs(res[0][s] - res[1][s]).max() <= 1e-12 * scale, (wind, s)
+
+
+def _fields(rng, m_max, mu, n_inst):
+    """n_inst random Fourier fields [3][(m_max+1) * n] of a smooth shape."""
+    base = np.exp(-1.5 * (1 - mu))
+    out = []
+    for _ in range(n_inst):
+        I = np.concatenate([base * rng.uniform(0.5, 1.5) * 0.05 * 0.6 ** m for m in range(m_max + 1)])
+        out.append(np.stack([I, I * rng.uniform(-0.3, 0.1, I.size), I * rng.uniform(-0.1, 0.1, I.size)]))
+    return np.ascontiguousarray(np.stack(out))
+
+
+@pytest.mark.parametrize("case", ["rough_weights", "flat", "rough_btdf"])
+def test_batched_down_coupling_equals_single_calls(case):
+    """ocrt_couple_atm_to_water_batch: every instance of a batch bit for bit
+    what ocrt_couple_atm_to_water gives it alone."""
+    import ocrt_amd
+    lib = ocrt_amd.library()
+    VP = C.c_void_p
+    rng = np.random.default_rng(11)
+    m_max, n_inst = 3, 5
+    x, w = np.polynomial.legendre.leggauss(24)
+    mu_a = np.ascontiguousarray(np.concatenate([x[12:], [0.61]]))
+    w_a = np.ascontiguousarray(np.concatenate([w[12:], [0.0]]))
+    mu_w = np.ascontiguousarray(np.polynomial.legendre.leggauss(20)[0][10:])
+    wind, wa = {"rough_weights": (7.0, w_a), "flat": (0.0, w_a), "rough_btdf": (4.0, None)}[case]
+    f = _fields(rng, m_max, mu_a, n_inst)
+    n_a, n_w, M = mu_a.size, mu_w.size, m_max + 1
+    print(f"[gpu] batched down coupling {case}", flush=True)
+    out = np.zeros((n_inst, 3, M * n_w))
+    q = lambda a: None if a is None else a.ctypes.data_as(VP)
+    rc = lib.ocrt_couple_atm_to_water_batch(n_inst, m_max, n_a, q(mu_a), q(wa), q(f), C.c_double(1.34), 1, q(mu_w),
+                                            n_w, C.c_double(wind), 1, q(out), None)
+    assert rc == 0
+    for i in range(n_inst):
+        one = [np.zeros(M * n_w) for _ in range(3)]
+        fi = [np.ascontiguousarray(f[i, s]) for s in range(3)]
+        rc = lib.ocrt_couple_atm_to_water(m_max, n_a, q(mu_a), q(wa), q(fi[0]), q(fi[1]), q(fi[2]), C.c_double(1.34),
+                                          1, q(mu_w), n_w, C.c_double(wind), 1, q(one[0]), q(one[1]), q(one[2]), None)
+        assert rc == 0
+        assert np.array_equal(out[i], np.stack(one)), (case, i)
+    assert np.abs(out).max() > 0
+
+
+@pytest.mark.parametrize("wind", [6.0, 0.0])
+def test_batched_up_coupling_equals_single_calls(wind):
+    """ocrt_couple_water_to_atm_batch: every instance of a batch bit for bit
+    what ocrt_couple_water_to_atm gives it alone."""
+    import ocrt_amd
+    lib = ocrt_amd.library()
+    VP = C.c_void_p
+    rng = np.random.default_rng(12)
+    m_max, n_inst = 2, 4
+    mu_w = np.ascontiguousarray(np.polynomial.legendre.leggauss(32)[0][16:])
+    x = np.polynomial.legendre.leggauss(16)[0]
+    mu_a = np.ascontiguousarray(np.concatenate([x[8:], [0.38]]))
+    f = _fields(rng, m_max, mu_w, n_inst)
+    n_a, n_w, M = mu_a.size, mu_w.size, m_max + 1
+    print(f"[gpu] batched up coupling wind {wind}", flush=True)
+    out = np.zeros((n_inst, 3, M * n_a))
+    q = lambda a: a.ctypes.data_as(VP)
+    rc = lib.ocrt_couple_water_to_atm_batch(n_inst, m_max, n_w, q(mu_w), q(f), C.c_double(1.34), 1, C.c_double(wind),
+                                            1, n_a, q(mu_a), q(out), None)
+    assert rc == 0
+    for i in range(n_inst):
+        one = [np.zeros(M * n_a) for _ in range(3)]
+        fi = [np.ascontiguousarray(f[i, s]) for s in range(3)]
+        rc = lib.ocrt_couple_water_to_atm(m_max, n_w, q(mu_w), q(fi[0]), q(fi[1]), q(fi[2]), C.c_double(1.34), 1,
+                                          C.c_double(wind), 1, n_a, q(mu_a), q(one[0]), q(one[1]), q(one[2]), None)
+        assert rc == 0
+        assert np.array_equal(out[i], np.stack(one)), (wind, i)
+    assert np.abs(out).max() > 0
