@@ -282,6 +282,25 @@ int ocrt_plan_set_tuning(ocrt_plan *plan, int sweep_unroll, int sweep_waves);
 int ocrt_plan_set_gemm_shape(ocrt_plan *plan, int op_class, int shape);
 int ocrt_plan_gemm_tuning(const ocrt_plan *plan, int *shape_of_class, double *ms);
 
+/* Precision policy of the mixed form's scattering integral (no counterpart in
+ * the reference).  The first operator products of an item use the bf16-pair
+ * operator (A_hi + A_lo) and, for the modes m < 3, the bf16-pair field.  After
+ * each order the item's remaining share of its order sum is estimated as
+ * res r / (1 - r), with res = max |I_k| / max |order sum| (the convergence
+ * test's ratio) and r = res_k / res_(k-1); once it is below tau, the next
+ * order's product takes the reduced form: A_hi times the bf16 field, one MFMA
+ * per block.  The reduced orders carry about tau of the item's total, so
+ * their bf16 rounding (2^-9 relative) stays near tau * 2^-9 of it; slowly
+ * converging items (thick water columns) wait for a proportionally smaller
+ * residual.  tau = 0 keeps the full form for every order; 0 <= tau <= 1;
+ * mixed plans only.  ocrt_plan_gemm_issued reports, from the last results
+ * call, the MFMA flops the scattering integral launched (tile padding, low
+ * halves and reduced orders counted as run), the item-orders that ran it and
+ * how many of them ran reduced. */
+int ocrt_plan_set_reduced_threshold(ocrt_plan *plan, double tau);
+int ocrt_plan_gemm_issued(const ocrt_plan *plan, double *issued_flops, double *item_orders,
+                          double *reduced_orders);
+
 /* ocrt_plan_create with option flags.  No counterpart in the reference; the
  * adjoint is the Jacobian path of the batched solver (ocrt_adjoint_batch). */
 int ocrt_plan_create_ex(const ocrt_instance_t *inst, int n_inst, int precision, int flags,

@@ -79,6 +79,8 @@ def library():
         lib.ocrt_plan_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.ocrt_plan_set_gemm_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.ocrt_plan_gemm_tuning.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ocrt_plan_set_reduced_threshold.argtypes = [C.c_void_p, C.c_double]
+        lib.ocrt_plan_gemm_issued.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ocrt_plan_residuals.argtypes = [C.c_void_p, C.c_void_p]
         lib.ocrt_plan_export_nodes.argtypes = [C.c_void_p]
         lib.ocrt_plan_node_fields.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
@@ -218,6 +220,21 @@ class Plan:
         n = self.lib.ocrt_plan_gemm_tuning(self._handle, sh, ms)
         return {"shape_of_class": [self.GEMM_SHAPES[k] for k in sh],
                 "ms": [[ms[3 * k + j] for j in range(3)] for k in range(3)] if n else None}
+
+    def set_reduced_threshold(self, tau):
+        """Estimated remaining share of an item's order sum below which its
+        next order takes the reduced (bf16 operator and field) scattering
+        integral; 0: never."""
+        rc = self.lib.ocrt_plan_set_reduced_threshold(self._handle, float(tau))
+        if rc != 0:
+            raise OcrtError(f"reduced threshold {tau} refused (mixed plans, 0 <= tau <= 1)")
+
+    def gemm_issued(self):
+        """From the last results(): MFMA flops the scattering integral
+        launched, item-orders that ran it, and those that ran reduced."""
+        f, o, r = C.c_double(), C.c_double(), C.c_double()
+        self.lib.ocrt_plan_gemm_issued(self._handle, C.byref(f), C.byref(o), C.byref(r))
+        return {"issued_flops": f.value, "item_orders": o.value, "reduced_orders": r.value}
 
     def set_lut(self, vza_deg, raa_deg):
         """Sample every following execute on the grid vza x raa (full-grid / LUT mode)."""

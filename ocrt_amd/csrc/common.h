@@ -89,7 +89,7 @@ struct DevItem {
     long long off_tab;    // pool_tab: 6 x [(n+1)][2n+1]
     long long off_A;      // pool_A: [Mp][K]
     long long off_F;      // pool_F / pool_J: [L][P]
-    long long off_tot;    // pool_tot: [L][P] (fp64; fp32 in the mixed form)
+    long long off_tot;    // pool_tot: [L][P] fp64 (the fp64 form only)
     long long off_bc;     // pool_bc: [3][n] bottom boundary condition, [3][n] bottom downward field,
                           // [3][n] top boundary condition, [3][n] top upward field
     long long off_cmp;    // pool_comp: [2][P] compensation of the order sum at k = 0 and k = nt
@@ -108,6 +108,8 @@ struct ItemState {
     int active, orders, converged, status;
     double residual;
     unsigned long long max_cur, max_tot;   // bit patterns of non-negative doubles
+    int reduced;                           // mixed form: orders whose operator product ran reduced
+    int rnext;                             // mixed form: the next order's operator product runs reduced
 };
 
 // Per-item outputs gathered after the order loop.
@@ -117,6 +119,7 @@ struct ItemOut {
     double boa_hemi;      // sum_o mu_o w_o I^m downward at BOA
     int orders, converged;
     double residual;
+    int reduced;          // mixed form: orders whose operator product ran reduced (Pools::lo_tau)
 };
 
 struct InstOut {
@@ -135,7 +138,7 @@ struct Pools {
     const double *gout;  // adjoint: upstream gradient [instance][3] of (rho_I, rho_Q, rho_U)
     float *comp;          // fp32 compensation of the mixed-form order sum on the two output rows
     double *trow;         // mixed form: fp64 order sum of the two output rows (k = 0, k = nt), [2][P] per item
-                          // at off_cmp; the interior rows keep an fp32 sum in pool_tot
+                          // at off_cmp (the mixed form keeps no order sum at the interior levels)
     float *Jf;            // mixed form: fp32 source of the items that keep their field as a pair, [L][P] at off_Flo
     float *lr;            // mixed form: output factors U of the rank-3 molecular operator, [3][P] at off_lr
     float *W;             // mixed form: its projections W = G F per level, [L][4] at off_W
@@ -150,6 +153,12 @@ struct Pools {
                           // I, Q, U at every node
     const DevGas *gas_tab;    // gas absorption tables of the plan
     const double *gas_pool;   // their arrays
+    // mixed form: an item whose estimated remaining share of its order sum,
+    // res r / (1 - r) with res = max |I_k| / max |order sum| (the convergence
+    // test's ratio) and r = res_k / res_(k-1) its last decay rate, is below
+    // lo_tau takes its next operator product in the reduced form: the bf16
+    // operator and the bf16 field, one MFMA per block, no low halves (0: never)
+    float lo_tau;
 };
 
 // Workgroups of a launch are dealt round-robin over the 8 XCDs (block b runs

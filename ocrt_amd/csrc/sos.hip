@@ -134,13 +134,15 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
         s.residual = 0.0;
         s.max_cur = 0ull;
         s.max_tot = 0ull;
+        s.reduced = 0;
+        s.rnext = 0;
         st[item] = s;
     }
     if (col >= ct.col_end) return;
     FT *F = reinterpret_cast<FT *>(P.F) + it.off_F;
-    double *tot = P.tot + it.off_tot;
-    // mixed form: interior order sum in fp32 (pool_tot holds floats), output rows in fp64
-    float *tin = reinterpret_cast<float *>(P.tot) + it.off_tot;
+    // the fp64 form keeps the order sum at every level; the mixed form only at
+    // its two output rows (fp64 with an fp32 compensation term)
+    double *tot = MIXED ? nullptr : P.tot + it.off_tot;
     if (MIXED) {
         P.comp[it.off_cmp + col] = 0.f;
         P.comp[it.off_cmp + Pp + col] = 0.f;
@@ -200,8 +202,7 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
                          }
                          const long long ix = (long long)k * Pp + col;
                          if constexpr (MIXED) {
-                             if constexpr (kWhere == 1) tin[ix] = (float)v;
-                             else P.trow[it.off_cmp + (k == 0 ? 0 : Pp) + col] = v;
+                             if constexpr (kWhere != 1) P.trow[it.off_cmp + (k == 0 ? 0 : Pp) + col] = v;
                              store_field_pair(reinterpret_cast<__hip_bfloat16 *>(F), Flo, ix, v);
                          } else {
                              tot[ix] = v;
@@ -231,11 +232,18 @@ __global__ void order_status(const DevInst *__restrict__ inst, const DevItem *__
             const double mc = __longlong_as_double((long long)s.max_cur);
             const double mt = __longlong_as_double((long long)s.max_tot);
             const double res = (mt > 0.0) ? mc / mt : 0.0;
+            // the orders still to come, estimated from the last decay rate
+            // (none before two residuals exist, none unless it decays)
+            const double prev = s.residual, r = prev > 0.0 ? res / prev : 1.0;
+            const double tail = r < 1.0 ? res * r / (1.0 - r) : INFINITY;
             s.orders = it_next - 1;
             s.residual = res;
             if (!isfinite(mc) || !isfinite(mt)) { s.status = -2; s.active = 0; }
             else if (res < I.tol) { s.converged = 1; s.active = 0; }
             else if (it_next - 1 >= I.max_orders) s.active = 0;
+            // the next order's operator product runs reduced (gemm_bf16)
+            s.rnext = (s.active && tail < (double)P.lo_tau) ? 1 : 0;
+            s.reduced += s.rnext;
         }
         s.max_cur = 0ull;
         s.max_tot = 0ull;
@@ -374,7 +382,10 @@ template <int NP, bool SPLIT, int TM = 64, int TN = kGemmLevels> struct GemmLds 
 // used).  EPI = 0: the fp32 results, scaled by the level fractions, go to
 // out(lev, r, v) for the four rows r .. r + 3 of v; EPI = 1: the parts go
 // unscaled to out(lev, r, v0, v1) (v1 zero for one part); levels < L only.
-template <int NP, bool SPLIT, bool F64IN = false, int EPI = 0, int TM = 64, int TN = kGemmLevels, typename Out>
+// LO = false is the reduced product: the operator's hi halves times the bf16
+// field alone (no A_lo, no field residual), one MFMA per block.
+template <int NP, bool SPLIT, bool F64IN = false, int EPI = 0, int TM = 64, int TN = kGemmLevels, bool LO = true,
+          typename Out>
 __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, const unsigned short *A,
                                       const unsigned short *F, const unsigned short *Flo, const double *F64,
                                       float *Wp, int row0, int lev0, GemmLds<NP, SPLIT, TM, TN> &S, Out out)
@@ -382,7 +393,7 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     constexpr int AB = TM / 32, BB = TN / 32;   // 16 x 16 blocks per wave along rows and levels
     constexpr int AR = TM / 64, FR = TN / 64;   // operator rows and field levels staged per thread
     const int Pp = it.P, L = it.L, Mp = it.Mp;
-    const bool split = SPLIT && (F64IN || it.fsplit != 0);
+    const bool split = LO && SPLIT && (F64IN || it.fsplit != 0);
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
     float4_ acc[NP][AB][BB];
@@ -428,18 +439,18 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     auto load_slab = [&](int k0) {
         const long long ro0 = (long long)min(row0 + lr, Mp - 1) * Pp + k0 + lk;
         a0h0 = ld(A + ro0);
-        a0l0 = ld(A + slab + ro0);
+        if constexpr (LO) a0l0 = ld(A + slab + ro0);
         if constexpr (two) {
             a1h0 = ld(A + 2 * slab + ro0);
-            a1l0 = ld(A + 3 * slab + ro0);
+            if constexpr (LO) a1l0 = ld(A + 3 * slab + ro0);
         }
         if constexpr (AR > 1) {
             const long long ro1 = (long long)min(row0 + lr + 64, Mp - 1) * Pp + k0 + lk;
             a0h1 = ld(A + ro1);
-            a0l1 = ld(A + slab + ro1);
+            if constexpr (LO) a0l1 = ld(A + slab + ro1);
             if constexpr (two) {
                 a1h1 = ld(A + 2 * slab + ro1);
-                a1l1 = ld(A + 3 * slab + ro1);
+                if constexpr (LO) a1l1 = ld(A + 3 * slab + ro1);
             }
         }
         load_field(lev0 + lr, k0, rb0, rl0);
@@ -448,17 +459,17 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     auto st = [](unsigned short *q, uint4 v) { *reinterpret_cast<uint4 *>(q) = v; };
     auto store_slab = [&]() {
         st(&S.A[0][0][lr][lk], a0h0);
-        st(&S.A[0][1][lr][lk], a0l0);
+        if constexpr (LO) st(&S.A[0][1][lr][lk], a0l0);
         if constexpr (two) {
             st(&S.A[1][0][lr][lk], a1h0);
-            st(&S.A[1][1][lr][lk], a1l0);
+            if constexpr (LO) st(&S.A[1][1][lr][lk], a1l0);
         }
         if constexpr (AR > 1) {
             st(&S.A[0][0][lr + 64][lk], a0h1);
-            st(&S.A[0][1][lr + 64][lk], a0l1);
+            if constexpr (LO) st(&S.A[0][1][lr + 64][lk], a0l1);
             if constexpr (two) {
                 st(&S.A[1][0][lr + 64][lk], a1h1);
-                st(&S.A[1][1][lr + 64][lk], a1l1);
+                if constexpr (LO) st(&S.A[1][1][lr + 64][lk], a1l1);
             }
         }
         st(&S.B[0][lr][lk], rb0);
@@ -476,16 +487,17 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
 #pragma unroll
         for (int a = 0; a < AB; ++a) {
             const bf16x8_ hi = lds_frag(&S.A[q][0][wm + a * 16 + fr][fk]);
-            const bf16x8_ lo = lds_frag(&S.A[q][1][wm + a * 16 + fr][fk]);
+            bf16x8_ lo;
+            if constexpr (LO) lo = lds_frag(&S.A[q][1][wm + a * 16 + fr][fk]);
 #pragma unroll
             for (int b = 0; b < BB; ++b) {
                 const bf16x8_ fb = lds_frag(&S.B[0][wn + b * 16 + fr][fk]);
-                if constexpr (SPLIT)
+                if constexpr (SPLIT && LO)
                     if (split) {
                         const bf16x8_ fl = lds_frag(&S.B[SPLIT ? 1 : 0][wn + b * 16 + fr][fk]);
                         acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, fl, acc[q][a][b], 0, 0, 0);
                     }
-                acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, fb, acc[q][a][b], 0, 0, 0);
+                if constexpr (LO) acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, fb, acc[q][a][b], 0, 0, 0);
                 acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, fb, acc[q][a][b], 0, 0, 0);
             }
         }
@@ -565,7 +577,12 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(co
                                                  const ItemState *__restrict__ st, int order)
 {
     const GemmTile tl = tiles[blockIdx.x];
-    if (tl.item < 0 || !item_runs(st[tl.item], order)) return;
+    if (tl.item < 0) return;
+    const ItemState is = st[tl.item];
+    if (!item_runs(is, order)) return;
+    // the reduced product once the item's estimated remaining share fell
+    // below lo_tau (order_status); the test is workgroup-uniform
+    const bool reduced = is.rnext != 0;
     const DevItem &it = items[tl.item];
     const DevInst &I = inst[it.inst];
     const int Pp = it.P;
@@ -582,8 +599,7 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(co
     const LevelView lv(P, I);
     __shared__ GemmLds<NP, SPLIT, TM, TN> S;
     float *Wp = (SPLIT && it.lowrank) ? P.W + it.off_W : nullptr;
-    gemm_tile_bf16<NP, SPLIT, false, 0, TM, TN>(it, lv, A, F, Flo, nullptr, Wp, tl.tm * TM, tl.tn * TN, S,
-                                                [&](int lev, int r, float4 v) {
+    auto out = [&](int lev, int r, float4 v) {
         if (r >= Pp) return;
         if constexpr (SPLIT) {
             *reinterpret_cast<float4 *>(Jf + (long long)lev * Pp + r) = v;
@@ -593,7 +609,13 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(co
                               (_Float16)(v.w * js)};
             *reinterpret_cast<half4_ *>(J + (long long)lev * Pp + r) = h;
         }
-    });
+    };
+    if (reduced)
+        gemm_tile_bf16<NP, SPLIT, false, 0, TM, TN, false>(it, lv, A, F, Flo, nullptr, Wp, tl.tm * TM, tl.tn * TN, S,
+                                                           out);
+    else
+        gemm_tile_bf16<NP, SPLIT, false, 0, TM, TN, true>(it, lv, A, F, Flo, nullptr, Wp, tl.tm * TM, tl.tn * TN, S,
+                                                          out);
 }
 
 // Scattering integral of the adjoint plans' split-bf16 form: the fp64 field
@@ -674,9 +696,13 @@ __device__ inline float mixed_step(float run, float s_in, float s_out, float om,
 // the items that keep the pair), the order sum of its two output rows k = 0
 // and k = nt (fp64 with an fp32 Neumaier term) and the column's two
 // max-norms.  The sweep walks the interior levels itself, branch-free.
+// The mixed form keeps no order sum at the interior levels: its convergence
+// test divides the order's largest value over every level by the largest
+// order sum over the output rows, a denominator no larger than the one over
+// every level, so the test stops at the same order as the all-level test or
+// later, never earlier.
 struct MixedColumn {
     __hip_bfloat16 *F, *Flo;   // level stride P; Flo null unless the item keeps the pair
-    float *tin;                   // interior order sum (fp32)
     double *row_top, *row_bot;    // fp64 order sum of the output rows k = 0 and k = nt
     double *bottom, *top;         // top: upward field at the top of an in-water column, else null
     float *cmp_top, *cmp_bot;
@@ -689,7 +715,6 @@ struct MixedColumn {
         const int n = it.n;
         F = reinterpret_cast<__hip_bfloat16 *>(P.F) + it.off_F + col;
         Flo = it.fsplit ? reinterpret_cast<__hip_bfloat16 *>(P.Flo) + it.off_Flo + col : nullptr;
-        tin = reinterpret_cast<float *>(P.tot) + it.off_tot + col;
         row_top = P.trow + it.off_cmp + col;
         row_bot = row_top + it.P;
         bottom = P.bc + it.off_bc + 3 * n + c * n + o;
@@ -795,10 +820,10 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                 float run = (float)bc;
                 float s_in = jload(k_first);
                 mc.emit_row(k_first, bc);
-                float cur = 0.f, tot = 0.f;
+                float cur = 0.f;
                 // interior levels, i = 1 .. nt - 1 steps along the scan: running
-                // pointers at the level (source, field, order sum) and at the
-                // layer crossed to reach it (1 - T, 1 / dtau)
+                // pointers at the level (source, field) and at the layer crossed
+                // to reach it (1 - T, 1 / dtau)
                 const long long dP = hemi ? (long long)Pp : -(long long)Pp;
                 const int dL = hemi ? 1 : -1;
                 const int k1 = k_first + dL, l1 = layer_of(k1);
@@ -807,18 +832,14 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                 const float *__restrict__ pJf = Jf + ix1;
                 __hip_bfloat16 *__restrict__ pF = mc.F + ix1;
                 __hip_bfloat16 *__restrict__ pFlo = SPLIT ? mc.Flo + ix1 : nullptr;
-                float *__restrict__ pT = mc.tin + ix1;
                 const float *__restrict__ pOM = OM + (long long)l1 * n + o;
                 const float *__restrict__ pIDT = IDT + l1;
-                auto level = [&](long long e, float sj, float om, float idt, float to) {
+                auto level = [&](long long e, float sj, float om, float idt) {
                     run = mixed_step(run, s_in, sj, om, idt, smu, hemi);
                     const __hip_bfloat16 h = __float2bfloat16(run);
                     pF[e] = h;
                     if constexpr (SPLIT) pFlo[e] = __float2bfloat16(run - __bfloat162float(h));
-                    const float ts = to + run;
-                    pT[e] = ts;
                     cur = fmaxf(cur, fabsf(run));
-                    tot = fmaxf(tot, fabsf(ts));
                     s_in = sj;
                 };
                 auto src = [&](long long e, int k) -> float {
@@ -830,24 +851,23 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                 // whole blocks of U levels: every load of the block is issued
                 // before its dependent steps
                 for (; i + U <= nt; i += U) {
-                    float sj[U], om[U], idt[U], to[U];
+                    float sj[U], om[U], idt[U];
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         sj[u] = src(u * dP, k1 + dL * (i - 1 + u));
                         om[u] = pOM[u * dL * n];
                         idt[u] = pIDT[u * dL];
-                        to[u] = pT[u * dP];
                     }
 #pragma unroll
-                    for (int u = 0; u < U; ++u) level(u * dP, sj[u], om[u], idt[u], to[u]);
-                    pJ += U * dP; pJf += U * dP; pF += U * dP; pT += U * dP;
+                    for (int u = 0; u < U; ++u) level(u * dP, sj[u], om[u], idt[u]);
+                    pJ += U * dP; pJf += U * dP; pF += U * dP;
                     if constexpr (SPLIT) pFlo += U * dP;
                     pOM += U * dL * n; pIDT += U * dL;
                 }
                 // the remaining interior levels one at a time
                 for (; i < nt; ++i) {
-                    level(0, src(0, k1 + dL * (i - 1)), pOM[0], pIDT[0], pT[0]);
-                    pJ += dP; pJf += dP; pF += dP; pT += dP;
+                    level(0, src(0, k1 + dL * (i - 1)), pOM[0], pIDT[0]);
+                    pJ += dP; pJf += dP; pF += dP;
                     if constexpr (SPLIT) pFlo += dP;
                     pOM += dL * n; pIDT += dL;
                 }
@@ -855,7 +875,6 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                 const int kl = hemi ? nt : 0, ll = layer_of(kl);
                 run = mixed_step(run, s_in, jload(kl), OM[(long long)ll * n + o], IDT[ll], smu, hemi);
                 mc.mcur = fmaxf(mc.mcur, cur);
-                mc.mtot = fmaxf(mc.mtot, tot);
                 mc.emit_row(kl, (double)run);
             };
             if (it.lowrank)
@@ -974,6 +993,7 @@ __global__ void extract(const DevInst *__restrict__ inst, const DevItem *__restr
         r.orders = s.orders;
         r.converged = s.converged;
         r.residual = s.residual;
+        r.reduced = s.reduced;
         out[item] = r;
     }
     if (P.nodes) {
