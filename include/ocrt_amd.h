@@ -273,12 +273,13 @@ int ocrt_plan_set_tuning(ocrt_plan *plan, int sweep_unroll, int sweep_waves);
 /* Tile shape of the mixed form's scattering integral per operator class
  * (0: one operator part and a single bf16 field, 1: one part and a field
  * pair, 2: two parts): shape 0 = 64 operator rows x 128 levels, 1 = 64 x 64,
- * 2 = 128 x 64.  All shapes give the same bits.  ocrt_plan_set_gemm_shape
- * selects one by hand; ocrt_plan_gemm_tuning reports the shape of each class
- * and, after autotuning (OCRT_PLAN_AUTOTUNE times every shape on the plan's
- * own work before the sweep's launch shapes), ms[class * 3 + shape], the
- * class's summed GEMM time per execute; it returns the number of shapes
- * timed (0 before autotuning).  No counterpart in the reference. */
+ * 2 = 128 x 64, 3 = 128 x 128.  All shapes give the same bits.
+ * ocrt_plan_set_gemm_shape selects one by hand; ocrt_plan_gemm_tuning reports
+ * the shape of each class and, after autotuning (OCRT_PLAN_AUTOTUNE times
+ * every shape on the plan's own work before the sweep's launch shapes),
+ * ms[class * 4 + shape], the class's summed GEMM time per execute; it returns
+ * the number of shapes timed (0 before autotuning).  No counterpart in the
+ * reference. */
 int ocrt_plan_set_gemm_shape(ocrt_plan *plan, int op_class, int shape);
 int ocrt_plan_gemm_tuning(const ocrt_plan *plan, int *shape_of_class, double *ms);
 

@@ -202,7 +202,7 @@ class Plan:
         self.lib.ocrt_plan_tuning(self._handle, C.byref(u), C.byref(w), ms, C.byref(n))
         return {"unroll": u.value, "waves": w.value, "ms_per_shape": list(ms[: n.value])}
 
-    GEMM_SHAPES = ((64, 128), (64, 64), (128, 64))   # operator rows x levels per tile
+    GEMM_SHAPES = ((64, 128), (64, 64), (128, 64), (128, 128))   # operator rows x levels per tile
 
     def set_gemm_shape(self, op_class, shape):
         """Tile shape (index into GEMM_SHAPES) of one operator class of the
@@ -216,10 +216,11 @@ class Plan:
         """Shape chosen per operator class and, after autotuning, the class's
         GEMM ms per execute with every shape ([class][shape])."""
         sh = (C.c_int * 3)()
-        ms = (C.c_double * 9)()
+        ns = len(self.GEMM_SHAPES)
+        ms = (C.c_double * (3 * ns))()
         n = self.lib.ocrt_plan_gemm_tuning(self._handle, sh, ms)
         return {"shape_of_class": [self.GEMM_SHAPES[k] for k in sh],
-                "ms": [[ms[3 * k + j] for j in range(3)] for k in range(3)] if n else None}
+                "ms": [[ms[ns * k + j] for j in range(ns)] for k in range(3)] if n else None}
 
     def set_reduced_threshold(self, tau):
         """Estimated remaining share of an item's order sum below which its

@@ -96,10 +96,12 @@ struct DevItem {
     long long off_Flo;    // pool_Flo: [L][P] low half of the bf16 field pair (fsplit items)
     long long off_nodes;  // pool_nodes (LUT or node-export plans, global over the plan): [6][n]
     // mixed form, m <= 2 with aerosol and Rayleigh: the molecular operator
-    // (rank 3) runs outside the dense GEMM.  The operator gets 64 more rows
-    // from pbase: rows pbase .. pbase + 2 hold its input factors G, so the
-    // GEMM's projection tile yields W = G F per level (pool_W [L][4]); the
-    // sweep adds ydel (U W) with the output factors U = E C (pool_lr [3][P]).
+    // (rank 3) runs outside the dense GEMM.  Operator rows pbase .. pbase + 2
+    // hold its input factors G (row pbase + 3 zero), so the GEMM yields
+    // W = G F per level (pool_W [L][4]); the sweep adds ydel (U W) with the
+    // output factors U = E C (pool_lr [3][P]).  pbase = 6 n rounded up to 4
+    // when those four rows fit in the operator's padding to 64 rows, else the
+    // operator gets 64 more rows from pbase = its padded row count.
     int lowrank, pbase;
     long long off_lr, off_W;
 };
@@ -220,8 +222,8 @@ struct GemmTile { int item, tm, tn, pad; };   // item < 0: idle slot
 // plan keeps one tile list per shape and runs each operator class with the
 // shape its autotuner chose (all shapes give the same bits).
 struct GemmShape { int tm, tn; };
-constexpr GemmShape kGemmShapes[] = {{64, 128}, {64, 64}, {128, 64}};
-constexpr int kGemmShapeCount = 3;
+constexpr GemmShape kGemmShapes[] = {{64, 128}, {64, 64}, {128, 64}, {128, 128}};
+constexpr int kGemmShapeCount = 4;
 // Level width of a scattering-integral tile: bf16 form (gemm_bf16) and fp64
 // form (gemm_f64); the row width is 64 in both.
 constexpr int kGemmLevels = 128;

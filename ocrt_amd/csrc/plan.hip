@@ -726,8 +726,11 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 it.parts = kinds[0] | (np > 1 ? (kinds[1] << 8) : 0);
                 it.K = np * Pp;
                 const int tile_lev = gemm_bf ? kGemmLevels : kGemmLevelsF64;
-                it.pbase = Mp;
-                const int Mop = Mp + (it.lowrank ? 64 : 0);   // operator rows, with the projection tile
+                // the projection rows in the operator's padding when they fit
+                const int pin = roundup(6 * n, 4);
+                const bool pfit = pin + 4 <= Mp;
+                it.pbase = (it.lowrank && pfit) ? pin : Mp;
+                const int Mop = Mp + ((it.lowrank && !pfit) ? 64 : 0);   // operator rows
                 it.Mp = Mop; it.tiles_m = Mop / 64; it.tiles_n = (L + tile_lev - 1) / tile_lev;
                 it.lmax = I.lmax;
                 int cls;
@@ -960,11 +963,12 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t desc_bytes = 0;
     for (const Chunk &c : p->chunks)
+    for (const Chunk &c : p->chunks)
+        for (int sh = 0; sh < kGemmShapeCount; ++sh)
+            desc_bytes += al(std::max<size_t>(c.htiles_s[sh].size(), 1) * sizeof(GemmTile));
+    for (const Chunk &c : p->chunks)
         desc_bytes += al(c.hinst.size() * sizeof(DevInst)) + al(c.hitem.size() * sizeof(DevItem)) +
                       al(std::max<size_t>(c.htiles.size(), 1) * sizeof(GemmTile)) +
-                      al(std::max<size_t>(c.htiles_s[0].size(), 1) * sizeof(GemmTile)) +
-                      al(std::max<size_t>(c.htiles_s[1].size(), 1) * sizeof(GemmTile)) +
-                      al(std::max<size_t>(c.htiles_s[2].size(), 1) * sizeof(GemmTile)) +
                       al(std::max<size_t>(c.hcols.size(), 1) * sizeof(ColTile)) +
                       al(std::max<size_t>(c.hjobs.size(), 1) * sizeof(IpssJob)) +
                       al((size_t)(c.i1 - c.i0) * sizeof(int)) + al(std::max<size_t>(c.hcm_owner.size(), 1) * 4) +
@@ -1040,7 +1044,8 @@ static void launch_gemm_class(int sh, unsigned n, const GemmTile *tiles, const C
     switch (sh) {
     case 0: hipLaunchKernelGGL((gemm_bf16<NP, SP, 64, 128>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
     case 1: hipLaunchKernelGGL((gemm_bf16<NP, SP, 64, 64>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
-    default: hipLaunchKernelGGL((gemm_bf16<NP, SP, 128, 64>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
+    case 2: hipLaunchKernelGGL((gemm_bf16<NP, SP, 128, 64>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
+    default: hipLaunchKernelGGL((gemm_bf16<NP, SP, 128, 128>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
     }
 }
 
@@ -1863,7 +1868,8 @@ extern "C" int ocrt_plan_results(ocrt_plan *p, ocrt_result_t *out, double *toa_m
                 const bool split = m < kSplitModes;
                 const int cls = np > 1 ? 2 : (split ? 1 : 0);
                 const GemmShape sh = kGemmShapes[p->gemm_shape[cls]];
-                const int Pp = roundup(6 * I.n, 32), Mop = roundup(6 * I.n, 64) + (lowrank ? 64 : 0);
+                const int Mp6 = roundup(6 * I.n, 64);
+                const int Pp = roundup(6 * I.n, 32), Mop = Mp6 + ((lowrank && roundup(6 * I.n, 4) + 4 > Mp6) ? 64 : 0);
                 const double blocks = (double)((Mop + sh.tm - 1) / sh.tm) * ((I.nt + 1 + sh.tn - 1) / sh.tn) *
                                       (sh.tm / 16) * (sh.tn / 16) * (Pp / 32) * std::max(np, 1);
                 const int red = std::min(r.reduced, ord);

@@ -363,16 +363,21 @@ __device__ inline bf16x8_ lds_frag(const unsigned short *p)
 }
 
 // LDS staging of one K slab of a tile: [part][hi, lo][row][k] of the operator
-// and [hi, lo][level][k] of the field, rows padded to 40 entries.
+// and [hi, lo][level][k] of the field, 32 entries (four 16-byte chunks) per
+// row, chunk c of row r stored at position c ^ ((r >> 1) & 3).  On gfx950
+// that swizzle keeps both the staging stores (thread t: row t / 4, chunk
+// t % 4) and the fragment reads (lane l: row l % 16, chunk l / 16) free of
+// bank conflicts; a padded 40-entry row (80 bytes) conflicts on both, which
+// doubles the LDS cycles of the tile loop.
 template <int NP, bool SPLIT, int TM = 64, int TN = kGemmLevels> struct GemmLds {
-    unsigned short A[NP][2][TM][40];
-    unsigned short B[SPLIT ? 2 : 1][TN][40];
+    unsigned short A[NP][2][TM][32];
+    unsigned short B[SPLIT ? 2 : 1][TN][32];
 };
 
 // One tile of the bf16 scattering integral: operator rows row0 .. row0 + TM - 1
 // and levels lev0 .. lev0 + TN - 1 of one item with NP operator parts (the
 // item's K is NP P), four waves of TM/2 x TN/2.  The tile shapes (64 x 128,
-// 64 x 64, 128 x 64) give the same bits: every output takes the same K slabs
+// 64 x 64, 128 x 64, 128 x 128) give the same bits: every output takes the same K slabs
 // in the same order.  NP is a template parameter so that single-part items
 // hold half the accumulators (higher occupancy); SPLIT = false compiles out
 // the bf16 pair field (its registers and LDS plane) for tiles whose items keep
@@ -406,6 +411,8 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     // global -> LDS staging: one 8-wide k chunk of operator rows lr + 64 j per
     // thread and part, and of field levels lr + 64 j
     const int lr = t >> 2, lk = (t & 3) * 8;
+    // the staged chunk's swizzled position (rows lr and lr + 64 share it)
+    const int lks = (((t & 3) ^ (lr >> 1)) & 3) * 8;
     const long long slab = (long long)Mp * Pp;
     constexpr bool two = NP > 1;
     // staged operator chunks a<part><hi/lo><row block> and field chunks
@@ -458,30 +465,30 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     };
     auto st = [](unsigned short *q, uint4 v) { *reinterpret_cast<uint4 *>(q) = v; };
     auto store_slab = [&]() {
-        st(&S.A[0][0][lr][lk], a0h0);
-        if constexpr (LO) st(&S.A[0][1][lr][lk], a0l0);
+        st(&S.A[0][0][lr][lks], a0h0);
+        if constexpr (LO) st(&S.A[0][1][lr][lks], a0l0);
         if constexpr (two) {
-            st(&S.A[1][0][lr][lk], a1h0);
-            if constexpr (LO) st(&S.A[1][1][lr][lk], a1l0);
+            st(&S.A[1][0][lr][lks], a1h0);
+            if constexpr (LO) st(&S.A[1][1][lr][lks], a1l0);
         }
         if constexpr (AR > 1) {
-            st(&S.A[0][0][lr + 64][lk], a0h1);
-            if constexpr (LO) st(&S.A[0][1][lr + 64][lk], a0l1);
+            st(&S.A[0][0][lr + 64][lks], a0h1);
+            if constexpr (LO) st(&S.A[0][1][lr + 64][lks], a0l1);
             if constexpr (two) {
-                st(&S.A[1][0][lr + 64][lk], a1h1);
-                if constexpr (LO) st(&S.A[1][1][lr + 64][lk], a1l1);
+                st(&S.A[1][0][lr + 64][lks], a1h1);
+                if constexpr (LO) st(&S.A[1][1][lr + 64][lks], a1l1);
             }
         }
-        st(&S.B[0][lr][lk], rb0);
-        if constexpr (FR > 1) st(&S.B[0][lr + 64][lk], rb1);
+        st(&S.B[0][lr][lks], rb0);
+        if constexpr (FR > 1) st(&S.B[0][lr + 64][lks], rb1);
         if constexpr (SPLIT)
             if (split) {
-                st(&S.B[SPLIT ? 1 : 0][lr][lk], rl0);
-                if constexpr (FR > 1) st(&S.B[SPLIT ? 1 : 0][lr + 64][lk], rl1);
+                st(&S.B[SPLIT ? 1 : 0][lr][lks], rl0);
+                if constexpr (FR > 1) st(&S.B[SPLIT ? 1 : 0][lr + 64][lks], rl1);
             }
     };
     // fragment of a 16 x 32 block: lane l holds row (l & 15), k 8 (l >> 4) .. + 7
-    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    const int fr = lane & 15, fk = (((lane >> 4) ^ (fr >> 1)) & 3) * 8;   // k chunk lane / 16, swizzled
     auto mfma_part = [&](auto part) {
         constexpr int q = decltype(part)::value;
 #pragma unroll
@@ -545,14 +552,13 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
 #pragma unroll
             for (int a = 0; a < AB; ++a) {
                 const int rb = row0 + wm + a * 16, r = rb + 4 * (lane >> 4);
-                if (Wp && rb >= it.pbase) {
-                    // projection rows of a rank-3 molecular item: W = G F,
-                    // unscaled, rows pbase .. pbase + 2 (and a zero fourth)
-                    if (rb == it.pbase && (lane >> 4) == 0)
-                        *reinterpret_cast<float4 *>(Wp + 4LL * lev) =
-                            make_float4(acc[0][a][b][0], acc[0][a][b][1], acc[0][a][b][2], acc[0][a][b][3]);
-                    continue;
-                }
+                // projection rows of a rank-3 molecular item: W = G F, unscaled,
+                // rows pbase .. pbase + 2 and a zero fourth (pbase is a multiple
+                // of 4, so one lane group holds all four); those rows lie past
+                // the item's 6 n columns, so their J entries below are unused
+                if (Wp && r == it.pbase)
+                    *reinterpret_cast<float4 *>(Wp + 4LL * lev) =
+                        make_float4(acc[0][a][b][0], acc[0][a][b][1], acc[0][a][b][2], acc[0][a][b][3]);
                 float4 v;
                 v.x = s0 * acc[0][a][b][0]; v.y = s0 * acc[0][a][b][1];
                 v.z = s0 * acc[0][a][b][2]; v.w = s0 * acc[0][a][b][3];
@@ -568,10 +574,10 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
 
 // NP: operator parts of every item in the tile list; SPLIT: whether its items
 // may keep their field as a bf16 pair (the plan keeps one list per kind).
-// Single-part unsplit tiles fit 96 registers and 20 KB of LDS: five waves per
-// SIMD.
+// Single-part unsplit tiles of up to 64 x 128 fit 96 registers and 20 KB of
+// LDS: five waves per SIMD; their 128 x 128 tiles two.
 template <int NP, bool SPLIT, int TM, int TN>
-__global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? 5 : 1) void gemm_bf16(const DevInst *__restrict__ inst,
+__global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? (TM * TN > 8192 ? 2 : 5) : 1) void gemm_bf16(const DevInst *__restrict__ inst,
                                                  const DevItem *__restrict__ items, Pools P,
                                                  const GemmTile *__restrict__ tiles,
                                                  const ItemState *__restrict__ st, int order)
@@ -1124,6 +1130,9 @@ OCRT_GEMM_INST(2, true, 64, 64)
 OCRT_GEMM_INST(1, false, 128, 64)
 OCRT_GEMM_INST(1, true, 128, 64)
 OCRT_GEMM_INST(2, true, 128, 64)
+OCRT_GEMM_INST(1, false, 128, 128)
+OCRT_GEMM_INST(1, true, 128, 128)
+OCRT_GEMM_INST(2, true, 128, 128)
 #undef OCRT_GEMM_INST
 template __global__ void gemm_x3<1, kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                               const ItemState *, int);

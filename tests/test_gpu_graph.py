@@ -117,7 +117,7 @@ def test_every_launch_shape_gives_the_same_bits(ocrt, graph):
 @pytest.mark.parametrize("graph", [False, True])
 def test_every_gemm_tile_shape_gives_the_same_bits(ocrt, graph):
     """Every tile shape of the scattering integral (64 x 128, 64 x 64,
-    128 x 64), for every operator class at once and class by class, gives the
+    128 x 64, 128 x 128), for every operator class at once and class by class, gives the
     results of the default shape bit for bit, over repeated executes, also
     replayed from a graph; the batch has all three classes (modes 0..8: field
     pairs below m = 3, Rayleigh parts up to m = 2)."""
@@ -127,7 +127,7 @@ def test_every_gemm_tile_shape_gives_the_same_bits(ocrt, graph):
     want = a.results(modes=True)
     a.close()
     print(f"[gpu] GEMM tile shapes, {len(descs)} instances, graph={graph}", flush=True)
-    cases = [(sh, sh, sh) for sh in range(3)] + [(1, 2, 0), (2, 0, 1)]
+    cases = [(sh, sh, sh) for sh in range(4)] + [(1, 2, 0), (2, 3, 1), (3, 0, 2)]
     for shapes in cases:
         p = ocrt.Plan(descs, precision="mixed", graph=graph)
         for k, sh in enumerate(shapes):
