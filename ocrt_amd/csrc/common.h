@@ -150,6 +150,7 @@ struct Pools {
     void *J2;             // second output of a parts-split scattering integral (adjoint)
     void *Flo;            // bf16 low half of the field pair (mixed form, fsplit items)
     double *cm_w;         // cos/sin(m phi) table [(Mall+1)][2][kNPhi]
+    float *cm_wf;         // mixed form: its fp32 rounding, same layout
     const double *rtop;   // water-side reflection kernels of in-water columns (DevInst::off_rtop)
     double *nodes;        // LUT / node-export plans: per item [6][n] TOA upward I, Q, U and BOA downward
                           // I, Q, U at every node

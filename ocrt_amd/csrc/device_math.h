@@ -238,80 +238,88 @@ __device__ inline void coxmunk_R_core(double mu_out, double mu_in_dn, double cph
 // amplitudes share one reciprocal, and R = L(i2) . MF . L(i1) . brdf is
 // written out with the zeros of MF and L dropped.  Same branches, limits and
 // thresholds as coxmunk_R_core; the results agree to rounding.
-struct CoxMunkPair {
-    double mo, mia, s1, s2, mm, ss, msum, inv_s1, inv_s2, ps_pref, inv_s2v, fac, nw, inv_nw2;
+// T = double for every kernel of the fp64 form and the parity paths; the
+// mixed form's diffuse surface kernels evaluate it in float (coxmunk_pair
+// with float R), its fp32 copy being all the mixed form reads of them.
+template <typename T> struct CoxMunkPairT {
+    T mo, mia, s1, s2, mm, ss, msum, inv_s1, inv_s2, ps_pref, inv_s2v, fac, nw, inv_nw2;
     int qc;
 
-    __device__ CoxMunkPair(double mu_out, double mu_in_dn, double s2v, double shadow, double n_water, int q_conv)
+    __device__ CoxMunkPairT(double mu_out, double mu_in_dn, double s2v, double shadow, double n_water, int q_conv)
     {
-        mo = mu_out;
-        mia = -mu_in_dn;
-        s1 = sqrt(fmax(0.0, 1.0 - mu_out * mu_out));
-        s2 = sqrt(fmax(0.0, 1.0 - mia * mia));
-        mm = mu_out * mia;
-        ss = s1 * s2;
-        msum = mu_out + mu_in_dn;
-        inv_s1 = 1.0 / fmax(1e-12, s1);
-        inv_s2 = 1.0 / fmax(1e-12, s2);
-        ps_pref = 1.0 / (OCRT_PI * s2v);
-        inv_s2v = 1.0 / s2v;
-        fac = shadow / (4.0 * mu_in_dn * mu_out);
-        nw = n_water;
-        inv_nw2 = (1.0 / n_water) * (1.0 / n_water);
+        // the pair's constants in fp64, then rounded to T
+        const double mia_ = -mu_in_dn;
+        const double s1_ = sqrt(fmax(0.0, 1.0 - mu_out * mu_out));
+        const double s2_ = sqrt(fmax(0.0, 1.0 - mia_ * mia_));
+        mo = (T)mu_out;
+        mia = (T)mia_;
+        s1 = (T)s1_;
+        s2 = (T)s2_;
+        mm = (T)(mu_out * mia_);
+        ss = (T)(s1_ * s2_);
+        msum = (T)(mu_out + mu_in_dn);
+        inv_s1 = (T)(1.0 / fmax(1e-12, s1_));
+        inv_s2 = (T)(1.0 / fmax(1e-12, s2_));
+        ps_pref = (T)(1.0 / (OCRT_PI * s2v));
+        inv_s2v = (T)(1.0 / s2v);
+        fac = (T)(shadow / (4.0 * mu_in_dn * mu_out));
+        nw = (T)n_water;
+        inv_nw2 = (T)((1.0 / n_water) * (1.0 / n_water));
         qc = q_conv;
     }
 
     // cos 2i, sin 2i of one rotation from the unnormalised (x, y)
-    __device__ static void double_angle(double x, double y, double &c, double &s)
+    __device__ static void double_angle(T x, T y, T &c, T &s)
     {
-        const double qq = x * x + y * y;
-        if (qq < 1e-24) { c = 1.0; s = 0.0; return; }
-        const double iq = 1.0 / qq;
+        const T qq = x * x + y * y;
+        if (qq < (T)1e-24) { c = (T)1; s = (T)0; return; }
+        const T iq = (T)1 / qq;
         c = (x * x - y * y) * iq;
-        s = 2.0 * x * y * iq;
+        s = (T)2 * x * y * iq;
     }
 
-    __device__ void eval(double cphi, double sphi, double *R, double &tb2) const
+    __device__ void eval(T cphi, T sphi, T *R, T &tb2) const
     {
+        const T one = (T)1, zero = (T)0, half = (T)0.5;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) R[i] = 0.0;
-        tb2 = -1.0;
-        double cT = mm + ss * cphi;
-        cT = fmin(1.0, fmax(-1.0, cT));
-        const double co = sqrt(fmax(0.0, 0.5 * (1.0 - cT)));
-        if (co <= 1e-6) return;
-        const double cb = msum / (2.0 * co);
-        if (cb <= 1e-6) return;
-        const double rden = 1.0 / fmax(1e-12, sqrt(fmax(0.0, 1.0 - cT * cT)));
-        double c1, sn1, c2, sn2;
-        if (s2 <= 1.0e-8) {
-            const double ci = -copysign(1.0, mia) * cphi;
-            c1 = ci * ci - sphi * sphi; sn1 = 2.0 * ci * sphi;
+        for (int i = 0; i < 9; ++i) R[i] = zero;
+        tb2 = -one;
+        T cT = mm + ss * cphi;
+        cT = fmin(one, fmax(-one, cT));
+        const T co = sqrt(fmax(zero, half * (one - cT)));
+        if (co <= (T)1e-6) return;
+        const T cb = msum / ((T)2 * co);
+        if (cb <= (T)1e-6) return;
+        const T rden = one / fmax((T)1e-12, sqrt(fmax(zero, one - cT * cT)));
+        T c1, sn1, c2, sn2;
+        if (s2 <= (T)1.0e-8) {
+            const T ci = -copysign(one, mia) * cphi;
+            c1 = ci * ci - sphi * sphi; sn1 = (T)2 * ci * sphi;
         } else {
             double_angle((mo - mia * cT) * rden * inv_s2, s1 * sphi * rden, c1, sn1);
         }
-        if (s1 <= 1.0e-8) {
-            const double ci = -copysign(1.0, mo) * cphi;
-            c2 = ci * ci - sphi * sphi; sn2 = 2.0 * ci * sphi;
+        if (s1 <= (T)1.0e-8) {
+            const T ci = -copysign(one, mo) * cphi;
+            c2 = ci * ci - sphi * sphi; sn2 = (T)2 * ci * sphi;
         } else {
             double_angle((mia - mo * cT) * rden * inv_s1, s2 * sphi * rden, c2, sn2);
         }
         // Fresnel reflection at the facet (fresnel_R at mu = co)
-        double A = 1.0, q = 0.0, m8 = 1.0;
-        const double st2 = inv_nw2 * fmax(0.0, 1.0 - co * co);
-        if (st2 < 1.0) {
-            const double mt = sqrt(1.0 - st2);
-            const double a = co - nw * mt, b = co + nw * mt, c = nw * co - mt, d = nw * co + mt;
-            const double inv = 1.0 / (b * d);
-            const double rs = a * d * inv, rp = c * b * inv;
-            const double Rs = rs * rs, Rp = rp * rp;
-            A = 0.5 * (Rs + Rp);
-            q = (qc == 1) ? 0.5 * (Rp - Rs) : 0.5 * (Rs - Rp);
+        T A = one, q = zero, m8 = one;
+        const T st2 = inv_nw2 * fmax(zero, one - co * co);
+        if (st2 < one) {
+            const T mt = sqrt(one - st2);
+            const T a = co - nw * mt, b = co + nw * mt, c = nw * co - mt, d = nw * co + mt;
+            const T inv = one / (b * d);
+            const T rs = a * d * inv, rp = c * b * inv;
+            const T Rs = rs * rs, Rp = rp * rp;
+            A = half * (Rs + Rp);
+            q = (qc == 1) ? half * (Rp - Rs) : half * (Rs - Rp);
             m8 = rs * rp;
         }
-        const double cb2 = cb * cb;
-        tb2 = (1.0 - cb2) / fmax(cb2, 1e-8);
-        const double brdf = ps_pref * exp(-tb2 * inv_s2v) * fac / (cb2 * cb2);
+        const T cb2 = cb * cb;
+        tb2 = (one - cb2) / fmax(cb2, (T)1e-8);
+        const T brdf = ps_pref * exp(-tb2 * inv_s2v) * fac / (cb2 * cb2);
         R[0] = A * brdf;
         R[1] = q * c1 * brdf;
         R[2] = -q * sn1 * brdf;
@@ -323,6 +331,7 @@ struct CoxMunkPair {
         R[8] = (-sn2 * A * sn1 + c2 * m8 * c1) * brdf;
     }
 };
+using CoxMunkPair = CoxMunkPairT<double>;
 
 __device__ inline void coxmunk_R(double mu_out, double mu_in_dn, double cphi, double sphi, double ws,
                                  int sigma_type, double nw, int qc, double *R)

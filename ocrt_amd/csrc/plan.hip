@@ -23,8 +23,8 @@ __global__ void medium_gas(const DevInst *, Pools);
 __global__ void medium_transmit(const DevInst *, Pools);
 __global__ void adj_step_table(const DevInst *, Pools);
 __global__ void lowrank_factors(const DevInst *, const DevItem *, Pools);
-__global__ void fourier_weights(double *, int);
-template <bool DERIV>
+__global__ void fourier_weights(double *, float *, int);
+template <bool DERIV, typename RT, int SET>
 __global__ void coxmunk_fourier(const DevInst *, Pools, const int *, const int *, int, int);
 __global__ void angular_basis(const DevInst *, const DevItem *, Pools);
 __global__ void phase_tables(const DevInst *, const DevItem *, Pools, int, int);
@@ -164,8 +164,10 @@ struct Chunk {
     ColTile *dcols = nullptr;
     // rough-surface kernel owners of the chunk and the prefix sums of their
     // (outgoing, incoming-or-sun) pair counts: one Cox-Munk workgroup per pair
-    std::vector<int> hcm_owner, hcm_first;
-    int *dcm_owner = nullptr, *dcm_first = nullptr;
+    // Cox-Munk owners and the prefix sums of their pairs: every pair (n^2 + n
+    // each), and in mixed plans the solar (n) and diffuse (n^2) sets apart
+    std::vector<int> hcm_owner, hcm_first, hcm_sfirst, hcm_dfirst;
+    int *dcm_owner = nullptr, *dcm_first = nullptr, *dcm_sfirst = nullptr, *dcm_dfirst = nullptr;
     // instances of the chunk whose aerosol I -> I table comes from the value
     // kernel (operator owners only) and the prefix sums of their (row,
     // column) entry counts: one value_phase workgroup per entry
@@ -243,6 +245,7 @@ struct ocrt_plan {
     std::vector<int> item_of_inst;   // first global item of each instance
     // global device arrays
     double *ring = nullptr, *greek = nullptr, *cm_w = nullptr, *layers = nullptr;
+    float *cm_wf = nullptr;
     double *vk_w = nullptr;   // cos(m phi_q) of the value kernel, [360][mmax_all + 1]; null without one
     double *rtop = nullptr;   // water-side Cox-Munk kernels of rough in-water columns (DevInst::off_rtop)
     std::vector<int> rtop_owner;   // instances that compute a kernel set (the others share one)
@@ -697,9 +700,15 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                     I.off_Rsun = R + (long long)(I.mmax + 1) * I.n * I.n * 9;
                     R += h.R_n;
                     I.surf_owner = 1;
-                    if (c.hcm_first.empty()) c.hcm_first.push_back(0);
+                    if (c.hcm_first.empty()) {
+                        c.hcm_first.push_back(0);
+                        c.hcm_sfirst.push_back(0);
+                        c.hcm_dfirst.push_back(0);
+                    }
                     c.hcm_owner.push_back(ii - c.i0);
                     c.hcm_first.push_back(c.hcm_first.back() + I.n * I.n + I.n);
+                    c.hcm_sfirst.push_back(c.hcm_sfirst.back() + I.n);
+                    c.hcm_dfirst.push_back(c.hcm_dfirst.back() + I.n * I.n);
                 }
             }
             I.item0 = p->item_of_inst[ii] - c.it0;
@@ -858,6 +867,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     // cos / sin (m phi) for m = 0 .. max(mmax, 1): the m = 1 row also gives the
     // azimuths of the Cox-Munk quadrature
     rc |= dmalloc(p, (void **)&p->cm_w, (size_t)(std::max(p->mmax_all, 1) + 1) * 2 * kNPhi * 8);
+    if (mixed) rc |= dmalloc(p, (void **)&p->cm_wf, (size_t)(std::max(p->mmax_all, 1) + 1) * 2 * kNPhi * 4);
     bool any_vk = false;
     for (const Chunk &c : p->chunks) any_vk |= !c.hvk_owner.empty();
     if (any_vk) rc |= dmalloc(p, (void **)&p->vk_w, (size_t)kVkHalf * (p->mmax_all + 1) * 8);
@@ -910,6 +920,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     P.greek = p->greek;
     P.layers = p->layers;
     P.cm_w = p->cm_w;
+    P.cm_wf = p->cm_wf;
     P.rtop = p->rtop;
     P.nodes = nullptr;
     P.gas_tab = p->gas_tab;
@@ -972,7 +983,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                       al(std::max<size_t>(c.hcols.size(), 1) * sizeof(ColTile)) +
                       al(std::max<size_t>(c.hjobs.size(), 1) * sizeof(IpssJob)) +
                       al((size_t)(c.i1 - c.i0) * sizeof(int)) + al(std::max<size_t>(c.hcm_owner.size(), 1) * 4) +
-                      al(std::max<size_t>(c.hcm_first.size(), 1) * 4) +
+                      al(std::max<size_t>(c.hcm_first.size(), 1) * 4) * 3 +
                       al(std::max<size_t>(c.hvk_owner.size(), 1) * 4) + al(std::max<size_t>(c.hvk_first.size(), 1) * 4);
     std::vector<unsigned char> hdesc(desc_bytes, 0);
     unsigned char *ddesc = nullptr;
@@ -1001,6 +1012,10 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         c.djob_of = (int *)put(job_of.data(), job_of.size() * sizeof(int), job_of.size() * sizeof(int));
         c.dcm_owner = (int *)put(c.hcm_owner.data(), c.hcm_owner.size() * 4, std::max<size_t>(c.hcm_owner.size(), 1) * 4);
         c.dcm_first = (int *)put(c.hcm_first.data(), c.hcm_first.size() * 4, std::max<size_t>(c.hcm_first.size(), 1) * 4);
+        c.dcm_sfirst = (int *)put(c.hcm_sfirst.data(), c.hcm_sfirst.size() * 4,
+                                  std::max<size_t>(c.hcm_sfirst.size(), 1) * 4);
+        c.dcm_dfirst = (int *)put(c.hcm_dfirst.data(), c.hcm_dfirst.size() * 4,
+                                  std::max<size_t>(c.hcm_dfirst.size(), 1) * 4);
         c.dvk_owner = (int *)put(c.hvk_owner.data(), c.hvk_owner.size() * 4, std::max<size_t>(c.hvk_owner.size(), 1) * 4);
         c.dvk_first = (int *)put(c.hvk_first.data(), c.hvk_first.size() * 4, std::max<size_t>(c.hvk_first.size(), 1) * 4);
     }
@@ -1009,7 +1024,8 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         return OCRT_ERR_DEVICE;
     }
     const int cm_rows = std::max(p->mmax_all, 1);
-    hipLaunchKernelGGL(fourier_weights, dim3((kNPhi + 255) / 256, cm_rows + 1), dim3(256), 0, 0, p->cm_w, cm_rows);
+    hipLaunchKernelGGL(fourier_weights, dim3((kNPhi + 255) / 256, cm_rows + 1), dim3(256), 0, 0, p->cm_w, p->cm_wf,
+                       cm_rows);
     if (p->vk_w)
         hipLaunchKernelGGL(value_weights, dim3((kVkHalf * (p->mmax_all + 1) + 255) / 256), dim3(256), 0, 0, p->vk_w,
                            p->mmax_all + 1);
@@ -1125,17 +1141,32 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
         hipLaunchKernelGGL(adj_step_table, dim3((c.max_L + 127) / 128, ni), dim3(128), 0, s, c.dinst, P);
     if (!c.hcm_owner.empty()) {
         // one workgroup per pair, in launches of at most 2^22 workgroups (far
-        // below 2^31 work-items each)
-        const int total = c.hcm_first.back(), no = (int)c.hcm_owner.size();
+        // below 2^31 work-items each); mixed plans take the diffuse pairs in
+        // fp32 (only their fp32 copy is read) and the solar ones in fp64
+        const int no = (int)c.hcm_owner.size();
         const dim3 blk(kCoxMunkThreads);
-        for (int base = 0; base < total; base += 1 << 22) {
-            const dim3 grid((unsigned)std::min(total - base, 1 << 22));
-            if (p->adjoint)
-                hipLaunchKernelGGL(coxmunk_fourier<true>, grid, blk, 0, s, c.dinst, P, c.dcm_owner, c.dcm_first, no,
-                                   base);
-            else
-                hipLaunchKernelGGL(coxmunk_fourier<false>, grid, blk, 0, s, c.dinst, P, c.dcm_owner, c.dcm_first, no,
-                                   base);
+        auto pairs = [&](int total, auto launch) {
+            for (int base = 0; base < total; base += 1 << 22) launch(dim3((unsigned)std::min(total - base, 1 << 22)), base);
+        };
+        if (mixed) {
+            pairs(c.hcm_sfirst.back(), [&](dim3 grid, int base) {
+                hipLaunchKernelGGL((coxmunk_fourier<false, double, 1>), grid, blk, 0, s, c.dinst, P, c.dcm_owner,
+                                   c.dcm_sfirst, no, base);
+            });
+            pairs(c.hcm_dfirst.back(), [&](dim3 grid, int base) {
+                hipLaunchKernelGGL((coxmunk_fourier<false, float, 2>), grid, blk, 0, s, c.dinst, P, c.dcm_owner,
+                                   c.dcm_dfirst, no, base);
+            });
+        } else if (p->adjoint) {
+            pairs(c.hcm_first.back(), [&](dim3 grid, int base) {
+                hipLaunchKernelGGL((coxmunk_fourier<true, double, 0>), grid, blk, 0, s, c.dinst, P, c.dcm_owner,
+                                   c.dcm_first, no, base);
+            });
+        } else {
+            pairs(c.hcm_first.back(), [&](dim3 grid, int base) {
+                hipLaunchKernelGGL((coxmunk_fourier<false, double, 0>), grid, blk, 0, s, c.dinst, P, c.dcm_owner,
+                                   c.dcm_first, no, base);
+            });
         }
     }
     hipLaunchKernelGGL(angular_basis, dim3(nit), dim3(128), 0, s, c.dinst, c.ditem, P);

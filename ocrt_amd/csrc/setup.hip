@@ -251,15 +251,21 @@ __global__ void medium_transmit(const DevInst *__restrict__ inst, Pools P)
     }
 }
 
-// cos(m phi_p), sin(m phi_p) on the midpoint azimuth rule, [m][2][kNPhi].
-__global__ void fourier_weights(double *w, int mmax)
+// cos(m phi_p), sin(m phi_p) on the midpoint azimuth rule, [m][2][kNPhi],
+// and (wf non-null) their fp32 roundings in the same layout.
+__global__ void fourier_weights(double *w, float *wf, int mmax)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = blockIdx.y;
     if (p >= kNPhi || m > mmax) return;
     const double phi = (p + 0.5) * (2.0 * OCRT_PI / (double)kNPhi);
-    w[((long long)m * 2 + 0) * kNPhi + p] = cos(m * phi);
-    w[((long long)m * 2 + 1) * kNPhi + p] = sin(m * phi);
+    const double c = cos(m * phi), s = sin(m * phi);
+    w[((long long)m * 2 + 0) * kNPhi + p] = c;
+    w[((long long)m * 2 + 1) * kNPhi + p] = s;
+    if (wf) {
+        wf[((long long)m * 2 + 0) * kNPhi + p] = (float)c;
+        wf[((long long)m * 2 + 1) * kNPhi + p] = (float)s;
+    }
 }
 
 // cos(m phi_q) on the value kernel's half circle, [q][w_rows] (mode fastest):
@@ -295,14 +301,22 @@ __global__ void value_weights(double *w, int w_rows)
 // azimuths over the lanes), forms its products, and transposed wavefront
 // reductions (three entries at a time) leave each row's totals in one group of
 // eight lanes; the four wavefronts' totals are added through LDS.
-template <bool DERIV>
+//
+// RT = float (mixed-form plans, diffuse pairs only): the Mueller matrices, the
+// Fourier products and the wavefront reductions in fp32 from the fp32 weight
+// table, the four wavefronts' totals added in fp64, and only the fp32 copy
+// Rf written (the mixed form reads no other copy of the diffuse kernels).
+template <bool DERIV, typename RT = double>
 __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
 {
     constexpr int kHalf = kNPhi / 2;
     static_assert(kHalf == 2 * kCoxMunkThreads, "two azimuths per thread");
+    static_assert(!DERIV || sizeof(RT) == 8, "derivatives in fp64 only");
+    constexpr bool F32 = sizeof(RT) == 4;
     constexpr int kWaves = kCoxMunkThreads / 64;
     constexpr int kRowModes = DERIV ? 4 : 8;   // Fourier modes per batch
-    __shared__ double red[kWaves][8][9];
+    __shared__ RT red[kWaves][8][9];
+    const RT *cmw = F32 ? reinterpret_cast<const RT *>(P.cm_wf) : reinterpret_cast<const RT *>(P.cm_w);
     const int n = I.n;
     const double *mu = P.ring + I.off_ring;
     const double *w = mu + n;
@@ -315,7 +329,7 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
             const int nout = (I.mmax + 1) * 9;
             for (int t = threadIdx.x; t < nout; t += blockDim.x) {
                 const int m = t / 9, kl = t % 9;
-                P.R[r_index(I, m, o, i, kl)] = 0.0;
+                if (!F32) P.R[r_index(I, m, o, i, kl)] = 0.0;
                 if (P.Rf) P.Rf[r_index(I, m, o, i, kl)] = 0.f;
                 if (DERIV) P.dR[r_index(I, m, o, i, kl)] = 0.0;
             }
@@ -331,33 +345,34 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
     const double s2v = slope_variance(I.wind, I.sigma_type);
     const double shadow = sancer_shadow(mo, mi, s2v);
     const double dls = DERIV ? sancer_dlog(mo, mi, s2v, shadow) : 0.0;
-    const CoxMunkPair cm(mo, mi, s2v, shadow, I.n_water, I.q_conv);
-    double r[2][9], dl[2];
+    const CoxMunkPairT<RT> cm(mo, mi, s2v, shadow, I.n_water, I.q_conv);
+    RT r[2][9];
+    double dl[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         // azimuth (p + 1/2) dphi: its cosine and sine are the m = 1 row of the
         // Fourier weight table
         const int p = t + kCoxMunkThreads * q;
-        double tb2;
-        cm.eval(P.cm_w[2 * kNPhi + p], P.cm_w[3 * kNPhi + p], r[q], tb2);
-        dl[q] = (DERIV && tb2 >= 0.0) ? tb2 / (s2v * s2v) - 1.0 / s2v + dls : 0.0;
+        RT tb2;
+        cm.eval(cmw[2 * kNPhi + p], cmw[3 * kNPhi + p], r[q], tb2);
+        dl[q] = (DERIV && tb2 >= 0.0) ? (double)tb2 / (s2v * s2v) - 1.0 / s2v + dls : 0.0;
     }
     const int M = I.mmax;
     // one reduction over the entries [E0, E0 + C) of a batch's eight rows
     auto reduce_entries = [&](int m0, auto e0, auto cnt) {
         constexpr int E0 = decltype(e0)::value, C = decltype(cnt)::value;
-        double v[8 * C];   // [row][entry - E0]
+        RT v[8 * C];   // [row][entry - E0]
 #pragma unroll
         for (int j = 0; j < kRowModes; ++j) {
             const int m = m0 + j <= M ? m0 + j : M;   // rows past the last mode are computed, not written
-            const double *wc = P.cm_w + (long long)m * 2 * kNPhi, *ws = wc + kNPhi;
+            const RT *wc = cmw + (long long)m * 2 * kNPhi, *ws = wc + kNPhi;
             const int t1 = t + kCoxMunkThreads;
-            const double c0 = wc[t], c1 = wc[t1], s0 = ws[t], s1 = ws[t1];
+            const RT c0 = wc[t], c1 = wc[t1], s0 = ws[t], s1 = ws[t1];
 #pragma unroll
             for (int e = 0; e < C; ++e) {
                 const int kl = E0 + e;
                 const bool use_cos = (kl == 0 || kl == 1 || kl == 3 || kl == 4 || kl == 8);
-                const double a0 = r[0][kl] * (use_cos ? c0 : s0), a1 = r[1][kl] * (use_cos ? c1 : s1);
+                const RT a0 = r[0][kl] * (use_cos ? c0 : s0), a1 = r[1][kl] * (use_cos ? c1 : s1);
                 if (DERIV) {
                     v[(2 * j) * C + e] = a0 + a1;
                     v[(2 * j + 1) * C + e] = a0 * dl[0] + a1 * dl[1];
@@ -385,7 +400,7 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
             const int row = t / 9, kl = t % 9;
             double tot = 0.0;
 #pragma unroll
-            for (int wv = 0; wv < kWaves; ++wv) tot += red[wv][row][kl];
+            for (int wv = 0; wv < kWaves; ++wv) tot += (double)red[wv][row][kl];
             const int m = m0 + (DERIV ? row >> 1 : row);
             if (m <= M) {
                 const double nrm = (m == 0) ? 1.0 / (2.0 * OCRT_PI) : 1.0 / OCRT_PI;
@@ -396,7 +411,7 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
                 if (DERIV && (row & 1)) {
                     P.dR[at] = val;
                 } else {
-                    P.R[at] = val;
+                    if (!F32) P.R[at] = val;
                     if (P.Rf && !solar) P.Rf[at] = (float)val;
                 }
             }
@@ -404,15 +419,15 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
         __syncthreads();
     }
     if (tail > 0) {
-        double v[24];
+        RT v[24];
         const int t1 = t + kCoxMunkThreads;
 #pragma unroll
         for (int q = 0; q < 24; ++q) {
             const int mq = m_tail + (q < 9 * tail ? q / 9 : 0), kl = q % 9;
-            const double *wc = P.cm_w + (long long)mq * 2 * kNPhi, *ws = wc + kNPhi;
+            const RT *wc = cmw + (long long)mq * 2 * kNPhi, *ws = wc + kNPhi;
             const bool use_cos = (kl == 0 || kl == 1 || kl == 3 || kl == 4 || kl == 8);
             v[q] = (q < 9 * tail) ? r[0][kl] * (use_cos ? wc[t] : ws[t]) + r[1][kl] * (use_cos ? wc[t1] : ws[t1])
-                                  : 0.0;
+                                  : (RT)0;
         }
         wave_sum_8xC<3>(v, lane);
         if ((lane & 7) == 0)
@@ -423,18 +438,20 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
             const int row = t / 3, e = t % 3, m = m_tail + t / 9, kl = t % 9;
             double tot = 0.0;
 #pragma unroll
-            for (int wv = 0; wv < kWaves; ++wv) tot += red[wv][row][e];
+            for (int wv = 0; wv < kWaves; ++wv) tot += (double)red[wv][row][e];
             const double nrm = (m == 0) ? 1.0 / (2.0 * OCRT_PI) : 1.0 / OCRT_PI;
             double val = 2.0 * tot * (dphi * nrm);
             if (m > 0 && (kl == 2 || kl == 5)) val = -val;
             const long long at = !solar ? r_index(I, m, o, i, kl) : I.off_Rsun + ((long long)m * n + o) * 9 + kl;
-            P.R[at] = val;
+            if (!F32) P.R[at] = val;
             if (P.Rf && !solar) P.Rf[at] = (float)val;
         }
     }
 }
 
-template <bool DERIV>
+// SET 0: every pair of an owner (n^2 diffuse, then n solar); 1: its solar
+// pairs only; 2: its diffuse pairs only.  first[] counts the pairs of the set.
+template <bool DERIV, typename RT, int SET>
 // three workgroups per CU: the register budget that keeps the Mueller
 // evaluations free of spills (measured faster than two or four)
 __global__ __launch_bounds__(kCoxMunkThreads, 3) void coxmunk_fourier(const DevInst *__restrict__ inst, Pools P,
@@ -450,11 +467,14 @@ __global__ __launch_bounds__(kCoxMunkThreads, 3) void coxmunk_fourier(const DevI
         const int mid = (lo + hi) >> 1;
         if (first[mid] <= b) lo = mid; else hi = mid;
     }
-    coxmunk_pair<DERIV>(inst[owners[lo]], P, b - first[lo]);
+    const DevInst &I = inst[owners[lo]];
+    coxmunk_pair<DERIV, RT>(I, P, b - first[lo] + (SET == 1 ? I.n * I.n : 0));
 }
 
-template __global__ void coxmunk_fourier<false>(const DevInst *, Pools, const int *, const int *, int, int);
-template __global__ void coxmunk_fourier<true>(const DevInst *, Pools, const int *, const int *, int, int);
+template __global__ void coxmunk_fourier<false, double, 0>(const DevInst *, Pools, const int *, const int *, int, int);
+template __global__ void coxmunk_fourier<true, double, 0>(const DevInst *, Pools, const int *, const int *, int, int);
+template __global__ void coxmunk_fourier<false, double, 1>(const DevInst *, Pools, const int *, const int *, int, int);
+template __global__ void coxmunk_fourier<false, float, 2>(const DevInst *, Pools, const int *, const int *, int, int);
 
 // ---------------------------------------------------------------------------
 // Signed direction cosine of direction index d = j + n, j in [-n, n]; j = 0 is

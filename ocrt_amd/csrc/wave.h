@@ -55,22 +55,39 @@ template <int CTRL> __device__ inline double lanes_dpp(double v)
     return join32(h, l);
 }
 
+// The same moves for a float: one 32-bit lane value.
+template <int OFF> __device__ inline void lanes_swap(float &x, float &y)
+{
+    lanes_u2 r;
+    if constexpr (OFF == 32)
+        r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    else
+        r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    x = __uint_as_float(r.x);
+    y = __uint_as_float(r.y);
+}
+
+template <int CTRL> __device__ inline float lanes_dpp(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
 // One halving exchange of a transposed wavefront reduction: lanes with
 // (lane & off) == 0 keep the first H values, their partners the last H, and
 // each adds what the partner sends for the half it keeps.  After a permlane
 // swap of (v[i], v[H + i]) both halves are in place: every lane adds the two.
-template <int H, int OFF>
-__device__ inline void halve_exchange(double *v, bool upper)
+template <int H, int OFF, typename T>
+__device__ inline void halve_exchange(T *v, bool upper)
 {
 #pragma unroll
     for (int i = 0; i < H; ++i) {
         if constexpr (OFF >= 16) {
-            double x = v[i], y = v[H + i];
+            T x = v[i], y = v[H + i];
             lanes_swap<OFF>(x, y);
             v[i] = x + y;
         } else {
-            const double give = upper ? v[i] : v[H + i];
-            const double keep = upper ? v[H + i] : v[i];
+            const T give = upper ? v[i] : v[H + i];
+            const T keep = upper ? v[H + i] : v[i];
             v[i] = keep + lanes_dpp<0x128>(give);   // OFF = 8
         }
     }
@@ -80,8 +97,8 @@ __device__ inline void halve_exchange(double *v, bool upper)
 // (instead of 8 C x 6 for one reduction per value), all on the VALU.  With
 // the values ordered [8][C], every lane of the group of eight lanes
 // l = 8 r .. 8 r + 7 ends with the totals of row r in v[0..C-1].
-template <int C>
-__device__ inline void wave_sum_8xC(double (&v)[8 * C], int lane)
+template <int C, typename T>
+__device__ inline void wave_sum_8xC(T (&v)[8 * C], int lane)
 {
     halve_exchange<4 * C, 32>(v, (lane & 32) != 0);
     halve_exchange<2 * C, 16>(v, (lane & 16) != 0);
