@@ -39,10 +39,12 @@ TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
                                "sos_traffic_8192inst_mixed.md")
 
 
-def measured_traffic(geoms):
-    """HBM bytes per launch of gemm_bf16 and of the sweep, measured with
-    rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-calibrated) on this workload and
-    committed with the profile summary; None for another workload size."""
+def measured_traffic(geoms, gemm_tuning, sweep_tuning):
+    """HBM bytes per launch of the scattering integral (the classes that ran,
+    each with its chosen tile shape, summed) and of the sweep (its chosen
+    launch shape), measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (copy-calibrated) on this workload and committed with the profile
+    summary; None for another workload size or a shape not profiled."""
     try:
         lines = open(TRAFFIC_PROFILE).read().splitlines()
     except OSError:
@@ -53,7 +55,15 @@ def measured_traffic(geoms):
             kv = dict(zip(tok[0::2], tok[1::2]))
             if kv.get("workload") != f"geoms={geoms}":
                 return None
-            return {"gemm_bf16": float(kv["gemm_bf16"]), "sweep": float(kv["sweep"])}
+            try:
+                gemm = 0.0
+                for k, (tm, tn) in enumerate(gemm_tuning["shape_of_class"]):
+                    if gemm_tuning["ms"] is None or max(gemm_tuning["ms"][k]) > 0:
+                        gemm += float(kv[f"gemm_c{k}_{tm}x{tn}"])
+                sweep = float(kv[f"sweep_{sweep_tuning['unroll']}x{sweep_tuning['waves']}"])
+            except KeyError:
+                return None
+            return {"gemm_bf16": gemm, "sweep": sweep}
     return None
 
 
@@ -140,6 +150,60 @@ def survey_work(descs, orders, n_ring):
     return F, B
 
 
+def adjoint_leg(sync):
+    """Config-3 Jacobian timing: 256 instances, 200 layers, 32 streams,
+    modes 0..16, checkpoints every 4 orders."""
+    import ocrt_amd
+    from ocrt_amd import synthetic as S
+    log("adjoint workload: 256 instances, 200 layers, 32 streams, modes 0..16")
+    d3, k3 = S.config3()
+    g3 = np.tile(np.array([1.0, 0.5, -0.5]), (len(d3), 1))
+
+    def jacobian_times(prec):
+        """(forward with checkpoints every 4 orders + backward, backward
+        alone) in seconds: the Jacobian of one new input, and of one more
+        output weighting on the same forward."""
+        pa = ocrt_amd.Plan(d3, precision=prec, adjoint=True, ckpt_sparse=True)
+        pa.execute()
+        pa.adjoint(g3)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            pa.execute()
+            pa.adjoint(g3)
+        sync()
+        t_full = (time.perf_counter() - t0) / 3
+        t0 = time.perf_counter()
+        for _ in range(3):
+            pa.adjoint(g3)
+        sync()
+        t_bwd = (time.perf_counter() - t0) / 3
+        pa.close()
+        return t_full, t_bwd
+
+    t_j64, t_b64 = jacobian_times("fp64")
+    t_jx3, t_bx3 = jacobian_times("mixed")
+    pf = ocrt_amd.Plan(d3, precision="fp64")
+    t_fwd = timed_steps(pf, 3, 1, sync)
+    pf.close()
+    pm3 = ocrt_amd.Plan(d3, precision="mixed")
+    t_mix = timed_steps(pm3, 3, 1, sync)
+    pm3.close()
+    adjoint = {"instances": len(d3), "checkpoint_every": 4,
+               "fp64_forward_ms": t_fwd * 1e3, "mixed_forward_ms": t_mix * 1e3,
+               "jacobian_fp64_ms": t_j64 * 1e3, "backward_only_fp64_ms": t_b64 * 1e3,
+               "jacobian_split_bf16_ms": t_jx3 * 1e3, "backward_only_split_bf16_ms": t_bx3 * 1e3,
+               "jacobian_fp64_over_mixed_forward": t_j64 / t_mix,
+               "jacobian_fp64_over_fp64_forward": t_j64 / t_fwd,
+               "jacobian_split_bf16_over_mixed_forward": t_jx3 / t_mix,
+               "jacobian_rows_per_s_split_bf16": len(d3) / t_jx3,
+               "note": "jacobian = checkpointing forward + backward recursion (recomputing the orders "
+                       "between checkpoints) for one new input; split_bf16: fp64 fields and sweeps, "
+                       "operator products on bf16-pair MFMA (agrees with the fp64 adjoint to ~5e-6)"}
+    log(f"adjoint: {adjoint}")
+    return adjoint
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +219,8 @@ def main():
     ap.add_argument("--no-ocean", action="store_true")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time only the mixed form (for counter-collection runs)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="time only the Jacobian workload (for profiling runs of the adjoint kernels)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (3 times): the reference "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -178,6 +244,9 @@ def main():
         sync()
         log(f"copy calibration: 3 copies of {nbytes} bytes (read {nbytes}, write {nbytes} each)")
         del src, dst
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_leg(sync)}))
+        return
     log("building workload")
     descs, keep = S.config2(n_geom=args.geoms)
     n = len(descs)
@@ -229,54 +298,7 @@ def main():
     pp.close()
     plain_rate = len(plain) / (ms_plain / 1e3)
 
-    adjoint = None
-    if not args.no_adjoint:
-        log("adjoint workload: 256 instances, 200 layers, 32 streams, modes 0..16")
-        d3, k3 = S.config3()
-        g3 = np.tile(np.array([1.0, 0.5, -0.5]), (len(d3), 1))
-
-        def jacobian_times(prec):
-            """(forward with checkpoints every 4 orders + backward, backward
-            alone) in seconds: the Jacobian of one new input, and of one more
-            output weighting on the same forward."""
-            pa = ocrt_amd.Plan(d3, precision=prec, adjoint=True, ckpt_sparse=True)
-            pa.execute()
-            pa.adjoint(g3)
-            sync()
-            t0 = time.perf_counter()
-            for _ in range(3):
-                pa.execute()
-                pa.adjoint(g3)
-            sync()
-            t_full = (time.perf_counter() - t0) / 3
-            t0 = time.perf_counter()
-            for _ in range(3):
-                pa.adjoint(g3)
-            sync()
-            t_bwd = (time.perf_counter() - t0) / 3
-            pa.close()
-            return t_full, t_bwd
-
-        t_j64, t_b64 = jacobian_times("fp64")
-        t_jx3, t_bx3 = jacobian_times("mixed")
-        pf = ocrt_amd.Plan(d3, precision="fp64")
-        t_fwd = timed_steps(pf, 3, 1, sync)
-        pf.close()
-        pm3 = ocrt_amd.Plan(d3, precision="mixed")
-        t_mix = timed_steps(pm3, 3, 1, sync)
-        pm3.close()
-        adjoint = {"instances": len(d3), "checkpoint_every": 4,
-                   "fp64_forward_ms": t_fwd * 1e3, "mixed_forward_ms": t_mix * 1e3,
-                   "jacobian_fp64_ms": t_j64 * 1e3, "backward_only_fp64_ms": t_b64 * 1e3,
-                   "jacobian_split_bf16_ms": t_jx3 * 1e3, "backward_only_split_bf16_ms": t_bx3 * 1e3,
-                   "jacobian_fp64_over_mixed_forward": t_j64 / t_mix,
-                   "jacobian_fp64_over_fp64_forward": t_j64 / t_fwd,
-                   "jacobian_split_bf16_over_mixed_forward": t_jx3 / t_mix,
-                   "jacobian_rows_per_s_split_bf16": len(d3) / t_jx3,
-                   "note": "jacobian = checkpointing forward + backward recursion (recomputing the orders "
-                           "between checkpoints) for one new input; split_bf16: fp64 fields and sweeps, "
-                           "operator products on bf16-pair MFMA (agrees with the fp64 adjoint to ~5e-6)"}
-        log(f"adjoint: {adjoint}")
+    adjoint = None if args.no_adjoint else adjoint_leg(sync)
 
     ocean_leg = None
     if not args.no_ocean:
@@ -334,7 +356,7 @@ def main():
     F_alg, B_alg = survey_work(descs, res_m["orders_per_m"], res_m["n_ring"])
     gemm_tf = F_alg / gemm_s / 1e12 if gemm_s > 0 else 0.0
     sweep_gbs = B_alg / sweep_s / 1e9 if sweep_s > 0 else 0.0
-    traffic = measured_traffic(args.geoms)
+    traffic = measured_traffic(args.geoms, gemm_tuning, tuning)
     gl, sl = max(kt["gemm_launches"], 1), max(kt["sweep_launches"], 1)
     rl_gemm = {"kernel": "gemm_bf16", "bound": "mfma", "achieved": gemm_tf, "peak": BF16_PEAK_TFLOPS,
                "unit": "TFLOP/s", "frac": gemm_tf / BF16_PEAK_TFLOPS,
