@@ -1,4 +1,4 @@
-// Adjoint of the SOS solve (fp64).  For one Fourier mode the forward solve is
+// Adjoint of the SOS solve.  For one Fourier mode the forward solve is
 //   u_1 = sweep(S_1) (+ rough-surface seed),
 //   u_o = sweep(M u_{o-1}, R B u_{o-1}),  o = 2..N,
 //   X   = sum_o u_o at the TOA view node (minus the rough seed there),
@@ -8,6 +8,10 @@
 // (Sbar_o, bbar_o) is the adjoint sweep of lambda_o.  The forward fields are
 // kept for every order, or checkpointed every 2 or 4 orders and recomputed
 // segment by segment (plan.hip).
+// Two storage forms share these kernels: fp64 plans keep fp64 fields and an
+// fp64 adjoint (FieldF64, double); mixed plans keep every forward field as its
+// bf16 pair and the adjoint fields in fp32 (FieldPair, float).  The arithmetic
+// of every kernel here is fp64 in both.
 // Gradients accumulate per instance with respect to the level optical depths
 // (explicit dependence gH, layer steps gD), the scattering fractions gX, gY
 // and the wind speed gW.  The reference has no adjoint; the finite-difference
@@ -17,7 +21,8 @@
 
 namespace ocrt {
 
-__global__ void fill_f64(double *__restrict__ p, long long n, double v)
+template <typename T>
+__global__ void fill(T *__restrict__ p, long long n, T v)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         p[i] = v;
@@ -34,11 +39,12 @@ __device__ inline double seed_weight(const DevInst &I, const Pools &P, int m, in
     return P.gout[3LL * I.gidx + c] * wt / I.mus;
 }
 
-// lambda += e at the TOA view entries of one item.
+// lambda += e at the TOA view entries of one item (LT: lambda's storage).
+template <typename LT>
 __device__ inline void add_seed(const DevInst &I, const DevItem &it, const Pools &P)
 {
-    double *lam = P.lam + it.off_F;
-    for (int c = 0; c < 3; ++c) lam[col_up(it.n, c, I.vidx)] += seed_weight(I, P, it.m, c);
+    LT *lam = reinterpret_cast<LT *>(P.lam) + it.off_F;
+    for (int c = 0; c < 3; ++c) lam[col_up(it.n, c, I.vidx)] += (LT)seed_weight(I, P, it.m, c);
 }
 
 // Zero the instance accumulators (one thread per entry of one instance).
@@ -50,24 +56,27 @@ __global__ void adj_zero_grad(const DevInst *__restrict__ inst, Pools P)
         P.grad[I.off_grad + e] = 0.0;
 }
 
+template <typename LT>
 __global__ void adj_seed(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P)
 {
     const DevItem &it = items[blockIdx.x];
-    if (threadIdx.x == 0) add_seed(inst[it.inst], it, P);
+    if (threadIdx.x == 0) add_seed<LT>(inst[it.inst], it, P);
 }
 
 // Bottom downward field of the field in P.F into bc + 3n (order `order`
-// reads the field of order - 1).
+// reads the field of order - 1; FV: the field's storage).
+template <typename FV>
 __global__ void adj_bottom(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P,
                            const ItemState *__restrict__ st, int order)
 {
     const DevItem &it = items[blockIdx.x];
     if (!item_runs(st[blockIdx.x], order)) return;
     const int n = it.n;
-    const double *F = reinterpret_cast<const double *>(P.F) + it.off_F + (long long)it.nt * it.P;
+    const FV F(P, it, 0);
+    const long long bot = (long long)it.nt * it.P;
     for (int q = threadIdx.x; q < 3 * n; q += blockDim.x) {
         const int c = q / n, o = q % n;
-        P.bc[it.off_bc + 3 * n + q] = F[col_dn(n, c, o)];
+        P.bc[it.off_bc + 3 * n + q] = F[bot + col_dn(n, c, o)];
     }
 }
 
@@ -97,13 +106,15 @@ __global__ void adj_step_table(const DevInst *__restrict__ inst, Pools P)
         step_coefs(dt, mu[o], c[6 * o], c[6 * o + 1], c[6 * o + 2], c[6 * o + 3], c[6 * o + 4], c[6 * o + 5]);
 }
 
-// Adjoint sweep of order `order` >= 2.  P.F holds u_o, P.J / P.J2 the
-// unscaled operator parts A_0 u_{o-1} / A_1 u_{o-1}, P.lam lambda_o.  Writes
+// Adjoint sweep of order `order` >= 2.  P.F holds u_o (FV: its storage), P.J /
+// P.J2 the unscaled operator parts A_0 u_{o-1} / A_1 u_{o-1}, P.lam lambda_o
+// (both LT: double in fp64 plans, float in mixed ones; the arithmetic is fp64).  Writes
 // Sbar_o over P.J, bbar_o over the boundary slots of P.bc, and accumulates
 // gX, gY (Sbar . part) per level and gD per layer.  One thread per column,
 // one 64-column tile (of one direction) per wavefront, walking each column
 // against its forward direction; the per-level sums of eight steps are
 // reduced over the wavefront together.
+template <typename FV, typename LT>
 __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ inst, const DevItem *__restrict__ items,
                                                  Pools P, const ItemState *__restrict__ st,
                                                  const ColTile *__restrict__ tiles, int n_tiles, int order)
@@ -125,10 +136,10 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
     int c = 0, hh = 0, o = 0;
     if (live) col_decode(col, n, c, hh, o);
     const LevelView lv(P, I);
-    const double *__restrict__ U = reinterpret_cast<const double *>(P.F) + off_F + col;
-    double *__restrict__ J0 = reinterpret_cast<double *>(P.J) + off_F + col;
-    const double *__restrict__ J1 = reinterpret_cast<const double *>(P.J2) + off_F + col;
-    const double *__restrict__ lam = P.lam + off_F + col;
+    const FV U(P, it, col);
+    LT *__restrict__ J0 = reinterpret_cast<LT *>(P.J) + off_F + col;
+    const LT *__restrict__ J1 = reinterpret_cast<const LT *>(P.J2) + off_F + col;
+    const LT *__restrict__ lam = reinterpret_cast<const LT *>(P.lam) + off_F + col;
     double *gX = P.grad + I.off_grad + L, *gY = gX + L, *gD = gY + L;
     const bool two = nparts > 1;
     auto source = [&](int k, double p0, double p1) {
@@ -148,8 +159,9 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
         }
     };
     const int k0 = hemi ? nt : 0;
-    double rho = live ? lam[(long long)k0 * Pp] : 0.0;
-    double pk0 = live ? J0[(long long)k0 * Pp] : 0.0, pk1 = (live && two) ? J1[(long long)k0 * Pp] : 0.0;
+    double rho = live ? (double)lam[(long long)k0 * Pp] : 0.0;
+    double pk0 = live ? (double)J0[(long long)k0 * Pp] : 0.0;
+    double pk1 = (live && two) ? (double)J1[(long long)k0 * Pp] : 0.0;
     double sk = source(k0, pk0, pk1);
     double pending = 0.0;
     constexpr int kB = 8;
@@ -161,10 +173,10 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
             const int tc = t0 + u < nt ? t0 + u : nt - 1;
             const int kp = hemi ? nt - tc - 1 : tc + 1;
             const long long ix = (long long)kp * Pp;
-            q0[u] = live ? J0[ix] : 0.0;
-            q1[u] = (live && two) ? J1[ix] : 0.0;
+            q0[u] = live ? (double)J0[ix] : 0.0;
+            q1[u] = (live && two) ? (double)J1[ix] : 0.0;
             uv[u] = live ? U[ix] : 0.0;
-            lv_[u] = live ? lam[ix] : 0.0;
+            lv_[u] = live ? (double)lam[ix] : 0.0;
         }
         // per step: d/d xdel, d/d ydel at level k and d/d(layer step)
         double red[3 * kB];
@@ -183,7 +195,7 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
             const double sbar = pending + al * rho;
             pending = be * rho;
             if (live) {
-                J0[(long long)k * Pp] = sbar;
+                J0[(long long)k * Pp] = (LT)sbar;
                 red[3 * u + 0] = (kind0 == 1) ? sbar * pk0 : 0.0;
                 red[3 * u + 1] = (kind0 == 1) ? sbar * pk1 : sbar * pk0;
                 red[3 * u + 2] = rho * (dT * uv[u] + dal * sk + dbe * sp);
@@ -202,7 +214,7 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
     }
     // last level of the walk: nt for upward columns, 0 for downward ones
     const int kl = hemi ? 0 : nt;
-    if (live) J0[(long long)kl * Pp] = pending;
+    if (live) J0[(long long)kl * Pp] = (LT)pending;
     level_sums(kl, pending, pk0, pk1);
     // the upward columns start from the boundary condition: its adjoint is rho
     if (live && hemi == 0 && I.surf != 0) P.bc[off_bc + c * n + o] = rho;
@@ -212,19 +224,21 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
 // entries) + B^T R^T bbar_o (bottom downward entries), and the wind gradient
 // bbar_o . dR/dwind . u_{o-1}(bottom), with P.F = u_{o-1}.  One workgroup
 // (64 threads) per item.
+template <typename FV, typename LT>
 __global__ void surface_adj(const DevInst *__restrict__ inst, const DevItem *__restrict__ items, Pools P,
                             const ItemState *__restrict__ st, int order)
 {
     const DevItem &it = items[blockIdx.x];
     if (!item_runs(st[blockIdx.x], order)) return;
     const DevInst &I = inst[it.inst];
-    if (threadIdx.x == 0) add_seed(I, it, P);
+    if (threadIdx.x == 0) add_seed<LT>(I, it, P);
     if (I.surf == 0) return;
     const int n = it.n, nt = it.nt, m = it.m, L = nt + 1;
     const double *mu = P.ring + I.off_ring, *w = mu + n;
     const double *bb = P.bc + it.off_bc;   // [3][n] adjoint of the boundary condition
-    double *lamb = P.lam + it.off_F + (long long)nt * it.P;
-    const double *Ub = reinterpret_cast<const double *>(P.F) + it.off_F + (long long)nt * it.P;
+    LT *lamb = reinterpret_cast<LT *>(P.lam) + it.off_F + (long long)nt * it.P;
+    const FV Ub(P, it, 0);
+    const long long bot = (long long)nt * it.P;
     const double W = (I.surf == 1) ? 2.0 * OCRT_PI : ((m == 0) ? 2.0 * OCRT_PI : OCRT_PI);
     double gw = 0.0;
     if (I.surf == 1) {
@@ -236,7 +250,7 @@ __global__ void surface_adj(const DevInst *__restrict__ inst, const DevItem *__r
                              : (W * mu[i] * w[i]) * (1.0 / (2.0 * OCRT_PI * mu[i] * w[i]));
             for (int c2 = 0; c2 < 3; ++c2)
                 acc[c2] = f * (bb[i] * Fr[c2] + bb[n + i] * Fr[3 + c2] + bb[2 * n + i] * Fr[6 + c2]);
-            for (int c2 = 0; c2 < 3; ++c2) lamb[col_dn(n, c2, i)] += acc[c2];
+            for (int c2 = 0; c2 < 3; ++c2) lamb[col_dn(n, c2, i)] += (LT)acc[c2];
         }
     } else {
         // rough surface, one wavefront: for every incoming node i the lanes
@@ -251,7 +265,8 @@ __global__ void surface_adj(const DevInst *__restrict__ inst, const DevItem *__r
                 double a0 = 0.0, a1 = 0.0, a2 = 0.0;
                 if (i0 + j < n && w[i] != 0.0) {
                     const double pw = W * mu[i] * w[i];
-                    const double u0 = Ub[col_dn(n, 0, i)], u1 = Ub[col_dn(n, 1, i)], u2 = Ub[col_dn(n, 2, i)];
+                    const double u0 = Ub[bot + col_dn(n, 0, i)], u1 = Ub[bot + col_dn(n, 1, i)],
+                                 u2 = Ub[bot + col_dn(n, 2, i)];
                     // entry kl of R^m[op][i] at R[kl n + op]
                     const double *R = P.R + r_index(I, m, 0, i, 0);
                     const double *dR = P.dR + r_index(I, m, 0, i, 0);
@@ -271,7 +286,7 @@ __global__ void surface_adj(const DevInst *__restrict__ inst, const DevItem *__r
             wave_sum_8xC_lds<3>(v, lane);
             const int i = i0 + (lane >> 3);
             if ((lane & 7) == 0 && i < n)
-                for (int c2 = 0; c2 < 3; ++c2) lamb[col_dn(n, c2, i)] += v[c2];
+                for (int c2 = 0; c2 < 3; ++c2) lamb[col_dn(n, c2, i)] += (LT)v[c2];
         }
     }
     if (I.surf == 2) {
@@ -285,6 +300,7 @@ __global__ void surface_adj(const DevInst *__restrict__ inst, const DevItem *__r
 // the combined single-scattering and flat-seed source, the explicit
 // dependence of that source on xdel, ydel and h, and the rough direct-beam
 // seed (which enters u_1 and leaves the TOA output).
+template <typename FV, typename LT>
 __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__ inst,
                                                      const DevItem *__restrict__ items, Pools P,
                                                      const ColTile *__restrict__ tiles, int n_tiles)
@@ -306,8 +322,8 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
     const LevelView lv(P, I);
     const ColumnSource src(I, it, P, lv, live ? col : 0);
     const double mus = I.mus, amu = P.ring[I.off_ring + o];
-    const double *__restrict__ U = reinterpret_cast<const double *>(P.F) + it.off_F + col;
-    const double *__restrict__ lam = P.lam + it.off_F + col;
+    const FV U(P, it, col);
+    const LT *__restrict__ lam = reinterpret_cast<const LT *>(P.lam) + it.off_F + col;
     double *gH = P.grad + I.off_grad, *gX = gH + L, *gY = gX + L, *gD = gY + L, *gW = gD + nt;
     // rough direct-beam seed of the upward columns
     const bool rough = live && I.surf == 2 && hemi == 0;
@@ -345,7 +361,7 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
         }
     };
     const int k0 = hemi ? nt : 0;
-    double rho = live ? lam[(long long)k0 * Pp] : 0.0;
+    double rho = live ? (double)lam[(long long)k0 * Pp] : 0.0;
     double lam_k = rho;
     double sk = S(k0);
     double pending = 0.0;
@@ -358,7 +374,7 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
             const int tc = t0 + u < nt ? t0 + u : nt - 1;
             const long long ix = (long long)(hemi ? nt - tc - 1 : tc + 1) * Pp;
             uv[u] = live ? U[ix] : 0.0;
-            lv_[u] = live ? lam[ix] : 0.0;
+            lv_[u] = live ? (double)lam[ix] : 0.0;
         }
         // per step: d/d xdel, d/d ydel, d/dh at level k and d/d(layer step)
         double red[4 * kB];
@@ -429,5 +445,19 @@ __global__ void glint_adj(const DevInst *__restrict__ inst, int n_inst, Pools P)
     gH[nt] += -v * (1.0 / I.mus + 1.0 / muv1);
     if (I.wind >= 1e-6) gH[4 * L - 1] += v * dlog * slope_variance_dwind(I.wind, I.sigma_type);
 }
+
+template __global__ void fill<double>(double *, long long, double);
+template __global__ void fill<float>(float *, long long, float);
+template __global__ void adj_seed<double>(const DevInst *, const DevItem *, Pools);
+template __global__ void adj_seed<float>(const DevInst *, const DevItem *, Pools);
+#define OCRT_ADJ_INST(FV, LT)                                                                                  \
+    template __global__ void adj_bottom<FV>(const DevInst *, const DevItem *, Pools, const ItemState *, int);   \
+    template __global__ void sweep_adj<FV, LT>(const DevInst *, const DevItem *, Pools, const ItemState *,      \
+                                               const ColTile *, int, int);                                     \
+    template __global__ void surface_adj<FV, LT>(const DevInst *, const DevItem *, Pools, const ItemState *, int); \
+    template __global__ void order_one_adj<FV, LT>(const DevInst *, const DevItem *, Pools, const ColTile *, int);
+OCRT_ADJ_INST(FieldF64, double)
+OCRT_ADJ_INST(FieldPair, float)
+#undef OCRT_ADJ_INST
 
 }  // namespace ocrt

@@ -133,7 +133,8 @@ struct Pools {
     double *layers;       // explicit layering of the instances that carry one
     double *dR;           // adjoint: d R^m / d sigma^2, layout of R
     float *Rf;            // mixed form: fp32 copy of pool_R read by the per-order rough boundary condition
-    double *lam;          // adjoint: [L][P] per item, adjoint of the current order's field
+    void *lam;            // adjoint: [L][P] per item, adjoint of the current order's field (fp64 plans: double,
+                          // mixed plans: float)
     double *sc;           // adjoint: per layer and node the scan-step coefficients (T, al, be, dT, dal, dbe),
                           // [nt][n][6] at 6 off_T
     double *grad;         // adjoint: per-instance accumulators (see DevInst::off_grad)

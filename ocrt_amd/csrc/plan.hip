@@ -40,21 +40,27 @@ template <int MODE>
 __global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <int NP, bool SPLIT, int TM, int TN>
 __global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
-template <int NP, int MODE>
-__global__ void gemm_x3(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
+template <int NP, int MODE, int FIN>
+__global__ void gemm_adj(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <typename FT, typename JT, bool MIXED, int U, int W>
 __global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int, int);
 __global__ void extract(const DevInst *, const DevItem *, Pools, const ItemState *, ItemOut *, int);
 __global__ void finalize(const DevInst *, int, const ItemOut *, const InstOut *, const Pools, ocrt_result_t *);
 __global__ void ipss_kappa(const DevInst *, const IpssJob *, Pools, double *, double *);
-// adjoint.hip
-__global__ void fill_f64(double *, long long, double);
+// adjoint.hip; FV: FieldF64 (fp64 plans) or FieldPair (mixed plans), LT the
+// matching adjoint storage (double or float)
+struct FieldF64;
+struct FieldPair;
+template <typename T> __global__ void fill(T *, long long, T);
 __global__ void adj_zero_grad(const DevInst *, Pools);
-__global__ void adj_seed(const DevInst *, const DevItem *, Pools);
-__global__ void adj_bottom(const DevInst *, const DevItem *, Pools, const ItemState *, int);
+template <typename LT> __global__ void adj_seed(const DevInst *, const DevItem *, Pools);
+template <typename FV> __global__ void adj_bottom(const DevInst *, const DevItem *, Pools, const ItemState *, int);
 __global__ void adj_surface_bc(const DevInst *, const DevItem *, Pools, const ItemState *, int);
+template <typename FV, typename LT>
 __global__ void sweep_adj(const DevInst *, const DevItem *, Pools, const ItemState *, const ColTile *, int, int);
+template <typename FV, typename LT>
 __global__ void surface_adj(const DevInst *, const DevItem *, Pools, const ItemState *, int);
+template <typename FV, typename LT>
 __global__ void order_one_adj(const DevInst *, const DevItem *, Pools, const ColTile *, int);
 __global__ void glint_adj(const DevInst *, int, Pools);
 // lut.hip
@@ -240,6 +246,7 @@ std::vector<GemmTile> xcd_tile_order(const std::vector<std::vector<GemmTile>> &p
 
 struct ocrt_plan {
     int n_inst = 0, prec = 0, per_inst = 0, mmax_all = 0, n_items = 0;
+    long long mx_F = 0;   // entries of the largest chunk's field
     std::vector<HostInst> inst;
     std::vector<Chunk> chunks;
     std::vector<int> item_of_inst;   // first global item of each instance
@@ -291,7 +298,7 @@ struct ocrt_plan {
     // u_{1+2e}, ... for interval e), the e - 1 recomputed fields of a segment,
     // per-instance accumulators
     int adjoint = 0;
-    int adj_mixed = 0;                // adjoint plan with split-bf16 operator products (gemm_x3)
+    int adj_mixed = 0;                // mixed adjoint plan: bf16-pair fields, fp32 adjoint (backward_chunk)
     int dev_inputs = 0;               // OCRT_PLAN_DEVICE_INPUTS: no shared surface kernels
     int fwd_ready = 0;                // the last execute left this plan's checkpoints intact
     int ckpt_every = 1;
@@ -425,9 +432,10 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                   OCRT_PLAN_GRAPH | OCRT_PLAN_DEVICE_INPUTS))
         return OCRT_ERR_ARGS;
     const bool adjoint = (flags & OCRT_PLAN_ADJOINT) != 0;
-    // the adjoint keeps fp64 fields and needs the plane-parallel solve (no
-    // IPSS); with precision MIXED its operator products take the split-bf16
-    // MFMA form (gemm_x3) instead of fp64 MFMA
+    // the adjoint needs the plane-parallel solve (no IPSS).  With precision
+    // MIXED it runs the mixed order loop with every field kept as its bf16 pair
+    // (16 significant bits; no rank-3 molecular items, no reduced products) and
+    // keeps the adjoint fields in fp32; with FP64 everything is fp64
     const bool adj_mixed = adjoint && precision == OCRT_PREC_MIXED;
     if (adjoint)
         for (int i = 0; i < n_inst; ++i)
@@ -436,15 +444,15 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     *out = nullptr;
     ocrt_plan *p = new ocrt_plan();
     p->n_inst = n_inst;
-    p->prec = adj_mixed ? OCRT_PREC_FP64 : precision;
+    p->prec = precision;
     p->adj_mixed = adj_mixed ? 1 : 0;
     p->per_inst = (flags & OCRT_PLAN_ONE_PER_LAUNCH) ? 1 : 0;
-    p->autotune = (flags & OCRT_PLAN_AUTOTUNE) ? 1 : 0;
+    p->autotune = (flags & OCRT_PLAN_AUTOTUNE) && !adjoint ? 1 : 0;
     p->use_graph = (flags & OCRT_PLAN_GRAPH) && !adjoint ? 1 : 0;
     p->adjoint = adjoint ? 1 : 0;
     p->dev_inputs = (flags & OCRT_PLAN_DEVICE_INPUTS) ? 1 : 0;
     const bool mixed = p->prec == OCRT_PREC_MIXED;
-    const bool gemm_bf = mixed || adj_mixed;   // bf16-pair operator, 32-wide K slabs, 128-level tiles
+    const bool gemm_bf = mixed;   // bf16-pair operator, 32-wide K slabs, 128-level tiles
     const int Palign = gemm_bf ? 32 : 16;
     p->inst.resize(n_inst);
     long long ring_n = 0, greek_n = 0, layers_n = 0;
@@ -639,7 +647,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 b += (double)L * P * (fe + je + (mixed ? 0 : 8)) + (mixed ? 24.0 * P : 0.0) +
                      ((mixed && m < kSplitModes) ? 6.0 * L * P : 0.0) + 6.0 * n * 8;
                 if (adjoint)   // checkpoints, segment fields, lambda, second parts
-                    b += (double)L * P * 8 * ((I.max_orders - 1) / every + 1 + (every - 1) + 2);
+                    b += (double)L * P * (mixed ? 4 : 8) * ((I.max_orders - 1) / every + 1 + (every - 1) + 2);
             }
             if (adjoint) b += (double)h.R_n * 8;
             const int items_after = (item_global - c.it0) + I.mmax + 1;
@@ -728,7 +736,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 int kinds[2], np = 0;
                 // the mixed form takes the molecular operator of an item with
                 // aerosol out of the dense product (rank 3, see DevItem::lowrank)
-                it.lowrank = (mixed && m <= 2 && I.ray && I.aer) ? 1 : 0;
+                it.lowrank = (mixed && !adjoint && m <= 2 && I.ray && I.aer) ? 1 : 0;
                 if (I.aer) kinds[np++] = 1;
                 if (m <= 2 && I.ray && !it.lowrank) kinds[np++] = 2;
                 if (np == 0) kinds[np++] = 1;
@@ -854,7 +862,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                     c.i1, c.it1 - c.it0, c.htiles.size(), c.hcols.size(), c.max_pack, c.tab_groups);
     int rc = 0;
     Pools &P = p->pools;
-    P.lo_tau = mixed ? kReducedTau : 0.f;
+    P.lo_tau = (mixed && !adjoint) ? kReducedTau : 0.f;
     rc |= dmalloc(p, (void **)&p->ring, ring_n * 8);
     rc |= dmalloc(p, (void **)&p->greek, std::max(greek_n, 1LL) * 8);
     rc |= dmalloc(p, (void **)&p->layers, std::max(layers_n, 1LL) * 8);
@@ -883,11 +891,13 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     rc |= dmalloc(p, (void **)&P.Tf, mx_T * 4);
     rc |= dmalloc(p, (void **)&P.idt, mx_lev * 4);
     rc |= dmalloc(p, (void **)&P.R, mx_R * 8);
-    if (mixed) rc |= dmalloc(p, (void **)&P.Rf, std::max(mx_R, 1LL) * 4);
+    // the mixed adjoint reads the fp64 rough kernels (with their derivatives)
+    if (mixed && !adjoint) rc |= dmalloc(p, (void **)&P.Rf, std::max(mx_R, 1LL) * 4);
     // padded: the phase tables read the rows of a full group past the last row
     rc |= dmalloc(p, (void **)&P.basis, (mx_basis + kTabRows) * 8);
     rc |= dmalloc(p, (void **)&P.tab, mx_tab * 8);
     rc |= dmalloc(p, (void **)&P.A, mx_A * ae);
+    p->mx_F = mx_F;
     rc |= dmalloc(p, (void **)&P.F, mx_F * fe);
     rc |= dmalloc(p, (void **)&P.J, mx_F * je);
     if (!mixed) rc |= dmalloc(p, (void **)&P.tot, mx_F * 8);
@@ -903,12 +913,22 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         for (const HostInst &h : p->inst) maxit = std::max(maxit, h.d.max_orders);
         p->ckpt.assign((maxit - 1) / p->ckpt_every + 1, nullptr);
         p->seg.assign(p->ckpt_every - 1, nullptr);
-        for (void *&q : p->ckpt) rc |= dmalloc(p, &q, mx_F * 8);
-        for (void *&q : p->seg) rc |= dmalloc(p, &q, mx_F * 8);
+        // fields: fp64, or the bf16 pair of the mixed plans as two planes of
+        // mx_F entries (hi, then lo); lambda and the second parts fp64 / fp32
+        const size_t fb = mixed ? 4 : 8;
+        for (void *&q : p->ckpt) rc |= dmalloc(p, &q, mx_F * fb);
+        for (void *&q : p->seg) rc |= dmalloc(p, &q, mx_F * fb);
+        // the mixed sweeps write the live columns only: the pad columns of
+        // every checkpoint and segment buffer stay the zeros set here (the
+        // operator products multiply them by zero entries)
+        if (mixed && !rc)
+            for (void *q : p->ckpt) rc |= hipMemset(q, 0, mx_F * fb) != hipSuccess;
+        if (mixed && !rc)
+            for (void *q : p->seg) rc |= hipMemset(q, 0, mx_F * fb) != hipSuccess;
         rc |= dmalloc(p, &p->At, mx_A * ae);
-        rc |= dmalloc(p, (void **)&P.lam, mx_F * 8);
+        rc |= dmalloc(p, (void **)&P.lam, mx_F * fb);
         rc |= dmalloc(p, (void **)&P.sc, mx_T * 6 * 8);
-        rc |= dmalloc(p, (void **)&P.J2, mx_F * 8);
+        rc |= dmalloc(p, (void **)&P.J2, mx_F * fb);
         rc |= dmalloc(p, (void **)&P.dR, std::max(mx_R, 1LL) * 8);
         rc |= dmalloc(p, (void **)&p->grad, p->grad_n * 8);
         rc |= dmalloc(p, (void **)&p->gout, (size_t)n_inst * 3 * 8);
@@ -1055,17 +1075,20 @@ constexpr int kSweepShapeCount = sizeof(kSweepShapes) / sizeof(kSweepShapes[0]);
 // each class's launch (kinds 2 + class).
 template <int NP, bool SP>
 static void launch_gemm_class(int sh, unsigned n, const GemmTile *tiles, const Chunk &c, hipStream_t s,
-                              const Pools &P, const ItemState *st)
+                              const Pools &P, const ItemState *st, int order)
 {
     switch (sh) {
-    case 0: hipLaunchKernelGGL((gemm_bf16<NP, SP, 64, 128>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
-    case 1: hipLaunchKernelGGL((gemm_bf16<NP, SP, 64, 64>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
-    case 2: hipLaunchKernelGGL((gemm_bf16<NP, SP, 128, 64>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
-    default: hipLaunchKernelGGL((gemm_bf16<NP, SP, 128, 128>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, 0); break;
+    case 0: hipLaunchKernelGGL((gemm_bf16<NP, SP, 64, 128>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, order); break;
+    case 1: hipLaunchKernelGGL((gemm_bf16<NP, SP, 64, 64>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, order); break;
+    case 2: hipLaunchKernelGGL((gemm_bf16<NP, SP, 128, 64>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, order); break;
+    default: hipLaunchKernelGGL((gemm_bf16<NP, SP, 128, 128>), dim3(n), dim3(256), 0, s, c.dinst, c.ditem, P, tiles, st, order); break;
     }
 }
 
-static void launch_gemm_mixed(ocrt_plan *p, const Chunk &c, hipStream_t s, const Pools &P, const ItemState *st)
+// order 0: the forward order loop (active items); order > 0: recomputation of
+// that order (the adjoint), every item whose forward solve reached it
+static void launch_gemm_mixed(ocrt_plan *p, const Chunk &c, hipStream_t s, const Pools &P, const ItemState *st,
+                              int order = 0)
 {
     for (int k = 0; k < 3; ++k) {
         const int sh = p->gemm_shape[k];
@@ -1074,45 +1097,55 @@ static void launch_gemm_mixed(ocrt_plan *p, const Chunk &c, hipStream_t s, const
         const GemmTile *tiles = c.dtiles_s[sh] + c.cls_off[sh][k];
         const EvPair e = p->tune_classes ? ev_pair(p, 2 + k) : EvPair{{nullptr, nullptr}, false};
         if (e.ok) (void)hipEventRecord(e.e[0], s);
-        if (k == 0) launch_gemm_class<1, false>(sh, n, tiles, c, s, P, st);
-        else if (k == 1) launch_gemm_class<1, true>(sh, n, tiles, c, s, P, st);
-        else launch_gemm_class<2, true>(sh, n, tiles, c, s, P, st);
+        if (k == 0) launch_gemm_class<1, false>(sh, n, tiles, c, s, P, st, order);
+        else if (k == 1) launch_gemm_class<1, true>(sh, n, tiles, c, s, P, st, order);
+        else launch_gemm_class<2, true>(sh, n, tiles, c, s, P, st, order);
         if (e.ok) (void)hipEventRecord(e.e[1], s);
     }
 }
 
-// Operator product of an adjoint plan's split-bf16 form over the chunk's tile
-// list: the single-part tiles (the first two classes), then the two-part ones.
-static void launch_x3(int mode, const Chunk &c, hipStream_t s, const Pools &P, const ItemState *st, int order)
+// Point Q's field at a checkpoint or segment buffer of an adjoint plan: fp64,
+// or (mixed plans) the bf16 pair as two planes of mx_F entries, hi then lo.
+static void set_field(const ocrt_plan *p, Pools &Q, void *buf)
 {
-    const unsigned nt = (unsigned)c.htiles.size(), n1 = (unsigned)c.ntiles2;
+    Q.F = buf;
+    if (p->adj_mixed) Q.Flo = reinterpret_cast<__hip_bfloat16 *>(buf) + p->mx_F;
+}
+
+// Operator products of a mixed adjoint plan over the chunk's 64 x 128 tile
+// list (its items all keep the field pair: classes 1 and 2):
+// MODE kGemmParts on the field in P.F / P.Flo, kGemmMix (A^T) on the fp32 field in P.F.
+static void launch_adj_gemm(int mode, const Chunk &c, hipStream_t s, const Pools &P, const ItemState *st, int order)
+{
+    const GemmTile *t = c.dtiles_s[0];
+    const unsigned n1 = (unsigned)(c.cls_off[0][2] - c.cls_off[0][1]), n2 = (unsigned)(c.cls_off[0][3] - c.cls_off[0][2]);
     if (n1 > 0) {
-        if (mode == kGemmMix)
-            hipLaunchKernelGGL((gemm_x3<1, kGemmMix>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st,
-                               order);
+        if (mode == kGemmParts)
+            hipLaunchKernelGGL((gemm_adj<1, kGemmParts, 0>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P,
+                               t + c.cls_off[0][1], st, order);
         else
-            hipLaunchKernelGGL((gemm_x3<1, kGemmParts>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P, c.dtiles, st,
-                               order);
+            hipLaunchKernelGGL((gemm_adj<1, kGemmMix, 2>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P,
+                               t + c.cls_off[0][1], st, order);
     }
-    if (nt > n1) {
-        if (mode == kGemmMix)
-            hipLaunchKernelGGL((gemm_x3<2, kGemmMix>), dim3(nt - n1), dim3(256), 0, s, c.dinst, c.ditem, P,
-                               c.dtiles + n1, st, order);
+    if (n2 > 0) {
+        if (mode == kGemmParts)
+            hipLaunchKernelGGL((gemm_adj<2, kGemmParts, 0>), dim3(n2), dim3(256), 0, s, c.dinst, c.ditem, P,
+                               t + c.cls_off[0][2], st, order);
         else
-            hipLaunchKernelGGL((gemm_x3<2, kGemmParts>), dim3(nt - n1), dim3(256), 0, s, c.dinst, c.ditem, P,
-                               c.dtiles + n1, st, order);
+            hipLaunchKernelGGL((gemm_adj<2, kGemmMix, 2>), dim3(n2), dim3(256), 0, s, c.dinst, c.ditem, P,
+                               t + c.cls_off[0][2], st, order);
     }
 }
 
 static void launch_sweep_mixed(int v, int ncols, hipStream_t s, const DevInst *di, const DevItem *dt, Pools P,
-                               ItemState *st, const ColTile *tiles)
+                               ItemState *st, const ColTile *tiles, int order = 0)
 {
     const int W = kSweepShapes[v].W;
     const dim3 grid((unsigned)((ncols + W - 1) / W)), block((unsigned)(64 * W));
 #define OCRT_SWEEP_CASE(I, U_, W_)                                                                              \
     case I:                                                                                                      \
         hipLaunchKernelGGL((sweep<__hip_bfloat16, _Float16, true, U_, W_>), grid, block, 0, s, di, dt, P, st, tiles, \
-                           ncols, 0);                                                                            \
+                           ncols, order);                                                                        \
         break;
     switch (v) {
         OCRT_SWEEP_CASE(0, 8, 4)
@@ -1148,7 +1181,12 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
         auto pairs = [&](int total, auto launch) {
             for (int base = 0; base < total; base += 1 << 22) launch(dim3((unsigned)std::min(total - base, 1 << 22)), base);
         };
-        if (mixed) {
+        if (p->adjoint) {
+            pairs(c.hcm_first.back(), [&](dim3 grid, int base) {
+                hipLaunchKernelGGL((coxmunk_fourier<true, double, 0>), grid, blk, 0, s, c.dinst, P, c.dcm_owner,
+                                   c.dcm_first, no, base);
+            });
+        } else if (mixed) {
             pairs(c.hcm_sfirst.back(), [&](dim3 grid, int base) {
                 hipLaunchKernelGGL((coxmunk_fourier<false, double, 1>), grid, blk, 0, s, c.dinst, P, c.dcm_owner,
                                    c.dcm_sfirst, no, base);
@@ -1156,11 +1194,6 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             pairs(c.hcm_dfirst.back(), [&](dim3 grid, int base) {
                 hipLaunchKernelGGL((coxmunk_fourier<false, float, 2>), grid, blk, 0, s, c.dinst, P, c.dcm_owner,
                                    c.dcm_dfirst, no, base);
-            });
-        } else if (p->adjoint) {
-            pairs(c.hcm_first.back(), [&](dim3 grid, int base) {
-                hipLaunchKernelGGL((coxmunk_fourier<true, double, 0>), grid, blk, 0, s, c.dinst, P, c.dcm_owner,
-                                   c.dcm_first, no, base);
             });
         } else {
             pairs(c.hcm_first.back(), [&](dim3 grid, int base) {
@@ -1185,7 +1218,7 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     const int pack_blocks = (int)std::min<long long>(16, (c.max_pack + 255) / 256);
     const dim3 pack_grid(xcd_item_grid(nit, pack_blocks));
     if (mixed) hipLaunchKernelGGL(lowrank_factors, dim3(nit), dim3(256), 0, s, c.dinst, c.ditem, P);
-    if (mixed || p->adj_mixed)
+    if (mixed)
         hipLaunchKernelGGL((pack_operator<__hip_bfloat16, false>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem, P,
                            nit, pack_blocks);
     else
@@ -1203,51 +1236,55 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     }
     // adjoint plans keep u_1, u_{1+e}, u_{1+2e}, ... of every item.  With
     // e = 1 the sweeps write straight into the checkpoint of their order and
-    // the next order reads it there (no copies).
+    // the next order reads it there (no copies).  Mixed adjoint plans keep the
+    // bf16 pair (both planes).
     const int every = p->ckpt_every;
     const bool direct = p->adjoint && every == 1;
-    auto field_of = [&](int order) -> void * { return direct ? p->ckpt[order - 1] : P.F; };
+    auto fields_of = [&](int order) -> Pools {
+        Pools Q = P;
+        if (direct) set_field(p, Q, p->ckpt[order - 1]);
+        return Q;
+    };
     auto checkpoint = [&](int order) -> int {
         if (!p->adjoint || direct || (order - 1) % every != 0) return 0;
-        return hipMemcpyAsync(p->ckpt[(order - 1) / every], P.F, (size_t)c.F * 8, hipMemcpyDeviceToDevice, s) ==
-                       hipSuccess ? 0 : OCRT_ERR_DEVICE;
+        void *dst = p->ckpt[(order - 1) / every];
+        if (!p->adj_mixed)
+            return hipMemcpyAsync(dst, P.F, (size_t)c.F * 8, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0
+                                                                                                     : OCRT_ERR_DEVICE;
+        Pools Q = P;
+        set_field(p, Q, dst);
+        return (hipMemcpyAsync(Q.F, P.F, (size_t)c.F * 2, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+                hipMemcpyAsync(Q.Flo, P.Flo, (size_t)c.F * 2, hipMemcpyDeviceToDevice, s) == hipSuccess)
+                   ? 0 : OCRT_ERR_DEVICE;
     };
     // per-column kernels: one 64-column tile per wavefront, four per workgroup
     const int ncols = (int)c.hcols.size();
     const dim3 cols((unsigned)((ncols + 3) / 4));
-    if (mixed) {
-        hipLaunchKernelGGL((order_one<__hip_bfloat16, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, P, st,
+    if (mixed)
+        hipLaunchKernelGGL((order_one<__hip_bfloat16, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, fields_of(1),
+                           st, c.dcols, ncols);
+    else
+        hipLaunchKernelGGL((order_one<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, fields_of(1), st,
                            c.dcols, ncols);
-    } else {
-        Pools P1 = P;
-        P1.F = field_of(1);
-        hipLaunchKernelGGL((order_one<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, P1, st, c.dcols,
-                           ncols);
-    }
     if (checkpoint(1)) return OCRT_ERR_DEVICE;
     const unsigned ntiles = (unsigned)c.htiles.size();
     for (int it = 2; it <= c.maxit; ++it) {
         const EvPair eg = p->profile ? ev_pair(p, 0) : EvPair{{nullptr, nullptr}, false};
         const EvPair es = p->profile ? ev_pair(p, 1) : EvPair{{nullptr, nullptr}, false};
+        const Pools Pg = fields_of(it - 1), Ps = fields_of(it);
         if (mixed) {
             hipLaunchKernelGGL(order_status<__hip_bfloat16>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
             if (eg.ok) (void)hipEventRecord(eg.e[0], s);
-            launch_gemm_mixed(p, c, s, P, st);
+            launch_gemm_mixed(p, c, s, Pg, st);
             if (eg.ok) (void)hipEventRecord(eg.e[1], s);
             if (es.ok) (void)hipEventRecord(es.e[0], s);
-            launch_sweep_mixed(p->sweep_variant, ncols, s, c.dinst, c.ditem, P, st, c.dcols);
+            launch_sweep_mixed(p->sweep_variant, ncols, s, c.dinst, c.ditem, Ps, st, c.dcols);
             if (es.ok) (void)hipEventRecord(es.e[1], s);
         } else {
             hipLaunchKernelGGL(order_status<double>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, it);
-            Pools Pg = P, Ps = P;
-            Pg.F = field_of(it - 1);
-            Ps.F = field_of(it);
             if (eg.ok) (void)hipEventRecord(eg.e[0], s);
-            if (p->adj_mixed)
-                launch_x3(kGemmMix, c, s, Pg, st, 0);
-            else
-                hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pg, c.dtiles,
-                                   st, 0);
+            hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pg, c.dtiles,
+                               st, 0);
             if (eg.ok) (void)hipEventRecord(eg.e[1], s);
             if (es.ok) (void)hipEventRecord(es.e[0], s);
             hipLaunchKernelGGL((sweep<double, double, false, 8, 4>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st,
@@ -1273,13 +1310,20 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : OCRT_ERR_DEVICE;
 }
 
-// Backward pass of one chunk after its forward pass (fp64).  With checkpoint
+// Backward pass of one chunk after its forward pass.  With checkpoint
 // interval e, segment s holds u_{es+1} (the checkpoint) and the recomputed
 // u_{es+2..es+e}; the backward step of order o needs u_o and u_{o-1}, so
 // segments are rebuilt from the top down and order e(s+1)+1 (whose u_o is the
 // next checkpoint) is taken with segment s.  With e = 1 nothing is recomputed.
-static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
+// fp64 plans: fp64 fields, adjoint and operator products (gemm_f64).  Mixed
+// plans: the fields as their bf16 pairs, rebuilt by the mixed order loop's own
+// kernels (gemm_bf16, the fp32 sweep); lambda, the adjoint source and the
+// operator parts in fp32, the parts and A^T products on bf16 MFMA (gemm_adj),
+// the adjoint sweeps in fp64 arithmetic; gradients accumulate in fp64.
+template <typename FV, typename LT>
+static void backward_steps(ocrt_plan *p, Chunk &c, hipStream_t s)
 {
+    constexpr bool MX = sizeof(LT) == 4;
     const Pools P0 = p->pools;
     const int ni = c.i1 - c.i0, nit = c.it1 - c.it0;
     ItemState *st = p->state + c.it0;
@@ -1288,60 +1332,76 @@ static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     const unsigned ntiles = (unsigned)c.htiles.size();
     const int maxit = c.maxit;
     hipLaunchKernelGGL(adj_zero_grad, dim3(4, ni), dim3(256), 0, s, c.dinst, P0);
-    hipLaunchKernelGGL(fill_f64, dim3((unsigned)std::min<long long>((c.F + 255) / 256, 65536)), dim3(256), 0, s,
-                       P0.lam, c.F, 0.0);
-    hipLaunchKernelGGL(adj_seed, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P0);
+    hipLaunchKernelGGL(fill<LT>, dim3((unsigned)std::min<long long>((c.F + 255) / 256, 65536)), dim3(256), 0, s,
+                       reinterpret_cast<LT *>(P0.lam), c.F, (LT)0);
+    hipLaunchKernelGGL(adj_seed<LT>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P0);
     auto slot = [&](int seg, int j) -> void * { return j == 0 ? p->ckpt[seg] : p->seg[j - 1]; };
+    auto at = [&](void *buf) {
+        Pools Q = P0;
+        set_field(p, Q, buf);
+        return Q;
+    };
+    // mixed plans: the operator parts (and the adjoint source written over the
+    // first) in the fp32 source pool
+    Pools Pw = P0;
+    if (MX) Pw.J = P0.Jf;
     const int every = p->ckpt_every;
     for (int sg = (maxit - 1) / every; sg >= 0; --sg) {
         const int base = every * sg + 1;   // order of the checkpoint
         // rebuild u_{base+1..base+every-1}
         for (int j = 1; j < every && base + j <= maxit; ++j) {
             const int o = base + j;
-            Pools Pin = P0;
-            Pin.F = slot(sg, j - 1);
-            hipLaunchKernelGGL(adj_bottom, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pin, st, o);
+            const Pools Pin = at(slot(sg, j - 1));
+            hipLaunchKernelGGL(adj_bottom<FV>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pin, st, o);
             hipLaunchKernelGGL(adj_surface_bc, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pin, st, o);
-            if (p->adj_mixed)
-                launch_x3(kGemmMix, c, s, Pin, st, o);
-            else
+            const Pools Pout = at(slot(sg, j));
+            if constexpr (MX) {
+                launch_gemm_mixed(p, c, s, Pin, st, o);
+                launch_sweep_mixed(p->sweep_variant, ncols, s, c.dinst, c.ditem, Pout, st, c.dcols, o);
+            } else {
                 hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pin, c.dtiles,
                                    st, o);
-            Pools Pout = P0;
-            Pout.F = slot(sg, j);
-            hipLaunchKernelGGL((sweep<double, double, false, 8, 4>), cols, dim3(256), 0, s, c.dinst, c.ditem, Pout, st,
-                               c.dcols, ncols, o);
+                hipLaunchKernelGGL((sweep<double, double, false, 8, 4>), cols, dim3(256), 0, s, c.dinst, c.ditem,
+                                   Pout, st, c.dcols, ncols, o);
+            }
         }
         // backward steps
         for (int o = std::min(base + every, maxit); o >= std::max(base + 1, 2); --o) {
             void *u_o = (o == base + every) ? p->ckpt[sg + 1] : slot(sg, o - base);
             void *u_prev = slot(sg, o - 1 - base);
-            Pools Pg = P0;
-            Pg.F = u_prev;
-            if (p->adj_mixed)
-                launch_x3(kGemmParts, c, s, Pg, st, o);
+            Pools Pg = at(u_prev);
+            Pg.J = Pw.J;
+            if constexpr (MX)
+                launch_adj_gemm(kGemmParts, c, s, Pg, st, o);
             else
                 hipLaunchKernelGGL(gemm_f64<kGemmParts>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pg,
                                    c.dtiles, st, o);
-            Pools Ps = P0;
-            Ps.F = u_o;
-            hipLaunchKernelGGL(sweep_adj, cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st, c.dcols, ncols, o);
+            Pools Ps = at(u_o);
+            Ps.J = Pw.J;
+            hipLaunchKernelGGL((sweep_adj<FV, LT>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st, c.dcols, ncols, o);
             Pools Pt = P0;
             Pt.A = p->At;
-            Pt.F = P0.J;
+            Pt.F = Pw.J;
             Pt.J = P0.lam;
-            if (p->adj_mixed)
-                launch_x3(kGemmMix, c, s, Pt, st, o);
+            if constexpr (MX)
+                launch_adj_gemm(kGemmMix, c, s, Pt, st, o);
             else
                 hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pt, c.dtiles,
                                    st, o);
-            hipLaunchKernelGGL(surface_adj, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pg, st, o);
+            hipLaunchKernelGGL((surface_adj<FV, LT>), dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pg, st, o);
         }
     }
-    Pools P1 = P0;
-    P1.F = p->ckpt[0];
-    hipLaunchKernelGGL(order_one_adj, cols, dim3(256), 0, s, c.dinst, c.ditem, P1, c.dcols, ncols);
+    hipLaunchKernelGGL((order_one_adj<FV, LT>), cols, dim3(256), 0, s, c.dinst, c.ditem, at(p->ckpt[0]), c.dcols,
+                       ncols);
     hipLaunchKernelGGL(glint_adj, dim3((ni + 63) / 64), dim3(64), 0, s, c.dinst, ni, P0);
+}
+
+static int backward_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
+{
+    if (p->adj_mixed)
+        backward_steps<FieldPair, float>(p, c, s);
+    else
+        backward_steps<FieldF64, double>(p, c, s);
     return hipGetLastError() == hipSuccess ? 0 : OCRT_ERR_DEVICE;
 }
 
@@ -1890,7 +1950,7 @@ extern "C" int ocrt_plan_results(ocrt_plan *p, ocrt_result_t *out, double *toa_m
             if (orders) orders[(long long)i * M + m] = r.orders;
             const int ord = std::max(r.orders - 1, 0);
             const double L = I.nt + 1.0;
-            const bool lowrank = mixed && m <= 2 && I.ray && I.aer;
+            const bool lowrank = mixed && !p->adjoint && m <= 2 && I.ray && I.aer;
             if (mixed) {
                 // the tiles of the item's class shape; per 16 x 16 x 32 block
                 // and part: 2 MFMAs (A_hi, A_lo), 3 with the field pair, 1

@@ -381,7 +381,8 @@ template <int NP, bool SPLIT, int TM = 64, int TN = kGemmLevels> struct GemmLds 
 // in the same order.  NP is a template parameter so that single-part items
 // hold half the accumulators (higher occupancy); SPLIT = false compiles out
 // the bf16 pair field (its registers and LDS plane) for tiles whose items keep
-// a single bf16 field.  F64IN reads an fp64 field (F64) instead and splits
+// a single bf16 field.  FIN selects the field's storage: 0 the bf16 planes F
+// (and Flo), 1 an fp64 field FW, 2 an fp32 field FW; the wide forms split
 // every value into its bf16 pair on the way into LDS.
 // Rows past the operator's Mp rows load its last row (their results are never
 // used).  EPI = 0: the fp32 results, scaled by the level fractions, go to
@@ -389,16 +390,16 @@ template <int NP, bool SPLIT, int TM = 64, int TN = kGemmLevels> struct GemmLds 
 // unscaled to out(lev, r, v0, v1) (v1 zero for one part); levels < L only.
 // LO = false is the reduced product: the operator's hi halves times the bf16
 // field alone (no A_lo, no field residual), one MFMA per block.
-template <int NP, bool SPLIT, bool F64IN = false, int EPI = 0, int TM = 64, int TN = kGemmLevels, bool LO = true,
+template <int NP, bool SPLIT, int FIN = 0, int EPI = 0, int TM = 64, int TN = kGemmLevels, bool LO = true,
           typename Out>
 __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, const unsigned short *A,
-                                      const unsigned short *F, const unsigned short *Flo, const double *F64,
+                                      const unsigned short *F, const unsigned short *Flo, const void *FW,
                                       float *Wp, int row0, int lev0, GemmLds<NP, SPLIT, TM, TN> &S, Out out)
 {
     constexpr int AB = TM / 32, BB = TN / 32;   // 16 x 16 blocks per wave along rows and levels
     constexpr int AR = TM / 64, FR = TN / 64;   // operator rows and field levels staged per thread
     const int Pp = it.P, L = it.L, Mp = it.Mp;
-    const bool split = LO && SPLIT && (F64IN || it.fsplit != 0);
+    const bool split = LO && SPLIT && (FIN != 0 || it.fsplit != 0);
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
     float4_ acc[NP][AB][BB];
@@ -423,16 +424,17 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
     auto load_field = [&](int l, int k0, uint4 &hi, uint4 &lo) {
         hi = lo = make_uint4(0u, 0u, 0u, 0u);
         if (l >= L) return;
-        if constexpr (F64IN) {
-            // eight fp64 values -> their bf16 pairs (hi of the fp32 rounding,
-            // lo of the remainder), as every bf16 pair of the solver
-            const double *src = F64 + (long long)l * Pp + k0 + lk;
+        if constexpr (FIN != 0) {
+            // eight fp64 or fp32 values -> their bf16 pairs (hi of the fp32
+            // rounding, lo of the remainder), as every bf16 pair of the solver
+            typedef typename std::conditional<FIN == 1, double, float>::type WT;
+            const WT *src = reinterpret_cast<const WT *>(FW) + (long long)l * Pp + k0 + lk;
             unsigned h[8], w[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const double v = src[e];
+                const WT v = src[e];
                 const __hip_bfloat16 bh = __float2bfloat16((float)v);
-                const __hip_bfloat16 bl = __float2bfloat16((float)(v - (double)__bfloat162float(bh)));
+                const __hip_bfloat16 bl = __float2bfloat16((float)(v - (WT)__bfloat162float(bh)));
                 h[e] = __builtin_bit_cast(unsigned short, bh);
                 w[e] = __builtin_bit_cast(unsigned short, bl);
             }
@@ -617,55 +619,54 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? (TM * TN > 8192 ? 2 : 5)
         }
     };
     if (reduced)
-        gemm_tile_bf16<NP, SPLIT, false, 0, TM, TN, false>(it, lv, A, F, Flo, nullptr, Wp, tl.tm * TM, tl.tn * TN, S,
+        gemm_tile_bf16<NP, SPLIT, 0, 0, TM, TN, false>(it, lv, A, F, Flo, nullptr, Wp, tl.tm * TM, tl.tn * TN, S,
                                                            out);
     else
-        gemm_tile_bf16<NP, SPLIT, false, 0, TM, TN, true>(it, lv, A, F, Flo, nullptr, Wp, tl.tm * TM, tl.tn * TN, S,
+        gemm_tile_bf16<NP, SPLIT, 0, 0, TM, TN, true>(it, lv, A, F, Flo, nullptr, Wp, tl.tm * TM, tl.tn * TN, S,
                                                           out);
 }
 
-// Scattering integral of the adjoint plans' split-bf16 form: the fp64 field
-// (the forward u or the adjoint lambda) split into bf16 pairs on the way into
-// LDS, the bf16-pair operator (A, or A^T for the adjoint), three MFMAs per
-// block (A_hi F_hi + A_lo F_hi + A_hi F_lo), fp64 results:
-//   MODE kGemmMix   J[lev][r] = s0(lev) (A_0 F)[r][lev] + ydel(lev) (A_1 F)[r][lev]
-//   MODE kGemmParts J[lev][r] = (A_0 F)[r][lev], J2[lev][r] = (A_1 F)[r][lev]
-// It replaces gemm_f64 in plans that trade the fp64 operator product for
-// about 16 significant bits per factor.
-template <int NP, int MODE>
-__global__ __launch_bounds__(256, 1) void gemm_x3(const DevInst *__restrict__ inst, const DevItem *__restrict__ items,
-                                                  Pools P, const GemmTile *__restrict__ tiles,
-                                                  const ItemState *__restrict__ st, int order)
+// Operator products of the mixed adjoint plans on bf16 MFMA, three MFMAs per
+// block (A_hi F_hi + A_lo F_hi + A_hi F_lo), fp32 results:
+//   MODE kGemmParts, FIN 0: the forward field u_{o-1}, kept as its bf16 pair
+//     (P.F, P.Flo), times the operator A, the parts unscaled:
+//     J[lev][r] = (A_0 u)[r][lev], J2[lev][r] = (A_1 u)[r][lev]
+//   MODE kGemmMix, FIN 2: the fp32 adjoint source Sbar (P.F) split into bf16
+//     pairs on the way into LDS, times the transposed operator (P.A):
+//     J[lev][r] = s0(lev) (A_0^T Sbar)[r][lev] + ydel(lev) (A_1^T Sbar)[r][lev]
+// with s0 = xdel for a particle first part, ydel for a molecular one.
+template <int NP, int MODE, int FIN>
+__global__ __launch_bounds__(256, 1) void gemm_adj(const DevInst *__restrict__ inst, const DevItem *__restrict__ items,
+                                                   Pools P, const GemmTile *__restrict__ tiles,
+                                                   const ItemState *__restrict__ st, int order)
 {
+    static_assert((MODE == kGemmParts && FIN == 0) || (MODE == kGemmMix && FIN == 2), "adjoint product forms");
     const GemmTile tl = tiles[blockIdx.x];
     if (tl.item < 0 || !item_runs(st[tl.item], order)) return;
     const DevItem &it = items[tl.item];
     const DevInst &I = inst[it.inst];
     const int Pp = it.P;
     const unsigned short *A = reinterpret_cast<const unsigned short *>(P.A) + it.off_A;
-    const double *F = reinterpret_cast<const double *>(P.F) + it.off_F;
-    double *J = reinterpret_cast<double *>(P.J) + it.off_F;
-    double *J2 = MODE == kGemmParts ? reinterpret_cast<double *>(P.J2) + it.off_F : nullptr;
+    const unsigned short *F = FIN == 0 ? reinterpret_cast<const unsigned short *>(P.F) + it.off_F : nullptr;
+    const unsigned short *Flo = FIN == 0 ? reinterpret_cast<const unsigned short *>(P.Flo) + it.off_Flo : nullptr;
+    const float *FW = FIN == 2 ? reinterpret_cast<const float *>(P.F) + it.off_F : nullptr;
+    float *J = reinterpret_cast<float *>(P.J) + it.off_F;
+    float *J2 = MODE == kGemmParts ? reinterpret_cast<float *>(P.J2) + it.off_F : nullptr;
     const LevelView lv(P, I);
     __shared__ GemmLds<NP, true> S;
     if constexpr (MODE == kGemmMix) {
-        gemm_tile_bf16<NP, true, true, 0>(it, lv, A, nullptr, nullptr, F, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
-                                          [&](int lev, int r, float4 v) {
-                                              if (r >= Pp) return;
-                                              double *d = J + (long long)lev * Pp + r;
-                                              d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
-                                          });
+        gemm_tile_bf16<NP, true, FIN, 0>(it, lv, A, F, Flo, FW, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
+                                         [&](int lev, int r, float4 v) {
+                                             if (r >= Pp) return;
+                                             *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v;
+                                         });
     } else {
-        gemm_tile_bf16<NP, true, true, 1>(it, lv, A, nullptr, nullptr, F, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
-                                          [&](int lev, int r, float4 v0, float4 v1) {
-                                              if (r >= Pp) return;
-                                              double *d = J + (long long)lev * Pp + r;
-                                              d[0] = v0.x; d[1] = v0.y; d[2] = v0.z; d[3] = v0.w;
-                                              if (NP > 1) {
-                                                  double *d2 = J2 + (long long)lev * Pp + r;
-                                                  d2[0] = v1.x; d2[1] = v1.y; d2[2] = v1.z; d2[3] = v1.w;
-                                              }
-                                          });
+        gemm_tile_bf16<NP, true, FIN, 1>(it, lv, A, F, Flo, FW, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
+                                         [&](int lev, int r, float4 v0, float4 v1) {
+                                             if (r >= Pp) return;
+                                             *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v0;
+                                             if (NP > 1) *reinterpret_cast<float4 *>(J2 + (long long)lev * Pp + r) = v1;
+                                         });
     }
 }
 
@@ -714,9 +715,10 @@ struct MixedColumn {
     float *cmp_top, *cmp_bot;
     long long Pp;
     int nt, down;
+    bool acc;                     // false: recomputation, the field alone (no order sum)
     float mcur, mtot;
 
-    __device__ MixedColumn(const Pools &P, const DevItem &it, int col, int c, int o, int down_)
+    __device__ MixedColumn(const Pools &P, const DevItem &it, int col, int c, int o, int down_, bool acc_ = true)
     {
         const int n = it.n;
         F = reinterpret_cast<__hip_bfloat16 *>(P.F) + it.off_F + col;
@@ -730,6 +732,7 @@ struct MixedColumn {
         Pp = it.P;
         nt = it.nt;
         down = down_;
+        acc = acc_;
         mcur = mtot = 0.f;
     }
 
@@ -741,6 +744,7 @@ struct MixedColumn {
         mcur = fmaxf(mcur, fabsf((float)v));
         if (down && k == nt) *bottom = v;
         if (top && k == 0) *top = v;
+        if (!acc) return;
         double *rs = k == 0 ? row_top : row_bot;
         const double t_old = *rs, tsum = t_old + v;
         *rs = tsum;
@@ -793,7 +797,7 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
         if constexpr (MIXED) {
             const float *OM = P.Tf + I.off_T;
             const float *IDT = P.idt + I.off_lev;
-            MixedColumn mc(P, it, col, c, o, hemi);
+            MixedColumn mc(P, it, col, c, o, hemi, accumulate);
             mc.top = top_up;
             const float smu = hemi ? -(float)mu : (float)mu;
             // the source: fp32 for the items that keep their field as a pair,
@@ -1134,14 +1138,14 @@ OCRT_GEMM_INST(1, false, 128, 128)
 OCRT_GEMM_INST(1, true, 128, 128)
 OCRT_GEMM_INST(2, true, 128, 128)
 #undef OCRT_GEMM_INST
-template __global__ void gemm_x3<1, kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
-                                              const ItemState *, int);
-template __global__ void gemm_x3<2, kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
-                                              const ItemState *, int);
-template __global__ void gemm_x3<1, kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
-                                                const ItemState *, int);
-template __global__ void gemm_x3<2, kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
-                                                const ItemState *, int);
+template __global__ void gemm_adj<1, kGemmParts, 0>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                                      const ItemState *, int);
+template __global__ void gemm_adj<2, kGemmParts, 0>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                                      const ItemState *, int);
+template __global__ void gemm_adj<1, kGemmMix, 2>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                                    const ItemState *, int);
+template __global__ void gemm_adj<2, kGemmMix, 2>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                                    const ItemState *, int);
 template __global__ void gemm_f64<kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                             const ItemState *, int);
 template __global__ void gemm_f64<kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,

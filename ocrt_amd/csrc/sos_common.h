@@ -20,6 +20,27 @@ struct LevelView {
     }
 };
 
+// Read access to the forward field of an adjoint plan, as fp64, at column
+// `col` of one item: the fp64 field (fp64 plans) or the bf16 pair hi + lo that
+// the mixed plans keep (Pools::F and Pools::Flo; their items all keep the pair,
+// at off_Flo == off_F).
+struct FieldF64 {
+    const double *p;
+    __device__ FieldF64(const Pools &P, const DevItem &it, int col)
+        : p(reinterpret_cast<const double *>(P.F) + it.off_F + col) {}
+    __device__ double operator[](long long ix) const { return p[ix]; }
+};
+struct FieldPair {
+    const __hip_bfloat16 *hi, *lo;
+    __device__ FieldPair(const Pools &P, const DevItem &it, int col)
+        : hi(reinterpret_cast<const __hip_bfloat16 *>(P.F) + it.off_F + col),
+          lo(reinterpret_cast<const __hip_bfloat16 *>(P.Flo) + it.off_Flo + col) {}
+    __device__ double operator[](long long ix) const
+    {
+        return (double)__bfloat162float(hi[ix]) + (double)__bfloat162float(lo[ix]);
+    }
+};
+
 // Whether an item takes part in a launch: the forward order loop follows the
 // convergence flag (order == 0); recomputation and the adjoint pass an order
 // and take every item whose forward solve reached it.

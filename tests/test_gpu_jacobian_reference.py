@@ -43,17 +43,18 @@ def test_gpu_adjoint_matches_reference_finite_differences(oracle):
           flush=True)
 
 
-def test_split_bf16_adjoint_matches_reference_finite_differences(oracle):
-    """The adjoint plan with split-bf16 operator products (precision "mixed":
-    fp64 fields and sweeps, gemm_x3 for A u, A^T lambda and the parts): the
-    same Jacobians within 1e-3 of the reference's central differences, and
-    every per-layer gradient within 1e-3 of the fp64 adjoint's scale."""
+def test_mixed_adjoint_matches_reference_finite_differences(oracle):
+    """The mixed adjoint plan (precision "mixed": the mixed order loop with
+    every forward field kept as its bf16 pair, fp32 adjoint fields, bf16 MFMA
+    for the operator parts and A^T): the same Jacobians within 1e-3 of the
+    reference's central differences, and every per-layer gradient within 1e-3
+    of the fp64 adjoint's scale."""
     import ocrt_amd as ocrt
     fx = load()
     g = np.asarray(fx["g"])
     built = [layered(oracle, c["desc"]) for c in fx["cases"]]
     descs = [b[0] for b in built]
-    print(f"[gpu] split-bf16 adjoint of {len(descs)} workload instances vs reference FD", flush=True)
+    print(f"[gpu] mixed adjoint of {len(descs)} workload instances vs reference FD", flush=True)
     out = {}
     for prec in ("fp64", "mixed"):
         plan = ocrt.Plan(descs, precision=prec, adjoint=True, ckpt_sparse=True)
@@ -75,5 +76,5 @@ def test_split_bf16_adjoint_matches_reference_finite_differences(oracle):
         el = max(np.max(np.abs(gl[r] - gl64[r])) / np.max(np.abs(gl64[r])) for r in (0, 1, 2))
         worst = [max(worst[0], rel), max(worst[1], ew), max(worst[2], ea), max(worst[3], el)]
         assert rel < 1e-4 and ew < 1e-3 and ea < 1e-3 and el < 1e-3, (i, rel, ew, ea, el)
-    print(f"[gpu] split-bf16 adjoint: rho_I rel {worst[0]:.1e}; vs reference FD wind {worst[1]:.1e}, "
+    print(f"[gpu] mixed adjoint: rho_I rel {worst[0]:.1e}; vs reference FD wind {worst[1]:.1e}, "
           f"AOD {worst[2]:.1e}; per-layer vs fp64 adjoint {worst[3]:.1e}", flush=True)
