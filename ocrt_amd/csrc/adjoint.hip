@@ -129,19 +129,25 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
     const int n = it.n, nt = it.nt, Pp = it.P, L = nt + 1;
     const int nparts = it.K / Pp, kind0 = it.parts & 0xff;
     const long long off_F = it.off_F, off_bc = it.off_bc;
+    if (ct.col0 >= 6 * n) return;   // a tile of pad columns
     const int col = ct.col0 + lane;
     const bool live = col < ct.col_end && col < 6 * n;
+    // the dead lanes of a partial tile load a live column of the item (their
+    // values reach no output), so every load is unconditional and a block's
+    // loads are all in flight before its first step
+    const int colr = live ? col : ct.col0;
     // tiles never mix directions: the whole wavefront walks one way
     const int hemi = __builtin_amdgcn_readfirstlane(ct.col0 >= 3 * n ? 1 : 0);
     int c = 0, hh = 0, o = 0;
-    if (live) col_decode(col, n, c, hh, o);
+    col_decode(colr, n, c, hh, o);
     const LevelView lv(P, I);
-    const FV U(P, it, col);
-    LT *__restrict__ J0 = reinterpret_cast<LT *>(P.J) + off_F + col;
-    const LT *__restrict__ J1 = reinterpret_cast<const LT *>(P.J2) + off_F + col;
-    const LT *__restrict__ lam = reinterpret_cast<const LT *>(P.lam) + off_F + col;
-    double *gX = P.grad + I.off_grad + L, *gY = gX + L, *gD = gY + L;
+    const FV U(P, it, colr);
     const bool two = nparts > 1;
+    LT *__restrict__ J0 = reinterpret_cast<LT *>(P.J) + off_F + colr;
+    // single-part items have no second part: read (and drop) the first
+    const LT *__restrict__ J1 = two ? reinterpret_cast<const LT *>(P.J2) + off_F + colr : J0;
+    const LT *__restrict__ lam = reinterpret_cast<const LT *>(P.lam) + off_F + colr;
+    double *gX = P.grad + I.off_grad + L, *gY = gX + L, *gD = gY + L;
     auto source = [&](int k, double p0, double p1) {
         const double s0 = (kind0 == 1) ? lv.xd[k] : lv.yd[k];
         return s0 * p0 + lv.yd[k] * p1;
@@ -159,9 +165,10 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
         }
     };
     const int k0 = hemi ? nt : 0;
-    double rho = live ? (double)lam[(long long)k0 * Pp] : 0.0;
-    double pk0 = live ? (double)J0[(long long)k0 * Pp] : 0.0;
-    double pk1 = (live && two) ? (double)J1[(long long)k0 * Pp] : 0.0;
+    double rho = (double)lam[(long long)k0 * Pp];
+    double pk0 = (double)J0[(long long)k0 * Pp];
+    const double j1k0 = (double)J1[(long long)k0 * Pp];
+    double pk1 = two ? j1k0 : 0.0;
     double sk = source(k0, pk0, pk1);
     double pending = 0.0;
     constexpr int kB = 8;
@@ -173,10 +180,11 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
             const int tc = t0 + u < nt ? t0 + u : nt - 1;
             const int kp = hemi ? nt - tc - 1 : tc + 1;
             const long long ix = (long long)kp * Pp;
-            q0[u] = live ? (double)J0[ix] : 0.0;
-            q1[u] = (live && two) ? (double)J1[ix] : 0.0;
-            uv[u] = live ? U[ix] : 0.0;
-            lv_[u] = live ? (double)lam[ix] : 0.0;
+            q0[u] = (double)J0[ix];
+            const double j1 = (double)J1[ix];
+            q1[u] = two ? j1 : 0.0;
+            uv[u] = U[ix];
+            lv_[u] = (double)lam[ix];
         }
         // per step: d/d xdel, d/d ydel at level k and d/d(layer step)
         double red[3 * kB];
@@ -313,17 +321,22 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
     const DevItem &it = items[item];
     const DevInst &I = inst[it.inst];
     const int n = it.n, nt = it.nt, Pp = it.P, L = nt + 1, m = it.m;
+    if (ct.col0 >= 6 * n) return;   // a tile of pad columns
     const int col = ct.col0 + lane;
     const bool live = col < ct.col_end && col < 6 * n;
+    // dead lanes load a live column (unconditional loads); lmask zeroes their
+    // adjoint, the only value of theirs that reaches the reductions unmasked
+    const int colr = live ? col : ct.col0;
+    const double lmask = live ? 1.0 : 0.0;
     // tiles never mix directions: the whole wavefront walks one way
     const int hemi = __builtin_amdgcn_readfirstlane(ct.col0 >= 3 * n ? 1 : 0);
     int c = 0, hh = 0, o = 0;
-    if (live) col_decode(col, n, c, hh, o);
+    col_decode(colr, n, c, hh, o);
     const LevelView lv(P, I);
-    const ColumnSource src(I, it, P, lv, live ? col : 0);
+    const ColumnSource src(I, it, P, lv, colr);
     const double mus = I.mus, amu = P.ring[I.off_ring + o];
-    const FV U(P, it, col);
-    const LT *__restrict__ lam = reinterpret_cast<const LT *>(P.lam) + it.off_F + col;
+    const FV U(P, it, colr);
+    const LT *__restrict__ lam = reinterpret_cast<const LT *>(P.lam) + it.off_F + colr;
     double *gH = P.grad + I.off_grad, *gX = gH + L, *gY = gX + L, *gD = gY + L, *gW = gD + nt;
     // rough direct-beam seed of the upward columns
     const bool rough = live && I.surf == 2 && hemi == 0;
@@ -361,7 +374,7 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
         }
     };
     const int k0 = hemi ? nt : 0;
-    double rho = live ? (double)lam[(long long)k0 * Pp] : 0.0;
+    double rho = (double)lam[(long long)k0 * Pp] * lmask;
     double lam_k = rho;
     double sk = S(k0);
     double pending = 0.0;
@@ -373,8 +386,8 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
         for (int u = 0; u < kB; ++u) {
             const int tc = t0 + u < nt ? t0 + u : nt - 1;
             const long long ix = (long long)(hemi ? nt - tc - 1 : tc + 1) * Pp;
-            uv[u] = live ? U[ix] : 0.0;
-            lv_[u] = live ? (double)lam[ix] : 0.0;
+            uv[u] = U[ix];
+            lv_[u] = (double)lam[ix] * lmask;
         }
         // per step: d/d xdel, d/d ydel, d/dh at level k and d/d(layer step)
         double red[4 * kB];

@@ -1181,7 +1181,17 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
         auto pairs = [&](int total, auto launch) {
             for (int base = 0; base < total; base += 1 << 22) launch(dim3((unsigned)std::min(total - base, 1 << 22)), base);
         };
-        if (p->adjoint) {
+        if (p->adj_mixed) {
+            // with the derivatives: the solar pairs in fp64, the diffuse ones in fp32
+            pairs(c.hcm_sfirst.back(), [&](dim3 grid, int base) {
+                hipLaunchKernelGGL((coxmunk_fourier<true, double, 1>), grid, blk, 0, s, c.dinst, P, c.dcm_owner,
+                                   c.dcm_sfirst, no, base);
+            });
+            pairs(c.hcm_dfirst.back(), [&](dim3 grid, int base) {
+                hipLaunchKernelGGL((coxmunk_fourier<true, float, 2>), grid, blk, 0, s, c.dinst, P, c.dcm_owner,
+                                   c.dcm_dfirst, no, base);
+            });
+        } else if (p->adjoint) {
             pairs(c.hcm_first.back(), [&](dim3 grid, int base) {
                 hipLaunchKernelGGL((coxmunk_fourier<true, double, 0>), grid, blk, 0, s, c.dinst, P, c.dcm_owner,
                                    c.dcm_first, no, base);

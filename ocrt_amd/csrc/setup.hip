@@ -305,13 +305,13 @@ __global__ void value_weights(double *w, int w_rows)
 // RT = float (mixed-form plans, diffuse pairs only): the Mueller matrices, the
 // Fourier products and the wavefront reductions in fp32 from the fp32 weight
 // table, the four wavefronts' totals added in fp64, and only the fp32 copy
-// Rf written (the mixed form reads no other copy of the diffuse kernels).
+// Rf written (the mixed form reads no other copy of the diffuse kernels); the
+// mixed adjoint plans keep no fp32 copy and take the fp64 arrays R and dR.
 template <bool DERIV, typename RT = double>
 __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
 {
     constexpr int kHalf = kNPhi / 2;
     static_assert(kHalf == 2 * kCoxMunkThreads, "two azimuths per thread");
-    static_assert(!DERIV || sizeof(RT) == 8, "derivatives in fp64 only");
     constexpr bool F32 = sizeof(RT) == 4;
     constexpr int kWaves = kCoxMunkThreads / 64;
     constexpr int kRowModes = DERIV ? 4 : 8;   // Fourier modes per batch
@@ -329,7 +329,7 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
             const int nout = (I.mmax + 1) * 9;
             for (int t = threadIdx.x; t < nout; t += blockDim.x) {
                 const int m = t / 9, kl = t % 9;
-                if (!F32) P.R[r_index(I, m, o, i, kl)] = 0.0;
+                if (!F32 || !P.Rf) P.R[r_index(I, m, o, i, kl)] = 0.0;
                 if (P.Rf) P.Rf[r_index(I, m, o, i, kl)] = 0.f;
                 if (DERIV) P.dR[r_index(I, m, o, i, kl)] = 0.0;
             }
@@ -375,7 +375,7 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
                 const RT a0 = r[0][kl] * (use_cos ? c0 : s0), a1 = r[1][kl] * (use_cos ? c1 : s1);
                 if (DERIV) {
                     v[(2 * j) * C + e] = a0 + a1;
-                    v[(2 * j + 1) * C + e] = a0 * dl[0] + a1 * dl[1];
+                    v[(2 * j + 1) * C + e] = a0 * (RT)dl[0] + a1 * (RT)dl[1];
                 } else {
                     v[j * C + e] = a0 + a1;
                 }
@@ -411,7 +411,7 @@ __device__ inline void coxmunk_pair(const DevInst &I, const Pools &P, int pair)
                 if (DERIV && (row & 1)) {
                     P.dR[at] = val;
                 } else {
-                    if (!F32) P.R[at] = val;
+                    if (!F32 || !P.Rf) P.R[at] = val;
                     if (P.Rf && !solar) P.Rf[at] = (float)val;
                 }
             }
@@ -475,6 +475,8 @@ template __global__ void coxmunk_fourier<false, double, 0>(const DevInst *, Pool
 template __global__ void coxmunk_fourier<true, double, 0>(const DevInst *, Pools, const int *, const int *, int, int);
 template __global__ void coxmunk_fourier<false, double, 1>(const DevInst *, Pools, const int *, const int *, int, int);
 template __global__ void coxmunk_fourier<false, float, 2>(const DevInst *, Pools, const int *, const int *, int, int);
+template __global__ void coxmunk_fourier<true, double, 1>(const DevInst *, Pools, const int *, const int *, int, int);
+template __global__ void coxmunk_fourier<true, float, 2>(const DevInst *, Pools, const int *, const int *, int, int);
 
 // ---------------------------------------------------------------------------
 // Signed direction cosine of direction index d = j + n, j in [-n, n]; j = 0 is
