@@ -109,7 +109,8 @@ __global__ void adj_step_table(const DevInst *__restrict__ inst, Pools P)
 // Adjoint sweep of order `order` >= 2.  P.F holds u_o (FV: its storage), P.J /
 // P.J2 the unscaled operator parts A_0 u_{o-1} / A_1 u_{o-1}, P.lam lambda_o
 // (both LT: double in fp64 plans, float in mixed ones; the arithmetic is fp64).  Writes
-// Sbar_o over P.J, bbar_o over the boundary slots of P.bc, and accumulates
+// Sbar_o (fp64 plans: over P.J; mixed plans: as its bf16 pair to P.sb_hi /
+// P.sb_lo, the form the A^T product reads), bbar_o over the boundary slots of P.bc, and accumulates
 // gX, gY (Sbar . part) per level and gD per layer.  One thread per column,
 // one 64-column tile (of one direction) per wavefront, walking each column
 // against its forward direction; the per-level sums of eight steps are
@@ -148,6 +149,17 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
     const LT *__restrict__ J1 = two ? reinterpret_cast<const LT *>(P.J2) + off_F + colr : J0;
     const LT *__restrict__ lam = reinterpret_cast<const LT *>(P.lam) + off_F + colr;
     double *gX = P.grad + I.off_grad + L, *gY = gX + L, *gD = gY + L;
+    constexpr bool MX = sizeof(LT) == 4;
+    auto put_sbar = [&](int k, double v) {
+        const long long ix = (long long)k * Pp;
+        if constexpr (MX) {
+            const __hip_bfloat16 h = __float2bfloat16((float)v);
+            P.sb_hi[off_F + colr + ix] = h;
+            P.sb_lo[off_F + colr + ix] = __float2bfloat16((float)(v - (double)__bfloat162float(h)));
+        } else {
+            J0[ix] = (LT)v;
+        }
+    };
     auto source = [&](int k, double p0, double p1) {
         const double s0 = (kind0 == 1) ? lv.xd[k] : lv.yd[k];
         return s0 * p0 + lv.yd[k] * p1;
@@ -203,7 +215,7 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
             const double sbar = pending + al * rho;
             pending = be * rho;
             if (live) {
-                J0[(long long)k * Pp] = (LT)sbar;
+                put_sbar(k, sbar);
                 red[3 * u + 0] = (kind0 == 1) ? sbar * pk0 : 0.0;
                 red[3 * u + 1] = (kind0 == 1) ? sbar * pk1 : sbar * pk0;
                 red[3 * u + 2] = rho * (dT * uv[u] + dal * sk + dbe * sp);
@@ -222,7 +234,7 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
     }
     // last level of the walk: nt for upward columns, 0 for downward ones
     const int kl = hemi ? 0 : nt;
-    if (live) J0[(long long)kl * Pp] = (LT)pending;
+    if (live) put_sbar(kl, pending);
     level_sums(kl, pending, pk0, pk1);
     // the upward columns start from the boundary condition: its adjoint is rho
     if (live && hemi == 0 && I.surf != 0) P.bc[off_bc + c * n + o] = rho;
@@ -349,8 +361,8 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
     auto S = [&](int k) { return live ? src.primary(lv, k) + src.flat_value(lv, mus, k) : 0.0; };
     double g_nt = 0.0, g_w = 0.0;   // per-lane d/dh[nt] and d/dsigma^2 terms
     // explicit level terms of the source and the seed at level k for adjoint
-    // sbar; lam_k is lambda_1 at level k
-    auto level_terms = [&](int k, double sbar, double lam_k, double &vx, double &vy, double &vh) {
+    // sbar; lam_k is lambda_1 at level k, f the seed's attenuation seed_f(k)
+    auto level_terms = [&](int k, double sbar, double lam_k, double f, double &vx, double &vy, double &vh) {
         vx = vy = vh = 0.0;
         if (!live) return;
         const double ch = lv.ch[k];
@@ -367,7 +379,7 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
         }
         if (rough) {
             const double gd = lam_k - (k == 0 ? e_here : 0.0);
-            const double f = seed_f(k), dv = rsun * f;
+            const double dv = rsun * f;
             vh += gd * dv / amu;
             g_nt += -gd * dv * (1.0 / mus + 1.0 / amu);
             g_w += gd * drsun * f;
@@ -378,6 +390,8 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
     double lam_k = rho;
     double sk = S(k0);
     double pending = 0.0;
+    // the seed's attenuation at the current level, carried to the next step
+    double fk = rough ? seed_f(k0) : 0.0;
     constexpr int kB = 8;
     for (int t0 = 0; t0 < nt; t0 += kB) {
         // gather the block's upstream levels first (clamped: no branches)
@@ -405,8 +419,10 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
             const double T = cf[0], al = cf[1], be = cf[2], dT = cf[3], dal = cf[4], dbe = cf[5];
             const double sbar = pending + al * rho;
             pending = be * rho;
-            const double rk = live ? uv[u] - (rough ? rsun * seed_f(kp) : 0.0) : 0.0;
-            level_terms(k, sbar, lam_k, red[4 * u + 0], red[4 * u + 1], red[4 * u + 2]);
+            const double fkp = rough ? seed_f(kp) : 0.0;
+            const double rk = live ? uv[u] - rsun * fkp : 0.0;
+            level_terms(k, sbar, lam_k, fk, red[4 * u + 0], red[4 * u + 1], red[4 * u + 2]);
+            fk = fkp;
             red[4 * u + 3] = rho * (dT * rk + dal * sk + dbe * sp);
             rho = lv_[u] + T * rho;
             lam_k = lv_[u];
@@ -425,7 +441,7 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
     // last level of the walk: nt for upward columns, 0 for downward ones
     const int kl = hemi ? 0 : nt;
     double vx, vy, vh;
-    level_terms(kl, pending, lam_k, vx, vy, vh);
+    level_terms(kl, pending, lam_k, fk, vx, vy, vh);
     vx = wave_sum(vx);
     vy = wave_sum(vy);
     vh = wave_sum(vh);

@@ -16,6 +16,7 @@
 //     [I_up(n) Q_up(n) U_up(n) I_dn(n) Q_dn(n) U_dn(n)], node o = 0..n-1
 //   padded to the row pitch P; pad columns stay zero.
 #pragma once
+#include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -163,6 +164,9 @@ struct Pools {
     // lo_tau takes its next operator product in the reduced form: the bf16
     // operator and the bf16 field, one MFMA per block, no low halves (0: never)
     float lo_tau;
+    // mixed adjoint: the adjoint source Sbar of the current order as a bf16
+    // pair (two planes, [L][P] per item at off_F), the A^T product's input
+    __hip_bfloat16 *sb_hi, *sb_lo;
 };
 
 // Workgroups of a launch are dealt round-robin over the 8 XCDs (block b runs

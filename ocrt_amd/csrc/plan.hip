@@ -298,7 +298,8 @@ struct ocrt_plan {
     // u_{1+2e}, ... for interval e), the e - 1 recomputed fields of a segment,
     // per-instance accumulators
     int adjoint = 0;
-    int adj_mixed = 0;                // mixed adjoint plan: bf16-pair fields, fp32 adjoint (backward_chunk)
+    int adj_mixed = 0;                // mixed adjoint plan: mixed-form fields, fp32 adjoint (backward_chunk)
+    void *Jadj = nullptr;             // mixed adjoint: fp32 first operator part / adjoint source, [L][P] per item
     int dev_inputs = 0;               // OCRT_PLAN_DEVICE_INPUTS: no shared surface kernels
     int fwd_ready = 0;                // the last execute left this plan's checkpoints intact
     int ckpt_every = 1;
@@ -772,6 +773,9 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
                 it.off_W = wlr; wlr += it.lowrank ? 4LL * L : 0;
                 it.off_nodes = p->nodes_n;
                 p->nodes_n += 6LL * n;
+                // the mixed adjoint keeps every field as its pair: a bf16 field
+                // alone at the higher modes moves the per-layer gradients by
+                // about 5e-3 of their scale
                 it.fsplit = ((mixed && m < kSplitModes) || adj_mixed) ? 1 : 0;
                 it.off_Flo = flo; flo += it.fsplit ? (long long)L * Pp : 0;
                 c.max_pack = std::max(c.max_pack, (long long)Mp * it.K);
@@ -913,9 +917,13 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         for (const HostInst &h : p->inst) maxit = std::max(maxit, h.d.max_orders);
         p->ckpt.assign((maxit - 1) / p->ckpt_every + 1, nullptr);
         p->seg.assign(p->ckpt_every - 1, nullptr);
-        // fields: fp64, or the bf16 pair of the mixed plans as two planes of
-        // mx_F entries (hi, then lo); lambda and the second parts fp64 / fp32
+        // fields: fp64, or those of the mixed plans as two bf16 planes, the
+        // field (mx_F entries) then the low halves of the pairs (mx_flo <=
+        // mx_F); lambda and the operator parts fp64 / fp32
         const size_t fb = mixed ? 4 : 8;
+        if (mixed) rc |= dmalloc(p, &p->Jadj, mx_F * 4);
+        if (mixed) rc |= dmalloc(p, (void **)&P.sb_hi, mx_F * 2);
+        if (mixed) rc |= dmalloc(p, (void **)&P.sb_lo, mx_F * 2);
         for (void *&q : p->ckpt) rc |= dmalloc(p, &q, mx_F * fb);
         for (void *&q : p->seg) rc |= dmalloc(p, &q, mx_F * fb);
         // the mixed sweeps write the live columns only: the pad columns of
@@ -925,6 +933,15 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
             for (void *q : p->ckpt) rc |= hipMemset(q, 0, mx_F * fb) != hipSuccess;
         if (mixed && !rc)
             for (void *q : p->seg) rc |= hipMemset(q, 0, mx_F * fb) != hipSuccess;
+        // and of the adjoint source's pair planes (the sweeps write the live
+        // columns only)
+        if (mixed && !rc)
+            rc |= (hipMemset(P.sb_hi, 0, mx_F * 2) != hipSuccess) || (hipMemset(P.sb_lo, 0, mx_F * 2) != hipSuccess);
+        // and of the working field, which order 1 (written to its checkpoint)
+        // no longer clears
+        if (mixed && !rc)
+            rc |= (hipMemset(P.F, 0, mx_F * fe) != hipSuccess) ||
+                  (hipMemset(P.Flo, 0, std::max(mx_flo, 1LL) * 2) != hipSuccess);
         rc |= dmalloc(p, &p->At, mx_A * ae);
         rc |= dmalloc(p, (void **)&P.lam, mx_F * fb);
         rc |= dmalloc(p, (void **)&P.sc, mx_T * 6 * 8);
@@ -1105,7 +1122,8 @@ static void launch_gemm_mixed(ocrt_plan *p, const Chunk &c, hipStream_t s, const
 }
 
 // Point Q's field at a checkpoint or segment buffer of an adjoint plan: fp64,
-// or (mixed plans) the bf16 pair as two planes of mx_F entries, hi then lo.
+// or (mixed plans) two bf16 planes: the field (mx_F entries), then the low
+// halves of the items that keep the pair.
 static void set_field(const ocrt_plan *p, Pools &Q, void *buf)
 {
     Q.F = buf;
@@ -1113,26 +1131,27 @@ static void set_field(const ocrt_plan *p, Pools &Q, void *buf)
 }
 
 // Operator products of a mixed adjoint plan over the chunk's 64 x 128 tile
-// list (its items all keep the field pair: classes 1 and 2):
-// MODE kGemmParts on the field in P.F / P.Flo, kGemmMix (A^T) on the fp32 field in P.F.
+// list (single-part classes 0 and 1, two-part class 2):
+// MODE kGemmParts on the field in P.F / P.Flo, kGemmMix (A^T) on the adjoint
+// source's bf16 pair in P.F / P.Flo.
 static void launch_adj_gemm(int mode, const Chunk &c, hipStream_t s, const Pools &P, const ItemState *st, int order)
 {
     const GemmTile *t = c.dtiles_s[0];
-    const unsigned n1 = (unsigned)(c.cls_off[0][2] - c.cls_off[0][1]), n2 = (unsigned)(c.cls_off[0][3] - c.cls_off[0][2]);
+    const unsigned n1 = (unsigned)(c.cls_off[0][2] - c.cls_off[0][0]), n2 = (unsigned)(c.cls_off[0][3] - c.cls_off[0][2]);
     if (n1 > 0) {
         if (mode == kGemmParts)
             hipLaunchKernelGGL((gemm_adj<1, kGemmParts, 0>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P,
-                               t + c.cls_off[0][1], st, order);
+                               t + c.cls_off[0][0], st, order);
         else
-            hipLaunchKernelGGL((gemm_adj<1, kGemmMix, 2>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P,
-                               t + c.cls_off[0][1], st, order);
+            hipLaunchKernelGGL((gemm_adj<1, kGemmMix, 0>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P,
+                               t + c.cls_off[0][0], st, order);
     }
     if (n2 > 0) {
         if (mode == kGemmParts)
             hipLaunchKernelGGL((gemm_adj<2, kGemmParts, 0>), dim3(n2), dim3(256), 0, s, c.dinst, c.ditem, P,
                                t + c.cls_off[0][2], st, order);
         else
-            hipLaunchKernelGGL((gemm_adj<2, kGemmMix, 2>), dim3(n2), dim3(256), 0, s, c.dinst, c.ditem, P,
+            hipLaunchKernelGGL((gemm_adj<2, kGemmMix, 0>), dim3(n2), dim3(256), 0, s, c.dinst, c.ditem, P,
                                t + c.cls_off[0][2], st, order);
     }
 }
@@ -1244,28 +1263,14 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             hipLaunchKernelGGL((pack_operator<double, true>), pack_grid, dim3(256), 0, s, c.dinst, c.ditem, Pt, nit,
                                pack_blocks);
     }
-    // adjoint plans keep u_1, u_{1+e}, u_{1+2e}, ... of every item.  With
-    // e = 1 the sweeps write straight into the checkpoint of their order and
-    // the next order reads it there (no copies).  Mixed adjoint plans keep the
-    // bf16 pair (both planes).
+    // adjoint plans keep u_1, u_{1+e}, u_{1+2e}, ... of every item: the sweep
+    // of such an order writes straight into its checkpoint and the next order
+    // reads it there (no copies); the other orders use the working buffer.
     const int every = p->ckpt_every;
-    const bool direct = p->adjoint && every == 1;
     auto fields_of = [&](int order) -> Pools {
         Pools Q = P;
-        if (direct) set_field(p, Q, p->ckpt[order - 1]);
+        if (p->adjoint && (order - 1) % every == 0) set_field(p, Q, p->ckpt[(order - 1) / every]);
         return Q;
-    };
-    auto checkpoint = [&](int order) -> int {
-        if (!p->adjoint || direct || (order - 1) % every != 0) return 0;
-        void *dst = p->ckpt[(order - 1) / every];
-        if (!p->adj_mixed)
-            return hipMemcpyAsync(dst, P.F, (size_t)c.F * 8, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0
-                                                                                                     : OCRT_ERR_DEVICE;
-        Pools Q = P;
-        set_field(p, Q, dst);
-        return (hipMemcpyAsync(Q.F, P.F, (size_t)c.F * 2, hipMemcpyDeviceToDevice, s) == hipSuccess &&
-                hipMemcpyAsync(Q.Flo, P.Flo, (size_t)c.F * 2, hipMemcpyDeviceToDevice, s) == hipSuccess)
-                   ? 0 : OCRT_ERR_DEVICE;
     };
     // per-column kernels: one 64-column tile per wavefront, four per workgroup
     const int ncols = (int)c.hcols.size();
@@ -1276,7 +1281,6 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     else
         hipLaunchKernelGGL((order_one<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, fields_of(1), st,
                            c.dcols, ncols);
-    if (checkpoint(1)) return OCRT_ERR_DEVICE;
     const unsigned ntiles = (unsigned)c.htiles.size();
     for (int it = 2; it <= c.maxit; ++it) {
         const EvPair eg = p->profile ? ev_pair(p, 0) : EvPair{{nullptr, nullptr}, false};
@@ -1302,7 +1306,6 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             if (es.ok) (void)hipEventRecord(es.e[1], s);
         }
         p->gemm_launches += 1.0;
-        if (checkpoint(it)) return OCRT_ERR_DEVICE;
     }
     if (mixed)
         hipLaunchKernelGGL(order_status<__hip_bfloat16>, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, P, st, c.maxit + 1);
@@ -1351,10 +1354,10 @@ static void backward_steps(ocrt_plan *p, Chunk &c, hipStream_t s)
         set_field(p, Q, buf);
         return Q;
     };
-    // mixed plans: the operator parts (and the adjoint source written over the
-    // first) in the fp32 source pool
+    // mixed plans: the first operator part (and the adjoint source written
+    // over it) in a buffer of its own, fp32
     Pools Pw = P0;
-    if (MX) Pw.J = P0.Jf;
+    if (MX) Pw.J = p->Jadj;
     const int every = p->ckpt_every;
     for (int sg = (maxit - 1) / every; sg >= 0; --sg) {
         const int base = every * sg + 1;   // order of the checkpoint
@@ -1391,7 +1394,8 @@ static void backward_steps(ocrt_plan *p, Chunk &c, hipStream_t s)
             hipLaunchKernelGGL((sweep_adj<FV, LT>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st, c.dcols, ncols, o);
             Pools Pt = P0;
             Pt.A = p->At;
-            Pt.F = Pw.J;
+            Pt.F = MX ? (void *)P0.sb_hi : Pw.J;
+            Pt.Flo = P0.sb_lo;
             Pt.J = P0.lam;
             if constexpr (MX)
                 launch_adj_gemm(kGemmMix, c, s, Pt, st, o);

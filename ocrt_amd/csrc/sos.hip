@@ -631,8 +631,9 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? (TM * TN > 8192 ? 2 : 5)
 //   MODE kGemmParts, FIN 0: the forward field u_{o-1}, kept as its bf16 pair
 //     (P.F, P.Flo), times the operator A, the parts unscaled:
 //     J[lev][r] = (A_0 u)[r][lev], J2[lev][r] = (A_1 u)[r][lev]
-//   MODE kGemmMix, FIN 2: the fp32 adjoint source Sbar (P.F) split into bf16
-//     pairs on the way into LDS, times the transposed operator (P.A):
+//   MODE kGemmMix: the adjoint source Sbar, as its bf16 pair (FIN 0, P.F and
+//     P.Flo) or fp32 (FIN 2, P.F, split on the way into LDS), times the
+//     transposed operator (P.A):
 //     J[lev][r] = s0(lev) (A_0^T Sbar)[r][lev] + ydel(lev) (A_1^T Sbar)[r][lev]
 // with s0 = xdel for a particle first part, ydel for a molecular one.
 template <int NP, int MODE, int FIN>
@@ -640,7 +641,7 @@ __global__ __launch_bounds__(256, 1) void gemm_adj(const DevInst *__restrict__ i
                                                    Pools P, const GemmTile *__restrict__ tiles,
                                                    const ItemState *__restrict__ st, int order)
 {
-    static_assert((MODE == kGemmParts && FIN == 0) || (MODE == kGemmMix && FIN == 2), "adjoint product forms");
+    static_assert(FIN == 0 || (MODE == kGemmMix && FIN == 2), "adjoint product forms");
     const GemmTile tl = tiles[blockIdx.x];
     if (tl.item < 0 || !item_runs(st[tl.item], order)) return;
     const DevItem &it = items[tl.item];
@@ -1142,9 +1143,9 @@ template __global__ void gemm_adj<1, kGemmParts, 0>(const DevInst *, const DevIt
                                                       const ItemState *, int);
 template __global__ void gemm_adj<2, kGemmParts, 0>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                                       const ItemState *, int);
-template __global__ void gemm_adj<1, kGemmMix, 2>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+template __global__ void gemm_adj<1, kGemmMix, 0>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                                     const ItemState *, int);
-template __global__ void gemm_adj<2, kGemmMix, 2>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+template __global__ void gemm_adj<2, kGemmMix, 0>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                                     const ItemState *, int);
 template __global__ void gemm_f64<kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                             const ItemState *, int);

@@ -21,9 +21,10 @@ struct LevelView {
 };
 
 // Read access to the forward field of an adjoint plan, as fp64, at column
-// `col` of one item: the fp64 field (fp64 plans) or the bf16 pair hi + lo that
-// the mixed plans keep (Pools::F and Pools::Flo; their items all keep the pair,
-// at off_Flo == off_F).
+// `col` of one item: the fp64 field (fp64 plans), or the mixed form's bf16
+// field plus, for the items that keep the pair, its low half (Pools::F and
+// Pools::Flo).  Items with a bf16 field alone read the field twice and weigh
+// the second read by zero, so every load is unconditional.
 struct FieldF64 {
     const double *p;
     __device__ FieldF64(const Pools &P, const DevItem &it, int col)
@@ -32,12 +33,14 @@ struct FieldF64 {
 };
 struct FieldPair {
     const __hip_bfloat16 *hi, *lo;
+    float wlo;
     __device__ FieldPair(const Pools &P, const DevItem &it, int col)
         : hi(reinterpret_cast<const __hip_bfloat16 *>(P.F) + it.off_F + col),
-          lo(reinterpret_cast<const __hip_bfloat16 *>(P.Flo) + it.off_Flo + col) {}
+          lo(it.fsplit ? reinterpret_cast<const __hip_bfloat16 *>(P.Flo) + it.off_Flo + col : hi),
+          wlo(it.fsplit ? 1.f : 0.f) {}
     __device__ double operator[](long long ix) const
     {
-        return (double)__bfloat162float(hi[ix]) + (double)__bfloat162float(lo[ix]);
+        return (double)__bfloat162float(hi[ix]) + (double)(wlo * __bfloat162float(lo[ix]));
     }
 };
 
