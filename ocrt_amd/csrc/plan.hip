@@ -42,6 +42,8 @@ template <int NP, bool SPLIT, int TM, int TN>
 __global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <int NP, int MODE, int FIN>
 __global__ void gemm_adj(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
+template <int NP>
+__global__ void gemm_rebuild(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <typename FT, typename JT, bool MIXED, int U, int W>
 __global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int, int);
 __global__ void extract(const DevInst *, const DevItem *, Pools, const ItemState *, ItemOut *, int);
@@ -300,6 +302,9 @@ struct ocrt_plan {
     int adjoint = 0;
     int adj_mixed = 0;                // mixed adjoint plan: mixed-form fields, fp32 adjoint (backward_chunk)
     void *Jadj = nullptr;             // mixed adjoint: fp32 first operator part / adjoint source, [L][P] per item
+    // mixed adjoint with sparse checkpoints: the fp32 operator parts (A_0 u,
+    // A_1 u) of the orders rebuilt between two checkpoints, one pair per slot
+    std::vector<void *> rpart0, rpart1;
     int dev_inputs = 0;               // OCRT_PLAN_DEVICE_INPUTS: no shared surface kernels
     int fwd_ready = 0;                // the last execute left this plan's checkpoints intact
     int ckpt_every = 1;
@@ -922,6 +927,12 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         // mx_F); lambda and the operator parts fp64 / fp32
         const size_t fb = mixed ? 4 : 8;
         if (mixed) rc |= dmalloc(p, &p->Jadj, mx_F * 4);
+        if (mixed) {
+            p->rpart0.assign(p->ckpt_every - 1, nullptr);
+            p->rpart1.assign(p->ckpt_every - 1, nullptr);
+            for (void *&q : p->rpart0) rc |= dmalloc(p, &q, mx_F * 4);
+            for (void *&q : p->rpart1) rc |= dmalloc(p, &q, mx_F * 4);
+        }
         if (mixed) rc |= dmalloc(p, (void **)&P.sb_hi, mx_F * 2);
         if (mixed) rc |= dmalloc(p, (void **)&P.sb_lo, mx_F * 2);
         for (void *&q : p->ckpt) rc |= dmalloc(p, &q, mx_F * fb);
@@ -1128,6 +1139,20 @@ static void set_field(const ocrt_plan *p, Pools &Q, void *buf)
 {
     Q.F = buf;
     if (p->adj_mixed) Q.Flo = reinterpret_cast<__hip_bfloat16 *>(buf) + p->mx_F;
+}
+
+// Rebuilt order of a mixed adjoint plan (gemm_rebuild) over the chunk's
+// 64 x 128 tile list: the sweep's source and the order's operator parts.
+static void launch_rebuild_gemm(const Chunk &c, hipStream_t s, const Pools &P, const ItemState *st, int order)
+{
+    const GemmTile *t = c.dtiles_s[0];
+    const unsigned n1 = (unsigned)(c.cls_off[0][2] - c.cls_off[0][0]), n2 = (unsigned)(c.cls_off[0][3] - c.cls_off[0][2]);
+    if (n1 > 0)
+        hipLaunchKernelGGL(gemm_rebuild<1>, dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P, t + c.cls_off[0][0], st,
+                           order);
+    if (n2 > 0)
+        hipLaunchKernelGGL(gemm_rebuild<2>, dim3(n2), dim3(256), 0, s, c.dinst, c.ditem, P, t + c.cls_off[0][2], st,
+                           order);
 }
 
 // Operator products of a mixed adjoint plan over the chunk's 64 x 128 tile
@@ -1369,7 +1394,12 @@ static void backward_steps(ocrt_plan *p, Chunk &c, hipStream_t s)
             hipLaunchKernelGGL(adj_surface_bc, dim3(nit), dim3(64), 0, s, c.dinst, c.ditem, Pin, st, o);
             const Pools Pout = at(slot(sg, j));
             if constexpr (MX) {
-                launch_gemm_mixed(p, c, s, Pin, st, o);
+                // the source of the rebuilt sweep, and the operator parts the
+                // backward step of this order reads
+                Pools Pr = Pin;
+                Pr.J = p->rpart0[j - 1];
+                Pr.J2 = p->rpart1[j - 1];
+                launch_rebuild_gemm(c, s, Pr, st, o);
                 launch_sweep_mixed(p->sweep_variant, ncols, s, c.dinst, c.ditem, Pout, st, c.dcols, o);
             } else {
                 hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pin, c.dtiles,
@@ -1384,13 +1414,21 @@ static void backward_steps(ocrt_plan *p, Chunk &c, hipStream_t s)
             void *u_prev = slot(sg, o - 1 - base);
             Pools Pg = at(u_prev);
             Pg.J = Pw.J;
-            if constexpr (MX)
+            // the operator parts A u_{o-1}: kept by the rebuild of a mixed
+            // plan's orders between checkpoints, else one product here
+            const bool kept = MX && o - base >= 1 && o - base < every;
+            if (kept) {
+                Pg.J = p->rpart0[o - base - 1];
+                Pg.J2 = p->rpart1[o - base - 1];
+            } else if constexpr (MX) {
                 launch_adj_gemm(kGemmParts, c, s, Pg, st, o);
-            else
+            } else {
                 hipLaunchKernelGGL(gemm_f64<kGemmParts>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pg,
                                    c.dtiles, st, o);
+            }
             Pools Ps = at(u_o);
-            Ps.J = Pw.J;
+            Ps.J = Pg.J;
+            Ps.J2 = Pg.J2;
             hipLaunchKernelGGL((sweep_adj<FV, LT>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st, c.dcols, ncols, o);
             Pools Pt = P0;
             Pt.A = p->At;

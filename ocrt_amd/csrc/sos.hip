@@ -387,7 +387,8 @@ template <int NP, bool SPLIT, int TM = 64, int TN = kGemmLevels> struct GemmLds 
 // Rows past the operator's Mp rows load its last row (their results are never
 // used).  EPI = 0: the fp32 results, scaled by the level fractions, go to
 // out(lev, r, v) for the four rows r .. r + 3 of v; EPI = 1: the parts go
-// unscaled to out(lev, r, v0, v1) (v1 zero for one part); levels < L only.
+// unscaled to out(lev, r, v0, v1) (v1 zero for one part); EPI = 2: both,
+// out(lev, r, v, v0, v1); levels < L only.
 // LO = false is the reduced product: the operator's hi halves times the bf16
 // field alone (no A_lo, no field residual), one MFMA per block.
 template <int NP, bool SPLIT, int FIN = 0, int EPI = 0, int TM = 64, int TN = kGemmLevels, bool LO = true,
@@ -568,7 +569,16 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
                     v.x = fmaf(y, acc[NP - 1][a][b][0], v.x); v.y = fmaf(y, acc[NP - 1][a][b][1], v.y);
                     v.z = fmaf(y, acc[NP - 1][a][b][2], v.z); v.w = fmaf(y, acc[NP - 1][a][b][3], v.w);
                 }
-                out(lev, r, v);
+                if constexpr (EPI == 2) {
+                    // the scaled sum and the unscaled parts
+                    const float4 p0 = {acc[0][a][b][0], acc[0][a][b][1], acc[0][a][b][2], acc[0][a][b][3]};
+                    float4 p1 = {0.f, 0.f, 0.f, 0.f};
+                    if constexpr (two) p1 = {acc[NP - 1][a][b][0], acc[NP - 1][a][b][1], acc[NP - 1][a][b][2],
+                                             acc[NP - 1][a][b][3]};
+                    out(lev, r, v, p0, p1);
+                } else {
+                    out(lev, r, v);
+                }
             }
         }
     }
@@ -669,6 +679,40 @@ __global__ __launch_bounds__(256, 1) void gemm_adj(const DevInst *__restrict__ i
                                              if (NP > 1) *reinterpret_cast<float4 *>(J2 + (long long)lev * Pp + r) = v1;
                                          });
     }
+}
+
+// Scattering integral of an order rebuilt by the mixed adjoint between two
+// checkpoints (its items all keep the field pair): the source of the rebuilt
+// sweep, J[lev][r] = s0(lev) (A_0 F)[r][lev] + ydel(lev) (A_1 F)[r][lev] in the
+// fp32 source pool, and the unscaled parts the same order's backward step reads
+// (P.J = A_0 F, P.J2 = A_1 F, fp32), from one product.  64 x 128 tiles.
+template <int NP>
+__global__ __launch_bounds__(256, 1) void gemm_rebuild(const DevInst *__restrict__ inst,
+                                                       const DevItem *__restrict__ items, Pools P,
+                                                       const GemmTile *__restrict__ tiles,
+                                                       const ItemState *__restrict__ st, int order)
+{
+    const GemmTile tl = tiles[blockIdx.x];
+    if (tl.item < 0 || !item_runs(st[tl.item], order)) return;
+    const DevItem &it = items[tl.item];
+    const DevInst &I = inst[it.inst];
+    const int Pp = it.P;
+    const unsigned short *A = reinterpret_cast<const unsigned short *>(P.A) + it.off_A;
+    const unsigned short *F = reinterpret_cast<const unsigned short *>(P.F) + it.off_F;
+    const unsigned short *Flo = reinterpret_cast<const unsigned short *>(P.Flo) + it.off_Flo;
+    float *Jf = P.Jf + it.off_Flo;
+    float *J0 = reinterpret_cast<float *>(P.J) + it.off_F;
+    float *J1 = reinterpret_cast<float *>(P.J2) + it.off_F;
+    const LevelView lv(P, I);
+    __shared__ GemmLds<NP, true> S;
+    gemm_tile_bf16<NP, true, 0, 2>(it, lv, A, F, Flo, nullptr, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
+                                   [&](int lev, int r, float4 v, float4 p0, float4 p1) {
+                                       if (r >= Pp) return;
+                                       const long long ix = (long long)lev * Pp + r;
+                                       *reinterpret_cast<float4 *>(Jf + ix) = v;
+                                       *reinterpret_cast<float4 *>(J0 + ix) = p0;
+                                       if (NP > 1) *reinterpret_cast<float4 *>(J1 + ix) = p1;
+                                   });
 }
 
 // Formal solution of the current order for every column, fused with the order
@@ -1139,6 +1183,10 @@ OCRT_GEMM_INST(1, false, 128, 128)
 OCRT_GEMM_INST(1, true, 128, 128)
 OCRT_GEMM_INST(2, true, 128, 128)
 #undef OCRT_GEMM_INST
+template __global__ void gemm_rebuild<1>(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *,
+                                          int);
+template __global__ void gemm_rebuild<2>(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *,
+                                          int);
 template __global__ void gemm_adj<1, kGemmParts, 0>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                                       const ItemState *, int);
 template __global__ void gemm_adj<2, kGemmParts, 0>(const DevInst *, const DevItem *, Pools, const GemmTile *,
