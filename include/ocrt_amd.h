@@ -246,12 +246,16 @@ void ocrt_plan_destroy(ocrt_plan *plan);
 /* Plan options for ocrt_plan_create_ex. */
 enum {
     OCRT_PLAN_ONE_PER_LAUNCH = 1,  /* instances one after another (the plain form) */
-    OCRT_PLAN_ADJOINT = 2,         /* keep order checkpoints for ocrt_plan_adjoint (FP64, IPSS off) */
+    OCRT_PLAN_ADJOINT = 2,         /* keep order checkpoints for ocrt_plan_adjoint (IPSS off); FP64: fp64
+                                      fields and adjoint; MIXED: the mixed order loop with every field kept
+                                      as its bf16 pair, fp32 adjoint fields, bf16 MFMA products */
     OCRT_PLAN_CKPT_SPARSE = 4,     /* adjoint: checkpoint every 4th order and recompute the rest, whatever
                                       the memory allows (default: every order kept when the batch fits one
                                       chunk, else every 2nd, else every 4th) */
-    OCRT_PLAN_AUTOTUNE = 8,        /* mixed form: the first execute times every launch shape of the layer
-                                      sweep on this plan's work and keeps the fastest (ocrt_plan_tuning) */
+    OCRT_PLAN_AUTOTUNE = 8,        /* mixed form: the first execute times every GEMM tile shape per
+                                      operator class, then every launch shape of the layer sweep, on this
+                                      plan's work and keeps the fastest (ocrt_plan_gemm_tuning,
+                                      ocrt_plan_tuning); ignored by adjoint plans */
     OCRT_PLAN_GRAPH = 16,          /* the first execute captures the whole forward launch sequence into a
                                       hipGraph; later executes replay it (not with profiling or adjoint) */
     OCRT_PLAN_DEVICE_INPUTS = 32   /* the per-layer optical depths and wind speeds may be replaced from
