@@ -102,8 +102,12 @@ __global__ void adj_step_table(const DevInst *__restrict__ inst, Pools P)
     const double *mu = P.ring + I.off_ring;
     const double dt = fabs(h[k] - h[k + 1]);
     double *c = P.sc + 6 * (I.off_T + (long long)k * I.n);
-    for (int o = 0; o < I.n; ++o)
+    float *cf = P.scf ? P.scf + 6 * (I.off_T + (long long)k * I.n) : nullptr;
+    for (int o = 0; o < I.n; ++o) {
         step_coefs(dt, mu[o], c[6 * o], c[6 * o + 1], c[6 * o + 2], c[6 * o + 3], c[6 * o + 4], c[6 * o + 5]);
+        if (cf)
+            for (int e = 0; e < 6; ++e) cf[6 * o + e] = (float)c[6 * o + e];
+    }
 }
 
 // Adjoint sweep of order `order` >= 2.  P.F holds u_o (FV: its storage), P.J /
@@ -153,9 +157,15 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
     auto put_sbar = [&](int k, double v) {
         const long long ix = (long long)k * Pp;
         if constexpr (MX) {
-            const __hip_bfloat16 h = __float2bfloat16((float)v);
-            P.sb_hi[off_F + colr + ix] = h;
-            P.sb_lo[off_F + colr + ix] = __float2bfloat16((float)(v - (double)__bfloat162float(h)));
+            // the fp32 value's bf16 pair, rounded to nearest even on the bits
+            // (finite values; one fp64 -> fp32 conversion, no double rounding)
+            const float vf = (float)v;
+            const unsigned ub = __float_as_uint(vf);
+            const unsigned hb = (ub + 0x7fffu + ((ub >> 16) & 1u)) & 0xffff0000u;
+            const unsigned ur = __float_as_uint(vf - __uint_as_float(hb));
+            const unsigned lb = (ur + 0x7fffu + ((ur >> 16) & 1u)) >> 16;
+            reinterpret_cast<unsigned short *>(P.sb_hi)[off_F + colr + ix] = (unsigned short)(hb >> 16);
+            reinterpret_cast<unsigned short *>(P.sb_lo)[off_F + colr + ix] = (unsigned short)lb;
         } else {
             J0[ix] = (LT)v;
         }
@@ -198,6 +208,20 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
             uv[u] = U[ix];
             lv_[u] = (double)lam[ix];
         }
+        // mixed plans: the block's step coefficients from the fp32 table, with
+        // its other loads (no load waits inside the steps)
+        float cfv[MX ? 6 * kB : 1];
+        if constexpr (MX) {
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const int tc = t0 + u < nt ? t0 + u : nt - 1;
+                const long long layer = hemi ? nt - tc - 1 : tc;   // the layer between k and kp
+                const float2 *c2 = reinterpret_cast<const float2 *>(P.scf + 6 * (I.off_T + layer * n + o));
+                const float2 a = c2[0], b = c2[1], e = c2[2];
+                cfv[6 * u + 0] = a.x; cfv[6 * u + 1] = a.y; cfv[6 * u + 2] = b.x;
+                cfv[6 * u + 3] = b.y; cfv[6 * u + 4] = e.x; cfv[6 * u + 5] = e.y;
+            }
+        }
         // per step: d/d xdel, d/d ydel at level k and d/d(layer step)
         double red[3 * kB];
 #pragma unroll
@@ -210,8 +234,14 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
             const int kp = hemi ? k - 1 : k + 1;
             const double sp = source(kp, q0[u], q1[u]);
             // the step's coefficients over the layer between k and kp (table)
-            const double *cf = P.sc + 6 * (I.off_T + (long long)(k < kp ? k : kp) * n + o);
-            const double T = cf[0], al = cf[1], be = cf[2], dT = cf[3], dal = cf[4], dbe = cf[5];
+            double T, al, be, dT, dal, dbe;
+            if constexpr (MX) {
+                T = cfv[6 * u]; al = cfv[6 * u + 1]; be = cfv[6 * u + 2];
+                dT = cfv[6 * u + 3]; dal = cfv[6 * u + 4]; dbe = cfv[6 * u + 5];
+            } else {
+                const double *cf = P.sc + 6 * (I.off_T + (long long)(k < kp ? k : kp) * n + o);
+                T = cf[0]; al = cf[1]; be = cf[2]; dT = cf[3]; dal = cf[4]; dbe = cf[5];
+            }
             const double sbar = pending + al * rho;
             pending = be * rho;
             if (live) {

@@ -138,6 +138,7 @@ struct Pools {
                           // mixed plans: float)
     double *sc;           // adjoint: per layer and node the scan-step coefficients (T, al, be, dT, dal, dbe),
                           // [nt][n][6] at 6 off_T
+    float *scf;           // mixed adjoint: their fp32 rounding, same layout (the adjoint sweep's table)
     double *grad;         // adjoint: per-instance accumulators (see DevInst::off_grad)
     const double *gout;  // adjoint: upstream gradient [instance][3] of (rho_I, rho_Q, rho_U)
     float *comp;          // fp32 compensation of the mixed-form order sum on the two output rows

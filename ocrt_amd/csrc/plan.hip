@@ -956,6 +956,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         rc |= dmalloc(p, &p->At, mx_A * ae);
         rc |= dmalloc(p, (void **)&P.lam, mx_F * fb);
         rc |= dmalloc(p, (void **)&P.sc, mx_T * 6 * 8);
+        if (mixed) rc |= dmalloc(p, (void **)&P.scf, mx_T * 6 * 4);
         rc |= dmalloc(p, (void **)&P.J2, mx_F * fb);
         rc |= dmalloc(p, (void **)&P.dR, std::max(mx_R, 1LL) * 8);
         rc |= dmalloc(p, (void **)&p->grad, p->grad_n * 8);
