@@ -139,6 +139,8 @@ struct Pools {
     double *sc;           // adjoint: per layer and node the scan-step coefficients (T, al, be, dT, dal, dbe),
                           // [nt][n][6] at 6 off_T
     float *scf;           // mixed adjoint: their fp32 rounding, same layout (the adjoint sweep's table)
+    double *rp0, *rp1;    // fp64 adjoint: where the rebuild of an order between checkpoints puts its operator
+                          // parts A_0 F, A_1 F (gemm_f64<kGemmBoth>)
     double *grad;         // adjoint: per-instance accumulators (see DevInst::off_grad)
     const double *gout;  // adjoint: upstream gradient [instance][3] of (rho_I, rho_Q, rho_U)
     float *comp;          // fp32 compensation of the mixed-form order sum on the two output rows
@@ -239,6 +241,7 @@ constexpr int kGemmLevelsF64 = 64;
 // Epilogues of the scattering-integral kernels.
 constexpr int kGemmMix = 0;     // J = xdel A_p F + ydel A_r F
 constexpr int kGemmParts = 1;   // J = A_0 F, J2 = A_1 F (unscaled)
+constexpr int kGemmBoth = 2;    // J as kGemmMix, and the parts to Pools::rp0 / rp1 (fp64 adjoint rebuild)
 
 // Up to 64 consecutive state columns [col0, col_end) of one item: the unit of
 // the per-column kernels (one wavefront each).  A tile never mixes upward and

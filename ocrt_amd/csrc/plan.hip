@@ -302,8 +302,9 @@ struct ocrt_plan {
     int adjoint = 0;
     int adj_mixed = 0;                // mixed adjoint plan: mixed-form fields, fp32 adjoint (backward_chunk)
     void *Jadj = nullptr;             // mixed adjoint: fp32 first operator part / adjoint source, [L][P] per item
-    // mixed adjoint with sparse checkpoints: the fp32 operator parts (A_0 u,
-    // A_1 u) of the orders rebuilt between two checkpoints, one pair per slot
+    // adjoint with sparse checkpoints: the operator parts (A_0 u, A_1 u) of
+    // the orders rebuilt between two checkpoints, one pair per slot (fp64, or
+    // fp32 in mixed plans)
     std::vector<void *> rpart0, rpart1;
     int dev_inputs = 0;               // OCRT_PLAN_DEVICE_INPUTS: no shared surface kernels
     int fwd_ready = 0;                // the last execute left this plan's checkpoints intact
@@ -927,12 +928,10 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         // mx_F); lambda and the operator parts fp64 / fp32
         const size_t fb = mixed ? 4 : 8;
         if (mixed) rc |= dmalloc(p, &p->Jadj, mx_F * 4);
-        if (mixed) {
-            p->rpart0.assign(p->ckpt_every - 1, nullptr);
-            p->rpart1.assign(p->ckpt_every - 1, nullptr);
-            for (void *&q : p->rpart0) rc |= dmalloc(p, &q, mx_F * 4);
-            for (void *&q : p->rpart1) rc |= dmalloc(p, &q, mx_F * 4);
-        }
+        p->rpart0.assign(p->ckpt_every - 1, nullptr);
+        p->rpart1.assign(p->ckpt_every - 1, nullptr);
+        for (void *&q : p->rpart0) rc |= dmalloc(p, &q, mx_F * fb);
+        for (void *&q : p->rpart1) rc |= dmalloc(p, &q, mx_F * fb);
         if (mixed) rc |= dmalloc(p, (void **)&P.sb_hi, mx_F * 2);
         if (mixed) rc |= dmalloc(p, (void **)&P.sb_lo, mx_F * 2);
         for (void *&q : p->ckpt) rc |= dmalloc(p, &q, mx_F * fb);
@@ -1403,8 +1402,11 @@ static void backward_steps(ocrt_plan *p, Chunk &c, hipStream_t s)
                 launch_rebuild_gemm(c, s, Pr, st, o);
                 launch_sweep_mixed(p->sweep_variant, ncols, s, c.dinst, c.ditem, Pout, st, c.dcols, o);
             } else {
-                hipLaunchKernelGGL(gemm_f64<kGemmMix>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pin, c.dtiles,
-                                   st, o);
+                Pools Pr = Pin;
+                Pr.rp0 = static_cast<double *>(p->rpart0[j - 1]);
+                Pr.rp1 = static_cast<double *>(p->rpart1[j - 1]);
+                hipLaunchKernelGGL(gemm_f64<kGemmBoth>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pr,
+                                   c.dtiles, st, o);
                 hipLaunchKernelGGL((sweep<double, double, false, 8, 4>), cols, dim3(256), 0, s, c.dinst, c.ditem,
                                    Pout, st, c.dcols, ncols, o);
             }
@@ -1415,9 +1417,9 @@ static void backward_steps(ocrt_plan *p, Chunk &c, hipStream_t s)
             void *u_prev = slot(sg, o - 1 - base);
             Pools Pg = at(u_prev);
             Pg.J = Pw.J;
-            // the operator parts A u_{o-1}: kept by the rebuild of a mixed
-            // plan's orders between checkpoints, else one product here
-            const bool kept = MX && o - base >= 1 && o - base < every;
+            // the operator parts A u_{o-1}: kept by the rebuild of the orders
+            // between checkpoints, else one product here
+            const bool kept = o - base >= 1 && o - base < every;
             if (kept) {
                 Pg.J = p->rpart0[o - base - 1];
                 Pg.J2 = p->rpart1[o - base - 1];
@@ -1433,7 +1435,7 @@ static void backward_steps(ocrt_plan *p, Chunk &c, hipStream_t s)
             hipLaunchKernelGGL((sweep_adj<FV, LT>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st, c.dcols, ncols, o);
             Pools Pt = P0;
             Pt.A = p->At;
-            Pt.F = MX ? (void *)P0.sb_hi : Pw.J;
+            Pt.F = MX ? (void *)P0.sb_hi : Ps.J;   // fp64: the sweep wrote Sbar over the first part
             Pt.Flo = P0.sb_lo;
             Pt.J = P0.lam;
             if constexpr (MX)

@@ -262,6 +262,8 @@ __global__ void order_status(const DevInst *__restrict__ inst, const DevItem *__
 // set per operator part.  Epilogue:
 //   kGemmMix   J[lev][r]  = s0(lev) (A_0 F)[r][lev] + ydel(lev) (A_1 F)[r][lev]
 //   kGemmParts J[lev][r]  = (A_0 F)[r][lev], J2[lev][r] = (A_1 F)[r][lev]
+//   kGemmBoth  J as kGemmMix, and rp0 / rp1 as J / J2 of kGemmParts (an
+//              order the adjoint rebuilds between checkpoints)
 // with s0 = xdel for a particle first part, ydel for a molecular one.
 template <int MODE>
 __global__ __launch_bounds__(256) void gemm_f64(const DevInst *__restrict__ inst,
@@ -332,10 +334,15 @@ __global__ __launch_bounds__(256) void gemm_f64(const DevInst *__restrict__ inst
             double *dst2 = J2 + (long long)lev * Pp;
             for (int i = 0; i < 4; ++i) {
                 if (r + 4 * i >= Pp) continue;
-                if (MODE == kGemmMix) {
+                if (MODE == kGemmMix || MODE == kGemmBoth) {
                     double v = s0 * acc[0][a][b][i];
                     if (nparts > 1) v += y * acc[1][a][b][i];
                     dst[r + 4 * i] = v;
+                    if (MODE == kGemmBoth) {
+                        const long long ix = it.off_F + (long long)lev * Pp + r + 4 * i;
+                        P.rp0[ix] = acc[0][a][b][i];
+                        if (nparts > 1) P.rp1[ix] = acc[1][a][b][i];
+                    }
                 } else {
                     dst[r + 4 * i] = acc[0][a][b][i];
                     if (nparts > 1) dst2[r + 4 * i] = acc[1][a][b][i];
@@ -1198,6 +1205,8 @@ template __global__ void gemm_adj<2, kGemmMix, 0>(const DevInst *, const DevItem
 template __global__ void gemm_f64<kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                             const ItemState *, int);
 template __global__ void gemm_f64<kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+                                              const ItemState *, int);
+template __global__ void gemm_f64<kGemmBoth>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                               const ItemState *, int);
 
 }  // namespace ocrt
