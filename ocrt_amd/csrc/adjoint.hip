@@ -433,6 +433,21 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
             uv[u] = U[ix];
             lv_[u] = (double)lam[ix] * lmask;
         }
+        // mixed plans: the block's step coefficients from the fp32 table, with
+        // its other loads
+        constexpr bool MX = sizeof(LT) == 4;
+        float cfv[MX ? 6 * kB : 1];
+        if constexpr (MX) {
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const int tc = t0 + u < nt ? t0 + u : nt - 1;
+                const long long layer = hemi ? nt - tc - 1 : tc;   // the layer between k and kp
+                const float2 *c2 = reinterpret_cast<const float2 *>(P.scf + 6 * (I.off_T + layer * n + o));
+                const float2 a = c2[0], b = c2[1], e = c2[2];
+                cfv[6 * u + 0] = a.x; cfv[6 * u + 1] = a.y; cfv[6 * u + 2] = b.x;
+                cfv[6 * u + 3] = b.y; cfv[6 * u + 4] = e.x; cfv[6 * u + 5] = e.y;
+            }
+        }
         // per step: d/d xdel, d/d ydel, d/dh at level k and d/d(layer step)
         double red[4 * kB];
 #pragma unroll
@@ -445,8 +460,14 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
             const int kp = hemi ? k - 1 : k + 1;
             const double sp = S(kp);
             // the step's coefficients over the layer between k and kp (table)
-            const double *cf = P.sc + 6 * (I.off_T + (long long)(k < kp ? k : kp) * n + o);
-            const double T = cf[0], al = cf[1], be = cf[2], dT = cf[3], dal = cf[4], dbe = cf[5];
+            double T, al, be, dT, dal, dbe;
+            if constexpr (MX) {
+                T = cfv[6 * u]; al = cfv[6 * u + 1]; be = cfv[6 * u + 2];
+                dT = cfv[6 * u + 3]; dal = cfv[6 * u + 4]; dbe = cfv[6 * u + 5];
+            } else {
+                const double *cf = P.sc + 6 * (I.off_T + (long long)(k < kp ? k : kp) * n + o);
+                T = cf[0]; al = cf[1]; be = cf[2]; dT = cf[3]; dal = cf[4]; dbe = cf[5];
+            }
             const double sbar = pending + al * rho;
             pending = be * rho;
             const double fkp = rough ? seed_f(kp) : 0.0;
