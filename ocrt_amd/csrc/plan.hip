@@ -40,7 +40,7 @@ template <int MODE>
 __global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <int NP, bool SPLIT, int TM, int TN>
 __global__ void gemm_bf16(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
-template <int NP, int MODE, int FIN>
+template <int NP, int MODE>
 __global__ void gemm_adj(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <int NP>
 __global__ void gemm_rebuild(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
@@ -1165,18 +1165,18 @@ static void launch_adj_gemm(int mode, const Chunk &c, hipStream_t s, const Pools
     const unsigned n1 = (unsigned)(c.cls_off[0][2] - c.cls_off[0][0]), n2 = (unsigned)(c.cls_off[0][3] - c.cls_off[0][2]);
     if (n1 > 0) {
         if (mode == kGemmParts)
-            hipLaunchKernelGGL((gemm_adj<1, kGemmParts, 0>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P,
+            hipLaunchKernelGGL((gemm_adj<1, kGemmParts>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P,
                                t + c.cls_off[0][0], st, order);
         else
-            hipLaunchKernelGGL((gemm_adj<1, kGemmMix, 0>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P,
+            hipLaunchKernelGGL((gemm_adj<1, kGemmMix>), dim3(n1), dim3(256), 0, s, c.dinst, c.ditem, P,
                                t + c.cls_off[0][0], st, order);
     }
     if (n2 > 0) {
         if (mode == kGemmParts)
-            hipLaunchKernelGGL((gemm_adj<2, kGemmParts, 0>), dim3(n2), dim3(256), 0, s, c.dinst, c.ditem, P,
+            hipLaunchKernelGGL((gemm_adj<2, kGemmParts>), dim3(n2), dim3(256), 0, s, c.dinst, c.ditem, P,
                                t + c.cls_off[0][2], st, order);
         else
-            hipLaunchKernelGGL((gemm_adj<2, kGemmMix, 0>), dim3(n2), dim3(256), 0, s, c.dinst, c.ditem, P,
+            hipLaunchKernelGGL((gemm_adj<2, kGemmMix>), dim3(n2), dim3(256), 0, s, c.dinst, c.ditem, P,
                                t + c.cls_off[0][2], st, order);
     }
 }

@@ -376,10 +376,33 @@ __device__ inline bf16x8_ lds_frag(const unsigned short *p)
 // t % 4) and the fragment reads (lane l: row l % 16, chunk l / 16) free of
 // bank conflicts; a padded 40-entry row (80 bytes) conflicts on both, which
 // doubles the LDS cycles of the tile loop.
+// K slabs of the bf16 scattering integral in flight per tile: each stage of
+// LDS holds one slab of 32, filled by direct global -> LDS loads.
+constexpr int kGemmStages = 2;
+
 template <int NP, bool SPLIT, int TM = 64, int TN = kGemmLevels> struct GemmLds {
-    unsigned short A[NP][2][TM][32];
-    unsigned short B[SPLIT ? 2 : 1][TN][32];
+    unsigned short A[kGemmStages][NP][2][TM][32];
+    unsigned short B[kGemmStages][SPLIT ? 2 : 1][TN][32];
 };
+
+// One 16-byte chunk per lane of a wavefront straight from global memory into
+// LDS (gfx950 global_load_lds_dwordx4): lane l's chunk lands at lds + 16 l.
+// The destination base goes in M0 and must be wavefront-uniform.  The load
+// counts in vmcnt like any vector load; nothing else in the kernels that use it
+// sets M0.
+__device__ inline void lds_dma16(const unsigned short *g, unsigned short *lds)
+{
+    const uint32_t la = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void *)lds;
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(la), "v"(g) : "memory");
+}
+
+// Wait until at most N vector-memory operations of this wavefront are
+// outstanding and every LDS read has returned, then the workgroup barrier.
+template <int N> __device__ inline void wait_vm_barrier()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
+}
 
 // One tile of the bf16 scattering integral: operator rows row0 .. row0 + TM - 1
 // and levels lev0 .. lev0 + TN - 1 of one item with NP operator parts (the
@@ -387,27 +410,35 @@ template <int NP, bool SPLIT, int TM = 64, int TN = kGemmLevels> struct GemmLds 
 // 64 x 64, 128 x 64, 128 x 128) give the same bits: every output takes the same K slabs
 // in the same order.  NP is a template parameter so that single-part items
 // hold half the accumulators (higher occupancy); SPLIT = false compiles out
-// the bf16 pair field (its registers and LDS plane) for tiles whose items keep
-// a single bf16 field.  FIN selects the field's storage: 0 the bf16 planes F
-// (and Flo), 1 an fp64 field FW, 2 an fp32 field FW; the wide forms split
-// every value into its bf16 pair on the way into LDS.
-// Rows past the operator's Mp rows load its last row (their results are never
-// used).  EPI = 0: the fp32 results, scaled by the level fractions, go to
-// out(lev, r, v) for the four rows r .. r + 3 of v; EPI = 1: the parts go
-// unscaled to out(lev, r, v0, v1) (v1 zero for one part); EPI = 2: both,
-// out(lev, r, v, v0, v1); levels < L only.
-// LO = false is the reduced product: the operator's hi halves times the bf16
-// field alone (no A_lo, no field residual), one MFMA per block.
-template <int NP, bool SPLIT, int FIN = 0, int EPI = 0, int TM = 64, int TN = kGemmLevels, bool LO = true,
-          typename Out>
+// the bf16 pair field (its LDS plane) for tiles whose items keep a single
+// bf16 field (SPLIT = true tiles hold only items that keep the pair).
+// K slabs of 32 go from global memory straight into one of kGemmStages LDS
+// stages (lds_dma16), kGemmStages - 1 slabs ahead of the MFMAs, so no staging
+// registers are held and several slabs' loads are in flight.  Each row of a
+// slab is 32 entries (four 16-byte chunks); chunk c of row r sits at position
+// c ^ ((r >> 1) & 3), which keeps the fragment reads (lane l: row l % 16,
+// chunk l / 16) free of bank conflicts on gfx950: lane l of a load writes
+// position l % 4 of row l / 4, so it fetches the chunk that belongs there.
+// Rows past the operator's Mp rows load its last row and levels past L the
+// item's last level (their results are never used).  EPI = 0: the fp32
+// results, scaled by the level fractions, go to out(lev, r, v) for the four
+// rows r .. r + 3 of v; EPI = 1: the parts go unscaled to out(lev, r, v0, v1)
+// (v1 zero for one part); EPI = 2: both, out(lev, r, v, v0, v1); levels < L
+// only.  LO = false is the reduced product: the operator's hi halves times the
+// bf16 field alone (no A_lo, no field residual), one MFMA per block.
+template <int NP, bool SPLIT, int EPI = 0, int TM = 64, int TN = kGemmLevels, bool LO = true, typename Out>
 __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, const unsigned short *A,
-                                      const unsigned short *F, const unsigned short *Flo, const void *FW,
-                                      float *Wp, int row0, int lev0, GemmLds<NP, SPLIT, TM, TN> &S, Out out)
+                                      const unsigned short *F, const unsigned short *Flo, float *Wp, int row0,
+                                      int lev0, GemmLds<NP, SPLIT, TM, TN> &S, Out out)
 {
     constexpr int AB = TM / 32, BB = TN / 32;   // 16 x 16 blocks per wave along rows and levels
-    constexpr int AR = TM / 64, FR = TN / 64;   // operator rows and field levels staged per thread
+    constexpr int AR = TM / 64, FR = TN / 64;   // 64-row groups of the operator and the field per slab
+    constexpr bool split = LO && SPLIT;
+    constexpr bool two = NP > 1;
+    // loads per thread and slab
+    constexpr int kLoads = NP * (LO ? 2 : 1) * AR + (split ? 2 : 1) * FR;
+    constexpr int ST = kGemmStages;
     const int Pp = it.P, L = it.L, Mp = it.Mp;
-    const bool split = LO && SPLIT && (FIN != 0 || it.fsplit != 0);
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
     float4_ acc[NP][AB][BB];
@@ -417,123 +448,70 @@ __device__ inline void gemm_tile_bf16(const DevItem &it, const LevelView &lv, co
         for (int a = 0; a < AB; ++a)
 #pragma unroll
             for (int b = 0; b < BB; ++b) acc[q][a][b] = (float4_){0.f, 0.f, 0.f, 0.f};
-    // global -> LDS staging: one 8-wide k chunk of operator rows lr + 64 j per
-    // thread and part, and of field levels lr + 64 j
-    const int lr = t >> 2, lk = (t & 3) * 8;
-    // the staged chunk's swizzled position (rows lr and lr + 64 share it)
-    const int lks = (((t & 3) ^ (lr >> 1)) & 3) * 8;
+    // this lane's row within its wavefront's 16-row group and the chunk it
+    // fetches for its position (the swizzle's inverse; rows 16 apart share it)
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int rl = lane >> 2;
+    const int ck = ((lane & 3) ^ ((rl >> 1) & 3)) * 8;
     const long long slab = (long long)Mp * Pp;
-    constexpr bool two = NP > 1;
-    // staged operator chunks a<part><hi/lo><row block> and field chunks
-    // rb/rl<level block>, as named registers (no indexed arrays, which the
-    // compiler would place in scratch)
-    uint4 a0h0, a0l0, a1h0, a1l0, a0h1, a0l1, a1h1, a1l1, rb0, rb1, rl0, rl1;
-    auto ld = [](const unsigned short *q) { return *reinterpret_cast<const uint4 *>(q); };
-    auto load_field = [&](int l, int k0, uint4 &hi, uint4 &lo) {
-        hi = lo = make_uint4(0u, 0u, 0u, 0u);
-        if (l >= L) return;
-        if constexpr (FIN != 0) {
-            // eight fp64 or fp32 values -> their bf16 pairs (hi of the fp32
-            // rounding, lo of the remainder), as every bf16 pair of the solver
-            typedef typename std::conditional<FIN == 1, double, float>::type WT;
-            const WT *src = reinterpret_cast<const WT *>(FW) + (long long)l * Pp + k0 + lk;
-            unsigned h[8], w[8];
+    // one slab (k0) into stage g: the operator planes [part][hi, lo] and the
+    // field planes [hi, lo], 64-row groups j, wavefront wv taking rows
+    // 64 j + 16 wv .. + 15 of each
+    auto issue = [&](int k0, int g) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const WT v = src[e];
-                const __hip_bfloat16 bh = __float2bfloat16((float)v);
-                const __hip_bfloat16 bl = __float2bfloat16((float)(v - (WT)__bfloat162float(bh)));
-                h[e] = __builtin_bit_cast(unsigned short, bh);
-                w[e] = __builtin_bit_cast(unsigned short, bl);
-            }
-            hi = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
-            lo = make_uint4(w[0] | (w[1] << 16), w[2] | (w[3] << 16), w[4] | (w[5] << 16), w[6] | (w[7] << 16));
-        } else {
-            hi = ld(F + (long long)l * Pp + k0 + lk);
-            if (split) lo = ld(Flo + (long long)l * Pp + k0 + lk);
-        }
-    };
-    auto load_slab = [&](int k0) {
-        const long long ro0 = (long long)min(row0 + lr, Mp - 1) * Pp + k0 + lk;
-        a0h0 = ld(A + ro0);
-        if constexpr (LO) a0l0 = ld(A + slab + ro0);
-        if constexpr (two) {
-            a1h0 = ld(A + 2 * slab + ro0);
-            if constexpr (LO) a1l0 = ld(A + 3 * slab + ro0);
-        }
-        if constexpr (AR > 1) {
-            const long long ro1 = (long long)min(row0 + lr + 64, Mp - 1) * Pp + k0 + lk;
-            a0h1 = ld(A + ro1);
-            if constexpr (LO) a0l1 = ld(A + slab + ro1);
-            if constexpr (two) {
-                a1h1 = ld(A + 2 * slab + ro1);
-                if constexpr (LO) a1l1 = ld(A + 3 * slab + ro1);
+        for (int j = 0; j < AR; ++j) {
+            const int r = 64 * j + 16 * wv + rl;
+            const unsigned short *src = A + (long long)min(row0 + r, Mp - 1) * Pp + k0 + ck;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                lds_dma16(src + (2 * q) * slab, &S.A[g][q][0][64 * j + 16 * wv][0]);
+                if constexpr (LO) lds_dma16(src + (2 * q + 1) * slab, &S.A[g][q][1][64 * j + 16 * wv][0]);
             }
         }
-        load_field(lev0 + lr, k0, rb0, rl0);
-        if constexpr (FR > 1) load_field(lev0 + lr + 64, k0, rb1, rl1);
-    };
-    auto st = [](unsigned short *q, uint4 v) { *reinterpret_cast<uint4 *>(q) = v; };
-    auto store_slab = [&]() {
-        st(&S.A[0][0][lr][lks], a0h0);
-        if constexpr (LO) st(&S.A[0][1][lr][lks], a0l0);
-        if constexpr (two) {
-            st(&S.A[1][0][lr][lks], a1h0);
-            if constexpr (LO) st(&S.A[1][1][lr][lks], a1l0);
+#pragma unroll
+        for (int j = 0; j < FR; ++j) {
+            const int r = 64 * j + 16 * wv + rl;
+            const long long off = (long long)min(lev0 + r, L - 1) * Pp + k0 + ck;
+            lds_dma16(F + off, &S.B[g][0][64 * j + 16 * wv][0]);
+            if constexpr (split) lds_dma16(Flo + off, &S.B[g][split ? 1 : 0][64 * j + 16 * wv][0]);
         }
-        if constexpr (AR > 1) {
-            st(&S.A[0][0][lr + 64][lks], a0h1);
-            if constexpr (LO) st(&S.A[0][1][lr + 64][lks], a0l1);
-            if constexpr (two) {
-                st(&S.A[1][0][lr + 64][lks], a1h1);
-                if constexpr (LO) st(&S.A[1][1][lr + 64][lks], a1l1);
-            }
-        }
-        st(&S.B[0][lr][lks], rb0);
-        if constexpr (FR > 1) st(&S.B[0][lr + 64][lks], rb1);
-        if constexpr (SPLIT)
-            if (split) {
-                st(&S.B[SPLIT ? 1 : 0][lr][lks], rl0);
-                if constexpr (FR > 1) st(&S.B[SPLIT ? 1 : 0][lr + 64][lks], rl1);
-            }
     };
     // fragment of a 16 x 32 block: lane l holds row (l & 15), k 8 (l >> 4) .. + 7
     const int fr = lane & 15, fk = (((lane >> 4) ^ (fr >> 1)) & 3) * 8;   // k chunk lane / 16, swizzled
-    auto mfma_part = [&](auto part) {
+    auto mfma_part = [&](int g, auto part) {
         constexpr int q = decltype(part)::value;
 #pragma unroll
         for (int a = 0; a < AB; ++a) {
-            const bf16x8_ hi = lds_frag(&S.A[q][0][wm + a * 16 + fr][fk]);
+            const bf16x8_ hi = lds_frag(&S.A[g][q][0][wm + a * 16 + fr][fk]);
             bf16x8_ lo;
-            if constexpr (LO) lo = lds_frag(&S.A[q][1][wm + a * 16 + fr][fk]);
+            if constexpr (LO) lo = lds_frag(&S.A[g][q][1][wm + a * 16 + fr][fk]);
 #pragma unroll
             for (int b = 0; b < BB; ++b) {
-                const bf16x8_ fb = lds_frag(&S.B[0][wn + b * 16 + fr][fk]);
-                if constexpr (SPLIT && LO)
-                    if (split) {
-                        const bf16x8_ fl = lds_frag(&S.B[SPLIT ? 1 : 0][wn + b * 16 + fr][fk]);
-                        acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, fl, acc[q][a][b], 0, 0, 0);
-                    }
+                const bf16x8_ fb = lds_frag(&S.B[g][0][wn + b * 16 + fr][fk]);
+                if constexpr (split) {
+                    const bf16x8_ fl = lds_frag(&S.B[g][split ? 1 : 0][wn + b * 16 + fr][fk]);
+                    acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, fl, acc[q][a][b], 0, 0, 0);
+                }
                 if constexpr (LO) acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, fb, acc[q][a][b], 0, 0, 0);
                 acc[q][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, fb, acc[q][a][b], 0, 0, 0);
             }
         }
     };
-    // K slabs of 32 go global -> registers -> LDS; the next slab's loads are
-    // issued before the current slab's MFMAs so their latency overlaps them
-    load_slab(0);
-    store_slab();
-    __syncthreads();
-    for (int k0 = 0; k0 < Pp; k0 += 32) {
-        const bool more = k0 + 32 < Pp;
-        if (more) load_slab(k0 + 32);
-        mfma_part(std::integral_constant<int, 0>{});
-        if constexpr (two) mfma_part(std::integral_constant<int, 1>{});
-        __syncthreads();
-        if (more) {
-            store_slab();
-            __syncthreads();
-        }
+    // slabs 0 .. ST - 2 first; step s then waits for slab s (the later slabs'
+    // loads may stay in flight), passes the barrier (every wavefront is done
+    // with the stage read in step s - 1), refills that stage with slab
+    // s + ST - 1 and runs slab s's MFMAs
+    const int ns = Pp / 32;
+#pragma unroll
+    for (int j = 0; j < ST - 1; ++j)
+        if (j < ns) issue(32 * j, j);
+    for (int s = 0; s < ns; ++s) {
+        const int g = s % ST;
+        if (s + ST - 2 < ns) wait_vm_barrier<(ST - 2) * kLoads>();
+        else wait_vm_barrier<0>();
+        if (s + ST - 1 < ns) issue(32 * (s + ST - 1), (s + ST - 1) % ST);
+        mfma_part(g, std::integral_constant<int, 0>{});
+        if constexpr (NP > 1) mfma_part(g, std::integral_constant<int, 1>{});
     }
     // 16 x 16 result layout: lane holds rows 4 (lane >> 4) + i of column lane & 15
     if constexpr (EPI == 1) {
@@ -636,50 +614,47 @@ __global__ __launch_bounds__(256, (NP == 1 && !SPLIT) ? (TM * TN > 8192 ? 2 : 5)
         }
     };
     if (reduced)
-        gemm_tile_bf16<NP, SPLIT, 0, 0, TM, TN, false>(it, lv, A, F, Flo, nullptr, Wp, tl.tm * TM, tl.tn * TN, S,
+        gemm_tile_bf16<NP, SPLIT, 0, TM, TN, false>(it, lv, A, F, Flo, Wp, tl.tm * TM, tl.tn * TN, S,
                                                            out);
     else
-        gemm_tile_bf16<NP, SPLIT, 0, 0, TM, TN, true>(it, lv, A, F, Flo, nullptr, Wp, tl.tm * TM, tl.tn * TN, S,
+        gemm_tile_bf16<NP, SPLIT, 0, TM, TN, true>(it, lv, A, F, Flo, Wp, tl.tm * TM, tl.tn * TN, S,
                                                           out);
 }
 
 // Operator products of the mixed adjoint plans on bf16 MFMA, three MFMAs per
 // block (A_hi F_hi + A_lo F_hi + A_hi F_lo), fp32 results:
-//   MODE kGemmParts, FIN 0: the forward field u_{o-1}, kept as its bf16 pair
+//   MODE kGemmParts: the forward field u_{o-1}, kept as its bf16 pair
 //     (P.F, P.Flo), times the operator A, the parts unscaled:
 //     J[lev][r] = (A_0 u)[r][lev], J2[lev][r] = (A_1 u)[r][lev]
-//   MODE kGemmMix: the adjoint source Sbar, as its bf16 pair (FIN 0, P.F and
-//     P.Flo) or fp32 (FIN 2, P.F, split on the way into LDS), times the
-//     transposed operator (P.A):
+//   MODE kGemmMix: the adjoint source Sbar, as its bf16 pair (P.F and P.Flo),
+//     times the transposed operator (P.A):
 //     J[lev][r] = s0(lev) (A_0^T Sbar)[r][lev] + ydel(lev) (A_1^T Sbar)[r][lev]
 // with s0 = xdel for a particle first part, ydel for a molecular one.
-template <int NP, int MODE, int FIN>
+template <int NP, int MODE>
 __global__ __launch_bounds__(256, 1) void gemm_adj(const DevInst *__restrict__ inst, const DevItem *__restrict__ items,
                                                    Pools P, const GemmTile *__restrict__ tiles,
                                                    const ItemState *__restrict__ st, int order)
 {
-    static_assert(FIN == 0 || (MODE == kGemmMix && FIN == 2), "adjoint product forms");
     const GemmTile tl = tiles[blockIdx.x];
     if (tl.item < 0 || !item_runs(st[tl.item], order)) return;
     const DevItem &it = items[tl.item];
     const DevInst &I = inst[it.inst];
     const int Pp = it.P;
     const unsigned short *A = reinterpret_cast<const unsigned short *>(P.A) + it.off_A;
-    const unsigned short *F = FIN == 0 ? reinterpret_cast<const unsigned short *>(P.F) + it.off_F : nullptr;
-    const unsigned short *Flo = FIN == 0 ? reinterpret_cast<const unsigned short *>(P.Flo) + it.off_Flo : nullptr;
-    const float *FW = FIN == 2 ? reinterpret_cast<const float *>(P.F) + it.off_F : nullptr;
+    const unsigned short *F = reinterpret_cast<const unsigned short *>(P.F) + it.off_F;
+    const unsigned short *Flo = reinterpret_cast<const unsigned short *>(P.Flo) + it.off_Flo;
     float *J = reinterpret_cast<float *>(P.J) + it.off_F;
     float *J2 = MODE == kGemmParts ? reinterpret_cast<float *>(P.J2) + it.off_F : nullptr;
     const LevelView lv(P, I);
     __shared__ GemmLds<NP, true> S;
     if constexpr (MODE == kGemmMix) {
-        gemm_tile_bf16<NP, true, FIN, 0>(it, lv, A, F, Flo, FW, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
+        gemm_tile_bf16<NP, true, 0>(it, lv, A, F, Flo, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
                                          [&](int lev, int r, float4 v) {
                                              if (r >= Pp) return;
                                              *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v;
                                          });
     } else {
-        gemm_tile_bf16<NP, true, FIN, 1>(it, lv, A, F, Flo, FW, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
+        gemm_tile_bf16<NP, true, 1>(it, lv, A, F, Flo, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
                                          [&](int lev, int r, float4 v0, float4 v1) {
                                              if (r >= Pp) return;
                                              *reinterpret_cast<float4 *>(J + (long long)lev * Pp + r) = v0;
@@ -712,7 +687,7 @@ __global__ __launch_bounds__(256, 1) void gemm_rebuild(const DevInst *__restrict
     float *J1 = reinterpret_cast<float *>(P.J2) + it.off_F;
     const LevelView lv(P, I);
     __shared__ GemmLds<NP, true> S;
-    gemm_tile_bf16<NP, true, 0, 2>(it, lv, A, F, Flo, nullptr, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
+    gemm_tile_bf16<NP, true, 2>(it, lv, A, F, Flo, nullptr, tl.tm * 64, tl.tn * kGemmLevels, S,
                                    [&](int lev, int r, float4 v, float4 p0, float4 p1) {
                                        if (r >= Pp) return;
                                        const long long ix = (long long)lev * Pp + r;
@@ -1194,13 +1169,13 @@ template __global__ void gemm_rebuild<1>(const DevInst *, const DevItem *, Pools
                                           int);
 template __global__ void gemm_rebuild<2>(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *,
                                           int);
-template __global__ void gemm_adj<1, kGemmParts, 0>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+template __global__ void gemm_adj<1, kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                                       const ItemState *, int);
-template __global__ void gemm_adj<2, kGemmParts, 0>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+template __global__ void gemm_adj<2, kGemmParts>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                                       const ItemState *, int);
-template __global__ void gemm_adj<1, kGemmMix, 0>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+template __global__ void gemm_adj<1, kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                                     const ItemState *, int);
-template __global__ void gemm_adj<2, kGemmMix, 0>(const DevInst *, const DevItem *, Pools, const GemmTile *,
+template __global__ void gemm_adj<2, kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                                     const ItemState *, int);
 template __global__ void gemm_f64<kGemmMix>(const DevInst *, const DevItem *, Pools, const GemmTile *,
                                             const ItemState *, int);
