@@ -158,3 +158,57 @@ def test_node_fields_agree_with_the_view_modes(oracle):
         assert not up[b, d.m_max + 1:].any() and not dn[b, d.m_max + 1:].any()
         assert not up[b, :, :, n_ring:].any() and not dn[b, :, :, n_ring:].any()
         assert np.all(dn[b, 0, 0, :n_ring] > 0.0)   # diffuse skylight reaches every node at m = 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["aerosol_rayleigh", "aerosol_only", "rayleigh_only"])
+def test_solve_atm_batch_equals_reference_package(name):
+    """The mirror against the reference package's own solve_atm_batch on media
+    from its own builders (tests/golden/ref_atmos_batch.json, written by
+    make_atmos_batch_golden.py in the build container): an aerosol + Rayleigh
+    batch, aerosol only (tau_R = 0) and Rayleigh only.  TOA values, the per-mode
+    BOA fields and the hemispheric diffuse transmittance within 1e-11, equal
+    order counts."""
+    import json
+    import os
+    from conftest import GOLDEN
+    from ocrt_amd.atmos_batch import solve_atm_batch
+    with open(os.path.join(GOLDEN, "ref_atmos_batch.json")) as f:
+        rec = {r["name"]: r for r in json.load(f)}[name]
+    a = rec["args"]
+    atms = [SimpleNamespace(**{k: np.asarray(v) for k, v in m.items()}) for m in rec["media"]]
+    print(f"[gpu] solve_atm_batch vs the reference package: {name}, {len(atms)} cases", flush=True)
+    out = solve_atm_batch(atms, rec["vjs"], rec["mu_suns"], rec["dphis"], rec["winds"], a["n_mu"], a["nt"],
+                          a["m_max"], a["max_iterations"], a["tolerance"], mode=a["mode"], L_max=a["L_max"])
+    ref = {k: np.asarray(v) for k, v in rec["out"].items()}
+    scale = np.abs(ref["rho_I"])
+    for f in ("rho_I", "rho_Q", "rho_U", "I_TOA", "Q_TOA", "U_TOA"):
+        assert np.all(np.abs(out[f] - ref[f]) <= 1e-11 * scale), (f, out[f], ref[f])
+    for f in ("boa_I", "boa_Q", "boa_U"):
+        assert out[f].shape == ref[f].shape, f
+        assert np.max(np.abs(out[f] - ref[f])) <= 1e-11 * np.max(np.abs(ref["boa_I"])), f
+    assert np.all(np.abs(out["T_diff_dn_hemi"] - ref["T_diff_dn_hemi"]) <= 1e-11 * np.abs(ref["T_diff_dn_hemi"]))
+    assert list(out["n_orders"]) == [int(v) for v in ref["n_orders"]]
+
+
+def test_reference_built_media_map_onto_layers():
+    """The media of the reference package's own builders (stored in
+    ref_atmos_batch.json) go through the mirror's layering with their level
+    optical depths and scattering fractions reproduced."""
+    import json
+    import os
+    from conftest import GOLDEN
+    from ocrt_amd.atmos_batch import _layers
+    with open(os.path.join(GOLDEN, "ref_atmos_batch.json")) as f:
+        recs = json.load(f)
+    for rec in recs:
+        nt = rec["args"]["nt"]
+        for m in rec["media"]:
+            a = SimpleNamespace(**{k: np.asarray(v) for k, v in m.items()})
+            lay, x0, y0 = _layers(a, nt)
+            dt = lay[0] + lay[1] + lay[3]
+            h = np.concatenate([[0.0], np.cumsum(dt)])
+            assert np.allclose(h, a.h, rtol=1e-14, atol=1e-16), rec["name"]
+            assert np.allclose(lay[1] / dt, a.xdel[1:], rtol=1e-13, atol=1e-16), rec["name"]
+            assert np.allclose(lay[0] / dt, a.ydel[1:], rtol=1e-13, atol=1e-16), rec["name"]
+            assert (x0, y0) == (a.xdel[0], a.ydel[0])
