@@ -267,6 +267,9 @@ def main():
     work = pm.work()
     log(f"mixed: {ms_mixed:.1f} ms/step, kernels {kt}")
     pg = ocrt_amd.Plan(descs, precision="mixed", graph=True)
+    pg.set_tuning(tuning["unroll"], tuning["waves"])        # the eager plan's autotuned shapes
+    for k, shape in enumerate(gemm_tuning["shape_of_class"]):
+        pg.set_gemm_shape(k, ocrt_amd.Plan.GEMM_SHAPES.index(tuple(shape)))
     ms_graph = timed_steps(pg, args.steps, 1, sync) * 1e3
     same_bits = all(np.array_equal(pg.results()[f], res_m[f]) for f in ("rho_I", "rho_Q", "rho_U"))
     pg.close()

@@ -196,6 +196,15 @@ struct Chunk {
 // same rounding contributes ~1e-6.
 constexpr int kSplitModes = 3;
 
+// Launch shapes of plans that do not autotune: the sweep's (4 levels per
+// block, 2 wavefronts per workgroup; kSweepShapes index 4) and the
+// scattering integral's per operator class (128 x 128 tiles for the
+// single-part classes, 64 x 128 for two-part items; kGemmShapes indices), the
+// shapes the autotuner picks on the 8192-instance workload or within 2 % of
+// them.
+constexpr int kDefaultSweepShape = 4;
+constexpr int kDefaultGemmShape[3] = {3, 3, 0};
+
 // Default estimated remaining share below which a mixed-form item's next
 // order takes the reduced operator product (ocrt_plan_set_reduced_threshold).
 // On the batched workload of 8192 uneven instances (modes 0..16, 50-400
@@ -278,13 +287,13 @@ struct ocrt_plan {
     int profile = 0;
     // launch shape of the mixed-form sweep (kSweepShapes), optionally chosen
     // by timing every shape on the plan's own work at the first execute
-    int sweep_variant = 0;
+    int sweep_variant = kDefaultSweepShape;
     int autotune = 0, tuned = 0;
     std::vector<double> tune_ms;
     // tile shape (kGemmShapes) of each operator class of the bf16 scattering
     // integral: single-part single-field, single-part pair, two-part; and the
     // autotuner's per-class GEMM time with each shape
-    int gemm_shape[3] = {0, 0, 0};
+    int gemm_shape[3] = {kDefaultGemmShape[0], kDefaultGemmShape[1], kDefaultGemmShape[2]};
     int gemm_tuned = 0, tune_classes = 0;
     double gemm_ms[3][kGemmShapeCount] = {};
     // replay of the captured forward launch sequence (hipGraph)
@@ -1642,7 +1651,7 @@ static int autotune(ocrt_plan *p, hipStream_t s)
 {
     const int saved = p->profile;
     p->profile = 0;
-    p->sweep_variant = 0;
+    p->sweep_variant = kDefaultSweepShape;
     int rc = execute_eager(p, s);
     p->profile = 1;
     // GEMM tile shape per operator class: one execute per shape, every class
