@@ -355,7 +355,8 @@ int ocrt_plan_adjoint_layers(ocrt_plan *plan, const double *d_grad_out, double *
                              void *hip_stream);
 
 /* One-call Jacobian form (the boundary's ocrt_adjoint_batch): plan, forward,
- * adjoint, copy results and gradients, destroy.  precision must be FP64. */
+ * adjoint, copy results and gradients, destroy.  precision FP64 or MIXED
+ * (the two adjoint forms of ocrt_plan_create_ex with OCRT_PLAN_ADJOINT). */
 int ocrt_adjoint_batch(const ocrt_instance_t *inst, int n_inst, int precision, const double *grad_out,
                        ocrt_result_t *out, double *grad_levels, double *grad_wind, void *hip_stream);
 

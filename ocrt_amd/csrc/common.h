@@ -37,7 +37,7 @@ struct DevInst {
     int layered;         // 1: explicit per-layer optical depths (pool_layers)
     int gidx;            // index of the instance in the plan
     int gas;             // gas absorption table (Pools::gas_tab), -1: none
-    long long off_lev;   // pool_lev: 6 arrays of L doubles (h, ch, xd, yd, z, layer absorption)
+    long long off_lev;   // pool_lev: 7 arrays of L doubles (h, ch, xd, yd, z, layer absorption, 1 / dtau)
     long long off_ring;  // pool_ring: mu[n], w[n]
     long long off_T;     // pool_T (double) / pool_Tf (float): [nt][n]
     long long off_greek; // pool_greek: [4][La+1]

@@ -574,7 +574,7 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
         }
         I.off_grad = p->grad_n;
         p->grad_n += 4LL * (I.nt + 1);
-        h.lev_n = 6LL * (I.nt + 1);
+        h.lev_n = 7LL * (I.nt + 1);
         h.T_n = (long long)I.nt * n;
         h.R_n = (I.surf == 2) ? (long long)(I.mmax + 1) * n * n * 9 + (long long)(I.mmax + 1) * n * 9 : 0;
         p->mmax_all = std::max(p->mmax_all, I.mmax);

@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void medium_gas(const DevInst *__restrict__ in
 }
 
 // Beer-Lambert factors T[layer][node] = exp(-dtau / mu_node) in fp64, 1 - T
-// in fp32, and 1 / dtau in fp32.
+// in fp32, and 1 / dtau in fp32 and in fp64 (row 6 of the level arrays).
 __global__ void medium_transmit(const DevInst *__restrict__ inst, Pools P)
 {
     const DevInst &I = inst[blockIdx.y];
@@ -242,6 +242,7 @@ __global__ void medium_transmit(const DevInst *__restrict__ inst, Pools P)
         float *Tf = P.Tf + I.off_T + (long long)k * I.n;
         const double dtau = h[k + 1] - h[k];
         P.idt[I.off_lev + k] = (float)(1.0 / dtau);
+        P.lev[I.off_lev + 6LL * (nt + 1) + k] = 1.0 / dtau;
         for (int o = 0; o < I.n; ++o) {
             T[o] = exp(-dtau / fabs(mu[o]));
             // fp32 sweeps carry 1 - T: with dtau / mu ~ 1e-4 per layer, T itself

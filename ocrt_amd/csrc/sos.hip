@@ -33,10 +33,12 @@ namespace ocrt {
 // level k; emit(k, value, tr, where) receives the field at level k, with tr
 // the Beer-Lambert factor of the layer just crossed and where = 0 for the
 // starting row, 1 for an interior level, 2 for the far row (a compile-time
-// constant, so the interior levels carry no row tests).
+// constant, so the interior levels carry no row tests).  The source's slope
+// takes the layer's 1 / dtau from rdt (wavefront-uniform scalar loads) in
+// place of an fp64 division per column and level.
 template <typename SrcFn, typename EmitFn>
 __device__ inline void sweep_column_f64(int hemi, int nt, double mu, double bc, const double *h,
-                                        const double *T, int n, int o, SrcFn S, EmitFn emit)
+                                        const double *rdt, const double *T, int n, int o, SrcFn S, EmitFn emit)
 {
     using first_t = std::integral_constant<int, 0>;
     using inner_t = std::integral_constant<int, 1>;
@@ -47,15 +49,13 @@ __device__ inline void sweep_column_f64(int hemi, int nt, double mu, double bc, 
     emit(k_first, run, 1.0, first_t{});
     double s_in = S(k_first);
     double hu = h[k_first];   // depth of the upstream level
-    auto step = [&](double s_out, double hk, double tr) {
+    auto step = [&](double s_out, double hk, double tr, double rd) {
         if (hemi == 0) {
-            const double dt = hu - hk;
-            const double a = (s_in - s_out) / dt;
+            const double a = (s_in - s_out) * rd;
             const double b = s_out - a * hk;
             run = tr * run + ((1.0 - tr) * (b + a * mu) + a * (hk - hu * tr)) * 0.5;
         } else {
-            const double dt = hk - hu;
-            const double a = (s_out - s_in) / dt;
+            const double a = (s_out - s_in) * rd;
             const double b = s_out - a * hk;
             run = tr * run + ((1.0 - tr) * (b - a * mu) + a * (hk - hu * tr)) * 0.5;
         }
@@ -70,29 +70,30 @@ __device__ inline void sweep_column_f64(int hemi, int nt, double mu, double bc, 
     auto layer_of = [&](int k) { return hemi ? k - 1 : k; };
     int i = 1;
     for (; i + kB <= nt; i += kB) {
-        double sv[kB], hk[kB], tr[kB];
+        double sv[kB], hk[kB], tr[kB], rd[kB];
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
             const int k = k_first + dL * (i + u);
             sv[u] = S(k);
             hk[u] = h[k];
             tr[u] = T[(long long)layer_of(k) * n + o];
+            rd[u] = rdt[layer_of(k)];
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-            step(sv[u], hk[u], tr[u]);
+            step(sv[u], hk[u], tr[u], rd[u]);
             emit(k_first + dL * (i + u), run, tr[u], inner_t{});
         }
     }
     for (; i < nt; ++i) {
         const int k = k_first + dL * i;
         const double tr = T[(long long)layer_of(k) * n + o];
-        step(S(k), h[k], tr);
+        step(S(k), h[k], tr, rdt[layer_of(k)]);
         emit(k, run, tr, inner_t{});
     }
     const int kl = k_first + dL * nt;
     const double tr = T[(long long)layer_of(kl) * n + o];
-    step(S(kl), h[kl], tr);
+    step(S(kl), h[kl], tr, rdt[layer_of(kl)]);
     emit(kl, run, tr, far_t{});
 }
 
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
                         : hemi == 0 ? P.greek[I.off_bsrc + ((long long)c * (I.bsrc_mmax + 1) + m) * n + o]
                                     : P.greek[I.off_tsrc + ((long long)c * (I.tsrc_mmax + 1) + m) * n + o];
     double att = 1.0;
-    sweep_column_f64(hemi, nt, mu, 0.0, lv.h, T, n, o,
+    sweep_column_f64(hemi, nt, mu, 0.0, lv.h, lv.rdt, T, n, o,
                      [&](int k) { return solar ? src.primary(lv, k) + src.flat_value(lv, I.mus, k) : 0.0; },
                      [&](int k, double v, double tr, auto where) {
                          constexpr int kWhere = decltype(where)::value;   // 0 starting row, 1 interior, 2 far row
@@ -960,8 +961,8 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
                     if (k < 0 || k > nt) break;
                     const int ku = k - step;                // upstream level
                     const double s_out = sj[u];
-                    const double dt = hemi ? lv.h[k] - lv.h[ku] : lv.h[ku] - lv.h[k];
-                    const double a = hemi ? (s_out - s_in) / dt : (s_in - s_out) / dt;
+                    const double rd = lv.rdt[hemi ? k - 1 : k];   // 1 / dtau of the layer crossed
+                    const double a = hemi ? (s_out - s_in) * rd : (s_in - s_out) * rd;
                     const double b = s_out - a * lv.h[k];
                     run = tr[u] * run + ((1.0 - tr[u]) * (b + a * smu) + a * (lv.h[k] - lv.h[ku] * tr[u])) * 0.5;
                     emit(k, run, to[u]);

@@ -12,11 +12,11 @@ typedef float float4_ __attribute__((ext_vector_type(4)));
 typedef short short4_ __attribute__((ext_vector_type(4)));
 
 struct LevelView {
-    const double *h, *ch, *xd, *yd, *z;
+    const double *h, *ch, *xd, *yd, *z, *rdt;   // rdt[l] = 1 / (h[l + 1] - h[l])
     __device__ LevelView(const Pools &P, const DevInst &I)
     {
         const int L = I.nt + 1;
-        h = P.lev + I.off_lev; ch = h + L; xd = ch + L; yd = xd + L; z = yd + L;
+        h = P.lev + I.off_lev; ch = h + L; xd = ch + L; yd = xd + L; z = yd + L; rdt = h + 6 * L;
     }
 };
 

@@ -5,12 +5,16 @@ adjoint (ocrt_plan_adjoint).
 
     rho = toa_stokes(layers, wind, base)        # layers [B, 4, nt] float64, wind [B]
     rho.backward(g)                             # fills layers.grad, wind.grad
+    rho = toa_stokes(layers, wind, base, precision="mixed")
 
 `layers` rows: Rayleigh tau, aerosol tau, aerosol single-scattering albedo,
 absorption tau, top down.  `base` is a list of B keyword dicts for
 ocrt_amd.abi.make_instance (geometry, surface, Greek coefficients, streams,
 modes, ...); their layers and wind are replaced by the tensors.  The solve
-runs the FP64 form with IPSS off (the adjoint's domain).
+runs with IPSS off (the adjoint's domain), in the FP64 form (the default) or,
+with precision="mixed", in the mixed adjoint form: the forward's fields kept
+as bf16 pairs, fp32 adjoint fields and bf16 MFMA operator products (a few
+1e-6 of each gradient's scale from the FP64 form's).
 
 Two ways through the boundary:
 
@@ -46,6 +50,12 @@ _CACHE = OrderedDict()
 _CACHE_MAX = 4
 
 
+def _check_precision(precision):
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed', not %r" % (precision,))
+    return precision
+
+
 def _descriptors(layers, wind, base):
     descs = []
     for b, kw in enumerate(base):
@@ -67,17 +77,17 @@ def _hashable(v):
     return v
 
 
-def _signature(layers, base):
+def _signature(layers, base, precision):
     kws = tuple(_hashable({k: v for k, v in kw.items() if k not in ("layers", "wind")}) for kw in base)
-    return (tuple(layers.shape), str(layers.device), kws)
+    return (tuple(layers.shape), str(layers.device), precision, kws)
 
 
 class _CachedPlan:
     """An adjoint plan with device inputs, and the forward it last ran."""
 
-    def __init__(self, layers, wind, base):
+    def __init__(self, layers, wind, base, precision):
         descs = _descriptors(layers.detach().cpu().numpy(), wind.detach().cpu().numpy(), base)
-        self.plan = Plan(descs, precision="fp64", adjoint=True, device_inputs=True)
+        self.plan = Plan(descs, precision=precision, adjoint=True, device_inputs=True)
         self.keep = descs
         self.last = 0      # token of the forward whose state the plan holds
         self.issued = 0
@@ -90,11 +100,11 @@ class _CachedPlan:
         return self.last
 
 
-def _cached_plan(layers, wind, base):
-    key = _signature(layers, base)
+def _cached_plan(layers, wind, base, precision):
+    key = _signature(layers, base, precision)
     entry = _CACHE.get(key)
     if entry is None:
-        entry = _CachedPlan(layers, wind, base)
+        entry = _CachedPlan(layers, wind, base, precision)
         _CACHE[key] = entry
         while len(_CACHE) > _CACHE_MAX:
             _CACHE.popitem(last=False)   # the plan lives on while a graph still holds it
@@ -105,11 +115,11 @@ def _cached_plan(layers, wind, base):
 
 class TOAStokes(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, layers, wind, base, device_io):
+    def forward(ctx, layers, wind, base, device_io, precision):
         if device_io:
             lay = layers.detach().to(torch.float64).contiguous()
             wnd = wind.detach().to(device=lay.device, dtype=torch.float64).contiguous()
-            entry = _cached_plan(lay, wnd, base)
+            entry = _cached_plan(lay, wnd, base, precision)
             stream = torch.cuda.current_stream(lay.device)
             token = entry.run(lay, wnd, stream)
             out = torch.empty((lay.shape[0], 3), dtype=torch.float64, device=lay.device)
@@ -120,7 +130,7 @@ class TOAStokes(torch.autograd.Function):
         lay = layers.detach().to("cpu", torch.float64).numpy()
         wnd = wind.detach().to("cpu", torch.float64).numpy()
         descs = _descriptors(lay, wnd, base)
-        plan = Plan(descs, precision="fp64", adjoint=True)
+        plan = Plan(descs, precision=precision, adjoint=True)
         plan.execute()
         res = plan.results()
         if not (res["status"] == 0).all():
@@ -147,18 +157,20 @@ class TOAStokes(torch.autograd.Function):
             gl = torch.empty_like(lay)
             gw = torch.empty_like(wnd)
             entry.plan.adjoint_layers(g.data_ptr(), gl.data_ptr(), gw.data_ptr(), stream)
-            return gl, gw, None, None
+            return gl, gw, None, None, None
         g = grad_out.detach().to("cpu", torch.float64).numpy()
         adj = ctx.plan.adjoint(g)
         gl = np.stack([layer_gradients(ctx.layers[b], adj["grad_h"][b], adj["grad_x"][b], adj["grad_y"][b],
                                        aerosol=bool(ctx.aerosol[b])) for b in range(len(ctx.layers))])
         dev = grad_out.device
         return (torch.as_tensor(gl, dtype=torch.float64, device=dev),
-                torch.as_tensor(adj["grad_wind"], dtype=torch.float64, device=dev), None, None)
+                torch.as_tensor(adj["grad_wind"], dtype=torch.float64, device=dev), None, None, None)
 
 
-def toa_stokes(layers, wind, base, device_io=None):
-    """TOA (rho_I, rho_Q, rho_U) [B, 3]; device_io defaults to layers.is_cuda."""
+def toa_stokes(layers, wind, base, device_io=None, precision="fp64"):
+    """TOA (rho_I, rho_Q, rho_U) [B, 3]; device_io defaults to layers.is_cuda;
+    precision "fp64" or "mixed" (the adjoint form the forward and the
+    gradients run in)."""
     if device_io is None:
         device_io = bool(layers.is_cuda)
-    return TOAStokes.apply(layers, wind, base, bool(device_io))
+    return TOAStokes.apply(layers, wind, base, bool(device_io), _check_precision(precision))
