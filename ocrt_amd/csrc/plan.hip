@@ -1031,7 +1031,6 @@ extern "C" int ocrt_plan_create_ex(const ocrt_instance_t *in, int n_inst, int pr
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t desc_bytes = 0;
     for (const Chunk &c : p->chunks)
-    for (const Chunk &c : p->chunks)
         for (int sh = 0; sh < kGemmShapeCount; ++sh)
             desc_bytes += al(std::max<size_t>(c.htiles_s[sh].size(), 1) * sizeof(GemmTile));
     for (const Chunk &c : p->chunks)
