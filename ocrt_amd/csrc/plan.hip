@@ -34,7 +34,8 @@ __global__ void value_phase(const DevInst *, const DevItem *, Pools, const int *
 template <typename AT, bool TRANS>
 __global__ void pack_operator(const DevInst *, const DevItem *, Pools, int, int);
 template <typename FT, bool MIXED>
-__global__ void order_one(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int);
+__global__ void order_one(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int, const double *,
+                          const double *);
 template <typename FT> __global__ void order_status(const DevInst *, const DevItem *, Pools, ItemState *, int);
 template <int MODE>
 __global__ void gemm_f64(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
@@ -1310,10 +1311,10 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
     const dim3 cols((unsigned)((ncols + 3) / 4));
     if (mixed)
         hipLaunchKernelGGL((order_one<__hip_bfloat16, true>), cols, dim3(256), 0, s, c.dinst, c.ditem, fields_of(1),
-                           st, c.dcols, ncols);
+                           st, c.dcols, ncols, (const double *)P.lev, (const double *)P.T);
     else
         hipLaunchKernelGGL((order_one<double, false>), cols, dim3(256), 0, s, c.dinst, c.ditem, fields_of(1), st,
-                           c.dcols, ncols);
+                           c.dcols, ncols, (const double *)P.lev, (const double *)P.T);
     const unsigned ntiles = (unsigned)c.htiles.size();
     for (int it = 2; it <= c.maxit; ++it) {
         const EvPair eg = p->profile ? ev_pair(p, 0) : EvPair{{nullptr, nullptr}, false};

@@ -113,7 +113,8 @@ template <typename FT, bool MIXED>
 __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ inst,
                                                  const DevItem *__restrict__ items, Pools P,
                                                  ItemState *__restrict__ st, const ColTile *__restrict__ tiles,
-                                                 int n_tiles)
+                                                 int n_tiles, const double *__restrict__ lev,
+                                                 const double *__restrict__ Tpool)
 {
     const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (tix >= n_tiles) return;
@@ -157,13 +158,15 @@ __global__ __launch_bounds__(256) void order_one(const DevInst *__restrict__ ins
         }
         return;
     }
-    const LevelView lv(P, I);
+    // level and Beer-Lambert tables through the __restrict__ arguments (P.lev,
+    // P.T): the per-level reads of the scan stay scalar loads
+    const LevelView lv(lev, I);
     int c, hemi, o;
     col_decode(col, n, c, hemi, o);
     hemi = __builtin_amdgcn_readfirstlane(hemi);   // column tiles are direction-pure
     const double *mu_ring = P.ring + I.off_ring;
     const double mu = mu_ring[o];
-    const double *T = P.T + I.off_T;
+    const double *T = Tpool + I.off_T;
     const ColumnSource src(I, it, P, lv, col);
     // one pass: the sweep is linear in its source, so single scattering and
     // the flat-surface seed (the specularly reflected direct beam scattered
@@ -1134,9 +1137,11 @@ __global__ void finalize(const DevInst *__restrict__ inst, int n_inst, const Ite
 
 // explicit instantiations
 template __global__ void order_one<double, false>(const DevInst *, const DevItem *, Pools, ItemState *,
-                                                  const ColTile *, int);
+                                                  const ColTile *, int, const double *,
+                                                  const double *);
 template __global__ void order_one<__hip_bfloat16, true>(const DevInst *, const DevItem *, Pools, ItemState *,
-                                                         const ColTile *, int);
+                                                         const ColTile *, int, const double *,
+                                                  const double *);
 template __global__ void order_status<double>(const DevInst *, const DevItem *, Pools, ItemState *, int);
 template __global__ void order_status<__hip_bfloat16>(const DevInst *, const DevItem *, Pools, ItemState *, int);
 #define OCRT_SWEEP_INST(FT, JT, MX, U, W)                                                                   \

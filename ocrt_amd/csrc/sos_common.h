@@ -18,6 +18,14 @@ struct LevelView {
         const int L = I.nt + 1;
         h = P.lev + I.off_lev; ch = h + L; xd = ch + L; yd = xd + L; z = yd + L; rdt = h + 6 * L;
     }
+    // from the level pool passed to a kernel as a __restrict__ argument: the
+    // kernel's stores cannot alias it, so its wavefront-uniform reads compile
+    // to scalar loads
+    __device__ LevelView(const double *lev, const DevInst &I)
+    {
+        const int L = I.nt + 1;
+        h = lev + I.off_lev; ch = h + L; xd = ch + L; yd = xd + L; z = yd + L; rdt = h + 6 * L;
+    }
 };
 
 // Read access to the forward field of an adjoint plan, as fp64, at column
