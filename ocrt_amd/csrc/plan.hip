@@ -46,7 +46,8 @@ __global__ void gemm_adj(const DevInst *, const DevItem *, Pools, const GemmTile
 template <int NP>
 __global__ void gemm_rebuild(const DevInst *, const DevItem *, Pools, const GemmTile *, const ItemState *, int);
 template <typename FT, typename JT, bool MIXED, int U, int W>
-__global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int, int);
+__global__ void sweep(const DevInst *, const DevItem *, Pools, ItemState *, const ColTile *, int, int, const double *,
+                      const float *, const float *);
 __global__ void extract(const DevInst *, const DevItem *, Pools, const ItemState *, ItemOut *, int);
 __global__ void finalize(const DevInst *, int, const ItemOut *, const InstOut *, const Pools, ocrt_result_t *);
 __global__ void ipss_kappa(const DevInst *, const IpssJob *, Pools, double *, double *);
@@ -1198,7 +1199,7 @@ static void launch_sweep_mixed(int v, int ncols, hipStream_t s, const DevInst *d
 #define OCRT_SWEEP_CASE(I, U_, W_)                                                                              \
     case I:                                                                                                      \
         hipLaunchKernelGGL((sweep<__hip_bfloat16, _Float16, true, U_, W_>), grid, block, 0, s, di, dt, P, st, tiles, \
-                           ncols, order);                                                                        \
+                           ncols, order, (const double *)P.lev, (const float *)P.idt, (const float *)P.W);     \
         break;
     switch (v) {
         OCRT_SWEEP_CASE(0, 8, 4)
@@ -1336,7 +1337,7 @@ static int run_chunk(ocrt_plan *p, Chunk &c, hipStream_t s)
             if (eg.ok) (void)hipEventRecord(eg.e[1], s);
             if (es.ok) (void)hipEventRecord(es.e[0], s);
             hipLaunchKernelGGL((sweep<double, double, false, 8, 4>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st,
-                               c.dcols, ncols, 0);
+                               c.dcols, ncols, 0, (const double *)Ps.lev, (const float *)Ps.idt, (const float *)Ps.W);
             if (es.ok) (void)hipEventRecord(es.e[1], s);
         }
         p->gemm_launches += 1.0;
@@ -1417,7 +1418,8 @@ static void backward_steps(ocrt_plan *p, Chunk &c, hipStream_t s)
                 hipLaunchKernelGGL(gemm_f64<kGemmBoth>, dim3(ntiles), dim3(256), 0, s, c.dinst, c.ditem, Pr,
                                    c.dtiles, st, o);
                 hipLaunchKernelGGL((sweep<double, double, false, 8, 4>), cols, dim3(256), 0, s, c.dinst, c.ditem,
-                                   Pout, st, c.dcols, ncols, o);
+                                   Pout, st, c.dcols, ncols, o, (const double *)Pout.lev, (const float *)Pout.idt,
+                                   (const float *)Pout.W);
             }
         }
         // backward steps

@@ -791,7 +791,8 @@ template <typename FT, typename JT, bool MIXED, int U, int W>
 __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst,
                                                 const DevItem *__restrict__ items, Pools P,
                                                 ItemState *__restrict__ st, const ColTile *__restrict__ tiles,
-                                                int n_tiles, int order)
+                                                int n_tiles, int order, const double *__restrict__ lev,
+                                                const float *__restrict__ idtp, const float *__restrict__ wpool)
 {
     const int tix = blockIdx.x * W + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (tix >= n_tiles) return;
@@ -809,7 +810,9 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
     const int col = ct.col0 + lane;
     double mcur = 0.0, mtot = 0.0;
     if (col < ct.col_end && col < 6 * n) {
-        const LevelView lv(P, I);
+        // level tables, 1 / dtau and the rank-3 projections through the __restrict__
+        // arguments (P.lev, P.idt, P.W): the per-level uniform reads are scalar loads
+        const LevelView lv(lev, I);
         int c, hemi, o;
         col_decode(col, n, c, hemi, o);
         hemi = __builtin_amdgcn_readfirstlane(hemi);   // column tiles are direction-pure
@@ -827,7 +830,7 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
         auto layer_of = [&](int k) { return hemi ? k - 1 : k; };
         if constexpr (MIXED) {
             const float *OM = P.Tf + I.off_T;
-            const float *IDT = P.idt + I.off_lev;
+            const float *IDT = idtp + I.off_lev;
             MixedColumn mc(P, it, col, c, o, hemi, accumulate);
             mc.top = top_up;
             const float smu = hemi ? -(float)mu : (float)mu;
@@ -837,7 +840,7 @@ __global__ __launch_bounds__(64 * W) void sweep(const DevInst *__restrict__ inst
             const float *__restrict__ Jf = P.Jf + it.off_Flo + col;
             // rank-3 molecular items add ydel(k) (U W[k]) to the dense source:
             // the column's three output factors, the item's projections
-            const float *__restrict__ Wl = P.W + it.off_W;
+            const float *__restrict__ Wl = wpool + it.off_W;
             const double *__restrict__ YD = lv.yd;
             float u0 = 0.f, u1 = 0.f, u2 = 0.f;
             if (it.lowrank) {
@@ -1146,7 +1149,8 @@ template __global__ void order_status<double>(const DevInst *, const DevItem *, 
 template __global__ void order_status<__hip_bfloat16>(const DevInst *, const DevItem *, Pools, ItemState *, int);
 #define OCRT_SWEEP_INST(FT, JT, MX, U, W)                                                                   \
     template __global__ void sweep<FT, JT, MX, U, W>(const DevInst *, const DevItem *, Pools, ItemState *,    \
-                                                     const ColTile *, int, int);
+                                                     const ColTile *, int, int, const double *, const float *,   \
+                                                     const float *);
 OCRT_SWEEP_INST(double, double, false, 8, 4)
 OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 4, 2)
 OCRT_SWEEP_INST(__hip_bfloat16, _Float16, true, 4, 4)
