@@ -122,7 +122,8 @@ __global__ void adj_step_table(const DevInst *__restrict__ inst, Pools P)
 template <typename FV, typename LT>
 __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ inst, const DevItem *__restrict__ items,
                                                  Pools P, const ItemState *__restrict__ st,
-                                                 const ColTile *__restrict__ tiles, int n_tiles, int order)
+                                                 const ColTile *__restrict__ tiles, int n_tiles, int order,
+                                                 const double *__restrict__ lev)
 {
     const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (tix >= n_tiles) return;
@@ -145,7 +146,8 @@ __global__ __launch_bounds__(256) void sweep_adj(const DevInst *__restrict__ ins
     const int hemi = __builtin_amdgcn_readfirstlane(ct.col0 >= 3 * n ? 1 : 0);
     int c = 0, hh = 0, o = 0;
     col_decode(colr, n, c, hh, o);
-    const LevelView lv(P, I);
+    // the level tables through the __restrict__ argument (P.lev): scalar loads
+    const LevelView lv(lev, I);
     const FV U(P, it, colr);
     const bool two = nparts > 1;
     LT *__restrict__ J0 = reinterpret_cast<LT *>(P.J) + off_F + colr;
@@ -353,7 +355,8 @@ __global__ void surface_adj(const DevInst *__restrict__ inst, const DevItem *__r
 template <typename FV, typename LT>
 __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__ inst,
                                                      const DevItem *__restrict__ items, Pools P,
-                                                     const ColTile *__restrict__ tiles, int n_tiles)
+                                                     const ColTile *__restrict__ tiles, int n_tiles,
+                                                     const double *__restrict__ lev)
 {
     const int tix = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (tix >= n_tiles) return;
@@ -374,7 +377,8 @@ __global__ __launch_bounds__(256) void order_one_adj(const DevInst *__restrict__
     const int hemi = __builtin_amdgcn_readfirstlane(ct.col0 >= 3 * n ? 1 : 0);
     int c = 0, hh = 0, o = 0;
     col_decode(colr, n, c, hh, o);
-    const LevelView lv(P, I);
+    // the level tables through the __restrict__ argument (P.lev): scalar loads
+    const LevelView lv(lev, I);
     const ColumnSource src(I, it, P, lv, colr);
     const double mus = I.mus, amu = P.ring[I.off_ring + o];
     const FV U(P, it, colr);
@@ -533,9 +537,10 @@ template __global__ void adj_seed<float>(const DevInst *, const DevItem *, Pools
 #define OCRT_ADJ_INST(FV, LT)                                                                                  \
     template __global__ void adj_bottom<FV>(const DevInst *, const DevItem *, Pools, const ItemState *, int);   \
     template __global__ void sweep_adj<FV, LT>(const DevInst *, const DevItem *, Pools, const ItemState *,      \
-                                               const ColTile *, int, int);                                     \
+                                               const ColTile *, int, int, const double *);                     \
     template __global__ void surface_adj<FV, LT>(const DevInst *, const DevItem *, Pools, const ItemState *, int); \
-    template __global__ void order_one_adj<FV, LT>(const DevInst *, const DevItem *, Pools, const ColTile *, int);
+    template __global__ void order_one_adj<FV, LT>(const DevInst *, const DevItem *, Pools, const ColTile *, int, \
+                                                   const double *);
 OCRT_ADJ_INST(FieldF64, double)
 OCRT_ADJ_INST(FieldPair, float)
 #undef OCRT_ADJ_INST

@@ -61,11 +61,12 @@ template <typename LT> __global__ void adj_seed(const DevInst *, const DevItem *
 template <typename FV> __global__ void adj_bottom(const DevInst *, const DevItem *, Pools, const ItemState *, int);
 __global__ void adj_surface_bc(const DevInst *, const DevItem *, Pools, const ItemState *, int);
 template <typename FV, typename LT>
-__global__ void sweep_adj(const DevInst *, const DevItem *, Pools, const ItemState *, const ColTile *, int, int);
+__global__ void sweep_adj(const DevInst *, const DevItem *, Pools, const ItemState *, const ColTile *, int, int,
+                          const double *);
 template <typename FV, typename LT>
 __global__ void surface_adj(const DevInst *, const DevItem *, Pools, const ItemState *, int);
 template <typename FV, typename LT>
-__global__ void order_one_adj(const DevInst *, const DevItem *, Pools, const ColTile *, int);
+__global__ void order_one_adj(const DevInst *, const DevItem *, Pools, const ColTile *, int, const double *);
 __global__ void glint_adj(const DevInst *, int, Pools);
 // lut.hip
 __global__ void lut_eval(const DevInst *, const DevItem *, Pools, const InstOut *, const int *, const double *, int,
@@ -198,13 +199,13 @@ struct Chunk {
 // same rounding contributes ~1e-6.
 constexpr int kSplitModes = 3;
 
-// Launch shapes of plans that do not autotune: the sweep's (4 levels per
-// block, 2 wavefronts per workgroup; kSweepShapes index 4) and the
+// Launch shapes of plans that do not autotune: the sweep's (8 levels per
+// block, 4 wavefronts per workgroup; kSweepShapes index 0) and the
 // scattering integral's per operator class (128 x 128 tiles for the
 // single-part classes, 64 x 128 for two-part items; kGemmShapes indices), the
 // shapes the autotuner picks on the 8192-instance workload or within 2 % of
 // them.
-constexpr int kDefaultSweepShape = 4;
+constexpr int kDefaultSweepShape = 0;
 constexpr int kDefaultGemmShape[3] = {3, 3, 0};
 
 // Default estimated remaining share below which a mixed-form item's next
@@ -1443,7 +1444,8 @@ static void backward_steps(ocrt_plan *p, Chunk &c, hipStream_t s)
             Pools Ps = at(u_o);
             Ps.J = Pg.J;
             Ps.J2 = Pg.J2;
-            hipLaunchKernelGGL((sweep_adj<FV, LT>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st, c.dcols, ncols, o);
+            hipLaunchKernelGGL((sweep_adj<FV, LT>), cols, dim3(256), 0, s, c.dinst, c.ditem, Ps, st, c.dcols, ncols, o,
+                               (const double *)Ps.lev);
             Pools Pt = P0;
             Pt.A = p->At;
             Pt.F = MX ? (void *)P0.sb_hi : Ps.J;   // fp64: the sweep wrote Sbar over the first part
@@ -1458,7 +1460,7 @@ static void backward_steps(ocrt_plan *p, Chunk &c, hipStream_t s)
         }
     }
     hipLaunchKernelGGL((order_one_adj<FV, LT>), cols, dim3(256), 0, s, c.dinst, c.ditem, at(p->ckpt[0]), c.dcols,
-                       ncols);
+                       ncols, (const double *)p->pools.lev);
     hipLaunchKernelGGL(glint_adj, dim3((ni + 63) / 64), dim3(64), 0, s, c.dinst, ni, P0);
 }
 
